@@ -1,0 +1,184 @@
+/* obvhs_hip.h -- C ABI of the MI355X (gfx950) implementation of the obvhs hot path:
+ *   PLOC BVH2 construction -> BVH2-to-CWBVH (80-byte node) conversion -> CWBVH closest-hit / occlusion traversal.
+ *
+ * Every entry point cites the obvhs v0.3.1 interface it replaces (paths relative to the crate root).  Plain C:
+ * pointers and sizes only.  All device pointers are HIP device memory on the context's device; all work is
+ * enqueued on the context's stream unless the function says it synchronises.  A context is externally
+ * synchronised (one host thread at a time); separate contexts may be used concurrently.  Handles returned by the
+ * build calls own their device memory until destroyed.  Each reference panic site maps to one status code.
+ */
+#ifndef OBVHS_HIP_H
+#define OBVHS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum obvhs_status {
+    OBVHS_OK = 0,
+    OBVHS_ERR_INVALID_ARG = 1,      /* argument outside what the reference accepts (ploc/mod.rs:559 etc.) */
+    OBVHS_ERR_NAN_INPUT = 2,        /* NaN primitive bounds: ploc/mod.rs:191-192 debug_assert */
+    OBVHS_ERR_PLOC_SELF_MERGE = 3,  /* a cluster without any finite-cost neighbour: ploc/mod.rs:451,466 panic */
+    OBVHS_ERR_CW_DISTRIBUTE = 4,    /* bvh2_to_cwbvh.rs:365-366 debug_assert (infinite costs) */
+    OBVHS_ERR_CW_NONUNIT_LEAF = 5,  /* bvh2_to_cwbvh.rs:243-247 panic */
+    OBVHS_ERR_CW_ORDER = 6,         /* bvh2_to_cwbvh.rs:460-465: a child left without a slot */
+    OBVHS_ERR_CW_PRIMCOUNT = 7,     /* bvh2_to_cwbvh.rs:155 panic */
+    OBVHS_ERR_CAPACITY = 8,         /* an internal fixed-size structure was exceeded */
+    OBVHS_ERR_DEPTH = 9,            /* CWBVH deeper than the 32-entry traversal stack: cwbvh/mod.rs:791 */
+    OBVHS_ERR_OOM = 10,             /* hipMalloc failed */
+    OBVHS_ERR_DEVICE = 11,          /* any other HIP runtime error; see obvhs_last_error_message */
+    OBVHS_ERR_NO_DEVICE = 12        /* no HIP device present */
+} obvhs_status;
+
+/* ---- plain-data types, byte-identical to the crate's repr(C) types ---- */
+typedef struct obvhs_vec3a { float x, y, z, w; } obvhs_vec3a;                       /* glam::Vec3A, 16 B */
+typedef struct obvhs_aabb { obvhs_vec3a min, max; } obvhs_aabb;                     /* aabb.rs:11-16, 32 B */
+typedef struct obvhs_triangle { obvhs_vec3a v0, v1, v2; } obvhs_triangle;           /* triangle.rs:8-13, 48 B */
+typedef struct obvhs_ray {                                                          /* ray.rs:15-30, 64 B */
+    obvhs_vec3a origin, direction, inv_direction;
+    float tmin, tmax;
+    float _pad[2];
+} obvhs_ray;
+typedef struct obvhs_ray_hit { uint32_t primitive_id, geometry_id, instance_id; float t; } obvhs_ray_hit; /* ray.rs:63-70 */
+typedef struct obvhs_bvh2_node {                                                    /* bvh2/node.rs:38-66, 48 B */
+    obvhs_aabb aabb;
+    uint32_t prim_count, first_index, meta1, meta2;
+} obvhs_bvh2_node;
+typedef struct obvhs_bvh2_node_small {                                              /* bvh2/node.rs:12-36, 32 B */
+    float min[3];
+    uint32_t prim_count;
+    float max[3];
+    uint32_t first_index;
+} obvhs_bvh2_node_small;
+typedef struct obvhs_cwbvh_node {                                                   /* cwbvh/node.rs:12-54, 80 B */
+    float p[3];
+    uint8_t e[3];
+    uint8_t imask;
+    uint32_t child_base_idx;
+    uint32_t primitive_base_idx;
+    uint8_t child_meta[8];
+    uint8_t child_min_x[8], child_max_x[8], child_min_y[8], child_max_y[8], child_min_z[8], child_max_z[8];
+} obvhs_cwbvh_node;
+
+/* BvhBuildParams (lib.rs:207-230).  ploc_search_distance is the numeric distance {1,2,6,14,24,32}
+ * (PlocSearchDistance, ploc/mod.rs:534-562); sort_precision 64 or 128 (SortPrecision, ploc/mod.rs:657-661). */
+typedef struct obvhs_build_params {
+    uint32_t pre_split;
+    uint32_t ploc_search_distance;
+    uint32_t search_depth_threshold;
+    float reinsertion_batch_ratio;
+    float post_collapse_reinsertion_batch_ratio_multiplier;
+    uint32_t sort_precision;
+    uint32_t max_prims_per_leaf;
+    float collapse_traversal_cost;
+} obvhs_build_params;
+
+typedef struct obvhs_context obvhs_context;
+typedef struct obvhs_bvh2 obvhs_bvh2;
+typedef struct obvhs_cwbvh obvhs_cwbvh;
+
+/* Per-build statistics (the reference's `scope!` stage names, lib.rs:197-205). */
+typedef struct obvhs_build_stats {
+    uint32_t prim_count;
+    uint32_t ploc_iterations;     /* PLOC passes (depth in ploc/mod.rs:492) */
+    uint32_t bvh2_node_count;
+    uint32_t cwbvh_node_count;
+    uint64_t ploc_cluster_sum;    /* sum of the cluster counts over all PLOC passes */
+} obvhs_build_stats;
+
+/* ---- context, memory, timing (no reference counterpart: the crate is CPU-only) ---- */
+const char* obvhs_status_string(obvhs_status s);
+obvhs_status obvhs_context_create(int device, void* hip_stream, obvhs_context** out);
+obvhs_status obvhs_context_destroy(obvhs_context* ctx);
+obvhs_status obvhs_context_synchronize(obvhs_context* ctx);
+const char* obvhs_last_error_message(const obvhs_context* ctx);
+obvhs_status obvhs_device_malloc(obvhs_context* ctx, uint64_t bytes, void** dptr);
+obvhs_status obvhs_device_free(obvhs_context* ctx, void* dptr);
+obvhs_status obvhs_memcpy_htod(obvhs_context* ctx, void* dst, const void* src, uint64_t bytes); /* sync */
+obvhs_status obvhs_memcpy_dtoh(obvhs_context* ctx, void* dst, const void* src, uint64_t bytes); /* sync */
+obvhs_status obvhs_memset_device(obvhs_context* ctx, void* dst, int value, uint64_t bytes);      /* async */
+/* Keep each build's Morton codes and sort permutation for obvhs_bvh2_copy_sort_trace (parity tests). */
+obvhs_status obvhs_context_set_trace(obvhs_context* ctx, int enable);
+/* Stream timer: records a start/stop event on the context stream; obvhs_timer_stop synchronises. */
+obvhs_status obvhs_timer_start(obvhs_context* ctx);
+obvhs_status obvhs_timer_stop(obvhs_context* ctx, float* ms);
+
+/* ---- PLOC BVH2 build ----
+ * Replaces PlocBuilder::build / build_with_bvh (ploc/mod.rs:95-113, 129-159) for T = Aabb, with
+ * SortPrecision::U64.  d_aabbs: n primitive AABBs (device).  d_indices: the `indices` vector (device), or NULL
+ * for 0..n.  `reuse` may be a handle from an earlier build whose allocations are reused (build_with_bvh);
+ * pass NULL to allocate a new handle. */
+obvhs_status obvhs_ploc_build_aabbs(obvhs_context* ctx, const obvhs_aabb* d_aabbs, const uint32_t* d_indices,
+                                    uint32_t n, uint32_t search_distance, uint32_t sort_precision,
+                                    uint32_t search_depth_threshold, obvhs_bvh2* reuse, obvhs_bvh2** out);
+/* Same for T = Triangle (Boundable via Triangle::aabb, triangle.rs:28-30). */
+obvhs_status obvhs_ploc_build_triangles(obvhs_context* ctx, const obvhs_triangle* d_tris, const uint32_t* d_indices,
+                                        uint32_t n, uint32_t search_distance, uint32_t sort_precision,
+                                        uint32_t search_depth_threshold, obvhs_bvh2* reuse, obvhs_bvh2** out);
+/* Bvh2 fields (bvh2/mod.rs:31-85): node count (2n-1), primitive count, max_depth (ploc/mod.rs:501). */
+obvhs_status obvhs_bvh2_info(const obvhs_bvh2* bvh, uint32_t* node_count, uint32_t* prim_count, uint32_t* max_depth);
+/* Copies Bvh2::nodes in the default 48-byte layout, or the 32-byte small_bvh2_node layout (synchronous). */
+obvhs_status obvhs_bvh2_copy_nodes(obvhs_context* ctx, const obvhs_bvh2* bvh, obvhs_bvh2_node* host_out);
+obvhs_status obvhs_bvh2_copy_nodes_small(obvhs_context* ctx, const obvhs_bvh2* bvh, obvhs_bvh2_node_small* host_out);
+obvhs_status obvhs_bvh2_copy_primitive_indices(obvhs_context* ctx, const obvhs_bvh2* bvh, uint32_t* host_out);
+/* Debug/parity outputs of the last build into this handle: Morton codes in original leaf order and the sorted
+ * permutation (sorted position -> leaf index), ploc/mod.rs:782-846. */
+obvhs_status obvhs_bvh2_copy_sort_trace(obvhs_context* ctx, const obvhs_bvh2* bvh, uint64_t* host_codes,
+                                        uint32_t* host_sorted_index);
+obvhs_status obvhs_bvh2_destroy(obvhs_bvh2* bvh);
+
+/* ---- CWBVH ----
+ * Replaces bvh2_to_cwbvh(bvh2, max_prims_per_leaf, order_children, include_exact_node_aabbs)
+ * (cwbvh/bvh2_to_cwbvh.rs:490-510).  include_exact_node_aabbs must be 0 (the flag is kept for the signature). */
+obvhs_status obvhs_bvh2_to_cwbvh(obvhs_context* ctx, const obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,
+                                 int order_children, int include_exact_node_aabbs, obvhs_cwbvh* reuse,
+                                 obvhs_cwbvh** out);
+/* Replaces build_cwbvh_from_tris (cwbvh/builder.rs:20-85): PLOC -> [reinsertion] -> bvh2_to_cwbvh(.., clamp(max
+ * prims,1,3), true, false).  Pre-splits and reinsertion ratios other than 0 return OBVHS_ERR_INVALID_ARG in this
+ * version.  d_tris is device memory.  stats may be NULL. */
+obvhs_status obvhs_build_cwbvh_from_tris(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
+                                         const obvhs_build_params* params, obvhs_cwbvh* reuse, obvhs_cwbvh** out,
+                                         obvhs_build_stats* stats);
+/* CwBvh fields (cwbvh/mod.rs:43-55). */
+obvhs_status obvhs_cwbvh_info(const obvhs_cwbvh* bvh, uint32_t* node_count, uint32_t* prim_count,
+                              obvhs_aabb* total_aabb);
+obvhs_status obvhs_cwbvh_copy_nodes(obvhs_context* ctx, const obvhs_cwbvh* bvh, obvhs_cwbvh_node* host_out);
+obvhs_status obvhs_cwbvh_copy_primitive_indices(obvhs_context* ctx, const obvhs_cwbvh* bvh, uint32_t* host_out);
+/* Device pointers of CwBvh::nodes and CwBvh::primitive_indices (valid until the handle is destroyed/rebuilt). */
+obvhs_status obvhs_cwbvh_device_arrays(const obvhs_cwbvh* bvh, const obvhs_cwbvh_node** d_nodes,
+                                       const uint32_t** d_primitive_indices);
+/* bvh_tris[i] = tris[primitive_indices[i]] (the reordering of examples/obj_cwbvh.rs:63-67), device to device. */
+obvhs_status obvhs_cwbvh_reorder_triangles(obvhs_context* ctx, const obvhs_cwbvh* bvh, const obvhs_triangle* d_tris,
+                                           obvhs_triangle* d_bvh_tris);
+obvhs_status obvhs_cwbvh_destroy(obvhs_cwbvh* bvh);
+
+/* ---- traversal ----
+ * Replaces CwBvh::ray_traverse (cwbvh/mod.rs:169-198) with the closure |ray, id| bvh_tris[id].intersect(ray)
+ * (triangle.rs:35-76), for n rays at once.  d_bvh_tris: triangles in primitive_indices order.  Each hit starts as
+ * RayHit::none() (ray.rs:76-83); primitive_id is the index into CwBvh::primitive_indices like the reference's. */
+obvhs_status obvhs_cwbvh_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh* bvh, const obvhs_triangle* d_bvh_tris,
+                                      const obvhs_ray* d_rays, uint32_t n, obvhs_ray_hit* d_hits);
+/* Replaces CwBvh::ray_traverse_miss (cwbvh/mod.rs:201-225): d_miss[i] = 1 when ray i hit nothing in [tmin,tmax]. */
+obvhs_status obvhs_cwbvh_ray_traverse_miss(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                           const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays, uint32_t n,
+                                           uint8_t* d_miss);
+/* Instrumented closest-hit traversal: also writes per-ray node-test and triangle-test counts (for the byte
+ * model of the roofline).  Same hits as obvhs_cwbvh_ray_traverse. */
+obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                              const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays, uint32_t n,
+                                              obvhs_ray_hit* d_hits, uint32_t* d_node_visits,
+                                              uint32_t* d_tri_tests);
+
+/* Host copy of the CWBVH exponent table: entry E is the largest 23-bit mantissa M with
+ * log2f(2^(E-127) * (1 + M/2^23)) == E - 127 under the host libm, i.e. the inputs whose
+ * exp2(ceil(log2(x))) stays at 2^(E-127) in bvh2_to_cwbvh.rs:85-88.  Needs no device. */
+obvhs_status obvhs_log2_threshold_table(uint32_t* out256);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OBVHS_HIP_H */

@@ -1,0 +1,396 @@
+"""Host-side mirror of the obvhs hot-path API over the HIP library.
+
+Same names and argument meaning as the crate (PlocBuilder, PlocSearchDistance, SortPrecision, BvhBuildParams, Bvh2,
+bvh2_to_cwbvh, build_cwbvh_from_tris, CwBvh.ray_traverse / ray_traverse_miss), batched over arrays: the reference
+traverses one ray per call with a closure (cwbvh/mod.rs:169-225); here a call traverses a whole ray array with the
+intersector fixed to Triangle::intersect (triangle.rs:35-76).  Reference panics surface as ObvhsError with the
+matching status.  Every call runs on the GPU through libobvhs_hip.so; there is no CPU fallback.
+"""
+import ctypes as C
+import enum
+
+import numpy as np
+
+from . import native
+from .native import BuildParams, BuildStats, ObvhsError, check, lib
+from .types import (AABB, BVH2_NODE, BVH2_NODE_SMALL, CWBVH_NODE, RAY, RAY_HIT, TRIANGLE)
+
+
+class PlocSearchDistance(enum.IntEnum):
+    """ploc/mod.rs:534-562"""
+    Minimum = 1
+    VeryLow = 2
+    Low = 6
+    Medium = 14
+    High = 24
+    VeryHigh = 32
+
+    @classmethod
+    def from_u32(cls, v):
+        try:
+            return cls(int(v))
+        except ValueError:
+            raise ValueError(f"Invalid value for PlocSearchDistance: {v}") from None  # ploc/mod.rs:559
+
+
+class SortPrecision(enum.IntEnum):
+    """ploc/mod.rs:657-661 (value = key bits)."""
+    U128 = 128
+    U64 = 64
+
+
+class BvhBuildParams:
+    """lib.rs:207-306 with its six presets."""
+
+    def __init__(self, pre_split=False, ploc_search_distance=PlocSearchDistance.Medium, search_depth_threshold=0,
+                 reinsertion_batch_ratio=0.0, post_collapse_reinsertion_batch_ratio_multiplier=0.0,
+                 sort_precision=SortPrecision.U64, max_prims_per_leaf=1, collapse_traversal_cost=1.0):
+        self.pre_split = bool(pre_split)
+        self.ploc_search_distance = PlocSearchDistance(ploc_search_distance)
+        self.search_depth_threshold = int(search_depth_threshold)
+        self.reinsertion_batch_ratio = float(reinsertion_batch_ratio)
+        self.post_collapse_reinsertion_batch_ratio_multiplier = float(post_collapse_reinsertion_batch_ratio_multiplier)
+        self.sort_precision = SortPrecision(sort_precision)
+        self.max_prims_per_leaf = int(max_prims_per_leaf)
+        self.collapse_traversal_cost = float(collapse_traversal_cost)
+
+    def to_c(self):
+        return BuildParams(int(self.pre_split), int(self.ploc_search_distance), self.search_depth_threshold,
+                           self.reinsertion_batch_ratio, self.post_collapse_reinsertion_batch_ratio_multiplier,
+                           int(self.sort_precision), self.max_prims_per_leaf, self.collapse_traversal_cost)
+
+    def __repr__(self):
+        return f"BvhBuildParams({self.__dict__})"
+
+    @classmethod
+    def fastest_build(cls):
+        return cls(False, PlocSearchDistance.Minimum, 0, 0.0, 0.0, SortPrecision.U64, 1, 1.0)
+
+    @classmethod
+    def very_fast_build(cls):
+        return cls(False, PlocSearchDistance.Minimum, 0, 0.01, 0.0, SortPrecision.U64, 8, 3.0)
+
+    @classmethod
+    def fast_build(cls):
+        return cls(False, PlocSearchDistance.Low, 2, 0.02, 0.0, SortPrecision.U64, 8, 3.0)
+
+    @classmethod
+    def medium_build(cls):
+        return cls(False, PlocSearchDistance.Medium, 3, 0.05, 2.0, SortPrecision.U64, 8, 3.0)
+
+    @classmethod
+    def slow_build(cls):
+        return cls(True, PlocSearchDistance.High, 2, 0.2, 2.0, SortPrecision.U128, 8, 3.0)
+
+    @classmethod
+    def very_slow_build(cls):
+        return cls(True, PlocSearchDistance.Medium, 1, 1.0, 1.0, SortPrecision.U128, 8, 3.0)
+
+
+class Context:
+    """A device + stream.  All builds and traversals of this context are ordered on its stream."""
+
+    def __init__(self, device=0, stream=None):
+        h = C.c_void_p()
+        check(lib().obvhs_context_create(int(device), stream, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().obvhs_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        check(lib().obvhs_context_synchronize(self.handle), self.handle)
+
+    def set_trace(self, enable=True):
+        check(lib().obvhs_context_set_trace(self.handle, int(bool(enable))), self.handle)
+
+    def upload(self, array):
+        return DeviceArray.from_host(self, array)
+
+    def empty(self, dtype, count):
+        return DeviceArray(self, dtype, count)
+
+    def timer_start(self):
+        check(lib().obvhs_timer_start(self.handle), self.handle)
+
+    def timer_stop(self):
+        ms = C.c_float()
+        check(lib().obvhs_timer_stop(self.handle, C.byref(ms)), self.handle)
+        return float(ms.value)
+
+
+class DeviceArray:
+    """A typed device allocation owned by a context (keeps inputs resident in HBM)."""
+
+    def __init__(self, ctx, dtype, count):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.dtype.itemsize * self.count
+        p = C.c_void_p()
+        check(lib().obvhs_device_malloc(ctx.handle, max(self.nbytes, 1), C.byref(p)), ctx.handle)
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, ctx, array):
+        a = np.ascontiguousarray(array)
+        d = cls(ctx, a.dtype, a.size)
+        if a.nbytes:
+            check(lib().obvhs_memcpy_htod(ctx.handle, d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes), ctx.handle)
+        return d
+
+    def to_host(self):
+        out = np.empty(self.count, dtype=self.dtype)
+        if self.nbytes:
+            check(lib().obvhs_memcpy_dtoh(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes),
+                  self.ctx.handle)
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().obvhs_device_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.handle:
+                self.free()
+        except Exception:
+            pass
+
+
+def _as_device(ctx, data, dtype):
+    if isinstance(data, DeviceArray):
+        if data.dtype != np.dtype(dtype):
+            raise TypeError(f"expected device array of {dtype}, got {data.dtype}")
+        return data, data.count
+    a = np.ascontiguousarray(data, dtype=dtype)
+    return DeviceArray.from_host(ctx, a), a.size
+
+
+class Bvh2:
+    """Handle of a device-resident Bvh2 (bvh2/mod.rs:31-85)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def _info(self):
+        nc, pc, md = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().obvhs_bvh2_info(self.handle, C.byref(nc), C.byref(pc), C.byref(md)))
+        return nc.value, pc.value, md.value
+
+    @property
+    def node_count(self):
+        return self._info()[0]
+
+    @property
+    def max_depth(self):
+        return self._info()[2]
+
+    @property
+    def nodes(self):
+        n = self._info()[0]
+        out = np.zeros(n, dtype=BVH2_NODE)
+        check(lib().obvhs_bvh2_copy_nodes(self.ctx.handle, self.handle, out.ctypes.data_as(C.c_void_p)),
+              self.ctx.handle)
+        return out
+
+    @property
+    def nodes_small(self):
+        n = self._info()[0]
+        out = np.zeros(n, dtype=BVH2_NODE_SMALL)
+        check(lib().obvhs_bvh2_copy_nodes_small(self.ctx.handle, self.handle, out.ctypes.data_as(C.c_void_p)),
+              self.ctx.handle)
+        return out
+
+    @property
+    def primitive_indices(self):
+        n = self._info()[1]
+        out = np.zeros(n, dtype=np.uint32)
+        check(lib().obvhs_bvh2_copy_primitive_indices(self.ctx.handle, self.handle, out.ctypes.data_as(C.c_void_p)),
+              self.ctx.handle)
+        return out
+
+    def sort_trace(self):
+        """(Morton codes in leaf order, sorted position -> leaf index) of the last build (context trace on)."""
+        n = self._info()[1]
+        codes = np.zeros(n, dtype=np.uint64)
+        order = np.zeros(n, dtype=np.uint32)
+        check(lib().obvhs_bvh2_copy_sort_trace(self.ctx.handle, self.handle, codes.ctypes.data_as(C.c_void_p),
+                                               order.ctypes.data_as(C.c_void_p)), self.ctx.handle)
+        return codes, order
+
+    def close(self):
+        if self.handle:
+            lib().obvhs_bvh2_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PlocBuilder:
+    """ploc/mod.rs:34-159.  Keeps the context's scratch allocations between builds like the crate's builder."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def build(self, search_distance, aabbs, indices=None, sort_precision=SortPrecision.U64,
+              search_depth_threshold=0):
+        return self.build_with_bvh(None, search_distance, aabbs, indices, sort_precision, search_depth_threshold)
+
+    def build_with_bvh(self, bvh, search_distance, aabbs, indices=None, sort_precision=SortPrecision.U64,
+                       search_depth_threshold=0):
+        sd = PlocSearchDistance.from_u32(int(search_distance))
+        is_tris = (isinstance(aabbs, DeviceArray) and aabbs.dtype == TRIANGLE) or \
+            (isinstance(aabbs, np.ndarray) and aabbs.dtype == TRIANGLE)
+        dt = TRIANGLE if is_tris else AABB
+        d, n = _as_device(self.ctx, aabbs, dt)
+        di = None
+        if indices is not None:
+            di, ni = _as_device(self.ctx, np.asarray(indices, dtype=np.uint32) if not isinstance(indices, DeviceArray)
+                                else indices, np.uint32)
+            if ni != n:
+                raise ValueError("indices must have one entry per primitive")
+        out = C.c_void_p()
+        fn = lib().obvhs_ploc_build_triangles if is_tris else lib().obvhs_ploc_build_aabbs
+        reuse = bvh.handle if bvh is not None else None
+        check(fn(self.ctx.handle, d.ptr, di.ptr if di is not None else None, n, int(sd), int(sort_precision),
+                 int(search_depth_threshold), reuse, C.byref(out)), self.ctx.handle)
+        if bvh is not None:
+            return bvh
+        return Bvh2(self.ctx, out)
+
+
+class CwBvh:
+    """Handle of a device-resident CwBvh (cwbvh/mod.rs:43-55)."""
+
+    def __init__(self, ctx, handle, stats=None):
+        self.ctx = ctx
+        self.handle = handle
+        self.stats = stats
+
+    def _info(self):
+        nc, pc, bb = C.c_uint32(), C.c_uint32(), native.Aabb()
+        check(lib().obvhs_cwbvh_info(self.handle, C.byref(nc), C.byref(pc), C.byref(bb)))
+        return nc.value, pc.value, bb
+
+    @property
+    def node_count(self):
+        return self._info()[0]
+
+    @property
+    def total_aabb(self):
+        bb = self._info()[2]
+        return np.array(bb.min[:3], dtype=np.float32), np.array(bb.max[:3], dtype=np.float32)
+
+    @property
+    def nodes(self):
+        n = self._info()[0]
+        out = np.zeros(n, dtype=CWBVH_NODE)
+        check(lib().obvhs_cwbvh_copy_nodes(self.ctx.handle, self.handle, out.ctypes.data_as(C.c_void_p)),
+              self.ctx.handle)
+        return out
+
+    @property
+    def primitive_indices(self):
+        n = self._info()[1]
+        out = np.zeros(n, dtype=np.uint32)
+        check(lib().obvhs_cwbvh_copy_primitive_indices(self.ctx.handle, self.handle,
+                                                       out.ctypes.data_as(C.c_void_p)), self.ctx.handle)
+        return out
+
+    def reorder_triangles(self, tris):
+        """bvh_tris[i] = tris[primitive_indices[i]] on the device (examples/obj_cwbvh.rs:63-67)."""
+        d, n = _as_device(self.ctx, tris, TRIANGLE)
+        out = DeviceArray(self.ctx, TRIANGLE, self._info()[1])
+        check(lib().obvhs_cwbvh_reorder_triangles(self.ctx.handle, self.handle, d.ptr, out.ptr), self.ctx.handle)
+        return out
+
+    def ray_traverse(self, rays, bvh_tris, counted=False):
+        """CwBvh::ray_traverse for every ray (cwbvh/mod.rs:169-198).  bvh_tris in primitive_indices order.
+        Returns RayHit[n] (primitive_id indexes primitive_indices); with counted=True also per-ray node and
+        triangle test counts."""
+        dr, n = _as_device(self.ctx, rays, RAY)
+        dt, _ = _as_device(self.ctx, bvh_tris, TRIANGLE)
+        hits = DeviceArray(self.ctx, RAY_HIT, n)
+        if counted:
+            nv = DeviceArray(self.ctx, np.uint32, n)
+            tv = DeviceArray(self.ctx, np.uint32, n)
+            check(lib().obvhs_cwbvh_ray_traverse_counted(self.ctx.handle, self.handle, dt.ptr, dr.ptr, n, hits.ptr,
+                                                         nv.ptr, tv.ptr), self.ctx.handle)
+            return hits.to_host(), nv.to_host(), tv.to_host()
+        check(lib().obvhs_cwbvh_ray_traverse(self.ctx.handle, self.handle, dt.ptr, dr.ptr, n, hits.ptr),
+              self.ctx.handle)
+        return hits.to_host()
+
+    def ray_traverse_miss(self, rays, bvh_tris):
+        """CwBvh::ray_traverse_miss for every ray (cwbvh/mod.rs:201-225): True where the ray hit nothing."""
+        dr, n = _as_device(self.ctx, rays, RAY)
+        dt, _ = _as_device(self.ctx, bvh_tris, TRIANGLE)
+        miss = DeviceArray(self.ctx, np.uint8, n)
+        check(lib().obvhs_cwbvh_ray_traverse_miss(self.ctx.handle, self.handle, dt.ptr, dr.ptr, n, miss.ptr),
+              self.ctx.handle)
+        return miss.to_host().astype(bool)
+
+    def close(self):
+        if self.handle:
+            lib().obvhs_cwbvh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bvh2_to_cwbvh(bvh2, max_prims_per_leaf, order_children=True, include_exact_node_aabbs=False):
+    """cwbvh/bvh2_to_cwbvh.rs:490-510."""
+    out = C.c_void_p()
+    check(lib().obvhs_bvh2_to_cwbvh(bvh2.ctx.handle, bvh2.handle, int(max_prims_per_leaf), int(bool(order_children)),
+                                    int(bool(include_exact_node_aabbs)), None, C.byref(out)), bvh2.ctx.handle)
+    return CwBvh(bvh2.ctx, out)
+
+
+def build_cwbvh_from_tris(ctx, triangles, config, reuse=None):
+    """cwbvh/builder.rs:20-85 (pre-splits and reinsertion are not available in this version)."""
+    d, n = _as_device(ctx, triangles, TRIANGLE)
+    out = C.c_void_p()
+    stats = BuildStats()
+    params = config.to_c()
+    check(lib().obvhs_build_cwbvh_from_tris(ctx.handle, d.ptr, n, C.byref(params),
+                                            reuse.handle if reuse is not None else None, C.byref(out),
+                                            C.byref(stats)), ctx.handle)
+    if reuse is not None:
+        reuse.stats = stats
+        return reuse
+    return CwBvh(ctx, out, stats)
+
+
+def log2_threshold_table():
+    out = np.zeros(256, dtype=np.uint32)
+    check(lib().obvhs_log2_threshold_table(out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+__all__ = ["PlocSearchDistance", "SortPrecision", "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder",
+           "CwBvh", "bvh2_to_cwbvh", "build_cwbvh_from_tris", "log2_threshold_table", "ObvhsError"]

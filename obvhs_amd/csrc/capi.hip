@@ -1,0 +1,430 @@
+// C ABI of the obvhs MI355X library (declared in include/obvhs_hip.h): contexts, device memory helpers, the
+// PLOC / CWBVH builders and the traversal entry points.  Host orchestration only; kernels live in sort.hip,
+// ploc.hip, cwbvh.hip and traverse.hip.
+#include <string.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace obvhs {
+
+obvhs_status set_hip_error(obvhs_context* ctx, hipError_t e, const char* where) {
+    if (ctx) {
+        ctx->last_error = std::string(where) + ": " + hipGetErrorString(e);
+    }
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return OBVHS_ERR_OOM;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return OBVHS_ERR_NO_DEVICE;
+    return OBVHS_ERR_DEVICE;
+}
+
+obvhs_status device_error_to_status(uint32_t err) {
+    if (err & DERR_NAN_INPUT) return OBVHS_ERR_NAN_INPUT;
+    if (err & DERR_SELF_MERGE) return OBVHS_ERR_PLOC_SELF_MERGE;
+    if (err & DERR_CW_NONUNIT_LEAF) return OBVHS_ERR_CW_NONUNIT_LEAF;
+    if (err & DERR_CW_DISTRIBUTE) return OBVHS_ERR_CW_DISTRIBUTE;
+    if (err & DERR_CW_ORDER) return OBVHS_ERR_CW_ORDER;
+    if (err & DERR_CW_PRIMCOUNT) return OBVHS_ERR_CW_PRIMCOUNT;
+    if (err & DERR_CW_DEPTH) return OBVHS_ERR_DEPTH;
+    if (err & DERR_CAPACITY) return OBVHS_ERR_CAPACITY;
+    return OBVHS_ERR_DEVICE;
+}
+
+}  // namespace obvhs
+
+using namespace obvhs;
+
+#define CTX_TRY(expr, where)                                        \
+    do {                                                            \
+        hipError_t _e = (expr);                                     \
+        if (_e != hipSuccess) return set_hip_error(ctx, _e, where); \
+    } while (0)
+
+extern "C" {
+
+const char* obvhs_status_string(obvhs_status s) {
+    switch (s) {
+        case OBVHS_OK: return "ok";
+        case OBVHS_ERR_INVALID_ARG: return "invalid argument";
+        case OBVHS_ERR_NAN_INPUT: return "NaN in primitive bounds";
+        case OBVHS_ERR_PLOC_SELF_MERGE: return "PLOC cluster without a finite-cost neighbour";
+        case OBVHS_ERR_CW_DISTRIBUTE: return "CWBVH cost table without a finite distribution";
+        case OBVHS_ERR_CW_NONUNIT_LEAF: return "CWBVH conversion expects one primitive per BVH2 leaf";
+        case OBVHS_ERR_CW_ORDER: return "CWBVH child ordering left a child unassigned";
+        case OBVHS_ERR_CW_PRIMCOUNT: return "CWBVH leaf with an invalid primitive count";
+        case OBVHS_ERR_CAPACITY: return "internal capacity exceeded";
+        case OBVHS_ERR_DEPTH: return "CWBVH too deep";
+        case OBVHS_ERR_OOM: return "out of device memory";
+        case OBVHS_ERR_DEVICE: return "HIP runtime error";
+        case OBVHS_ERR_NO_DEVICE: return "no HIP device";
+    }
+    return "unknown status";
+}
+
+obvhs_status obvhs_context_create(int device, void* hip_stream, obvhs_context** out) {
+    if (!out) return OBVHS_ERR_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return OBVHS_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return OBVHS_ERR_INVALID_ARG;
+    obvhs_context* ctx = new (std::nothrow) obvhs_context();
+    if (!ctx) return OBVHS_ERR_OOM;
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) {
+        if (hip_stream) {
+            ctx->stream = static_cast<hipStream_t>(hip_stream);
+        } else {
+            e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+            ctx->own_stream = e == hipSuccess;
+        }
+    }
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_stop);
+    if (e == hipSuccess) e = hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault);
+    if (e == hipSuccess) e = upload_log2_table();
+    if (e != hipSuccess) {
+        obvhs_status s = set_hip_error(ctx, e, "context create");
+        obvhs_context_destroy(ctx);
+        return s;
+    }
+    *out = ctx;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_context_destroy(obvhs_context* ctx) {
+    if (!ctx) return OBVHS_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    BuildScratch& S = ctx->scratch;
+    S.leaves.release();
+    S.clusters_a.release();
+    S.clusters_b.release();
+    S.keys.release();
+    S.vals.release();
+    S.merge.release();
+    S.blocksums.release();
+    S.sort_scratch.release();
+    S.misc.release();
+    S.cw_scratch.release();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_context_synchronize(obvhs_context* ctx) {
+    if (!ctx) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipStreamSynchronize(ctx->stream), "synchronize");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_context_set_trace(obvhs_context* ctx, int enable) {
+    if (!ctx) return OBVHS_ERR_INVALID_ARG;
+    ctx->keep_trace = enable != 0;
+    return OBVHS_OK;
+}
+
+const char* obvhs_last_error_message(const obvhs_context* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+obvhs_status obvhs_device_malloc(obvhs_context* ctx, uint64_t bytes, void** dptr) {
+    if (!ctx || !dptr) return OBVHS_ERR_INVALID_ARG;
+    *dptr = nullptr;
+    CTX_TRY(hipMalloc(dptr, bytes ? bytes : 1), "device malloc");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_device_free(obvhs_context* ctx, void* dptr) {
+    if (!ctx) return OBVHS_ERR_INVALID_ARG;
+    if (dptr) CTX_TRY(hipFree(dptr), "device free");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_memcpy_htod(obvhs_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx) return OBVHS_ERR_INVALID_ARG;
+    if (!bytes) return OBVHS_OK;
+    CTX_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "memcpy htod");
+    CTX_TRY(hipStreamSynchronize(ctx->stream), "memcpy htod sync");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_memcpy_dtoh(obvhs_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx) return OBVHS_ERR_INVALID_ARG;
+    if (!bytes) return OBVHS_OK;
+    CTX_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "memcpy dtoh");
+    CTX_TRY(hipStreamSynchronize(ctx->stream), "memcpy dtoh sync");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_memset_device(obvhs_context* ctx, void* dst, int value, uint64_t bytes) {
+    if (!ctx) return OBVHS_ERR_INVALID_ARG;
+    if (!bytes) return OBVHS_OK;
+    CTX_TRY(hipMemsetAsync(dst, value, bytes, ctx->stream), "memset");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_timer_start(obvhs_context* ctx) {
+    if (!ctx) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipEventRecord(ctx->ev_start, ctx->stream), "timer start");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_timer_stop(obvhs_context* ctx, float* ms) {
+    if (!ctx || !ms) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipEventRecord(ctx->ev_stop, ctx->stream), "timer stop");
+    CTX_TRY(hipEventSynchronize(ctx->ev_stop), "timer sync");
+    CTX_TRY(hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop), "timer elapsed");
+    return OBVHS_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// BVH2
+// ---------------------------------------------------------------------------------------------------------------
+
+static obvhs_status ploc_entry(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
+                               const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
+                               uint32_t sort_precision, uint32_t search_depth_threshold, obvhs_bvh2* reuse,
+                               obvhs_bvh2** out) {
+    if (!ctx || !out) return OBVHS_ERR_INVALID_ARG;
+    if (n > 0 && !d_tris && !d_aabbs) return OBVHS_ERR_INVALID_ARG;
+    if (n >= (1u << 31)) return OBVHS_ERR_INVALID_ARG;
+    if (sort_precision != 64u) return OBVHS_ERR_INVALID_ARG;  // SortPrecision::U64 only in this version
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    obvhs_bvh2* bvh = reuse ? reuse : new (std::nothrow) obvhs_bvh2();
+    if (!bvh) return OBVHS_ERR_OOM;
+    obvhs_status s = ploc_build(ctx, d_tris, d_aabbs, d_indices, n, search_distance, search_depth_threshold, bvh);
+    if (s != OBVHS_OK) {
+        if (!reuse) obvhs_bvh2_destroy(bvh);
+        *out = nullptr;
+        return s;
+    }
+    *out = bvh;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_ploc_build_aabbs(obvhs_context* ctx, const obvhs_aabb* d_aabbs, const uint32_t* d_indices,
+                                    uint32_t n, uint32_t search_distance, uint32_t sort_precision,
+                                    uint32_t search_depth_threshold, obvhs_bvh2* reuse, obvhs_bvh2** out) {
+    return ploc_entry(ctx, nullptr, d_aabbs, d_indices, n, search_distance, sort_precision, search_depth_threshold,
+                      reuse, out);
+}
+
+obvhs_status obvhs_ploc_build_triangles(obvhs_context* ctx, const obvhs_triangle* d_tris, const uint32_t* d_indices,
+                                        uint32_t n, uint32_t search_distance, uint32_t sort_precision,
+                                        uint32_t search_depth_threshold, obvhs_bvh2* reuse, obvhs_bvh2** out) {
+    return ploc_entry(ctx, d_tris, nullptr, d_indices, n, search_distance, sort_precision, search_depth_threshold,
+                      reuse, out);
+}
+
+obvhs_status obvhs_bvh2_info(const obvhs_bvh2* bvh, uint32_t* node_count, uint32_t* prim_count,
+                             uint32_t* max_depth) {
+    if (!bvh) return OBVHS_ERR_INVALID_ARG;
+    if (node_count) *node_count = bvh->node_count;
+    if (prim_count) *prim_count = bvh->prim_count;
+    if (max_depth) *max_depth = bvh->max_depth;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_bvh2_copy_nodes_small(obvhs_context* ctx, const obvhs_bvh2* bvh, obvhs_bvh2_node_small* host_out) {
+    if (!ctx || !bvh || (!host_out && bvh->node_count)) return OBVHS_ERR_INVALID_ARG;
+    if (!bvh->node_count) return OBVHS_OK;
+    return obvhs_memcpy_dtoh(ctx, host_out, bvh->nodes.ptr, (size_t)bvh->node_count * sizeof(Node32));
+}
+
+obvhs_status obvhs_bvh2_copy_nodes(obvhs_context* ctx, const obvhs_bvh2* bvh, obvhs_bvh2_node* host_out) {
+    if (!ctx || !bvh || (!host_out && bvh->node_count)) return OBVHS_ERR_INVALID_ARG;
+    if (!bvh->node_count) return OBVHS_OK;
+    std::vector<Node32> tmp(bvh->node_count);
+    obvhs_status s = obvhs_memcpy_dtoh(ctx, tmp.data(), bvh->nodes.ptr, tmp.size() * sizeof(Node32));
+    if (s != OBVHS_OK) return s;
+    for (size_t i = 0; i < tmp.size(); ++i) {
+        const Node32& a = tmp[i];
+        obvhs_bvh2_node& o = host_out[i];
+        o.aabb.min = obvhs_vec3a{a.minx, a.miny, a.minz, 0.0f};
+        o.aabb.max = obvhs_vec3a{a.maxx, a.maxy, a.maxz, 0.0f};
+        o.prim_count = a.prim_count;
+        o.first_index = a.first_index;
+        o.meta1 = 0u;
+        o.meta2 = 0u;
+    }
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_bvh2_copy_primitive_indices(obvhs_context* ctx, const obvhs_bvh2* bvh, uint32_t* host_out) {
+    if (!ctx || !bvh) return OBVHS_ERR_INVALID_ARG;
+    return obvhs_memcpy_dtoh(ctx, host_out, bvh->prim_indices.ptr, (size_t)bvh->prim_count * 4);
+}
+
+obvhs_status obvhs_bvh2_copy_sort_trace(obvhs_context* ctx, const obvhs_bvh2* bvh, uint64_t* host_codes,
+                                        uint32_t* host_sorted_index) {
+    if (!ctx || !bvh) return OBVHS_ERR_INVALID_ARG;
+    if (!bvh->prim_count) return OBVHS_OK;
+    if (!bvh->codes.ptr || !bvh->sorted_index.ptr) return OBVHS_ERR_INVALID_ARG;  // build without trace
+    obvhs_status s = OBVHS_OK;
+    if (host_codes) s = obvhs_memcpy_dtoh(ctx, host_codes, bvh->codes.ptr, (size_t)bvh->prim_count * 8);
+    if (s == OBVHS_OK && host_sorted_index)
+        s = obvhs_memcpy_dtoh(ctx, host_sorted_index, bvh->sorted_index.ptr, (size_t)bvh->prim_count * 4);
+    return s;
+}
+
+obvhs_status obvhs_bvh2_destroy(obvhs_bvh2* bvh) {
+    if (!bvh) return OBVHS_OK;
+    bvh->nodes.release();
+    bvh->prim_indices.release();
+    bvh->codes.release();
+    bvh->sorted_index.release();
+    delete bvh;
+    return OBVHS_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// CWBVH
+// ---------------------------------------------------------------------------------------------------------------
+
+obvhs_status obvhs_bvh2_to_cwbvh(obvhs_context* ctx, const obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,
+                                 int order_children, int include_exact_node_aabbs, obvhs_cwbvh* reuse,
+                                 obvhs_cwbvh** out) {
+    if (!ctx || !bvh2 || !out) return OBVHS_ERR_INVALID_ARG;
+    if (include_exact_node_aabbs) return OBVHS_ERR_INVALID_ARG;
+    if (max_prims_per_leaf > 3) return OBVHS_ERR_INVALID_ARG;  // bvh2_to_cwbvh.rs:489 (0..=3)
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    obvhs_cwbvh* cw = reuse ? reuse : new (std::nothrow) obvhs_cwbvh();
+    if (!cw) return OBVHS_ERR_OOM;
+    obvhs_status s = cwbvh_convert(ctx, bvh2, max_prims_per_leaf, order_children != 0, cw);
+    if (s != OBVHS_OK) {
+        if (!reuse) obvhs_cwbvh_destroy(cw);
+        *out = nullptr;
+        return s;
+    }
+    *out = cw;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_build_cwbvh_from_tris(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
+                                         const obvhs_build_params* params, obvhs_cwbvh* reuse, obvhs_cwbvh** out,
+                                         obvhs_build_stats* stats) {
+    if (!ctx || !params || !out) return OBVHS_ERR_INVALID_ARG;
+    if (params->pre_split || params->reinsertion_batch_ratio > 0.0f) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    obvhs_cwbvh* cw = reuse ? reuse : new (std::nothrow) obvhs_cwbvh();
+    if (!cw) return OBVHS_ERR_OOM;
+    obvhs_bvh2* b2 = cw->owned_bvh2;
+    obvhs_status s = ploc_entry(ctx, d_tris, nullptr, nullptr, n, params->ploc_search_distance,
+                                params->sort_precision, params->search_depth_threshold, b2, &b2);
+    if (s == OBVHS_OK) {
+        cw->owned_bvh2 = b2;
+        uint32_t mp = params->max_prims_per_leaf;
+        mp = mp < 1u ? 1u : (mp > 3u ? 3u : mp);  // cwbvh/builder.rs:74 clamp(1, 3)
+        s = cwbvh_convert(ctx, b2, mp, true, cw);
+    }
+    if (s != OBVHS_OK) {
+        if (!reuse) obvhs_cwbvh_destroy(cw);
+        *out = nullptr;
+        return s;
+    }
+    if (stats) {
+        stats->prim_count = n;
+        stats->ploc_iterations = b2 ? b2->ploc_iterations : 0;
+        stats->bvh2_node_count = b2 ? b2->node_count : 0;
+        stats->cwbvh_node_count = cw->node_count;
+        stats->ploc_cluster_sum = b2 ? b2->cluster_sum : 0;
+    }
+    *out = cw;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_cwbvh_info(const obvhs_cwbvh* bvh, uint32_t* node_count, uint32_t* prim_count,
+                              obvhs_aabb* total_aabb) {
+    if (!bvh) return OBVHS_ERR_INVALID_ARG;
+    if (node_count) *node_count = bvh->node_count;
+    if (prim_count) *prim_count = bvh->prim_count;
+    if (total_aabb) *total_aabb = bvh->total_aabb;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_cwbvh_copy_nodes(obvhs_context* ctx, const obvhs_cwbvh* bvh, obvhs_cwbvh_node* host_out) {
+    if (!ctx || !bvh) return OBVHS_ERR_INVALID_ARG;
+    return obvhs_memcpy_dtoh(ctx, host_out, bvh->nodes.ptr, (size_t)bvh->node_count * sizeof(CwNode));
+}
+
+obvhs_status obvhs_cwbvh_copy_primitive_indices(obvhs_context* ctx, const obvhs_cwbvh* bvh, uint32_t* host_out) {
+    if (!ctx || !bvh) return OBVHS_ERR_INVALID_ARG;
+    return obvhs_memcpy_dtoh(ctx, host_out, bvh->prim_indices.ptr, (size_t)bvh->prim_count * 4);
+}
+
+obvhs_status obvhs_cwbvh_device_arrays(const obvhs_cwbvh* bvh, const obvhs_cwbvh_node** d_nodes,
+                                       const uint32_t** d_primitive_indices) {
+    if (!bvh) return OBVHS_ERR_INVALID_ARG;
+    if (d_nodes) *d_nodes = static_cast<const obvhs_cwbvh_node*>(bvh->nodes.ptr);
+    if (d_primitive_indices) *d_primitive_indices = static_cast<const uint32_t*>(bvh->prim_indices.ptr);
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_cwbvh_reorder_triangles(obvhs_context* ctx, const obvhs_cwbvh* bvh, const obvhs_triangle* d_tris,
+                                           obvhs_triangle* d_bvh_tris) {
+    if (!ctx || !bvh || (bvh->prim_count && (!d_tris || !d_bvh_tris))) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(launch_reorder_triangles(d_tris, bvh->prim_indices.as<uint32_t>(), bvh->prim_count, d_bvh_tris,
+                                     ctx->stream),
+            "reorder triangles");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_cwbvh_destroy(obvhs_cwbvh* bvh) {
+    if (!bvh) return OBVHS_OK;
+    bvh->nodes.release();
+    bvh->prim_indices.release();
+    if (bvh->owned_bvh2) obvhs_bvh2_destroy(bvh->owned_bvh2);
+    delete bvh;
+    return OBVHS_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Traversal
+// ---------------------------------------------------------------------------------------------------------------
+
+obvhs_status obvhs_cwbvh_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh* bvh, const obvhs_triangle* d_bvh_tris,
+                                      const obvhs_ray* d_rays, uint32_t n, obvhs_ray_hit* d_hits) {
+    if (!ctx || !bvh || (n && (!d_rays || !d_hits))) return OBVHS_ERR_INVALID_ARG;
+    if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0, d_hits, nullptr,
+                            nullptr, nullptr, ctx->stream),
+            "traverse");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_cwbvh_ray_traverse_miss(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                           const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays, uint32_t n,
+                                           uint8_t* d_miss) {
+    if (!ctx || !bvh || (n && (!d_rays || !d_miss))) return OBVHS_ERR_INVALID_ARG;
+    if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 1, nullptr, d_miss,
+                            nullptr, nullptr, ctx->stream),
+            "traverse miss");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                              const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays, uint32_t n,
+                                              obvhs_ray_hit* d_hits, uint32_t* d_node_visits,
+                                              uint32_t* d_tri_tests) {
+    if (!ctx || !bvh || (n && (!d_rays || !d_hits || !d_node_visits || !d_tri_tests))) return OBVHS_ERR_INVALID_ARG;
+    if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0, d_hits, nullptr,
+                            d_node_visits, d_tri_tests, ctx->stream),
+            "traverse counted");
+    return OBVHS_OK;
+}
+
+// The CWBVH exponent threshold table (host copy; built from the host libm on first use, no device needed).
+obvhs_status obvhs_log2_threshold_table(uint32_t* out256) {
+    if (!out256) return OBVHS_ERR_INVALID_ARG;
+    memcpy(out256, host_log2_table(), 256 * sizeof(uint32_t));
+    return OBVHS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// Host-side declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/obvhs_hip.h"
+
+namespace obvhs {
+
+struct Node32;
+struct CwNode;
+
+// Grow-only device allocation owned by a context or a handle.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t want);
+    void release();
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+// PLOC per-pass state, double buffered on the device (pass k reads slot k&1, writes slot (k+1)&1).
+struct PlocState {
+    uint32_t count;
+    uint32_t insert;
+    uint32_t depth;
+    uint32_t pad;
+    unsigned long long cluster_sum;
+};
+
+struct BuildScratch {
+    DevBuf leaves;      // Node32[n] leaf nodes in input order
+    DevBuf clusters_a;  // Node32[n]
+    DevBuf clusters_b;  // Node32[n]
+    DevBuf keys;        // u64[n]
+    DevBuf vals;        // u32[n]
+    DevBuf merge;       // i8[n]
+    DevBuf blocksums;   // uint2[n/256+1]
+    DevBuf sort_scratch;
+    DevBuf misc;        // bounds keys, states, error word
+    DevBuf cw_scratch;  // conversion working memory
+};
+
+}  // namespace obvhs
+
+struct obvhs_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    obvhs::BuildScratch scratch;
+    void* pinned = nullptr;  // small pinned host staging area
+    bool keep_trace = false; // keep Morton codes and the sort permutation of each build (parity tests)
+    std::string last_error;
+};
+
+struct obvhs_bvh2 {
+    obvhs::DevBuf nodes;           // Node32[2n-1]
+    obvhs::DevBuf prim_indices;    // u32[n]
+    obvhs::DevBuf codes;           // u64[n] Morton codes (original leaf order), for parity tests
+    obvhs::DevBuf sorted_index;    // u32[n]
+    uint32_t node_count = 0;
+    uint32_t prim_count = 0;
+    uint32_t max_depth = 96;
+    uint32_t ploc_iterations = 0;
+    uint64_t cluster_sum = 0;
+};
+
+struct obvhs_cwbvh {
+    obvhs::DevBuf nodes;          // CwNode[node_count]
+    obvhs::DevBuf prim_indices;   // u32[prim_count]
+    uint32_t node_count = 0;
+    uint32_t prim_count = 0;
+    obvhs_aabb total_aabb{};
+    obvhs_bvh2* owned_bvh2 = nullptr;  // intermediate BVH2 kept for reuse by build_cwbvh_from_tris
+};
+
+namespace obvhs {
+
+hipError_t launch_iota(uint32_t* p, uint32_t n, hipStream_t stream);
+
+// sort.hip
+size_t radix_sort_scratch_bytes(uint32_t n);
+hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identity, uint32_t n, int bits,
+                            void* scratch, hipStream_t stream);
+
+// ploc.hip
+obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
+                        const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
+                        uint32_t search_depth_threshold, obvhs_bvh2* bvh);
+
+// cwbvh.hip
+obvhs_status cwbvh_convert(obvhs_context* ctx, const obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,
+                           bool order_children, obvhs_cwbvh* out);
+const uint32_t* host_log2_table();
+hipError_t upload_log2_table();
+hipError_t launch_reorder_triangles(const obvhs_triangle* tris, const uint32_t* prim_indices, uint32_t n,
+                                    obvhs_triangle* out, hipStream_t stream);
+
+// traverse.hip
+hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
+                           const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,
+                           uint32_t* node_visits, uint32_t* tri_tests, hipStream_t stream);
+
+// capi.cpp
+obvhs_status set_hip_error(obvhs_context* ctx, hipError_t e, const char* where);
+obvhs_status device_error_to_status(uint32_t err_bits);
+
+}  // namespace obvhs

@@ -1,0 +1,712 @@
+// PLOC BVH2 construction on gfx950.
+//
+// Restates PlocBuilder::build_ploc / build_ploc_from_leaves (ploc/mod.rs:169-503) as a sequence of data-parallel
+// passes whose outputs are bit-identical to the reference's serial loop:
+//   k_leaf_init_*  : leaf Bvh2Node per primitive (ploc/mod.rs:187-243) + scene bounds (atomic min/max on
+//                    order-preserving keys; the zero sign of the bounds cannot change a Morton code)
+//   k_morton       : 63-bit Morton code of the f32 centre normalised in f64 (ploc/mod.rs:287-288, 782-785;
+//                    morton.rs:35-58)
+//   radix sort     : stable (sort.hip)
+//   k_gather       : sorted leaves become the first cluster array (ploc/mod.rs:829-846)
+//   per pass       : k_ploc_search (ploc/mod.rs:329-419; window in LDS), k_ploc_count + k_ploc_scan (prefix
+//                    counts of emitted clusters and of merges), k_ploc_compact (ploc/mod.rs:420-487: the parent
+//                    of a mutual pair is emitted at the higher index, its children go to
+//                    nodes[insert - 2(s+1)] and [+1] where s is the merge's rank in the pass)
+//   k_ploc_tail    : once at most 4096 clusters remain, one workgroup finishes every remaining pass with the
+//                    clusters resident in LDS and writes the root (ploc/mod.rs:499)
+// Pass state (cluster count, insert index, depth) lives on the device so passes are enqueued without host syncs;
+// the host only reads the count every few passes to shrink the grids.
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace obvhs {
+
+namespace {
+
+constexpr int T256 = 256;
+constexpr uint32_t TAIL_MAX = 4096;
+constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_ITEMS = TAIL_MAX / TAIL_THREADS;
+constexpr uint32_t MAX_RADIUS = 32;
+constexpr int PASSES_PER_SYNC = 4;
+
+struct Misc {
+    uint32_t bounds[6];  // order keys: min x,y,z (atomicMin) then max x,y,z (atomicMax)
+    uint32_t err;
+    uint32_t pad;
+    PlocState state[2];
+};
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t t = (uint32_t)__shfl_up((int)v, off);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+__global__ void k_init_misc(Misc* misc, uint32_t n) {
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; ++a) {
+            misc->bounds[a] = 0xffffffffu;
+            misc->bounds[3 + a] = 0u;
+        }
+        misc->err = 0u;
+        misc->state[0] = PlocState{n, 2u * n - 1u, 0u, 0u, 0ull};
+        misc->state[1] = PlocState{n, 2u * n - 1u, 0u, 0u, 0ull};
+    }
+}
+
+// Reduces the leaf bounds of one block into the scene bounds.  Both the min and the max corner of every leaf go
+// into both reductions, like `local_aabb.extend(aabb.min); local_aabb.extend(aabb.max)` (ploc/mod.rs:189-190).
+__device__ __forceinline__ void reduce_scene_bounds(bool valid, const Box& b, bool nan, Misc* misc) {
+    __shared__ uint32_t red[6][T256 / 64];
+    uint32_t k[6];
+    if (valid) {
+        uint32_t kl[3] = {f32_order_key(b.lx), f32_order_key(b.ly), f32_order_key(b.lz)};
+        uint32_t kh[3] = {f32_order_key(b.hx), f32_order_key(b.hy), f32_order_key(b.hz)};
+        for (int a = 0; a < 3; ++a) {
+            k[a] = min(kl[a], kh[a]);
+            k[3 + a] = max(kl[a], kh[a]);
+        }
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            k[a] = 0xffffffffu;
+            k[3 + a] = 0u;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        k[a] = wave_min_u32(k[a]);
+        k[3 + a] = wave_max_u32(k[3 + a]);
+    }
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (lane == 0)
+        for (int a = 0; a < 6; ++a) red[a][wave] = k[a];
+    if (__ballot(nan) != 0ull && lane == 0) atomicOr(&misc->err, DERR_NAN_INPUT);
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        uint32_t v = red[a][0];
+        for (int w = 1; w < T256 / 64; ++w) v = a < 3 ? min(v, red[a][w]) : max(v, red[a][w]);
+        if (a < 3) {
+            if (v != 0xffffffffu) atomicMin(&misc->bounds[a], v);
+        } else {
+            if (v != 0u) atomicMax(&misc->bounds[a], v);
+        }
+    }
+}
+
+__device__ __forceinline__ bool box_has_nan(const Box& b) {
+    return isnan(b.lx) || isnan(b.ly) || isnan(b.lz) || isnan(b.hx) || isnan(b.hy) || isnan(b.hz);
+}
+
+// Triangle::aabb = Aabb::from_points([v0, v1, v2]) (triangle.rs:28-30; aabb.rs:59-66).
+__global__ void __launch_bounds__(T256) k_leaf_init_tris(const float4* __restrict__ tris,
+                                                         const uint32_t* __restrict__ indices, uint32_t n,
+                                                         Node32* __restrict__ leaves, Misc* misc) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    const bool valid = i < n;
+    Box b{0, 0, 0, 0, 0, 0};
+    bool nan = false;
+    if (valid) {
+        float4 v0 = tris[3 * (size_t)i], v1 = tris[3 * (size_t)i + 1], v2 = tris[3 * (size_t)i + 2];
+        b = Box{v0.x, v0.y, v0.z, v0.x, v0.y, v0.z};
+        b = Box{smin(b.lx, v1.x), smin(b.ly, v1.y), smin(b.lz, v1.z), smax(b.hx, v1.x), smax(b.hy, v1.y),
+                smax(b.hz, v1.z)};
+        b = Box{smin(b.lx, v2.x), smin(b.ly, v2.y), smin(b.lz, v2.z), smax(b.hx, v2.x), smax(b.hy, v2.y),
+                smax(b.hz, v2.z)};
+        nan = box_has_nan(b);
+        store_node(&leaves[i], make_node(b, 1u, indices ? indices[i] : i));
+    }
+    reduce_scene_bounds(valid, b, nan, misc);
+}
+
+// T = Aabb: the leaf box is the input box (aabb.rs:209-214).
+__global__ void __launch_bounds__(T256) k_leaf_init_aabbs(const float4* __restrict__ aabbs,
+                                                          const uint32_t* __restrict__ indices, uint32_t n,
+                                                          Node32* __restrict__ leaves, Misc* misc) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    const bool valid = i < n;
+    Box b{0, 0, 0, 0, 0, 0};
+    bool nan = false;
+    if (valid) {
+        float4 lo = aabbs[2 * (size_t)i], hi = aabbs[2 * (size_t)i + 1];
+        b = Box{lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+        nan = box_has_nan(b);
+        store_node(&leaves[i], make_node(b, 1u, indices ? indices[i] : i));
+    }
+    reduce_scene_bounds(valid, b, nan, misc);
+}
+
+// morton.rs:35-43
+__device__ __forceinline__ uint64_t split_by_3_u64(uint32_t a) {
+    uint64_t x = (uint64_t)a & 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+// Rust `f64 as u32` (saturating, NaN -> 0).
+__device__ __forceinline__ uint32_t f64_as_u32(double v) {
+    if (!(v >= 1.0)) return 0u;
+    if (v >= 4294967296.0) return 0xffffffffu;
+    return (uint32_t)v;
+}
+
+// ploc/mod.rs:287-288 and 782-785; morton.rs:55-58.
+__global__ void __launch_bounds__(T256) k_morton(const Node32* __restrict__ leaves, uint32_t n,
+                                                 const Misc* __restrict__ misc, uint64_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= n) return;
+    const float mnx = f32_from_order_key(misc->bounds[0]), mny = f32_from_order_key(misc->bounds[1]),
+                mnz = f32_from_order_key(misc->bounds[2]);
+    const float mxx = f32_from_order_key(misc->bounds[3]), mxy = f32_from_order_key(misc->bounds[4]),
+                mxz = f32_from_order_key(misc->bounds[5]);
+    const double sx = 1.0 / (double)(mxx - mnx), sy = 1.0 / (double)(mxy - mny), sz = 1.0 / (double)(mxz - mnz);
+    const double ox = -(double)mnx * sx, oy = -(double)mny * sy, oz = -(double)mnz * sz;
+    Node32 l = load_node(&leaves[i]);
+    const float cx = (l.maxx + l.minx) * 0.5f, cy = (l.maxy + l.miny) * 0.5f, cz = (l.maxz + l.minz) * 0.5f;
+    const double px = (double)cx * sx + ox, py = (double)cy * sy + oy, pz = (double)cz * sz + oz;
+    const double s = 2097152.0;
+    keys[i] = split_by_3_u64(f64_as_u32(px * s)) | split_by_3_u64(f64_as_u32(py * s)) << 1 |
+              split_by_3_u64(f64_as_u32(pz * s)) << 2;
+}
+
+__global__ void __launch_bounds__(T256) k_gather(const Node32* __restrict__ leaves,
+                                                 const uint32_t* __restrict__ order, uint32_t n,
+                                                 Node32* __restrict__ out) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= n) return;
+    store_node(&out[i], load_node(&leaves[order[i]]));
+}
+
+__device__ __forceinline__ float window_cost(const float (*s)[T256 + 2 * MAX_RADIUS], const Box& me, int j) {
+    Box o{s[0][j], s[1][j], s[2][j], s[3][j], s[4][j], s[5][j]};
+    return half_area(box_union(me, o));
+}
+
+// One PLOC neighbour pass: ploc/mod.rs:329-419.  The best neighbour is the LAST minimum over the window in
+// ascending index order (`<=`, find_best_node_basic ploc/mod.rs:508-528, equal to the cached search).
+__global__ void __launch_bounds__(T256) k_ploc_search(const Node32* __restrict__ cur, Misc* misc, uint32_t slot,
+                                                      uint32_t r, uint32_t dist1, int8_t* __restrict__ merge) {
+    __shared__ float s[6][T256 + 2 * MAX_RADIUS];
+    const uint32_t count = misc->state[slot].count;
+    const uint32_t base = blockIdx.x * T256;
+    if (count <= 1 || base >= count) return;
+    const int wstart = (int)base - (int)r;
+    const int wlen = T256 + 2 * (int)r;
+    for (int w = threadIdx.x; w < wlen; w += T256) {
+        int g = wstart + w;
+        if (g >= 0 && (uint32_t)g < count) {
+            Node32 nd = load_node(&cur[g]);
+            s[0][w] = nd.minx;
+            s[1][w] = nd.miny;
+            s[2][w] = nd.minz;
+            s[3][w] = nd.maxx;
+            s[4][w] = nd.maxy;
+            s[5][w] = nd.maxz;
+        }
+    }
+    __syncthreads();
+    const uint32_t i = base + threadIdx.x;
+    if (i >= count) return;
+    const int li = (int)threadIdx.x + (int)r;
+    const Box me{s[0][li], s[1][li], s[2][li], s[3][li], s[4][li], s[5][li]};
+    int8_t m;
+    if (dist1) {
+        if (i == count - 1) {
+            m = -1;
+        } else {
+            float cost = window_cost(s, me, li + 1);
+            float last = INFINITY;
+            if (i > 0) {
+                Box prev{s[0][li - 1], s[1][li - 1], s[2][li - 1], s[3][li - 1], s[4][li - 1], s[5][li - 1]};
+                last = half_area(box_union(prev, me));
+            }
+            m = last < cost ? (int8_t)-1 : (int8_t)1;
+        }
+    } else {
+        const uint32_t begin = i - min(r, i);
+        const uint32_t end = min(i + r + 1u, count);
+        uint32_t best = i;
+        float best_cost = INFINITY;
+        for (uint32_t j = begin; j < end; ++j) {
+            if (j == i) continue;
+            float c = window_cost(s, me, (int)j - wstart);
+            if (c <= best_cost) {
+                best = j;
+                best_cost = c;
+            }
+        }
+        if (best == i) atomicOr(&misc->err, DERR_SELF_MERGE);
+        m = (int8_t)((int)best - (int)i);
+    }
+    merge[i] = m;
+}
+
+// Classification of cluster i (ploc/mod.rs:423-439): `emit` when it goes to the next pass (kept, or the higher
+// index of a mutual pair, which becomes the parent), `mrg` when it is the higher index of a mutual pair.
+// A self-pair (no finite neighbour, already flagged as an error) is treated as kept so that no slot underflows.
+__device__ __forceinline__ void classify(const int8_t* merge, uint32_t i, uint32_t count, bool& emit, bool& mrg,
+                                         uint32_t& b) {
+    emit = false;
+    mrg = false;
+    b = i;
+    if (i >= count) return;
+    if (count == 1) {
+        emit = true;
+        return;
+    }
+    int m = merge[i];
+    int bi = (int)i + m;
+    b = (uint32_t)bi;
+    bool mutual = bi + (int)merge[bi] == (int)i;
+    if (!mutual || bi == (int)i) {
+        emit = true;
+    } else if (bi < (int)i) {
+        emit = true;
+        mrg = true;
+    }
+}
+
+__global__ void __launch_bounds__(T256) k_ploc_count(const int8_t* __restrict__ merge, const Misc* misc,
+                                                     uint32_t slot, uint2* __restrict__ blocksums) {
+    __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
+    const uint32_t count = misc->state[slot].count;
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    bool emit, mrg;
+    uint32_t b;
+    classify(merge, i, count, emit, mrg, b);
+    const uint32_t e = (uint32_t)__popcll(__ballot(emit));
+    const uint32_t m = (uint32_t)__popcll(__ballot(mrg));
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        we[wave] = e;
+        wm[wave] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t E = 0, M = 0;
+        for (int w = 0; w < T256 / 64; ++w) {
+            E += we[w];
+            M += wm[w];
+        }
+        blocksums[blockIdx.x] = make_uint2(E, M);
+    }
+}
+
+// Single workgroup: exclusive scan of the per-block (emit, merge) counts and the next pass's state.
+__global__ void __launch_bounds__(1024) k_ploc_scan(uint2* __restrict__ blocksums, uint32_t nblocks, Misc* misc,
+                                                    uint32_t slot) {
+    __shared__ uint32_t se[16], sm[16];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t chunk = (nblocks + 1023u) / 1024u;
+    const uint32_t b0 = tid * chunk, b1 = min(nblocks, b0 + chunk);
+    uint32_t E = 0, M = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint2 v = blocksums[b];
+        E += v.x;
+        M += v.y;
+    }
+    uint32_t ie = wave_inclusive_scan(E), im = wave_inclusive_scan(M);
+    const uint32_t wave = tid >> 6, lane = tid & 63u;
+    if (lane == 63) {
+        se[wave] = ie;
+        sm[wave] = im;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t re = 0, rm = 0;
+        for (int w = 0; w < 16; ++w) {
+            uint32_t te = se[w], tm = sm[w];
+            se[w] = re;
+            sm[w] = rm;
+            re += te;
+            rm += tm;
+        }
+        const PlocState st = misc->state[slot];
+        PlocState nx = st;
+        if (st.count > 1) {
+            nx.count = re;
+            nx.insert = st.insert - 2u * rm;
+            nx.depth = st.depth + 1u;
+            nx.cluster_sum = st.cluster_sum + st.count;
+        }
+        misc->state[slot ^ 1u] = nx;
+    }
+    __syncthreads();
+    uint32_t ee = se[wave] + ie - E, em = sm[wave] + im - M;
+    for (uint32_t b = b0; b < b1; ++b) {
+        uint2 v = blocksums[b];
+        blocksums[b] = make_uint2(ee, em);
+        ee += v.x;
+        em += v.y;
+    }
+}
+
+__global__ void __launch_bounds__(T256) k_ploc_compact(const Node32* __restrict__ cur, Node32* __restrict__ next,
+                                                       const int8_t* __restrict__ merge, Misc* misc, uint32_t slot,
+                                                       const uint2* __restrict__ offsets,
+                                                       Node32* __restrict__ bvh_nodes) {
+    __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
+    const PlocState st = misc->state[slot];
+    const uint32_t count = st.count;
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    bool emit, mrg;
+    uint32_t b;
+    classify(merge, i, count, emit, mrg, b);
+    const uint64_t be = __ballot(emit), bm = __ballot(mrg);
+    const uint64_t lt = lanemask_lt();
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        we[wave] = (uint32_t)__popcll(be);
+        wm[wave] = (uint32_t)__popcll(bm);
+    }
+    __syncthreads();
+    if (!emit) return;
+    uint32_t pe = offsets[blockIdx.x].x, pm = offsets[blockIdx.x].y;
+    for (uint32_t w = 0; w < wave; ++w) {
+        pe += we[w];
+        pm += wm[w];
+    }
+    const uint32_t pos = pe + (uint32_t)__popcll(be & lt);
+    const Node32 a = load_node(&cur[i]);
+    if (mrg) {
+        const uint32_t s = pm + (uint32_t)__popcll(bm & lt);
+        const Node32 c = load_node(&cur[b]);
+        const uint32_t first = st.insert - 2u * (s + 1u);
+        store_node(&bvh_nodes[first], a);
+        store_node(&bvh_nodes[first + 1], c);
+        store_node(&next[pos], make_node(box_union(node_box(a), node_box(c)), 0u, first));
+    } else {
+        store_node(&next[pos], a);
+    }
+}
+
+// Finishes the build inside one workgroup once count <= TAIL_MAX.  Clusters live in LDS as SoA.
+__global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __restrict__ cur, Misc* misc,
+                                                            uint32_t slot, uint32_t first_pass, uint32_t r,
+                                                            uint32_t sd_is_one, uint32_t depth_threshold,
+                                                            Node32* __restrict__ bvh_nodes) {
+    __shared__ float lx[TAIL_MAX], ly[TAIL_MAX], lz[TAIL_MAX], hx[TAIL_MAX], hy[TAIL_MAX], hz[TAIL_MAX];
+    __shared__ uint32_t pc[TAIL_MAX], fi[TAIL_MAX];
+    __shared__ int8_t mg[TAIL_MAX];
+    __shared__ uint32_t wsum[TAIL_THREADS / 64];
+    __shared__ uint32_t s_total;
+    const uint32_t tid = threadIdx.x;
+    PlocState st = misc->state[slot];
+    uint32_t count = st.count;
+    if (count > TAIL_MAX) {
+        if (tid == 0) atomicOr(&misc->err, DERR_CAPACITY);
+        return;
+    }
+    for (uint32_t i = tid; i < count; i += TAIL_THREADS) {
+        Node32 nd = load_node(&cur[i]);
+        lx[i] = nd.minx;
+        ly[i] = nd.miny;
+        lz[i] = nd.minz;
+        hx[i] = nd.maxx;
+        hy[i] = nd.maxy;
+        hz[i] = nd.maxz;
+        pc[i] = nd.prim_count;
+        fi[i] = nd.first_index;
+    }
+    __syncthreads();
+    uint32_t insert = st.insert;
+    uint32_t depth = st.depth;
+    unsigned long long csum = st.cluster_sum;
+    uint32_t pass = first_pass;
+    bool self_merge = false;
+    while (count > 1) {
+        csum += count;
+        const bool dist1 = sd_is_one || pass < depth_threshold;
+        for (uint32_t i = tid; i < count; i += TAIL_THREADS) {
+            const Box me{lx[i], ly[i], lz[i], hx[i], hy[i], hz[i]};
+            int8_t m;
+            if (dist1) {
+                if (i == count - 1) {
+                    m = -1;
+                } else {
+                    const Box nx{lx[i + 1], ly[i + 1], lz[i + 1], hx[i + 1], hy[i + 1], hz[i + 1]};
+                    float cost = half_area(box_union(me, nx));
+                    float last = INFINITY;
+                    if (i > 0) {
+                        const Box pv{lx[i - 1], ly[i - 1], lz[i - 1], hx[i - 1], hy[i - 1], hz[i - 1]};
+                        last = half_area(box_union(pv, me));
+                    }
+                    m = last < cost ? (int8_t)-1 : (int8_t)1;
+                }
+            } else {
+                const uint32_t begin = i - min(r, i);
+                const uint32_t end = min(i + r + 1u, count);
+                uint32_t best = i;
+                float best_cost = INFINITY;
+                for (uint32_t j = begin; j < end; ++j) {
+                    if (j == i) continue;
+                    const Box o{lx[j], ly[j], lz[j], hx[j], hy[j], hz[j]};
+                    float c = half_area(box_union(me, o));
+                    if (c <= best_cost) {
+                        best = j;
+                        best_cost = c;
+                    }
+                }
+                if (best == i) self_merge = true;
+                m = (int8_t)((int)best - (int)i);
+            }
+            mg[i] = m;
+        }
+        __syncthreads();
+        // Blocked assignment for the ordered scan: thread t owns clusters [4t, 4t+4).
+        bool emit[TAIL_ITEMS], mrg[TAIL_ITEMS];
+        uint32_t bidx[TAIL_ITEMS];
+        uint32_t local = 0;
+#pragma unroll
+        for (int q = 0; q < TAIL_ITEMS; ++q) {
+            classify(mg, tid * TAIL_ITEMS + q, count, emit[q], mrg[q], bidx[q]);
+            local += (emit[q] ? 1u : 0u) | (mrg[q] ? (1u << 16) : 0u);
+        }
+        uint32_t incl = wave_inclusive_scan(local);
+        const uint32_t wave = tid >> 6, lane = tid & 63u;
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t run = 0;
+            for (int w = 0; w < TAIL_THREADS / 64; ++w) {
+                uint32_t t = wsum[w];
+                wsum[w] = run;
+                run += t;
+            }
+            s_total = run;
+        }
+        __syncthreads();
+        uint32_t excl = wsum[wave] + incl - local;
+        const uint32_t total = s_total;
+        // Read phase (everything this thread writes is loaded before anyone overwrites the LDS arrays).
+        Box va[TAIL_ITEMS], vb[TAIL_ITEMS];
+        uint32_t vpc[TAIL_ITEMS], vfi[TAIL_ITEMS], vbpc[TAIL_ITEMS], vbfi[TAIL_ITEMS];
+#pragma unroll
+        for (int q = 0; q < TAIL_ITEMS; ++q) {
+            const uint32_t i = tid * TAIL_ITEMS + q;
+            if (emit[q]) {
+                va[q] = Box{lx[i], ly[i], lz[i], hx[i], hy[i], hz[i]};
+                vpc[q] = pc[i];
+                vfi[q] = fi[i];
+                if (mrg[q]) {
+                    const uint32_t b = bidx[q];
+                    vb[q] = Box{lx[b], ly[b], lz[b], hx[b], hy[b], hz[b]};
+                    vbpc[q] = pc[b];
+                    vbfi[q] = fi[b];
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t pe = excl & 0xffffu, pm = excl >> 16;
+#pragma unroll
+        for (int q = 0; q < TAIL_ITEMS; ++q) {
+            if (!emit[q]) continue;
+            Box outb = va[q];
+            uint32_t opc = vpc[q], ofi = vfi[q];
+            if (mrg[q]) {
+                const uint32_t first = insert - 2u * (pm + 1u);
+                store_node(&bvh_nodes[first], make_node(va[q], vpc[q], vfi[q]));
+                store_node(&bvh_nodes[first + 1], make_node(vb[q], vbpc[q], vbfi[q]));
+                outb = box_union(va[q], vb[q]);
+                opc = 0u;
+                ofi = first;
+                pm++;
+            }
+            lx[pe] = outb.lx;
+            ly[pe] = outb.ly;
+            lz[pe] = outb.lz;
+            hx[pe] = outb.hx;
+            hy[pe] = outb.hy;
+            hz[pe] = outb.hz;
+            pc[pe] = opc;
+            fi[pe] = ofi;
+            pe++;
+        }
+        __syncthreads();
+        count = total & 0xffffu;
+        insert -= 2u * (total >> 16);
+        depth += 1u;
+        pass += 1u;
+        if ((total >> 16) == 0u) {
+            // No mutual pair in this pass: the reference would loop forever; report instead of hanging.
+            self_merge = true;
+            break;
+        }
+    }
+    if (self_merge) atomicOr(&misc->err, DERR_SELF_MERGE);
+    if (tid == 0) {
+        store_node(&bvh_nodes[0], make_node(Box{lx[0], ly[0], lz[0], hx[0], hy[0], hz[0]}, pc[0], fi[0]));
+        PlocState out{count, insert, depth, 0u, csum};
+        misc->state[slot] = out;
+    }
+}
+
+}  // namespace
+
+hipError_t DevBuf::ensure(size_t want) {
+    if (want <= bytes && ptr) return hipSuccess;
+    release();
+    size_t sz = want < 256 ? 256 : want;
+    hipError_t e = hipMalloc(&ptr, sz);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        return e;
+    }
+    bytes = sz;
+    return hipSuccess;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+#define OBVHS_TRY(expr, where)                                              \
+    do {                                                                    \
+        hipError_t _e = (expr);                                             \
+        if (_e != hipSuccess) return set_hip_error(ctx, _e, where);         \
+    } while (0)
+
+obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
+                        const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
+                        uint32_t search_depth_threshold, obvhs_bvh2* bvh) {
+    if (!(search_distance == 1 || search_distance == 2 || search_distance == 6 || search_distance == 14 ||
+          search_distance == 24 || search_distance == 32))
+        return OBVHS_ERR_INVALID_ARG;  // ploc/mod.rs:550-561
+    hipStream_t st = ctx->stream;
+    bvh->prim_count = n;
+    bvh->node_count = n == 0 ? 0u : 2u * n - 1u;
+    bvh->max_depth = 96;
+    bvh->ploc_iterations = 0;
+    bvh->cluster_sum = 0;
+    OBVHS_TRY(bvh->prim_indices.ensure((size_t)n * 4 + 4), "alloc primitive_indices");
+    if (n == 0) return OBVHS_OK;
+    OBVHS_TRY(bvh->nodes.ensure((size_t)bvh->node_count * sizeof(Node32)), "alloc bvh2 nodes");
+    BuildScratch& S = ctx->scratch;
+    OBVHS_TRY(S.leaves.ensure((size_t)n * sizeof(Node32)), "alloc leaves");
+    OBVHS_TRY(S.clusters_a.ensure((size_t)n * sizeof(Node32)), "alloc clusters");
+    OBVHS_TRY(S.clusters_b.ensure((size_t)n * sizeof(Node32)), "alloc clusters");
+    OBVHS_TRY(S.keys.ensure((size_t)n * 8), "alloc keys");
+    OBVHS_TRY(S.vals.ensure((size_t)n * 4), "alloc vals");
+    OBVHS_TRY(S.merge.ensure((size_t)n + 64), "alloc merge");
+    const uint32_t nblk = (n + T256 - 1) / T256;
+    OBVHS_TRY(S.blocksums.ensure((size_t)nblk * sizeof(uint2) + 64), "alloc blocksums");
+    OBVHS_TRY(S.sort_scratch.ensure(radix_sort_scratch_bytes(n)), "alloc sort scratch");
+    OBVHS_TRY(S.misc.ensure(sizeof(Misc)), "alloc misc");
+    Misc* misc = S.misc.as<Misc>();
+    Node32* leaves = S.leaves.as<Node32>();
+    Node32* bvh_nodes = bvh->nodes.as<Node32>();
+    uint64_t* keys = S.keys.as<uint64_t>();
+    uint32_t* vals = S.vals.as<uint32_t>();
+
+    // primitive_indices = indices (Bvh2::reset_for_reuse, bvh2/mod.rs:106-121)
+    if (d_indices) {
+        OBVHS_TRY(hipMemcpyAsync(bvh->prim_indices.ptr, d_indices, (size_t)n * 4, hipMemcpyDeviceToDevice, st),
+                  "copy indices");
+    }
+    k_init_misc<<<1, 64, 0, st>>>(misc, n);
+    if (d_tris) {
+        k_leaf_init_tris<<<nblk, T256, 0, st>>>(reinterpret_cast<const float4*>(d_tris), d_indices, n, leaves,
+                                                misc);
+    } else {
+        k_leaf_init_aabbs<<<nblk, T256, 0, st>>>(reinterpret_cast<const float4*>(d_aabbs), d_indices, n, leaves,
+                                                 misc);
+    }
+    k_morton<<<nblk, T256, 0, st>>>(leaves, n, misc, keys);
+    OBVHS_TRY(hipGetLastError(), "leaf/morton launch");
+    if (ctx->keep_trace) {
+        OBVHS_TRY(bvh->codes.ensure((size_t)n * 8), "alloc codes");
+        OBVHS_TRY(hipMemcpyAsync(bvh->codes.ptr, keys, (size_t)n * 8, hipMemcpyDeviceToDevice, st), "copy codes");
+    }
+    OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 63, S.sort_scratch.ptr, st), "radix sort");
+    if (ctx->keep_trace) {
+        OBVHS_TRY(bvh->sorted_index.ensure((size_t)n * 4), "alloc sorted index");
+        OBVHS_TRY(hipMemcpyAsync(bvh->sorted_index.ptr, vals, (size_t)n * 4, hipMemcpyDeviceToDevice, st),
+                  "copy sorted index");
+    }
+    Node32* cur = S.clusters_a.as<Node32>();
+    Node32* nxt = S.clusters_b.as<Node32>();
+    k_gather<<<nblk, T256, 0, st>>>(leaves, vals, n, cur);
+    OBVHS_TRY(hipGetLastError(), "gather launch");
+
+    const uint32_t r = search_distance;
+    int8_t* merge = S.merge.as<int8_t>();
+    uint2* blocksums = S.blocksums.as<uint2>();
+    uint32_t known = n;
+    uint32_t pass = 0;
+    uint32_t slot = 0;
+    PlocState* hstate = static_cast<PlocState*>(ctx->pinned);
+    while (known > TAIL_MAX) {
+        const uint32_t blocks = (known + T256 - 1) / T256;
+        for (int k = 0; k < PASSES_PER_SYNC; ++k) {
+            const uint32_t dist1 = (r == 1 || pass < search_depth_threshold) ? 1u : 0u;
+            k_ploc_search<<<blocks, T256, 0, st>>>(cur, misc, slot, r, dist1, merge);
+            k_ploc_count<<<blocks, T256, 0, st>>>(merge, misc, slot, blocksums);
+            k_ploc_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, slot);
+            k_ploc_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, bvh_nodes);
+            std::swap(cur, nxt);
+            slot ^= 1u;
+            pass++;
+        }
+        OBVHS_TRY(hipGetLastError(), "ploc pass launch");
+        uint32_t* herr_loop = reinterpret_cast<uint32_t*>(hstate + 1);
+        OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
+                  "read ploc state");
+        OBVHS_TRY(hipMemcpyAsync(herr_loop, &misc->err, 4, hipMemcpyDeviceToHost, st), "read err");
+        OBVHS_TRY(hipStreamSynchronize(st), "ploc sync");
+        if (*herr_loop) return device_error_to_status(*herr_loop);
+        // Every pass merges at least one pair unless costs are NaN; a batch without progress would never end.
+        if (hstate->count >= known) return OBVHS_ERR_PLOC_SELF_MERGE;
+        known = hstate->count;
+        // The device depth counts real passes only; later enqueued passes may have been no-ops.
+        pass = hstate->depth;
+    }
+    k_ploc_tail<<<1, TAIL_THREADS, 0, st>>>(cur, misc, slot, pass, r, r == 1 ? 1u : 0u, search_depth_threshold,
+                                            bvh_nodes);
+    OBVHS_TRY(hipGetLastError(), "ploc tail launch");
+    uint32_t* herr = reinterpret_cast<uint32_t*>(hstate + 1);
+    OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
+              "read ploc state");
+    OBVHS_TRY(hipMemcpyAsync(herr, &misc->err, 4, hipMemcpyDeviceToHost, st), "read err");
+    if (!d_indices) OBVHS_TRY(launch_iota(bvh->prim_indices.as<uint32_t>(), n, st), "iota");
+    OBVHS_TRY(hipStreamSynchronize(st), "ploc final sync");
+    if (*herr) return device_error_to_status(*herr);
+    if (hstate->insert != 1u) return OBVHS_ERR_PLOC_SELF_MERGE;  // ploc/mod.rs:496
+    bvh->ploc_iterations = hstate->depth;
+    bvh->cluster_sum = hstate->cluster_sum;
+    bvh->max_depth = hstate->depth + 1 > 96 ? hstate->depth + 1 : 96;  // ploc/mod.rs:501
+    return OBVHS_OK;
+}
+
+namespace {
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+}  // namespace
+
+hipError_t launch_iota(uint32_t* p, uint32_t n, hipStream_t st) {
+    k_iota<<<(n + 255) / 256, 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+}  // namespace obvhs

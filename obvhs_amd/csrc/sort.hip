@@ -1,0 +1,230 @@
+// Stable LSD radix sort of (u64 Morton code, u32 leaf index) pairs for gfx950.
+//
+// Replaces the Morton sort of ploc/mod.rs:811-827.  The reference uses unstable sorts whose tie order is not
+// reproducible; this sort is stable, so ties between equal codes keep ascending leaf index (the project rule the
+// oracle follows too).
+//
+// Per 8-bit digit pass (reduce-then-scan, three launches):
+//   upsweep   : per-tile digit histogram in LDS -> counts[digit][tile], plus the pass's digit totals;
+//   scan      : one workgroup per digit, exclusive scan of counts[digit][*] offset by the digits below it;
+//   downsweep : wave-level multisplit ranking (eight 64-lane ballots per item) gives each key its stable rank
+//               inside the tile, the tile is reordered in LDS and written out coalesced per digit run.
+// Tiles are 2048 keys (256 threads x 8 items); each wave owns a contiguous 512-key segment.
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace obvhs {
+
+namespace {
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 8;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+constexpr int SORT_WAVES = SORT_THREADS / 64;
+
+__global__ void __launch_bounds__(SORT_THREADS) k_zero_u32(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void __launch_bounds__(SORT_THREADS)
+    k_radix_upsweep(const uint64_t* __restrict__ keys, uint32_t n, uint32_t shift, uint32_t num_tiles,
+                    uint32_t* __restrict__ counts, uint32_t* __restrict__ totals) {
+    __shared__ uint32_t hist[256];
+    const uint32_t tid = threadIdx.x;
+    hist[tid] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * SORT_TILE;
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+        uint32_t idx = base + j * SORT_THREADS + tid;
+        if (idx < n) {
+            uint32_t d = (uint32_t)(keys[idx] >> shift) & 0xffu;
+            atomicAdd(&hist[d], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t c = hist[tid];
+    counts[tid * num_tiles + blockIdx.x] = c;
+    if (c) atomicAdd(&totals[tid], c);
+}
+
+// Exclusive block scan of one u32 per thread (256 threads).
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* tmp /*256*/, uint32_t& total) {
+    const uint32_t tid = threadIdx.x;
+    tmp[tid] = v;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        uint32_t add = tid >= off ? tmp[tid - off] : 0u;
+        __syncthreads();
+        tmp[tid] += add;
+        __syncthreads();
+    }
+    uint32_t incl = tmp[tid];
+    total = tmp[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(SORT_THREADS)
+    k_radix_scan(uint32_t* __restrict__ counts, const uint32_t* __restrict__ totals, uint32_t num_tiles) {
+    __shared__ uint32_t tmp[256];
+    const uint32_t d = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    // digit base = sum of totals of lower digits
+    uint32_t t = tid < d ? totals[tid] : 0u;
+    uint32_t dummy;
+    uint32_t tot_lower;
+    (void)block_exclusive_scan_256(t, tmp, tot_lower);
+    uint32_t carry = tot_lower;
+    uint32_t* row = counts + (size_t)d * num_tiles;
+    for (uint32_t start = 0; start < num_tiles; start += 256) {
+        uint32_t i = start + tid;
+        uint32_t v = i < num_tiles ? row[i] : 0u;
+        uint32_t chunk_total;
+        uint32_t ex = block_exclusive_scan_256(v, tmp, chunk_total);
+        if (i < num_tiles) row[i] = carry + ex;
+        carry += chunk_total;
+    }
+    (void)dummy;
+}
+
+__global__ void __launch_bounds__(SORT_THREADS)
+    k_radix_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                      uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t shift,
+                      const uint32_t* __restrict__ offsets, uint32_t num_tiles) {
+    __shared__ uint32_t whist[SORT_WAVES][256];
+    __shared__ uint32_t digit_start[256];
+    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t scan_tmp[256];
+    __shared__ uint64_t skeys[SORT_TILE];
+    __shared__ uint32_t svals[SORT_TILE];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
+    const uint32_t lane = tid & 63u;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tile_base = tile * SORT_TILE;
+    const uint32_t tile_count = min((uint32_t)SORT_TILE, n - tile_base);
+
+#pragma unroll
+    for (int w = 0; w < SORT_WAVES; ++w) whist[w][tid] = 0u;
+    __syncthreads();
+
+    const uint32_t seg = tile_base + wave * (64 * SORT_ITEMS);
+    uint64_t k[SORT_ITEMS];
+    uint32_t v[SORT_ITEMS];
+    uint32_t rank[SORT_ITEMS];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+        uint32_t idx = seg + j * 64 + lane;
+        bool valid = idx < n;
+        k[j] = valid ? kin[idx] : 0ull;
+        v[j] = valid ? (vin ? vin[idx] : idx) : 0u;
+        uint32_t d = (uint32_t)(k[j] >> shift) & 0xffu;
+        uint64_t peers = ballot64(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bm = ballot64(bit);
+            peers &= bit ? bm : ~bm;
+        }
+        uint32_t below = (uint32_t)__popcll(peers & lt);
+        uint32_t base = 0;
+        if (valid) base = whist[wave][d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) whist[wave][d] = base + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rank[j] = base + below;
+    }
+    __syncthreads();
+    // Exclusive prefix across waves for each digit, then across digits for the tile.
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < SORT_WAVES; ++w) {
+        uint32_t c = whist[w][tid];
+        whist[w][tid] = run;
+        run += c;
+    }
+    uint32_t tile_total;
+    uint32_t start = block_exclusive_scan_256(run, scan_tmp, tile_total);
+    digit_start[tid] = start;
+    gbase[tid] = offsets[(size_t)tid * num_tiles + tile];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+        uint32_t idx = seg + j * 64 + lane;
+        if (idx < n) {
+            uint32_t d = (uint32_t)(k[j] >> shift) & 0xffu;
+            uint32_t pos = digit_start[d] + whist[wave][d] + rank[j];
+            skeys[pos] = k[j];
+            svals[pos] = v[j];
+        }
+    }
+    __syncthreads();
+    for (uint32_t s = tid; s < tile_count; s += SORT_THREADS) {
+        uint64_t key = skeys[s];
+        uint32_t d = (uint32_t)(key >> shift) & 0xffu;
+        uint32_t dst = gbase[d] + (s - digit_start[d]);
+        kout[dst] = key;
+        vout[dst] = svals[s];
+    }
+}
+
+}  // namespace
+
+size_t radix_sort_scratch_bytes(uint32_t n) {
+    uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    size_t b = 0;
+    b += (size_t)n * 8;                     // alternate keys
+    b += (size_t)n * 4;                     // alternate values
+    b += (size_t)256 * tiles * 4 + 256;     // counts
+    b += 8 * 256 * 4 + 256;                 // per-pass totals
+    return b + 1024;
+}
+
+// Sorts keys[0..n) ascending (stable) with values.  On entry values may be null (identity 0..n).  `bits` is the
+// number of low key bits that can be non-zero (63 for Morton64).  Result lands in keys/values (even pass count)
+// or is copied back.  scratch must hold radix_sort_scratch_bytes(n).
+hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identity, uint32_t n, int bits,
+                            void* scratch, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    char* p = static_cast<char*>(scratch);
+    auto take = [&](size_t bytes) {
+        char* r = p;
+        p += (bytes + 255) & ~size_t(255);
+        return r;
+    };
+    uint64_t* kalt = reinterpret_cast<uint64_t*>(take((size_t)n * 8));
+    uint32_t* valt = reinterpret_cast<uint32_t*>(take((size_t)n * 4));
+    uint32_t* counts = reinterpret_cast<uint32_t*>(take((size_t)256 * tiles * 4));
+    uint32_t* totals = reinterpret_cast<uint32_t*>(take(8 * 256 * 4));
+    int passes = (bits + 7) / 8;
+    k_zero_u32<<<(8 * 256 + 255) / 256, 256, 0, stream>>>(totals, 8 * 256);
+    uint64_t* kin = keys;
+    uint32_t* vin = values;
+    uint64_t* kout = kalt;
+    uint32_t* vout = valt;
+    for (int pass = 0; pass < passes; ++pass) {
+        uint32_t shift = (uint32_t)(pass * 8);
+        k_radix_upsweep<<<tiles, SORT_THREADS, 0, stream>>>(kin, n, shift, tiles, counts, totals + pass * 256);
+        k_radix_scan<<<256, SORT_THREADS, 0, stream>>>(counts, totals + pass * 256, tiles);
+        const uint32_t* vsrc = (pass == 0 && values_identity) ? nullptr : vin;
+        k_radix_downsweep<<<tiles, SORT_THREADS, 0, stream>>>(kin, vsrc, kout, vout, n, shift, counts, tiles);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (kin != keys) {
+        e = hipMemcpyAsync(keys, kin, (size_t)n * 8, hipMemcpyDeviceToDevice, stream);
+        if (e != hipSuccess) return e;
+        e = hipMemcpyAsync(values, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, stream);
+    }
+    return e;
+}
+
+}  // namespace obvhs

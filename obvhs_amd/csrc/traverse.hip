@@ -1,0 +1,230 @@
+// CWBVH closest-hit and occlusion traversal on gfx950.
+//
+// One lane per ray runs the reference's traversal state machine exactly (traverse!, cwbvh/traverse_macro.rs:59-126;
+// CwBvh::ray_traverse cwbvh/mod.rs:169-198 and ray_traverse_miss 201-225), so node visit order, primitive test
+// order and the resulting hits match the CPU path bit for bit:
+//   * node groups and primitive groups are consumed highest bit first (firstbithigh, cwbvh/mod.rs:996-998);
+//   * the 32-entry group stack keeps StackStack semantics (faststack.rs:299-310): a push past slot 31 overwrites
+//     slot 31, a pop of the empty stack ends traversal;
+//   * the node test is the SSE path of cwbvh/simd.rs:17-100 (mul then add, SSE min/max, tmin clamped to 1e-4,
+//     inclusive compare) and the primitive test is Triangle::intersect (triangle.rs:35-76).
+// The stack lives in LDS, one 64-lane column per wave (entry-major so a wave's pushes hit distinct banks);
+// nodes are fetched as five 16-byte loads, triangles (in primitive_indices order) as three.
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace obvhs {
+
+namespace {
+
+constexpr int TRAV_THREADS = 64;
+constexpr int STACK_SIZE = 32;  // TRAVERSAL_STACK_SIZE, cwbvh/mod.rs:60
+constexpr float NODE_EPSILON = 0.0001f;  // cwbvh/node.rs:82
+
+struct RayData {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float ix, iy, iz;
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ uint32_t firstbithigh(uint32_t v) { return 31u - (uint32_t)__clz((int)v); }
+
+// cwbvh/mod.rs:1001-1010
+__device__ __forceinline__ uint32_t octant_inv4(float dx, float dy, float dz) {
+    return (dx < 0.0f ? 0u : 0x04040404u) | (dy < 0.0f ? 0u : 0x02020202u) | (dz < 0.0f ? 0u : 0x01010101u);
+}
+
+__device__ __forceinline__ float byte_f(uint32_t w, int b) { return (float)((w >> (8 * b)) & 0xffu); }
+
+// CwBvhNode::intersect_ray_simd (cwbvh/simd.rs:17-100) with get_child_and_index_bits (cwbvh/node.rs:207-231).
+// Node words: q0 = {p.x, p.y, p.z, e|imask}, q1 = {child_base, prim_base, meta lo, meta hi},
+// q2 = {min_x lo/hi, max_x lo/hi}, q3 = {min_y, max_y}, q4 = {min_z, max_z}.
+__device__ __forceinline__ uint32_t intersect_node(const uint4& q0, const uint4& q1, const uint4& q2,
+                                                   const uint4& q3, const uint4& q4, const RayData& r,
+                                                   float ray_tmax, uint32_t oct_inv4) {
+    const float ex = __uint_as_float((q0.w & 0xffu) << 23);
+    const float ey = __uint_as_float(((q0.w >> 8) & 0xffu) << 23);
+    const float ez = __uint_as_float(((q0.w >> 16) & 0xffu) << 23);
+    const float adx = ex * r.ix, ady = ey * r.iy, adz = ez * r.iz;
+    const float aox = (__uint_as_float(q0.x) - r.ox) * r.ix;
+    const float aoy = (__uint_as_float(q0.y) - r.oy) * r.iy;
+    const float aoz = (__uint_as_float(q0.z) - r.oz) * r.iz;
+    const bool rdx = r.dx < 0.0f, rdy = r.dy < 0.0f, rdz = r.dz < 0.0f;
+
+    uint64_t oct_inv8 = (uint64_t)oct_inv4;
+    oct_inv8 |= oct_inv8 << 32;
+    const uint64_t meta8 = (uint64_t)q1.z | ((uint64_t)q1.w << 32);
+    const uint64_t is_inner8 = (meta8 & (meta8 << 1)) & 0x1010101010101010ull;
+    const uint64_t inner_mask8 = (is_inner8 >> 4) * 0xffull;
+    const uint64_t bit_index8 = (meta8 ^ (oct_inv8 & inner_mask8)) & 0x1f1f1f1f1f1f1f1full;
+    const uint64_t child_bits8 = (meta8 >> 5) & 0x0707070707070707ull;
+
+    uint32_t hit_mask = 0u;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int h = c >> 2, b = c & 3;
+        const uint32_t wminx = h ? q2.y : q2.x, wmaxx = h ? q2.w : q2.z;
+        const uint32_t wminy = h ? q3.y : q3.x, wmaxy = h ? q3.w : q3.z;
+        const uint32_t wminz = h ? q4.y : q4.x, wmaxz = h ? q4.w : q4.z;
+        const float qlx = byte_f(wminx, b), qhx = byte_f(wmaxx, b);
+        const float qly = byte_f(wminy, b), qhy = byte_f(wmaxy, b);
+        const float qlz = byte_f(wminz, b), qhz = byte_f(wmaxz, b);
+        const float xmin = rdx ? qhx : qlx, xmax = rdx ? qlx : qhx;
+        const float ymin = rdy ? qhy : qly, ymax = rdy ? qly : qhy;
+        const float zmin = rdz ? qhz : qlz, zmax = rdz ? qlz : qhz;
+        const float tminx = xmin * adx + aox, tmaxx = xmax * adx + aox;
+        const float tminy = ymin * ady + aoy, tmaxy = ymax * ady + aoy;
+        const float tminz = zmin * adz + aoz, tmaxz = zmax * adz + aoz;
+        float tmin = smax(tminx, smax(tminy, tminz));
+        float tmax = smin(tmaxx, smin(tmaxy, tmaxz));
+        tmin = smax(tmin, NODE_EPSILON);
+        tmax = smin(tmax, ray_tmax);
+        if (tmin <= tmax) {
+            const uint32_t cb = (uint32_t)(child_bits8 >> (c * 8)) & 0xffu;
+            const uint32_t bi = (uint32_t)(bit_index8 >> (c * 8)) & 0xffu;
+            hit_mask |= cb << bi;
+        }
+    }
+    return hit_mask;
+}
+
+// Triangle::intersect (triangle.rs:35-76); glam dot = (x*x'+y*y')+z*z', cross unfused.
+__device__ __forceinline__ float tri_intersect(const float4& v0, const float4& v1, const float4& v2,
+                                               const RayData& r, float tmin, float tmax) {
+    const float e1x = v0.x - v1.x, e1y = v0.y - v1.y, e1z = v0.z - v1.z;
+    const float e2x = v2.x - v0.x, e2y = v2.y - v0.y, e2z = v2.z - v0.z;
+    const float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const float cx = v0.x - r.ox, cy = v0.y - r.oy, cz = v0.z - r.oz;
+    const float rx = r.dy * cz - r.dz * cy, ry = r.dz * cx - r.dx * cz, rz = r.dx * cy - r.dy * cx;
+    const float inv_det = 1.0f / ((nx * r.dx + ny * r.dy) + nz * r.dz);
+    const float u = ((rx * e2x + ry * e2y) + rz * e2z) * inv_det;
+    const float v = ((rx * e1x + ry * e1y) + rz * e1z) * inv_det;
+    const float w = 1.0f - u - v;
+    const uint32_t hit = __float_as_uint(u) | __float_as_uint(v) | __float_as_uint(w);
+    if (inv_det != 0.0f && (hit & 0x80000000u) == 0u) {
+        const float t = ((nx * cx + ny * cy) + nz * cz) * inv_det;
+        if (t >= tmin && t <= tmax) return t;
+    }
+    return INFINITY;
+}
+
+template <int MODE, bool COUNT>
+__global__ void __launch_bounds__(TRAV_THREADS)
+    k_traverse(const uint4* __restrict__ nodes, uint32_t node_count, uint32_t prim_count,
+               const float4* __restrict__ tris, const float4* __restrict__ rays, uint32_t n, obvhs_ray_hit* __restrict__ hits,
+               uint8_t* __restrict__ miss_out, uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests) {
+    __shared__ uint2 stack[STACK_SIZE][TRAV_THREADS];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = blockIdx.x * TRAV_THREADS + lane;
+    if (i >= n) return;
+    const float4 ra = rays[4 * (size_t)i], rb = rays[4 * (size_t)i + 1], rc = rays[4 * (size_t)i + 2],
+                 rd = rays[4 * (size_t)i + 3];
+    const RayData r{ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, rc.x, rc.y, rc.z, rd.x, rd.y};
+    const uint32_t oct = octant_inv4(r.dx, r.dy, r.dz);
+    float tmax = r.tmax;
+    uint32_t hit_id = INVALID32;
+    float hit_t = INFINITY;
+    bool is_miss = true;
+    uint32_t nv = 0, tv = 0;
+    int sp = 0;
+    uint2 cur = node_count == 0u ? make_uint2(0u, 0u) : make_uint2(0u, 0x80000000u);
+    uint2 prim = make_uint2(0u, 0u);
+    for (;;) {
+        bool stop = false;
+        while (prim.y != 0u) {
+            const uint32_t local = firstbithigh(prim.y);
+            prim.y &= ~(1u << local);
+            const uint32_t pid = prim.x + local;
+            if (pid >= prim_count) continue;  // malformed input only: never index past the triangle buffer
+            const float4 v0 = tris[3 * (size_t)pid], v1 = tris[3 * (size_t)pid + 1], v2 = tris[3 * (size_t)pid + 2];
+            if (COUNT) tv++;
+            if (MODE == 0) {
+                const float t = tri_intersect(v0, v1, v2, r, r.tmin, tmax);
+                if (t < tmax) {
+                    hit_id = pid;
+                    hit_t = t;
+                    tmax = t;
+                }
+            } else {
+                const float t = tri_intersect(v0, v1, v2, r, r.tmin, r.tmax);
+                if (t < r.tmax) {
+                    is_miss = false;
+                    stop = true;
+                    break;
+                }
+            }
+        }
+        if (stop) break;
+        prim = make_uint2(0u, 0u);
+        if ((cur.y & 0xff000000u) != 0u) {
+            const uint32_t hits_imask = cur.y;
+            const uint32_t off = firstbithigh(hits_imask);
+            const uint32_t base = cur.x;
+            cur.y &= ~(1u << off);
+            if ((cur.y & 0xff000000u) != 0u) {
+                stack[sp][lane] = cur;
+                sp = min(sp + 1, STACK_SIZE - 1);
+            }
+            const uint32_t slot_index = (off - 24u) ^ (oct & 0xffu);
+            const uint32_t rel = (uint32_t)__popc(hits_imask & ~(0xffffffffu << slot_index));
+            const uint32_t child = base + rel;
+            if (child >= node_count) break;  // malformed input only: never index past the node buffer
+            const uint4* np = nodes + (size_t)child * 5;
+            const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3], q4 = np[4];
+            if (COUNT) nv++;
+            const uint32_t hm = intersect_node(q0, q1, q2, q3, q4, r, tmax, oct);
+            const uint32_t imask = (q0.w >> 24) & 0xffu;
+            cur = make_uint2(q1.x, (hm & 0xff000000u) | imask);
+            prim = make_uint2(q1.y, hm & 0x00ffffffu);
+        } else {
+            cur = make_uint2(0u, 0u);
+        }
+        if (prim.y == 0u && (cur.y & 0xff000000u) == 0u) {
+            if (sp == 0) break;
+            sp -= 1;
+            cur = stack[sp][lane];
+        }
+    }
+    if (MODE == 0) {
+        obvhs_ray_hit h;
+        h.primitive_id = hit_id;
+        h.geometry_id = INVALID32;
+        h.instance_id = INVALID32;
+        h.t = hit_t;
+        reinterpret_cast<uint4*>(hits)[i] = make_uint4(h.primitive_id, h.geometry_id, h.instance_id,
+                                                       __float_as_uint(h.t));
+    } else {
+        miss_out[i] = is_miss ? 1 : 0;
+    }
+    if (COUNT) {
+        node_visits[i] = nv;
+        tri_tests[i] = tv;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
+                           const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,
+                           uint32_t* node_visits, uint32_t* tri_tests, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = (n + TRAV_THREADS - 1) / TRAV_THREADS;
+    const uint4* nd = reinterpret_cast<const uint4*>(nodes);
+    const float4* tr = reinterpret_cast<const float4*>(tris);
+    const float4* ry = reinterpret_cast<const float4*>(rays);
+    if (mode == 0) {
+        if (node_visits)
+            k_traverse<0, true><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss,
+                                                                     node_visits, tri_tests);
+        else
+            k_traverse<0, false><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss,
+                                                                      nullptr, nullptr);
+    } else {
+        k_traverse<1, false><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss, nullptr,
+                                                                  nullptr);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace obvhs

@@ -1,0 +1,112 @@
+"""ctypes binding of libobvhs_hip.so (the C ABI declared in include/obvhs_hip.h).
+
+The library is loaded from the package directory only; when it is missing the import of this module fails loudly --
+there is no CPU fallback anywhere in the product path."""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libobvhs_hip.so")
+
+STATUS = {
+    0: "OBVHS_OK", 1: "OBVHS_ERR_INVALID_ARG", 2: "OBVHS_ERR_NAN_INPUT", 3: "OBVHS_ERR_PLOC_SELF_MERGE",
+    4: "OBVHS_ERR_CW_DISTRIBUTE", 5: "OBVHS_ERR_CW_NONUNIT_LEAF", 6: "OBVHS_ERR_CW_ORDER", 7: "OBVHS_ERR_CW_PRIMCOUNT",
+    8: "OBVHS_ERR_CAPACITY", 9: "OBVHS_ERR_DEPTH", 10: "OBVHS_ERR_OOM", 11: "OBVHS_ERR_DEVICE",
+    12: "OBVHS_ERR_NO_DEVICE",
+}
+STATUS_CODE = {v: k for k, v in STATUS.items()}
+
+
+class ObvhsError(RuntimeError):
+    def __init__(self, status, message=""):
+        self.status = status
+        self.name = STATUS.get(status, f"status {status}")
+        super().__init__(f"{self.name}: {message}" if message else self.name)
+
+
+class BuildParams(C.Structure):
+    """obvhs_build_params -- BvhBuildParams (lib.rs:207-230)."""
+    _fields_ = [("pre_split", C.c_uint32), ("ploc_search_distance", C.c_uint32),
+                ("search_depth_threshold", C.c_uint32), ("reinsertion_batch_ratio", C.c_float),
+                ("post_collapse_reinsertion_batch_ratio_multiplier", C.c_float), ("sort_precision", C.c_uint32),
+                ("max_prims_per_leaf", C.c_uint32), ("collapse_traversal_cost", C.c_float)]
+
+
+class BuildStats(C.Structure):
+    _fields_ = [("prim_count", C.c_uint32), ("ploc_iterations", C.c_uint32), ("bvh2_node_count", C.c_uint32),
+                ("cwbvh_node_count", C.c_uint32), ("ploc_cluster_sum", C.c_uint64)]
+
+
+class Aabb(C.Structure):
+    _fields_ = [("min", C.c_float * 4), ("max", C.c_float * 4)]
+
+
+P = C.c_void_p
+U32 = C.c_uint32
+U64 = C.c_uint64
+PU32 = C.POINTER(C.c_uint32)
+INT = C.c_int
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "obvhs_status_string": (C.c_char_p, [INT]),
+    "obvhs_context_create": (INT, [INT, P, C.POINTER(P)]),
+    "obvhs_context_destroy": (INT, [P]),
+    "obvhs_context_synchronize": (INT, [P]),
+    "obvhs_context_set_trace": (INT, [P, INT]),
+    "obvhs_last_error_message": (C.c_char_p, [P]),
+    "obvhs_device_malloc": (INT, [P, U64, C.POINTER(P)]),
+    "obvhs_device_free": (INT, [P, P]),
+    "obvhs_memcpy_htod": (INT, [P, P, P, U64]),
+    "obvhs_memcpy_dtoh": (INT, [P, P, P, U64]),
+    "obvhs_memset_device": (INT, [P, P, INT, U64]),
+    "obvhs_timer_start": (INT, [P]),
+    "obvhs_timer_stop": (INT, [P, C.POINTER(C.c_float)]),
+    "obvhs_ploc_build_aabbs": (INT, [P, P, P, U32, U32, U32, U32, P, C.POINTER(P)]),
+    "obvhs_ploc_build_triangles": (INT, [P, P, P, U32, U32, U32, U32, P, C.POINTER(P)]),
+    "obvhs_bvh2_info": (INT, [P, PU32, PU32, PU32]),
+    "obvhs_bvh2_copy_nodes": (INT, [P, P, P]),
+    "obvhs_bvh2_copy_nodes_small": (INT, [P, P, P]),
+    "obvhs_bvh2_copy_primitive_indices": (INT, [P, P, P]),
+    "obvhs_bvh2_copy_sort_trace": (INT, [P, P, P, P]),
+    "obvhs_bvh2_destroy": (INT, [P]),
+    "obvhs_bvh2_to_cwbvh": (INT, [P, P, U32, INT, INT, P, C.POINTER(P)]),
+    "obvhs_build_cwbvh_from_tris": (INT, [P, P, U32, C.POINTER(BuildParams), P, C.POINTER(P),
+                                          C.POINTER(BuildStats)]),
+    "obvhs_cwbvh_info": (INT, [P, PU32, PU32, C.POINTER(Aabb)]),
+    "obvhs_cwbvh_copy_nodes": (INT, [P, P, P]),
+    "obvhs_cwbvh_copy_primitive_indices": (INT, [P, P, P]),
+    "obvhs_cwbvh_device_arrays": (INT, [P, C.POINTER(P), C.POINTER(P)]),
+    "obvhs_cwbvh_reorder_triangles": (INT, [P, P, P, P]),
+    "obvhs_cwbvh_destroy": (INT, [P]),
+    "obvhs_cwbvh_ray_traverse": (INT, [P, P, P, P, U32, P]),
+    "obvhs_cwbvh_ray_traverse_miss": (INT, [P, P, P, P, U32, P]),
+    "obvhs_cwbvh_ray_traverse_counted": (INT, [P, P, P, P, U32, P, P, P]),
+    "obvhs_log2_threshold_table": (INT, [P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads the library (once).  Raises ImportError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m obvhs_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, ctx=None):
+    if status != 0:
+        msg = ""
+        if ctx is not None:
+            m = lib().obvhs_last_error_message(ctx)
+            msg = m.decode() if m else ""
+        raise ObvhsError(status, msg)

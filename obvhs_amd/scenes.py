@@ -1,0 +1,152 @@
+"""Seeded synthetic workloads (triangle scenes and ray batches) for the benchmark and the parity tests.
+
+All randomness comes from the crate's integer hash (test_util.rs:9-46) so inputs are identical on every machine.
+
+* soup(n)            -- n small triangles: centroids U[0,1]^3, vertex offsets U[-0.005, 0.005]^3.
+* pinhole_rays(...)  -- primary rays of a pinhole camera (pixel centres, vertical field of view).
+* heightfield_scene  -- a g x g grid over [0,1]^2 (two triangles per cell), height = sum of 8 seeded sines with
+                        frequencies in [1,32], phases in [0,2pi), amplitudes proportional to 1/frequency, plus seeded
+                        12-triangle boxes resting on the surface.
+* bounce_rays        -- one cosine-weighted diffuse bounce per primary hit (geometric normal, seeded).
+"""
+import numpy as np
+
+from .test_util import CUBE, build_orthonormal_basis, cosine_sample_hemisphere, hash_noise
+from .types import RAY, make_rays, triangles_from_vertices
+
+
+def soup(n, seed=0):
+    """Triangle soup of n triangles (65,536 for the CPU-sized plain workload)."""
+    i = np.arange(n, dtype=np.uint32)
+    c = np.stack([hash_noise(i, a, seed) for a in range(3)], axis=1)
+    verts = []
+    for k in range(3):
+        off = np.stack([hash_noise(i, 3 + 3 * k + a, seed) for a in range(3)], axis=1)
+        verts.append(c + (off - np.float32(0.5)) * np.float32(0.01))
+    return triangles_from_vertices(*verts)
+
+
+def look_at_basis(eye, target, up=(0.0, 1.0, 0.0)):
+    eye = np.asarray(eye, dtype=np.float64)
+    fwd = np.asarray(target, dtype=np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.asarray(up, dtype=np.float64))
+    if np.linalg.norm(right) < 1e-12:
+        right = np.cross(fwd, np.array([1.0, 0.0, 0.0]))
+    right /= np.linalg.norm(right)
+    upv = np.cross(right, fwd)
+    return fwd, right, upv
+
+
+def pinhole_rays(width, height, eye, forward=(0.0, 0.0, 1.0), up=(0.0, 1.0, 0.0), fov_deg=60.0, tmax=1e30,
+                 target=None):
+    """Rays through pixel centres; vertical field of view fov_deg.  Directions are normalised in f64 and stored
+    in f32 together with Ray::new's safe inverse (ray.rs:34-52)."""
+    eye = np.asarray(eye, dtype=np.float64)
+    if target is None:
+        target = eye + np.asarray(forward, dtype=np.float64)
+    fwd, right, upv = look_at_basis(eye, target, up)
+    tan_half = np.tan(np.radians(fov_deg) * 0.5)
+    aspect = width / height
+    ys, xs = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
+    sx = ((xs + 0.5) / width * 2.0 - 1.0) * tan_half * aspect
+    sy = (1.0 - (ys + 0.5) / height * 2.0) * tan_half
+    d = fwd[None, None, :] + sx[..., None] * right[None, None, :] + sy[..., None] * upv[None, None, :]
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    d = d.reshape(-1, 3).astype(np.float32)
+    o = np.broadcast_to(eye.astype(np.float32), d.shape)
+    return make_rays(o, d, 0.0, tmax)
+
+
+def plain_workload(n=65536, seed=0, width=256, height=256):
+    """65,536-triangle soup traced by 256x256 pinhole rays from (0.5,0.5,-1.5) looking +z, 60 deg, tmax 1e30."""
+    tris = soup(n, seed)
+    rays = pinhole_rays(width, height, eye=(0.5, 0.5, -1.5), forward=(0.0, 0.0, 1.0), fov_deg=60.0, tmax=1e30)
+    return tris, rays
+
+
+def _sines(seed, count=8):
+    k = np.arange(count, dtype=np.uint32)
+    f = 1.0 + 31.0 * hash_noise(k, 100, seed).astype(np.float64)
+    phase = 2.0 * np.pi * hash_noise(k, 101, seed).astype(np.float64)
+    ang = 2.0 * np.pi * hash_noise(k, 102, seed).astype(np.float64)
+    amp = 0.05 / f
+    return f, phase, ang, amp
+
+
+def _height(x, z, sines):
+    f, phase, ang, amp = sines
+    x = np.asarray(x, dtype=np.float64)[..., None]
+    z = np.asarray(z, dtype=np.float64)[..., None]
+    return np.sum(amp * np.sin(2.0 * np.pi * f * (x * np.cos(ang) + z * np.sin(ang)) + phase), axis=-1)
+
+
+def heightfield_scene(grid=724, boxes=20000, seed=0):
+    """Heightfield of grid x grid cells (2 triangles each) over [0,1]^2 in x/z with y up, plus seeded boxes."""
+    sines = _sines(seed)
+    g = np.arange(grid + 1, dtype=np.float64) / grid
+    X, Z = np.meshgrid(g, g)  # [z, x]
+    H = _height(X, Z, sines).astype(np.float32)
+    Xf, Zf = X.astype(np.float32), Z.astype(np.float32)
+    v = lambda zi, xi: np.stack([Xf[zi, xi], H[zi, xi], Zf[zi, xi]], axis=-1)
+    zi, xi = np.meshgrid(np.arange(grid), np.arange(grid), indexing="ij")
+    zi, xi = zi.ravel(), xi.ravel()
+    v00, v10, v01, v11 = v(zi, xi), v(zi, xi + 1), v(zi + 1, xi), v(zi + 1, xi + 1)
+    t_a = np.stack([v00, v10], axis=1).reshape(-1, 3)
+    t_b = np.stack([v01, v01], axis=1).reshape(-1, 3)
+    t_c = np.stack([v10, v11], axis=1).reshape(-1, 3)
+    terrain = triangles_from_vertices(t_a, t_b, t_c)
+    if boxes == 0:
+        return terrain
+    b = np.arange(boxes, dtype=np.uint32)
+    cx = hash_noise(b, 200, seed).astype(np.float64)
+    cz = hash_noise(b, 201, seed).astype(np.float64)
+    s = 0.001 + 0.003 * hash_noise(b, 202, seed).astype(np.float64)
+    cy = _height(cx, cz, sines) + s
+    center = np.stack([cx, cy, cz], axis=1)
+    cube = [CUBE["v0"][:, :3].astype(np.float64), CUBE["v1"][:, :3].astype(np.float64),
+            CUBE["v2"][:, :3].astype(np.float64)]
+    bv = [(center[:, None, :] + s[:, None, None] * c[None, :, :]).reshape(-1, 3).astype(np.float32) for c in cube]
+    box_tris = triangles_from_vertices(*bv)
+    return np.concatenate([terrain, box_tris])
+
+
+def heightfield_workload(grid=724, boxes=20000, width=1920, height=1080, seed=0):
+    """Heightfield scene with 1920x1080 primary rays from above the terrain."""
+    tris = heightfield_scene(grid, boxes, seed)
+    rays = pinhole_rays(width, height, eye=(0.5, 0.45, -0.35), target=(0.5, 0.0, 0.55), fov_deg=60.0, tmax=1e30)
+    return tris, rays
+
+
+def triangle_normals(tris):
+    v0 = tris["v0"][:, :3].astype(np.float64)
+    v1 = tris["v1"][:, :3].astype(np.float64)
+    v2 = tris["v2"][:, :3].astype(np.float64)
+    n = np.cross(v1 - v0, v2 - v0)
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    return n / np.where(ln > 0, ln, 1.0)
+
+
+def bounce_rays(rays, hits, bvh_tris, seed=0, eps=1e-4, tmax=1e30):
+    """One cosine-weighted bounce per hit (hits[i].primitive_id indexes bvh_tris).  Misses produce no ray."""
+    hit = np.isfinite(hits["t"]) & (hits["primitive_id"] != 0xFFFFFFFF)
+    idx = np.nonzero(hit)[0]
+    o = rays["origin"][idx, :3].astype(np.float64)
+    d = rays["direction"][idx, :3].astype(np.float64)
+    t = hits["t"][idx].astype(np.float64)
+    p = o + d * t[:, None]
+    n = triangle_normals(bvh_tris[hits["primitive_id"][idx]])
+    flip = np.sum(n * d, axis=1) > 0
+    n[flip] = -n[flip]
+    u0 = hash_noise(idx.astype(np.uint32), 300, seed)
+    u1 = hash_noise(idx.astype(np.uint32), 301, seed)
+    local = cosine_sample_hemisphere(u0, u1).astype(np.float64)
+    tb, bt, nn = build_orthonormal_basis(n.astype(np.float32))
+    wd = local[:, 0:1] * tb + local[:, 1:2] * bt + local[:, 2:3] * nn
+    wd /= np.linalg.norm(wd, axis=1, keepdims=True)
+    origin = p + n * eps
+    return make_rays(origin.astype(np.float32), wd.astype(np.float32), 0.0, tmax)
+
+
+def empty_rays(n):
+    return np.zeros(n, dtype=RAY)

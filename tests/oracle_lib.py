@@ -1,0 +1,165 @@
+"""ctypes loader of the CPU oracle (oracle/liboracle.so) -- the checker the GPU path is compared against.
+
+Only the test suite, __graft_entry__.smoke() and the CPU-baseline leg of bench.py load it."""
+import ctypes as C
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(REPO, "oracle", "liboracle.so")
+
+_lib = None
+P = C.c_void_p
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            import importlib.util
+            spec = importlib.util.spec_from_file_location("build_oracle", os.path.join(REPO, "oracle",
+                                                                                       "build_oracle.py"))
+            m = importlib.util.module_from_spec(spec)
+            spec.loader.exec_module(m)
+            m.build()
+        L = C.CDLL(LIB)
+        L.orc_sizeof.restype = C.c_int
+        L.orc_sizeof.argtypes = [C.c_int]
+        L.orc_tri_aabbs.argtypes = [P, C.c_uint64, P]
+        L.orc_morton_encode_u64_unorm.restype = C.c_uint64
+        L.orc_morton_encode_u64_unorm.argtypes = [C.c_double] * 3
+        L.orc_split_by_3_u64.restype = C.c_uint64
+        L.orc_split_by_3_u64.argtypes = [C.c_uint32]
+        L.orc_f64_as_u32.restype = C.c_uint32
+        L.orc_f64_as_u32.argtypes = [C.c_double]
+        L.orc_half_area.restype = C.c_float
+        L.orc_half_area.argtypes = [P]
+        L.orc_ploc_build.restype = C.c_int
+        L.orc_ploc_build.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, P, P, P, P, P, C.c_uint32,
+                                     P, P, P]
+        L.orc_bvh2_to_cwbvh.restype = C.c_int
+        L.orc_bvh2_to_cwbvh.argtypes = [P, C.c_uint64, P, C.c_uint64, C.c_uint32, C.c_int, P, C.c_uint64, P,
+                                        C.c_uint64, P, P, P]
+        L.orc_cwbvh_traverse.argtypes = [P, C.c_uint64, P, P, C.c_uint64, C.c_int, P, P, P, P, C.c_int]
+        L.orc_brute_force.argtypes = [P, C.c_uint64, P, C.c_uint64, P]
+        L.orc_tri_intersect.restype = C.c_float
+        L.orc_tri_intersect.argtypes = [P, P]
+        L.orc_intersect_node.restype = C.c_uint32
+        L.orc_intersect_node.argtypes = [P, P, C.c_uint32]
+        L.orc_ray_octant_inv4.restype = C.c_uint32
+        L.orc_ray_octant_inv4.argtypes = [P]
+        L.orc_bvh2_validate.restype = C.c_int
+        L.orc_bvh2_validate.argtypes = [P, C.c_uint64, P, C.c_uint64, P, C.c_int, P]
+        L.orc_cwbvh_validate.restype = C.c_int
+        L.orc_cwbvh_validate.argtypes = [P, C.c_uint64, P, C.c_uint64, P, P]
+        L.orc_now.restype = C.c_double
+        L.orc_log2f.restype = C.c_float
+        L.orc_log2f.argtypes = [C.c_float]
+        L.orc_set_mutation.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code):
+        self.code = code
+        super().__init__(f"oracle error code {code}")
+
+
+def tri_aabbs(tris):
+    from obvhs_amd.types import AABB
+    out = np.zeros(len(tris), dtype=AABB)
+    lib().orc_tri_aabbs(_ptr(tris), len(tris), _ptr(out))
+    return out
+
+
+def ploc_build(aabbs, search_distance=14, search_depth_threshold=0, indices=None, basic_search=False):
+    """Returns dict(nodes (Bvh2Node[2n-1]), prim_indices, codes, sorted_index, iter_counts, depth, max_offset)."""
+    from obvhs_amd.types import BVH2_NODE
+    n = len(aabbs)
+    aabbs = np.ascontiguousarray(aabbs)
+    idx = np.arange(n, dtype=np.uint32) if indices is None else np.ascontiguousarray(indices, dtype=np.uint32)
+    nodes = np.zeros(max(2 * n - 1, 0), dtype=BVH2_NODE)
+    prims = np.zeros(n, dtype=np.uint32)
+    codes = np.zeros(n, dtype=np.uint64)
+    order = np.zeros(n, dtype=np.uint32)
+    cap = 1 << 20
+    iters = np.zeros(cap, dtype=np.uint32)
+    ni, depth, maxoff = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    rc = lib().orc_ploc_build(_ptr(aabbs), _ptr(idx), n, search_distance, search_depth_threshold,
+                              int(basic_search), _ptr(nodes), _ptr(prims), _ptr(codes), _ptr(order), _ptr(iters),
+                              cap, C.byref(ni), C.byref(depth), C.byref(maxoff))
+    if rc != 0:
+        raise OracleError(rc)
+    return dict(nodes=nodes, prim_indices=prims, codes=codes, sorted_index=order,
+                iter_counts=iters[:min(ni.value, cap)].copy(), depth=depth.value, max_offset=maxoff.value)
+
+
+def bvh2_to_cwbvh(nodes, prim_indices, max_prims_per_leaf=3, order_children=True):
+    from obvhs_amd.types import CWBVH_NODE
+    nn = len(nodes)
+    np_ = len(prim_indices)
+    cap = max(nn, 1)
+    out = np.zeros(cap, dtype=CWBVH_NODE)
+    prims = np.zeros(max(np_, 1), dtype=np.uint32)
+    cw2b = np.zeros(cap, dtype=np.uint32)
+    nnodes, nprims = C.c_uint64(), C.c_uint64()
+    rc = lib().orc_bvh2_to_cwbvh(_ptr(np.ascontiguousarray(nodes)), nn, _ptr(np.ascontiguousarray(prim_indices)),
+                                 np_, max_prims_per_leaf, int(order_children), _ptr(out), cap, _ptr(prims),
+                                 len(prims), C.byref(nnodes), C.byref(nprims), _ptr(cw2b))
+    if rc != 0:
+        raise OracleError(rc)
+    return out[:nnodes.value].copy(), prims[:nprims.value].copy(), cw2b[:nnodes.value].copy()
+
+
+def traverse(cw_nodes, bvh_tris, rays, mode=0, threads=1, counts=False):
+    from obvhs_amd.types import RAY_HIT
+    n = len(rays)
+    hits = np.zeros(n, dtype=RAY_HIT) if mode == 0 else None
+    miss = np.zeros(n, dtype=np.uint8) if mode == 1 else None
+    nv = np.zeros(n, dtype=np.uint32) if counts else None
+    tv = np.zeros(n, dtype=np.uint32) if counts else None
+    lib().orc_cwbvh_traverse(_ptr(np.ascontiguousarray(cw_nodes)), len(cw_nodes),
+                             _ptr(np.ascontiguousarray(bvh_tris)), _ptr(np.ascontiguousarray(rays)), n, mode,
+                             _ptr(hits), _ptr(miss), _ptr(nv), _ptr(tv), threads)
+    res = hits if mode == 0 else miss.astype(bool)
+    if counts:
+        return res, nv, tv
+    return res
+
+
+def brute_force(tris, rays):
+    from obvhs_amd.types import RAY_HIT
+    out = np.zeros(len(rays), dtype=RAY_HIT)
+    lib().orc_brute_force(_ptr(np.ascontiguousarray(tris)), len(tris), _ptr(np.ascontiguousarray(rays)), len(rays),
+                          _ptr(out))
+    return out
+
+
+def bvh2_validate(nodes, prim_indices, prim_aabbs, tight_fit=True):
+    md = C.c_uint32()
+    rc = lib().orc_bvh2_validate(_ptr(np.ascontiguousarray(nodes)), len(nodes),
+                                 _ptr(np.ascontiguousarray(prim_indices)), len(prim_indices),
+                                 _ptr(np.ascontiguousarray(prim_aabbs)), int(tight_fit), C.byref(md))
+    return rc, md.value
+
+
+def cwbvh_validate(cw_nodes, prim_indices, prim_aabbs):
+    md = C.c_uint32()
+    rc = lib().orc_cwbvh_validate(_ptr(np.ascontiguousarray(cw_nodes)), len(cw_nodes),
+                                  _ptr(np.ascontiguousarray(prim_indices)), len(prim_indices),
+                                  _ptr(np.ascontiguousarray(prim_aabbs)), C.byref(md))
+    return rc, md.value
+
+
+def build_cwbvh(tris, search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3):
+    """Oracle build_cwbvh_from_tris without pre-splits/reinsertion (cwbvh/builder.rs:61-74)."""
+    aabbs = tri_aabbs(tris)
+    b = ploc_build(aabbs, search_distance, search_depth_threshold)
+    cw, prims, _ = bvh2_to_cwbvh(b["nodes"], b["prim_indices"], max(1, min(3, max_prims_per_leaf)), True)
+    return b, cw, prims
