@@ -188,8 +188,8 @@ obvhs_status obvhs_timer_stop(obvhs_context* ctx, float* ms) {
 
 static obvhs_status ploc_entry(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
                                const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
-                               uint32_t sort_precision, uint32_t search_depth_threshold, obvhs_bvh2* reuse,
-                               obvhs_bvh2** out) {
+                               uint32_t sort_precision, uint32_t search_depth_threshold, uint32_t fused_max_prims,
+                               obvhs_bvh2* reuse, obvhs_bvh2** out) {
     if (!ctx || !out) return OBVHS_ERR_INVALID_ARG;
     if (n > 0 && !d_tris && !d_aabbs) return OBVHS_ERR_INVALID_ARG;
     if (n >= (1u << 31)) return OBVHS_ERR_INVALID_ARG;
@@ -197,7 +197,8 @@ static obvhs_status ploc_entry(obvhs_context* ctx, const obvhs_triangle* d_tris,
     CTX_TRY(hipSetDevice(ctx->device), "set device");
     obvhs_bvh2* bvh = reuse ? reuse : new (std::nothrow) obvhs_bvh2();
     if (!bvh) return OBVHS_ERR_OOM;
-    obvhs_status s = ploc_build(ctx, d_tris, d_aabbs, d_indices, n, search_distance, search_depth_threshold, bvh);
+    obvhs_status s = ploc_build(ctx, d_tris, d_aabbs, d_indices, n, search_distance, search_depth_threshold,
+                                fused_max_prims, bvh);
     if (s != OBVHS_OK) {
         if (!reuse) obvhs_bvh2_destroy(bvh);
         *out = nullptr;
@@ -211,14 +212,14 @@ obvhs_status obvhs_ploc_build_aabbs(obvhs_context* ctx, const obvhs_aabb* d_aabb
                                     uint32_t n, uint32_t search_distance, uint32_t sort_precision,
                                     uint32_t search_depth_threshold, obvhs_bvh2* reuse, obvhs_bvh2** out) {
     return ploc_entry(ctx, nullptr, d_aabbs, d_indices, n, search_distance, sort_precision, search_depth_threshold,
-                      reuse, out);
+                      0u, reuse, out);
 }
 
 obvhs_status obvhs_ploc_build_triangles(obvhs_context* ctx, const obvhs_triangle* d_tris, const uint32_t* d_indices,
                                         uint32_t n, uint32_t search_distance, uint32_t sort_precision,
                                         uint32_t search_depth_threshold, obvhs_bvh2* reuse, obvhs_bvh2** out) {
     return ploc_entry(ctx, d_tris, nullptr, d_indices, n, search_distance, sort_precision, search_depth_threshold,
-                      reuse, out);
+                      0u, reuse, out);
 }
 
 obvhs_status obvhs_bvh2_info(const obvhs_bvh2* bvh, uint32_t* node_count, uint32_t* prim_count,
@@ -278,6 +279,8 @@ obvhs_status obvhs_bvh2_destroy(obvhs_bvh2* bvh) {
     bvh->prim_indices.release();
     bvh->codes.release();
     bvh->sorted_index.release();
+    bvh->hist.release();
+    bvh->dp.release();
     delete bvh;
     return OBVHS_OK;
 }
@@ -295,7 +298,8 @@ obvhs_status obvhs_bvh2_to_cwbvh(obvhs_context* ctx, const obvhs_bvh2* bvh2, uin
     CTX_TRY(hipSetDevice(ctx->device), "set device");
     obvhs_cwbvh* cw = reuse ? reuse : new (std::nothrow) obvhs_cwbvh();
     if (!cw) return OBVHS_ERR_OOM;
-    obvhs_status s = cwbvh_convert(ctx, bvh2, max_prims_per_leaf, order_children != 0, cw);
+    // The handle's cached cost table is internal state: converting does not change the BVH2 itself.
+    obvhs_status s = cwbvh_convert(ctx, const_cast<obvhs_bvh2*>(bvh2), max_prims_per_leaf, order_children != 0, cw);
     if (s != OBVHS_OK) {
         if (!reuse) obvhs_cwbvh_destroy(cw);
         *out = nullptr;
@@ -314,12 +318,13 @@ obvhs_status obvhs_build_cwbvh_from_tris(obvhs_context* ctx, const obvhs_triangl
     obvhs_cwbvh* cw = reuse ? reuse : new (std::nothrow) obvhs_cwbvh();
     if (!cw) return OBVHS_ERR_OOM;
     obvhs_bvh2* b2 = cw->owned_bvh2;
+    uint32_t mp = params->max_prims_per_leaf;
+    mp = mp < 1u ? 1u : (mp > 3u ? 3u : mp);  // cwbvh/builder.rs:74 clamp(1, 3)
+    // The cost table of the conversion is evaluated by the PLOC merges themselves.
     obvhs_status s = ploc_entry(ctx, d_tris, nullptr, nullptr, n, params->ploc_search_distance,
-                                params->sort_precision, params->search_depth_threshold, b2, &b2);
+                                params->sort_precision, params->search_depth_threshold, mp, b2, &b2);
     if (s == OBVHS_OK) {
         cw->owned_bvh2 = b2;
-        uint32_t mp = params->max_prims_per_leaf;
-        mp = mp < 1u ? 1u : (mp > 3u ? 3u : mp);  // cwbvh/builder.rs:74 clamp(1, 3)
         s = cwbvh_convert(ctx, b2, mp, true, cw);
     }
     if (s != OBVHS_OK) {

@@ -2,22 +2,26 @@
 //
 // Restates Bvh2Converter (cwbvh/bvh2_to_cwbvh.rs:15-510) with outputs byte-identical to the reference's two
 // recursive passes:
-//   k_cw_parents : parent links of the BVH2 (bvh2/mod.rs:586-619) and arrival counters.
-//   k_cw_dp      : bottom-up cost table (calculate_cost_impl, bvh2_to_cwbvh.rs:220-344).  One thread per leaf
-//                  climbs; the second thread to arrive at a node (acq_rel atomic) evaluates it.  The same thread
-//                  also evaluates get_children(node, 0) to know the size of the CWBVH subtree that the node would
-//                  root, which lets the top-down pass number nodes exactly like the reference's pre-order DFS.
-//   k_cw_emit    : one launch per CWBVH level.  Each node computes its children (get_children, order_children),
-//                  its exponents and quantised child boxes, writes the 80-byte node and its leaf primitive
-//                  indices, and hands its internal children their DFS-derived node index, child base and
-//                  primitive base (convert_to_cwbvh_impl, bvh2_to_cwbvh.rs:75-193; count_primitives 197-211).
+//   cost table : calculate_cost_impl (bvh2_to_cwbvh.rs:220-344), evaluated by the PLOC merge that creates each node
+//                (cwdp.hpp) or, for a BVH2 built without it, by k_dp_range over the PLOC pass ranges: pass p emitted
+//                the child slots [hist[p+1], hist[p]) and every internal node's children were emitted by an
+//                earlier pass, so ranges processed in pass order see finished children (no atomics, no fences).
+//                Alongside the seven decisions each node stores the size of the CWBVH subtree it roots, which
+//                lets the top-down pass number nodes exactly like the reference's pre-order DFS.
+//   k_cw_emit  : one launch per CWBVH level.  Each node computes its children (get_children, order_children),
+//                its exponents and quantised child boxes, writes the 80-byte node and its leaf primitive indices,
+//                and hands its internal children their DFS-derived node index, child base and primitive base
+//                (convert_to_cwbvh_impl, bvh2_to_cwbvh.rs:75-193; count_primitives 197-211).
 // The exponent exp2(ceil(log2(x))) (bvh2_to_cwbvh.rs:85-88) goes through the host libm's log2f in the reference;
 // a 256-entry table of per-exponent mantissa thresholds, generated from that same log2f when the context is
 // created, reproduces it with integer compares.
 #include <math.h>
 #include <string.h>
 
+#include <vector>
+
 #include "common.hpp"
+#include "cwdp.hpp"
 #include "internal.hpp"
 
 namespace obvhs {
@@ -25,10 +29,8 @@ namespace obvhs {
 namespace {
 
 constexpr int T256 = 256;
-constexpr uint32_t KIND_LEAF = 0, KIND_INTERNAL = 1, KIND_DISTRIBUTE = 2;  // bvh2_to_cwbvh.rs:470-476
-constexpr uint32_t DINVALID = 0xffu;
-constexpr float PRIM_COST = 0.3f;  // bvh2_to_cwbvh.rs:30
 constexpr uint32_t MAX_CW_LEVELS = 64;
+constexpr uint32_t DP_SMALL_RANGE = 4096;  // ranges at most this large are finished by one workgroup
 
 __constant__ uint32_t c_log2_thresh[256];
 
@@ -44,183 +46,44 @@ struct CwMisc {
     uint32_t level_count[MAX_CW_LEVELS + 1];
 };
 
-// Decision packing: kind | distribute_left << 8 | distribute_right << 16 (bvh2_to_cwbvh.rs:478-484).
-__device__ __forceinline__ uint32_t dkind(uint32_t m) { return m & 0xffu; }
-__device__ __forceinline__ uint32_t dleft(uint32_t m) { return (m >> 8) & 0xffu; }
-__device__ __forceinline__ uint32_t dright(uint32_t m) { return (m >> 16) & 0xffu; }
-__device__ __forceinline__ uint32_t dpack(uint32_t k, uint32_t l, uint32_t r) { return k | (l << 8) | (r << 16); }
-
-__global__ void __launch_bounds__(T256) k_cw_parents(const Node32* __restrict__ nodes, uint32_t N,
-                                                     uint32_t* __restrict__ parent, uint32_t* __restrict__ arrive,
-                                                     CwMisc* misc) {
-    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
-    if (i == 0) {
+__global__ void k_cw_init(CwItem* items, CwMisc* misc) {
+    if (threadIdx.x == 0) {
         misc->err = 0u;
         for (uint32_t l = 0; l <= MAX_CW_LEVELS; ++l) misc->level_count[l] = 0u;
-        parent[0] = 0u;
-    }
-    if (i >= N) return;
-    arrive[i] = 0u;
-    const Node32 nd = load_node(&nodes[i]);
-    if (nd.prim_count == 0u) {
-        parent[nd.first_index] = i;
-        parent[nd.first_index + 1] = i;
-    }
-}
-
-// get_children(node, 0) (bvh2_to_cwbvh.rs:346-397) with an explicit stack; left before right.
-// Returns the number of children or -1 on a malformed decision.
-__device__ __forceinline__ int get_children(const Node32* __restrict__ nodes, const uint32_t* __restrict__ dmeta,
-                                            uint32_t node, uint32_t* children) {
-    uint32_t stk_node[8];
-    uint32_t stk_i[8];
-    int sp = 0;
-    int cc = 0;
-    const Node32 root = load_node(&nodes[node]);
-    if (root.prim_count != 0u) {
-        children[cc++] = node;
-        return cc;
-    }
-    stk_node[sp] = node;
-    stk_i[sp] = 0u;
-    sp++;
-    while (sp > 0) {
-        sp--;
-        const uint32_t x = stk_node[sp], i = stk_i[sp];
-        if (i == 7u) {  // emit as a child
-            if (cc >= 8) return -1;
-            children[cc++] = x;
-            continue;
-        }
-        const uint32_t first = nodes[x].first_index;
-        const uint32_t m = dmeta[(size_t)x * 7 + i];
-        const uint32_t dl = dleft(m), dr = dright(m);
-        if (dl >= 7u || dr >= 7u) return -2;
-        if (sp + 2 > 8) return -1;
-        // right pushed first so that the left subtree is produced first
-        const uint32_t kr = dkind(dmeta[(size_t)(first + 1) * 7 + dr]);
-        stk_node[sp] = first + 1;
-        stk_i[sp] = kr == KIND_DISTRIBUTE ? dr : 7u;
-        sp++;
-        const uint32_t kl = dkind(dmeta[(size_t)first * 7 + dl]);
-        stk_node[sp] = first;
-        stk_i[sp] = kl == KIND_DISTRIBUTE ? dl : 7u;
-        sp++;
-    }
-    return cc;
-}
-
-// Evaluates calculate_cost_impl for an internal node whose children are finished (bvh2_to_cwbvh.rs:268-340).
-__device__ __forceinline__ void dp_internal(uint32_t node, float ha, uint32_t first, uint32_t num_primitives,
-                                            uint32_t max_prims, float* __restrict__ dcost,
-                                            uint32_t* __restrict__ dmeta) {
-    float lc[7], rc[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        lc[k] = dcost[(size_t)first * 7 + k];
-        rc[k] = dcost[(size_t)(first + 1) * 7 + k];
-    }
-    float cost[7];
-    uint32_t meta[7];
-    {
-        const float cost_leaf = num_primitives <= max_prims ? ((float)num_primitives) * ha * PRIM_COST : INFINITY;
-        float cost_distribute = INFINITY;
-        uint32_t dl = DINVALID, dr = DINVALID;
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            const float c = lc[k] + rc[6 - k];
-            if (c < cost_distribute) {
-                cost_distribute = c;
-                dl = (uint32_t)k;
-                dr = (uint32_t)(6 - k);
-            }
-        }
-        const float cost_internal = cost_distribute + ha;
-        if (cost_leaf < cost_internal) {
-            cost[0] = cost_leaf;
-            meta[0] = dpack(KIND_LEAF, dl, dr);
-        } else {
-            cost[0] = cost_internal;
-            meta[0] = dpack(KIND_INTERNAL, dl, dr);
-        }
-    }
-#pragma unroll
-    for (int i = 1; i < 7; ++i) {
-        float cost_distribute = cost[i - 1];
-        uint32_t dl = DINVALID, dr = DINVALID;
-#pragma unroll
-        for (int k = 0; k < i; ++k) {
-            const float c = lc[k] + rc[i - k - 1];
-            if (c < cost_distribute) {
-                cost_distribute = c;
-                dl = (uint32_t)k;
-                dr = (uint32_t)(i - k - 1);
-            }
-        }
-        if (dl != DINVALID) {
-            cost[i] = cost_distribute;
-            meta[i] = dpack(KIND_DISTRIBUTE, dl, dr);
-        } else {
-            cost[i] = cost[i - 1];
-            meta[i] = meta[i - 1];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        dcost[(size_t)node * 7 + k] = cost[k];
-        dmeta[(size_t)node * 7 + k] = meta[k];
-    }
-}
-
-__global__ void __launch_bounds__(T256) k_cw_dp(const Node32* __restrict__ nodes, uint32_t N,
-                                                const uint32_t* __restrict__ parent, uint32_t* arrive,
-                                                float* dcost, uint32_t* dmeta, uint32_t* nprims, uint32_t* cwsize,
-                                                uint32_t max_prims, CwMisc* misc) {
-    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
-    if (i >= N) return;
-    const Node32 leaf = load_node(&nodes[i]);
-    if (leaf.prim_count == 0u) return;
-    if (leaf.prim_count != 1u) {  // bvh2_to_cwbvh.rs:241-247
-        atomicOr(&misc->err, DERR_CW_NONUNIT_LEAF);
-        return;
-    }
-    const float cost_leaf = half_area(node_box(leaf)) * 1.0f * PRIM_COST;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        dcost[(size_t)i * 7 + k] = cost_leaf;
-        dmeta[(size_t)i * 7 + k] = dpack(KIND_LEAF, 0u, 0u);
-    }
-    nprims[i] = 1u;
-    cwsize[i] = 1u;
-    uint32_t x = i;
-    while (x != 0u) {
-        const uint32_t p = parent[x];
-        const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) return;  // the sibling's thread finishes this node
-        const Node32 pn = load_node(&nodes[p]);
-        const uint32_t np = nprims[pn.first_index] + nprims[pn.first_index + 1];
-        dp_internal(p, half_area(node_box(pn)), pn.first_index, np, max_prims, dcost, dmeta);
-        nprims[p] = np;
-        uint32_t ch[8];
-        const int cc = get_children(nodes, dmeta, p, ch);
-        if (cc < 0) {
-            atomicOr(&misc->err, cc == -2 ? DERR_CW_DISTRIBUTE : DERR_CAPACITY);
-            cwsize[p] = 1u;
-        } else {
-            uint32_t sz = 1u;
-            for (int c = 0; c < cc; ++c)
-                if (dkind(dmeta[(size_t)ch[c] * 7]) == KIND_INTERNAL) sz += cwsize[ch[c]];
-            cwsize[p] = sz;
-        }
-        x = p;
-    }
-}
-
-__global__ void k_cw_root(CwItem* items, CwMisc* misc) {
-    if (threadIdx.x == 0) {
         items[0] = CwItem{0u, 0u, 0u, 0u};
         misc->level_count[0] = 1u;
     }
+}
+
+__device__ __forceinline__ void dp_node(const Node32* __restrict__ nodes, const DpView& dp, uint32_t x,
+                                        uint32_t* err) {
+    const Node32 nx = load_node(&nodes[x]);
+    if (nx.prim_count != 0u) return;
+    const Node32 l = load_node(&nodes[nx.first_index]);
+    const Node32 r = load_node(&nodes[nx.first_index + 1]);
+    const uint32_t e = dp_evaluate(dp, node_box(nx), nx.first_index, l, r);
+    if (e) atomicOr(err, e);
+}
+
+// Cost table of the nodes a large PLOC pass emitted (slots [lo, hi)).
+__global__ void __launch_bounds__(T256) k_dp_range(const Node32* __restrict__ nodes, DpView dp, uint32_t lo,
+                                                   uint32_t hi, CwMisc* misc) {
+    const uint32_t x = lo + blockIdx.x * T256 + threadIdx.x;
+    if (x >= hi) return;
+    dp_node(nodes, dp, x, &misc->err);
+}
+
+// Cost table of every node in slots [1, hi) -- the small late passes, processed pass by pass inside one
+// workgroup -- and of the root.  hist[p] for p in [p0, passes] bounds the ranges.
+__global__ void __launch_bounds__(1024) k_dp_small(const Node32* __restrict__ nodes, DpView dp,
+                                                   const uint32_t* __restrict__ hist, uint32_t p0,
+                                                   uint32_t passes, CwMisc* misc) {
+    for (uint32_t p = p0; p < passes; ++p) {
+        const uint32_t lo = hist[p + 1], hi = hist[p];
+        for (uint32_t x = lo + threadIdx.x; x < hi; x += 1024) dp_node(nodes, dp, x, &misc->err);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dp_node(nodes, dp, 0u, &misc->err);
 }
 
 // exp2(ceil(log2(x))) for x = max(extent, 1e-20) * (1/255) as the reference computes it with libm; returns the
@@ -237,9 +100,7 @@ __device__ __forceinline__ uint32_t cw_exponent(float ext) {
 
 __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nodes,
                                                   const uint32_t* __restrict__ bvh2_prims,
-                                                  const uint32_t* __restrict__ dmeta,
-                                                  const uint32_t* __restrict__ nprims,
-                                                  const uint32_t* __restrict__ cwsize,
+                                                  DpView dp,
                                                   const CwItem* __restrict__ in_items, CwItem* __restrict__ out_items,
                                                   uint32_t level, uint32_t order_children, CwNode* __restrict__ out,
                                                   uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
@@ -252,16 +113,30 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     const Box nb = node_box(nd);
 
     uint32_t ch[8];
-    int cc = get_children(nodes, dmeta, it.bvh2, ch);
+    int cc;
+    if (nd.prim_count != 0u) {  // a single-leaf BVH2: the root is its own only child
+        ch[0] = it.bvh2;
+        cc = 1;
+    } else {
+        cc = get_children_keyed(nodes, dp, nd.first_index, node_decision(dp, nd.first_index, 0), ch);
+    }
     if (cc < 0) {
         atomicOr(&misc->err, cc == -2 ? DERR_CW_DISTRIBUTE : DERR_CAPACITY);
         return;
     }
     Box cb[8];
+    uint32_t ckind = 0u;   // 2 bits per child: its decision-0 kind
+    uint32_t cfirst[8];    // first_index of each child
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        if (c < cc) cb[c] = node_box(load_node(&nodes[ch[c]]));
-        else cb[c] = Box{0, 0, 0, 0, 0, 0};
+        cb[c] = Box{0, 0, 0, 0, 0, 0};
+        cfirst[c] = 0u;
+        if (c < cc) {
+            const Node32 cn = load_node(&nodes[ch[c]]);
+            cb[c] = node_box(cn);
+            cfirst[c] = cn.first_index;
+            ckind |= node_kind(dp, cn, 0) << (2 * c);
+        }
     }
     // order_children (bvh2_to_cwbvh.rs:402-467): greedy octant assignment, child-major / slot-minor, strict <.
     int slot_of[8];
@@ -355,7 +230,10 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
 #pragma unroll
         for (int q = 1; q < 8; ++q)
             if (q == c) b = cb[q];
-        const uint32_t ci = ch[c];
+        uint32_t ci = ch[0];
+#pragma unroll
+        for (int q = 1; q < 8; ++q)
+            if (q == c) ci = ch[q];
         float lo, hi;
         lo = floorf(((b.lx - o.p[0]) - 0.0f) * rx);
         hi = ceilf(((b.hx - o.p[0]) + 0.0f) * rx);
@@ -369,7 +247,7 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
         hi = ceilf(((b.hz - o.p[2]) + 0.0f) * rz);
         o.child_min_z[s] = (uint8_t)smin(smax(lo, 0.0f), 255.0f);
         o.child_max_z[s] = (uint8_t)smin(smax(hi, 0.0f), 255.0f);
-        const uint32_t kind = dkind(dmeta[(size_t)ci * 7]);
+        const uint32_t kind = (ckind >> (2 * c)) & 3u;
         if (kind == KIND_LEAF) {
             // count_primitives: left-first DFS of the (at most 3-leaf) subtree.
             uint32_t stk[8];
@@ -434,13 +312,16 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     for (uint32_t j = 0; j < num_internal; ++j) {
         const uint32_t s = internal_slots[j];
         const int c = slot_child[s];
-        uint32_t ci = ch[0];
+        uint32_t ci = ch[0], cf = cfirst[0];
 #pragma unroll
         for (int q = 1; q < 8; ++q)
-            if (q == c) ci = ch[q];
+            if (q == c) {
+                ci = ch[q];
+                cf = cfirst[q];
+            }
         out_items[base + j] = CwItem{child_base + j, ci, a_run, b_run};
-        a_run += cwsize[ci] - 1u;
-        b_run += nprims[ci];
+        a_run += dp.cwsize[dkey(cf)] - 1u;
+        b_run += dp.nprims[dkey(cf)];
     }
 }
 
@@ -497,7 +378,25 @@ hipError_t upload_log2_table() {
         if (_e != hipSuccess) return set_hip_error(ctx, _e, where); \
     } while (0)
 
-obvhs_status cwbvh_convert(obvhs_context* ctx, const obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,
+hipError_t bvh_dp_alloc(obvhs_bvh2* bvh) {
+    const size_t k = bvh->prim_count > 1 ? bvh->prim_count - 1 : 1;
+    return bvh->dp.ensure(k * (28 + 8 + 28 + 4 + 4) + 1024);
+}
+
+DpView bvh_dp_view(const obvhs_bvh2* bvh, uint32_t max_prims) {
+    const size_t k = bvh->prim_count > 1 ? bvh->prim_count - 1 : 1;
+    char* base = bvh->dp.as<char>();
+    DpView v;
+    v.cost = reinterpret_cast<float*>(base);
+    v.meta = reinterpret_cast<uint2*>(base + k * 28);
+    v.sub = reinterpret_cast<uint32_t*>(base + k * 36);
+    v.nprims = reinterpret_cast<uint32_t*>(base + k * 64);
+    v.cwsize = reinterpret_cast<uint32_t*>(base + k * 68);
+    v.max_prims = max_prims;
+    return v;
+}
+
+obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,
                            bool order_children, obvhs_cwbvh* out) {
     hipStream_t st = ctx->stream;
     const uint32_t N = bvh2->node_count;
@@ -506,40 +405,56 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, const obvhs_bvh2* bvh2, uint32_t 
     memset(&out->total_aabb, 0, sizeof(out->total_aabb));
     if (N == 0) return OBVHS_OK;  // bvh2_to_cwbvh.rs:496-498
     const uint32_t n = bvh2->prim_count;
-    // Scratch layout.
     size_t off = 0;
     auto carve = [&](size_t bytes) {
         size_t o = off;
         off += (bytes + 255) & ~size_t(255);
         return o;
     };
-    const size_t o_parent = carve((size_t)N * 4), o_arrive = carve((size_t)N * 4), o_cost = carve((size_t)N * 28),
-                 o_meta = carve((size_t)N * 28), o_np = carve((size_t)N * 4), o_sz = carve((size_t)N * 4),
-                 o_items0 = carve((size_t)N * sizeof(CwItem)), o_items1 = carve((size_t)N * sizeof(CwItem)),
+    const size_t o_items0 = carve((size_t)N * sizeof(CwItem)), o_items1 = carve((size_t)N * sizeof(CwItem)),
                  o_misc = carve(sizeof(CwMisc));
     OBVHS_TRY(ctx->scratch.cw_scratch.ensure(off), "alloc cwbvh scratch");
     char* base = ctx->scratch.cw_scratch.as<char>();
-    uint32_t* parent = reinterpret_cast<uint32_t*>(base + o_parent);
-    uint32_t* arrive = reinterpret_cast<uint32_t*>(base + o_arrive);
-    float* dcost = reinterpret_cast<float*>(base + o_cost);
-    uint32_t* dmeta = reinterpret_cast<uint32_t*>(base + o_meta);
-    uint32_t* nprims = reinterpret_cast<uint32_t*>(base + o_np);
-    uint32_t* cwsize = reinterpret_cast<uint32_t*>(base + o_sz);
     CwItem* items[2] = {reinterpret_cast<CwItem*>(base + o_items0), reinterpret_cast<CwItem*>(base + o_items1)};
     CwMisc* misc = reinterpret_cast<CwMisc*>(base + o_misc);
     const Node32* nodes = bvh2->nodes.as<Node32>();
-    const uint32_t blocks = (N + T256 - 1) / T256;
-    k_cw_parents<<<blocks, T256, 0, st>>>(nodes, N, parent, arrive, misc);
-    k_cw_dp<<<blocks, T256, 0, st>>>(nodes, N, parent, arrive, dcost, dmeta, nprims, cwsize, max_prims_per_leaf,
-                                     misc);
-    k_cw_root<<<1, 64, 0, st>>>(items[0], misc);
-    OBVHS_TRY(hipGetLastError(), "cwbvh dp launch");
+    k_cw_init<<<1, 64, 0, st>>>(items[0], misc);
     uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
-    OBVHS_TRY(hipMemcpyAsync(h, cwsize, 4, hipMemcpyDeviceToHost, st), "read cw size");
+    if (bvh2->dp_max_prims != max_prims_per_leaf && n > 1) {
+        OBVHS_TRY(bvh_dp_alloc(bvh2), "alloc cost table");
+        const DpView dpv = bvh_dp_view(bvh2, max_prims_per_leaf);
+        const uint32_t passes = bvh2->ploc_iterations;
+        std::vector<uint32_t> hist(passes + 1);
+        OBVHS_TRY(hipMemcpyAsync(hist.data(), bvh2->hist.ptr, hist.size() * 4, hipMemcpyDeviceToHost, st),
+                  "read pass history");
+        OBVHS_TRY(hipStreamSynchronize(st), "pass history sync");
+        uint32_t p = 0;
+        for (; p < passes; ++p) {
+            const uint32_t lo = hist[p + 1], hi = hist[p];
+            if (hi - lo <= DP_SMALL_RANGE) break;
+            k_dp_range<<<(hi - lo + T256 - 1) / T256, T256, 0, st>>>(nodes, dpv, lo, hi, misc);
+        }
+        k_dp_small<<<1, 1024, 0, st>>>(nodes, dpv, bvh2->hist.as<uint32_t>(), p, passes, misc);
+        OBVHS_TRY(hipGetLastError(), "cost table launch");
+        bvh2->dp_max_prims = max_prims_per_leaf;
+    }
+    const DpView dp = bvh_dp_view(bvh2, max_prims_per_leaf);
+    // Root record, CWBVH node count (the root's subtree size) and device errors.
+    Node32 root;
+    OBVHS_TRY(hipMemcpyAsync(h, nodes, sizeof(Node32), hipMemcpyDeviceToHost, st), "read root");
+    OBVHS_TRY(hipStreamSynchronize(st), "root sync");
+    memcpy(&root, h, sizeof(Node32));
+    uint32_t ncw = 1;
+    if (root.prim_count == 0u) {
+        OBVHS_TRY(hipMemcpyAsync(h, dp.cwsize + dkey(root.first_index), 4, hipMemcpyDeviceToHost, st),
+                  "read cw size");
+    }
     OBVHS_TRY(hipMemcpyAsync(h + 1, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cw err");
     OBVHS_TRY(hipStreamSynchronize(st), "cwbvh dp sync");
     if (h[1]) return device_error_to_status(h[1]);
-    const uint32_t ncw = h[0];
+    if (root.prim_count == 0u) ncw = h[0];
+    if (ncw == 0u) return OBVHS_ERR_CW_DISTRIBUTE;  // some CWBVH node's children cannot be distributed
+    if (ncw > N) return OBVHS_ERR_CAPACITY;
     OBVHS_TRY(out->nodes.ensure((size_t)ncw * sizeof(CwNode)), "alloc cwbvh nodes");
     OBVHS_TRY(out->prim_indices.ensure((size_t)n * 4 + 4), "alloc cwbvh primitive indices");
     uint32_t level_count = 1;
@@ -547,8 +462,8 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, const obvhs_bvh2* bvh2, uint32_t 
     while (level_count > 0) {
         if (level >= MAX_CW_LEVELS) return OBVHS_ERR_DEPTH;
         const uint32_t g = (level_count + T256 - 1) / T256;
-        k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2->prim_indices.as<uint32_t>(), dmeta, nprims, cwsize,
-                                      items[level & 1], items[(level + 1) & 1], level, order_children ? 1u : 0u,
+        k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2->prim_indices.as<uint32_t>(), dp, items[level & 1],
+                                      items[(level + 1) & 1], level, order_children ? 1u : 0u,
                                       out->nodes.as<CwNode>(), out->prim_indices.as<uint32_t>(), ncw, n, misc);
         OBVHS_TRY(hipGetLastError(), "cwbvh emit launch");
         OBVHS_TRY(hipMemcpyAsync(h, &misc->level_count[level + 1], 4, hipMemcpyDeviceToHost, st), "read level");
@@ -559,13 +474,10 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, const obvhs_bvh2* bvh2, uint32_t 
         level++;
     }
     // total_aabb = bvh2.nodes[0].aabb (bvh2_to_cwbvh.rs:506)
-    Node32 root;
-    OBVHS_TRY(hipMemcpy(&root, nodes, sizeof(Node32), hipMemcpyDeviceToHost), "read root");
     out->total_aabb.min = obvhs_vec3a{root.minx, root.miny, root.minz, 0.0f};
     out->total_aabb.max = obvhs_vec3a{root.maxx, root.maxy, root.maxz, 0.0f};
     out->node_count = ncw;
     out->prim_count = n;
-    // The CWBVH is a tree of depth `level`; traversal's 32-entry stack bounds it (cwbvh/mod.rs:791).
     return OBVHS_OK;
 }
 

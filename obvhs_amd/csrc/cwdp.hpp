@@ -1,0 +1,223 @@
+// The BVH2 -> CWBVH cost table (calculate_cost_impl, cwbvh/bvh2_to_cwbvh.rs:220-344) and get_children
+// (bvh2_to_cwbvh.rs:346-397) over a BVH2 whose internal nodes are identified by their first child slot.
+//
+// A PLOC internal node's children always exist before the node does, and the node's own array slot is assigned
+// only later, when it is merged in turn.  Its first_index (the slot of its left child, always odd) is known at
+// creation, so the table is keyed by first_index >> 1.  That lets the merge that creates a node evaluate the
+// node's entry immediately (children finished earlier, no cross-workgroup synchronisation), and the same
+// functions serve a standalone pass over a finished BVH2.  Leaves need no entry: all seven of their decisions are
+// LEAF with cost half_area * 1 * 0.3 (bvh2_to_cwbvh.rs:249-256).
+//
+// Besides the reference's seven (cost, kind, distribute_left, distribute_right) decisions, each entry holds, per
+// decision i, sub[i] = the number of CWBVH nodes in the subtrees of the INTERNAL children that get_children(node, i)
+// would produce.  It obeys the same recurrence as the costs (children only), so no tree walk is needed, and
+// cwsize = 1 + sub[0] is the size of the CWBVH subtree the node roots if it becomes a CWBVH node -- what the
+// top-down pass needs to number nodes in the reference's pre-order (bvh2_to_cwbvh.rs:172-191).
+#pragma once
+
+#include "common.hpp"
+
+namespace obvhs {
+
+constexpr uint32_t KIND_LEAF = 0, KIND_INTERNAL = 1, KIND_DISTRIBUTE = 2;  // bvh2_to_cwbvh.rs:470-476
+constexpr uint32_t DIDX_INVALID = 7u;      // distribute index "INVALID" (u8::MAX in the reference) in 3 bits
+constexpr uint32_t SUB_INVALID = 0xffffffffu;
+constexpr float PRIM_COST = 0.3f;  // bvh2_to_cwbvh.rs:30
+
+// Decision byte: kind (2 bits) | distribute_left << 2 (3 bits) | distribute_right << 5 (3 bits).
+struct DpView {
+    float* cost;        // [k][7]
+    uint2* meta;        // [k] seven decision bytes (byte 7 unused)
+    uint32_t* sub;      // [k][7]
+    uint32_t* nprims;   // [k] primitives under the node
+    uint32_t* cwsize;   // [k] 1 + sub[0], or 0 when the node's CWBVH subtree cannot be built
+    uint32_t max_prims; // max_prims_per_leaf the table was evaluated for
+};
+
+__host__ __device__ __forceinline__ uint32_t dkey(uint32_t first_index) { return first_index >> 1; }
+__device__ __forceinline__ uint32_t dbyte(uint2 m, uint32_t i) {
+    return ((i < 4u ? m.x : m.y) >> (8u * (i & 3u))) & 0xffu;
+}
+__device__ __forceinline__ uint32_t dkind(uint32_t b) { return b & 3u; }
+__device__ __forceinline__ uint32_t dleft(uint32_t b) { return (b >> 2) & 7u; }
+__device__ __forceinline__ uint32_t dright(uint32_t b) { return (b >> 5) & 7u; }
+__device__ __forceinline__ uint32_t dpack(uint32_t k, uint32_t l, uint32_t r) { return k | (l << 2) | (r << 5); }
+
+// Decision i of an internal node's entry.
+__device__ __forceinline__ uint32_t node_decision(const DpView& dp, uint32_t first_index, uint32_t i) {
+    return dbyte(dp.meta[dkey(first_index)], i);
+}
+__device__ __forceinline__ uint32_t node_kind(const DpView& dp, const Node32& nx, uint32_t i) {
+    return nx.prim_count != 0u ? KIND_LEAF : dkind(node_decision(dp, nx.first_index, i));
+}
+
+// Everything the parent's evaluation needs from one child.  (Per-decision values selected by a runtime index are
+// read from memory, not from a register array, so the evaluation stays free of private-memory spills.)
+struct ChildDp {
+    float cost[7];
+    uint2 meta;       // seven decision bytes
+    size_t key;
+    uint32_t nprims;
+    uint32_t cwsize;
+    bool leaf;
+};
+
+__device__ __forceinline__ void load_child_dp(const DpView& dp, const Node32& c, ChildDp& o) {
+    o.leaf = c.prim_count != 0u;
+    if (o.leaf) {
+        const float leaf = half_area(node_box(c)) * (float)c.prim_count * PRIM_COST;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) o.cost[k] = leaf;
+        o.meta = make_uint2(0u, 0u);
+        o.key = 0;
+        o.nprims = c.prim_count;
+        o.cwsize = 1u;
+    } else {
+        o.key = dkey(c.first_index);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) o.cost[k] = dp.cost[o.key * 7 + k];
+        o.meta = dp.meta[o.key];
+        o.nprims = dp.nprims[o.key];
+        o.cwsize = dp.cwsize[o.key];
+    }
+}
+
+// Contribution of child c taken through its decision d: its own CWBVH subtree when it becomes a CWBVH child,
+// the contributions of its distributed descendants when it is DISTRIBUTE, nothing when it is a leaf child.
+__device__ __forceinline__ uint32_t child_contribution(const DpView& dp, const ChildDp& c, uint32_t d) {
+    if (c.leaf) return 0u;
+    if (dkind(dbyte(c.meta, d)) == KIND_DISTRIBUTE) return dp.sub[c.key * 7 + d];
+    if (dkind(dbyte(c.meta, 0u)) == KIND_INTERNAL) return c.cwsize == 0u ? SUB_INVALID : c.cwsize;
+    return 0u;
+}
+
+// Evaluates the entry of the internal node with box `box` whose children records are `l` and `r` (slots first,
+// first + 1) and stores it under dkey(first).  Returns a DERR_* bit or 0.
+__device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box, uint32_t first, const Node32& l,
+                                                const Node32& r) {
+    if ((l.prim_count != 0u && l.prim_count != 1u) || (r.prim_count != 0u && r.prim_count != 1u))
+        return DERR_CW_NONUNIT_LEAF;  // bvh2_to_cwbvh.rs:243-247
+    ChildDp L, R;
+    load_child_dp(dp, l, L);
+    load_child_dp(dp, r, R);
+    const uint32_t num_primitives = L.nprims + R.nprims;
+    const float ha = half_area(box);
+    float cost[7];
+    uint32_t meta[7];
+    {
+        const float cost_leaf =
+            num_primitives <= dp.max_prims ? ((float)num_primitives) * ha * PRIM_COST : INFINITY;
+        float cost_distribute = INFINITY;
+        uint32_t dl = DIDX_INVALID, dr = DIDX_INVALID;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const float c = L.cost[k] + R.cost[6 - k];
+            if (c < cost_distribute) {
+                cost_distribute = c;
+                dl = (uint32_t)k;
+                dr = (uint32_t)(6 - k);
+            }
+        }
+        const float cost_internal = cost_distribute + ha;
+        if (cost_leaf < cost_internal) {
+            cost[0] = cost_leaf;
+            meta[0] = dpack(KIND_LEAF, dl, dr);
+        } else {
+            cost[0] = cost_internal;
+            meta[0] = dpack(KIND_INTERNAL, dl, dr);
+        }
+    }
+#pragma unroll
+    for (int i = 1; i < 7; ++i) {
+        float cost_distribute = cost[i - 1];
+        uint32_t dl = DIDX_INVALID, dr = DIDX_INVALID;
+#pragma unroll
+        for (int k = 0; k < i; ++k) {
+            const float c = L.cost[k] + R.cost[i - k - 1];
+            if (c < cost_distribute) {
+                cost_distribute = c;
+                dl = (uint32_t)k;
+                dr = (uint32_t)(i - k - 1);
+            }
+        }
+        if (dl != DIDX_INVALID) {
+            cost[i] = cost_distribute;
+            meta[i] = dpack(KIND_DISTRIBUTE, dl, dr);
+        } else {
+            cost[i] = cost[i - 1];
+            meta[i] = meta[i - 1];
+        }
+    }
+    uint32_t sub[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        const uint32_t dl = dleft(meta[i]), dr = dright(meta[i]);
+        if (dl == DIDX_INVALID || dr == DIDX_INVALID) {
+            sub[i] = SUB_INVALID;
+        } else {
+            const uint32_t a = child_contribution(dp, L, dl), b = child_contribution(dp, R, dr);
+            sub[i] = (a == SUB_INVALID || b == SUB_INVALID) ? SUB_INVALID : a + b;
+        }
+    }
+    const size_t key = dkey(first);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        dp.cost[key * 7 + k] = cost[k];
+        dp.sub[key * 7 + k] = sub[k];
+    }
+    dp.meta[key] = make_uint2(meta[0] | meta[1] << 8 | meta[2] << 16 | meta[3] << 24,
+                              meta[4] | meta[5] << 8 | meta[6] << 16);
+    dp.nprims[key] = num_primitives;
+    dp.cwsize[key] = sub[0] == SUB_INVALID ? 0u : 1u + sub[0];
+    return 0u;
+}
+
+// get_children(node, 0) for an internal node whose first child slot is `first` and whose decision 0 is `meta0`.
+// Children are appended left-first like the reference's recursion.  Returns the count, -1 when more than 8
+// children would be produced, -2 on a malformed decision (distribute index invalid).
+__device__ __forceinline__ int get_children_keyed(const Node32* __restrict__ nodes, const DpView& dp,
+                                                  uint32_t first, uint32_t meta0, uint32_t* children) {
+    uint32_t stk_node[8];
+    uint32_t stk_i[8];
+    int sp = 0;
+    int cc = 0;
+    uint32_t dl = dleft(meta0), dr = dright(meta0);
+    if (dl == DIDX_INVALID || dr == DIDX_INVALID) return -2;
+    {
+        const Node32 r = load_node(&nodes[first + 1]);
+        const Node32 l = load_node(&nodes[first]);
+        stk_node[sp] = first + 1;
+        stk_i[sp] = node_kind(dp, r, dr) == KIND_DISTRIBUTE ? dr : 7u;
+        sp++;
+        stk_node[sp] = first;
+        stk_i[sp] = node_kind(dp, l, dl) == KIND_DISTRIBUTE ? dl : 7u;
+        sp++;
+    }
+    while (sp > 0) {
+        sp--;
+        const uint32_t x = stk_node[sp], i = stk_i[sp];
+        if (i == 7u) {
+            if (cc >= 8) return -1;
+            children[cc++] = x;
+            continue;
+        }
+        const Node32 nx = load_node(&nodes[x]);  // internal (a leaf is never DISTRIBUTE)
+        const uint32_t m = node_decision(dp, nx.first_index, i);
+        const uint32_t f = nx.first_index;
+        dl = dleft(m);
+        dr = dright(m);
+        if (dl == DIDX_INVALID || dr == DIDX_INVALID) return -2;
+        if (sp + 2 > 8) return -1;
+        const Node32 r = load_node(&nodes[f + 1]);
+        const Node32 l = load_node(&nodes[f]);
+        stk_node[sp] = f + 1;
+        stk_i[sp] = node_kind(dp, r, dr) == KIND_DISTRIBUTE ? dr : 7u;
+        sp++;
+        stk_node[sp] = f;
+        stk_i[sp] = node_kind(dp, l, dl) == KIND_DISTRIBUTE ? dl : 7u;
+        sp++;
+    }
+    return cc;
+}
+
+}  // namespace obvhs

@@ -71,6 +71,12 @@ struct obvhs_bvh2 {
     uint32_t max_depth = 96;
     uint32_t ploc_iterations = 0;
     uint64_t cluster_sum = 0;
+    // Insert index after each PLOC pass: pass p emitted the child slots [hist[p+1], hist[p]).
+    obvhs::DevBuf hist;
+    // CWBVH cost table keyed by first_index >> 1 (cost[7], meta[7], nprims, cwsize per internal node), valid for
+    // dp_max_prims (0 = not evaluated).
+    obvhs::DevBuf dp;
+    uint32_t dp_max_prims = 0;
 };
 
 struct obvhs_cwbvh {
@@ -94,10 +100,13 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identi
 // ploc.hip
 obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
                         const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
-                        uint32_t search_depth_threshold, obvhs_bvh2* bvh);
+                        uint32_t search_depth_threshold, uint32_t fused_max_prims, obvhs_bvh2* bvh);
+struct DpView;
+hipError_t bvh_dp_alloc(obvhs_bvh2* bvh);
+DpView bvh_dp_view(const obvhs_bvh2* bvh, uint32_t max_prims);
 
 // cwbvh.hip
-obvhs_status cwbvh_convert(obvhs_context* ctx, const obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,
+obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,
                            bool order_children, obvhs_cwbvh* out);
 const uint32_t* host_log2_table();
 hipError_t upload_log2_table();

@@ -17,6 +17,7 @@
 // Pass state (cluster count, insert index, depth) lives on the device so passes are enqueued without host syncs;
 // the host only reads the count every few passes to shrink the grids.
 #include "common.hpp"
+#include "cwdp.hpp"
 #include "internal.hpp"
 
 namespace obvhs {
@@ -57,8 +58,9 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
     return v;
 }
 
-__global__ void k_init_misc(Misc* misc, uint32_t n) {
+__global__ void k_init_misc(Misc* misc, uint32_t n, uint32_t* hist) {
     if (threadIdx.x == 0) {
+        hist[0] = 2u * n - 1u;
         for (int a = 0; a < 3; ++a) {
             misc->bounds[a] = 0xffffffffu;
             misc->bounds[3 + a] = 0u;
@@ -71,7 +73,8 @@ __global__ void k_init_misc(Misc* misc, uint32_t n) {
 
 // Reduces the leaf bounds of one block into the scene bounds.  Both the min and the max corner of every leaf go
 // into both reductions, like `local_aabb.extend(aabb.min); local_aabb.extend(aabb.max)` (ploc/mod.rs:189-190).
-__device__ __forceinline__ void reduce_scene_bounds(bool valid, const Box& b, bool nan, Misc* misc) {
+__device__ __forceinline__ void reduce_scene_bounds(bool valid, const Box& b, bool nan, Misc* misc,
+                                                    uint32_t* __restrict__ partials) {
     __shared__ uint32_t red[6][T256 / 64];
     uint32_t k[6];
     if (valid) {
@@ -100,11 +103,34 @@ __device__ __forceinline__ void reduce_scene_bounds(bool valid, const Box& b, bo
         const int a = threadIdx.x;
         uint32_t v = red[a][0];
         for (int w = 1; w < T256 / 64; ++w) v = a < 3 ? min(v, red[a][w]) : max(v, red[a][w]);
-        if (a < 3) {
-            if (v != 0xffffffffu) atomicMin(&misc->bounds[a], v);
-        } else {
-            if (v != 0u) atomicMax(&misc->bounds[a], v);
+        partials[blockIdx.x * 6 + a] = v;
+    }
+}
+
+// Final reduction of the per-block scene bounds (one workgroup).
+__global__ void __launch_bounds__(1024) k_reduce_bounds(const uint32_t* __restrict__ partials, uint32_t nblocks,
+                                                        Misc* misc) {
+    __shared__ uint32_t red[6][16];
+    uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 1024) {
+        for (int a = 0; a < 3; ++a) {
+            k[a] = min(k[a], partials[b * 6 + a]);
+            k[3 + a] = max(k[3 + a], partials[b * 6 + 3 + a]);
         }
+    }
+    for (int a = 0; a < 3; ++a) {
+        k[a] = wave_min_u32(k[a]);
+        k[3 + a] = wave_max_u32(k[3 + a]);
+    }
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (lane == 0)
+        for (int a = 0; a < 6; ++a) red[a][wave] = k[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        uint32_t v = red[a][0];
+        for (int w = 1; w < 16; ++w) v = a < 3 ? min(v, red[a][w]) : max(v, red[a][w]);
+        misc->bounds[a] = v;
     }
 }
 
@@ -115,7 +141,8 @@ __device__ __forceinline__ bool box_has_nan(const Box& b) {
 // Triangle::aabb = Aabb::from_points([v0, v1, v2]) (triangle.rs:28-30; aabb.rs:59-66).
 __global__ void __launch_bounds__(T256) k_leaf_init_tris(const float4* __restrict__ tris,
                                                          const uint32_t* __restrict__ indices, uint32_t n,
-                                                         Node32* __restrict__ leaves, Misc* misc) {
+                                                         Node32* __restrict__ leaves, Misc* misc,
+                                                         uint32_t* __restrict__ partials) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     const bool valid = i < n;
     Box b{0, 0, 0, 0, 0, 0};
@@ -130,13 +157,14 @@ __global__ void __launch_bounds__(T256) k_leaf_init_tris(const float4* __restric
         nan = box_has_nan(b);
         store_node(&leaves[i], make_node(b, 1u, indices ? indices[i] : i));
     }
-    reduce_scene_bounds(valid, b, nan, misc);
+    reduce_scene_bounds(valid, b, nan, misc, partials);
 }
 
 // T = Aabb: the leaf box is the input box (aabb.rs:209-214).
 __global__ void __launch_bounds__(T256) k_leaf_init_aabbs(const float4* __restrict__ aabbs,
                                                           const uint32_t* __restrict__ indices, uint32_t n,
-                                                          Node32* __restrict__ leaves, Misc* misc) {
+                                                          Node32* __restrict__ leaves, Misc* misc,
+                                                         uint32_t* __restrict__ partials) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     const bool valid = i < n;
     Box b{0, 0, 0, 0, 0, 0};
@@ -147,7 +175,7 @@ __global__ void __launch_bounds__(T256) k_leaf_init_aabbs(const float4* __restri
         nan = box_has_nan(b);
         store_node(&leaves[i], make_node(b, 1u, indices ? indices[i] : i));
     }
-    reduce_scene_bounds(valid, b, nan, misc);
+    reduce_scene_bounds(valid, b, nan, misc, partials);
 }
 
 // morton.rs:35-43
@@ -311,7 +339,7 @@ __global__ void __launch_bounds__(T256) k_ploc_count(const int8_t* __restrict__ 
 
 // Single workgroup: exclusive scan of the per-block (emit, merge) counts and the next pass's state.
 __global__ void __launch_bounds__(1024) k_ploc_scan(uint2* __restrict__ blocksums, uint32_t nblocks, Misc* misc,
-                                                    uint32_t slot) {
+                                                    uint32_t slot, uint32_t* __restrict__ hist) {
     __shared__ uint32_t se[16], sm[16];
     const uint32_t tid = threadIdx.x;
     const uint32_t chunk = (nblocks + 1023u) / 1024u;
@@ -345,6 +373,7 @@ __global__ void __launch_bounds__(1024) k_ploc_scan(uint2* __restrict__ blocksum
             nx.insert = st.insert - 2u * rm;
             nx.depth = st.depth + 1u;
             nx.cluster_sum = st.cluster_sum + st.count;
+            hist[nx.depth] = nx.insert;
         }
         misc->state[slot ^ 1u] = nx;
     }
@@ -361,7 +390,7 @@ __global__ void __launch_bounds__(1024) k_ploc_scan(uint2* __restrict__ blocksum
 __global__ void __launch_bounds__(T256) k_ploc_compact(const Node32* __restrict__ cur, Node32* __restrict__ next,
                                                        const int8_t* __restrict__ merge, Misc* misc, uint32_t slot,
                                                        const uint2* __restrict__ offsets,
-                                                       Node32* __restrict__ bvh_nodes) {
+                                                       Node32* __restrict__ bvh_nodes, DpView dp, uint32_t fused) {
     __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
     const PlocState st = misc->state[slot];
     const uint32_t count = st.count;
@@ -391,7 +420,12 @@ __global__ void __launch_bounds__(T256) k_ploc_compact(const Node32* __restrict_
         const uint32_t first = st.insert - 2u * (s + 1u);
         store_node(&bvh_nodes[first], a);
         store_node(&bvh_nodes[first + 1], c);
-        store_node(&next[pos], make_node(box_union(node_box(a), node_box(c)), 0u, first));
+        const Box u = box_union(node_box(a), node_box(c));
+        store_node(&next[pos], make_node(u, 0u, first));
+        if (fused) {
+            const uint32_t e = dp_evaluate(dp, u, first, a, c);
+            if (e) atomicOr(&misc->err, e);
+        }
     } else {
         store_node(&next[pos], a);
     }
@@ -401,7 +435,8 @@ __global__ void __launch_bounds__(T256) k_ploc_compact(const Node32* __restrict_
 __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __restrict__ cur, Misc* misc,
                                                             uint32_t slot, uint32_t first_pass, uint32_t r,
                                                             uint32_t sd_is_one, uint32_t depth_threshold,
-                                                            Node32* __restrict__ bvh_nodes) {
+                                                            Node32* __restrict__ bvh_nodes,
+                                                            uint32_t* __restrict__ hist, DpView dp, uint32_t fused) {
     __shared__ float lx[TAIL_MAX], ly[TAIL_MAX], lz[TAIL_MAX], hx[TAIL_MAX], hy[TAIL_MAX], hz[TAIL_MAX];
     __shared__ uint32_t pc[TAIL_MAX], fi[TAIL_MAX];
     __shared__ int8_t mg[TAIL_MAX];
@@ -431,6 +466,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
     unsigned long long csum = st.cluster_sum;
     uint32_t pass = first_pass;
     bool self_merge = false;
+    uint32_t dp_err = 0u;
     while (count > 1) {
         csum += count;
         const bool dist1 = sd_is_one || pass < depth_threshold;
@@ -522,9 +558,12 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
             uint32_t opc = vpc[q], ofi = vfi[q];
             if (mrg[q]) {
                 const uint32_t first = insert - 2u * (pm + 1u);
-                store_node(&bvh_nodes[first], make_node(va[q], vpc[q], vfi[q]));
-                store_node(&bvh_nodes[first + 1], make_node(vb[q], vbpc[q], vbfi[q]));
+                const Node32 lnode = make_node(va[q], vpc[q], vfi[q]);
+                const Node32 rnode = make_node(vb[q], vbpc[q], vbfi[q]);
+                store_node(&bvh_nodes[first], lnode);
+                store_node(&bvh_nodes[first + 1], rnode);
                 outb = box_union(va[q], vb[q]);
+                if (fused) dp_err |= dp_evaluate(dp, outb, first, lnode, rnode);
                 opc = 0u;
                 ofi = first;
                 pm++;
@@ -544,6 +583,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
         insert -= 2u * (total >> 16);
         depth += 1u;
         pass += 1u;
+        if (tid == 0) hist[depth] = insert;
         if ((total >> 16) == 0u) {
             // No mutual pair in this pass: the reference would loop forever; report instead of hanging.
             self_merge = true;
@@ -551,6 +591,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
         }
     }
     if (self_merge) atomicOr(&misc->err, DERR_SELF_MERGE);
+    if (dp_err) atomicOr(&misc->err, dp_err);
     if (tid == 0) {
         store_node(&bvh_nodes[0], make_node(Box{lx[0], ly[0], lz[0], hx[0], hy[0], hz[0]}, pc[0], fi[0]));
         PlocState out{count, insert, depth, 0u, csum};
@@ -588,7 +629,7 @@ void DevBuf::release() {
 
 obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
                         const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
-                        uint32_t search_depth_threshold, obvhs_bvh2* bvh) {
+                        uint32_t search_depth_threshold, uint32_t fused_max_prims, obvhs_bvh2* bvh) {
     if (!(search_distance == 1 || search_distance == 2 || search_distance == 6 || search_distance == 14 ||
           search_distance == 24 || search_distance == 32))
         return OBVHS_ERR_INVALID_ARG;  // ploc/mod.rs:550-561
@@ -598,8 +639,16 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
     bvh->max_depth = 96;
     bvh->ploc_iterations = 0;
     bvh->cluster_sum = 0;
+    bvh->dp_max_prims = 0;
     OBVHS_TRY(bvh->prim_indices.ensure((size_t)n * 4 + 4), "alloc primitive_indices");
     if (n == 0) return OBVHS_OK;
+    OBVHS_TRY(bvh->hist.ensure((size_t)(n + 1) * 4), "alloc pass history");
+    DpView dp{};
+    if (fused_max_prims) {
+        OBVHS_TRY(bvh_dp_alloc(bvh), "alloc cost table");
+        dp = bvh_dp_view(bvh, fused_max_prims);
+    }
+    const uint32_t fused = fused_max_prims ? 1u : 0u;
     OBVHS_TRY(bvh->nodes.ensure((size_t)bvh->node_count * sizeof(Node32)), "alloc bvh2 nodes");
     BuildScratch& S = ctx->scratch;
     OBVHS_TRY(S.leaves.ensure((size_t)n * sizeof(Node32)), "alloc leaves");
@@ -609,7 +658,7 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
     OBVHS_TRY(S.vals.ensure((size_t)n * 4), "alloc vals");
     OBVHS_TRY(S.merge.ensure((size_t)n + 64), "alloc merge");
     const uint32_t nblk = (n + T256 - 1) / T256;
-    OBVHS_TRY(S.blocksums.ensure((size_t)nblk * sizeof(uint2) + 64), "alloc blocksums");
+    OBVHS_TRY(S.blocksums.ensure((size_t)nblk * sizeof(uint2) + 64 + (size_t)nblk * 6 * 4), "alloc blocksums");
     OBVHS_TRY(S.sort_scratch.ensure(radix_sort_scratch_bytes(n)), "alloc sort scratch");
     OBVHS_TRY(S.misc.ensure(sizeof(Misc)), "alloc misc");
     Misc* misc = S.misc.as<Misc>();
@@ -617,20 +666,24 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
     Node32* bvh_nodes = bvh->nodes.as<Node32>();
     uint64_t* keys = S.keys.as<uint64_t>();
     uint32_t* vals = S.vals.as<uint32_t>();
+    uint32_t* hist = bvh->hist.as<uint32_t>();
+    // per-block bound partials share the blocksums allocation (they are dead before the first pass)
+    uint32_t* partials = reinterpret_cast<uint32_t*>(S.blocksums.as<char>() + (size_t)nblk * sizeof(uint2) + 64);
 
     // primitive_indices = indices (Bvh2::reset_for_reuse, bvh2/mod.rs:106-121)
     if (d_indices) {
         OBVHS_TRY(hipMemcpyAsync(bvh->prim_indices.ptr, d_indices, (size_t)n * 4, hipMemcpyDeviceToDevice, st),
                   "copy indices");
     }
-    k_init_misc<<<1, 64, 0, st>>>(misc, n);
+    k_init_misc<<<1, 64, 0, st>>>(misc, n, hist);
     if (d_tris) {
         k_leaf_init_tris<<<nblk, T256, 0, st>>>(reinterpret_cast<const float4*>(d_tris), d_indices, n, leaves,
-                                                misc);
+                                                misc, partials);
     } else {
         k_leaf_init_aabbs<<<nblk, T256, 0, st>>>(reinterpret_cast<const float4*>(d_aabbs), d_indices, n, leaves,
-                                                 misc);
+                                                 misc, partials);
     }
+    k_reduce_bounds<<<1, 1024, 0, st>>>(partials, nblk, misc);
     k_morton<<<nblk, T256, 0, st>>>(leaves, n, misc, keys);
     OBVHS_TRY(hipGetLastError(), "leaf/morton launch");
     if (ctx->keep_trace) {
@@ -661,8 +714,8 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
             const uint32_t dist1 = (r == 1 || pass < search_depth_threshold) ? 1u : 0u;
             k_ploc_search<<<blocks, T256, 0, st>>>(cur, misc, slot, r, dist1, merge);
             k_ploc_count<<<blocks, T256, 0, st>>>(merge, misc, slot, blocksums);
-            k_ploc_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, slot);
-            k_ploc_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, bvh_nodes);
+            k_ploc_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, slot, hist);
+            k_ploc_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, bvh_nodes, dp, fused);
             std::swap(cur, nxt);
             slot ^= 1u;
             pass++;
@@ -681,7 +734,7 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
         pass = hstate->depth;
     }
     k_ploc_tail<<<1, TAIL_THREADS, 0, st>>>(cur, misc, slot, pass, r, r == 1 ? 1u : 0u, search_depth_threshold,
-                                            bvh_nodes);
+                                            bvh_nodes, hist, dp, fused);
     OBVHS_TRY(hipGetLastError(), "ploc tail launch");
     uint32_t* herr = reinterpret_cast<uint32_t*>(hstate + 1);
     OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
@@ -694,6 +747,7 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
     bvh->ploc_iterations = hstate->depth;
     bvh->cluster_sum = hstate->cluster_sum;
     bvh->max_depth = hstate->depth + 1 > 96 ? hstate->depth + 1 : 96;  // ploc/mod.rs:501
+    bvh->dp_max_prims = fused_max_prims;
     return OBVHS_OK;
 }
 

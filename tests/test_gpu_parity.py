@@ -103,3 +103,57 @@ def test_empty_build(ctx):
     rays = ob.make_rays([[0, 0, 1]], [[0, 0, -1]])
     hits = cw.ray_traverse(rays, np.zeros(1, dtype=ob.TRIANGLE))
     assert hits["primitive_id"][0] == ob.INVALID_ID and np.isinf(hits["t"][0])
+
+
+@pytest.mark.parametrize("n,sd,sdt,mp", [(1, 14, 0, 3), (2, 14, 0, 3), (3, 1, 0, 1), (100, 6, 2, 2), (5000, 14, 3, 3),
+                                         (40000, 14, 0, 3), (40000, 32, 0, 1), (70000, 2, 0, 3), (70000, 1, 0, 2)])
+def test_build_cwbvh_from_tris_fused(ctx, n, sd, sdt, mp):
+    """build_cwbvh_from_tris evaluates the conversion's cost table inside the PLOC merges; same bytes."""
+    print(f"[case] fused build n={n} sd={sd} sdt={sdt} mp={mp}", flush=True)
+    tris = scenes.soup(n, seed=n + sd)
+    params = ob.BvhBuildParams(ploc_search_distance=sd, search_depth_threshold=sdt, max_prims_per_leaf=mp)
+    cw = ob.build_cwbvh_from_tris(ctx, tris, params)
+    _, rcw, rprims = O.build_cwbvh(tris, sd, sdt, mp)
+    assert_same_bytes(cw.nodes, rcw, "CWBVH nodes (fused)")
+    np.testing.assert_array_equal(cw.primitive_indices, rprims)
+
+
+def test_build_reuse_handles(ctx):
+    """build_with_bvh-style reuse (ploc/mod.rs:129-159): a handle rebuilt with other inputs equals a fresh build."""
+    params = ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3)
+    cw = None
+    for n, seed in ((30000, 1), (9000, 2), (50000, 3)):
+        print(f"[case] reuse n={n}", flush=True)
+        tris = scenes.soup(n, seed=seed)
+        cw = ob.build_cwbvh_from_tris(ctx, tris, params, reuse=cw)
+        _, rcw, rprims = O.build_cwbvh(tris, 14, 0, 3)
+        assert_same_bytes(cw.nodes, rcw, "CWBVH nodes (reuse)")
+        np.testing.assert_array_equal(cw.primitive_indices, rprims)
+
+
+def test_conversion_of_same_bvh2_with_other_leaf_sizes(ctx):
+    """One BVH2 converted with max_prims_per_leaf 3, then 1, then 3 again (cost table re-evaluated each time)."""
+    tris = scenes.soup(25000, seed=9)
+    bvh2 = ob.PlocBuilder(ctx).build(14, tris)
+    ref = O.ploc_build(O.tri_aabbs(tris), 14, 0)
+    for mp in (3, 1, 3, 2):
+        print(f"[case] reconvert mp={mp}", flush=True)
+        cw = ob.bvh2_to_cwbvh(bvh2, mp, True, False)
+        rcw, rprims, _ = O.bvh2_to_cwbvh(ref["nodes"], ref["prim_indices"], mp, True)
+        assert_same_bytes(cw.nodes, rcw, f"CWBVH nodes mp={mp}")
+
+
+def test_full_heightfield_parity(ctx):
+    """The benchmark's scene at full size: 1,288,352 triangles, bit-identical CWBVH; a strided ray sample traced."""
+    tris, rays = scenes.heightfield_workload()
+    print(f"[case] full heightfield n={len(tris)}", flush=True)
+    params = ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3)
+    cw = ob.build_cwbvh_from_tris(ctx, tris, params)
+    b, rcw, rprims = O.build_cwbvh(tris, 14, 0, 3)
+    assert_same_bytes(cw.nodes, rcw, "CWBVH nodes")
+    np.testing.assert_array_equal(cw.primitive_indices, rprims)
+    sample = rays[::7]
+    bvh_tris = cw.reorder_triangles(tris)
+    hits = cw.ray_traverse(sample, bvh_tris)
+    ref = O.traverse(rcw, tris[rprims], sample, 0, threads=16)
+    assert_same_bytes(hits, ref, "RayHit heightfield sample")
