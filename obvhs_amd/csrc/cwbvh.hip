@@ -380,7 +380,7 @@ hipError_t upload_log2_table() {
 
 hipError_t bvh_dp_alloc(obvhs_bvh2* bvh) {
     const size_t k = bvh->prim_count > 1 ? bvh->prim_count - 1 : 1;
-    return bvh->dp.ensure(k * (28 + 8 + 28 + 4 + 4) + 1024);
+    return bvh->dp.ensure(k * (28 + 8 + 28 + 4 + 4 + 8 + 4) + 1024);
 }
 
 DpView bvh_dp_view(const obvhs_bvh2* bvh, uint32_t max_prims) {
@@ -392,6 +392,8 @@ DpView bvh_dp_view(const obvhs_bvh2* bvh, uint32_t max_prims) {
     v.sub = reinterpret_cast<uint32_t*>(base + k * 36);
     v.nprims = reinterpret_cast<uint32_t*>(base + k * 64);
     v.cwsize = reinterpret_cast<uint32_t*>(base + k * 68);
+    v.subdepth = reinterpret_cast<uint2*>(base + k * 72);
+    v.cwdepth = reinterpret_cast<uint32_t*>(base + k * 80);
     v.max_prims = max_prims;
     return v;
 }
@@ -439,40 +441,46 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
         bvh2->dp_max_prims = max_prims_per_leaf;
     }
     const DpView dp = bvh_dp_view(bvh2, max_prims_per_leaf);
-    // Root record, CWBVH node count (the root's subtree size) and device errors.
+    // Root record, CWBVH node count and depth (the root's subtree size and levels) and device errors.
     Node32 root;
     OBVHS_TRY(hipMemcpyAsync(h, nodes, sizeof(Node32), hipMemcpyDeviceToHost, st), "read root");
     OBVHS_TRY(hipStreamSynchronize(st), "root sync");
     memcpy(&root, h, sizeof(Node32));
-    uint32_t ncw = 1;
+    uint32_t ncw = 1, levels = 1;
     if (root.prim_count == 0u) {
         OBVHS_TRY(hipMemcpyAsync(h, dp.cwsize + dkey(root.first_index), 4, hipMemcpyDeviceToHost, st),
                   "read cw size");
+        OBVHS_TRY(hipMemcpyAsync(h + 2, dp.cwdepth + dkey(root.first_index), 4, hipMemcpyDeviceToHost, st),
+                  "read cw depth");
     }
     OBVHS_TRY(hipMemcpyAsync(h + 1, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cw err");
     OBVHS_TRY(hipStreamSynchronize(st), "cwbvh dp sync");
     if (h[1]) return device_error_to_status(h[1]);
-    if (root.prim_count == 0u) ncw = h[0];
+    if (root.prim_count == 0u) {
+        ncw = h[0];
+        levels = h[2];
+    }
     if (ncw == 0u) return OBVHS_ERR_CW_DISTRIBUTE;  // some CWBVH node's children cannot be distributed
     if (ncw > N) return OBVHS_ERR_CAPACITY;
+    if (levels > MAX_CW_LEVELS) return OBVHS_ERR_DEPTH;
     OBVHS_TRY(out->nodes.ensure((size_t)ncw * sizeof(CwNode)), "alloc cwbvh nodes");
     OBVHS_TRY(out->prim_indices.ensure((size_t)n * 4 + 4), "alloc cwbvh primitive indices");
-    uint32_t level_count = 1;
-    uint32_t level = 0;
-    while (level_count > 0) {
-        if (level >= MAX_CW_LEVELS) return OBVHS_ERR_DEPTH;
-        const uint32_t g = (level_count + T256 - 1) / T256;
+    // One launch per level, no host round trips: a level holds at most min(8^level, ncw) nodes.
+    uint64_t bound = 1;
+    for (uint32_t level = 0; level < levels; ++level) {
+        const uint32_t cap = (uint32_t)(bound < ncw ? bound : ncw);
+        const uint32_t g = (cap + T256 - 1) / T256;
         k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2->prim_indices.as<uint32_t>(), dp, items[level & 1],
                                       items[(level + 1) & 1], level, order_children ? 1u : 0u,
                                       out->nodes.as<CwNode>(), out->prim_indices.as<uint32_t>(), ncw, n, misc);
-        OBVHS_TRY(hipGetLastError(), "cwbvh emit launch");
-        OBVHS_TRY(hipMemcpyAsync(h, &misc->level_count[level + 1], 4, hipMemcpyDeviceToHost, st), "read level");
-        OBVHS_TRY(hipMemcpyAsync(h + 1, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cw err");
-        OBVHS_TRY(hipStreamSynchronize(st), "cwbvh level sync");
-        if (h[1]) return device_error_to_status(h[1]);
-        level_count = h[0];
-        level++;
+        bound *= 8;
     }
+    OBVHS_TRY(hipGetLastError(), "cwbvh emit launch");
+    OBVHS_TRY(hipMemcpyAsync(h, &misc->level_count[levels], 4, hipMemcpyDeviceToHost, st), "read level");
+    OBVHS_TRY(hipMemcpyAsync(h + 1, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cw err");
+    OBVHS_TRY(hipStreamSynchronize(st), "cwbvh emit sync");
+    if (h[1]) return device_error_to_status(h[1]);
+    if (h[0] != 0u) return OBVHS_ERR_DEPTH;  // deeper than the depth table said (cannot happen)
     // total_aabb = bvh2.nodes[0].aabb (bvh2_to_cwbvh.rs:506)
     out->total_aabb.min = obvhs_vec3a{root.minx, root.miny, root.minz, 0.0f};
     out->total_aabb.max = obvhs_vec3a{root.maxx, root.maxy, root.maxz, 0.0f};

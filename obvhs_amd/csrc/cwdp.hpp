@@ -12,7 +12,9 @@
 // decision i, sub[i] = the number of CWBVH nodes in the subtrees of the INTERNAL children that get_children(node, i)
 // would produce.  It obeys the same recurrence as the costs (children only), so no tree walk is needed, and
 // cwsize = 1 + sub[0] is the size of the CWBVH subtree the node roots if it becomes a CWBVH node -- what the
-// top-down pass needs to number nodes in the reference's pre-order (bvh2_to_cwbvh.rs:172-191).
+// top-down pass needs to number nodes in the reference's pre-order (bvh2_to_cwbvh.rs:172-191).  The same
+// recurrence with max instead of + gives the depth of that CWBVH subtree, so the top-down pass knows its level
+// count without reading anything back.
 #pragma once
 
 #include "common.hpp"
@@ -31,6 +33,8 @@ struct DpView {
     uint32_t* sub;      // [k][7]
     uint32_t* nprims;   // [k] primitives under the node
     uint32_t* cwsize;   // [k] 1 + sub[0], or 0 when the node's CWBVH subtree cannot be built
+    uint2* subdepth;    // [k] seven bytes: CWBVH depth below the children get_children(node, i) produces
+    uint32_t* cwdepth;  // [k] 1 + subdepth[0]: levels of the CWBVH subtree the node roots
     uint32_t max_prims; // max_prims_per_leaf the table was evaluated for
 };
 
@@ -57,8 +61,10 @@ struct ChildDp {
     float cost[7];
     uint2 meta;       // seven decision bytes
     size_t key;
+    uint2 subdepth;
     uint32_t nprims;
     uint32_t cwsize;
+    uint32_t cwdepth;
     bool leaf;
 };
 
@@ -69,16 +75,20 @@ __device__ __forceinline__ void load_child_dp(const DpView& dp, const Node32& c,
 #pragma unroll
         for (int k = 0; k < 7; ++k) o.cost[k] = leaf;
         o.meta = make_uint2(0u, 0u);
+        o.subdepth = make_uint2(0u, 0u);
         o.key = 0;
         o.nprims = c.prim_count;
         o.cwsize = 1u;
+        o.cwdepth = 1u;
     } else {
         o.key = dkey(c.first_index);
 #pragma unroll
         for (int k = 0; k < 7; ++k) o.cost[k] = dp.cost[o.key * 7 + k];
         o.meta = dp.meta[o.key];
+        o.subdepth = dp.subdepth[o.key];
         o.nprims = dp.nprims[o.key];
         o.cwsize = dp.cwsize[o.key];
+        o.cwdepth = dp.cwdepth[o.key];
     }
 }
 
@@ -88,6 +98,14 @@ __device__ __forceinline__ uint32_t child_contribution(const DpView& dp, const C
     if (c.leaf) return 0u;
     if (dkind(dbyte(c.meta, d)) == KIND_DISTRIBUTE) return dp.sub[c.key * 7 + d];
     if (dkind(dbyte(c.meta, 0u)) == KIND_INTERNAL) return c.cwsize == 0u ? SUB_INVALID : c.cwsize;
+    return 0u;
+}
+
+// Depth counterpart of child_contribution.
+__device__ __forceinline__ uint32_t child_depth(const ChildDp& c, uint32_t d) {
+    if (c.leaf) return 0u;
+    if (dkind(dbyte(c.meta, d)) == KIND_DISTRIBUTE) return dbyte(c.subdepth, d);
+    if (dkind(dbyte(c.meta, 0u)) == KIND_INTERNAL) return c.cwdepth;
     return 0u;
 }
 
@@ -148,15 +166,17 @@ __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box
             meta[i] = meta[i - 1];
         }
     }
-    uint32_t sub[7];
+    uint32_t sub[7], sd[7];
 #pragma unroll
     for (int i = 0; i < 7; ++i) {
         const uint32_t dl = dleft(meta[i]), dr = dright(meta[i]);
         if (dl == DIDX_INVALID || dr == DIDX_INVALID) {
             sub[i] = SUB_INVALID;
+            sd[i] = 0u;
         } else {
             const uint32_t a = child_contribution(dp, L, dl), b = child_contribution(dp, R, dr);
             sub[i] = (a == SUB_INVALID || b == SUB_INVALID) ? SUB_INVALID : a + b;
+            sd[i] = min(255u, max(child_depth(L, dl), child_depth(R, dr)));
         }
     }
     const size_t key = dkey(first);
@@ -167,8 +187,10 @@ __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box
     }
     dp.meta[key] = make_uint2(meta[0] | meta[1] << 8 | meta[2] << 16 | meta[3] << 24,
                               meta[4] | meta[5] << 8 | meta[6] << 16);
+    dp.subdepth[key] = make_uint2(sd[0] | sd[1] << 8 | sd[2] << 16 | sd[3] << 24, sd[4] | sd[5] << 8 | sd[6] << 16);
     dp.nprims[key] = num_primitives;
     dp.cwsize[key] = sub[0] == SUB_INVALID ? 0u : 1u + sub[0];
+    dp.cwdepth[key] = 1u + sd[0];
     return 0u;
 }
 

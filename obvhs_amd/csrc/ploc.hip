@@ -440,6 +440,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
     __shared__ float lx[TAIL_MAX], ly[TAIL_MAX], lz[TAIL_MAX], hx[TAIL_MAX], hy[TAIL_MAX], hz[TAIL_MAX];
     __shared__ uint32_t pc[TAIL_MAX], fi[TAIL_MAX];
     __shared__ int8_t mg[TAIL_MAX];
+    __shared__ uint32_t mlist[TAIL_MAX / 2];  // first child slot of every merge of the pass (cost table work list)
     __shared__ uint32_t wsum[TAIL_THREADS / 64];
     __shared__ uint32_t s_total;
     const uint32_t tid = threadIdx.x;
@@ -563,7 +564,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
                 store_node(&bvh_nodes[first], lnode);
                 store_node(&bvh_nodes[first + 1], rnode);
                 outb = box_union(va[q], vb[q]);
-                if (fused) dp_err |= dp_evaluate(dp, outb, first, lnode, rnode);
+                if (fused) mlist[pm] = first;
                 opc = 0u;
                 ofi = first;
                 pm++;
@@ -579,6 +580,14 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
             pe++;
         }
         __syncthreads();
+        if (fused) {
+            // Cost-table entries of this pass's parents, one thread each; their children are finished.
+            for (uint32_t j = tid; j < (total >> 16); j += TAIL_THREADS) {
+                const uint32_t first = mlist[j];
+                const Node32 l = load_node(&bvh_nodes[first]), r = load_node(&bvh_nodes[first + 1]);
+                dp_err |= dp_evaluate(dp, box_union(node_box(l), node_box(r)), first, l, r);
+            }
+        }
         count = total & 0xffffu;
         insert -= 2u * (total >> 16);
         depth += 1u;
