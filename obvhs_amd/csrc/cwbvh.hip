@@ -145,17 +145,13 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     if (order_children) {
         const float px = (nb.hx + nb.lx) * 0.5f, py = (nb.hy + nb.ly) * 0.5f, pz = (nb.hz + nb.lz) * 0.5f;
         const float FMAX = 3.40282347e+38f;
-        float cost[8][8];
+        // Child centre offsets; the 8x8 cost table is evaluated on the fly (d . v with d in {+-1}^3 is exact).
+        float vx[8], vy[8], vz[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const float vx = (cb[c].hx + cb[c].lx) * 0.5f - px;
-            const float vy = (cb[c].hy + cb[c].ly) * 0.5f - py;
-            const float vz = (cb[c].hz + cb[c].lz) * 0.5f - pz;
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const float dx = (s & 4) ? -1.0f : 1.0f, dy = (s & 2) ? -1.0f : 1.0f, dz = (s & 1) ? -1.0f : 1.0f;
-                cost[c][s] = c < cc ? (dx * vx + dy * vy) + dz * vz : FMAX;
-            }
+            vx[c] = (cb[c].hx + cb[c].lx) * 0.5f - px;
+            vy[c] = (cb[c].hy + cb[c].ly) * 0.5f - py;
+            vz[c] = (cb[c].hz + cb[c].lz) * 0.5f - pz;
         }
         uint32_t assigned = 0u, filled = 0u;
 #pragma unroll
@@ -168,7 +164,9 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
                 if (c < cc && !((assigned >> c) & 1u)) {
 #pragma unroll
                     for (int s = 0; s < 8; ++s) {
-                        const float v = cost[c][s];
+                        const float dx = (s & 4) ? -1.0f : 1.0f, dy = (s & 2) ? -1.0f : 1.0f,
+                                    dz = (s & 1) ? -1.0f : 1.0f;
+                        const float v = (dx * vx[c] + dy * vy[c]) + dz * vz[c];
                         if (!((filled >> s) & 1u) && v < min_cost) {
                             min_cost = v;
                             min_slot = s;
