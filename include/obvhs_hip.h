@@ -79,6 +79,7 @@ typedef struct obvhs_build_params {
 typedef struct obvhs_context obvhs_context;
 typedef struct obvhs_bvh2 obvhs_bvh2;
 typedef struct obvhs_cwbvh obvhs_cwbvh;
+typedef struct obvhs_batch obvhs_batch;
 
 /* Per-build statistics (the reference's `scope!` stage names, lib.rs:197-205). */
 typedef struct obvhs_build_stats {
@@ -171,6 +172,34 @@ obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cw
                                               const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays, uint32_t n,
                                               obvhs_ray_hit* d_hits, uint32_t* d_node_visits,
                                               uint32_t* d_tri_tests);
+
+/* ---- batched build / traversal (many independent meshes in shared launches) ----
+ * No single reference entry point: equivalent to calling build_cwbvh_from_tris (cwbvh/builder.rs:20-85) once per
+ * mesh, with identical per-mesh results.  d_tris: concatenated triangles (device); tri_offsets: host array of
+ * mesh_count + 1 prefix offsets (tri_offsets[0] == 0).  Pre-splits and reinsertion are not supported. */
+obvhs_status obvhs_batch_build_cwbvh(obvhs_context* ctx, const obvhs_triangle* d_tris, const uint32_t* tri_offsets,
+                                     uint32_t mesh_count, const obvhs_build_params* params, obvhs_batch* reuse,
+                                     obvhs_batch** out);
+obvhs_status obvhs_batch_info(const obvhs_batch* b, uint32_t* mesh_count, uint32_t* total_nodes,
+                              uint32_t* total_prims, uint32_t* ploc_passes);
+/* Mesh i's CwBvh::nodes are nodes[node_offset .. node_offset + node_count) of obvhs_batch_copy_nodes, its
+ * primitive_indices (mesh-local) are prims[prim_offset .. prim_offset + prim_count). */
+obvhs_status obvhs_batch_mesh_info(const obvhs_batch* b, uint32_t mesh, uint32_t* node_offset, uint32_t* node_count,
+                                   uint32_t* prim_offset, uint32_t* prim_count);
+obvhs_status obvhs_batch_copy_nodes(obvhs_context* ctx, const obvhs_batch* b, obvhs_cwbvh_node* host_out);
+obvhs_status obvhs_batch_copy_primitive_indices(obvhs_context* ctx, const obvhs_batch* b, uint32_t* host_out);
+/* Per mesh, bvh_tris[prim_offset + i] = tris[prim_offset + primitive_indices[i]] (examples/obj_cwbvh.rs:63-67). */
+obvhs_status obvhs_batch_reorder_triangles(obvhs_context* ctx, const obvhs_batch* b, const obvhs_triangle* d_tris,
+                                           obvhs_triangle* d_bvh_tris);
+/* CwBvh::ray_traverse / ray_traverse_miss (cwbvh/mod.rs:169-225) for rays of many meshes in one launch: ray i is
+ * traced against mesh d_ray_mesh[i]; hits' primitive_id indexes that mesh's primitive_indices. */
+obvhs_status obvhs_batch_ray_traverse(obvhs_context* ctx, const obvhs_batch* b, const obvhs_triangle* d_bvh_tris,
+                                      const obvhs_ray* d_rays, const uint32_t* d_ray_mesh, uint32_t n,
+                                      obvhs_ray_hit* d_hits);
+obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch* b,
+                                           const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
+                                           const uint32_t* d_ray_mesh, uint32_t n, uint8_t* d_miss);
+obvhs_status obvhs_batch_destroy(obvhs_batch* b);
 
 /* Host copy of the CWBVH exponent table: entry E is the largest 23-bit mantissa M with
  * log2f(2^(E-127) * (1 + M/2^23)) == E - 127 under the host libm, i.e. the inputs whose

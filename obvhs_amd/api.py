@@ -386,11 +386,92 @@ def build_cwbvh_from_tris(ctx, triangles, config, reuse=None):
     return CwBvh(ctx, out, stats)
 
 
+class CwBvhBatch:
+    """Many CwBvhs built together (one per mesh), each identical to build_cwbvh_from_tris on that mesh alone."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def info(self):
+        m, nodes, prims, passes = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().obvhs_batch_info(self.handle, C.byref(m), C.byref(nodes), C.byref(prims), C.byref(passes)))
+        return dict(mesh_count=m.value, total_nodes=nodes.value, total_prims=prims.value, ploc_passes=passes.value)
+
+    def mesh_info(self, i):
+        vals = [C.c_uint32() for _ in range(4)]
+        check(lib().obvhs_batch_mesh_info(self.handle, int(i), *[C.byref(v) for v in vals]))
+        return tuple(v.value for v in vals)  # node_offset, node_count, prim_offset, prim_count
+
+    def all_nodes(self):
+        out = np.zeros(self.info()["total_nodes"], dtype=CWBVH_NODE)
+        check(lib().obvhs_batch_copy_nodes(self.ctx.handle, self.handle, out.ctypes.data_as(C.c_void_p)),
+              self.ctx.handle)
+        return out
+
+    def all_primitive_indices(self):
+        out = np.zeros(self.info()["total_prims"], dtype=np.uint32)
+        check(lib().obvhs_batch_copy_primitive_indices(self.ctx.handle, self.handle,
+                                                       out.ctypes.data_as(C.c_void_p)), self.ctx.handle)
+        return out
+
+    def reorder_triangles(self, tris):
+        d, n = _as_device(self.ctx, tris, TRIANGLE)
+        out = DeviceArray(self.ctx, TRIANGLE, n)
+        check(lib().obvhs_batch_reorder_triangles(self.ctx.handle, self.handle, d.ptr, out.ptr), self.ctx.handle)
+        return out
+
+    def ray_traverse(self, rays, ray_mesh, bvh_tris):
+        dr, n = _as_device(self.ctx, rays, RAY)
+        dm, _ = _as_device(self.ctx, ray_mesh, np.uint32)
+        dt, _ = _as_device(self.ctx, bvh_tris, TRIANGLE)
+        hits = DeviceArray(self.ctx, RAY_HIT, n)
+        check(lib().obvhs_batch_ray_traverse(self.ctx.handle, self.handle, dt.ptr, dr.ptr, dm.ptr, n, hits.ptr),
+              self.ctx.handle)
+        return hits.to_host()
+
+    def ray_traverse_miss(self, rays, ray_mesh, bvh_tris):
+        dr, n = _as_device(self.ctx, rays, RAY)
+        dm, _ = _as_device(self.ctx, ray_mesh, np.uint32)
+        dt, _ = _as_device(self.ctx, bvh_tris, TRIANGLE)
+        miss = DeviceArray(self.ctx, np.uint8, n)
+        check(lib().obvhs_batch_ray_traverse_miss(self.ctx.handle, self.handle, dt.ptr, dr.ptr, dm.ptr, n,
+                                                  miss.ptr), self.ctx.handle)
+        return miss.to_host().astype(bool)
+
+    def close(self):
+        if self.handle:
+            lib().obvhs_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_cwbvh_batch(ctx, triangles, offsets, config, reuse=None):
+    """Builds one CwBvh per mesh; mesh i is triangles[offsets[i]:offsets[i+1]] (concatenated Triangle array)."""
+    d, n = _as_device(ctx, triangles, TRIANGLE)
+    offs = np.ascontiguousarray(offsets, dtype=np.uint32)
+    if offs[0] != 0 or offs[-1] != n:
+        raise ValueError("offsets must start at 0 and end at the triangle count")
+    out = C.c_void_p()
+    params = config.to_c()
+    check(lib().obvhs_batch_build_cwbvh(ctx.handle, d.ptr, offs.ctypes.data_as(C.c_void_p), len(offs) - 1,
+                                        C.byref(params), reuse.handle if reuse is not None else None,
+                                        C.byref(out)), ctx.handle)
+    if reuse is not None:
+        return reuse
+    return CwBvhBatch(ctx, out)
+
+
 def log2_threshold_table():
     out = np.zeros(256, dtype=np.uint32)
     check(lib().obvhs_log2_threshold_table(out.ctypes.data_as(C.c_void_p)))
     return out
 
 
-__all__ = ["PlocSearchDistance", "SortPrecision", "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder",
+__all__ = ["CwBvhBatch", "build_cwbvh_batch", "PlocSearchDistance", "SortPrecision", "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder",
            "CwBvh", "bvh2_to_cwbvh", "build_cwbvh_from_tris", "log2_threshold_table", "ObvhsError"]

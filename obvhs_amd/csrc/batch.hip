@@ -1,0 +1,695 @@
+// Batched build and traversal: many independent meshes of uneven size in shared launches.
+//
+// Every mesh gets exactly the CWBVH build_cwbvh_from_tris would give it alone (cwbvh/builder.rs:61-74): the
+// kernels are the single-mesh ones (ploc.hip, cwbvh.hip) made segment-aware.
+//   * leaf init / Morton: one pass over the concatenated triangles; each triangle finds its mesh by binary search
+//     over the offsets; per-mesh scene bounds by wave-reduced atomics (ploc/mod.rs:187-243, 287-288, 782-809).
+//   * sort: stable radix sort by Morton code, then stable radix sort by mesh id -- every mesh's leaves end up in
+//     its own segment in (code, leaf index) order, exactly the single-mesh order (ploc/mod.rs:811-846).
+//   * PLOC passes over all segments at once: the neighbour window is clipped to the cluster's segment
+//     (ploc/mod.rs:508-528), prefix sums run over the whole array, and a per-mesh pass computes each segment's
+//     new start, count, insert index and merge base so that child slots are insert_m - 2(s_m + 1) with s_m the
+//     merge's rank inside its mesh (ploc/mod.rs:420-487).  Finished meshes keep one cluster that is copied along.
+//     The cost table is evaluated by the merges (cwdp.hpp) with the mesh's key base.
+//   * CWBVH emit: one launch per level over all meshes (cwbvh.hip).
+//   * traversal: one lane per ray, the ray's mesh selects the CWBVH and triangle slices.
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "cwdp.hpp"
+#include "device_util.hpp"
+#include "internal.hpp"
+
+struct obvhs_batch {
+    obvhs::DevBuf tri_offsets;    // u32[m+1]
+    obvhs::DevBuf mesh_table;     // MeshDesc[m]
+    obvhs::DevBuf bvh2_nodes;     // Node32[2 * n_total], mesh i at 2 * offset[i]
+    obvhs::DevBuf bvh2_prims;     // u32[n_total] (mesh-local 0..n_i)
+    obvhs::DevBuf dp;             // cost table, keys prim_base + first_index / 2
+    obvhs::DevBuf cw_nodes;       // CwNode[total]
+    obvhs::DevBuf cw_prims;       // u32[n_total] (mesh-local indices)
+    std::vector<obvhs::MeshDesc> meshes;
+    uint32_t n_total = 0;
+    uint32_t cw_total = 0;
+    uint32_t passes = 0;
+};
+
+namespace obvhs {
+
+namespace {
+
+constexpr int T256 = 256;
+constexpr uint32_t MAX_RADIUS = 32;
+constexpr int PASSES_PER_SYNC = 4;
+
+struct SegState {
+    uint32_t start;   // first cluster of the mesh in the current cluster array
+    uint32_t count;   // clusters of the mesh
+    uint32_t insert;  // next free child slot + 1 (mesh-local), ploc/mod.rs:283-284
+    uint32_t mbase;   // global exclusive merge count at the segment's first cluster (current pass)
+};
+
+struct BatchMisc {
+    uint32_t err;
+    uint32_t total;      // clusters in the current array
+    uint32_t active;     // meshes with more than one cluster after the last pass
+    uint32_t pad;
+    uint32_t totals[2];  // emits, merges of the pass
+};
+
+__device__ __forceinline__ uint32_t mesh_of(const uint32_t* __restrict__ offsets, uint32_t m, uint32_t g) {
+    uint32_t lo = 0, hi = m;  // offsets[lo] <= g < offsets[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (offsets[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(T256) k_b_init(uint32_t* __restrict__ bounds, const uint32_t* __restrict__ offsets,
+                                                 uint32_t m, SegState* __restrict__ seg, BatchMisc* misc,
+                                                 uint32_t n_total) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i == 0) {
+        misc->err = 0u;
+        misc->total = n_total;
+        misc->active = 0u;
+    }
+    if (i >= m) return;
+    for (int a = 0; a < 3; ++a) {
+        bounds[i * 6 + a] = 0xffffffffu;
+        bounds[i * 6 + 3 + a] = 0u;
+    }
+    const uint32_t n = offsets[i + 1] - offsets[i];
+    seg[i] = SegState{offsets[i], n, n == 0 ? 0u : 2u * n - 1u, 0u};
+}
+
+// Leaf boxes (triangle.rs:28-30) and per-mesh bounds from both corners (ploc/mod.rs:189-190).
+__global__ void __launch_bounds__(T256) k_b_leaf_init(const float4* __restrict__ tris,
+                                                      const uint32_t* __restrict__ offsets, uint32_t m,
+                                                      uint32_t n_total, Node32* __restrict__ leaves,
+                                                      uint32_t* __restrict__ bounds, BatchMisc* misc) {
+    const uint32_t g = blockIdx.x * T256 + threadIdx.x;
+    const bool valid = g < n_total;
+    uint32_t mesh = 0xffffffffu;
+    uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    bool nan = false;
+    if (valid) {
+        mesh = mesh_of(offsets, m, g);
+        const float4 v0 = tris[3 * (size_t)g], v1 = tris[3 * (size_t)g + 1], v2 = tris[3 * (size_t)g + 2];
+        Box b{v0.x, v0.y, v0.z, v0.x, v0.y, v0.z};
+        b = Box{smin(b.lx, v1.x), smin(b.ly, v1.y), smin(b.lz, v1.z), smax(b.hx, v1.x), smax(b.hy, v1.y),
+                smax(b.hz, v1.z)};
+        b = Box{smin(b.lx, v2.x), smin(b.ly, v2.y), smin(b.lz, v2.z), smax(b.hx, v2.x), smax(b.hy, v2.y),
+                smax(b.hz, v2.z)};
+        nan = box_has_nan(b);
+        store_node(&leaves[g], make_node(b, 1u, g - offsets[mesh]));
+        const uint32_t kl[3] = {f32_order_key(b.lx), f32_order_key(b.ly), f32_order_key(b.lz)};
+        const uint32_t kh[3] = {f32_order_key(b.hx), f32_order_key(b.hy), f32_order_key(b.hz)};
+        for (int a = 0; a < 3; ++a) {
+            k[a] = min(kl[a], kh[a]);
+            k[3 + a] = max(kl[a], kh[a]);
+        }
+    }
+    if (__ballot(nan) != 0ull && __lane_id() == 0) atomicOr(&misc->err, DERR_NAN_INPUT);
+    // Wave-reduce when the whole wave lies in one mesh (the common case), else per-lane atomics.
+    const uint32_t m0 = (uint32_t)__shfl((int)mesh, 0);
+    const bool uniform = __ballot(mesh == m0) == __ballot(true);
+    if (uniform) {
+        for (int a = 0; a < 3; ++a) {
+            k[a] = wave_min_u32(k[a]);
+            k[3 + a] = wave_max_u32(k[3 + a]);
+        }
+        if (__lane_id() == 0 && m0 != 0xffffffffu) {
+            for (int a = 0; a < 3; ++a) {
+                atomicMin(&bounds[m0 * 6 + a], k[a]);
+                atomicMax(&bounds[m0 * 6 + 3 + a], k[3 + a]);
+            }
+        }
+    } else if (valid) {
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&bounds[mesh * 6 + a], k[a]);
+            atomicMax(&bounds[mesh * 6 + 3 + a], k[3 + a]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T256) k_b_morton(const Node32* __restrict__ leaves,
+                                                   const uint32_t* __restrict__ offsets, uint32_t m, uint32_t n_total,
+                                                   const uint32_t* __restrict__ bounds, uint64_t* __restrict__ keys) {
+    const uint32_t g = blockIdx.x * T256 + threadIdx.x;
+    if (g >= n_total) return;
+    const uint32_t mesh = mesh_of(offsets, m, g);
+    const uint32_t* bk = bounds + mesh * 6;
+    const float mnx = f32_from_order_key(bk[0]), mny = f32_from_order_key(bk[1]), mnz = f32_from_order_key(bk[2]);
+    const float mxx = f32_from_order_key(bk[3]), mxy = f32_from_order_key(bk[4]), mxz = f32_from_order_key(bk[5]);
+    const double sx = 1.0 / (double)(mxx - mnx), sy = 1.0 / (double)(mxy - mny), sz = 1.0 / (double)(mxz - mnz);
+    const double ox = -(double)mnx * sx, oy = -(double)mny * sy, oz = -(double)mnz * sz;
+    keys[g] = morton_of_box(node_box(load_node(&leaves[g])), sx, sy, sz, ox, oy, oz);
+}
+
+__global__ void __launch_bounds__(T256) k_b_mesh_keys(const uint32_t* __restrict__ vals,
+                                                      const uint32_t* __restrict__ offsets, uint32_t m,
+                                                      uint32_t n_total, uint64_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= n_total) return;
+    keys[i] = mesh_of(offsets, m, vals[i]);
+}
+
+__global__ void __launch_bounds__(T256) k_b_gather(const Node32* __restrict__ leaves,
+                                                   const uint32_t* __restrict__ order,
+                                                   const uint32_t* __restrict__ offsets, uint32_t m, uint32_t n_total,
+                                                   Node32* __restrict__ out, uint32_t* __restrict__ out_mesh) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= n_total) return;
+    const uint32_t g = order[i];
+    store_node(&out[i], load_node(&leaves[g]));
+    out_mesh[i] = mesh_of(offsets, m, g);
+}
+
+// Neighbour pass (ploc/mod.rs:329-419) with windows clipped to the cluster's mesh segment.
+__global__ void __launch_bounds__(T256) k_b_search(const Node32* __restrict__ cur,
+                                                   const uint32_t* __restrict__ cmesh,
+                                                   const SegState* __restrict__ seg, BatchMisc* misc,
+                                                   uint32_t r, uint32_t dist1, int8_t* __restrict__ merge) {
+    __shared__ float s[6][T256 + 2 * MAX_RADIUS];
+    const uint32_t total = misc->total;
+    const uint32_t base = blockIdx.x * T256;
+    if (base >= total) return;
+    const int wstart = (int)base - (int)r;
+    const int wlen = T256 + 2 * (int)r;
+    for (int w = threadIdx.x; w < wlen; w += T256) {
+        const int g = wstart + w;
+        if (g >= 0 && (uint32_t)g < total) {
+            const Node32 nd = load_node(&cur[g]);
+            s[0][w] = nd.minx;
+            s[1][w] = nd.miny;
+            s[2][w] = nd.minz;
+            s[3][w] = nd.maxx;
+            s[4][w] = nd.maxy;
+            s[5][w] = nd.maxz;
+        }
+    }
+    __syncthreads();
+    const uint32_t i = base + threadIdx.x;
+    if (i >= total) return;
+    const SegState sg = seg[cmesh[i]];
+    if (sg.count <= 1u) return;
+    const uint32_t li = i - sg.start;
+    const int lw = (int)threadIdx.x + (int)r;
+    const Box me{s[0][lw], s[1][lw], s[2][lw], s[3][lw], s[4][lw], s[5][lw]};
+    int8_t mv;
+    if (dist1) {
+        if (li == sg.count - 1) {
+            mv = -1;
+        } else {
+            const Box nx{s[0][lw + 1], s[1][lw + 1], s[2][lw + 1], s[3][lw + 1], s[4][lw + 1], s[5][lw + 1]};
+            const float cost = half_area(box_union(me, nx));
+            float last = INFINITY;
+            if (li > 0) {
+                const Box pv{s[0][lw - 1], s[1][lw - 1], s[2][lw - 1], s[3][lw - 1], s[4][lw - 1], s[5][lw - 1]};
+                last = half_area(box_union(pv, me));
+            }
+            mv = last < cost ? (int8_t)-1 : (int8_t)1;
+        }
+    } else {
+        const uint32_t begin = li - min(r, li);
+        const uint32_t end = min(li + r + 1u, sg.count);
+        uint32_t best = li;
+        float best_cost = INFINITY;
+        for (uint32_t j = begin; j < end; ++j) {
+            if (j == li) continue;
+            const int w = (int)(sg.start + j) - wstart;
+            const Box o{s[0][w], s[1][w], s[2][w], s[3][w], s[4][w], s[5][w]};
+            const float c = half_area(box_union(me, o));
+            if (c <= best_cost) {
+                best = j;
+                best_cost = c;
+            }
+        }
+        if (best == li) atomicOr(&misc->err, DERR_SELF_MERGE);
+        mv = (int8_t)((int)best - (int)li);
+    }
+    merge[i] = mv;
+}
+
+// ploc/mod.rs:423-439 within the cluster's segment (indices are global, offsets stay inside the segment).
+__device__ __forceinline__ void b_classify(const int8_t* __restrict__ merge, const uint32_t* __restrict__ cmesh,
+                                           const SegState* __restrict__ seg, uint32_t i, uint32_t total, bool& emit,
+                                           bool& mrg, uint32_t& b) {
+    emit = false;
+    mrg = false;
+    b = i;
+    if (i >= total) return;
+    const SegState sg = seg[cmesh[i]];
+    if (sg.count <= 1u) {
+        emit = true;
+        return;
+    }
+    const int bi = (int)i + (int)merge[i];
+    b = (uint32_t)bi;
+    const bool mutual = bi + (int)merge[bi] == (int)i;
+    if (!mutual || bi == (int)i) {
+        emit = true;
+    } else if (bi < (int)i) {
+        emit = true;
+        mrg = true;
+    }
+}
+
+__global__ void __launch_bounds__(T256) k_b_count(const int8_t* __restrict__ merge, const uint32_t* __restrict__ cmesh,
+                                                  const SegState* __restrict__ seg, const BatchMisc* misc,
+                                                  uint2* __restrict__ blocksums) {
+    __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    bool emit, mrg;
+    uint32_t b;
+    b_classify(merge, cmesh, seg, i, misc->total, emit, mrg, b);
+    const uint32_t e = (uint32_t)__popcll(__ballot(emit)), m = (uint32_t)__popcll(__ballot(mrg));
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        we[wave] = e;
+        wm[wave] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t E = 0, M = 0;
+        for (int w = 0; w < T256 / 64; ++w) {
+            E += we[w];
+            M += wm[w];
+        }
+        blocksums[blockIdx.x] = make_uint2(E, M);
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, uint32_t nblocks, BatchMisc* misc) {
+    __shared__ uint32_t se[16], sm[16];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t chunk = (nblocks + 1023u) / 1024u;
+    const uint32_t b0 = tid * chunk, b1 = min(nblocks, b0 + chunk);
+    uint32_t E = 0, M = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint2 v = blocksums[b];
+        E += v.x;
+        M += v.y;
+    }
+    const uint32_t ie = wave_inclusive_scan(E), im = wave_inclusive_scan(M);
+    const uint32_t wave = tid >> 6, lane = tid & 63u;
+    if (lane == 63) {
+        se[wave] = ie;
+        sm[wave] = im;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t re = 0, rm = 0;
+        for (int w = 0; w < 16; ++w) {
+            const uint32_t te = se[w], tm = sm[w];
+            se[w] = re;
+            sm[w] = rm;
+            re += te;
+            rm += tm;
+        }
+        misc->totals[0] = re;
+        misc->totals[1] = rm;
+    }
+    __syncthreads();
+    uint32_t ee = se[wave] + ie - E, em = sm[wave] + im - M;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint2 v = blocksums[b];
+        blocksums[b] = make_uint2(ee, em);
+        ee += v.x;
+        em += v.y;
+    }
+}
+
+// Exclusive (emit, merge) prefix at cluster index i, from the scanned block offsets.
+__device__ __forceinline__ uint2 b_prefix_at(const int8_t* __restrict__ merge, const uint32_t* __restrict__ cmesh,
+                                             const SegState* __restrict__ seg, const uint2* __restrict__ offsets,
+                                             uint32_t i, uint32_t total, const BatchMisc* misc) {
+    if (i >= total) return make_uint2(misc->totals[0], misc->totals[1]);
+    const uint32_t blk = i / T256;
+    uint2 p = offsets[blk];
+    for (uint32_t j = blk * T256; j < i; ++j) {
+        bool emit, mrg;
+        uint32_t b;
+        b_classify(merge, cmesh, seg, j, total, emit, mrg, b);
+        p.x += emit ? 1u : 0u;
+        p.y += mrg ? 1u : 0u;
+    }
+    return p;
+}
+
+// Per mesh: the segment of the next pass, its insert index and this pass's merge base.
+__global__ void __launch_bounds__(T256) k_b_segments(const int8_t* __restrict__ merge,
+                                                     const uint32_t* __restrict__ cmesh, SegState* __restrict__ seg,
+                                                     SegState* __restrict__ seg_next, const uint2* __restrict__ offsets,
+                                                     uint32_t m, BatchMisc* misc) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    if (k >= m) return;
+    const SegState sg = seg[k];
+    if (sg.count == 0u) {
+        seg_next[k] = sg;
+        return;
+    }
+    const uint32_t total = misc->total;
+    const uint2 head = b_prefix_at(merge, cmesh, seg, offsets, sg.start, total, misc);
+    const uint2 end = b_prefix_at(merge, cmesh, seg, offsets, sg.start + sg.count, total, misc);
+    const uint32_t merges = end.y - head.y;
+    seg[k].mbase = head.y;
+    SegState nx{head.x, end.x - head.x, sg.insert - 2u * merges, 0u};
+    seg_next[k] = nx;
+    if (nx.count > 1u) atomicAdd(&misc->active, 1u);
+}
+
+__global__ void __launch_bounds__(T256) k_b_compact(const Node32* __restrict__ cur, Node32* __restrict__ next,
+                                                    const uint32_t* __restrict__ cmesh, uint32_t* __restrict__ nmesh,
+                                                    const int8_t* __restrict__ merge,
+                                                    const SegState* __restrict__ seg, BatchMisc* misc,
+                                                    const uint2* __restrict__ offsets,
+                                                    const MeshDesc* __restrict__ meshes, Node32* __restrict__ nodes,
+                                                    DpView dp) {
+    __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
+    const uint32_t total = misc->total;
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    bool emit, mrg;
+    uint32_t b;
+    b_classify(merge, cmesh, seg, i, total, emit, mrg, b);
+    const uint64_t be = __ballot(emit), bm = __ballot(mrg);
+    const uint64_t lt = lanemask_lt();
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        we[wave] = (uint32_t)__popcll(be);
+        wm[wave] = (uint32_t)__popcll(bm);
+    }
+    __syncthreads();
+    if (!emit) return;
+    uint32_t pe = offsets[blockIdx.x].x, pm = offsets[blockIdx.x].y;
+    for (uint32_t w = 0; w < wave; ++w) {
+        pe += we[w];
+        pm += wm[w];
+    }
+    const uint32_t pos = pe + (uint32_t)__popcll(be & lt);
+    const uint32_t mesh = cmesh[i];
+    const Node32 a = load_node(&cur[i]);
+    nmesh[pos] = mesh;
+    if (mrg) {
+        const SegState sg = seg[mesh];
+        const MeshDesc md = meshes[mesh];
+        const uint32_t s = pm + (uint32_t)__popcll(bm & lt) - sg.mbase;
+        const Node32 c = load_node(&cur[b]);
+        const uint32_t first = sg.insert - 2u * (s + 1u);
+        Node32* mn = nodes + md.node_base;
+        store_node(&mn[first], a);
+        store_node(&mn[first + 1], c);
+        const Box u = box_union(node_box(a), node_box(c));
+        store_node(&next[pos], make_node(u, 0u, first));
+        DpView dpm = dp;
+        dpm.kbase = md.prim_base;
+        const uint32_t e = dp_evaluate(dpm, u, first, a, c);
+        if (e) atomicOr(&misc->err, e);
+    } else {
+        store_node(&next[pos], a);
+    }
+}
+
+__global__ void k_b_after_pass(BatchMisc* misc) {
+    if (threadIdx.x == 0) misc->total = misc->totals[0];
+}
+
+__global__ void k_b_clear_active(BatchMisc* misc) {
+    if (threadIdx.x == 0) misc->active = 0u;
+}
+
+// Roots (ploc/mod.rs:499): the mesh's last cluster goes to its node 0.
+__global__ void __launch_bounds__(T256) k_b_roots(const Node32* __restrict__ cur, const SegState* __restrict__ seg,
+                                                  const MeshDesc* __restrict__ meshes, uint32_t m,
+                                                  Node32* __restrict__ nodes, uint32_t* __restrict__ prims,
+                                                  BatchMisc* misc) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    if (k >= m) return;
+    const SegState sg = seg[k];
+    const MeshDesc md = meshes[k];
+    if (md.prim_count == 0u) return;
+    if (sg.count != 1u || sg.insert != 1u) {
+        atomicOr(&misc->err, DERR_SELF_MERGE);  // ploc/mod.rs:496
+        return;
+    }
+    store_node(&nodes[md.node_base], load_node(&cur[sg.start]));
+}
+
+__global__ void __launch_bounds__(T256) k_b_iota_local(uint32_t* __restrict__ prims,
+                                                       const uint32_t* __restrict__ offsets, uint32_t m,
+                                                       uint32_t n_total) {
+    const uint32_t g = blockIdx.x * T256 + threadIdx.x;
+    if (g >= n_total) return;
+    prims[g] = g - offsets[mesh_of(offsets, m, g)];
+}
+
+}  // namespace
+
+#define OBVHS_TRY(expr, where)                                      \
+    do {                                                            \
+        hipError_t _e = (expr);                                     \
+        if (_e != hipSuccess) return set_hip_error(ctx, _e, where); \
+    } while (0)
+
+static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const uint32_t* h_offsets,
+                                uint32_t m, uint32_t r, uint32_t sdt, uint32_t max_prims, obvhs_batch* B) {
+    hipStream_t st = ctx->stream;
+    const uint32_t n = h_offsets[m];
+    B->n_total = n;
+    B->meshes.assign(m, MeshDesc{});
+    for (uint32_t k = 0; k < m; ++k) {
+        if (h_offsets[k + 1] < h_offsets[k]) return OBVHS_ERR_INVALID_ARG;
+        MeshDesc& md = B->meshes[k];
+        md.prim_base = h_offsets[k];
+        md.node_base = 2u * h_offsets[k];
+        md.prim_count = h_offsets[k + 1] - h_offsets[k];
+    }
+    OBVHS_TRY(B->tri_offsets.ensure((size_t)(m + 1) * 4), "alloc offsets");
+    OBVHS_TRY(B->mesh_table.ensure((size_t)m * sizeof(MeshDesc)), "alloc mesh table");
+    OBVHS_TRY(hipMemcpyAsync(B->tri_offsets.ptr, h_offsets, (size_t)(m + 1) * 4, hipMemcpyHostToDevice, st),
+              "upload offsets");
+    OBVHS_TRY(hipMemcpyAsync(B->mesh_table.ptr, B->meshes.data(), (size_t)m * sizeof(MeshDesc),
+                             hipMemcpyHostToDevice, st),
+              "upload mesh table");
+    OBVHS_TRY(hipStreamSynchronize(st), "upload sync");  // the host vectors may change below
+    if (n == 0) {
+        B->cw_total = 0;
+        for (auto& md : B->meshes) md.cw_count = 0;
+        return OBVHS_OK;
+    }
+    BuildScratch& S = ctx->scratch;
+    const uint32_t nblk = (n + T256 - 1) / T256;
+    OBVHS_TRY(S.leaves.ensure((size_t)n * sizeof(Node32)), "alloc leaves");
+    OBVHS_TRY(S.clusters_a.ensure((size_t)n * sizeof(Node32)), "alloc clusters");
+    OBVHS_TRY(S.clusters_b.ensure((size_t)n * sizeof(Node32)), "alloc clusters");
+    OBVHS_TRY(S.keys.ensure((size_t)n * 8), "alloc keys");
+    OBVHS_TRY(S.vals.ensure((size_t)n * 8), "alloc vals + cluster meshes");
+    const size_t cm_off = ((size_t)n + 64 + 255) & ~size_t(255);
+    OBVHS_TRY(S.merge.ensure(cm_off + (size_t)n * 4), "alloc merge + cluster meshes");
+    OBVHS_TRY(S.blocksums.ensure((size_t)nblk * sizeof(uint2) + 64), "alloc blocksums");
+    OBVHS_TRY(S.sort_scratch.ensure(radix_sort_scratch_bytes(n)), "alloc sort scratch");
+    const size_t seg_bytes = (size_t)m * sizeof(SegState);
+    OBVHS_TRY(S.misc.ensure(2 * seg_bytes + (size_t)m * 24 + sizeof(BatchMisc) + 1024), "alloc batch misc");
+    OBVHS_TRY(B->bvh2_nodes.ensure((size_t)2 * n * sizeof(Node32)), "alloc bvh2 nodes");
+    OBVHS_TRY(B->bvh2_prims.ensure((size_t)n * 4), "alloc bvh2 prims");
+    const size_t k = n;
+    OBVHS_TRY(B->dp.ensure(k * (28 + 8 + 28 + 4 + 4 + 8 + 4) + 1024), "alloc cost table");
+    char* mb = S.misc.as<char>();
+    SegState* seg[2] = {reinterpret_cast<SegState*>(mb), reinterpret_cast<SegState*>(mb + seg_bytes)};
+    uint32_t* bounds = reinterpret_cast<uint32_t*>(mb + 2 * seg_bytes);
+    BatchMisc* misc = reinterpret_cast<BatchMisc*>(mb + 2 * seg_bytes + (size_t)m * 24);
+    const uint32_t* offsets = B->tri_offsets.as<uint32_t>();
+    MeshDesc* d_meshes = B->mesh_table.as<MeshDesc>();
+    Node32* leaves = S.leaves.as<Node32>();
+    uint64_t* keys = S.keys.as<uint64_t>();
+    uint32_t* vals = S.vals.as<uint32_t>();
+    uint32_t* cmesh[2] = {S.vals.as<uint32_t>() + n, reinterpret_cast<uint32_t*>(S.merge.as<char>() + cm_off)};
+    int8_t* merge = S.merge.as<int8_t>();
+    uint2* blocksums = S.blocksums.as<uint2>();
+    Node32* nodes = B->bvh2_nodes.as<Node32>();
+    char* dpb = B->dp.as<char>();
+    DpView dp;
+    dp.cost = reinterpret_cast<float*>(dpb);
+    dp.meta = reinterpret_cast<uint2*>(dpb + k * 28);
+    dp.sub = reinterpret_cast<uint32_t*>(dpb + k * 36);
+    dp.nprims = reinterpret_cast<uint32_t*>(dpb + k * 64);
+    dp.cwsize = reinterpret_cast<uint32_t*>(dpb + k * 68);
+    dp.subdepth = reinterpret_cast<uint2*>(dpb + k * 72);
+    dp.cwdepth = reinterpret_cast<uint32_t*>(dpb + k * 80);
+    dp.max_prims = max_prims;
+    dp.kbase = 0u;
+
+    const uint32_t mblk = (m + T256 - 1) / T256;
+    k_b_init<<<mblk, T256, 0, st>>>(bounds, offsets, m, seg[0], misc, n);
+    k_b_leaf_init<<<nblk, T256, 0, st>>>(reinterpret_cast<const float4*>(d_tris), offsets, m, n, leaves, bounds,
+                                         misc);
+    k_b_morton<<<nblk, T256, 0, st>>>(leaves, offsets, m, n, bounds, keys);
+    OBVHS_TRY(hipGetLastError(), "batch leaf launch");
+    OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 63, S.sort_scratch.ptr, st), "batch morton sort");
+    k_b_mesh_keys<<<nblk, T256, 0, st>>>(vals, offsets, m, n, keys);
+    int mbits = 1;
+    while (mbits < 32 && (1ull << mbits) < m) mbits++;
+    OBVHS_TRY(radix_sort_pairs(keys, vals, false, n, mbits, S.sort_scratch.ptr, st), "batch mesh sort");
+    Node32* cur = S.clusters_a.as<Node32>();
+    Node32* nxt = S.clusters_b.as<Node32>();
+    k_b_gather<<<nblk, T256, 0, st>>>(leaves, vals, offsets, m, n, cur, cmesh[0]);
+    k_b_iota_local<<<nblk, T256, 0, st>>>(B->bvh2_prims.as<uint32_t>(), offsets, m, n);
+    OBVHS_TRY(hipGetLastError(), "batch gather launch");
+
+    uint32_t known = n;
+    uint32_t pass = 0;
+    int cm = 0;    // which cluster-mesh buffer is current
+    int cs = 0;    // which segment table is current
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    for (;;) {
+        const uint32_t blocks = (known + T256 - 1) / T256;
+        for (int q = 0; q < PASSES_PER_SYNC; ++q) {
+            const uint32_t dist1 = (r == 1 || pass < sdt) ? 1u : 0u;
+            k_b_clear_active<<<1, 64, 0, st>>>(misc);
+            k_b_search<<<blocks, T256, 0, st>>>(cur, cmesh[cm], seg[cs], misc, r, dist1, merge);
+            k_b_count<<<blocks, T256, 0, st>>>(merge, cmesh[cm], seg[cs], misc, blocksums);
+            k_b_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc);
+            k_b_segments<<<mblk, T256, 0, st>>>(merge, cmesh[cm], seg[cs], seg[cs ^ 1], blocksums, m, misc);
+            k_b_compact<<<blocks, T256, 0, st>>>(cur, nxt, cmesh[cm], cmesh[cm ^ 1], merge, seg[cs], misc, blocksums,
+                                                 d_meshes, nodes, dp);
+            k_b_after_pass<<<1, 64, 0, st>>>(misc);
+            std::swap(cur, nxt);
+            cm ^= 1;
+            cs ^= 1;
+            pass++;
+        }
+        OBVHS_TRY(hipGetLastError(), "batch pass launch");
+        OBVHS_TRY(hipMemcpyAsync(h, misc, 16, hipMemcpyDeviceToHost, st), "read batch state");
+        OBVHS_TRY(hipStreamSynchronize(st), "batch pass sync");
+        if (h[0]) return device_error_to_status(h[0]);
+        if (h[2] == 0u) break;  // every mesh is down to its root
+        // Every active mesh merges at least one pair per pass unless costs are NaN.
+        if (h[1] >= known) return OBVHS_ERR_PLOC_SELF_MERGE;
+        known = h[1];
+    }
+    B->passes = pass;
+    k_b_roots<<<mblk, T256, 0, st>>>(cur, seg[cs], d_meshes, m, nodes, B->bvh2_prims.as<uint32_t>(), misc);
+    OBVHS_TRY(hipGetLastError(), "batch roots launch");
+    OBVHS_TRY(hipMemcpyAsync(h, misc, 4, hipMemcpyDeviceToHost, st), "read batch err");
+    OBVHS_TRY(hipStreamSynchronize(st), "batch roots sync");
+    if (h[0]) return device_error_to_status(h[0]);
+    return cwbvh_convert_batch(ctx, nodes, B->bvh2_prims.as<uint32_t>(), dp, d_meshes, B->meshes, n, true,
+                               B->cw_nodes, B->cw_prims, B->cw_total);
+}
+
+}  // namespace obvhs
+
+using namespace obvhs;
+
+extern "C" {
+
+obvhs_status obvhs_batch_build_cwbvh(obvhs_context* ctx, const obvhs_triangle* d_tris, const uint32_t* tri_offsets,
+                                     uint32_t mesh_count, const obvhs_build_params* params, obvhs_batch* reuse,
+                                     obvhs_batch** out) {
+    if (!ctx || !params || !out || !tri_offsets) return OBVHS_ERR_INVALID_ARG;
+    if (params->pre_split || params->reinsertion_batch_ratio > 0.0f || params->sort_precision != 64u)
+        return OBVHS_ERR_INVALID_ARG;
+    const uint32_t r = params->ploc_search_distance;
+    if (!(r == 1 || r == 2 || r == 6 || r == 14 || r == 24 || r == 32)) return OBVHS_ERR_INVALID_ARG;
+    if (tri_offsets[0] != 0u || tri_offsets[mesh_count] >= (1u << 30)) return OBVHS_ERR_INVALID_ARG;
+    if (tri_offsets[mesh_count] > 0 && !d_tris) return OBVHS_ERR_INVALID_ARG;
+    uint32_t mp = params->max_prims_per_leaf;
+    mp = mp < 1u ? 1u : (mp > 3u ? 3u : mp);  // cwbvh/builder.rs:74
+    if (hipSetDevice(ctx->device) != hipSuccess) return OBVHS_ERR_DEVICE;
+    obvhs_batch* B = reuse ? reuse : new (std::nothrow) obvhs_batch();
+    if (!B) return OBVHS_ERR_OOM;
+    const obvhs_status s =
+        batch_build(ctx, d_tris, tri_offsets, mesh_count, r, params->search_depth_threshold, mp, B);
+    if (s != OBVHS_OK) {
+        if (!reuse) obvhs_batch_destroy(B);
+        *out = nullptr;
+        return s;
+    }
+    *out = B;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_batch_info(const obvhs_batch* b, uint32_t* mesh_count, uint32_t* total_nodes,
+                              uint32_t* total_prims, uint32_t* ploc_passes) {
+    if (!b) return OBVHS_ERR_INVALID_ARG;
+    if (mesh_count) *mesh_count = (uint32_t)b->meshes.size();
+    if (total_nodes) *total_nodes = b->cw_total;
+    if (total_prims) *total_prims = b->n_total;
+    if (ploc_passes) *ploc_passes = b->passes;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_batch_mesh_info(const obvhs_batch* b, uint32_t mesh, uint32_t* node_offset, uint32_t* node_count,
+                                   uint32_t* prim_offset, uint32_t* prim_count) {
+    if (!b || mesh >= b->meshes.size()) return OBVHS_ERR_INVALID_ARG;
+    const MeshDesc& md = b->meshes[mesh];
+    if (node_offset) *node_offset = md.cw_base;
+    if (node_count) *node_count = md.cw_count;
+    if (prim_offset) *prim_offset = md.prim_base;
+    if (prim_count) *prim_count = md.prim_count;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_batch_copy_nodes(obvhs_context* ctx, const obvhs_batch* b, obvhs_cwbvh_node* host_out) {
+    if (!ctx || !b) return OBVHS_ERR_INVALID_ARG;
+    return obvhs_memcpy_dtoh(ctx, host_out, b->cw_nodes.ptr, (size_t)b->cw_total * sizeof(CwNode));
+}
+
+obvhs_status obvhs_batch_copy_primitive_indices(obvhs_context* ctx, const obvhs_batch* b, uint32_t* host_out) {
+    if (!ctx || !b) return OBVHS_ERR_INVALID_ARG;
+    return obvhs_memcpy_dtoh(ctx, host_out, b->cw_prims.ptr, (size_t)b->n_total * 4);
+}
+
+obvhs_status obvhs_batch_reorder_triangles(obvhs_context* ctx, const obvhs_batch* b, const obvhs_triangle* d_tris,
+                                           obvhs_triangle* d_bvh_tris) {
+    if (!ctx || !b) return OBVHS_ERR_INVALID_ARG;
+    if (b->n_total == 0) return OBVHS_OK;
+    hipError_t e = launch_reorder_triangles_meshes(d_tris, b->cw_prims.as<uint32_t>(), b->tri_offsets.as<uint32_t>(),
+                                                   (uint32_t)b->meshes.size(), b->n_total, d_bvh_tris, ctx->stream);
+    if (e != hipSuccess) return set_hip_error(ctx, e, "batch reorder");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_batch_ray_traverse(obvhs_context* ctx, const obvhs_batch* b, const obvhs_triangle* d_bvh_tris,
+                                      const obvhs_ray* d_rays, const uint32_t* d_ray_mesh, uint32_t n,
+                                      obvhs_ray_hit* d_hits) {
+    if (!ctx || !b || (n && (!d_rays || !d_ray_mesh || !d_hits))) return OBVHS_ERR_INVALID_ARG;
+    hipError_t e = launch_traverse_meshes(b->cw_nodes.as<CwNode>(), b->mesh_table.as<MeshDesc>(),
+                                          (uint32_t)b->meshes.size(), d_ray_mesh,
+                                          d_bvh_tris, d_rays, n, 0, d_hits, nullptr, ctx->stream);
+    if (e != hipSuccess) return set_hip_error(ctx, e, "batch traverse");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch* b,
+                                           const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
+                                           const uint32_t* d_ray_mesh, uint32_t n, uint8_t* d_miss) {
+    if (!ctx || !b || (n && (!d_rays || !d_ray_mesh || !d_miss))) return OBVHS_ERR_INVALID_ARG;
+    hipError_t e = launch_traverse_meshes(b->cw_nodes.as<CwNode>(), b->mesh_table.as<MeshDesc>(),
+                                          (uint32_t)b->meshes.size(), d_ray_mesh,
+                                          d_bvh_tris, d_rays, n, 1, nullptr, d_miss, ctx->stream);
+    if (e != hipSuccess) return set_hip_error(ctx, e, "batch traverse miss");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_batch_destroy(obvhs_batch* b) {
+    if (!b) return OBVHS_OK;
+    b->tri_offsets.release();
+    b->mesh_table.release();
+    b->bvh2_nodes.release();
+    b->bvh2_prims.release();
+    b->dp.release();
+    b->cw_nodes.release();
+    b->cw_prims.release();
+    delete b;
+    return OBVHS_OK;
+}
+
+}  // extern "C"

@@ -39,6 +39,7 @@ struct CwItem {
     uint32_t bvh2;
     uint32_t a;  // CW nodes allocated before this node's child block, minus the root
     uint32_t b;  // primitives appended before this node
+    uint32_t mesh;
 };
 
 struct CwMisc {
@@ -50,7 +51,7 @@ __global__ void k_cw_init(CwItem* items, CwMisc* misc) {
     if (threadIdx.x == 0) {
         misc->err = 0u;
         for (uint32_t l = 0; l <= MAX_CW_LEVELS; ++l) misc->level_count[l] = 0u;
-        items[0] = CwItem{0u, 0u, 0u, 0u};
+        items[0] = CwItem{0u, 0u, 0u, 0u, 0u};
         misc->level_count[0] = 1u;
     }
 }
@@ -104,11 +105,22 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
                                                   const CwItem* __restrict__ in_items, CwItem* __restrict__ out_items,
                                                   uint32_t level, uint32_t order_children, CwNode* __restrict__ out,
                                                   uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
-                                                  CwMisc* misc) {
+                                                  CwMisc* misc, const MeshDesc* __restrict__ meshes) {
     const uint32_t t = blockIdx.x * T256 + threadIdx.x;
     const uint32_t count = misc->level_count[level];
     if (t >= count) return;
     const CwItem it = in_items[t];
+    // Batched conversions: every array is the item's mesh's slice.
+    if (meshes) {
+        const MeshDesc md = meshes[it.mesh];
+        nodes += md.node_base;
+        bvh2_prims += md.prim_base;
+        out += md.cw_base;
+        out_prims += md.prim_base;
+        cw_cap = md.cw_count;
+        prim_cap = md.prim_count;
+        dp.kbase = md.prim_base;
+    }
     const Node32 nd = load_node(&nodes[it.bvh2]);
     const Box nb = node_box(nd);
 
@@ -317,9 +329,9 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
                 ci = ch[q];
                 cf = cfirst[q];
             }
-        out_items[base + j] = CwItem{child_base + j, ci, a_run, b_run};
-        a_run += dp.cwsize[dkey(cf)] - 1u;
-        b_run += dp.nprims[dkey(cf)];
+        out_items[base + j] = CwItem{child_base + j, ci, a_run, b_run, it.mesh};
+        a_run += dp.cwsize[dp_key(dp, cf)] - 1u;
+        b_run += dp.nprims[dp_key(dp, cf)];
     }
 }
 
@@ -329,6 +341,66 @@ __global__ void __launch_bounds__(T256) k_reorder_tris(const float4* __restrict_
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     if (i >= n) return;
     const size_t s = (size_t)prims[i] * 3, d = (size_t)i * 3;
+    out[d] = tris[s];
+    out[d + 1] = tris[s + 1];
+    out[d + 2] = tris[s + 2];
+}
+
+// Batched conversions: each mesh's CWBVH node count and depth (from its root's cost-table entry) and its
+// level-0 work item.
+__global__ void __launch_bounds__(T256) k_b_cw_sizes(const Node32* __restrict__ nodes, DpView dp,
+                                                     MeshDesc* __restrict__ meshes, uint32_t m) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= m) return;
+    MeshDesc md = meshes[i];
+    if (md.prim_count == 0u) {
+        md.cw_count = 0u;
+        md.cw_depth = 0u;
+    } else {
+        const Node32 root = load_node(&nodes[md.node_base]);
+        if (root.prim_count != 0u) {
+            md.cw_count = 1u;
+            md.cw_depth = 1u;
+        } else {
+            dp.kbase = md.prim_base;
+            md.cw_count = dp.cwsize[dp_key(dp, root.first_index)];
+            md.cw_depth = dp.cwdepth[dp_key(dp, root.first_index)];
+        }
+    }
+    meshes[i] = md;
+}
+
+__global__ void __launch_bounds__(T256) k_b_cw_roots(const MeshDesc* __restrict__ meshes, uint32_t m,
+                                                     CwItem* __restrict__ items, CwMisc* misc) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i == 0) {
+        misc->err = 0u;
+        for (uint32_t l = 1; l <= MAX_CW_LEVELS; ++l) misc->level_count[l] = 0u;
+    }
+    if (i >= m) return;
+    if (meshes[i].prim_count == 0u) return;
+    const uint32_t slot = atomicAdd(&misc->level_count[0], 1u);
+    items[slot] = CwItem{0u, 0u, 0u, 0u, i};
+}
+
+__global__ void k_zero_level0(CwMisc* misc) {
+    if (threadIdx.x == 0) misc->level_count[0] = 0u;
+}
+
+// Batched reordering: bvh_tris[g] = tris[prim_base(mesh of g) + primitive_indices[g]] (indices are mesh-local).
+__global__ void __launch_bounds__(T256) k_reorder_tris_meshes(const float4* __restrict__ tris,
+                                                              const uint32_t* __restrict__ prims,
+                                                              const uint32_t* __restrict__ offsets, uint32_t m,
+                                                              uint32_t n, float4* __restrict__ out) {
+    const uint32_t g = blockIdx.x * T256 + threadIdx.x;
+    if (g >= n) return;
+    uint32_t lo = 0, hi = m;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (offsets[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const size_t s = ((size_t)offsets[lo] + prims[g]) * 3, d = (size_t)g * 3;
     out[d] = tris[s];
     out[d + 1] = tris[s + 1];
     out[d + 2] = tris[s + 2];
@@ -393,6 +465,7 @@ DpView bvh_dp_view(const obvhs_bvh2* bvh, uint32_t max_prims) {
     v.subdepth = reinterpret_cast<uint2*>(base + k * 72);
     v.cwdepth = reinterpret_cast<uint32_t*>(base + k * 80);
     v.max_prims = max_prims;
+    v.kbase = 0u;
     return v;
 }
 
@@ -470,7 +543,7 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
         const uint32_t g = (cap + T256 - 1) / T256;
         k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2->prim_indices.as<uint32_t>(), dp, items[level & 1],
                                       items[(level + 1) & 1], level, order_children ? 1u : 0u,
-                                      out->nodes.as<CwNode>(), out->prim_indices.as<uint32_t>(), ncw, n, misc);
+                                      out->nodes.as<CwNode>(), out->prim_indices.as<uint32_t>(), ncw, n, misc, nullptr);
         bound *= 8;
     }
     OBVHS_TRY(hipGetLastError(), "cwbvh emit launch");
@@ -487,6 +560,71 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
     return OBVHS_OK;
 }
 
+obvhs_status cwbvh_convert_batch(obvhs_context* ctx, const Node32* nodes, const uint32_t* bvh2_prims,
+                                 const DpView& dp, MeshDesc* d_meshes, std::vector<MeshDesc>& meshes,
+                                 uint32_t n_total, bool order_children, DevBuf& out_nodes, DevBuf& out_prims,
+                                 uint32_t& total_cw) {
+    hipStream_t st = ctx->stream;
+    const uint32_t m = (uint32_t)meshes.size();
+    total_cw = 0;
+    if (m == 0) return OBVHS_OK;
+    k_b_cw_sizes<<<(m + T256 - 1) / T256, T256, 0, st>>>(nodes, dp, d_meshes, m);
+    OBVHS_TRY(hipGetLastError(), "batch cw sizes");
+    OBVHS_TRY(hipMemcpyAsync(meshes.data(), d_meshes, m * sizeof(MeshDesc), hipMemcpyDeviceToHost, st),
+              "read mesh table");
+    OBVHS_TRY(hipStreamSynchronize(st), "mesh table sync");
+    uint64_t total = 0;
+    uint32_t levels = 0, active = 0;
+    for (auto& md : meshes) {
+        if (md.prim_count > 0 && md.cw_count == 0) return OBVHS_ERR_CW_DISTRIBUTE;
+        md.cw_base = (uint32_t)total;
+        total += md.cw_count;
+        levels = md.cw_depth > levels ? md.cw_depth : levels;
+        active += md.prim_count > 0 ? 1u : 0u;
+    }
+    if (total >= (1ull << 32)) return OBVHS_ERR_CAPACITY;
+    if (levels > MAX_CW_LEVELS) return OBVHS_ERR_DEPTH;
+    total_cw = (uint32_t)total;
+    OBVHS_TRY(hipMemcpyAsync(d_meshes, meshes.data(), m * sizeof(MeshDesc), hipMemcpyHostToDevice, st),
+              "write mesh table");
+    OBVHS_TRY(out_nodes.ensure((size_t)total * sizeof(CwNode) + 80), "alloc batch cwbvh nodes");
+    OBVHS_TRY(out_prims.ensure((size_t)n_total * 4 + 4), "alloc batch cwbvh primitive indices");
+    // Level-0 items: every non-empty mesh's root; items hold at most one entry per BVH2 node of all meshes.
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t cap = (size_t)2 * n_total + m;
+    const size_t o_items0 = carve(cap * sizeof(CwItem)), o_items1 = carve(cap * sizeof(CwItem)),
+                 o_misc = carve(sizeof(CwMisc));
+    OBVHS_TRY(ctx->scratch.cw_scratch.ensure(off), "alloc cwbvh scratch");
+    char* base = ctx->scratch.cw_scratch.as<char>();
+    CwItem* items[2] = {reinterpret_cast<CwItem*>(base + o_items0), reinterpret_cast<CwItem*>(base + o_items1)};
+    CwMisc* misc = reinterpret_cast<CwMisc*>(base + o_misc);
+    k_zero_level0<<<1, 64, 0, st>>>(misc);
+    k_b_cw_roots<<<(m + T256 - 1) / T256, T256, 0, st>>>(d_meshes, m, items[0], misc);
+    uint64_t bound = active;
+    for (uint32_t level = 0; level < levels; ++level) {
+        const uint32_t capl = (uint32_t)(bound < total ? bound : total);
+        const uint32_t g = (capl + T256 - 1) / T256;
+        if (g > 0)
+            k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2_prims, dp, items[level & 1], items[(level + 1) & 1], level,
+                                          order_children ? 1u : 0u, out_nodes.as<CwNode>(),
+                                          out_prims.as<uint32_t>(), 0u, 0u, misc, d_meshes);
+        bound *= 8;
+    }
+    OBVHS_TRY(hipGetLastError(), "batch cwbvh emit launch");
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    OBVHS_TRY(hipMemcpyAsync(h, &misc->level_count[levels], 4, hipMemcpyDeviceToHost, st), "read level");
+    OBVHS_TRY(hipMemcpyAsync(h + 1, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cw err");
+    OBVHS_TRY(hipStreamSynchronize(st), "batch cwbvh emit sync");
+    if (h[1]) return device_error_to_status(h[1]);
+    if (h[0] != 0u) return OBVHS_ERR_DEPTH;
+    return OBVHS_OK;
+}
+
 hipError_t launch_reorder_triangles(const obvhs_triangle* tris, const uint32_t* prim_indices, uint32_t n,
                                     obvhs_triangle* out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
@@ -495,4 +633,15 @@ hipError_t launch_reorder_triangles(const obvhs_triangle* tris, const uint32_t* 
     return hipGetLastError();
 }
 
+}  // namespace obvhs
+
+namespace obvhs {
+hipError_t launch_reorder_triangles_meshes(const obvhs_triangle* tris, const uint32_t* prim_indices,
+                                           const uint32_t* offsets, uint32_t m, uint32_t n, obvhs_triangle* out,
+                                           hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    k_reorder_tris_meshes<<<(n + T256 - 1) / T256, T256, 0, stream>>>(
+        reinterpret_cast<const float4*>(tris), prim_indices, offsets, m, n, reinterpret_cast<float4*>(out));
+    return hipGetLastError();
+}
 }  // namespace obvhs

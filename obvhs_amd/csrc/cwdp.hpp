@@ -36,9 +36,24 @@ struct DpView {
     uint2* subdepth;    // [k] seven bytes: CWBVH depth below the children get_children(node, i) produces
     uint32_t* cwdepth;  // [k] 1 + subdepth[0]: levels of the CWBVH subtree the node roots
     uint32_t max_prims; // max_prims_per_leaf the table was evaluated for
+    uint32_t kbase;     // key offset of the BVH2 being processed (batched builds: the mesh's first primitive)
+};
+
+// One BVH of a batched build: slices of the shared node / primitive / cost-table / CWBVH arrays.
+struct MeshDesc {
+    uint32_t node_base;   // first BVH2 slot (2 * prim_base, so cost-table keys stay disjoint)
+    uint32_t prim_base;   // first primitive; also the mesh's cost-table key base
+    uint32_t prim_count;
+    uint32_t cw_base;     // first CWBVH node
+    uint32_t cw_count;
+    uint32_t cw_depth;
+    uint32_t pad[2];
 };
 
 __host__ __device__ __forceinline__ uint32_t dkey(uint32_t first_index) { return first_index >> 1; }
+__device__ __forceinline__ size_t dp_key(const DpView& dp, uint32_t first_index) {
+    return (size_t)dp.kbase + (first_index >> 1);
+}
 __device__ __forceinline__ uint32_t dbyte(uint2 m, uint32_t i) {
     return ((i < 4u ? m.x : m.y) >> (8u * (i & 3u))) & 0xffu;
 }
@@ -49,7 +64,7 @@ __device__ __forceinline__ uint32_t dpack(uint32_t k, uint32_t l, uint32_t r) { 
 
 // Decision i of an internal node's entry.
 __device__ __forceinline__ uint32_t node_decision(const DpView& dp, uint32_t first_index, uint32_t i) {
-    return dbyte(dp.meta[dkey(first_index)], i);
+    return dbyte(dp.meta[dp_key(dp, first_index)], i);
 }
 __device__ __forceinline__ uint32_t node_kind(const DpView& dp, const Node32& nx, uint32_t i) {
     return nx.prim_count != 0u ? KIND_LEAF : dkind(node_decision(dp, nx.first_index, i));
@@ -81,7 +96,7 @@ __device__ __forceinline__ void load_child_dp(const DpView& dp, const Node32& c,
         o.cwsize = 1u;
         o.cwdepth = 1u;
     } else {
-        o.key = dkey(c.first_index);
+        o.key = dp_key(dp, c.first_index);
 #pragma unroll
         for (int k = 0; k < 7; ++k) o.cost[k] = dp.cost[o.key * 7 + k];
         o.meta = dp.meta[o.key];
@@ -110,7 +125,7 @@ __device__ __forceinline__ uint32_t child_depth(const ChildDp& c, uint32_t d) {
 }
 
 // Evaluates the entry of the internal node with box `box` whose children records are `l` and `r` (slots first,
-// first + 1) and stores it under dkey(first).  Returns a DERR_* bit or 0.
+// first + 1) and stores it under its key.  Returns a DERR_* bit or 0.
 __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box, uint32_t first, const Node32& l,
                                                 const Node32& r) {
     if ((l.prim_count != 0u && l.prim_count != 1u) || (r.prim_count != 0u && r.prim_count != 1u))
@@ -179,7 +194,7 @@ __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box
             sd[i] = min(255u, max(child_depth(L, dl), child_depth(R, dr)));
         }
     }
-    const size_t key = dkey(first);
+    const size_t key = dp_key(dp, first);
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
         dp.cost[key * 7 + k] = cost[k];

@@ -108,6 +108,14 @@ DpView bvh_dp_view(const obvhs_bvh2* bvh, uint32_t max_prims);
 // cwbvh.hip
 obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,
                            bool order_children, obvhs_cwbvh* out);
+struct MeshDesc;
+obvhs_status cwbvh_convert_batch(obvhs_context* ctx, const Node32* nodes, const uint32_t* bvh2_prims,
+                                 const DpView& dp, MeshDesc* d_meshes, std::vector<MeshDesc>& meshes,
+                                 uint32_t n_total, bool order_children, DevBuf& out_nodes, DevBuf& out_prims,
+                                 uint32_t& total_cw);
+hipError_t launch_reorder_triangles_meshes(const obvhs_triangle* tris, const uint32_t* prim_indices,
+                                           const uint32_t* offsets, uint32_t m, uint32_t n, obvhs_triangle* out,
+                                           hipStream_t stream);
 const uint32_t* host_log2_table();
 hipError_t upload_log2_table();
 hipError_t launch_reorder_triangles(const obvhs_triangle* tris, const uint32_t* prim_indices, uint32_t n,
@@ -117,6 +125,10 @@ hipError_t launch_reorder_triangles(const obvhs_triangle* tris, const uint32_t* 
 hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
                            const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,
                            uint32_t* node_visits, uint32_t* tri_tests, hipStream_t stream);
+// Batched traversal: ray i is traced against mesh ray_mesh[i] of the MeshDesc table (ids >= mesh_count miss).
+hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, uint32_t mesh_count,
+                                  const uint32_t* ray_mesh, const obvhs_triangle* tris, const obvhs_ray* rays,
+                                  uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss, hipStream_t stream);
 
 // capi.cpp
 obvhs_status set_hip_error(obvhs_context* ctx, hipError_t e, const char* where);

@@ -18,6 +18,7 @@
 // the host only reads the count every few passes to shrink the grids.
 #include "common.hpp"
 #include "cwdp.hpp"
+#include "device_util.hpp"
 #include "internal.hpp"
 
 namespace obvhs {
@@ -37,26 +38,6 @@ struct Misc {
     uint32_t pad;
     PlocState state[2];
 };
-
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-    const int lane = (int)__lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = (uint32_t)__shfl_up((int)v, off);
-        if (lane >= off) v += t;
-    }
-    return v;
-}
 
 __global__ void k_init_misc(Misc* misc, uint32_t n, uint32_t* hist) {
     if (threadIdx.x == 0) {
@@ -134,10 +115,6 @@ __global__ void __launch_bounds__(1024) k_reduce_bounds(const uint32_t* __restri
     }
 }
 
-__device__ __forceinline__ bool box_has_nan(const Box& b) {
-    return isnan(b.lx) || isnan(b.ly) || isnan(b.lz) || isnan(b.hx) || isnan(b.hy) || isnan(b.hz);
-}
-
 // Triangle::aabb = Aabb::from_points([v0, v1, v2]) (triangle.rs:28-30; aabb.rs:59-66).
 __global__ void __launch_bounds__(T256) k_leaf_init_tris(const float4* __restrict__ tris,
                                                          const uint32_t* __restrict__ indices, uint32_t n,
@@ -176,23 +153,6 @@ __global__ void __launch_bounds__(T256) k_leaf_init_aabbs(const float4* __restri
         store_node(&leaves[i], make_node(b, 1u, indices ? indices[i] : i));
     }
     reduce_scene_bounds(valid, b, nan, misc, partials);
-}
-
-// morton.rs:35-43
-__device__ __forceinline__ uint64_t split_by_3_u64(uint32_t a) {
-    uint64_t x = (uint64_t)a & 0x1fffffull;
-    x = (x | x << 32) & 0x1f00000000ffffull;
-    x = (x | x << 16) & 0x1f0000ff0000ffull;
-    x = (x | x << 8) & 0x100f00f00f00f00full;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
-    x = (x | x << 2) & 0x1249249249249249ull;
-    return x;
-}
-// Rust `f64 as u32` (saturating, NaN -> 0).
-__device__ __forceinline__ uint32_t f64_as_u32(double v) {
-    if (!(v >= 1.0)) return 0u;
-    if (v >= 4294967296.0) return 0xffffffffu;
-    return (uint32_t)v;
 }
 
 // ploc/mod.rs:287-288 and 782-785; morton.rs:55-58.

@@ -11,6 +11,7 @@
 // The stack lives in LDS, one 64-lane column per wave (entry-major so a wave's pushes hit distinct banks);
 // nodes are fetched as five 16-byte loads, triangles (in primitive_indices order) as three.
 #include "common.hpp"
+#include "cwdp.hpp"
 #include "internal.hpp"
 
 namespace obvhs {
@@ -109,15 +110,29 @@ __device__ __forceinline__ float tri_intersect(const float4& v0, const float4& v
     return INFINITY;
 }
 
-template <int MODE, bool COUNT>
+template <int MODE, bool COUNT, bool MESHES = false>
 __global__ void __launch_bounds__(TRAV_THREADS)
     k_traverse(const uint4* __restrict__ nodes, uint32_t node_count, uint32_t prim_count,
                const float4* __restrict__ tris, const float4* __restrict__ rays, uint32_t n, obvhs_ray_hit* __restrict__ hits,
-               uint8_t* __restrict__ miss_out, uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests) {
+               uint8_t* __restrict__ miss_out, uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests,
+               const MeshDesc* __restrict__ meshes = nullptr, const uint32_t* __restrict__ ray_mesh = nullptr) {
     __shared__ uint2 stack[STACK_SIZE][TRAV_THREADS];
     const uint32_t lane = threadIdx.x;
     const uint32_t i = blockIdx.x * TRAV_THREADS + lane;
     if (i >= n) return;
+    if (MESHES) {  // batched: the ray's mesh selects its CWBVH and triangle slices; node_count = mesh count
+        const uint32_t mi = ray_mesh[i];
+        if (mi < node_count) {
+            const MeshDesc md = meshes[mi];
+            nodes += (size_t)md.cw_base * 5;
+            tris += (size_t)md.prim_base * 3;
+            node_count = md.cw_count;
+            prim_count = md.prim_count;
+        } else {  // out-of-range mesh id: the ray sees an empty BVH (a miss)
+            node_count = 0u;
+            prim_count = 0u;
+        }
+    }
     const float4 ra = rays[4 * (size_t)i], rb = rays[4 * (size_t)i + 1], rc = rays[4 * (size_t)i + 2],
                  rd = rays[4 * (size_t)i + 3];
     const RayData r{ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, rc.x, rc.y, rc.z, rd.x, rd.y};
@@ -224,6 +239,28 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
         k_traverse<1, false><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss, nullptr,
                                                                   nullptr);
     }
+    return hipGetLastError();
+}
+
+}  // namespace obvhs
+
+namespace obvhs {
+
+hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, uint32_t mesh_count,
+                                  const uint32_t* ray_mesh,
+                                  const obvhs_triangle* tris, const obvhs_ray* rays, uint32_t n, int mode,
+                                  obvhs_ray_hit* hits, uint8_t* miss, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = (n + TRAV_THREADS - 1) / TRAV_THREADS;
+    const uint4* nd = reinterpret_cast<const uint4*>(nodes);
+    const float4* tr = reinterpret_cast<const float4*>(tris);
+    const float4* ry = reinterpret_cast<const float4*>(rays);
+    if (mode == 0)
+        k_traverse<0, false, true><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits, miss, nullptr,
+                                                                        nullptr, meshes, ray_mesh);
+    else
+        k_traverse<1, false, true><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits, miss, nullptr,
+                                                                        nullptr, meshes, ray_mesh);
     return hipGetLastError();
 }
 

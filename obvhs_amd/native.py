@@ -83,6 +83,15 @@ SIGNATURES = {
     "obvhs_cwbvh_ray_traverse_miss": (INT, [P, P, P, P, U32, P]),
     "obvhs_cwbvh_ray_traverse_counted": (INT, [P, P, P, P, U32, P, P, P]),
     "obvhs_log2_threshold_table": (INT, [P]),
+    "obvhs_batch_build_cwbvh": (INT, [P, P, P, U32, C.POINTER(BuildParams), P, C.POINTER(P)]),
+    "obvhs_batch_info": (INT, [P, PU32, PU32, PU32, PU32]),
+    "obvhs_batch_mesh_info": (INT, [P, U32, PU32, PU32, PU32, PU32]),
+    "obvhs_batch_copy_nodes": (INT, [P, P, P]),
+    "obvhs_batch_copy_primitive_indices": (INT, [P, P, P]),
+    "obvhs_batch_reorder_triangles": (INT, [P, P, P, P]),
+    "obvhs_batch_ray_traverse": (INT, [P, P, P, P, P, U32, P]),
+    "obvhs_batch_ray_traverse_miss": (INT, [P, P, P, P, P, U32, P]),
+    "obvhs_batch_destroy": (INT, [P]),
 }
 
 _lib = None

@@ -8,10 +8,14 @@ All randomness comes from the crate's integer hash (test_util.rs:9-46) so inputs
                         frequencies in [1,32], phases in [0,2pi), amplitudes proportional to 1/frequency, plus seeded
                         12-triangle boxes resting on the surface.
 * bounce_rays        -- one cosine-weighted diffuse bounce per primary hit (geometric normal, seeded).
+* icosphere_batch    -- many independent icosphere meshes (counts log-uniform, rounded down to 20*4^s), vertex
+                        radius jitter +-10% keyed on the vertex position (shared vertices move together), and a
+                        ray batch per mesh from a radius-2 sphere toward uniform points in the mesh AABB.
 """
 import numpy as np
 
-from .test_util import CUBE, build_orthonormal_basis, cosine_sample_hemisphere, hash_noise
+from .test_util import (CUBE, build_orthonormal_basis, cosine_sample_hemisphere, hash_noise, icosphere, uhash,
+                        uhash2, unormf, uniform_sample_sphere)
 from .types import RAY, make_rays, triangles_from_vertices
 
 
@@ -150,3 +154,63 @@ def bounce_rays(rays, hits, bvh_tris, seed=0, eps=1e-4, tmax=1e30):
 
 def empty_rays(n):
     return np.zeros(n, dtype=RAY)
+
+
+ICO_MAX_SUBDIV = 5
+
+
+def _icosphere_base(s, _cache={}):
+    if s not in _cache:
+        t = icosphere(s)
+        _cache[s] = np.stack([t["v0"][:, :3], t["v1"][:, :3], t["v2"][:, :3]], axis=1).copy()  # (20*4^s, 3, 3)
+    return _cache[s]
+
+
+def icosphere_batch_counts(mesh_count, seed=0, min_tris=64, max_tris=65536):
+    """Subdivision level per mesh: a log-uniform count in [min_tris, max_tris] rounded down to 20*4^s (s >= 0)."""
+    i = np.arange(mesh_count, dtype=np.uint32)
+    u = unormf(uhash2(i, np.uint32(0x9E3779B9) ^ np.uint32(seed))).astype(np.float64)
+    c = np.float64(min_tris) * np.power(np.float64(max_tris) / min_tris, u)
+    s = np.floor(np.log(c / 20.0) / np.log(4.0)).astype(np.int64)
+    return np.clip(s, 0, ICO_MAX_SUBDIV)
+
+
+def icosphere_batch(mesh_count=4096, seed=0, rays_per_mesh=1024, min_tris=64, max_tris=65536, levels=None):
+    """Returns (tris, offsets, rays, ray_mesh): concatenated Triangle array, mesh_count+1 prefix offsets, rays with
+    rays_per_mesh rays per mesh (grouped by mesh), and each ray's mesh id.  `levels` overrides the subdivision
+    level of each mesh."""
+    levels = icosphere_batch_counts(mesh_count, seed, min_tris, max_tris) if levels is None else np.asarray(levels)
+    counts = (20 * np.power(4, levels)).astype(np.int64)
+    offsets = np.zeros(mesh_count + 1, dtype=np.int64)
+    np.cumsum(counts, out=offsets[1:])
+    verts = np.empty((int(offsets[-1]), 3, 3), dtype=np.float32)
+    mesh_ids = np.arange(mesh_count, dtype=np.uint32)
+    for lvl in np.unique(levels):
+        sel = np.nonzero(levels == lvl)[0]
+        base = _icosphere_base(int(lvl))
+        bits = base.view(np.uint32)
+        key = uhash2(uhash2(bits[..., 0], bits[..., 1]), bits[..., 2])  # (T, 3): one key per vertex position
+        for chunk in np.array_split(sel, max(1, len(sel) * base.shape[0] // (1 << 22) + 1)):
+            h = uhash2(key[None, :, :], (mesh_ids[chunk] ^ np.uint32(seed))[:, None, None])
+            r = np.float32(1.0) + (unormf(h) * np.float32(2.0) - np.float32(1.0)) * np.float32(0.1)
+            v = base[None] * r[..., None]
+            for j, m in enumerate(chunk):
+                verts[offsets[m]:offsets[m + 1]] = v[j]
+    tris = triangles_from_vertices(verts[:, 0], verts[:, 1], verts[:, 2])
+    # rays: origins uniform on the radius-2 sphere, directions toward a uniform point of the mesh AABB
+    lo = np.empty((mesh_count, 3), dtype=np.float32)
+    hi = np.empty((mesh_count, 3), dtype=np.float32)
+    for m in range(mesh_count):
+        v = verts[offsets[m]:offsets[m + 1]].reshape(-1, 3)
+        lo[m], hi[m] = v.min(axis=0), v.max(axis=0)
+    n = mesh_count * rays_per_mesh
+    k = np.arange(n, dtype=np.uint32)
+    ray_mesh = (k // np.uint32(rays_per_mesh)).astype(np.uint32)
+    frame = np.uint32(seed) + np.uint32(1000)
+    o = uniform_sample_sphere(hash_noise(k, 0, frame), hash_noise(k, 1, frame)) * np.float32(2.0)
+    u = np.stack([hash_noise(k, 2 + a, frame) for a in range(3)], axis=1)
+    target = lo[ray_mesh] + (hi[ray_mesh] - lo[ray_mesh]) * u
+    d = (target - o).astype(np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = make_rays(o, d.astype(np.float32), 0.0, 1e30)
+    return tris, offsets.astype(np.uint32), rays, ray_mesh
