@@ -11,6 +11,10 @@ indices) and reported beside it as Mtris/s.  The roofline entry is for the trave
 with the algorithmic bytes of every ray (40 B of ray/hit + 80 B per node test + 36 B per triangle test, counts taken
 from an instrumented run of the same kernel) divided by its event-timed duration.  The CPU baseline is the C++
 restatement of the crate's CPU path (oracle/), timed on this host's cores in this process.
+
+The "batched" object reports the batched form on 4,096 seeded icospheres (~20 M triangles, 1,024 rays each): one
+shared build and one traversal launch per step, next to the plain form (a build_cwbvh_from_tris call and a
+traversal launch per mesh) on the same inputs, with a check that both give identical hits.
 """
 import argparse
 import json
@@ -48,6 +52,103 @@ def cpu_baseline(tris, rays_all, threads, sample_rays):
     return dict(build_s=build_s, trace_s=trace_s, n_rays=len(sample), n_tris=len(tris))
 
 
+def bench_batch(args, ob, ctx):
+    """Batched form: 4,096 icospheres (log-uniform 64..65,536 triangles rounded down to 20*4^s) built in shared
+    launches and traced with 1,024 rays per mesh in one launch per step; beside it the plain form on the same
+    workload (one build_cwbvh_from_tris call and one traversal launch per mesh)."""
+    import ctypes as C
+    from obvhs_amd import scenes
+    lib = ob.native.lib()
+    m = args.batch_meshes
+    log(f"generating {m} icospheres")
+    tris, offsets, rays, ray_mesh = scenes.icosphere_batch(m, seed=0, rays_per_mesh=args.batch_rays_per_mesh)
+    n_tris, n_rays = len(tris), len(rays)
+    d_tris, d_rays, d_mesh = ctx.upload(tris), ctx.upload(rays), ctx.upload(ray_mesh)
+    params = ob.BvhBuildParams(ploc_search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3)
+    log(f"batched build: {m} meshes, {n_tris} triangles")
+    batch = ob.build_cwbvh_batch(ctx, d_tris, offsets, params)
+    ms = []
+    for _ in range(max(1, args.build_reps)):
+        ctx.timer_start()
+        ob.build_cwbvh_batch(ctx, d_tris, offsets, params, reuse=batch)
+        ms.append(ctx.timer_stop())
+    build_ms = float(np.median(ms))
+    d_bvh_tris = batch.reorder_triangles(d_tris)
+    hits = ob.DeviceArray(ctx, ob.RAY_HIT, n_rays)
+
+    def trace_batch():
+        ob.native.check(lib.obvhs_batch_ray_traverse(ctx.handle, batch.handle, d_bvh_tris.ptr, d_rays.ptr,
+                                                     d_mesh.ptr, n_rays, hits.ptr), ctx.handle)
+
+    for _ in range(args.warmup):
+        trace_batch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        trace_batch()
+    ctx.synchronize()
+    trace_s = (time.perf_counter() - t0) / args.steps
+    batch_hits = hits.to_host()
+
+    # Plain form: per-mesh handles, builds and traversal launches through the single-mesh entry points.
+    log("plain per-mesh builds")
+    cp = params.to_c()
+    tri_sz, ray_sz, hit_sz = ob.TRIANGLE.itemsize, ob.RAY.itemsize, ob.RAY_HIT.itemsize
+    handles = [C.c_void_p() for _ in range(m)]
+    base_t, base_r, base_h = d_tris.ptr.value, d_rays.ptr.value, hits.ptr.value
+    d_plain_tris = ob.DeviceArray(ctx, ob.TRIANGLE, n_tris)
+    base_pt = d_plain_tris.ptr.value
+    rpm = args.batch_rays_per_mesh
+
+    def plain_build(reuse):
+        for i in range(m):
+            o, n = int(offsets[i]), int(offsets[i + 1] - offsets[i])
+            ob.native.check(lib.obvhs_build_cwbvh_from_tris(ctx.handle, C.c_void_p(base_t + o * tri_sz), n,
+                                                            C.byref(cp), handles[i] if reuse else None,
+                                                            C.byref(handles[i]), None), ctx.handle)
+
+    plain_build(False)
+    ctx.timer_start()
+    plain_build(True)
+    plain_build_ms = ctx.timer_stop()
+    for i in range(m):
+        o = int(offsets[i])
+        ob.native.check(lib.obvhs_cwbvh_reorder_triangles(ctx.handle, handles[i], C.c_void_p(base_t + o * tri_sz),
+                                                          C.c_void_p(base_pt + o * tri_sz)), ctx.handle)
+
+    def trace_plain():
+        for i in range(m):
+            o = int(offsets[i])
+            ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, handles[i], C.c_void_p(base_pt + o * tri_sz),
+                                                         C.c_void_p(base_r + i * rpm * ray_sz), rpm,
+                                                         C.c_void_p(base_h + i * rpm * hit_sz)), ctx.handle)
+
+    trace_plain()
+    ctx.synchronize()
+    plain_hits = hits.to_host()
+    same = bool(plain_hits.tobytes() == batch_hits.tobytes())
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 4)):
+        trace_plain()
+    ctx.synchronize()
+    plain_trace_s = (time.perf_counter() - t0) / max(1, args.steps // 4)
+    for h in handles:
+        lib.obvhs_cwbvh_destroy(h)
+    info = batch.info()
+    batch.close()
+    return {"workload": f"{m} icospheres (log-uniform 64..65536 tris, rounded down to 20*4^s) = {n_tris} tris; "
+                        f"{rpm} rays per mesh = {n_rays} closest-hit rays",
+            "build_ms": round(build_ms, 3), "build_mtris_per_s": round(n_tris / build_ms / 1e3, 2),
+            "trace_mrays_per_s": round(n_rays / trace_s / 1e6, 2), "trace_ms": round(trace_s * 1e3, 4),
+            "plain_build_ms": round(plain_build_ms, 3),
+            "plain_build_mtris_per_s": round(n_tris / plain_build_ms / 1e3, 2),
+            "plain_trace_mrays_per_s": round(n_rays / plain_trace_s / 1e6, 2),
+            "trace_speedup_vs_plain": round(plain_trace_s / trace_s, 2),
+            "build_speedup_vs_plain": round(plain_build_ms / build_ms, 2),
+            "hits_identical_to_plain": same, "ploc_passes": info["ploc_passes"],
+            "cwbvh_nodes": info["total_nodes"], "hit_fraction": round(float(np.isfinite(batch_hits["t"]).mean()), 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +161,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 20)
+    ap.add_argument("--batch-meshes", type=int, default=4096, help="0 skips the batched leg")
+    ap.add_argument("--batch-rays-per-mesh", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -164,6 +267,8 @@ def main():
                   "all_ms": [round(x, 3) for x in build_ms], "ploc_passes": int(stats.ploc_iterations),
                   "ploc_cluster_sum": int(stats.ploc_cluster_sum), "cwbvh_nodes": int(stats.cwbvh_node_count)},
     }
+    if args.batch_meshes > 0:
+        out["batched"] = bench_batch(args, ob, ctx)
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
         log(f"cpu baseline with {threads} threads")

@@ -8,6 +8,8 @@ All randomness comes from the crate's integer hash (test_util.rs:9-46) so inputs
                         frequencies in [1,32], phases in [0,2pi), amplitudes proportional to 1/frequency, plus seeded
                         12-triangle boxes resting on the surface.
 * bounce_rays        -- one cosine-weighted diffuse bounce per primary hit (geometric normal, seeded).
+* thin_triangles     -- long thin triangles (aspect ratio >= 50) scattered over the unit square, to exercise
+                        spatial pre-splits.
 * icosphere_batch    -- many independent icosphere meshes (counts log-uniform, rounded down to 20*4^s), vertex
                         radius jitter +-10% keyed on the vertex position (shared vertices move together), and a
                         ray batch per mesh from a radius-2 sphere toward uniform points in the mesh AABB.
@@ -154,6 +156,26 @@ def bounce_rays(rays, hits, bvh_tris, seed=0, eps=1e-4, tmax=1e30):
 
 def empty_rays(n):
     return np.zeros(n, dtype=RAY)
+
+
+def thin_triangles(n, seed=0, min_len=0.05, max_len=0.5):
+    """n long thin triangles over [0,1]^2 (heights in [0, 0.3]): length in [min_len, max_len], random direction in
+    3D, base width length / 50 .. length / 200 (aspect ratio >= 50)."""
+    i = np.arange(n, dtype=np.uint32)
+    f = np.uint32(seed) + np.uint32(7000)
+    u = [hash_noise(i, a, f).astype(np.float64) for a in range(8)]
+    start = np.stack([u[0], u[1] * 0.3, u[2]], axis=1)
+    length = min_len + (max_len - min_len) * u[3]
+    d = uniform_sample_sphere(u[4].astype(np.float32), u[5].astype(np.float32)).astype(np.float64)
+    end = start + d * length[:, None]
+    # a unit vector perpendicular to d for the base
+    helper = np.where(np.abs(d[:, :1]) < 0.9, np.array([[1.0, 0.0, 0.0]]), np.array([[0.0, 1.0, 0.0]]))
+    perp = np.cross(d, helper)
+    perp /= np.linalg.norm(perp, axis=1, keepdims=True)
+    width = length / (50.0 + 150.0 * u[6])
+    v0 = start - perp * (width / 2)[:, None]
+    v1 = start + perp * (width / 2)[:, None]
+    return triangles_from_vertices(v0.astype(np.float32), v1.astype(np.float32), end.astype(np.float32))
 
 
 ICO_MAX_SUBDIV = 5

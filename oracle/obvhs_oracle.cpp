@@ -94,6 +94,10 @@ enum {
     MUT_HALF_AREA = 8,         // (dx + dy) * dz + dx * dx
     MUT_PLOC_SWAP_CHILDREN = 9,  // left/right children of a merge swapped
     MUT_PLOC_SLOT_ORDER = 10,  // merges of one pass take child slots in ascending instead of descending order
+    MUT_SPLIT_UNFUSED = 11,    // split crossing point with a separate multiply and add instead of mul_add
+    MUT_REINS_PUSH_ORDER = 12,  // reinsertion search pushes the right child before the left one
+    MUT_REINS_NO_CONFLICT = 13,  // reinsertions applied without the conflict check
+    MUT_REFIT_FULL = 14,       // refit_from_fast never stops early
 };
 int g_mut = MUT_NONE;
 
@@ -873,6 +877,337 @@ inline void traverse_one(const CwBvhNode* nodes, uint32_t nnodes, const Triangle
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------------------------------------
+// Spatial pre-splits (splits.rs) and the pre-split leaf set of build_cwbvh_from_tris (cwbvh/builder.rs:26-55)
+// ---------------------------------------------------------------------------------------------------------------
+
+inline float vget(const V3& v, uint32_t d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
+inline void vset(V3& v, uint32_t d, float f) {
+    if (d == 0) v.x = f;
+    else if (d == 1) v.y = f;
+    else v.z = f;
+}
+// aabb.rs:125-134
+inline uint32_t largest_axis(const Aabb& a) {
+    const float dx = a.max.x - a.min.x, dy = a.max.y - a.min.y, dz = a.max.z - a.min.z;
+    if (dx < dy) return dy < dz ? 2u : 1u;
+    return dx < dz ? 2u : 0u;
+}
+// aabb.rs:98-103: min = self.min.max(other.min), max = self.max.min(other.max)
+inline Aabb aabb_intersection(const Aabb& a, const Aabb& b) { return Aabb{vmax(a.min, b.min), vmin(a.max, b.max)}; }
+// aabb.rs:23-26
+inline Aabb aabb_invalid() {
+    const float M = 3.40282347e+38f;
+    return Aabb{v3(M, M, M), v3(-M, -M, -M)};
+}
+
+// splits.rs:137-158.  The crossing point is Vec3A::mul_add(splat(t), v1 - v0, v0), a fused multiply-add per lane.
+void split_triangle(uint32_t dim, float pos, const V3 v[4], Aabb& left, Aabb& right) {
+    left = aabb_invalid();
+    right = aabb_invalid();
+    for (int i = 0; i < 3; ++i) {
+        const V3 v0 = v[i], v1 = v[i + 1];
+        const float v0d = vget(v0, dim), v1d = vget(v1, dim);
+        if (v0d <= pos) aabb_extend(left, v0);
+        if (v0d >= pos) aabb_extend(right, v0);
+        if ((v0d < pos && pos < v1d) || (v1d < pos && pos < v0d)) {
+            const float inv_length = 1.0f / (v1d - v0d);
+            const float t = (pos - v0d) * inv_length;
+            const V3 e = vsub(v1, v0);
+            V3 c;
+            if (g_mut == MUT_SPLIT_UNFUSED)
+                c = v3(t * e.x + v0.x, t * e.y + v0.y, t * e.z + v0.z);
+            else
+                c = v3(std::fmaf(t, e.x, v0.x), std::fmaf(t, e.y, v0.y), std::fmaf(t, e.z, v0.z));
+            aabb_extend(left, c);
+            aabb_extend(right, c);
+        }
+    }
+}
+
+// splits.rs:48-118.  `max_total` (0 = unlimited) is an extension: once the leaf list holds max_total entries no
+// further split is applied (candidates are still visited in the reference order, so the first splits are the
+// reference's).
+void split_aabbs_precise(std::vector<Aabb>& aabbs, std::vector<uint32_t>& indices, const Triangle* tris,
+                         float area_thresh_low, float area_thresh_high, float split_factor_low,
+                         float split_factor_high, uint32_t max_iterations, uint32_t split_tests, size_t max_total) {
+    std::vector<size_t> candidates;
+    for (size_t i = 0; i < aabbs.size(); ++i)
+        if (half_area(aabbs[i]) > area_thresh_low) candidates.push_back(i);
+    size_t old_len = candidates.size();
+    for (uint32_t it = 0; it < max_iterations; ++it) {
+        const size_t count = candidates.size();
+        for (size_t ci = 0; ci < count; ++ci) {
+            const size_t c = candidates[ci];
+            const Aabb aabb = aabbs[c];
+            const uint32_t index = indices[c];
+            const uint32_t axis = largest_axis(aabb);
+            const Triangle& tri = tris[index];
+            float best_cost = 3.40282347e+38f;
+            Aabb left = aabb, right = aabb;
+            for (uint32_t k = 1; k < split_tests; ++k) {
+                const float n = (float)k / (float)split_tests;
+                const float pos = vget(aabb.min, axis) * n + vget(aabb.max, axis) * (1.0f - n);
+                Aabb tl = aabb, tr = aabb;
+                vset(tl.max, axis, pos);
+                vset(tr.min, axis, pos);
+                const V3 verts[4] = {tri.v0, tri.v1, tri.v2, tri.v0};
+                Aabb sl, sr;
+                split_triangle(axis, pos, verts, sl, sr);
+                tl = aabb_intersection(sl, tl);
+                tr = aabb_intersection(sr, tr);
+                const float area = half_area(tl) + half_area(tr);
+                if (area < best_cost) {
+                    best_cost = area;
+                    left = tl;
+                    right = tr;
+                }
+            }
+            const float old_cost = half_area(aabb);
+            if ((area_thresh_high > old_cost && best_cost * split_factor_high < old_cost) ||
+                best_cost * split_factor_low < old_cost) {
+                if (max_total != 0 && aabbs.size() >= max_total) continue;
+                aabbs[c] = left;
+                candidates.push_back(aabbs.size());
+                aabbs.push_back(right);
+                indices.push_back(index);
+            }
+        }
+        if (old_len == candidates.size()) break;
+        size_t w = 0;
+        for (size_t ci = 0; ci < candidates.size(); ++ci)
+            if (half_area(aabbs[candidates[ci]]) > area_thresh_low) candidates[w++] = candidates[ci];
+        candidates.resize(w);
+        old_len = candidates.size();
+    }
+}
+
+// cwbvh/builder.rs:26-46 (leaf boxes, largest and average half area, summed in triangle order) + splits.rs:16-33.
+void presplit_leaves(const Triangle* tris, size_t n, size_t max_total, std::vector<Aabb>& aabbs,
+                     std::vector<uint32_t>& indices, float* out_avg, float* out_largest) {
+    float largest = 0.0f, avg = 0.0f;
+    aabbs.resize(n);
+    indices.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        aabbs[i] = tri_aabb(tris[i]);
+        const float ha = half_area(aabbs[i]);
+        largest = std::fmax(ha, largest);  // f32::max (IEEE maxNum)
+        avg += ha;
+        indices[i] = (uint32_t)i;
+    }
+    avg /= (float)n;
+    if (out_avg) *out_avg = avg;
+    if (out_largest) *out_largest = largest;
+    split_aabbs_precise(aabbs, indices, tris, avg * 3.0f, std::fmax(avg * 4.0f, avg * 0.9f + largest * 0.1f), 1.8f,
+                        1.6f, 12, 12, max_total);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Reinsertion optimizer (bvh2/reinsertion.rs) on a Bvh2 with parents (bvh2/mod.rs:572-619, 722-751)
+// ---------------------------------------------------------------------------------------------------------------
+
+inline size_t sibling_id(size_t id) { return (id % 2 == 1) ? id + 1 : id - 1; }      // bvh2/node.rs:158-164
+inline size_t left_sibling_id(size_t id) { return (id % 2 == 1) ? id : id - 1; }     // bvh2/node.rs:166-172
+
+// bvh2/mod.rs:586-609 (compute_parents)
+void compute_parents(const std::vector<Bvh2Node>& nodes, std::vector<uint32_t>& parents) {
+    parents.assign(nodes.size(), 0u);
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        if (nodes[i].prim_count == 0) {
+            parents[nodes[i].first_index] = (uint32_t)i;
+            parents[nodes[i].first_index + 1] = (uint32_t)i;
+        }
+    }
+}
+
+struct Reinsertion {
+    uint32_t from, to;
+    float area_diff;
+};
+
+// bvh2/mod.rs:722-751 (refit_from_fast, release build: stops at the second unchanged node)
+// A corrupted tree (only reachable through a planted mistake) could hold a parent cycle: the walks below are bounded
+// by the node count and report it through g_tree_corrupt instead of looping.
+bool g_tree_corrupt = false;
+
+void refit_from_fast(std::vector<Bvh2Node>& nodes, const std::vector<uint32_t>& parents, size_t index) {
+    int same_count = 0;
+    for (size_t steps = 0;; ++steps) {
+        if (steps > nodes.size()) {
+            g_tree_corrupt = true;
+            return;
+        }
+        Bvh2Node& node = nodes[index];
+        if (node.prim_count == 0) {
+            const Aabb nb = aabb_union(nodes[node.first_index].aabb, nodes[node.first_index + 1].aabb);
+            if (aabb_eq(node.aabb, nb)) {
+                same_count += 1;
+                if (same_count == 2 && g_mut != MUT_REFIT_FULL) return;
+            }
+            node.aabb = nb;
+        }
+        if (index == 0) break;
+        index = parents[index];
+    }
+}
+
+// reinsertion.rs:247-340 (find_reinsertion)
+Reinsertion find_reinsertion(const std::vector<Bvh2Node>& nodes, const std::vector<uint32_t>& parents,
+                             size_t node_id, std::vector<std::pair<float, uint32_t>>& stack) {
+    Reinsertion best{(uint32_t)node_id, 0u, 0.0f};
+    const float node_area = half_area(nodes[node_id].aabb);
+    const float parent_area = half_area(nodes[parents[node_id]].aabb);
+    float area_diff = parent_area;
+    size_t sib = sibling_id(node_id);
+    Aabb pivot_bbox = nodes[sib].aabb;
+    const size_t parent_id = parents[node_id];
+    size_t pivot_id = parent_id;
+    const Aabb aabb = nodes[node_id].aabb;
+    stack.clear();
+    size_t budget = 4 * nodes.size() + 64;
+    for (;;) {
+        stack.push_back({area_diff, (uint32_t)sib});
+        while (!stack.empty()) {
+            if (budget-- == 0) {
+                g_tree_corrupt = true;
+                return Reinsertion{0u, 0u, 0.0f};
+            }
+            const auto top = stack.back();
+            stack.pop_back();
+            if (top.first - node_area <= best.area_diff) continue;
+            const Bvh2Node& dst = nodes[top.second];
+            const float merged_area = half_area(aabb_union(dst.aabb, aabb));
+            const float reinsert_area = top.first - merged_area;
+            if (reinsert_area > best.area_diff) {
+                best.to = top.second;
+                best.area_diff = reinsert_area;
+            }
+            if (dst.prim_count == 0) {
+                const float child_area = reinsert_area + half_area(dst.aabb);
+                if (g_mut == MUT_REINS_PUSH_ORDER) {
+                    stack.push_back({child_area, dst.first_index + 1});
+                    stack.push_back({child_area, dst.first_index});
+                } else {
+                    stack.push_back({child_area, dst.first_index});
+                    stack.push_back({child_area, dst.first_index + 1});
+                }
+            }
+        }
+        if (pivot_id != parent_id) {
+            pivot_bbox = aabb_union(pivot_bbox, nodes[sib].aabb);
+            area_diff += half_area(nodes[pivot_id].aabb) - half_area(pivot_bbox);
+        }
+        if (pivot_id == 0) break;
+        sib = sibling_id(pivot_id);
+        pivot_id = parents[pivot_id];
+    }
+    if (best.to == sibling_id(best.from) || best.to == parents[best.from]) best = Reinsertion{0u, 0u, 0.0f};
+    return best;
+}
+
+// reinsertion.rs:342-382 (reinsert_node; primitives_to_nodes is not kept here)
+void reinsert_node(std::vector<Bvh2Node>& nodes, std::vector<uint32_t>& parents, size_t from, size_t to) {
+    const size_t sib = sibling_id(from);
+    const size_t parent_id = parents[from];
+    const Bvh2Node sibling_node = nodes[sib];
+    const Bvh2Node dst_node = nodes[to];
+    nodes[to].prim_count = 0;  // make_inner (bvh2/node.rs:211-214)
+    nodes[to].first_index = (uint32_t)left_sibling_id(from);
+    nodes[sib] = dst_node;
+    nodes[parent_id] = sibling_node;
+    if (nodes[sib].prim_count == 0) {
+        parents[nodes[sib].first_index] = (uint32_t)sib;
+        parents[nodes[sib].first_index + 1] = (uint32_t)sib;
+    }
+    if (nodes[parent_id].prim_count == 0) {
+        parents[nodes[parent_id].first_index] = (uint32_t)parent_id;
+        parents[nodes[parent_id].first_index + 1] = (uint32_t)parent_id;
+    }
+    parents[sib] = (uint32_t)to;
+    parents[from] = (uint32_t)to;
+    refit_from_fast(nodes, parents, to);
+    refit_from_fast(nodes, parents, parent_id);
+}
+
+// The candidate order key (reinsertion.rs:210-218): rdst's f32 radix key of -cost (sign-flip total order),
+// ascending.  rdst's sort is unstable; equal keys are ordered by node index here.
+inline uint32_t rdst_f32_key(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return (b >> 31) ? ~b : (b ^ 0x80000000u);
+}
+
+// reinsertion.rs:40-61 (run), 101-119 (optimize_impl), 128-147 (find_candidates), 149-204 (optimize_candidates).
+// Ties of the two unstable sorts (candidates by cost, reinsertions by area_diff) are broken by node index and by
+// candidate rank respectively.
+void reinsertion_run(std::vector<Bvh2Node>& nodes, float batch_size_ratio, const float* seq, size_t seq_len) {
+    if (nodes.empty() || nodes[0].prim_count != 0 || !(batch_size_ratio > 0.0f)) return;
+    std::vector<uint32_t> parents;
+    compute_parents(nodes, parents);
+    std::vector<float> default_seq;
+    if (!seq) {
+        for (int k = 1; k < 32; k += 2) default_seq.push_back(1.0f / (float)k);
+        seq = default_seq.data();
+        seq_len = default_seq.size();
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> cand;  // (key, node id)
+    std::vector<std::pair<float, uint32_t>> stack;
+    std::vector<Reinsertion> reins;
+    std::vector<uint8_t> touched(nodes.size());
+    g_tree_corrupt = false;
+    for (size_t si = 0; si < seq_len && !g_tree_corrupt; ++si) {
+        const float prod = ((float)nodes.size() * batch_size_ratio) * seq[si];
+        size_t batch_size = prod >= 18446744073709551615.0f ? SIZE_MAX : (prod > 0.0f ? (size_t)prod : 0);
+        batch_size = std::max<size_t>(batch_size, 1);
+        const size_t node_count = std::min(nodes.size(), batch_size + 1);
+        // find_candidates
+        cand.clear();
+        const size_t take = std::min(nodes.size(), node_count * 2);
+        for (size_t i = 1; i < take; ++i) cand.push_back({rdst_f32_key(-half_area(nodes[i].aabb)), (uint32_t)i});
+        std::sort(cand.begin(), cand.end());
+        // optimize_candidates(node_count - 1)
+        const size_t count = node_count - 1;
+        std::fill(touched.begin(), touched.end(), 0);
+        reins.clear();
+        for (size_t i = 0; i < count && i < cand.size(); ++i) {
+            Reinsertion r = find_reinsertion(nodes, parents, cand[i].second, stack);
+            if (r.area_diff > 0.0f) reins.push_back(r);
+        }
+        std::stable_sort(reins.begin(), reins.end(),
+                         [](const Reinsertion& a, const Reinsertion& b) { return a.area_diff > b.area_diff; });
+        for (const Reinsertion& r : reins) {
+            const size_t conf[5] = {r.to, r.from, sibling_id(r.from), parents[r.to], parents[r.from]};
+            bool any = false;
+            if (g_mut != MUT_REINS_NO_CONFLICT)
+                for (size_t c : conf) any |= touched[c] != 0;
+            if (any) continue;
+            for (size_t c : conf) touched[c] = 1;
+            reinsert_node(nodes, parents, r.from, r.to);
+            if (g_tree_corrupt) break;
+        }
+    }
+}
+
+// bvh2/mod.rs:527-569 (refit_all, general order: reverse of a depth-first visit)
+void refit_all(std::vector<Bvh2Node>& nodes) {
+    if (nodes.empty()) return;
+    std::vector<uint32_t> stack{0u}, rev{0u};
+    while (!stack.empty()) {
+        const uint32_t c = stack.back();
+        stack.pop_back();
+        if (nodes[c].prim_count == 0) {
+            rev.push_back(nodes[c].first_index);
+            rev.push_back(nodes[c].first_index + 1);
+            stack.push_back(nodes[c].first_index);
+            stack.push_back(nodes[c].first_index + 1);
+        }
+    }
+    for (size_t k = rev.size(); k-- > 0;) {
+        Bvh2Node& nd = nodes[rev[k]];
+        if (nd.prim_count == 0) nd.aabb = aabb_union(nodes[nd.first_index].aabb, nodes[nd.first_index + 1].aabb);
+    }
+}
+
 // ===============================================================================================================
 // C ABI for the tests (ctypes)
 // ===============================================================================================================
@@ -952,6 +1287,64 @@ int orc_bvh2_to_cwbvh(const Bvh2Node* nodes, uint64_t nnodes, const uint32_t* pr
     return ORC_OK;
 }
 
+// Pre-split leaf set of build_cwbvh_from_tris with pre_split = true (cwbvh/builder.rs:26-55): out_aabbs and
+// out_indices have capacity `cap`; max_total = 0 follows the reference (no limit).
+int orc_presplit(const Triangle* tris, uint64_t n, uint64_t max_total, Aabb* out_aabbs, uint32_t* out_indices,
+                 uint64_t cap, uint64_t* out_count, float* out_avg, float* out_largest) {
+    std::vector<Aabb> aabbs;
+    std::vector<uint32_t> indices;
+    *out_count = 0;
+    if (n == 0) return ORC_OK;
+    presplit_leaves(tris, (size_t)n, (size_t)max_total, aabbs, indices, out_avg, out_largest);
+    if (aabbs.size() > cap) return ORC_ERR_CAPACITY;
+    std::memcpy(out_aabbs, aabbs.data(), aabbs.size() * sizeof(Aabb));
+    std::memcpy(out_indices, indices.data(), indices.size() * 4);
+    *out_count = aabbs.size();
+    return ORC_OK;
+}
+
+// split_triangle (splits.rs:137-158) for one triangle and plane.
+void orc_split_triangle(const Triangle* tri, uint32_t dim, float pos, Aabb* out_left, Aabb* out_right) {
+    const V3 v[4] = {tri->v0, tri->v1, tri->v2, tri->v0};
+    split_triangle(dim, pos, v, *out_left, *out_right);
+}
+
+// ReinsertionOptimizer::default().run(bvh, ratio, seq) in place on Bvh2Node[nnodes] (seq = NULL: default sequence).
+void orc_reinsertion(Bvh2Node* nodes, uint64_t nnodes, float ratio, const float* seq, uint64_t seq_len) {
+    std::vector<Bvh2Node> v(nodes, nodes + nnodes);
+    reinsertion_run(v, ratio, seq, (size_t)seq_len);
+    std::memcpy(nodes, v.data(), nnodes * sizeof(Bvh2Node));
+}
+
+// find_reinsertion (reinsertion.rs:247-340) for one node of a Bvh2 (parents computed here).
+void orc_find_reinsertion(const Bvh2Node* nodes, uint64_t nnodes, uint32_t node_id, uint32_t* out_from,
+                          uint32_t* out_to, float* out_area_diff) {
+    std::vector<Bvh2Node> v(nodes, nodes + nnodes);
+    std::vector<uint32_t> parents;
+    compute_parents(v, parents);
+    std::vector<std::pair<float, uint32_t>> stack;
+    const Reinsertion r = find_reinsertion(v, parents, node_id, stack);
+    *out_from = r.from;
+    *out_to = r.to;
+    *out_area_diff = r.area_diff;
+}
+
+// reinsert_node (reinsertion.rs:342-382) in place (parents computed here).
+void orc_reinsert_node(Bvh2Node* nodes, uint64_t nnodes, uint32_t from, uint32_t to) {
+    std::vector<Bvh2Node> v(nodes, nodes + nnodes);
+    std::vector<uint32_t> parents;
+    compute_parents(v, parents);
+    reinsert_node(v, parents, from, to);
+    std::memcpy(nodes, v.data(), nnodes * sizeof(Bvh2Node));
+}
+
+// Bvh2::refit_all (bvh2/mod.rs:527-569) in place.
+void orc_refit_all(Bvh2Node* nodes, uint64_t nnodes) {
+    std::vector<Bvh2Node> v(nodes, nodes + nnodes);
+    refit_all(v);
+    std::memcpy(nodes, v.data(), nnodes * sizeof(Bvh2Node));
+}
+
 // Closest-hit (mode 0) or miss (mode 1) traversal of every ray.  `tris` is indexed by primitive_indices
 // position (the reference's examples reorder triangles this way: obj_cwbvh.rs:63-67).
 // Optional per-ray visit counters (node tests and triangle tests).  threads <= 1 runs serially.
@@ -1014,11 +1407,14 @@ uint32_t orc_ray_octant_inv4(const Ray* ray) { return ray_get_octant_inv4(ray->d
 
 // ----- validation (bvh2/mod.rs:786-981; cwbvh/mod.rs:747-908), restated as checks returning a failure code -----
 
-// Returns 0 when valid; otherwise a positive code naming the first failed check.  `tight_fit` as the reference.
-// Also reports the max depth and whether children are stored after parents.
+// Returns 0 when valid; otherwise a positive code naming the first failed check.  flags: bit 0 = tight_fit as
+// the reference; bit 1 = children need not be stored after parents (after reinsertion, reinsertion.rs:102);
+// bit 2 = spatial splits (a primitive may sit in several leaves whose boxes are parts of its box:
+// bvh2/mod.rs:823, 928, 945 skip those checks).  Also reports the max depth.
 int orc_bvh2_validate(const Bvh2Node* nodes, uint64_t nnodes, const uint32_t* prim_indices, uint64_t nprims,
-                      const Aabb* prim_aabbs, int tight_fit, uint32_t* out_max_depth) {
+                      const Aabb* prim_aabbs, int flags, uint32_t* out_max_depth) {
     if (nnodes == 0) return 0;
+    const bool tight_fit = (flags & 1) != 0, unordered = (flags & 2) != 0, splits = (flags & 4) != 0;
     std::vector<uint8_t> seen_node(nnodes, 0);
     std::vector<uint8_t> seen_prim(nprims, 0);
     uint64_t node_count = 0, prim_count = 0;
@@ -1054,29 +1450,31 @@ int orc_bvh2_validate(const Bvh2Node* nodes, uint64_t nnodes, const uint32_t* pr
                 if (slot >= nprims) return 14;
                 uint32_t p = prim_indices[slot];
                 if (p >= nprims) return 15;
+                prim_count++;
+                if (splits) continue;
                 if (seen_prim[p]) return 16;
                 seen_prim[p] = 1;
-                prim_count++;
                 const Aabb& pb = prim_aabbs[p];
                 tmp = aabb_union(tmp, pb);
                 if (!(pb.min.x >= nd.aabb.min.x && pb.min.y >= nd.aabb.min.y && pb.min.z >= nd.aabb.min.z &&
                       pb.max.x <= nd.aabb.max.x && pb.max.y <= nd.aabb.max.y && pb.max.z <= nd.aabb.max.z))
                     return 17;  // bvh2/mod.rs:944-954
             }
-            if (tight_fit && !aabb_eq(tmp, nd.aabb)) return 18;  // bvh2/mod.rs:957-963
+            if (tight_fit && !splits && !aabb_eq(tmp, nd.aabb)) return 18;  // bvh2/mod.rs:957-963
         } else {
             if (tight_fit) {
                 Aabb u = aabb_union(nodes[nd.first_index].aabb, nodes[nd.first_index + 1].aabb);
                 if (!aabb_eq(u, nd.aabb)) return 19;  // bvh2/mod.rs:965-976
             }
-            st.push_back({nd.first_index + 1, it.parent, it.depth + 1});
-            st.push_back({nd.first_index, it.parent, it.depth + 1});
+            st.push_back({nd.first_index + 1, it.node, it.depth + 1});
+            st.push_back({nd.first_index, it.node, it.depth + 1});
         }
     }
     if (node_count != nnodes) return 20;  // bvh2/mod.rs:816-817
     if (prim_count != nprims) return 21;  // bvh2/mod.rs:825-827
-    for (uint64_t i = 1; i < nnodes; ++i)
-        if (!(parents[i] < i)) return 22;  // bvh2/mod.rs:861-874 children ordered after parents
+    if (!unordered)
+        for (uint64_t i = 1; i < nnodes; ++i)
+            if (!(parents[i] < i)) return 22;  // bvh2/mod.rs:861-874 children ordered after parents
     if (out_max_depth) *out_max_depth = max_depth;
     return 0;
 }

@@ -61,6 +61,13 @@ def lib():
         L.orc_log2f.restype = C.c_float
         L.orc_log2f.argtypes = [C.c_float]
         L.orc_set_mutation.argtypes = [C.c_int]
+        L.orc_presplit.restype = C.c_int
+        L.orc_presplit.argtypes = [P, C.c_uint64, C.c_uint64, P, P, C.c_uint64, P, P, P]
+        L.orc_split_triangle.argtypes = [P, C.c_uint32, C.c_float, P, P]
+        L.orc_reinsertion.argtypes = [P, C.c_uint64, C.c_float, P, C.c_uint64]
+        L.orc_find_reinsertion.argtypes = [P, C.c_uint64, C.c_uint32, P, P, P]
+        L.orc_reinsert_node.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32]
+        L.orc_refit_all.argtypes = [P, C.c_uint64]
         _lib = L
     return _lib
 
@@ -141,11 +148,12 @@ def brute_force(tris, rays):
     return out
 
 
-def bvh2_validate(nodes, prim_indices, prim_aabbs, tight_fit=True):
+def bvh2_validate(nodes, prim_indices, prim_aabbs, tight_fit=True, ordered=True, splits=False):
     md = C.c_uint32()
+    flags = int(bool(tight_fit)) | (0 if ordered else 2) | (4 if splits else 0)
     rc = lib().orc_bvh2_validate(_ptr(np.ascontiguousarray(nodes)), len(nodes),
                                  _ptr(np.ascontiguousarray(prim_indices)), len(prim_indices),
-                                 _ptr(np.ascontiguousarray(prim_aabbs)), int(tight_fit), C.byref(md))
+                                 _ptr(np.ascontiguousarray(prim_aabbs)), flags, C.byref(md))
     return rc, md.value
 
 
@@ -157,9 +165,66 @@ def cwbvh_validate(cw_nodes, prim_indices, prim_aabbs):
     return rc, md.value
 
 
-def build_cwbvh(tris, search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3):
-    """Oracle build_cwbvh_from_tris without pre-splits/reinsertion (cwbvh/builder.rs:61-74)."""
-    aabbs = tri_aabbs(tris)
-    b = ploc_build(aabbs, search_distance, search_depth_threshold)
+def presplit(tris, max_total=0):
+    """Leaf boxes and primitive indices after the pre-splits of build_cwbvh_from_tris (cwbvh/builder.rs:26-55)."""
+    from obvhs_amd.types import AABB
+    n = len(tris)
+    cap = max(1, n * 8 if max_total == 0 else max_total)
+    aabbs = np.zeros(cap, dtype=AABB)
+    idx = np.zeros(cap, dtype=np.uint32)
+    cnt, avg, big = C.c_uint64(), C.c_float(), C.c_float()
+    rc = lib().orc_presplit(_ptr(np.ascontiguousarray(tris)), n, max_total, _ptr(aabbs), _ptr(idx), cap,
+                            C.byref(cnt), C.byref(avg), C.byref(big))
+    if rc != 0:
+        raise OracleError(rc)
+    return aabbs[:cnt.value].copy(), idx[:cnt.value].copy(), avg.value, big.value
+
+
+def split_triangle(tri, dim, pos):
+    from obvhs_amd.types import AABB
+    left, right = np.zeros(1, dtype=AABB), np.zeros(1, dtype=AABB)
+    lib().orc_split_triangle(_ptr(np.ascontiguousarray(tri)), dim, pos, _ptr(left), _ptr(right))
+    return left[0], right[0]
+
+
+def reinsertion(nodes, ratio, seq=None):
+    """ReinsertionOptimizer::default().run(bvh, ratio, seq) on a copy of the nodes (reinsertion.rs:40-61)."""
+    out = np.ascontiguousarray(nodes).copy()
+    s = None if seq is None else np.ascontiguousarray(seq, dtype=np.float32)
+    lib().orc_reinsertion(_ptr(out), len(out), ratio, _ptr(s), 0 if s is None else len(s))
+    return out
+
+
+def find_reinsertion(nodes, node_id):
+    f, t, a = C.c_uint32(), C.c_uint32(), C.c_float()
+    lib().orc_find_reinsertion(_ptr(np.ascontiguousarray(nodes)), len(nodes), node_id, C.byref(f), C.byref(t),
+                               C.byref(a))
+    return f.value, t.value, a.value
+
+
+def reinsert_node(nodes, frm, to):
+    out = np.ascontiguousarray(nodes).copy()
+    lib().orc_reinsert_node(_ptr(out), len(out), frm, to)
+    return out
+
+
+def refit_all(nodes):
+    out = np.ascontiguousarray(nodes).copy()
+    lib().orc_refit_all(_ptr(out), len(out))
+    return out
+
+
+def build_cwbvh(tris, search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3, pre_split=False,
+                reinsertion_batch_ratio=0.0, ratio_sequence=None, split_max_total=0):
+    """Oracle build_cwbvh_from_tris (cwbvh/builder.rs:20-85): [pre-splits] -> PLOC -> [reinsertion] ->
+    bvh2_to_cwbvh(.., clamp(max_prims, 1, 3), true, false).  Returns (bvh2 dict, cw nodes, cw primitive indices);
+    the dict's "nodes" are the BVH2 after reinsertion."""
+    if pre_split and len(tris):
+        aabbs, idx, _, _ = presplit(tris, split_max_total)
+    else:
+        aabbs, idx = tri_aabbs(tris), None
+    b = ploc_build(aabbs, search_distance, search_depth_threshold, indices=idx)
+    if reinsertion_batch_ratio > 0.0 and len(b["nodes"]):
+        b["nodes"] = reinsertion(b["nodes"], reinsertion_batch_ratio, ratio_sequence)
     cw, prims, _ = bvh2_to_cwbvh(b["nodes"], b["prim_indices"], max(1, min(3, max_prims_per_leaf)), True)
     return b, cw, prims

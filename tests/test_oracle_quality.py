@@ -1,0 +1,224 @@
+"""Pins of the oracle's quality passes: spatial pre-splits (splits.rs), the reinsertion optimizer
+(bvh2/reinsertion.rs) and refit_all (bvh2/mod.rs:527-569).
+
+What pins them (none of these re-types the oracle's formulas):
+  * split_triangle: hand-derived clipped boxes; the crossing point is the correctly rounded fused multiply-add
+    (checked with exact rational arithmetic), which a separate multiply and add misses;
+  * pre-splits: every piece lies inside its triangle's box, the pieces of one triangle cover the part of the
+    triangle's box the triangle touches, and rays traced through the split CWBVH hit exactly what brute force over
+    the unsplit triangles hits;
+  * find_reinsertion: its area_diff equals the measured decrease of the summed half areas after the move (apply it,
+    refit everything, re-sum), and no other legal target decreases them by more (brute force over all targets);
+  * the optimizer: trees stay valid and tight, the summed half area never grows, a dropped conflict check breaks
+    the tree, rays hit what brute force hits;
+  * refit_all: after a seeded vertex jitter every internal box is the union of its children (tight-fit validator).
+Tie orders of the reference's unstable sorts (candidates by area, reinsertions by area_diff) stay unpinned: the
+oracle orders ties by node index / candidate rank."""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from obvhs_amd import scenes, test_util
+from obvhs_amd.types import AABB, make_rays, triangles_from_vertices
+
+
+@pytest.fixture
+def mutation():
+    yield lambda m: O.lib().orc_set_mutation(m)
+    O.lib().orc_set_mutation(0)
+
+
+def box(a):
+    return np.array(a["min"][:3], dtype=np.float32), np.array(a["max"][:3], dtype=np.float32)
+
+
+def test_split_triangle_hand_derived():
+    tri = triangles_from_vertices([[0, 0, 0]], [[2, 0, 0]], [[0, 2, 0]])[0]
+    left, right = O.split_triangle(tri, 0, 1.0)
+    # edge v0->v1 crosses x=1 at (1,0,0); edge v1->v2 at (1,1,0); v2->v0 lies in x=0
+    np.testing.assert_array_equal(box(left)[0], [0, 0, 0])
+    np.testing.assert_array_equal(box(left)[1], [1, 2, 0])
+    np.testing.assert_array_equal(box(right)[0], [1, 0, 0])
+    np.testing.assert_array_equal(box(right)[1], [2, 1, 0])
+    # a plane that misses the triangle leaves one side empty (Aabb::INVALID)
+    left, right = O.split_triangle(tri, 1, 5.0)
+    assert box(right)[0][0] == np.float32(3.4028235e38) and box(right)[1][0] == np.float32(-3.4028235e38)
+
+
+def _f32_round(fr):
+    return np.float32(float(fr))
+
+
+@pytest.mark.parametrize("planted", [0, 11])
+def test_split_crossing_point_is_fused(mutation, planted):
+    """splits.rs:151: Vec3A::mul_add(splat(t), v1 - v0, v0) is one rounding per lane.  Triangles with v0 left of
+    the plane and v1, v2 right of it, v1 above and v2 below v0 in y and z: the left piece's box is spanned by v0 and
+    the crossings of edges v0->v1 (its max) and v2->v0 (its min)."""
+    mutation(planted)
+    rng = np.random.default_rng(5)
+    mismatches = 0
+
+    def crossing(a, b, pos):  # edge a -> b as split_triangle walks it
+        inv = np.float32(1.0) / (b[0] - a[0])
+        t = np.float32((pos - a[0]) * inv)
+        e = (b - a).astype(np.float32)
+        return [_f32_round(Fraction(float(t)) * Fraction(float(e[k])) + Fraction(float(a[k]))) for k in range(3)]
+
+    for _ in range(300):
+        v0 = np.array([rng.uniform(-1, 0), rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1)], dtype=np.float32)
+        v1 = np.array([rng.uniform(1, 2), rng.uniform(0.5, 3), rng.uniform(0.5, 3)], dtype=np.float32)
+        v2 = np.array([rng.uniform(1, 2), rng.uniform(-3, -0.5), rng.uniform(-3, -0.5)], dtype=np.float32)
+        pos = np.float32(v0[0] + (np.float32(1.0) - v0[0]) * np.float32(rng.uniform(0.05, 0.95)))
+        tri = triangles_from_vertices(v0[None], v1[None], v2[None])[0]
+        left, _ = O.split_triangle(tri, 0, float(pos))
+        lo, hi = box(left)
+        c01, c20 = crossing(v0, v1, pos), crossing(v2, v0, pos)
+        if any(hi[k] != c01[k] or lo[k] != c20[k] for k in (1, 2)):
+            mismatches += 1
+    if planted:
+        assert mismatches > 0
+    else:
+        assert mismatches == 0
+
+
+def test_presplit_pieces_and_traversal():
+    tris = np.concatenate([scenes.heightfield_scene(48, 100, seed=2), scenes.thin_triangles(300, seed=2)])
+    aabbs, idx, avg, big = O.presplit(tris)
+    assert len(aabbs) > len(tris) + 100  # thin triangles do split
+    assert (idx[:len(tris)] == np.arange(len(tris))).all()
+    tb = O.tri_aabbs(tris)
+    tmin, tmax = tb["min"][idx, :3], tb["max"][idx, :3]
+    pmin, pmax = aabbs["min"][:, :3], aabbs["max"][:, :3]
+    assert (pmin >= tmin).all() and (pmax <= tmax).all() and (pmin <= pmax).all()
+    # the pieces of a triangle together span its box along every axis they were not clipped on: at least the
+    # union of pieces reaches the triangle's extreme vertices
+    for t in np.unique(idx[len(tris):])[:50]:
+        sel = idx == t
+        umin, umax = pmin[sel].min(axis=0), pmax[sel].max(axis=0)
+        np.testing.assert_array_equal(umin, tb["min"][t, :3])
+        np.testing.assert_array_equal(umax, tb["max"][t, :3])
+    # end to end: closest hits through the split CWBVH equal brute force over the triangles
+    b, cw, prims = O.build_cwbvh(tris, 14, 0, 3, pre_split=True)
+    assert O.bvh2_validate(b["nodes"], b["prim_indices"], aabbs, True, splits=True)[0] == 0
+    rays = scenes.pinhole_rays(64, 64, eye=(0.5, 0.45, -0.35), target=(0.5, 0.0, 0.55))
+    h = O.traverse(cw, tris[prims], rays)
+    bf = O.brute_force(tris, rays)
+    np.testing.assert_array_equal(h["t"].view(np.uint32), bf["t"].view(np.uint32))
+    hit = np.isfinite(bf["t"])
+    assert hit.mean() > 0.5
+    assert (prims[h["primitive_id"][hit]] == bf["primitive_id"][hit]).mean() > 0.999
+
+
+def _sah(nodes):
+    d = nodes["max"][:, :3].astype(np.float64) - nodes["min"][:, :3].astype(np.float64)
+    ha = (d[:, 0] + d[:, 1]) * d[:, 2] + d[:, 0] * d[:, 1]
+    return ha[nodes["prim_count"] == 0].sum()
+
+
+def _parents(nodes):
+    p = np.zeros(len(nodes), dtype=np.int64)
+    for i in np.nonzero(nodes["prim_count"] == 0)[0]:
+        f = nodes["first_index"][i]
+        p[f] = p[f + 1] = i
+    return p
+
+
+def _subtree(nodes, root):
+    out, st = set(), [root]
+    while st:
+        c = st.pop()
+        out.add(c)
+        if nodes["prim_count"][c] == 0:
+            st += [int(nodes["first_index"][c]), int(nodes["first_index"][c]) + 1]
+    return out
+
+
+def test_find_reinsertion_matches_brute_force():
+    tris = scenes.soup(60, seed=4)
+    b = O.ploc_build(O.tri_aabbs(tris), 2, 0)
+    nodes = b["nodes"]
+    par = _parents(nodes)
+    base = _sah(nodes)
+    checked = 0
+    for frm in range(1, len(nodes)):
+        f, to, area = O.find_reinsertion(nodes, frm)
+        anc, c = set(), frm
+        while c != 0:
+            c = int(par[c])
+            anc.add(c)
+        sib = frm + 1 if frm % 2 == 1 else frm - 1
+        legal = [t for t in range(len(nodes))
+                 if t not in _subtree(nodes, frm) and t not in anc and t != sib and t != 0]
+        gains = {}
+        for t in legal:
+            moved = O.refit_all(O.reinsert_node(nodes, frm, t))
+            gains[t] = base - _sah(moved)
+        best_gain = max(gains.values()) if gains else 0.0
+        if area > 0:
+            assert f == frm
+            assert abs(gains[to] - area) <= 1e-4 * max(1.0, abs(area)), (frm, to, gains[to], area)
+            assert gains[to] >= best_gain - 1e-4 * max(1.0, best_gain)
+            checked += 1
+        else:
+            assert best_gain <= 1e-4, (frm, best_gain)
+    assert checked > 5
+
+
+@pytest.mark.parametrize("ratio", [0.01, 0.05, 0.3, 1.0])
+def test_reinsertion_keeps_tree_valid_and_lowers_sah(ratio):
+    tris = scenes.heightfield_scene(40, 60, seed=1)
+    aabbs = O.tri_aabbs(tris)
+    b = O.ploc_build(aabbs, 14, 0)
+    before = _sah(b["nodes"])
+    prev = before
+    after = None
+    for seq in ([1.0], [1.0, 1.0 / 3.0], None):
+        after = O.reinsertion(b["nodes"], ratio, seq)
+        rc, _ = O.bvh2_validate(after, b["prim_indices"], aabbs, True, ordered=False)
+        assert rc == 0
+        s = _sah(after)
+        assert s <= before + 1e-9
+        prev = s
+    assert prev < before
+    # determinism and traversal equivalence
+    again = O.reinsertion(b["nodes"], ratio, None)
+    assert again.tobytes() == after.tobytes()
+    cw, prims, _ = O.bvh2_to_cwbvh(after, b["prim_indices"], 3, True)
+    rays = scenes.pinhole_rays(48, 48, eye=(0.5, 0.6, -0.4), target=(0.5, 0.0, 0.5))
+    h = O.traverse(cw, tris[prims], rays)
+    bf = O.brute_force(tris, rays)
+    np.testing.assert_array_equal(h["t"].view(np.uint32), bf["t"].view(np.uint32))
+
+
+def test_reinsertion_without_conflict_check_breaks_tree(mutation):
+    tris = scenes.heightfield_scene(40, 60, seed=1)
+    aabbs = O.tri_aabbs(tris)
+    b = O.ploc_build(aabbs, 14, 0)
+    mutation(13)
+    bad = O.reinsertion(b["nodes"], 1.0, None)
+    mutation(0)
+    assert O.bvh2_validate(bad, b["prim_indices"], aabbs, True, ordered=False)[0] != 0
+
+
+def test_refit_all_after_jitter():
+    tris = scenes.soup(3000, seed=8)
+    b = O.ploc_build(O.tri_aabbs(tris), 14, 0)
+    nodes = O.reinsertion(b["nodes"], 0.2)
+    # jitter the vertices, write the new leaf boxes, refit
+    j = tris.copy()
+    for k in ("v0", "v1", "v2"):
+        j[k][:, :3] += (test_util.hash_noise(np.arange(len(j), dtype=np.uint32), 40 + ord(k[1]), 3)[:, None]
+                        - np.float32(0.5)) * np.float32(0.002)
+    ja = O.tri_aabbs(j)
+    leaves = np.nonzero(nodes["prim_count"] != 0)[0]
+    nodes = nodes.copy()
+    p = b["prim_indices"][nodes["first_index"][leaves]]
+    nodes["min"][leaves] = ja["min"][p]
+    nodes["max"][leaves] = ja["max"][p]
+    assert O.bvh2_validate(nodes, b["prim_indices"], ja, True, ordered=False)[0] != 0  # stale internal boxes
+    refit = O.refit_all(nodes)
+    assert O.bvh2_validate(refit, b["prim_indices"], ja, True, ordered=False)[0] == 0
+    untouched = refit["prim_count"] != 0
+    assert refit[untouched].tobytes() == nodes[untouched].tobytes()
