@@ -137,12 +137,46 @@ obvhs_status obvhs_bvh2_destroy(obvhs_bvh2* bvh);
 obvhs_status obvhs_bvh2_to_cwbvh(obvhs_context* ctx, const obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,
                                  int order_children, int include_exact_node_aabbs, obvhs_cwbvh* reuse,
                                  obvhs_cwbvh** out);
-/* Replaces build_cwbvh_from_tris (cwbvh/builder.rs:20-85): PLOC -> [reinsertion] -> bvh2_to_cwbvh(.., clamp(max
- * prims,1,3), true, false).  Pre-splits and reinsertion ratios other than 0 return OBVHS_ERR_INVALID_ARG in this
- * version.  d_tris is device memory.  stats may be NULL. */
+/* Replaces build_cwbvh_from_tris (cwbvh/builder.rs:20-85): [pre-splits] -> PLOC -> [reinsertion with the
+ * default ratio sequence] -> bvh2_to_cwbvh(.., clamp(max prims,1,3), true, false).  d_tris is device memory.
+ * stats may be NULL. */
 obvhs_status obvhs_build_cwbvh_from_tris(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
                                          const obvhs_build_params* params, obvhs_cwbvh* reuse, obvhs_cwbvh** out,
                                          obvhs_build_stats* stats);
+/* Options the crate's helper does not expose, used through its building blocks instead:
+ * split_max_total: stop splitting once this many leaves exist (0 = no limit, the crate's behaviour; splits are
+ * taken in the crate's candidate order, so the first ones are the crate's); ratio_sequence: the `ratio_sequence`
+ * argument of ReinsertionOptimizer::run (NULL = the default 1, 1/3, ..., 1/31; reinsertion.rs:40, 101-110). */
+typedef struct obvhs_build_ext {
+    uint64_t split_max_total;
+    const float* ratio_sequence;  /* host array */
+    uint32_t ratio_sequence_len;
+} obvhs_build_ext;
+obvhs_status obvhs_build_cwbvh_from_tris_ex(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
+                                            const obvhs_build_params* params, const obvhs_build_ext* ext,
+                                            obvhs_cwbvh* reuse, obvhs_cwbvh** out, obvhs_build_stats* stats);
+/* The BVH2 a CwBvh handle was built from by obvhs_build_cwbvh_from_tris (owned by the CwBvh handle; NULL for
+ * conversions).  Refit or reinsert it, then convert it again with obvhs_bvh2_to_cwbvh. */
+obvhs_status obvhs_cwbvh_bvh2(const obvhs_cwbvh* bvh, obvhs_bvh2** out);
+
+/* ---- BVH2 quality passes and refit ----
+ * ReinsertionOptimizer::default().run(bvh, batch_size_ratio, ratio_sequence) (bvh2/reinsertion.rs:40-61):
+ * ratio_sequence is a host array, NULL for the default.  Ties of the crate's two unstable sorts are ordered by
+ * node index (candidates) and by candidate rank (reinsertions). */
+obvhs_status obvhs_bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_size_ratio,
+                                    const float* ratio_sequence, uint32_t ratio_sequence_len);
+/* Bvh2::refit_all (bvh2/mod.rs:527-569). */
+obvhs_status obvhs_bvh2_refit_all(obvhs_context* ctx, obvhs_bvh2* bvh);
+/* The refit example of Bvh2::refit_all (bvh2/mod.rs:506-526): every one-primitive leaf takes the box of its
+ * triangle in d_tris (original order, device), then refit_all.  For BVH2s built without pre-splits. */
+obvhs_status obvhs_bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh, const obvhs_triangle* d_tris);
+/* The pre-split leaf set of build_cwbvh_from_tris with pre_split = true (cwbvh/builder.rs:26-55 +
+ * split_aabbs_preset, splits.rs:16-33): writes out_count boxes and triangle indices to caller device arrays of
+ * out_capacity entries (OBVHS_ERR_CAPACITY with *out_count set when they do not fit).  max_total as
+ * obvhs_build_ext::split_max_total. */
+obvhs_status obvhs_split_aabbs_preset(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
+                                      uint64_t max_total, obvhs_aabb* d_out_aabbs, uint32_t* d_out_indices,
+                                      uint32_t out_capacity, uint32_t* out_count);
 /* CwBvh fields (cwbvh/mod.rs:43-55). */
 obvhs_status obvhs_cwbvh_info(const obvhs_cwbvh* bvh, uint32_t* node_count, uint32_t* prim_count,
                               obvhs_aabb* total_aabb);

@@ -185,9 +185,11 @@ def _as_device(ctx, data, dtype):
 class Bvh2:
     """Handle of a device-resident Bvh2 (bvh2/mod.rs:31-85)."""
 
-    def __init__(self, ctx, handle):
+    def __init__(self, ctx, handle, owned=True, keepalive=None):
         self.ctx = ctx
         self.handle = handle
+        self.owned = owned
+        self._keepalive = keepalive  # the CwBvh owning a borrowed BVH2
 
     def _info(self):
         nc, pc, md = C.c_uint32(), C.c_uint32(), C.c_uint32()
@@ -235,10 +237,29 @@ class Bvh2:
                                                order.ctypes.data_as(C.c_void_p)), self.ctx.handle)
         return codes, order
 
+    def reinsertion(self, batch_size_ratio, ratio_sequence=None):
+        """ReinsertionOptimizer::default().run(self, batch_size_ratio, ratio_sequence) (reinsertion.rs:40-61)."""
+        seq = None if ratio_sequence is None else np.ascontiguousarray(ratio_sequence, dtype=np.float32)
+        check(lib().obvhs_bvh2_reinsertion(self.ctx.handle, self.handle, float(batch_size_ratio),
+                                           None if seq is None else seq.ctypes.data_as(C.c_void_p),
+                                           0 if seq is None else len(seq)), self.ctx.handle)
+        return self
+
+    def refit_all(self):
+        """Bvh2::refit_all (bvh2/mod.rs:527-569)."""
+        check(lib().obvhs_bvh2_refit_all(self.ctx.handle, self.handle), self.ctx.handle)
+        return self
+
+    def refit_from_triangles(self, triangles):
+        """Leaf boxes from the (moved) triangles, then refit_all (the example at bvh2/mod.rs:506-526)."""
+        d, _ = _as_device(self.ctx, triangles, TRIANGLE)
+        check(lib().obvhs_bvh2_refit_from_triangles(self.ctx.handle, self.handle, d.ptr), self.ctx.handle)
+        return self
+
     def close(self):
-        if self.handle:
+        if self.handle and self.owned:
             lib().obvhs_bvh2_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -351,6 +372,13 @@ class CwBvh:
               self.ctx.handle)
         return miss.to_host().astype(bool)
 
+    @property
+    def bvh2(self):
+        """The BVH2 this CwBvh was built from by build_cwbvh_from_tris (borrowed; None for conversions)."""
+        out = C.c_void_p()
+        check(lib().obvhs_cwbvh_bvh2(self.handle, C.byref(out)))
+        return Bvh2(self.ctx, out, owned=False, keepalive=self) if out.value else None
+
     def close(self):
         if self.handle:
             lib().obvhs_cwbvh_destroy(self.handle)
@@ -371,15 +399,19 @@ def bvh2_to_cwbvh(bvh2, max_prims_per_leaf, order_children=True, include_exact_n
     return CwBvh(bvh2.ctx, out)
 
 
-def build_cwbvh_from_tris(ctx, triangles, config, reuse=None):
-    """cwbvh/builder.rs:20-85 (pre-splits and reinsertion are not available in this version)."""
+def build_cwbvh_from_tris(ctx, triangles, config, reuse=None, split_max_total=0, ratio_sequence=None):
+    """cwbvh/builder.rs:20-85.  split_max_total (0 = no limit) caps the pre-split leaf count; ratio_sequence is the
+    reinsertion ratio sequence (None = the crate's default)."""
     d, n = _as_device(ctx, triangles, TRIANGLE)
     out = C.c_void_p()
     stats = BuildStats()
     params = config.to_c()
-    check(lib().obvhs_build_cwbvh_from_tris(ctx.handle, d.ptr, n, C.byref(params),
-                                            reuse.handle if reuse is not None else None, C.byref(out),
-                                            C.byref(stats)), ctx.handle)
+    seq = None if ratio_sequence is None else np.ascontiguousarray(ratio_sequence, dtype=np.float32)
+    ext = native.BuildExt(int(split_max_total), None if seq is None else seq.ctypes.data_as(C.c_void_p),
+                          0 if seq is None else len(seq))
+    check(lib().obvhs_build_cwbvh_from_tris_ex(ctx.handle, d.ptr, n, C.byref(params), C.byref(ext),
+                                               reuse.handle if reuse is not None else None, C.byref(out),
+                                               C.byref(stats)), ctx.handle)
     if reuse is not None:
         reuse.stats = stats
         return reuse
@@ -467,11 +499,26 @@ def build_cwbvh_batch(ctx, triangles, offsets, config, reuse=None):
     return CwBvhBatch(ctx, out)
 
 
+def split_aabbs_preset(ctx, triangles, max_total=0):
+    """The pre-split leaf set of build_cwbvh_from_tris (cwbvh/builder.rs:26-55): (Aabb array, u32 indices)."""
+    d, n = _as_device(ctx, triangles, TRIANGLE)
+    cap = max(1, 2 * n)
+    while True:
+        da, di = DeviceArray(ctx, AABB, cap), DeviceArray(ctx, np.uint32, cap)
+        cnt = C.c_uint32()
+        st = lib().obvhs_split_aabbs_preset(ctx.handle, d.ptr, n, int(max_total), da.ptr, di.ptr, cap, C.byref(cnt))
+        if st == native.STATUS_CODE["OBVHS_ERR_CAPACITY"] and cnt.value > cap:
+            cap = cnt.value
+            continue
+        check(st, ctx.handle)
+        return da.to_host()[:cnt.value].copy(), di.to_host()[:cnt.value].copy()
+
+
 def log2_threshold_table():
     out = np.zeros(256, dtype=np.uint32)
     check(lib().obvhs_log2_threshold_table(out.ctypes.data_as(C.c_void_p)))
     return out
 
 
-__all__ = ["CwBvhBatch", "build_cwbvh_batch", "PlocSearchDistance", "SortPrecision", "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder",
+__all__ = ["CwBvhBatch", "build_cwbvh_batch", "split_aabbs_preset", "PlocSearchDistance", "SortPrecision", "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder",
            "CwBvh", "bvh2_to_cwbvh", "build_cwbvh_from_tris", "log2_threshold_table", "ObvhsError"]

@@ -312,34 +312,100 @@ obvhs_status obvhs_bvh2_to_cwbvh(obvhs_context* ctx, const obvhs_bvh2* bvh2, uin
 obvhs_status obvhs_build_cwbvh_from_tris(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
                                          const obvhs_build_params* params, obvhs_cwbvh* reuse, obvhs_cwbvh** out,
                                          obvhs_build_stats* stats) {
+    return obvhs_build_cwbvh_from_tris_ex(ctx, d_tris, n, params, nullptr, reuse, out, stats);
+}
+
+obvhs_status obvhs_build_cwbvh_from_tris_ex(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
+                                            const obvhs_build_params* params, const obvhs_build_ext* ext,
+                                            obvhs_cwbvh* reuse, obvhs_cwbvh** out, obvhs_build_stats* stats) {
     if (!ctx || !params || !out) return OBVHS_ERR_INVALID_ARG;
-    if (params->pre_split || params->reinsertion_batch_ratio > 0.0f) return OBVHS_ERR_INVALID_ARG;
+    if (n > 0 && !d_tris) return OBVHS_ERR_INVALID_ARG;
+    if (ext && ext->ratio_sequence_len > 0 && !ext->ratio_sequence) return OBVHS_ERR_INVALID_ARG;
     CTX_TRY(hipSetDevice(ctx->device), "set device");
     obvhs_cwbvh* cw = reuse ? reuse : new (std::nothrow) obvhs_cwbvh();
     if (!cw) return OBVHS_ERR_OOM;
     obvhs_bvh2* b2 = cw->owned_bvh2;
     uint32_t mp = params->max_prims_per_leaf;
     mp = mp < 1u ? 1u : (mp > 3u ? 3u : mp);  // cwbvh/builder.rs:74 clamp(1, 3)
-    // The cost table of the conversion is evaluated by the PLOC merges themselves.
-    obvhs_status s = ploc_entry(ctx, d_tris, nullptr, nullptr, n, params->ploc_search_distance,
-                                params->sort_precision, params->search_depth_threshold, mp, b2, &b2);
-    if (s == OBVHS_OK) {
-        cw->owned_bvh2 = b2;
-        s = cwbvh_convert(ctx, b2, mp, true, cw);
+    const bool reinsert = params->reinsertion_batch_ratio > 0.0f;
+    // Without reinsertion the cost table of the conversion is evaluated by the PLOC merges themselves.
+    const uint32_t fused = reinsert ? 0u : mp;
+    obvhs_status s;
+    if (params->pre_split && n > 0) {  // cwbvh/builder.rs:26-55
+        uint32_t n_split = 0;
+        s = presplit_leaves(ctx, d_tris, n, ext ? ext->split_max_total : 0ull, n_split);
+        if (s == OBVHS_OK)
+            s = ploc_entry(ctx, nullptr, ctx->quality.split_aabbs.as<obvhs_aabb>(),
+                           ctx->quality.split_indices.as<uint32_t>(), n_split, params->ploc_search_distance,
+                           params->sort_precision, params->search_depth_threshold, fused, b2, &b2);
+    } else {
+        s = ploc_entry(ctx, d_tris, nullptr, nullptr, n, params->ploc_search_distance, params->sort_precision,
+                       params->search_depth_threshold, fused, b2, &b2);
     }
+    if (s == OBVHS_OK) cw->owned_bvh2 = b2;
+    if (s == OBVHS_OK && reinsert)  // cwbvh/builder.rs:73
+        s = bvh2_reinsertion(ctx, b2, params->reinsertion_batch_ratio, ext ? ext->ratio_sequence : nullptr,
+                             ext ? ext->ratio_sequence_len : 0u);
+    if (s == OBVHS_OK) s = cwbvh_convert(ctx, b2, mp, true, cw);
     if (s != OBVHS_OK) {
         if (!reuse) obvhs_cwbvh_destroy(cw);
         *out = nullptr;
         return s;
     }
     if (stats) {
-        stats->prim_count = n;
+        stats->prim_count = b2 ? b2->prim_count : n;
         stats->ploc_iterations = b2 ? b2->ploc_iterations : 0;
         stats->bvh2_node_count = b2 ? b2->node_count : 0;
         stats->cwbvh_node_count = cw->node_count;
         stats->ploc_cluster_sum = b2 ? b2->cluster_sum : 0;
     }
     *out = cw;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_cwbvh_bvh2(const obvhs_cwbvh* bvh, obvhs_bvh2** out) {
+    if (!bvh || !out) return OBVHS_ERR_INVALID_ARG;
+    *out = bvh->owned_bvh2;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_size_ratio,
+                                    const float* ratio_sequence, uint32_t ratio_sequence_len) {
+    if (!ctx || !bvh || (ratio_sequence_len > 0 && !ratio_sequence)) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    return bvh2_reinsertion(ctx, bvh, batch_size_ratio, ratio_sequence, ratio_sequence_len);
+}
+
+obvhs_status obvhs_bvh2_refit_all(obvhs_context* ctx, obvhs_bvh2* bvh) {
+    if (!ctx || !bvh) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    return bvh2_refit_all(ctx, bvh);
+}
+
+obvhs_status obvhs_bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh, const obvhs_triangle* d_tris) {
+    if (!ctx || !bvh || (bvh->prim_count > 0 && !d_tris)) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    return bvh2_refit_from_triangles(ctx, bvh, d_tris);
+}
+
+obvhs_status obvhs_split_aabbs_preset(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
+                                      uint64_t max_total, obvhs_aabb* d_out_aabbs, uint32_t* d_out_indices,
+                                      uint32_t out_capacity, uint32_t* out_count) {
+    if (!ctx || !out_count || (n > 0 && !d_tris)) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    uint32_t m = 0;
+    obvhs_status s = presplit_leaves(ctx, d_tris, n, max_total, m);
+    if (s != OBVHS_OK) return s;
+    *out_count = m;
+    if (m > out_capacity) return OBVHS_ERR_CAPACITY;
+    if (m == 0) return OBVHS_OK;
+    if (!d_out_aabbs || !d_out_indices) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipMemcpyAsync(d_out_aabbs, ctx->quality.split_aabbs.ptr, (size_t)m * 32, hipMemcpyDeviceToDevice,
+                           ctx->stream),
+            "copy split boxes");
+    CTX_TRY(hipMemcpyAsync(d_out_indices, ctx->quality.split_indices.ptr, (size_t)m * 4, hipMemcpyDeviceToDevice,
+                           ctx->stream),
+            "copy split indices");
     return OBVHS_OK;
 }
 

@@ -493,6 +493,13 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
     const Node32* nodes = bvh2->nodes.as<Node32>();
     k_cw_init<<<1, 64, 0, st>>>(items[0], misc);
     uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    if (bvh2->dp_max_prims != max_prims_per_leaf && n > 1 && !bvh2->pass_ordered) {
+        // children moved by reinsertion: evaluate the table over the level lists, deepest level first
+        OBVHS_TRY(bvh_dp_alloc(bvh2), "alloc cost table");
+        const obvhs_status ls = cost_table_levels(ctx, bvh2, bvh_dp_view(bvh2, max_prims_per_leaf), &misc->err);
+        if (ls != OBVHS_OK) return ls;
+        bvh2->dp_max_prims = max_prims_per_leaf;
+    }
     if (bvh2->dp_max_prims != max_prims_per_leaf && n > 1) {
         OBVHS_TRY(bvh_dp_alloc(bvh2), "alloc cost table");
         const DpView dpv = bvh_dp_view(bvh2, max_prims_per_leaf);

@@ -48,6 +48,17 @@ struct BuildScratch {
     DevBuf cw_scratch;  // conversion working memory
 };
 
+constexpr uint32_t MAX_TREE_LEVELS = 8192;
+
+// Working memory of the tree utilities and quality passes (quality.hip).
+struct QualityScratch {
+    DevBuf level_list, level_meta;                                   // levels
+    DevBuf parents, owner, touched, misc;                            // reinsertion state
+    DevBuf keys, vals, rkeys, rvals, rfrom, rto, conf, state, stack;  // candidates / reinsertions
+    DevBuf sort_scratch;
+    DevBuf split_aabbs, split_indices, ha, cand_a, cand_b, halves, flags, sums;  // pre-splits
+};
+
 }  // namespace obvhs
 
 struct obvhs_context {
@@ -56,6 +67,7 @@ struct obvhs_context {
     bool own_stream = false;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     obvhs::BuildScratch scratch;
+    obvhs::QualityScratch quality;
     void* pinned = nullptr;  // small pinned host staging area
     bool keep_trace = false; // keep Morton codes and the sort permutation of each build (parity tests)
     std::string last_error;
@@ -77,6 +89,9 @@ struct obvhs_bvh2 {
     // dp_max_prims (0 = not evaluated).
     obvhs::DevBuf dp;
     uint32_t dp_max_prims = 0;
+    // Children are stored in PLOC pass order (the pass ranges in hist order the nodes bottom-up); false after
+    // reinsertion, when bottom-up sweeps use the level lists instead.
+    bool pass_ordered = true;
 };
 
 struct obvhs_cwbvh {
@@ -129,6 +144,16 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
 hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, uint32_t mesh_count,
                                   const uint32_t* ray_mesh, const obvhs_triangle* tris, const obvhs_ray* rays,
                                   uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss, hipStream_t stream);
+
+// quality.hip
+obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32_t>& meta_host, uint32_t& levels);
+obvhs_status cost_table_levels(obvhs_context* ctx, obvhs_bvh2* bvh, const DpView& dp, uint32_t* d_err);
+obvhs_status bvh2_refit_all(obvhs_context* ctx, obvhs_bvh2* bvh);
+obvhs_status bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh, const obvhs_triangle* d_tris);
+obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_size_ratio, const float* seq,
+                              uint32_t seq_len);
+obvhs_status presplit_leaves(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n, uint64_t max_total,
+                             uint32_t& out_n);
 
 // capi.cpp
 obvhs_status set_hip_error(obvhs_context* ctx, hipError_t e, const char* where);

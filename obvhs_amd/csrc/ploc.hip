@@ -609,6 +609,7 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
     bvh->ploc_iterations = 0;
     bvh->cluster_sum = 0;
     bvh->dp_max_prims = 0;
+    bvh->pass_ordered = true;
     OBVHS_TRY(bvh->prim_indices.ensure((size_t)n * 4 + 4), "alloc primitive_indices");
     if (n == 0) return OBVHS_OK;
     OBVHS_TRY(bvh->hist.ensure((size_t)(n + 1) * 4), "alloc pass history");

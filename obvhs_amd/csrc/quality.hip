@@ -1,0 +1,951 @@
+// Tree utilities and the build-quality passes on a device-resident BVH2 (Node32 layout).
+//
+//   levels       : every node in exactly one top-down frontier list (root = level 0), built one launch per level;
+//                  bottom-up sweeps then walk the lists from the deepest level to the root.
+//   refit_all    : bvh2/mod.rs:527-569 -- each internal box becomes the union of its children, deepest level first.
+//   cost table   : the CWBVH cost table (cwdp.hpp) for a BVH2 whose children are no longer in PLOC pass order.
+//   reinsertion  : ReinsertionOptimizer::run (bvh2/reinsertion.rs:40-204) in parallel form.  Per ratio step:
+//                  candidates = nodes 1..2*node_count sorted by descending half area (stable radix sort of the
+//                  crate's f32 key of -cost, ties by node index); find_reinsertion (reinsertion.rs:247-340) for the
+//                  first node_count-1 candidates, one lane each, with its stack in HBM; the reinsertions sorted by
+//                  descending area_diff (ties by candidate rank); the crate's greedy conflict filter
+//                  (reinsertion.rs:180-203) as a parallel greedy maximal independent set over the five conflict
+//                  nodes of each reinsertion -- identical accept decisions, because a conflict node's parent can
+//                  only have changed through an accepted reinsertion that already touched it; the accepted moves
+//                  (reinsertion.rs:342-382) applied in parallel (their node sets are disjoint); then one full
+//                  refit, which leaves every internal box equal to the union of its children exactly like the
+//                  crate's refit_from_fast sequence does.
+//   pre-splits   : cwbvh/builder.rs:26-55 + splits.rs:16-158.  Largest and average half area (the average summed in
+//                  triangle order by one wave, as the crate sums it); per iteration every candidate's best of 11
+//                  split planes in parallel, appended right halves numbered by an order-preserving scan, the
+//                  candidate list extended and filtered in order.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "cwdp.hpp"
+#include "device_util.hpp"
+#include "internal.hpp"
+
+namespace obvhs {
+
+namespace {
+
+constexpr int T256 = 256;
+constexpr uint32_t LEVELS_PER_SYNC = 16;
+constexpr uint32_t MIS_ROUNDS_PER_SYNC = 8;
+constexpr uint32_t FIND_THREADS = 1u << 18;  // concurrent find_reinsertion lanes (each owns a stack column)
+
+__device__ __forceinline__ uint32_t sibling_of(uint32_t id) { return (id & 1u) ? id + 1u : id - 1u; }
+__device__ __forceinline__ uint32_t left_sibling_of(uint32_t id) { return (id & 1u) ? id : id - 1u; }
+
+// ---------------------------------------------------------------------------------------------------------------
+// levels
+// ---------------------------------------------------------------------------------------------------------------
+
+// meta layout: start[L] = meta[2L], count[L] = meta[2L + 1].
+__global__ void __launch_bounds__(T256) k_levels_init(uint32_t* __restrict__ meta, uint32_t words,
+                                                      uint32_t* __restrict__ list) {
+    for (uint32_t i = blockIdx.x * T256 + threadIdx.x; i < words; i += gridDim.x * T256) meta[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        list[0] = 0u;
+        meta[1] = 1u;
+    }
+}
+
+// Appends the children of level `level` as level + 1.  A node whose children would fall outside the node array,
+// or more than n_nodes entries in total (not a tree), sets the error word meta[err_word] instead of writing.
+__global__ void __launch_bounds__(T256) k_levels_expand(const Node32* __restrict__ nodes, uint32_t n_nodes,
+                                                        uint32_t* __restrict__ list, uint32_t* __restrict__ meta,
+                                                        uint32_t level, uint32_t err_word) {
+    const uint32_t s = meta[2 * level], c = meta[2 * level + 1];
+    const uint32_t out = s + c;
+    if (blockIdx.x == 0 && threadIdx.x == 0) meta[2 * (level + 1)] = out;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t b = blockIdx.x * T256 + (threadIdx.x & ~63u); b < c; b += gridDim.x * T256) {
+        const uint32_t i = b + lane;
+        uint32_t first = 0u;
+        bool inner = false;
+        if (i < c) {
+            const Node32 nd = load_node(&nodes[list[s + i]]);
+            inner = nd.prim_count == 0u;
+            first = nd.first_index;
+            if (inner && (first == 0u || first >= n_nodes - 1u)) {
+                atomicOr(&meta[err_word], 1u);
+                inner = false;
+            }
+        }
+        const uint64_t m = __ballot(inner);
+        if (m == 0ull) continue;
+        uint32_t base = 0u;
+        if (lane == 0) base = atomicAdd(&meta[2 * (level + 1) + 1], 2u * (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, 0);
+        if (inner) {
+            const uint32_t r = (uint32_t)__popcll(m & lanemask_lt());
+            const uint64_t at = (uint64_t)out + base + 2 * r;
+            if (at + 1 < n_nodes) {
+                list[at] = first;
+                list[at + 1] = first + 1u;
+            } else {
+                atomicOr(&meta[err_word], 1u);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T256) k_refit_level(Node32* __restrict__ nodes, const uint32_t* __restrict__ list,
+                                                      uint32_t start, uint32_t count) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t x = list[start + i];
+    Node32 nd = load_node(&nodes[x]);
+    if (nd.prim_count != 0u) return;
+    const Box u = box_union(node_box(load_node(&nodes[nd.first_index])), node_box(load_node(&nodes[nd.first_index + 1])));
+    store_node(&nodes[x], make_node(u, 0u, nd.first_index));
+}
+
+__global__ void __launch_bounds__(T256) k_dp_level(const Node32* __restrict__ nodes, DpView dp,
+                                                   const uint32_t* __restrict__ list, uint32_t start, uint32_t count,
+                                                   uint32_t* __restrict__ err) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t x = list[start + i];
+    const Node32 nx = load_node(&nodes[x]);
+    if (nx.prim_count != 0u) return;
+    const Node32 l = load_node(&nodes[nx.first_index]);
+    const Node32 r = load_node(&nodes[nx.first_index + 1]);
+    const uint32_t e = dp_evaluate(dp, node_box(nx), nx.first_index, l, r);
+    if (e) atomicOr(err, e);
+}
+
+// Leaf boxes from (moved) triangles: the primitive of a one-primitive leaf is primitive_indices[first_index].
+__global__ void __launch_bounds__(T256) k_leaf_refit(Node32* __restrict__ nodes, uint32_t n_nodes,
+                                                     const uint32_t* __restrict__ prims, uint32_t n_prims,
+                                                     const float4* __restrict__ tris, uint32_t* __restrict__ err) {
+    const uint32_t x = blockIdx.x * T256 + threadIdx.x;
+    if (x >= n_nodes) return;
+    const Node32 nd = load_node(&nodes[x]);
+    if (nd.prim_count == 0u) return;
+    if (nd.prim_count != 1u || nd.first_index >= n_prims) {
+        atomicOr(err, DERR_CW_NONUNIT_LEAF);
+        return;
+    }
+    const uint32_t p = prims[nd.first_index];
+    if (p >= n_prims) {
+        atomicOr(err, DERR_CAPACITY);
+        return;
+    }
+    const float4 v0 = tris[3 * (size_t)p], v1 = tris[3 * (size_t)p + 1], v2 = tris[3 * (size_t)p + 2];
+    Box b{v0.x, v0.y, v0.z, v0.x, v0.y, v0.z};
+    b = Box{smin(b.lx, v1.x), smin(b.ly, v1.y), smin(b.lz, v1.z), smax(b.hx, v1.x), smax(b.hy, v1.y), smax(b.hz, v1.z)};
+    b = Box{smin(b.lx, v2.x), smin(b.ly, v2.y), smin(b.lz, v2.z), smax(b.hx, v2.x), smax(b.hy, v2.y), smax(b.hz, v2.z)};
+    store_node(&nodes[x], make_node(b, 1u, nd.first_index));
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// reinsertion
+// ---------------------------------------------------------------------------------------------------------------
+
+// bvh2/mod.rs:586-609
+__global__ void __launch_bounds__(T256) k_parents(const Node32* __restrict__ nodes, uint32_t n,
+                                                  uint32_t* __restrict__ parents) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i == 0) parents[0] = 0u;
+    if (i >= n) return;
+    const Node32 nd = load_node(&nodes[i]);
+    if (nd.prim_count == 0u) {
+        parents[nd.first_index] = i;
+        parents[nd.first_index + 1] = i;
+    }
+}
+
+// reinsertion.rs:128-147 + 210-218: key of node i (1 <= i < take) is the crate's radix key of -half_area.
+__global__ void __launch_bounds__(T256) k_cand_keys(const Node32* __restrict__ nodes, uint32_t take,
+                                                    uint64_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x + 1u;
+    if (i >= take) return;
+    keys[i - 1] = f32_order_key(-half_area(node_box(load_node(&nodes[i]))));
+}
+
+struct ReinsMisc {
+    uint32_t err;
+    uint32_t positive;               // reinsertions with area_diff > 0
+    uint32_t undecided[MIS_ROUNDS_PER_SYNC];
+};
+
+// reinsertion.rs:247-340, one lane per candidate (grid-stride), stack column `tid` of an [sd][nthreads] array.
+__global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restrict__ nodes, uint32_t n_nodes,
+                                                           const uint32_t* __restrict__ parents,
+                                                           const uint32_t* __restrict__ cand, uint32_t count,
+                                                           uint2* __restrict__ stack, uint32_t sd,
+                                                           uint32_t* __restrict__ r_from, uint32_t* __restrict__ r_to,
+                                                           uint64_t* __restrict__ keys, ReinsMisc* misc) {
+    const uint32_t tid = blockIdx.x * T256 + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * T256;
+    for (uint32_t t = tid; t < count; t += nthreads) {
+        const uint32_t node_id = cand[t] + 1u;  // candidates were numbered from node 1
+        uint32_t best_to = 0u;
+        float best_area = 0.0f;
+        const Box aabb = node_box(load_node(&nodes[node_id]));
+        const float node_area = half_area(aabb);
+        const uint32_t parent_id = parents[node_id];
+        float area_diff = half_area(node_box(load_node(&nodes[parent_id])));
+        uint32_t sib = sibling_of(node_id);
+        Box pivot_bbox = node_box(load_node(&nodes[sib]));
+        uint32_t pivot_id = parent_id;
+        bool overflow = false;
+        // walks of a valid tree end long before these budgets; they only bound a malformed one
+        uint32_t climb = 0, steps = 0;
+        const uint32_t step_budget = 2u * n_nodes + 64u;
+        for (;;) {
+            uint32_t sp = 0;
+            stack[(size_t)sp * nthreads + tid] = make_uint2(__float_as_uint(area_diff), sib);
+            sp = 1;
+            while (sp > 0) {
+                --sp;
+                if (++steps > step_budget) {
+                    overflow = true;
+                    break;
+                }
+                const uint2 top = stack[(size_t)sp * nthreads + tid];
+                const float top_area = __uint_as_float(top.x);
+                if (top_area - node_area <= best_area) continue;
+                const Node32 dst = load_node(&nodes[top.y]);
+                const Box db = node_box(dst);
+                const float merged_area = half_area(box_union(db, aabb));
+                const float reinsert_area = top_area - merged_area;
+                if (reinsert_area > best_area) {
+                    best_to = top.y;
+                    best_area = reinsert_area;
+                }
+                if (dst.prim_count == 0u) {
+                    if (sp + 2 > sd) {
+                        overflow = true;
+                        break;
+                    }
+                    const float child_area = reinsert_area + half_area(db);
+                    stack[(size_t)sp * nthreads + tid] = make_uint2(__float_as_uint(child_area), dst.first_index);
+                    stack[(size_t)(sp + 1) * nthreads + tid] =
+                        make_uint2(__float_as_uint(child_area), dst.first_index + 1u);
+                    sp += 2;
+                }
+            }
+            if (overflow) break;
+            if (pivot_id != parent_id) {
+                pivot_bbox = box_union(pivot_bbox, node_box(load_node(&nodes[sib])));
+                area_diff += half_area(node_box(load_node(&nodes[pivot_id]))) - half_area(pivot_bbox);
+            }
+            if (pivot_id == 0u) break;
+            if (++climb > sd) {
+                overflow = true;
+                break;
+            }
+            sib = sibling_of(pivot_id);
+            pivot_id = parents[pivot_id];
+        }
+        if (overflow) atomicOr(&misc->err, DERR_CAPACITY);
+        uint32_t from = node_id;
+        if (best_to == sibling_of(node_id) || best_to == parent_id) {
+            from = 0u;
+            best_to = 0u;
+            best_area = 0.0f;
+        }
+        r_from[t] = from;
+        r_to[t] = best_to;
+        const bool pos = best_area > 0.0f;
+        keys[t] = pos ? (uint64_t)(~f32_order_key(best_area)) : 0xffffffffull;
+        const uint64_t m = __ballot(pos);
+        if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1) && m)
+            atomicAdd(&misc->positive, (uint32_t)__popcll(m));
+    }
+}
+
+constexpr uint32_t ST_UNDECIDED = 0u, ST_ACCEPTED_NEW = 1u, ST_ACCEPTED = 2u, ST_REJECTED = 3u;
+
+// Conflict nodes of the k-th reinsertion in priority order (reinsertion.rs:206-216), with the parents of the step.
+__global__ void __launch_bounds__(T256) k_mis_init(const uint32_t* __restrict__ order, const ReinsMisc* misc,
+                                                   const uint32_t* __restrict__ r_from,
+                                                   const uint32_t* __restrict__ r_to,
+                                                   const uint32_t* __restrict__ parents, uint32_t* __restrict__ conf,
+                                                   uint8_t* __restrict__ state) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    if (k >= misc->positive) return;
+    const uint32_t t = order[k];
+    const uint32_t f = r_from[t], to = r_to[t];
+    conf[5 * (size_t)k + 0] = to;
+    conf[5 * (size_t)k + 1] = f;
+    conf[5 * (size_t)k + 2] = sibling_of(f);
+    conf[5 * (size_t)k + 3] = parents[to];
+    conf[5 * (size_t)k + 4] = parents[f];
+    state[k] = (uint8_t)ST_UNDECIDED;
+}
+
+__global__ void __launch_bounds__(T256) k_mis_claim(const uint32_t* __restrict__ conf,
+                                                    const uint8_t* __restrict__ state, ReinsMisc* misc, uint32_t round,
+                                                    uint32_t* __restrict__ owner) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    if (k == 0) misc->undecided[round] = 0u;
+    if (k >= misc->positive || state[k] != ST_UNDECIDED) return;
+    for (int j = 0; j < 5; ++j) atomicMin(&owner[conf[5 * (size_t)k + j]], k);
+}
+
+__global__ void __launch_bounds__(T256) k_mis_decide(const uint32_t* __restrict__ conf, uint8_t* __restrict__ state,
+                                                     const ReinsMisc* misc, const uint32_t* __restrict__ owner,
+                                                     uint8_t* __restrict__ touched) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    if (k >= misc->positive || state[k] != ST_UNDECIDED) return;
+    bool mine = true;
+    for (int j = 0; j < 5; ++j) mine &= owner[conf[5 * (size_t)k + j]] == k;
+    if (!mine) return;
+    state[k] = (uint8_t)ST_ACCEPTED_NEW;
+    for (int j = 0; j < 5; ++j) touched[conf[5 * (size_t)k + j]] = 1u;
+}
+
+__global__ void __launch_bounds__(T256) k_mis_settle(const uint32_t* __restrict__ conf, uint8_t* __restrict__ state,
+                                                     ReinsMisc* misc, uint32_t round, uint32_t* __restrict__ owner,
+                                                     const uint8_t* __restrict__ touched) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    bool still = false;
+    if (k < misc->positive) {
+        const uint32_t s = state[k];
+        if (s == ST_UNDECIDED || s == ST_ACCEPTED_NEW) {
+            bool hit = false;
+            for (int j = 0; j < 5; ++j) {
+                const uint32_t c = conf[5 * (size_t)k + j];
+                owner[c] = 0xffffffffu;
+                hit |= touched[c] != 0u;
+            }
+            if (s == ST_ACCEPTED_NEW) state[k] = (uint8_t)ST_ACCEPTED;
+            else if (hit) state[k] = (uint8_t)ST_REJECTED;
+            else still = true;
+        }
+    }
+    const uint64_t m = __ballot(still);
+    if ((threadIdx.x & 63u) == 0u && m) atomicAdd(&misc->undecided[round], (uint32_t)__popcll(m));
+}
+
+// reinsertion.rs:342-382 without the refits (one full refit follows).  Accepted moves touch disjoint node sets.
+__global__ void __launch_bounds__(T256) k_reins_apply(const uint32_t* __restrict__ conf,
+                                                      const uint8_t* __restrict__ state, const ReinsMisc* misc,
+                                                      Node32* __restrict__ nodes) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    if (k >= misc->positive || state[k] != ST_ACCEPTED) return;
+    const uint32_t to = conf[5 * (size_t)k + 0], from = conf[5 * (size_t)k + 1];
+    const uint32_t sib = conf[5 * (size_t)k + 2], parent_id = conf[5 * (size_t)k + 4];
+    const Node32 sibling_node = load_node(&nodes[sib]);
+    const Node32 dst_node = load_node(&nodes[to]);
+    store_node(&nodes[to], make_node(node_box(dst_node), 0u, left_sibling_of(from)));
+    store_node(&nodes[sib], dst_node);
+    store_node(&nodes[parent_id], sibling_node);
+}
+
+__global__ void __launch_bounds__(T256) k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * T256 + threadIdx.x; i < n; i += (size_t)gridDim.x * T256) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// pre-splits
+// ---------------------------------------------------------------------------------------------------------------
+
+struct SplitMisc {
+    uint32_t largest_key;  // order key of the largest half area (atomicMax)
+    float avg;
+    float thresh_low;
+    float thresh_high;
+    uint32_t splits;       // splitters of the current iteration
+    uint32_t kept;         // candidates kept by the filter
+    uint32_t err;
+    uint32_t pad;
+};
+
+__device__ __forceinline__ float vget(const float4& v, uint32_t d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
+__device__ __forceinline__ float bget_lo(const Box& b, uint32_t d) { return d == 0 ? b.lx : (d == 1 ? b.ly : b.lz); }
+__device__ __forceinline__ float bget_hi(const Box& b, uint32_t d) { return d == 0 ? b.hx : (d == 1 ? b.hy : b.hz); }
+__device__ __forceinline__ void bset_lo(Box& b, uint32_t d, float f) {
+    if (d == 0) b.lx = f;
+    else if (d == 1) b.ly = f;
+    else b.lz = f;
+}
+__device__ __forceinline__ void bset_hi(Box& b, uint32_t d, float f) {
+    if (d == 0) b.hx = f;
+    else if (d == 1) b.hy = f;
+    else b.hz = f;
+}
+__device__ __forceinline__ Box box_invalid() {
+    const float M = 3.40282347e+38f;
+    return Box{M, M, M, -M, -M, -M};
+}
+__device__ __forceinline__ void box_extend(Box& b, float x, float y, float z) {
+    b = Box{smin(b.lx, x), smin(b.ly, y), smin(b.lz, z), smax(b.hx, x), smax(b.hy, y), smax(b.hz, z)};
+}
+// aabb.rs:98-103 (self = the clipped triangle box, other = the halved leaf box)
+__device__ __forceinline__ Box box_intersection(const Box& a, const Box& b) {
+    return Box{smax(a.lx, b.lx), smax(a.ly, b.ly), smax(a.lz, b.lz), smin(a.hx, b.hx), smin(a.hy, b.hy), smin(a.hz, b.hz)};
+}
+// aabb.rs:125-134
+__device__ __forceinline__ uint32_t largest_axis(const Box& a) {
+    const float dx = a.hx - a.lx, dy = a.hy - a.ly, dz = a.hz - a.lz;
+    if (dx < dy) return dy < dz ? 2u : 1u;
+    return dx < dz ? 2u : 0u;
+}
+
+// splits.rs:137-158; the crossing point is a fused multiply-add per lane (Vec3A::mul_add).
+__device__ __forceinline__ void split_triangle(uint32_t dim, float pos, const float4 v[4], Box& left, Box& right) {
+    left = box_invalid();
+    right = box_invalid();
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float4 v0 = v[i], v1 = v[i + 1];
+        const float v0d = vget(v0, dim), v1d = vget(v1, dim);
+        if (v0d <= pos) box_extend(left, v0.x, v0.y, v0.z);
+        if (v0d >= pos) box_extend(right, v0.x, v0.y, v0.z);
+        if ((v0d < pos && pos < v1d) || (v1d < pos && pos < v0d)) {
+            const float inv_length = 1.0f / (v1d - v0d);
+            const float t = (pos - v0d) * inv_length;
+            const float cx = __builtin_fmaf(t, v1.x - v0.x, v0.x);
+            const float cy = __builtin_fmaf(t, v1.y - v0.y, v0.y);
+            const float cz = __builtin_fmaf(t, v1.z - v0.z, v0.z);
+            box_extend(left, cx, cy, cz);
+            box_extend(right, cx, cy, cz);
+        }
+    }
+}
+
+__device__ __forceinline__ Box load_box(const float4* __restrict__ a, uint32_t i) {
+    const float4 lo = a[2 * (size_t)i], hi = a[2 * (size_t)i + 1];
+    return Box{lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+}
+__device__ __forceinline__ void store_box(float4* __restrict__ a, uint32_t i, const Box& b) {
+    a[2 * (size_t)i] = make_float4(b.lx, b.ly, b.lz, 0.0f);
+    a[2 * (size_t)i + 1] = make_float4(b.hx, b.hy, b.hz, 0.0f);
+}
+
+// Leaf boxes (triangle.rs:28-30), indices 0..n, half areas, largest half area.
+__global__ void __launch_bounds__(T256) k_split_prepare(const float4* __restrict__ tris, uint32_t n,
+                                                        float4* __restrict__ aabbs, uint32_t* __restrict__ indices,
+                                                        float* __restrict__ ha, SplitMisc* misc) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    float h = 0.0f;
+    if (i < n) {
+        const float4 v0 = tris[3 * (size_t)i], v1 = tris[3 * (size_t)i + 1], v2 = tris[3 * (size_t)i + 2];
+        Box b{v0.x, v0.y, v0.z, v0.x, v0.y, v0.z};
+        box_extend(b, v1.x, v1.y, v1.z);
+        box_extend(b, v2.x, v2.y, v2.z);
+        store_box(aabbs, i, b);
+        indices[i] = i;
+        h = half_area(b);
+        ha[i] = h;
+    }
+    // f32::max keeps the larger value; half areas of triangle boxes are never negative, so the order key's max is it
+    const uint32_t k = wave_max_u32(i < n ? f32_order_key(h) : 0u);
+    if (__lane_id() == 0) atomicMax(&misc->largest_key, k);
+}
+
+// The crate's running sum `avg_area += half_area` in triangle order (cwbvh/builder.rs:34-41) by one wave: lanes load
+// 64 values at a time, the sum walks them in order; then the split thresholds of split_aabbs_preset.
+__global__ void __launch_bounds__(64) k_split_avg(const float* __restrict__ ha, uint32_t n, SplitMisc* misc) {
+    const uint32_t lane = threadIdx.x;
+    float sum = 0.0f;
+    float next = lane < n ? ha[lane] : 0.0f;
+    for (uint32_t b = 0; b < n; b += 64) {
+        const float cur = next;
+        if (b + 64 + lane < n) next = ha[b + 64 + lane];
+        const uint32_t m = min(64u, n - b);
+        const int ci = __float_as_int(cur);
+        for (uint32_t k = 0; k < m; ++k) sum += __int_as_float(__builtin_amdgcn_readlane(ci, (int)k));
+    }
+    if (lane == 0) {
+        const float avg = sum / (float)n;
+        const float largest = f32_from_order_key(misc->largest_key);
+        misc->avg = avg;
+        misc->thresh_low = avg * 3.0f;                                       // splits.rs:27
+        misc->thresh_high = fmaxf(avg * 4.0f, avg * 0.9f + largest * 0.1f);  // splits.rs:28
+    }
+}
+
+// Order-preserving selection helpers: per-block counts then one scan (blocksums[b] = exclusive prefix).
+__device__ __forceinline__ uint32_t block_count(bool f) {
+    __shared__ uint32_t w[T256 / 64];
+    const uint32_t c = (uint32_t)__popcll(__ballot(f));
+    if ((threadIdx.x & 63u) == 0) w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    uint32_t s = 0;
+    for (int k = 0; k < T256 / 64; ++k) s += w[k];
+    return s;
+}
+__device__ __forceinline__ uint32_t block_rank(bool f, uint32_t block_base) {
+    __shared__ uint32_t w[T256 / 64];
+    const uint64_t m = __ballot(f);
+    if ((threadIdx.x & 63u) == 0) w[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t r = block_base;
+    for (uint32_t k = 0; k < (threadIdx.x >> 6); ++k) r += w[k];
+    return r + (uint32_t)__popcll(m & lanemask_lt());
+}
+
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* __restrict__ sums, uint32_t nblocks,
+                                                      uint32_t* __restrict__ total) {
+    __shared__ uint32_t sw[16];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t chunk = (nblocks + 1023u) / 1024u;
+    const uint32_t b0 = tid * chunk, b1 = min(nblocks, b0 + chunk);
+    uint32_t s = 0;
+    for (uint32_t b = b0; b < b1; ++b) s += sums[b];
+    const uint32_t inc = wave_inclusive_scan(s);
+    if ((tid & 63u) == 63u) sw[tid >> 6] = inc;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t r = 0;
+        for (int w = 0; w < 16; ++w) {
+            const uint32_t t = sw[w];
+            sw[w] = r;
+            r += t;
+        }
+        *total = r;
+    }
+    __syncthreads();
+    uint32_t e = sw[tid >> 6] + inc - s;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t v = sums[b];
+        sums[b] = e;
+        e += v;
+    }
+}
+
+// Initial candidates: leaves whose half area exceeds the low threshold (splits.rs:63-69).
+__global__ void __launch_bounds__(T256) k_split_cand_count(const float* __restrict__ ha, uint32_t n,
+                                                           const SplitMisc* misc, uint32_t* __restrict__ sums) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    const bool f = i < n && ha[i] > misc->thresh_low;
+    const uint32_t c = block_count(f);
+    if (threadIdx.x == 0) sums[blockIdx.x] = c;
+}
+__global__ void __launch_bounds__(T256) k_split_cand_write(const float* __restrict__ ha, uint32_t n,
+                                                           const SplitMisc* misc, const uint32_t* __restrict__ sums,
+                                                           uint32_t* __restrict__ cand) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    const bool f = i < n && ha[i] > misc->thresh_low;
+    const uint32_t r = block_rank(f, sums[blockIdx.x]);
+    if (f) cand[r] = i;
+}
+
+// splits.rs:72-112 for one candidate: the best of split_tests - 1 planes and the split decision.
+__global__ void __launch_bounds__(T256) k_split_eval(const float4* __restrict__ tris, const float4* __restrict__ aabbs,
+                                                     const uint32_t* __restrict__ indices,
+                                                     const uint32_t* __restrict__ cand, uint32_t count,
+                                                     const SplitMisc* misc, float4* __restrict__ halves,
+                                                     uint8_t* __restrict__ flag, uint32_t* __restrict__ sums) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    bool split = false;
+    if (k < count) {
+        const uint32_t c = cand[k];
+        const Box aabb = load_box(aabbs, c);
+        const uint32_t index = indices[c];
+        const uint32_t axis = largest_axis(aabb);
+        const float4 v[4] = {tris[3 * (size_t)index], tris[3 * (size_t)index + 1], tris[3 * (size_t)index + 2],
+                             tris[3 * (size_t)index]};
+        float best_cost = 3.40282347e+38f;
+        Box left = aabb, right = aabb;
+        const float amin = bget_lo(aabb, axis), amax = bget_hi(aabb, axis);
+        for (uint32_t i = 1; i < 12u; ++i) {
+            const float nf = (float)i / 12.0f;
+            const float pos = amin * nf + amax * (1.0f - nf);
+            Box tl = aabb, tr = aabb;
+            bset_hi(tl, axis, pos);
+            bset_lo(tr, axis, pos);
+            Box sl, sr;
+            split_triangle(axis, pos, v, sl, sr);
+            tl = box_intersection(sl, tl);
+            tr = box_intersection(sr, tr);
+            const float area = half_area(tl) + half_area(tr);
+            if (area < best_cost) {
+                best_cost = area;
+                left = tl;
+                right = tr;
+            }
+        }
+        const float old_cost = half_area(aabb);
+        split = (misc->thresh_high > old_cost && best_cost * 1.6f < old_cost) || best_cost * 1.8f < old_cost;
+        flag[k] = split ? 1u : 0u;
+        if (split) {
+            store_box(halves, 2 * k, left);
+            store_box(halves, 2 * k + 1, right);
+        }
+    }
+    const uint32_t c = block_count(split);
+    if (threadIdx.x == 0) sums[blockIdx.x] = c;
+}
+
+// Applies the splits in candidate order: the r-th splitter's right half becomes leaf len + r (while the optional
+// budget allows), and the extended candidate list cand ++ [len, len + accepted) is written to ext.
+__global__ void __launch_bounds__(T256) k_split_apply(const uint32_t* __restrict__ cand, uint32_t count,
+                                                      const uint8_t* __restrict__ flag,
+                                                      const uint32_t* __restrict__ sums,
+                                                      const float4* __restrict__ halves, uint32_t len,
+                                                      uint32_t accept, float4* __restrict__ aabbs,
+                                                      uint32_t* __restrict__ indices, uint32_t* __restrict__ ext) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    const bool f = k < count && flag[k] != 0u;
+    const uint32_t r = block_rank(f, sums[blockIdx.x]);
+    if (k >= count) return;
+    const uint32_t c = cand[k];
+    ext[k] = c;
+    if (f && r < accept) {
+        store_box(aabbs, c, load_box(halves, 2 * k));
+        store_box(aabbs, len + r, load_box(halves, 2 * k + 1));
+        indices[len + r] = indices[c];
+        ext[count + r] = len + r;
+    }
+}
+
+// splits.rs:113-116: keep the candidates whose box still exceeds the low threshold, in order.
+__global__ void __launch_bounds__(T256) k_split_keep_count(const uint32_t* __restrict__ ext, uint32_t count,
+                                                           const float4* __restrict__ aabbs, const SplitMisc* misc,
+                                                           uint32_t* __restrict__ sums) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    const bool f = k < count && half_area(load_box(aabbs, ext[k])) > misc->thresh_low;
+    const uint32_t c = block_count(f);
+    if (threadIdx.x == 0) sums[blockIdx.x] = c;
+}
+__global__ void __launch_bounds__(T256) k_split_keep_write(const uint32_t* __restrict__ ext, uint32_t count,
+                                                           const float4* __restrict__ aabbs, const SplitMisc* misc,
+                                                           const uint32_t* __restrict__ sums,
+                                                           uint32_t* __restrict__ cand) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    const bool f = k < count && half_area(load_box(aabbs, ext[k])) > misc->thresh_low;
+    const uint32_t r = block_rank(f, sums[blockIdx.x]);
+    if (f) cand[r] = ext[k];
+}
+
+}  // namespace
+
+#define OBVHS_TRY(expr, where)                                      \
+    do {                                                            \
+        hipError_t _e = (expr);                                     \
+        if (_e != hipSuccess) return set_hip_error(ctx, _e, where); \
+    } while (0)
+
+// Top-down level lists of the BVH2 in bvh->nodes.  On return meta_host[2L], meta_host[2L+1] = start, count of
+// level L for L < levels.
+obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32_t>& meta_host, uint32_t& levels) {
+    hipStream_t st = ctx->stream;
+    const uint32_t N = bvh->node_count;
+    levels = 0;
+    meta_host.clear();
+    if (N == 0) return OBVHS_OK;
+    QualityScratch& Q = ctx->quality;
+    const uint32_t words = 2 * (MAX_TREE_LEVELS + 2) + 1;  // + the error word
+    OBVHS_TRY(Q.level_list.ensure((size_t)N * 4), "alloc level list");
+    OBVHS_TRY(Q.level_meta.ensure((size_t)words * 4), "alloc level meta");
+    uint32_t* list = Q.level_list.as<uint32_t>();
+    uint32_t* meta = Q.level_meta.as<uint32_t>();
+    const Node32* nodes = bvh->nodes.as<Node32>();
+    k_levels_init<<<8, T256, 0, st>>>(meta, words, list);
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    uint32_t level = 0;
+    for (;;) {
+        for (uint32_t q = 0; q < LEVELS_PER_SYNC && level + q < MAX_TREE_LEVELS; ++q) {
+            const uint64_t bound = (level + q) < 31 ? std::min<uint64_t>(1ull << (level + q), N) : N;
+            const uint32_t g = (uint32_t)std::min<uint64_t>((bound + T256 - 1) / T256, 2048);
+            k_levels_expand<<<g, T256, 0, st>>>(nodes, N, list, meta, level + q, words - 1);
+        }
+        OBVHS_TRY(hipGetLastError(), "levels launch");
+        level = std::min(level + LEVELS_PER_SYNC, MAX_TREE_LEVELS);
+        OBVHS_TRY(hipMemcpyAsync(h, meta + 2 * level, 8, hipMemcpyDeviceToHost, st), "read level count");
+        OBVHS_TRY(hipMemcpyAsync(h + 2, meta + words - 1, 4, hipMemcpyDeviceToHost, st), "read level error");
+        OBVHS_TRY(hipStreamSynchronize(st), "levels sync");
+        if (h[2] != 0u) return OBVHS_ERR_CAPACITY;  // not a tree
+        if (h[1] == 0u) break;
+        if (level >= MAX_TREE_LEVELS) return OBVHS_ERR_DEPTH;
+    }
+    meta_host.resize(2 * (size_t)(level + 1));
+    OBVHS_TRY(hipMemcpy(meta_host.data(), meta, meta_host.size() * 4, hipMemcpyDeviceToHost), "read levels");
+    uint32_t total = 0;
+    levels = 0;
+    for (uint32_t l = 0; l <= level; ++l) {
+        if (meta_host[2 * l + 1] == 0u) break;
+        total += meta_host[2 * l + 1];
+        levels = l + 1;
+    }
+    if (total != N) return OBVHS_ERR_CAPACITY;  // not a tree: some node unreachable or reached twice
+    return OBVHS_OK;
+}
+
+static obvhs_status refit_levels(obvhs_context* ctx, obvhs_bvh2* bvh, const std::vector<uint32_t>& meta,
+                                 uint32_t levels) {
+    hipStream_t st = ctx->stream;
+    const uint32_t* list = ctx->quality.level_list.as<uint32_t>();
+    Node32* nodes = bvh->nodes.as<Node32>();
+    for (uint32_t l = levels; l-- > 0;) {
+        const uint32_t s = meta[2 * l], c = meta[2 * l + 1];
+        k_refit_level<<<(c + T256 - 1) / T256, T256, 0, st>>>(nodes, list, s, c);
+    }
+    OBVHS_TRY(hipGetLastError(), "refit launch");
+    return OBVHS_OK;
+}
+
+// Cost table over the level lists (any node order), deepest level first.
+obvhs_status cost_table_levels(obvhs_context* ctx, obvhs_bvh2* bvh, const DpView& dp, uint32_t* d_err) {
+    std::vector<uint32_t> meta;
+    uint32_t levels = 0;
+    obvhs_status s = tree_levels(ctx, bvh, meta, levels);
+    if (s != OBVHS_OK) return s;
+    hipStream_t st = ctx->stream;
+    const uint32_t* list = ctx->quality.level_list.as<uint32_t>();
+    const Node32* nodes = bvh->nodes.as<Node32>();
+    for (uint32_t l = levels; l-- > 0;) {
+        const uint32_t b = meta[2 * l], c = meta[2 * l + 1];
+        k_dp_level<<<(c + T256 - 1) / T256, T256, 0, st>>>(nodes, dp, list, b, c, d_err);
+    }
+    OBVHS_TRY(hipGetLastError(), "cost table launch");
+    return OBVHS_OK;
+}
+
+obvhs_status bvh2_refit_all(obvhs_context* ctx, obvhs_bvh2* bvh) {
+    std::vector<uint32_t> meta;
+    uint32_t levels = 0;
+    obvhs_status s = tree_levels(ctx, bvh, meta, levels);
+    if (s != OBVHS_OK) return s;
+    s = refit_levels(ctx, bvh, meta, levels);
+    bvh->dp_max_prims = 0;  // boxes changed: the cost table is stale
+    return s;
+}
+
+obvhs_status bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh, const obvhs_triangle* d_tris) {
+    if (bvh->node_count == 0) return OBVHS_OK;
+    uint32_t* err = ctx->quality.misc.as<uint32_t>();
+    OBVHS_TRY(ctx->quality.misc.ensure(4096), "alloc quality misc");
+    err = ctx->quality.misc.as<uint32_t>();
+    hipStream_t st = ctx->stream;
+    k_fill_u32<<<1, T256, 0, st>>>(err, 0u, 1);
+    k_leaf_refit<<<(bvh->node_count + T256 - 1) / T256, T256, 0, st>>>(
+        bvh->nodes.as<Node32>(), bvh->node_count, bvh->prim_indices.as<uint32_t>(), bvh->prim_count,
+        reinterpret_cast<const float4*>(d_tris), err);
+    OBVHS_TRY(hipGetLastError(), "leaf refit launch");
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    OBVHS_TRY(hipMemcpyAsync(h, err, 4, hipMemcpyDeviceToHost, st), "read leaf refit err");
+    OBVHS_TRY(hipStreamSynchronize(st), "leaf refit sync");
+    if (h[0]) return device_error_to_status(h[0]);
+    return bvh2_refit_all(ctx, bvh);
+}
+
+// ReinsertionOptimizer::run (reinsertion.rs:40-61, 101-119).
+obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_size_ratio, const float* seq,
+                              uint32_t seq_len) {
+    const uint32_t N = bvh->node_count;
+    if (N == 0 || !(batch_size_ratio > 0.0f)) return OBVHS_OK;
+    hipStream_t st = ctx->stream;
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    {
+        uint32_t root_pc = 0;
+        OBVHS_TRY(hipMemcpyAsync(h, reinterpret_cast<const char*>(bvh->nodes.ptr) + 12, 4, hipMemcpyDeviceToHost,
+                                 st),
+                  "read root");
+        OBVHS_TRY(hipStreamSynchronize(st), "root sync");
+        root_pc = h[0];
+        if (root_pc != 0u) return OBVHS_OK;  // a leaf root (reinsertion.rs:45)
+    }
+    std::vector<float> default_seq;
+    if (!seq || seq_len == 0) {
+        for (int k = 1; k < 32; k += 2) default_seq.push_back(1.0f / (float)k);  // reinsertion.rs:105-110
+        seq = default_seq.data();
+        seq_len = (uint32_t)default_seq.size();
+    }
+    QualityScratch& Q = ctx->quality;
+    OBVHS_TRY(Q.parents.ensure((size_t)N * 4), "alloc parents");
+    OBVHS_TRY(Q.owner.ensure((size_t)N * 4), "alloc owner");
+    OBVHS_TRY(Q.touched.ensure((size_t)N), "alloc touched");
+    OBVHS_TRY(Q.misc.ensure(4096), "alloc quality misc");
+    uint32_t* parents = Q.parents.as<uint32_t>();
+    uint32_t* owner = Q.owner.as<uint32_t>();
+    uint8_t* touched = Q.touched.as<uint8_t>();
+    ReinsMisc* misc = Q.misc.as<ReinsMisc>();
+    Node32* nodes = bvh->nodes.as<Node32>();
+    const uint32_t nblk = (N + T256 - 1) / T256;
+    k_fill_u32<<<std::min(nblk, 4096u), T256, 0, st>>>(owner, 0xffffffffu, N);
+
+    std::vector<uint32_t> meta;
+    uint32_t levels = 0;
+    obvhs_status s = tree_levels(ctx, bvh, meta, levels);
+    if (s != OBVHS_OK) return s;
+    for (uint32_t si = 0; si < seq_len; ++si) {
+        // reinsertion.rs:112-117 in f32, `as usize` saturating
+        const float prod = ((float)N * batch_size_ratio) * seq[si];
+        uint64_t batch = prod >= 1.8446744e19f ? UINT64_MAX : (prod > 0.0f ? (uint64_t)prod : 0ull);
+        batch = std::max<uint64_t>(batch, 1);
+        const uint32_t node_count = (uint32_t)std::min<uint64_t>(N, batch + 1);
+        const uint32_t take = (uint32_t)std::min<uint64_t>(N, 2ull * node_count);
+        const uint32_t ncand = take - 1;
+        const uint32_t count = node_count - 1;
+        if (count == 0) continue;
+        OBVHS_TRY(Q.keys.ensure((size_t)ncand * 8), "alloc candidate keys");
+        OBVHS_TRY(Q.vals.ensure((size_t)ncand * 4), "alloc candidate values");
+        OBVHS_TRY(Q.rkeys.ensure((size_t)count * 8), "alloc reinsertion keys");
+        OBVHS_TRY(Q.rvals.ensure((size_t)count * 4), "alloc reinsertion order");
+        OBVHS_TRY(Q.rfrom.ensure((size_t)count * 4), "alloc reinsertion from");
+        OBVHS_TRY(Q.rto.ensure((size_t)count * 4), "alloc reinsertion to");
+        OBVHS_TRY(Q.conf.ensure((size_t)count * 20), "alloc conflicts");
+        OBVHS_TRY(Q.state.ensure((size_t)count), "alloc states");
+        OBVHS_TRY(Q.sort_scratch.ensure(radix_sort_scratch_bytes(std::max(ncand, count))), "alloc sort scratch");
+        const uint32_t sd = levels + 2;
+        uint32_t fthreads = std::min<uint32_t>(FIND_THREADS, (count + T256 - 1) / T256 * T256);
+        const uint64_t stack_limit = 1ull << 31;  // bytes of stack columns at most
+        while (fthreads > T256 && (uint64_t)fthreads * sd * 8 > stack_limit) fthreads = fthreads / 2 / T256 * T256;
+        OBVHS_TRY(Q.stack.ensure((size_t)fthreads * sd * 8), "alloc reinsertion stacks");
+
+        k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(misc), 0u, sizeof(ReinsMisc) / 4);
+        k_parents<<<nblk, T256, 0, st>>>(nodes, N, parents);
+        k_cand_keys<<<(ncand + T256 - 1) / T256, T256, 0, st>>>(nodes, take, Q.keys.as<uint64_t>());
+        OBVHS_TRY(hipGetLastError(), "candidate launch");
+        OBVHS_TRY(radix_sort_pairs(Q.keys.as<uint64_t>(), Q.vals.as<uint32_t>(), true, ncand, 32,
+                                   Q.sort_scratch.ptr, st),
+                  "candidate sort");
+        k_find_reinsertion<<<fthreads / T256, T256, 0, st>>>(nodes, N, parents, Q.vals.as<uint32_t>(), count,
+                                                              Q.stack.as<uint2>(), sd, Q.rfrom.as<uint32_t>(),
+                                                              Q.rto.as<uint32_t>(), Q.rkeys.as<uint64_t>(), misc);
+        OBVHS_TRY(hipGetLastError(), "find reinsertion launch");
+        OBVHS_TRY(radix_sort_pairs(Q.rkeys.as<uint64_t>(), Q.rvals.as<uint32_t>(), true, count, 32,
+                                   Q.sort_scratch.ptr, st),
+                  "reinsertion sort");
+        OBVHS_TRY(hipMemcpyAsync(h, misc, 8, hipMemcpyDeviceToHost, st), "read reinsertion count");
+        OBVHS_TRY(hipStreamSynchronize(st), "reinsertion count sync");
+        if (h[0]) return device_error_to_status(h[0]);
+        const uint32_t npos = h[1];
+        if (npos == 0) continue;
+        const uint32_t pblk = (npos + T256 - 1) / T256;
+        k_fill_u32<<<std::min(nblk, 4096u), T256, 0, st>>>(reinterpret_cast<uint32_t*>(touched), 0u,
+                                                          ((size_t)N + 3) / 4);
+        k_mis_init<<<pblk, T256, 0, st>>>(Q.rvals.as<uint32_t>(), misc, Q.rfrom.as<uint32_t>(),
+                                          Q.rto.as<uint32_t>(), parents, Q.conf.as<uint32_t>(),
+                                          Q.state.as<uint8_t>());
+        // every round accepts at least the best-ranked undecided reinsertion, so npos rounds always suffice
+        for (uint32_t batch = 0;; ++batch) {
+            if ((uint64_t)batch * MIS_ROUNDS_PER_SYNC > (uint64_t)npos + MIS_ROUNDS_PER_SYNC) return OBVHS_ERR_DEVICE;
+            for (uint32_t r = 0; r < MIS_ROUNDS_PER_SYNC; ++r) {
+                k_mis_claim<<<pblk, T256, 0, st>>>(Q.conf.as<uint32_t>(), Q.state.as<uint8_t>(), misc, r, owner);
+                k_mis_decide<<<pblk, T256, 0, st>>>(Q.conf.as<uint32_t>(), Q.state.as<uint8_t>(), misc, owner,
+                                                    touched);
+                k_mis_settle<<<pblk, T256, 0, st>>>(Q.conf.as<uint32_t>(), Q.state.as<uint8_t>(), misc, r, owner,
+                                                    touched);
+            }
+            OBVHS_TRY(hipGetLastError(), "conflict rounds launch");
+            OBVHS_TRY(hipMemcpyAsync(h, &misc->undecided[MIS_ROUNDS_PER_SYNC - 1], 4, hipMemcpyDeviceToHost, st),
+                      "read undecided");
+            OBVHS_TRY(hipStreamSynchronize(st), "conflict rounds sync");
+            if (h[0] == 0u) break;
+        }
+        k_reins_apply<<<pblk, T256, 0, st>>>(Q.conf.as<uint32_t>(), Q.state.as<uint8_t>(), misc, nodes);
+        OBVHS_TRY(hipGetLastError(), "apply launch");
+        s = tree_levels(ctx, bvh, meta, levels);
+        if (s != OBVHS_OK) return s;
+        s = refit_levels(ctx, bvh, meta, levels);
+        if (s != OBVHS_OK) return s;
+    }
+    bvh->pass_ordered = false;  // children are no longer stored in PLOC pass order (reinsertion.rs:102)
+    bvh->dp_max_prims = 0;
+    return OBVHS_OK;
+}
+
+// Pre-split leaf set (cwbvh/builder.rs:26-55).  max_total = 0: no limit (the crate's behaviour).  On return
+// Q.split_aabbs / Q.split_indices hold `out_n` leaves.
+obvhs_status presplit_leaves(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n, uint64_t max_total,
+                             uint32_t& out_n) {
+    hipStream_t st = ctx->stream;
+    QualityScratch& Q = ctx->quality;
+    out_n = n;
+    if (n == 0) return OBVHS_OK;
+    uint64_t cap = std::max<uint64_t>((uint64_t)n + (uint64_t)n / 2, 1024);
+    if (max_total) cap = std::max<uint64_t>(std::min<uint64_t>(cap, max_total), n);
+    if (cap >= (1ull << 31)) return OBVHS_ERR_CAPACITY;
+    OBVHS_TRY(Q.split_aabbs.ensure(cap * 32), "alloc split boxes");
+    OBVHS_TRY(Q.split_indices.ensure(cap * 4), "alloc split indices");
+    OBVHS_TRY(Q.ha.ensure((size_t)n * 4), "alloc half areas");
+    OBVHS_TRY(Q.misc.ensure(4096), "alloc quality misc");
+    SplitMisc* misc = reinterpret_cast<SplitMisc*>(Q.misc.as<char>() + 1024);
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    const float4* tris = reinterpret_cast<const float4*>(d_tris);
+    const uint32_t nblk = (n + T256 - 1) / T256;
+    OBVHS_TRY(Q.sums.ensure((size_t)(nblk + 1) * 4 + 64), "alloc block sums");
+    k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(misc), 0u, sizeof(SplitMisc) / 4);
+    k_split_prepare<<<nblk, T256, 0, st>>>(tris, n, Q.split_aabbs.as<float4>(), Q.split_indices.as<uint32_t>(),
+                                           Q.ha.as<float>(), misc);
+    k_split_avg<<<1, 64, 0, st>>>(Q.ha.as<float>(), n, misc);
+    // candidate lists hold distinct leaf indices (at most cap); the extended list at most twice that
+    OBVHS_TRY(Q.cand_a.ensure(cap * 4), "alloc candidates");
+    OBVHS_TRY(Q.cand_b.ensure(cap * 8), "alloc extended candidates");
+    k_split_cand_count<<<nblk, T256, 0, st>>>(Q.ha.as<float>(), n, misc, Q.sums.as<uint32_t>());
+    k_scan_blocks<<<1, 1024, 0, st>>>(Q.sums.as<uint32_t>(), nblk, &misc->kept);
+    k_split_cand_write<<<nblk, T256, 0, st>>>(Q.ha.as<float>(), n, misc, Q.sums.as<uint32_t>(),
+                                              Q.cand_a.as<uint32_t>());
+    OBVHS_TRY(hipGetLastError(), "split prepare launch");
+    OBVHS_TRY(hipMemcpyAsync(h, &misc->kept, 4, hipMemcpyDeviceToHost, st), "read candidates");
+    OBVHS_TRY(hipStreamSynchronize(st), "split prepare sync");
+    uint32_t count = h[0];
+    uint64_t len = n;
+    for (uint32_t it = 0; it < 12 && count > 0; ++it) {  // max_iterations = 12 (splits.rs:31)
+        // every candidate may append one leaf: grow the leaf arrays (keeping their contents) when needed
+        if (len + count > cap) {
+            uint64_t ncap = std::max<uint64_t>(cap * 2, len + count);
+            if (max_total) ncap = std::max<uint64_t>(std::min<uint64_t>(ncap, max_total), len);
+            if (ncap >= (1ull << 31)) return OBVHS_ERR_CAPACITY;
+            DevBuf na, ni, nc;
+            OBVHS_TRY(na.ensure(ncap * 32), "grow split boxes");
+            OBVHS_TRY(ni.ensure(ncap * 4), "grow split indices");
+            OBVHS_TRY(nc.ensure(ncap * 4), "grow candidates");
+            OBVHS_TRY(hipMemcpyAsync(na.ptr, Q.split_aabbs.ptr, len * 32, hipMemcpyDeviceToDevice, st), "copy boxes");
+            OBVHS_TRY(hipMemcpyAsync(ni.ptr, Q.split_indices.ptr, len * 4, hipMemcpyDeviceToDevice, st),
+                      "copy indices");
+            OBVHS_TRY(hipMemcpyAsync(nc.ptr, Q.cand_a.ptr, (size_t)count * 4, hipMemcpyDeviceToDevice, st),
+                      "copy candidates");
+            OBVHS_TRY(hipStreamSynchronize(st), "grow sync");
+            Q.split_aabbs.release();
+            Q.split_indices.release();
+            Q.cand_a.release();
+            Q.split_aabbs = na;
+            Q.split_indices = ni;
+            Q.cand_a = nc;
+            OBVHS_TRY(Q.cand_b.ensure(ncap * 8), "grow extended candidates");
+            cap = ncap;
+        }
+        const uint32_t cblk = (count + T256 - 1) / T256;
+        OBVHS_TRY(Q.halves.ensure((size_t)count * 64), "alloc split halves");
+        OBVHS_TRY(Q.flags.ensure((size_t)count), "alloc split flags");
+        OBVHS_TRY(Q.sums.ensure((size_t)(2 * cblk + 1) * 4 + 64), "alloc block sums");
+        k_split_eval<<<cblk, T256, 0, st>>>(tris, Q.split_aabbs.as<float4>(), Q.split_indices.as<uint32_t>(),
+                                            Q.cand_a.as<uint32_t>(), count, misc, Q.halves.as<float4>(),
+                                            Q.flags.as<uint8_t>(), Q.sums.as<uint32_t>());
+        k_scan_blocks<<<1, 1024, 0, st>>>(Q.sums.as<uint32_t>(), cblk, &misc->splits);
+        OBVHS_TRY(hipGetLastError(), "split eval launch");
+        OBVHS_TRY(hipMemcpyAsync(h, &misc->splits, 4, hipMemcpyDeviceToHost, st), "read splits");
+        OBVHS_TRY(hipStreamSynchronize(st), "split eval sync");
+        const uint32_t splits = h[0];
+        uint32_t accept = splits;
+        if (max_total) accept = (uint32_t)std::min<uint64_t>(splits, max_total > len ? max_total - len : 0);
+        k_split_apply<<<cblk, T256, 0, st>>>(Q.cand_a.as<uint32_t>(), count, Q.flags.as<uint8_t>(),
+                                             Q.sums.as<uint32_t>(), Q.halves.as<float4>(), (uint32_t)len, accept,
+                                             Q.split_aabbs.as<float4>(), Q.split_indices.as<uint32_t>(),
+                                             Q.cand_b.as<uint32_t>());
+        OBVHS_TRY(hipGetLastError(), "split apply launch");
+        len += accept;
+        if (accept == 0) break;  // splits.rs:113: no candidate was added
+        const uint32_t ext = count + accept;
+        const uint32_t eblk = (ext + T256 - 1) / T256;
+        OBVHS_TRY(Q.sums.ensure((size_t)(eblk + 1) * 4 + 64), "alloc block sums");
+        k_split_keep_count<<<eblk, T256, 0, st>>>(Q.cand_b.as<uint32_t>(), ext, Q.split_aabbs.as<float4>(), misc,
+                                                  Q.sums.as<uint32_t>());
+        k_scan_blocks<<<1, 1024, 0, st>>>(Q.sums.as<uint32_t>(), eblk, &misc->kept);
+        k_split_keep_write<<<eblk, T256, 0, st>>>(Q.cand_b.as<uint32_t>(), ext, Q.split_aabbs.as<float4>(), misc,
+                                                  Q.sums.as<uint32_t>(), Q.cand_a.as<uint32_t>());
+        OBVHS_TRY(hipGetLastError(), "split filter launch");
+        OBVHS_TRY(hipMemcpyAsync(h, &misc->kept, 4, hipMemcpyDeviceToHost, st), "read kept");
+        OBVHS_TRY(hipStreamSynchronize(st), "split filter sync");
+        count = h[0];
+    }
+    if (len >= (1ull << 30)) return OBVHS_ERR_CAPACITY;
+    out_n = (uint32_t)len;
+    return OBVHS_OK;
+}
+
+}  // namespace obvhs

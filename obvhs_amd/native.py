@@ -37,6 +37,11 @@ class BuildStats(C.Structure):
                 ("cwbvh_node_count", C.c_uint32), ("ploc_cluster_sum", C.c_uint64)]
 
 
+class BuildExt(C.Structure):
+    """obvhs_build_ext: the split budget and the reinsertion ratio sequence."""
+    _fields_ = [("split_max_total", C.c_uint64), ("ratio_sequence", C.c_void_p), ("ratio_sequence_len", C.c_uint32)]
+
+
 class Aabb(C.Structure):
     _fields_ = [("min", C.c_float * 4), ("max", C.c_float * 4)]
 
@@ -92,6 +97,13 @@ SIGNATURES = {
     "obvhs_batch_ray_traverse": (INT, [P, P, P, P, P, U32, P]),
     "obvhs_batch_ray_traverse_miss": (INT, [P, P, P, P, P, U32, P]),
     "obvhs_batch_destroy": (INT, [P]),
+    "obvhs_build_cwbvh_from_tris_ex": (INT, [P, P, U32, C.POINTER(BuildParams), C.POINTER(BuildExt), P,
+                                             C.POINTER(P), C.POINTER(BuildStats)]),
+    "obvhs_cwbvh_bvh2": (INT, [P, C.POINTER(P)]),
+    "obvhs_bvh2_reinsertion": (INT, [P, P, C.c_float, P, U32]),
+    "obvhs_bvh2_refit_all": (INT, [P, P]),
+    "obvhs_bvh2_refit_from_triangles": (INT, [P, P, P]),
+    "obvhs_split_aabbs_preset": (INT, [P, P, U32, U64, P, P, U32, PU32]),
 }
 
 _lib = None

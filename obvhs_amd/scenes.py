@@ -10,6 +10,9 @@ All randomness comes from the crate's integer hash (test_util.rs:9-46) so inputs
 * bounce_rays        -- one cosine-weighted diffuse bounce per primary hit (geometric normal, seeded).
 * thin_triangles     -- long thin triangles (aspect ratio >= 50) scattered over the unit square, to exercise
                         spatial pre-splits.
+* large_workload     -- a 2896 x 2896 heightfield plus 500,000 thin triangles, incoherent closest-hit rays (origins
+                        uniform on a sphere around the scene, uniform directions) and shadow rays from seeded
+                        surface points toward a seeded point light (tmax = distance to the light).
 * icosphere_batch    -- many independent icosphere meshes (counts log-uniform, rounded down to 20*4^s), vertex
                         radius jitter +-10% keyed on the vertex position (shared vertices move together), and a
                         ray batch per mesh from a radius-2 sphere toward uniform points in the mesh AABB.
@@ -176,6 +179,37 @@ def thin_triangles(n, seed=0, min_len=0.05, max_len=0.5):
     v0 = start - perp * (width / 2)[:, None]
     v1 = start + perp * (width / 2)[:, None]
     return triangles_from_vertices(v0.astype(np.float32), v1.astype(np.float32), end.astype(np.float32))
+
+
+def large_workload(grid=2896, thin=500000, n_rays=1 << 22, n_shadow=1 << 22, seed=0):
+    """Returns (tris, closest-hit rays, shadow rays, light position)."""
+    tris = np.concatenate([heightfield_scene(grid, 0, seed), thin_triangles(thin, seed)])
+    f = np.uint32(seed) + np.uint32(9000)
+    i = np.arange(n_rays, dtype=np.uint32)
+    center = np.array([0.5, 0.1, 0.5], dtype=np.float32)
+    o = center + uniform_sample_sphere(hash_noise(i, 0, f), hash_noise(i, 1, f)) * np.float32(0.8)
+    d = uniform_sample_sphere(hash_noise(i, 2, f), hash_noise(i, 3, f))
+    rays = make_rays(o, d, 0.0, 1e30)
+    # shadow rays: seeded points on seeded triangles, nudged along the geometric normal toward the light
+    light = np.array([0.2 + 0.6 * float(hash_noise(np.uint32(0), 10, f)), 1.5,
+                      0.2 + 0.6 * float(hash_noise(np.uint32(0), 11, f))])
+    k = np.arange(n_shadow, dtype=np.uint32)
+    t = (uhash2(k, f + np.uint32(7)) % np.uint32(len(tris))).astype(np.int64)
+    a = hash_noise(k, 4, f).astype(np.float64)
+    b = hash_noise(k, 5, f).astype(np.float64)
+    flip = a + b > 1.0
+    a[flip], b[flip] = 1.0 - a[flip], 1.0 - b[flip]
+    v0 = tris["v0"][t, :3].astype(np.float64)
+    p = v0 + a[:, None] * (tris["v1"][t, :3] - v0) + b[:, None] * (tris["v2"][t, :3] - v0)
+    nrm = triangle_normals(tris[t])
+    to_light = light[None, :] - p
+    nrm[np.sum(nrm * to_light, axis=1) < 0] *= -1.0
+    p = p + nrm * 1e-4
+    to_light = light[None, :] - p
+    dist = np.linalg.norm(to_light, axis=1)
+    shadow = make_rays(p.astype(np.float32), (to_light / dist[:, None]).astype(np.float32), 0.0, 0.0)
+    shadow["tmax"] = (dist * (1.0 - 1e-4)).astype(np.float32)
+    return tris, rays, shadow, light
 
 
 ICO_MAX_SUBDIV = 5
