@@ -14,7 +14,9 @@ restatement of the crate's CPU path (oracle/), timed on this host's cores in thi
 
 The "batched" object reports the batched form on 4,096 seeded icospheres (~20 M triangles, 1,024 rays each): one
 shared build and one traversal launch per step, next to the plain form (a build_cwbvh_from_tris call and a
-traversal launch per mesh) on the same inputs, with a check that both give identical hits.
+traversal launch per mesh) on the same inputs, with a check that both give identical hits.  The "large" object
+reports the 17.3 M-triangle scene built with pre-splits and reinsertion, traced with 4M incoherent closest-hit rays
+and 4M shadow rays.
 """
 import argparse
 import json
@@ -149,6 +151,197 @@ def bench_batch(args, ob, ctx):
             "cwbvh_nodes": info["total_nodes"], "hit_fraction": round(float(np.isfinite(batch_hits["t"]).mean()), 4)}
 
 
+def bench_large(args, ob, ctx):
+    """Large scene with the quality passes: a 2896 x 2896 heightfield (16,773,632 triangles) plus 500,000 long thin
+    triangles; spatial pre-splits capped at 1.3x the triangle count and reinsertion at ratio 0.05 with the ratio
+    sequence [1, 1/3] (two optimizer iterations); 4M incoherent closest-hit rays and 4M shadow (any-hit) rays."""
+    from obvhs_amd import scenes
+    log("generating the large scene")
+    tris, rays, shadow, light = scenes.large_workload(args.large_grid, args.large_thin, args.large_rays,
+                                                      args.large_rays, seed=0)
+    n = len(tris)
+    d_tris, d_rays, d_shadow = ctx.upload(tris), ctx.upload(rays), ctx.upload(shadow)
+    del tris
+    params = ob.BvhBuildParams(pre_split=True, ploc_search_distance=14, search_depth_threshold=0,
+                               reinsertion_batch_ratio=0.05, max_prims_per_leaf=3)
+    cap = int(n * 1.3)
+    seq = [1.0, 1.0 / 3.0]
+    log(f"large build over {n} triangles")
+    cw = ob.build_cwbvh_from_tris(ctx, d_tris, params, split_max_total=cap, ratio_sequence=seq)
+    ms = []
+    for _ in range(max(1, min(args.build_reps, 3))):
+        ctx.timer_start()
+        ob.build_cwbvh_from_tris(ctx, d_tris, params, reuse=cw, split_max_total=cap, ratio_sequence=seq)
+        ms.append(ctx.timer_stop())
+    build_ms = float(np.median(ms))
+    # the same build without the quality passes, for reference
+    plain_cw = ob.build_cwbvh_from_tris(ctx, d_tris, ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3))
+    ctx.timer_start()
+    ob.build_cwbvh_from_tris(ctx, d_tris, ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3),
+                             reuse=plain_cw)
+    plain_ms = ctx.timer_stop()
+    d_bvh_tris = cw.reorder_triangles(d_tris)
+    lib = ob.native.lib()
+    nr, ns = len(rays), len(shadow)
+    hits = ob.DeviceArray(ctx, ob.RAY_HIT, nr)
+    miss = ob.DeviceArray(ctx, np.uint8, ns)
+
+    def closest():
+        ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, cw.handle, d_bvh_tris.ptr, d_rays.ptr, nr,
+                                                     hits.ptr), ctx.handle)
+
+    def anyhit():
+        ob.native.check(lib.obvhs_cwbvh_ray_traverse_miss(ctx.handle, cw.handle, d_bvh_tris.ptr, d_shadow.ptr, ns,
+                                                          miss.ptr), ctx.handle)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    tc, ta = timed(closest), timed(anyhit)
+    h, m = hits.to_host(), miss.to_host()
+    st = cw.stats
+    return {"workload": f"heightfield {args.large_grid}x{args.large_grid}x2 + {args.large_thin} thin = {n} tris; "
+                        f"pre-splits capped at 1.3n, reinsertion 0.05 x [1, 1/3]; {nr} closest-hit + {ns} any-hit rays",
+            "leaves_after_splits": int(st.prim_count), "build_ms": round(build_ms, 3),
+            "build_mtris_per_s": round(n / build_ms / 1e3, 2), "build_ms_without_quality_passes": round(plain_ms, 3),
+            "closest_mrays_per_s": round(nr / tc / 1e6, 2), "anyhit_mrays_per_s": round(ns / ta / 1e6, 2),
+            "closest_hit_fraction": round(float(np.isfinite(h["t"]).mean()), 4),
+            "shadowed_fraction": round(float((m == 0).mean()), 4), "cwbvh_nodes": int(st.cwbvh_node_count)}
+
+
+def bench_serve(args, ob, ctx):
+    """Serving: 64 resident scenes (100K-4M triangles, heightfields and soups), an open-loop Poisson request stream
+    (90% ray batches of log-uniform size in [1K, 1M] rays, half coherent pinhole and half incoherent; 10% refit or
+    rebuild after a +-1% vertex jitter), replayed through the server; beside it the plain form on the same trace
+    requests (one launch and one synchronisation per request, back to back)."""
+    import ctypes as C
+    from obvhs_amd import scenes
+    rng = np.random.default_rng(7)
+    log(f"generating {args.serve_scenes} serving scenes")
+    sc = scenes.serving_scenes(args.serve_scenes, seed=0)
+    params = ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3)
+    server = ob.Server(0, hbm_budget_bytes=200 << 30, max_inflight=args.serve_inflight)
+    d_jit = []
+    plain = []
+    for i, (tris, kind) in enumerate(sc):
+        d = ctx.upload(tris)
+        server.add_scene(d, params)
+        cw = ob.build_cwbvh_from_tris(ctx, d, params)
+        plain.append((cw, cw.reorder_triangles(d)))
+        d_jit.append(ctx.upload(scenes.jitter_triangles(tris, seed=i, frac=0.01)))
+        del d
+    # ray pools per scene kind: 1M coherent pinhole rays, 1M incoherent rays
+    pool = 1 << 20
+    side = 1024
+    hf_pin = scenes.pinhole_rays(side, side, eye=(0.5, 0.45, -0.35), target=(0.5, 0.0, 0.55))
+    soup_pin = scenes.pinhole_rays(side, side, eye=(0.5, 0.5, -1.5))
+    k = np.arange(pool, dtype=np.uint32)
+    from obvhs_amd.test_util import hash_noise, uniform_sample_sphere
+    o = np.stack([hash_noise(k, a, 13000) for a in range(3)], axis=1)
+    d = uniform_sample_sphere(hash_noise(k, 3, 13000), hash_noise(k, 4, 13000))
+    incoherent = ob.make_rays(o, d, 0.0, 1e30)
+    hf0 = plain[0]
+    bhits = hf0[0].ray_traverse(hf_pin, hf0[1])
+    hf_bounce = scenes.bounce_rays(hf_pin, bhits, hf0[1].to_host(), seed=3)
+    hf_bounce = np.concatenate([hf_bounce, incoherent])[:pool]
+    pools = {("heightfield", 0): ctx.upload(hf_pin), ("heightfield", 1): ctx.upload(hf_bounce),
+             ("soup", 0): ctx.upload(soup_pin), ("soup", 1): ctx.upload(incoherent)}
+    arena_n = 1 << 25
+    out_arena = ob.DeviceArray(ctx, ob.RAY_HIT, arena_n)
+    ray_sz, hit_sz = ob.RAY.itemsize, ob.RAY_HIT.itemsize
+
+    def make_requests(count, rate):
+        reqs, t, cursor = [], 0.0, 0
+        for q in range(count):
+            t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+            s = int(rng.integers(0, len(sc)))
+            if rng.random() < 0.1:
+                reqs.append(dict(arrival_ns=int(t * 1e9), kind=ob.api.REQ_REFIT if rng.random() < 0.5
+                                 else ob.api.REQ_REBUILD, scene=s, tris=d_jit[s]))
+                continue
+            n = int(np.exp(rng.uniform(np.log(1000.0), np.log(1 << 20))))
+            which = int(rng.integers(0, 2))
+            src = pools[(sc[s][1], which)]
+            off = int(rng.integers(0, pool - n + 1))
+            if cursor + n > arena_n:
+                cursor = 0
+            rays = _Slice(src.ptr.value + off * ray_sz, n)
+            out = _Slice(out_arena.ptr.value + cursor * hit_sz, n)
+            cursor += n
+            reqs.append(dict(arrival_ns=int(t * 1e9), kind=ob.api.REQ_TRACE_CLOSEST, scene=s, rays=rays, out=out,
+                             n=n, which=which))
+        return reqs
+
+    def summarize(reqs, infos):
+        tr = [(q, x) for q, x in zip(reqs, infos) if q["kind"] == ob.api.REQ_TRACE_CLOSEST]
+        up = [(q, x) for q, x in zip(reqs, infos) if q["kind"] != ob.api.REQ_TRACE_CLOSEST]
+        lat = np.array([(x["done_ns"] - x["submit_ns"]) * 1e-6 for _, x in tr])
+        ulat = np.array([(x["done_ns"] - x["submit_ns"]) * 1e-6 for _, x in up]) if up else np.zeros(1)
+        rays = sum(q["n"] for q, _ in tr)
+        t0 = min(x["submit_ns"] for x in infos)
+        t1 = max(x["done_ns"] for x in infos)
+        return {"requests": len(reqs), "trace_requests": len(tr), "updates": len(up),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "update_p50_ms": round(float(np.percentile(ulat, 50)), 3),
+                "update_p99_ms": round(float(np.percentile(ulat, 99)), 3),
+                "mrays_per_s": round(rays / ((t1 - t0) * 1e-9) / 1e6, 2),
+                "mean_requests_per_launch": round(float(np.mean([x["batch_requests"] for _, x in tr])), 2),
+                "errors": int(sum(1 for x in infos if x["status"] != "OBVHS_OK"))}
+
+    log("serving: Poisson replay")
+    reqs = make_requests(args.serve_requests, args.serve_rate)
+    poisson = summarize(reqs, server.replay(reqs))
+    log("serving: burst replay (capacity)")
+    burst_reqs = make_requests(args.serve_requests, 0.0)
+    burst = summarize(burst_reqs, server.replay(burst_reqs))
+    st = server.stats()
+    server.close()
+    # plain form: each trace request alone, back to back
+    lib = ob.native.lib()
+    traces = [q for q in reqs if q["kind"] == ob.api.REQ_TRACE_CLOSEST]
+    service = []
+    ctx.synchronize()
+    for q in traces:
+        cw, bt = plain[q["scene"]]
+        t0 = time.perf_counter()
+        ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, cw.handle, bt.ptr, C.c_void_p(q["rays"].ptr.value),
+                                                     q["n"], C.c_void_p(q["out"].ptr.value)), ctx.handle)
+        ctx.synchronize()
+        service.append(time.perf_counter() - t0)
+    plain_rays = sum(q["n"] for q in traces)
+    # the plain form's latency under the same arrivals: one request at a time, first come first served
+    free_at, plain_lat = 0.0, []
+    for q, sv in zip(traces, service):
+        a = q["arrival_ns"] * 1e-9
+        start = max(a, free_at)
+        free_at = start + sv
+        plain_lat.append((free_at - a) * 1e3)
+    plain = {"mrays_per_s": round(plain_rays / sum(service) / 1e6, 2),
+             "p50_ms_same_arrivals": round(float(np.percentile(plain_lat, 50)), 3),
+             "p99_ms_same_arrivals": round(float(np.percentile(plain_lat, 99)), 3)}
+    return {"workload": f"{len(sc)} resident scenes ({sum(len(t) for t, _ in sc)} tris, 100K-4M each); "
+                        f"{args.serve_requests} requests, 90% ray batches of 1K-1M rays, 10% refit/rebuild",
+            "poisson_rate_per_s": args.serve_rate, "served": poisson, "served_burst": burst, "plain": plain,
+            "served_vs_plain_mrays": round(burst["mrays_per_s"] / plain["mrays_per_s"], 2),
+            "launches": st["launches"], "resident_gb": round(st["resident_bytes"] / 2 ** 30, 2)}
+
+
+class _Slice:
+    """A device pointer + length for requests that name part of a pool."""
+
+    def __init__(self, ptr, count):
+        import ctypes as C
+        self.ptr = C.c_void_p(ptr)
+        self.count = count
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +356,13 @@ def main():
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 20)
     ap.add_argument("--batch-meshes", type=int, default=4096, help="0 skips the batched leg")
     ap.add_argument("--batch-rays-per-mesh", type=int, default=1024)
+    ap.add_argument("--large-grid", type=int, default=2896, help="0 skips the large-scene leg")
+    ap.add_argument("--large-thin", type=int, default=500000)
+    ap.add_argument("--large-rays", type=int, default=1 << 22)
+    ap.add_argument("--serve-scenes", type=int, default=64, help="0 skips the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-inflight", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -269,6 +469,10 @@ def main():
     }
     if args.batch_meshes > 0:
         out["batched"] = bench_batch(args, ob, ctx)
+    if args.large_grid > 0:
+        out["large"] = bench_large(args, ob, ctx)
+    if args.serve_scenes > 0:
+        out["serving"] = bench_serve(args, ob, ctx)
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
         log(f"cpu baseline with {threads} threads")

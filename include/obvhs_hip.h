@@ -235,6 +235,69 @@ obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch
                                            const uint32_t* d_ray_mesh, uint32_t n, uint8_t* d_miss);
 obvhs_status obvhs_batch_destroy(obvhs_batch* b);
 
+/* ---- serving (resident BVHs, continuous batching) ----
+ * No single reference counterpart: a server keeps scenes (CWBVH + triangles) resident in HBM, coalesces ray
+ * requests into shared traversal launches, runs refit / rebuild updates on a second stream, and admits requests in
+ * submission order while the HBM budget allows.  Each request's result equals the single-request call:
+ * CwBvh::ray_traverse / ray_traverse_miss (cwbvh/mod.rs:169-225) on the scene version current when it was
+ * submitted; a refit is Bvh2::refit_all + bvh2_to_cwbvh (bvh2/mod.rs:506-569), a rebuild build_cwbvh_from_tris
+ * (cwbvh/builder.rs:20-85) with the scene's parameters.  Device buffers passed in requests must stay valid until
+ * obvhs_server_wait returns for that request. */
+typedef struct obvhs_server obvhs_server;
+enum {
+    OBVHS_REQ_TRACE_CLOSEST = 0, /* out: RayHit[n] */
+    OBVHS_REQ_TRACE_MISS = 1,    /* out: u8[n], 1 = the ray hit nothing */
+    OBVHS_REQ_REFIT = 2,         /* moved triangles, same count and order */
+    OBVHS_REQ_REBUILD = 3        /* new triangle set */
+};
+typedef struct obvhs_server_config {
+    uint64_t hbm_budget_bytes;  /* resident scenes + in-flight request buffers (0 = 200 GiB) */
+    uint32_t max_batch_rays;    /* rays per coalesced launch (0 = 8 Mi) */
+    uint32_t max_inflight;      /* coalesced launches in flight on the trace stream (0 = 2, at most 8) */
+} obvhs_server_config;
+typedef struct obvhs_request_info {
+    obvhs_status status;
+    uint32_t kind;
+    uint32_t scene;
+    uint32_t version;         /* scene version read (traces) or produced (updates) */
+    uint64_t batch;           /* coalesced launch that carried a trace */
+    uint32_t batch_requests;  /* requests in that launch */
+    uint32_t pad;
+    int64_t submit_ns, start_ns, done_ns;  /* server clock */
+} obvhs_request_info;
+typedef struct obvhs_server_request {
+    uint64_t arrival_ns;  /* offset from the start of obvhs_server_replay */
+    uint32_t kind;
+    uint32_t scene;
+    const void* rays;     /* traces: device Ray[n] */
+    void* out;            /* traces: device RayHit[n] or u8[n] */
+    const void* tris;     /* updates: device Triangle[n] */
+    uint32_t n;
+    uint32_t pad;
+} obvhs_server_request;
+typedef struct obvhs_server_stats {
+    uint64_t launches, rays, segments, resident_bytes;
+    uint32_t scenes;
+    uint32_t pad;
+} obvhs_server_stats;
+obvhs_status obvhs_server_create(int device, const obvhs_server_config* cfg, obvhs_server** out);
+obvhs_status obvhs_server_destroy(obvhs_server* server);
+/* Builds a scene (version 0) from device triangles with the given build parameters (blocking). */
+obvhs_status obvhs_server_add_scene(obvhs_server* server, const obvhs_triangle* d_tris, uint32_t n,
+                                    const obvhs_build_params* params, uint32_t* out_scene);
+obvhs_status obvhs_server_submit_trace(obvhs_server* server, uint32_t scene, uint32_t kind, const obvhs_ray* d_rays,
+                                       uint32_t n, void* d_out, uint64_t* out_id);
+obvhs_status obvhs_server_submit_update(obvhs_server* server, uint32_t scene, uint32_t kind,
+                                        const obvhs_triangle* d_tris, uint32_t n, uint64_t* out_id);
+/* Blocks until the request is done; fills info (may be NULL) and forgets the request. */
+obvhs_status obvhs_server_wait(obvhs_server* server, uint64_t id, obvhs_request_info* info);
+/* Submits list[i] at list[i].arrival_ns after the call starts (open-loop load), then waits for all of them. */
+obvhs_status obvhs_server_replay(obvhs_server* server, const obvhs_server_request* list, uint32_t count,
+                                 obvhs_request_info* infos);
+obvhs_status obvhs_server_scene_info(obvhs_server* server, uint32_t scene, uint32_t* version, uint32_t* prim_count,
+                                     uint32_t* cwbvh_nodes);
+obvhs_status obvhs_server_get_stats(obvhs_server* server, obvhs_server_stats* stats);
+
 /* Host copy of the CWBVH exponent table: entry E is the largest 23-bit mantissa M with
  * log2f(2^(E-127) * (1 + M/2^23)) == E - 127 under the host libm, i.e. the inputs whose
  * exp2(ceil(log2(x))) stays at 2^(E-127) in bvh2_to_cwbvh.rs:85-88.  Needs no device. */

@@ -499,6 +499,97 @@ def build_cwbvh_batch(ctx, triangles, offsets, config, reuse=None):
     return CwBvhBatch(ctx, out)
 
 
+REQ_TRACE_CLOSEST, REQ_TRACE_MISS, REQ_REFIT, REQ_REBUILD = 0, 1, 2, 3
+
+
+def _info_dict(inf):
+    return dict(status=native.STATUS.get(inf.status, inf.status), kind=inf.kind, scene=inf.scene,
+                version=inf.version, batch=inf.batch, batch_requests=inf.batch_requests, submit_ns=inf.submit_ns,
+                start_ns=inf.start_ns, done_ns=inf.done_ns)
+
+
+class Server:
+    """Resident scenes, coalesced ray requests and overlapped refit / rebuild updates (the C server).
+
+    Requests name device arrays (DeviceArray of a client Context); they must stay alive until waited for."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_rays=0, max_inflight=0):
+        cfg = native.ServerConfig(int(hbm_budget_bytes), int(max_batch_rays), int(max_inflight))
+        h = C.c_void_p()
+        check(lib().obvhs_server_create(int(device), C.byref(cfg), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().obvhs_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_scene(self, d_tris, params):
+        """Builds a resident scene from a device Triangle array (copied) with BvhBuildParams."""
+        sid = C.c_uint32()
+        p = params.to_c()
+        check(lib().obvhs_server_add_scene(self.handle, d_tris.ptr, d_tris.count, C.byref(p), C.byref(sid)))
+        return sid.value
+
+    def submit_trace(self, scene, d_rays, d_out, miss=False, n=None):
+        rid = C.c_uint64()
+        n = d_rays.count if n is None else n
+        check(lib().obvhs_server_submit_trace(self.handle, int(scene), REQ_TRACE_MISS if miss else REQ_TRACE_CLOSEST,
+                                              d_rays.ptr, n, d_out.ptr, C.byref(rid)))
+        return rid.value
+
+    def submit_update(self, scene, d_tris, rebuild=False):
+        rid = C.c_uint64()
+        check(lib().obvhs_server_submit_update(self.handle, int(scene), REQ_REBUILD if rebuild else REQ_REFIT,
+                                               d_tris.ptr, d_tris.count, C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid):
+        inf = native.RequestInfo()
+        lib().obvhs_server_wait(self.handle, int(rid), C.byref(inf))
+        return _info_dict(inf)
+
+    def replay(self, requests):
+        """requests: list of dicts {arrival_ns, kind, scene, rays, out, tris} (device arrays); submitted open-loop
+        at their arrival offsets, then all waited for.  Returns one info dict per request."""
+        arr = (native.ServerRequest * len(requests))()
+        for i, q in enumerate(requests):
+            kind = q["kind"]
+            arr[i].arrival_ns = int(q.get("arrival_ns", 0))
+            arr[i].kind = kind
+            arr[i].scene = int(q["scene"])
+            if kind in (REQ_TRACE_CLOSEST, REQ_TRACE_MISS):
+                arr[i].rays, arr[i].out, arr[i].n = q["rays"].ptr, q["out"].ptr, q.get("n", q["rays"].count)
+            else:
+                arr[i].tris, arr[i].n = q["tris"].ptr, q["tris"].count
+        infos = (native.RequestInfo * len(requests))()
+        lib().obvhs_server_replay(self.handle, arr, len(requests), infos)
+        return [_info_dict(x) for x in infos]
+
+    def scene_info(self, scene):
+        v, p, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().obvhs_server_scene_info(self.handle, int(scene), C.byref(v), C.byref(p), C.byref(c)))
+        return dict(version=v.value, prim_count=p.value, cwbvh_nodes=c.value)
+
+    def stats(self):
+        st = native.ServerStats()
+        check(lib().obvhs_server_get_stats(self.handle, C.byref(st)))
+        return dict(launches=st.launches, rays=st.rays, segments=st.segments, resident_bytes=st.resident_bytes,
+                    scenes=st.scenes)
+
+
 def split_aabbs_preset(ctx, triangles, max_total=0):
     """The pre-split leaf set of build_cwbvh_from_tris (cwbvh/builder.rs:26-55): (Aabb array, u32 indices)."""
     d, n = _as_device(ctx, triangles, TRIANGLE)
@@ -520,5 +611,5 @@ def log2_threshold_table():
     return out
 
 
-__all__ = ["CwBvhBatch", "build_cwbvh_batch", "split_aabbs_preset", "PlocSearchDistance", "SortPrecision", "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder",
+__all__ = ["Server", "CwBvhBatch", "build_cwbvh_batch", "split_aabbs_preset", "PlocSearchDistance", "SortPrecision", "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder",
            "CwBvh", "bvh2_to_cwbvh", "build_cwbvh_from_tris", "log2_threshold_table", "ObvhsError"]

@@ -145,6 +145,21 @@ hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, u
                                   const uint32_t* ray_mesh, const obvhs_triangle* tris, const obvhs_ray* rays,
                                   uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss, hipStream_t stream);
 
+// One request's share of a coalesced traversal launch: rays [begin, begin + count) of the launch are rays[0..count)
+// of this request, traced against one CWBVH, results to out (RayHit[] in closest mode, u8[] in miss mode).
+struct TraceSegment {
+    const void* nodes;
+    const void* tris;
+    const void* rays;
+    void* out;
+    uint32_t node_count;
+    uint32_t prim_count;
+    uint32_t begin;
+    uint32_t count;
+};
+hipError_t launch_traverse_segments(const TraceSegment* segs, uint32_t nseg, uint32_t n, int mode,
+                                    hipStream_t stream);
+
 // quality.hip
 obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32_t>& meta_host, uint32_t& levels);
 obvhs_status cost_table_levels(obvhs_context* ctx, obvhs_bvh2* bvh, const DpView& dp, uint32_t* d_err);

@@ -110,17 +110,23 @@ __device__ __forceinline__ float tri_intersect(const float4& v0, const float4& v
     return INFINITY;
 }
 
-template <int MODE, bool COUNT, bool MESHES = false>
+// Where a launch's rays come from: one BVH (SRC_PLAIN), a mesh of a batched build per ray (SRC_MESHES), or request
+// segments that each name their own BVH, ray array and output array (SRC_SEGMENTS, the serving path).
+constexpr int SRC_PLAIN = 0, SRC_MESHES = 1, SRC_SEGMENTS = 2;
+
+template <int MODE, bool COUNT, int SRC = SRC_PLAIN>
 __global__ void __launch_bounds__(TRAV_THREADS)
     k_traverse(const uint4* __restrict__ nodes, uint32_t node_count, uint32_t prim_count,
                const float4* __restrict__ tris, const float4* __restrict__ rays, uint32_t n, obvhs_ray_hit* __restrict__ hits,
                uint8_t* __restrict__ miss_out, uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests,
-               const MeshDesc* __restrict__ meshes = nullptr, const uint32_t* __restrict__ ray_mesh = nullptr) {
+               const MeshDesc* __restrict__ meshes = nullptr, const uint32_t* __restrict__ ray_mesh = nullptr,
+               const TraceSegment* __restrict__ segs = nullptr) {
     __shared__ uint2 stack[STACK_SIZE][TRAV_THREADS];
     const uint32_t lane = threadIdx.x;
     const uint32_t i = blockIdx.x * TRAV_THREADS + lane;
     if (i >= n) return;
-    if (MESHES) {  // batched: the ray's mesh selects its CWBVH and triangle slices; node_count = mesh count
+    size_t ri = i;  // index into rays and outputs
+    if (SRC == SRC_MESHES) {  // batched: the ray's mesh selects its CWBVH and triangle slices; node_count = mesh count
         const uint32_t mi = ray_mesh[i];
         if (mi < node_count) {
             const MeshDesc md = meshes[mi];
@@ -133,8 +139,24 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             prim_count = 0u;
         }
     }
-    const float4 ra = rays[4 * (size_t)i], rb = rays[4 * (size_t)i + 1], rc = rays[4 * (size_t)i + 2],
-                 rd = rays[4 * (size_t)i + 3];
+    if (SRC == SRC_SEGMENTS) {  // node_count = segment count; segments are sorted by `begin`, the first begins at 0
+        uint32_t lo = 0, hi = node_count;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (segs[mid].begin <= i) lo = mid;
+            else hi = mid;
+        }
+        const TraceSegment sg = segs[lo];
+        nodes = reinterpret_cast<const uint4*>(sg.nodes);
+        tris = reinterpret_cast<const float4*>(sg.tris);
+        rays = reinterpret_cast<const float4*>(sg.rays);
+        hits = reinterpret_cast<obvhs_ray_hit*>(sg.out);
+        miss_out = reinterpret_cast<uint8_t*>(sg.out);
+        node_count = sg.node_count;
+        prim_count = sg.prim_count;
+        ri = i - sg.begin;
+    }
+    const float4 ra = rays[4 * ri], rb = rays[4 * ri + 1], rc = rays[4 * ri + 2], rd = rays[4 * ri + 3];
     const RayData r{ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, rc.x, rc.y, rc.z, rd.x, rd.y};
     const uint32_t oct = octant_inv4(r.dx, r.dy, r.dz);
     float tmax = r.tmax;
@@ -207,10 +229,10 @@ __global__ void __launch_bounds__(TRAV_THREADS)
         h.geometry_id = INVALID32;
         h.instance_id = INVALID32;
         h.t = hit_t;
-        reinterpret_cast<uint4*>(hits)[i] = make_uint4(h.primitive_id, h.geometry_id, h.instance_id,
-                                                       __float_as_uint(h.t));
+        reinterpret_cast<uint4*>(hits)[ri] = make_uint4(h.primitive_id, h.geometry_id, h.instance_id,
+                                                        __float_as_uint(h.t));
     } else {
-        miss_out[i] = is_miss ? 1 : 0;
+        miss_out[ri] = is_miss ? 1 : 0;
     }
     if (COUNT) {
         node_visits[i] = nv;
@@ -256,11 +278,28 @@ hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, u
     const float4* tr = reinterpret_cast<const float4*>(tris);
     const float4* ry = reinterpret_cast<const float4*>(rays);
     if (mode == 0)
-        k_traverse<0, false, true><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits, miss, nullptr,
+        k_traverse<0, false, SRC_MESHES><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits, miss, nullptr,
                                                                         nullptr, meshes, ray_mesh);
     else
-        k_traverse<1, false, true><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits, miss, nullptr,
+        k_traverse<1, false, SRC_MESHES><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits, miss, nullptr,
                                                                         nullptr, meshes, ray_mesh);
+    return hipGetLastError();
+}
+
+}  // namespace obvhs
+
+namespace obvhs {
+
+hipError_t launch_traverse_segments(const TraceSegment* segs, uint32_t nseg, uint32_t n, int mode,
+                                    hipStream_t stream) {
+    if (n == 0 || nseg == 0) return hipSuccess;
+    const uint32_t blocks = (n + TRAV_THREADS - 1) / TRAV_THREADS;
+    if (mode == 0)
+        k_traverse<0, false, SRC_SEGMENTS><<<blocks, TRAV_THREADS, 0, stream>>>(
+            nullptr, nseg, 0u, nullptr, nullptr, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, segs);
+    else
+        k_traverse<1, false, SRC_SEGMENTS><<<blocks, TRAV_THREADS, 0, stream>>>(
+            nullptr, nseg, 0u, nullptr, nullptr, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, segs);
     return hipGetLastError();
 }
 

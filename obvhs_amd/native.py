@@ -42,6 +42,26 @@ class BuildExt(C.Structure):
     _fields_ = [("split_max_total", C.c_uint64), ("ratio_sequence", C.c_void_p), ("ratio_sequence_len", C.c_uint32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_rays", C.c_uint32), ("max_inflight", C.c_uint32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("kind", C.c_uint32), ("scene", C.c_uint32), ("version", C.c_uint32),
+                ("batch", C.c_uint64), ("batch_requests", C.c_uint32), ("pad", C.c_uint32),
+                ("submit_ns", C.c_int64), ("start_ns", C.c_int64), ("done_ns", C.c_int64)]
+
+
+class ServerRequest(C.Structure):
+    _fields_ = [("arrival_ns", C.c_uint64), ("kind", C.c_uint32), ("scene", C.c_uint32), ("rays", C.c_void_p),
+                ("out", C.c_void_p), ("tris", C.c_void_p), ("n", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("rays", C.c_uint64), ("segments", C.c_uint64),
+                ("resident_bytes", C.c_uint64), ("scenes", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class Aabb(C.Structure):
     _fields_ = [("min", C.c_float * 4), ("max", C.c_float * 4)]
 
@@ -104,6 +124,15 @@ SIGNATURES = {
     "obvhs_bvh2_refit_all": (INT, [P, P]),
     "obvhs_bvh2_refit_from_triangles": (INT, [P, P, P]),
     "obvhs_split_aabbs_preset": (INT, [P, P, U32, U64, P, P, U32, PU32]),
+    "obvhs_server_create": (INT, [INT, C.POINTER(ServerConfig), C.POINTER(P)]),
+    "obvhs_server_destroy": (INT, [P]),
+    "obvhs_server_add_scene": (INT, [P, P, U32, C.POINTER(BuildParams), PU32]),
+    "obvhs_server_submit_trace": (INT, [P, U32, U32, P, U32, P, C.POINTER(U64)]),
+    "obvhs_server_submit_update": (INT, [P, U32, U32, P, U32, C.POINTER(U64)]),
+    "obvhs_server_wait": (INT, [P, U64, C.POINTER(RequestInfo)]),
+    "obvhs_server_replay": (INT, [P, C.POINTER(ServerRequest), U32, C.POINTER(RequestInfo)]),
+    "obvhs_server_scene_info": (INT, [P, U32, PU32, PU32, PU32]),
+    "obvhs_server_get_stats": (INT, [P, C.POINTER(ServerStats)]),
 }
 
 _lib = None

@@ -13,6 +13,9 @@ All randomness comes from the crate's integer hash (test_util.rs:9-46) so inputs
 * large_workload     -- a 2896 x 2896 heightfield plus 500,000 thin triangles, incoherent closest-hit rays (origins
                         uniform on a sphere around the scene, uniform directions) and shadow rays from seeded
                         surface points toward a seeded point light (tmax = distance to the light).
+* serving_scenes     -- resident scenes for the serving workload: sizes log-uniform in [100K, 4M] triangles, even
+                        indices heightfields (grid sized to the count), odd indices soups.
+* jitter_triangles   -- seeded per-vertex jitter of +-frac of the mesh's AABB extent (refit / rebuild requests).
 * icosphere_batch    -- many independent icosphere meshes (counts log-uniform, rounded down to 20*4^s), vertex
                         radius jitter +-10% keyed on the vertex position (shared vertices move together), and a
                         ray batch per mesh from a radius-2 sphere toward uniform points in the mesh AABB.
@@ -210,6 +213,40 @@ def large_workload(grid=2896, thin=500000, n_rays=1 << 22, n_shadow=1 << 22, see
     shadow = make_rays(p.astype(np.float32), (to_light / dist[:, None]).astype(np.float32), 0.0, 0.0)
     shadow["tmax"] = (dist * (1.0 - 1e-4)).astype(np.float32)
     return tris, rays, shadow, light
+
+
+def jitter_triangles(tris, seed=0, frac=0.01):
+    """Every vertex moved by U[-frac, frac] of the mesh's AABB extent per axis (seeded)."""
+    out = tris.copy()
+    if len(tris) == 0:
+        return out
+    lo = np.minimum(np.minimum(tris["v0"][:, :3].min(axis=0), tris["v1"][:, :3].min(axis=0)),
+                    tris["v2"][:, :3].min(axis=0)).astype(np.float64)
+    hi = np.maximum(np.maximum(tris["v0"][:, :3].max(axis=0), tris["v1"][:, :3].max(axis=0)),
+                    tris["v2"][:, :3].max(axis=0)).astype(np.float64)
+    ext = hi - lo
+    i = np.arange(len(tris), dtype=np.uint32)
+    f = np.uint32(seed) + np.uint32(11000)
+    for k, name in enumerate(("v0", "v1", "v2")):
+        for a in range(3):
+            u = hash_noise(i, 3 * k + a, f).astype(np.float64) * 2.0 - 1.0
+            out[name][:, a] = (tris[name][:, a].astype(np.float64) + u * frac * ext[a]).astype(np.float32)
+    return out
+
+
+def serving_scenes(count=64, seed=0, lo=100_000, hi=4_000_000):
+    """[(triangles, 'heightfield' | 'soup')]: log-uniform sizes, alternating heightfields and soups (seeded)."""
+    k = np.arange(count, dtype=np.uint32)
+    u = hash_noise(k, 20, np.uint32(seed) + np.uint32(12000)).astype(np.float64)
+    sizes = (lo * np.power(hi / lo, u)).astype(np.int64)
+    out = []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            grid = max(8, int(round(np.sqrt(n / 2.0))))
+            out.append((heightfield_scene(grid, 0, seed=seed * 1000 + i), "heightfield"))
+        else:
+            out.append((soup(int(n), seed=seed * 1000 + i), "soup"))
+    return out
 
 
 ICO_MAX_SUBDIV = 5

@@ -146,3 +146,29 @@ def test_refit_after_jitter_matches_oracle(ctx, reinsert):
     h = re.ray_traverse(rays, re.reorder_triangles(moved))
     bf = O.brute_force(moved, rays)
     np.testing.assert_array_equal(h["t"].view(np.uint32), bf["t"].view(np.uint32))
+
+
+def test_large_pipeline_scaled(ctx):
+    """The large-scene pipeline at 1/64 of its size (362 x 362 heightfield + 7,812 thin triangles): pre-splits capped
+    at 1.3n, reinsertion 0.05 with the sequence [1, 1/3], closest-hit and shadow rays, all against the oracle."""
+    tris, rays, shadow, _ = scenes.large_workload(362, 7812, 1 << 16, 1 << 16, seed=0)
+    print(f"[case] scaled large pipeline n={len(tris)}", flush=True)
+    p = ob.BvhBuildParams(pre_split=True, ploc_search_distance=14, reinsertion_batch_ratio=0.05,
+                          max_prims_per_leaf=3)
+    cap = int(len(tris) * 1.3)
+    seq = [1.0, 1.0 / 3.0]
+    cw = ob.build_cwbvh_from_tris(ctx, tris, p, split_max_total=cap, ratio_sequence=seq)
+    _, rcw, rprims = O.build_cwbvh(tris, 14, 0, 3, pre_split=True, reinsertion_batch_ratio=0.05,
+                                   ratio_sequence=seq, split_max_total=cap)
+    assert len(rprims) > len(tris)
+    same(cw.nodes, rcw, "CWBVH nodes (scaled large)")
+    np.testing.assert_array_equal(cw.primitive_indices, rprims)
+    bt = cw.reorder_triangles(tris)
+    h = cw.ray_traverse(rays, bt)
+    r = O.traverse(rcw, tris[rprims], rays, threads=8)
+    np.testing.assert_array_equal(h["primitive_id"], r["primitive_id"])
+    np.testing.assert_array_equal(h["t"].view(np.uint32), r["t"].view(np.uint32))
+    m = cw.ray_traverse_miss(shadow, bt)
+    rm = O.traverse(rcw, tris[rprims], shadow, mode=1, threads=8)
+    np.testing.assert_array_equal(m, rm)
+    assert 0.05 < (~m).mean() < 0.95 and np.isfinite(h["t"]).mean() > 0.05
