@@ -247,7 +247,7 @@ typedef struct obvhs_server obvhs_server;
 enum {
     OBVHS_REQ_TRACE_CLOSEST = 0, /* out: RayHit[n] */
     OBVHS_REQ_TRACE_MISS = 1,    /* out: u8[n], 1 = the ray hit nothing */
-    OBVHS_REQ_REFIT = 2,         /* moved triangles, same count and order */
+    OBVHS_REQ_REFIT = 2,         /* moved triangles, same count and order (else the request fails) */
     OBVHS_REQ_REBUILD = 3        /* new triangle set */
 };
 typedef struct obvhs_server_config {

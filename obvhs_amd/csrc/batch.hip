@@ -661,9 +661,14 @@ obvhs_status obvhs_batch_ray_traverse(obvhs_context* ctx, const obvhs_batch* b, 
                                       const obvhs_ray* d_rays, const uint32_t* d_ray_mesh, uint32_t n,
                                       obvhs_ray_hit* d_hits) {
     if (!ctx || !b || (n && (!d_rays || !d_ray_mesh || !d_hits))) return OBVHS_ERR_INVALID_ARG;
+    TravScratch ts;
+    if (n) {
+        const obvhs_status ss = traverse_scratch(ctx, ts);
+        if (ss != OBVHS_OK) return ss;
+    }
     hipError_t e = launch_traverse_meshes(b->cw_nodes.as<CwNode>(), b->mesh_table.as<MeshDesc>(),
                                           (uint32_t)b->meshes.size(), d_ray_mesh,
-                                          d_bvh_tris, d_rays, n, 0, d_hits, nullptr, ctx->stream);
+                                          d_bvh_tris, d_rays, n, 0, d_hits, nullptr, ts, ctx->stream);
     if (e != hipSuccess) return set_hip_error(ctx, e, "batch traverse");
     return OBVHS_OK;
 }
@@ -672,9 +677,14 @@ obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch
                                            const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
                                            const uint32_t* d_ray_mesh, uint32_t n, uint8_t* d_miss) {
     if (!ctx || !b || (n && (!d_rays || !d_ray_mesh || !d_miss))) return OBVHS_ERR_INVALID_ARG;
+    TravScratch ts;
+    if (n) {
+        const obvhs_status ss = traverse_scratch(ctx, ts);
+        if (ss != OBVHS_OK) return ss;
+    }
     hipError_t e = launch_traverse_meshes(b->cw_nodes.as<CwNode>(), b->mesh_table.as<MeshDesc>(),
                                           (uint32_t)b->meshes.size(), d_ray_mesh,
-                                          d_bvh_tris, d_rays, n, 1, nullptr, d_miss, ctx->stream);
+                                          d_bvh_tris, d_rays, n, 1, nullptr, d_miss, ts, ctx->stream);
     if (e != hipSuccess) return set_hip_error(ctx, e, "batch traverse miss");
     return OBVHS_OK;
 }

@@ -462,8 +462,13 @@ obvhs_status obvhs_cwbvh_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh* bvh
                                       const obvhs_ray* d_rays, uint32_t n, obvhs_ray_hit* d_hits) {
     if (!ctx || !bvh || (n && (!d_rays || !d_hits))) return OBVHS_ERR_INVALID_ARG;
     if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
+    TravScratch ts;
+    if (n) {
+        const obvhs_status ss = traverse_scratch(ctx, ts);
+        if (ss != OBVHS_OK) return ss;
+    }
     CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0, d_hits, nullptr,
-                            nullptr, nullptr, ctx->stream),
+                            nullptr, nullptr, ts, ctx->stream),
             "traverse");
     return OBVHS_OK;
 }
@@ -473,8 +478,13 @@ obvhs_status obvhs_cwbvh_ray_traverse_miss(obvhs_context* ctx, const obvhs_cwbvh
                                            uint8_t* d_miss) {
     if (!ctx || !bvh || (n && (!d_rays || !d_miss))) return OBVHS_ERR_INVALID_ARG;
     if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
+    TravScratch ts;
+    if (n) {
+        const obvhs_status ss = traverse_scratch(ctx, ts);
+        if (ss != OBVHS_OK) return ss;
+    }
     CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 1, nullptr, d_miss,
-                            nullptr, nullptr, ctx->stream),
+                            nullptr, nullptr, ts, ctx->stream),
             "traverse miss");
     return OBVHS_OK;
 }
@@ -485,8 +495,13 @@ obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cw
                                               uint32_t* d_tri_tests) {
     if (!ctx || !bvh || (n && (!d_rays || !d_hits || !d_node_visits || !d_tri_tests))) return OBVHS_ERR_INVALID_ARG;
     if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
+    TravScratch ts;
+    if (n) {
+        const obvhs_status ss = traverse_scratch(ctx, ts);
+        if (ss != OBVHS_OK) return ss;
+    }
     CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0, d_hits, nullptr,
-                            d_node_visits, d_tri_tests, ctx->stream),
+                            d_node_visits, d_tri_tests, ts, ctx->stream),
             "traverse counted");
     return OBVHS_OK;
 }

@@ -46,6 +46,7 @@ struct BuildScratch {
     DevBuf sort_scratch;
     DevBuf misc;        // bounds keys, states, error word
     DevBuf cw_scratch;  // conversion working memory
+    DevBuf trav_spill;  // traversal stack spill columns
 };
 
 constexpr uint32_t MAX_TREE_LEVELS = 8192;
@@ -70,6 +71,7 @@ struct obvhs_context {
     obvhs::QualityScratch quality;
     void* pinned = nullptr;  // small pinned host staging area
     bool keep_trace = false; // keep Morton codes and the sort permutation of each build (parity tests)
+    uint32_t trav_lanes = 0;  // persistent traversal grid size (0 = not yet queried)
     std::string last_error;
 };
 
@@ -137,13 +139,20 @@ hipError_t launch_reorder_triangles(const obvhs_triangle* tris, const uint32_t* 
                                     obvhs_triangle* out, hipStream_t stream);
 
 // traverse.hip
+// Persistent traversal grid: `lanes` lanes, each with a spill column for the deep part of its group stack.
+struct TravScratch {
+    void* spill = nullptr;
+    uint32_t lanes = 0;
+};
+obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts);
 hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
                            const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,
-                           uint32_t* node_visits, uint32_t* tri_tests, hipStream_t stream);
+                           uint32_t* node_visits, uint32_t* tri_tests, const TravScratch& ts, hipStream_t stream);
 // Batched traversal: ray i is traced against mesh ray_mesh[i] of the MeshDesc table (ids >= mesh_count miss).
 hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, uint32_t mesh_count,
                                   const uint32_t* ray_mesh, const obvhs_triangle* tris, const obvhs_ray* rays,
-                                  uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss, hipStream_t stream);
+                                  uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss, const TravScratch& ts,
+                                  hipStream_t stream);
 
 // One request's share of a coalesced traversal launch: rays [begin, begin + count) of the launch are rays[0..count)
 // of this request, traced against one CWBVH, results to out (RayHit[] in closest mode, u8[] in miss mode).
@@ -158,7 +167,7 @@ struct TraceSegment {
     uint32_t count;
 };
 hipError_t launch_traverse_segments(const TraceSegment* segs, uint32_t nseg, uint32_t n, int mode,
-                                    hipStream_t stream);
+                                    const TravScratch& ts, hipStream_t stream);
 
 // quality.hip
 obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32_t>& meta_host, uint32_t& levels);

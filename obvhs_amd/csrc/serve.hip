@@ -131,6 +131,7 @@ struct obvhs_server {
     obvhs::DevBuf seg_dev;   // TraceSegment ring: max_inflight * 2 launches * max_segments
     obvhs::TraceSegment* seg_host = nullptr;
     uint32_t seg_slot = 0;
+    obvhs::TravScratch trav;  // spill columns of the trace context's persistent traversal grid
 
     // update thread
     std::deque<Req*> updates;
@@ -163,7 +164,9 @@ obvhs_status run_update(obvhs_server* S, Scene* sc, Req* r, int slot) {
     if (dst.used) {  // the trace stream's last launch that read this slot must finish first
         if (hipStreamWaitEvent(st, dst.last_use, 0) != hipSuccess) return OBVHS_ERR_DEVICE;
     }
-    const uint32_t n = r->kind == REQ_REFIT ? src.n : r->n;
+    // a refit keeps the triangle count of the version it refits (checked when it runs: earlier rebuilds may change it)
+    if (r->kind == REQ_REFIT && r->n != src.n) return OBVHS_ERR_INVALID_ARG;
+    const uint32_t n = r->n;
     if (dst.tris.ensure((size_t)n * 48 + 48) != hipSuccess) return OBVHS_ERR_OOM;
     if (n && hipMemcpyAsync(dst.tris.ptr, r->tris, (size_t)n * 48, hipMemcpyDeviceToDevice, st) != hipSuccess)
         return OBVHS_ERR_DEVICE;
@@ -303,7 +306,7 @@ void launch_batch(obvhs_server* S, std::vector<Req*>& ready) {
         if (nseg == 0) continue;
         TraceSegment* ds = S->seg_dev.as<TraceSegment>() + (size_t)slot * MAX_SEGMENTS;
         hipError_t e = hipMemcpyAsync(ds, hs, (size_t)nseg * sizeof(TraceSegment), hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = launch_traverse_segments(ds, nseg, total, mode, st);
+        if (e == hipSuccess) e = launch_traverse_segments(ds, nseg, total, mode, S->trav, st);
         if (e != hipSuccess) {
             for (Req* r : ready)
                 if ((r->kind == REQ_TRACE_MISS) == (mode == 1)) r->status = OBVHS_ERR_DEVICE;
@@ -391,7 +394,13 @@ void scheduler_main(obvhs_server* S) {
                 progress = true;
             }
         }
-        if (!progress) S->cv_work.wait_for(lk, std::chrono::microseconds(S->inflight.empty() ? 2000 : 20));
+        if (!progress) {
+            S->cv_work.wait_for(lk, std::chrono::microseconds(S->inflight.empty() ? 2000 : 20));
+        } else {  // let submitters and waiters take the lock between busy iterations
+            lk.unlock();
+            std::this_thread::yield();
+            lk.lock();
+        }
     }
 }
 
@@ -415,6 +424,7 @@ obvhs_status obvhs_server_create(int device, const obvhs_server_config* cfg, obv
     if (s == OBVHS_OK) s = obvhs_context_create(device, nullptr, &S->uctx);
     const size_t nseg = (size_t)(2 * S->cfg.max_inflight + 2) * MAX_SEGMENTS;
     if (s == OBVHS_OK && S->seg_dev.ensure(nseg * sizeof(TraceSegment)) != hipSuccess) s = OBVHS_ERR_OOM;
+    if (s == OBVHS_OK) s = traverse_scratch(S->tctx, S->trav);
     if (s == OBVHS_OK &&
         hipHostMalloc(reinterpret_cast<void**>(&S->seg_host), nseg * sizeof(TraceSegment), hipHostMallocDefault) !=
             hipSuccess)
@@ -542,11 +552,6 @@ obvhs_status obvhs_server_submit_update(obvhs_server* S, uint32_t scene, uint32_
                                         uint32_t n, uint64_t* out_id) {
     if (!S || !out_id || (kind != OBVHS_REQ_REFIT && kind != OBVHS_REQ_REBUILD) || (n > 0 && !d_tris))
         return OBVHS_ERR_INVALID_ARG;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        if (scene >= S->scenes.size()) return OBVHS_ERR_INVALID_ARG;
-        if (kind == OBVHS_REQ_REFIT && n != S->scenes[scene]->v[S->scenes[scene]->cur].n) return OBVHS_ERR_INVALID_ARG;
-    }
     Req* r = new (std::nothrow) Req();
     if (!r) return OBVHS_ERR_OOM;
     r->kind = kind;

@@ -8,11 +8,20 @@
 //     slot 31, a pop of the empty stack ends traversal;
 //   * the node test is the SSE path of cwbvh/simd.rs:17-100 (mul then add, SSE min/max, tmin clamped to 1e-4,
 //     inclusive compare) and the primitive test is Triangle::intersect (triangle.rs:35-76).
-// The stack lives in LDS, one 64-lane column per wave (entry-major so a wave's pushes hit distinct banks);
-// nodes are fetched as five 16-byte loads, triangles (in primitive_indices order) as three.
+// Stack storage: the bottom LDS_STACK entries of each lane's stack live in LDS (one 64-lane column per wave,
+// entry-major), deeper entries in a per-lane column of an HBM spill array; a short LDS footprint keeps 8 waves per
+// SIMD resident, which the latency-bound walk needs.  Launches are persistent: a fixed grid of lanes (the resident
+// capacity) walks the rays with a stride, so each lane owns one spill column.  Nodes are fetched as five 16-byte
+// loads, triangles (in primitive_indices order) as three.
+#include <algorithm>
+
 #include "common.hpp"
 #include "cwdp.hpp"
 #include "internal.hpp"
+
+#ifndef TRAV_LDS_STACK
+#define TRAV_LDS_STACK 8
+#endif
 
 namespace obvhs {
 
@@ -20,6 +29,7 @@ namespace {
 
 constexpr int TRAV_THREADS = 64;
 constexpr int STACK_SIZE = 32;  // TRAVERSAL_STACK_SIZE, cwbvh/mod.rs:60
+constexpr int LDS_STACK = TRAV_LDS_STACK;  // entries kept in LDS; STACK_SIZE - LDS_STACK spill to HBM
 constexpr float NODE_EPSILON = 0.0001f;  // cwbvh/node.rs:82
 
 struct RayData {
@@ -116,15 +126,24 @@ constexpr int SRC_PLAIN = 0, SRC_MESHES = 1, SRC_SEGMENTS = 2;
 
 template <int MODE, bool COUNT, int SRC = SRC_PLAIN>
 __global__ void __launch_bounds__(TRAV_THREADS)
-    k_traverse(const uint4* __restrict__ nodes, uint32_t node_count, uint32_t prim_count,
-               const float4* __restrict__ tris, const float4* __restrict__ rays, uint32_t n, obvhs_ray_hit* __restrict__ hits,
-               uint8_t* __restrict__ miss_out, uint32_t* __restrict__ node_visits, uint32_t* __restrict__ tri_tests,
-               const MeshDesc* __restrict__ meshes = nullptr, const uint32_t* __restrict__ ray_mesh = nullptr,
-               const TraceSegment* __restrict__ segs = nullptr) {
-    __shared__ uint2 stack[STACK_SIZE][TRAV_THREADS];
+    k_traverse(const uint4* __restrict__ nodes_in, uint32_t node_count_in, uint32_t prim_count_in,
+               const float4* __restrict__ tris_in, const float4* __restrict__ rays_in, uint32_t n,
+               obvhs_ray_hit* __restrict__ hits_in, uint8_t* __restrict__ miss_in, uint32_t* __restrict__ node_visits,
+               uint32_t* __restrict__ tri_tests, const MeshDesc* __restrict__ meshes,
+               const uint32_t* __restrict__ ray_mesh, const TraceSegment* __restrict__ segs,
+               uint2* __restrict__ spill) {
+    __shared__ uint2 stack[LDS_STACK][TRAV_THREADS];
     const uint32_t lane = threadIdx.x;
-    const uint32_t i = blockIdx.x * TRAV_THREADS + lane;
-    if (i >= n) return;
+    const uint32_t lanes = gridDim.x * TRAV_THREADS;
+    const uint32_t gid = blockIdx.x * TRAV_THREADS + lane;
+    uint2* my_spill = spill + gid;  // entry k >= LDS_STACK at my_spill[(k - LDS_STACK) * lanes]
+    for (uint32_t i = gid; i < n; i += lanes) {
+    const uint4* nodes = nodes_in;
+    const float4* tris = tris_in;
+    const float4* rays = rays_in;
+    obvhs_ray_hit* hits = hits_in;
+    uint8_t* miss_out = miss_in;
+    uint32_t node_count = node_count_in, prim_count = prim_count_in;
     size_t ri = i;  // index into rays and outputs
     if (SRC == SRC_MESHES) {  // batched: the ray's mesh selects its CWBVH and triangle slices; node_count = mesh count
         const uint32_t mi = ray_mesh[i];
@@ -200,7 +219,8 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             const uint32_t base = cur.x;
             cur.y &= ~(1u << off);
             if ((cur.y & 0xff000000u) != 0u) {
-                stack[sp][lane] = cur;
+                if (sp < LDS_STACK) stack[sp][lane] = cur;
+                else my_spill[(size_t)(sp - LDS_STACK) * lanes] = cur;
                 sp = min(sp + 1, STACK_SIZE - 1);
             }
             const uint32_t slot_index = (off - 24u) ^ (oct & 0xffu);
@@ -220,7 +240,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
         if (prim.y == 0u && (cur.y & 0xff000000u) == 0u) {
             if (sp == 0) break;
             sp -= 1;
-            cur = stack[sp][lane];
+            cur = sp < LDS_STACK ? stack[sp][lane] : my_spill[(size_t)(sp - LDS_STACK) * lanes];
         }
     }
     if (MODE == 0) {
@@ -238,69 +258,84 @@ __global__ void __launch_bounds__(TRAV_THREADS)
         node_visits[i] = nv;
         tri_tests[i] = tv;
     }
+    }  // rays of this lane
 }
 
 }  // namespace
 
 hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
                            const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,
-                           uint32_t* node_visits, uint32_t* tri_tests, hipStream_t stream) {
+                           uint32_t* node_visits, uint32_t* tri_tests, const TravScratch& ts, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t blocks = (n + TRAV_THREADS - 1) / TRAV_THREADS;
+    const uint32_t blocks = std::min<uint32_t>((n + TRAV_THREADS - 1) / TRAV_THREADS, ts.lanes / TRAV_THREADS);
     const uint4* nd = reinterpret_cast<const uint4*>(nodes);
     const float4* tr = reinterpret_cast<const float4*>(tris);
     const float4* ry = reinterpret_cast<const float4*>(rays);
+    uint2* sp = static_cast<uint2*>(ts.spill);
     if (mode == 0) {
         if (node_visits)
             k_traverse<0, true><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss,
-                                                                     node_visits, tri_tests);
+                                                                     node_visits, tri_tests, nullptr, nullptr,
+                                                                     nullptr, sp);
         else
-            k_traverse<0, false><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss,
-                                                                      nullptr, nullptr);
+            k_traverse<0, false><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits,
+                                                                      miss, nullptr, nullptr, nullptr, nullptr,
+                                                                      nullptr, sp);
     } else {
-        k_traverse<1, false><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss, nullptr,
-                                                                  nullptr);
+        k_traverse<1, false><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss,
+                                                                  nullptr, nullptr, nullptr, nullptr, nullptr, sp);
     }
     return hipGetLastError();
 }
 
-}  // namespace obvhs
-
-namespace obvhs {
-
 hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, uint32_t mesh_count,
-                                  const uint32_t* ray_mesh,
-                                  const obvhs_triangle* tris, const obvhs_ray* rays, uint32_t n, int mode,
-                                  obvhs_ray_hit* hits, uint8_t* miss, hipStream_t stream) {
+                                  const uint32_t* ray_mesh, const obvhs_triangle* tris, const obvhs_ray* rays,
+                                  uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss, const TravScratch& ts,
+                                  hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t blocks = (n + TRAV_THREADS - 1) / TRAV_THREADS;
+    const uint32_t blocks = std::min<uint32_t>((n + TRAV_THREADS - 1) / TRAV_THREADS, ts.lanes / TRAV_THREADS);
     const uint4* nd = reinterpret_cast<const uint4*>(nodes);
     const float4* tr = reinterpret_cast<const float4*>(tris);
     const float4* ry = reinterpret_cast<const float4*>(rays);
+    uint2* sp = static_cast<uint2*>(ts.spill);
     if (mode == 0)
-        k_traverse<0, false, SRC_MESHES><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits, miss, nullptr,
-                                                                        nullptr, meshes, ray_mesh);
+        k_traverse<0, false, SRC_MESHES><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits,
+                                                                              miss, nullptr, nullptr, meshes,
+                                                                              ray_mesh, nullptr, sp);
     else
-        k_traverse<1, false, SRC_MESHES><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits, miss, nullptr,
-                                                                        nullptr, meshes, ray_mesh);
+        k_traverse<1, false, SRC_MESHES><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits,
+                                                                              miss, nullptr, nullptr, meshes,
+                                                                              ray_mesh, nullptr, sp);
     return hipGetLastError();
 }
 
-}  // namespace obvhs
-
-namespace obvhs {
-
 hipError_t launch_traverse_segments(const TraceSegment* segs, uint32_t nseg, uint32_t n, int mode,
-                                    hipStream_t stream) {
+                                    const TravScratch& ts, hipStream_t stream) {
     if (n == 0 || nseg == 0) return hipSuccess;
-    const uint32_t blocks = (n + TRAV_THREADS - 1) / TRAV_THREADS;
+    const uint32_t blocks = std::min<uint32_t>((n + TRAV_THREADS - 1) / TRAV_THREADS, ts.lanes / TRAV_THREADS);
+    uint2* sp = static_cast<uint2*>(ts.spill);
     if (mode == 0)
         k_traverse<0, false, SRC_SEGMENTS><<<blocks, TRAV_THREADS, 0, stream>>>(
-            nullptr, nseg, 0u, nullptr, nullptr, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, segs);
+            nullptr, nseg, 0u, nullptr, nullptr, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, segs, sp);
     else
         k_traverse<1, false, SRC_SEGMENTS><<<blocks, TRAV_THREADS, 0, stream>>>(
-            nullptr, nseg, 0u, nullptr, nullptr, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, segs);
+            nullptr, nseg, 0u, nullptr, nullptr, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, segs, sp);
     return hipGetLastError();
+}
+
+// Spill columns for every lane of a persistent traversal grid: the device's compute units x 32 resident waves.
+obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts) {
+    if (ctx->trav_lanes == 0) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return set_hip_error(ctx, hipGetLastError(), "device properties");
+        ctx->trav_lanes = (uint32_t)std::max(1, prop.multiProcessorCount) * 32u * TRAV_THREADS;
+    }
+    const size_t bytes = (size_t)ctx->trav_lanes * (STACK_SIZE - LDS_STACK) * sizeof(uint2);
+    const hipError_t e = ctx->scratch.trav_spill.ensure(bytes);
+    if (e != hipSuccess) return set_hip_error(ctx, e, "alloc traversal spill");
+    ts.spill = ctx->scratch.trav_spill.ptr;
+    ts.lanes = ctx->trav_lanes;
+    return OBVHS_OK;
 }
 
 }  // namespace obvhs

@@ -167,7 +167,9 @@ def test_invalid_requests(ctx):
         out = ob.DeviceArray(ctx, ob.RAY_HIT, 8)
         with pytest.raises(ob.ObvhsError):
             server.submit_trace(sid + 5, rays, out)
-        with pytest.raises(ob.ObvhsError):  # a refit must keep the triangle count
-            server.submit_update(sid, ctx.upload(scenes.soup(50, seed=2)), rebuild=False)
+        bad = ctx.upload(scenes.soup(50, seed=2))
+        uid = server.submit_update(sid, bad, rebuild=False)  # a refit must keep the triangle count
+        assert server.wait(uid)["status"] == "OBVHS_ERR_INVALID_ARG"
+        assert server.scene_info(sid)["version"] == 0
         rid = server.submit_trace(sid, rays, out)
         assert server.wait(rid)["status"] == "OBVHS_OK"
