@@ -30,11 +30,25 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+PMC_COPY_BYTES = 256 << 20
+# HBM bytes per traversal launch from the FETCH_SIZE / WRITE_SIZE counter runs, scaled by the copy kernel's reading
+# (profiles/heightfield_traversal_hbm_traffic.json); null until those runs exist.
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "heightfield_traversal_hbm_traffic.json")
 METRIC = "Mrays/s CWBVH closest-hit; Mtris/s PLOC build; p50/p99 batch latency"
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_per_launch():
+    """Measured HBM bytes per closest-hit launch (mean of the primary and bounce launches), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return int(round((t["primary"]["bytes"] + t["bounce"]["bytes"]) / 2))
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(tris, rays_all, threads, sample_rays):
@@ -227,7 +241,8 @@ def bench_serve(args, ob, ctx):
     log(f"generating {args.serve_scenes} serving scenes")
     sc = scenes.serving_scenes(args.serve_scenes, seed=0)
     params = ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3)
-    server = ob.Server(0, hbm_budget_bytes=200 << 30, max_inflight=args.serve_inflight)
+    server = ob.Server(0, hbm_budget_bytes=200 << 30, max_inflight=args.serve_inflight,
+                       update_streams=args.serve_update_streams)
     d_jit = []
     plain = []
     for i, (tris, kind) in enumerate(sc):
@@ -303,27 +318,43 @@ def bench_serve(args, ob, ctx):
     burst = summarize(burst_reqs, server.replay(burst_reqs))
     st = server.stats()
     server.close()
-    # plain form: each trace request alone, back to back
+    # plain form: the same requests one at a time through the single-request API (traces: one launch and a
+    # synchronisation each; refits: refit the BVH2 + convert + reorder; rebuilds: build + reorder)
     lib = ob.native.lib()
-    traces = [q for q in reqs if q["kind"] == ob.api.REQ_TRACE_CLOSEST]
+    base = [cw for cw, _ in plain]
+    cur = [cw for cw, _ in plain]
+    btris = [bt for _, bt in plain]
     service = []
     ctx.synchronize()
-    for q in traces:
-        cw, bt = plain[q["scene"]]
+    for q in reqs:
+        s_ = q["scene"]
         t0 = time.perf_counter()
-        ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, cw.handle, bt.ptr, C.c_void_p(q["rays"].ptr.value),
-                                                     q["n"], C.c_void_p(q["out"].ptr.value)), ctx.handle)
+        if q["kind"] == ob.api.REQ_TRACE_CLOSEST:
+            ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, cur[s_].handle, btris[s_].ptr,
+                                                         C.c_void_p(q["rays"].ptr.value), q["n"],
+                                                         C.c_void_p(q["out"].ptr.value)), ctx.handle)
+        elif q["kind"] == ob.api.REQ_REFIT:
+            base[s_].bvh2.refit_from_triangles(q["tris"])
+            cur[s_] = ob.bvh2_to_cwbvh(base[s_].bvh2, 3, True, False)
+            btris[s_] = cur[s_].reorder_triangles(q["tris"])
+        else:
+            base[s_] = ob.build_cwbvh_from_tris(ctx, q["tris"], params, reuse=base[s_])
+            cur[s_] = base[s_]
+            btris[s_] = cur[s_].reorder_triangles(q["tris"])
         ctx.synchronize()
         service.append(time.perf_counter() - t0)
-    plain_rays = sum(q["n"] for q in traces)
+    traces = [(q, sv) for q, sv in zip(reqs, service) if q["kind"] == ob.api.REQ_TRACE_CLOSEST]
+    plain_rays = sum(q["n"] for q, _ in traces)
     # the plain form's latency under the same arrivals: one request at a time, first come first served
     free_at, plain_lat = 0.0, []
-    for q, sv in zip(traces, service):
+    for q, sv in zip(reqs, service):
         a = q["arrival_ns"] * 1e-9
         start = max(a, free_at)
         free_at = start + sv
-        plain_lat.append((free_at - a) * 1e3)
+        if q["kind"] == ob.api.REQ_TRACE_CLOSEST:
+            plain_lat.append((free_at - a) * 1e3)
     plain = {"mrays_per_s": round(plain_rays / sum(service) / 1e6, 2),
+             "trace_only_mrays_per_s": round(plain_rays / sum(sv for _, sv in traces) / 1e6, 2),
              "p50_ms_same_arrivals": round(float(np.percentile(plain_lat, 50)), 3),
              "p99_ms_same_arrivals": round(float(np.percentile(plain_lat, 99)), 3)}
     return {"workload": f"{len(sc)} resident scenes ({sum(len(t) for t, _ in sc)} tris, 100K-4M each); "
@@ -353,6 +384,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true", help="one copy kernel + one traversal per batch, then exit")
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 20)
     ap.add_argument("--batch-meshes", type=int, default=4096, help="0 skips the batched leg")
     ap.add_argument("--batch-rays-per-mesh", type=int, default=1024)
@@ -363,6 +395,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-inflight", type=int, default=2)
+    ap.add_argument("--serve-update-streams", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -408,6 +441,22 @@ def main():
     bhits = ob.DeviceArray(ctx, ob.RAY_HIT, max(1, len(bounce)))
     n_rays = len(prim_rays) + len(bounce)
     log(f"{len(prim_rays)} primary + {len(bounce)} bounce rays")
+
+    if args.pmc_probe:
+        # Counter runs (rocprofv3 --pmc): one copy kernel of known size (the counters' scale), then one primary and
+        # one bounce traversal launch.
+        import torch
+        x = torch.ones(PMC_COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        trace(d_prim, len(prim_rays), hits)
+        trace(d_bounce, len(bounce), bhits)
+        ctx.synchronize()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": PMC_COPY_BYTES, "primary_rays": len(prim_rays),
+                          "bounce_rays": len(bounce)}), flush=True)
+        return
 
     # Per-ray node/triangle test counts for the byte model (instrumented launch, not timed).
     _, nv_p, tv_p = cw.ray_traverse(d_prim, d_bvh_tris, counted=True)
@@ -459,7 +508,7 @@ def main():
                                f"{len(prim_rays)} primary + {len(bounce)} bounce closest-hit rays",
                    "ploc_search_distance": 14, "max_prims_per_leaf": 3},
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),
                      "kernel": "k_traverse<closest>", "avg_kernel_ms": {"primary": round(float(np.mean(kp)), 4),
                                                                        "bounce": round(float(np.mean(kb)), 4)},
                      "bytes_per_step": algo_bytes, "node_tests": node_tests, "tri_tests": tri_tests},

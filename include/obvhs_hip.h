@@ -173,10 +173,12 @@ obvhs_status obvhs_bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh
 /* The pre-split leaf set of build_cwbvh_from_tris with pre_split = true (cwbvh/builder.rs:26-55 +
  * split_aabbs_preset, splits.rs:16-33): writes out_count boxes and triangle indices to caller device arrays of
  * out_capacity entries (OBVHS_ERR_CAPACITY with *out_count set when they do not fit).  max_total as
- * obvhs_build_ext::split_max_total. */
+ * obvhs_build_ext::split_max_total.  The builder's `avg_area` and `largest_half_area` (cwbvh/builder.rs:28-43) are
+ * returned through the last two pointers when they are not NULL. */
 obvhs_status obvhs_split_aabbs_preset(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
                                       uint64_t max_total, obvhs_aabb* d_out_aabbs, uint32_t* d_out_indices,
-                                      uint32_t out_capacity, uint32_t* out_count);
+                                      uint32_t out_capacity, uint32_t* out_count, float* out_avg_half_area,
+                                      float* out_largest_half_area);
 /* CwBvh fields (cwbvh/mod.rs:43-55). */
 obvhs_status obvhs_cwbvh_info(const obvhs_cwbvh* bvh, uint32_t* node_count, uint32_t* prim_count,
                               obvhs_aabb* total_aabb);
@@ -254,6 +256,8 @@ typedef struct obvhs_server_config {
     uint64_t hbm_budget_bytes;  /* resident scenes + in-flight request buffers (0 = 200 GiB) */
     uint32_t max_batch_rays;    /* rays per coalesced launch (0 = 8 Mi) */
     uint32_t max_inflight;      /* coalesced launches in flight on the trace stream (0 = 2, at most 8) */
+    uint32_t update_streams;    /* refits / rebuilds of different scenes run concurrently (0 = 2, at most 4) */
+    uint32_t pad;
 } obvhs_server_config;
 typedef struct obvhs_request_info {
     obvhs_status status;

@@ -513,8 +513,8 @@ class Server:
 
     Requests name device arrays (DeviceArray of a client Context); they must stay alive until waited for."""
 
-    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_rays=0, max_inflight=0):
-        cfg = native.ServerConfig(int(hbm_budget_bytes), int(max_batch_rays), int(max_inflight))
+    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_rays=0, max_inflight=0, update_streams=0):
+        cfg = native.ServerConfig(int(hbm_budget_bytes), int(max_batch_rays), int(max_inflight), int(update_streams), 0)
         h = C.c_void_p()
         check(lib().obvhs_server_create(int(device), C.byref(cfg), C.byref(h)))
         self.handle = h
@@ -590,19 +590,22 @@ class Server:
                     scenes=st.scenes)
 
 
-def split_aabbs_preset(ctx, triangles, max_total=0):
-    """The pre-split leaf set of build_cwbvh_from_tris (cwbvh/builder.rs:26-55): (Aabb array, u32 indices)."""
+def split_aabbs_preset(ctx, triangles, max_total=0, with_stats=False):
+    """The pre-split leaf set of build_cwbvh_from_tris (cwbvh/builder.rs:26-55): (Aabb array, u32 indices), plus
+    the builder's average and largest half area when with_stats."""
     d, n = _as_device(ctx, triangles, TRIANGLE)
     cap = max(1, 2 * n)
     while True:
         da, di = DeviceArray(ctx, AABB, cap), DeviceArray(ctx, np.uint32, cap)
-        cnt = C.c_uint32()
-        st = lib().obvhs_split_aabbs_preset(ctx.handle, d.ptr, n, int(max_total), da.ptr, di.ptr, cap, C.byref(cnt))
+        cnt, avg, big = C.c_uint32(), C.c_float(), C.c_float()
+        st = lib().obvhs_split_aabbs_preset(ctx.handle, d.ptr, n, int(max_total), da.ptr, di.ptr, cap, C.byref(cnt),
+                                            C.byref(avg), C.byref(big))
         if st == native.STATUS_CODE["OBVHS_ERR_CAPACITY"] and cnt.value > cap:
             cap = cnt.value
             continue
         check(st, ctx.handle)
-        return da.to_host()[:cnt.value].copy(), di.to_host()[:cnt.value].copy()
+        out = (da.to_host()[:cnt.value].copy(), di.to_host()[:cnt.value].copy())
+        return out + (avg.value, big.value) if with_stats else out
 
 
 def log2_threshold_table():

@@ -333,7 +333,7 @@ obvhs_status obvhs_build_cwbvh_from_tris_ex(obvhs_context* ctx, const obvhs_tria
     obvhs_status s;
     if (params->pre_split && n > 0) {  // cwbvh/builder.rs:26-55
         uint32_t n_split = 0;
-        s = presplit_leaves(ctx, d_tris, n, ext ? ext->split_max_total : 0ull, n_split);
+        s = presplit_leaves(ctx, d_tris, n, ext ? ext->split_max_total : 0ull, n_split, nullptr, nullptr);
         if (s == OBVHS_OK)
             s = ploc_entry(ctx, nullptr, ctx->quality.split_aabbs.as<obvhs_aabb>(),
                            ctx->quality.split_indices.as<uint32_t>(), n_split, params->ploc_search_distance,
@@ -390,13 +390,17 @@ obvhs_status obvhs_bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh
 
 obvhs_status obvhs_split_aabbs_preset(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
                                       uint64_t max_total, obvhs_aabb* d_out_aabbs, uint32_t* d_out_indices,
-                                      uint32_t out_capacity, uint32_t* out_count) {
+                                      uint32_t out_capacity, uint32_t* out_count, float* out_avg_half_area,
+                                      float* out_largest_half_area) {
     if (!ctx || !out_count || (n > 0 && !d_tris)) return OBVHS_ERR_INVALID_ARG;
     CTX_TRY(hipSetDevice(ctx->device), "set device");
     uint32_t m = 0;
-    obvhs_status s = presplit_leaves(ctx, d_tris, n, max_total, m);
+    float avg = 0.0f, largest = 0.0f;
+    obvhs_status s = presplit_leaves(ctx, d_tris, n, max_total, m, &avg, &largest);
     if (s != OBVHS_OK) return s;
     *out_count = m;
+    if (out_avg_half_area) *out_avg_half_area = avg;
+    if (out_largest_half_area) *out_largest_half_area = largest;
     if (m > out_capacity) return OBVHS_ERR_CAPACITY;
     if (m == 0) return OBVHS_OK;
     if (!d_out_aabbs || !d_out_indices) return OBVHS_ERR_INVALID_ARG;

@@ -177,7 +177,7 @@ obvhs_status bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh, cons
 obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_size_ratio, const float* seq,
                               uint32_t seq_len);
 obvhs_status presplit_leaves(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n, uint64_t max_total,
-                             uint32_t& out_n);
+                             uint32_t& out_n, float* out_avg, float* out_largest);
 
 // capi.cpp
 obvhs_status set_hip_error(obvhs_context* ctx, hipError_t e, const char* where);

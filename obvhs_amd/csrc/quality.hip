@@ -55,36 +55,65 @@ __global__ void __launch_bounds__(T256) k_levels_init(uint32_t* __restrict__ met
     }
 }
 
-// Appends the children of level `level` as level + 1.  A node whose children would fall outside the node array,
-// or more than n_nodes entries in total (not a tree), sets the error word meta[err_word] instead of writing.
+// Appends the children of level `level` as level + 1.  Each block takes a contiguous chunk of the level and
+// reserves its output with one atomic (the order inside a level does not matter).  A node whose children would fall
+// outside the node array, or more than n_nodes entries in total (not a tree), sets the error word meta[err_word]
+// instead of writing.
 __global__ void __launch_bounds__(T256) k_levels_expand(const Node32* __restrict__ nodes, uint32_t n_nodes,
                                                         uint32_t* __restrict__ list, uint32_t* __restrict__ meta,
                                                         uint32_t level, uint32_t err_word) {
+    __shared__ uint32_t wcount[T256 / 64];
+    __shared__ uint32_t block_base;
     const uint32_t s = meta[2 * level], c = meta[2 * level + 1];
     const uint32_t out = s + c;
     if (blockIdx.x == 0 && threadIdx.x == 0) meta[2 * (level + 1)] = out;
-    const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t b = blockIdx.x * T256 + (threadIdx.x & ~63u); b < c; b += gridDim.x * T256) {
-        const uint32_t i = b + lane;
-        uint32_t first = 0u;
+    const uint32_t per_block = ((c + gridDim.x - 1) / gridDim.x + T256 - 1) / T256 * T256;
+    const uint32_t b0 = blockIdx.x * per_block;
+    if (b0 >= c) return;
+    const uint32_t b1 = min(c, b0 + per_block);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    // pass 1: count the block's internal nodes
+    uint32_t mine = 0;
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += T256) {
+        const Node32 nd = load_node(&nodes[list[s + i]]);
+        if (nd.prim_count == 0u) {
+            if (nd.first_index == 0u || nd.first_index >= n_nodes - 1u) atomicOr(&meta[err_word], 1u);
+            else mine++;
+        }
+    }
+    uint32_t wsum = mine;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) wsum += (uint32_t)__shfl_xor((int)wsum, off);
+    if (lane == 0) wcount[wave] = wsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < T256 / 64; ++w) tot += wcount[w];
+        block_base = atomicAdd(&meta[2 * (level + 1) + 1], 2u * tot);
+    }
+    __syncthreads();
+    // pass 2: write the children at block_base + 2 * (rank among the block's internal nodes)
+    uint32_t base = block_base;
+    for (uint32_t i0 = b0; i0 < b1; i0 += T256) {
+        const uint32_t i = i0 + threadIdx.x;
         bool inner = false;
-        if (i < c) {
+        uint32_t first = 0u;
+        if (i < b1) {
             const Node32 nd = load_node(&nodes[list[s + i]]);
-            inner = nd.prim_count == 0u;
             first = nd.first_index;
-            if (inner && (first == 0u || first >= n_nodes - 1u)) {
-                atomicOr(&meta[err_word], 1u);
-                inner = false;
-            }
+            inner = nd.prim_count == 0u && first != 0u && first < n_nodes - 1u;
         }
         const uint64_t m = __ballot(inner);
-        if (m == 0ull) continue;
-        uint32_t base = 0u;
-        if (lane == 0) base = atomicAdd(&meta[2 * (level + 1) + 1], 2u * (uint32_t)__popcll(m));
-        base = (uint32_t)__shfl((int)base, 0);
+        __syncthreads();
+        if (lane == 0) wcount[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int w = 0; w < T256 / 64; ++w) {
+            if (w < (int)wave) before += wcount[w];
+            total += wcount[w];
+        }
         if (inner) {
-            const uint32_t r = (uint32_t)__popcll(m & lanemask_lt());
-            const uint64_t at = (uint64_t)out + base + 2 * r;
+            const uint64_t at = (uint64_t)out + base + 2u * (before + (uint32_t)__popcll(m & lanemask_lt()));
             if (at + 1 < n_nodes) {
                 list[at] = first;
                 list[at + 1] = first + 1u;
@@ -92,6 +121,7 @@ __global__ void __launch_bounds__(T256) k_levels_expand(const Node32* __restrict
                 atomicOr(&meta[err_word], 1u);
             }
         }
+        base += 2u * total;
     }
 }
 
@@ -443,21 +473,116 @@ __global__ void __launch_bounds__(T256) k_split_prepare(const float4* __restrict
     if (__lane_id() == 0) atomicMax(&misc->largest_key, k);
 }
 
-// The crate's running sum `avg_area += half_area` in triangle order (cwbvh/builder.rs:34-41) by one wave: lanes load
-// 64 values at a time, the sum walks them in order; then the split thresholds of split_aabbs_preset.
-__global__ void __launch_bounds__(64) k_split_avg(const float* __restrict__ ha, uint32_t n, SplitMisc* misc) {
-    const uint32_t lane = threadIdx.x;
-    float sum = 0.0f;
-    float next = lane < n ? ha[lane] : 0.0f;
-    for (uint32_t b = 0; b < n; b += 64) {
-        const float cur = next;
-        if (b + 64 + lane < n) next = ha[b + 64 + lane];
-        const uint32_t m = min(64u, n - b);
-        const int ci = __float_as_int(cur);
-        for (uint32_t k = 0; k < m; ++k) sum += __int_as_float(__builtin_amdgcn_readlane(ci, (int)k));
+// The crate's running sum `avg_area += half_area` in triangle order (cwbvh/builder.rs:34-41), reproduced exactly by
+// one workgroup without a sequential walk.  While the running sum s stays in one binade (ulp u), the f32 step
+// fl(s + x) equals s + u * round(x / u) for every x >= 0 -- unless x / u has fraction exactly 1/2 (a tie, decided by
+// the parity of s / u) or s + x leaves the binade.  So each tile's increments round(x / u) are prefix-summed as
+// integers; the first tie, binade crossing or non-finite value is then added with one IEEE f32 add (the sequential
+// step itself, from the exact s before it) and the scan resumes after it in the new binade.  The split thresholds
+// of split_aabbs_preset follow.
+constexpr int SUM_THREADS = 1024;
+constexpr int SUM_ITEMS = 4;
+constexpr uint32_t SUM_TILE = SUM_THREADS * SUM_ITEMS;
+
+__device__ __forceinline__ unsigned long long wave_inclusive_scan_u64(unsigned long long v) {
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long t = __shfl_up(v, off);
+        if (lane >= off) v += t;
     }
-    if (lane == 0) {
-        const float avg = sum / (float)n;
+    return v;
+}
+
+__global__ void __launch_bounds__(SUM_THREADS) k_split_avg(const float* __restrict__ ha, uint32_t n,
+                                                          SplitMisc* misc) {
+    __shared__ unsigned long long wtot[SUM_THREADS / 64];
+    __shared__ uint32_t first_stop;
+    __shared__ float s_next;
+    __shared__ uint32_t pos_next;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    float s = 0.0f;  // the running sum, identical in every thread
+    uint32_t pos = 0;
+    while (pos < n && !isnan(s)) {
+        float x[SUM_ITEMS];
+        unsigned long long a[SUM_ITEMS];
+        bool stop[SUM_ITEMS];
+        const bool zero = s == 0.0f;
+        const bool finite = isfinite(s);
+        const uint32_t bits = __float_as_uint(s);
+        const uint32_t be_raw = (bits >> 23) & 0xffu;
+        const uint32_t be = be_raw == 0u ? 1u : be_raw;  // subnormals share the first normal binade's ulp
+        const unsigned long long m0 = zero ? 0ull : (unsigned long long)((bits & 0x7fffffu) | (be_raw ? 0x800000u : 0u));
+        const double inv_u = ldexp(1.0, 150 - (int)be);
+        unsigned long long tsum = 0ull;
+#pragma unroll
+        for (int r = 0; r < SUM_ITEMS; ++r) {
+            const uint32_t k = pos + tid * SUM_ITEMS + (uint32_t)r;
+            x[r] = k < n ? ha[k] : 0.0f;
+            a[r] = 0ull;
+            stop[r] = false;
+            if (k >= n) continue;
+            if (zero) {
+                stop[r] = x[r] != 0.0f;  // 0 + x = x: the first non-zero value starts the sum
+            } else if (!finite || !isfinite(x[r])) {
+                stop[r] = true;
+            } else {
+                const double t = (double)x[r] * inv_u;  // exact: a power-of-two scale of an f32
+                const double fl = floor(t);
+                const double fr = t - fl;
+                if (fr == 0.5) {
+                    stop[r] = true;  // tie: rounding depends on the parity of s / u
+                } else {
+                    const double q = fl + (fr > 0.5 ? 1.0 : 0.0);
+                    a[r] = q >= 1099511627776.0 ? 1099511627776ull : (unsigned long long)q;  // >= 2^40 crosses anyway
+                }
+            }
+            tsum += a[r];
+        }
+        // block-wide exclusive prefix of the thread totals (element k = tid * SUM_ITEMS + r, in order)
+        const unsigned long long incl = wave_inclusive_scan_u64(tsum);
+        if (tid == 0) first_stop = SUM_TILE;
+        if (lane == 63u) wtot[wave] = incl;
+        __syncthreads();
+        unsigned long long excl = incl - tsum;
+        for (uint32_t w = 0; w < wave; ++w) excl += wtot[w];
+        unsigned long long run = excl;
+        uint32_t my_stop = SUM_TILE;
+#pragma unroll
+        for (int r = 0; r < SUM_ITEMS; ++r) {
+            run += a[r];
+            const bool cross = !zero && m0 + run >= (1ull << 24);  // the sum reaches the next binade
+            if ((stop[r] || cross) && my_stop == SUM_TILE) my_stop = tid * SUM_ITEMS + (uint32_t)r;
+        }
+        if (my_stop != SUM_TILE) atomicMin(&first_stop, my_stop);
+        __syncthreads();
+        const uint32_t idx = first_stop;
+        const uint32_t tile_len = min(SUM_TILE, n - pos);
+        if (idx == SUM_TILE) {
+            if (tid == SUM_THREADS - 1) {  // owns the tile's total
+                s_next = zero ? 0.0f : ldexpf((float)(m0 + run), (int)be - 150);
+                pos_next = pos + tile_len;
+            }
+        } else if (idx / SUM_ITEMS == tid) {
+            const int r0 = (int)(idx % SUM_ITEMS);
+            unsigned long long before = excl;
+            float xi = x[0];
+#pragma unroll
+            for (int r = 0; r < SUM_ITEMS; ++r) {
+                if (r < r0) before += a[r];
+                if (r == r0) xi = x[r];
+            }
+            const float sb = zero ? 0.0f : ldexpf((float)(m0 + before), (int)be - 150);  // exact s before element idx
+            s_next = sb + xi;                                                            // the sequential step
+            pos_next = pos + idx + 1u;
+        }
+        __syncthreads();
+        s = s_next;
+        pos = pos_next;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const float avg = s / (float)n;
         const float largest = f32_from_order_key(misc->largest_key);
         misc->avg = avg;
         misc->thresh_low = avg * 3.0f;                                       // splits.rs:27
@@ -851,7 +976,7 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
 // Pre-split leaf set (cwbvh/builder.rs:26-55).  max_total = 0: no limit (the crate's behaviour).  On return
 // Q.split_aabbs / Q.split_indices hold `out_n` leaves.
 obvhs_status presplit_leaves(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n, uint64_t max_total,
-                             uint32_t& out_n) {
+                             uint32_t& out_n, float* out_avg, float* out_largest) {
     hipStream_t st = ctx->stream;
     QualityScratch& Q = ctx->quality;
     out_n = n;
@@ -871,7 +996,7 @@ obvhs_status presplit_leaves(obvhs_context* ctx, const obvhs_triangle* d_tris, u
     k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(misc), 0u, sizeof(SplitMisc) / 4);
     k_split_prepare<<<nblk, T256, 0, st>>>(tris, n, Q.split_aabbs.as<float4>(), Q.split_indices.as<uint32_t>(),
                                            Q.ha.as<float>(), misc);
-    k_split_avg<<<1, 64, 0, st>>>(Q.ha.as<float>(), n, misc);
+    k_split_avg<<<1, SUM_THREADS, 0, st>>>(Q.ha.as<float>(), n, misc);
     // candidate lists hold distinct leaf indices (at most cap); the extended list at most twice that
     OBVHS_TRY(Q.cand_a.ensure(cap * 4), "alloc candidates");
     OBVHS_TRY(Q.cand_b.ensure(cap * 8), "alloc extended candidates");
@@ -880,9 +1005,13 @@ obvhs_status presplit_leaves(obvhs_context* ctx, const obvhs_triangle* d_tris, u
     k_split_cand_write<<<nblk, T256, 0, st>>>(Q.ha.as<float>(), n, misc, Q.sums.as<uint32_t>(),
                                               Q.cand_a.as<uint32_t>());
     OBVHS_TRY(hipGetLastError(), "split prepare launch");
-    OBVHS_TRY(hipMemcpyAsync(h, &misc->kept, 4, hipMemcpyDeviceToHost, st), "read candidates");
+    OBVHS_TRY(hipMemcpyAsync(h, misc, sizeof(SplitMisc), hipMemcpyDeviceToHost, st), "read candidates");
     OBVHS_TRY(hipStreamSynchronize(st), "split prepare sync");
-    uint32_t count = h[0];
+    SplitMisc hm;
+    memcpy(&hm, h, sizeof(SplitMisc));
+    if (out_avg) *out_avg = hm.avg;
+    if (out_largest) *out_largest = f32_from_order_key(hm.largest_key);
+    uint32_t count = hm.kept;
     uint64_t len = n;
     for (uint32_t it = 0; it < 12 && count > 0; ++it) {  // max_iterations = 12 (splits.rs:31)
         // every candidate may append one leaf: grow the leaf arrays (keeping their contents) when needed

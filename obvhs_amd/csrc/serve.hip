@@ -12,7 +12,7 @@
 //     budget allows, and launches one traversal per mode over every ready request at the head of its scene's FIFO
 //     (each request a segment of the launch: its own BVH, rays and outputs) while fewer than max_inflight launches
 //     are on the trace stream;
-//   * the update thread runs refits and rebuilds on the update context's stream.  An update writes the scene's
+//   * update threads (one context and stream each) run refits and rebuilds, several scenes at a time.  An update writes the scene's
 //     other version slot, after the trace stream's last launch that read that slot (event wait), and becomes the
 //     scene's current version when it completes; traces submitted after it wait for it, traces submitted before it
 //     keep reading the old slot.
@@ -110,11 +110,12 @@ struct obvhs_server {
     int device = 0;
     obvhs_server_config cfg{};
     obvhs_context* tctx = nullptr;  // trace stream
-    obvhs_context* uctx = nullptr;  // update stream
+    std::vector<obvhs_context*> uctx;  // one context (stream + build scratch) per updater thread, then one for
+                                       // add_scene
     Clock::time_point t0 = Clock::now();
 
     std::mutex mu;
-    std::mutex update_mu;  // serialises use of the update context (updater thread, add_scene)
+    std::mutex setup_mu;  // serialises add_scene's use of its context
     std::condition_variable cv_work, cv_done, cv_update;
     std::deque<Req*> incoming;
     std::unordered_map<uint64_t, Req*> reqs;
@@ -140,7 +141,8 @@ struct obvhs_server {
     // statistics
     uint64_t stat_launches = 0, stat_rays = 0, stat_segments = 0;
 
-    std::thread scheduler, updater;
+    std::thread scheduler;
+    std::vector<std::thread> updaters;
 };
 
 namespace obvhs {
@@ -156,8 +158,7 @@ void complete(obvhs_server* S, Req* r, obvhs_status st) {
 
 // Runs one update on the update context (blocking).  Returns the status; on success the scene's other slot holds the
 // new version.
-obvhs_status run_update(obvhs_server* S, Scene* sc, Req* r, int slot) {
-    obvhs_context* ctx = S->uctx;
+obvhs_status run_update(obvhs_server* S, obvhs_context* ctx, Scene* sc, Req* r, int slot) {
     hipStream_t st = ctx->stream;
     Version& dst = sc->v[slot];
     Version& src = sc->v[sc->cur];
@@ -220,7 +221,7 @@ obvhs_status run_update(obvhs_server* S, Scene* sc, Req* r, int slot) {
     return OBVHS_OK;
 }
 
-void updater_main(obvhs_server* S) {
+void updater_main(obvhs_server* S, obvhs_context* ctx) {
     (void)hipSetDevice(S->device);
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
@@ -233,11 +234,7 @@ void updater_main(obvhs_server* S) {
         const uint64_t old_bytes = sc->v[slot].bytes;
         lk.unlock();
         r->t_start = now_ns(S->t0);
-        obvhs_status st;
-        {
-            std::lock_guard<std::mutex> ul(S->update_mu);
-            st = run_update(S, sc, r, slot);
-        }
+        const obvhs_status st = run_update(S, ctx, sc, r, slot);
         lk.lock();
         S->resident = S->resident - std::min(S->resident, old_bytes) + sc->v[slot].bytes;
         if (st == OBVHS_OK) {
@@ -421,7 +418,12 @@ obvhs_status obvhs_server_create(int device, const obvhs_server_config* cfg, obv
     S->cfg.max_batch_rays = cfg && cfg->max_batch_rays ? cfg->max_batch_rays : (1u << 23);
     S->cfg.max_inflight = cfg && cfg->max_inflight ? std::min(cfg->max_inflight, 8u) : 2u;
     obvhs_status s = obvhs_context_create(device, nullptr, &S->tctx);
-    if (s == OBVHS_OK) s = obvhs_context_create(device, nullptr, &S->uctx);
+    S->cfg.update_streams = cfg && cfg->update_streams ? std::min(cfg->update_streams, 4u) : 2u;
+    for (uint32_t u = 0; s == OBVHS_OK && u <= S->cfg.update_streams; ++u) {
+        obvhs_context* c = nullptr;
+        s = obvhs_context_create(device, nullptr, &c);
+        if (s == OBVHS_OK) S->uctx.push_back(c);
+    }
     const size_t nseg = (size_t)(2 * S->cfg.max_inflight + 2) * MAX_SEGMENTS;
     if (s == OBVHS_OK && S->seg_dev.ensure(nseg * sizeof(TraceSegment)) != hipSuccess) s = OBVHS_ERR_OOM;
     if (s == OBVHS_OK) s = traverse_scratch(S->tctx, S->trav);
@@ -434,7 +436,7 @@ obvhs_status obvhs_server_create(int device, const obvhs_server_config* cfg, obv
         return s;
     }
     S->scheduler = std::thread(scheduler_main, S);
-    S->updater = std::thread(updater_main, S);
+    for (uint32_t u = 0; u < S->cfg.update_streams; ++u) S->updaters.emplace_back(updater_main, S, S->uctx[u]);
     *out = S;
     return OBVHS_OK;
 }
@@ -448,10 +450,11 @@ obvhs_status obvhs_server_destroy(obvhs_server* S) {
     S->cv_work.notify_all();
     S->cv_update.notify_all();
     if (S->scheduler.joinable()) S->scheduler.join();
-    if (S->updater.joinable()) S->updater.join();
+    for (std::thread& t : S->updaters)
+        if (t.joinable()) t.join();
     (void)hipSetDevice(S->device);
     if (S->tctx) (void)hipStreamSynchronize(S->tctx->stream);
-    if (S->uctx) (void)hipStreamSynchronize(S->uctx->stream);
+    for (obvhs_context* c : S->uctx) (void)hipStreamSynchronize(c->stream);
     for (Batch* b : S->inflight) {
         if (b->done) (void)hipEventDestroy(b->done);
         delete b;
@@ -473,7 +476,7 @@ obvhs_status obvhs_server_destroy(obvhs_server* S) {
     S->seg_dev.release();
     if (S->seg_host) (void)hipHostFree(S->seg_host);
     obvhs_context_destroy(S->tctx);
-    obvhs_context_destroy(S->uctx);
+    for (obvhs_context* c : S->uctx) obvhs_context_destroy(c);
     delete S;
     return OBVHS_OK;
 }
@@ -494,9 +497,9 @@ obvhs_status obvhs_server_add_scene(obvhs_server* S, const obvhs_triangle* d_tri
     // the initial version is built like a rebuild into slot 0 on the update context
     obvhs_status s;
     {
-        std::lock_guard<std::mutex> ul(S->update_mu);
+        std::lock_guard<std::mutex> ul(S->setup_mu);
         sc->cur = 1;  // run_update writes the other slot
-        s = run_update(S, sc, &r, 0);
+        s = run_update(S, S->uctx.back(), sc, &r, 0);
         sc->cur = 0;
     }
     std::lock_guard<std::mutex> lk(S->mu);

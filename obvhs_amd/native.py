@@ -43,7 +43,8 @@ class BuildExt(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_rays", C.c_uint32), ("max_inflight", C.c_uint32)]
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_rays", C.c_uint32), ("max_inflight", C.c_uint32),
+                ("update_streams", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class RequestInfo(C.Structure):
@@ -123,7 +124,8 @@ SIGNATURES = {
     "obvhs_bvh2_reinsertion": (INT, [P, P, C.c_float, P, U32]),
     "obvhs_bvh2_refit_all": (INT, [P, P]),
     "obvhs_bvh2_refit_from_triangles": (INT, [P, P, P]),
-    "obvhs_split_aabbs_preset": (INT, [P, P, U32, U64, P, P, U32, PU32]),
+    "obvhs_split_aabbs_preset": (INT, [P, P, U32, U64, P, P, U32, PU32, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float)]),
     "obvhs_server_create": (INT, [INT, C.POINTER(ServerConfig), C.POINTER(P)]),
     "obvhs_server_destroy": (INT, [P]),
     "obvhs_server_add_scene": (INT, [P, P, U32, C.POINTER(BuildParams), PU32]),

@@ -172,3 +172,37 @@ def test_large_pipeline_scaled(ctx):
     rm = O.traverse(rcw, tris[rprims], shadow, mode=1, threads=8)
     np.testing.assert_array_equal(m, rm)
     assert 0.05 < (~m).mean() < 0.95 and np.isfinite(h["t"]).mean() > 0.05
+
+
+def _area_triangles(values):
+    """Triangles (0,0,0), (v,0,0), (0,1,0): their boxes have half area exactly v."""
+    v = np.asarray(values, dtype=np.float32)
+    z = np.zeros_like(v)
+    return ob.triangles_from_vertices(np.stack([z, z, z], 1), np.stack([v, z, z], 1),
+                                      np.stack([z, np.ones_like(v), z], 1))
+
+
+def _sum_cases():
+    rng = np.random.default_rng(3)
+    half_ulp_of_one = np.float32(2.0 ** -24)
+    return {
+        "ties_at_one": np.concatenate([[1.0], np.full(20000, half_ulp_of_one)]),
+        "ties_then_carry": np.concatenate([[1.0], np.full(5000, half_ulp_of_one), np.full(5000, 3 * half_ulp_of_one)]),
+        "zeros_first": np.concatenate([np.zeros(7000), rng.uniform(0, 1, 9000)]),
+        "log_uniform": np.exp(rng.uniform(np.log(1e-30), np.log(1e3), 200000)),
+        "subnormal": np.concatenate([np.full(5000, np.float32(1e-45)), np.full(3, np.float32(1e-38))]),
+        "powers_of_two": 2.0 ** rng.integers(-40, 20, 50000),
+        "one_value": [0.25],
+    }
+
+
+@pytest.mark.parametrize("case", list(_sum_cases()))
+def test_split_average_is_the_sequential_f32_sum(ctx, case):
+    """cwbvh/builder.rs:34-41 sums half areas one f32 add at a time; the GPU reproduces every rounding (ties to
+    even, binade changes, subnormals) with a parallel integer scan."""
+    print(f"[case] ordered sum {case}", flush=True)
+    tris = _area_triangles(_sum_cases()[case])
+    _, _, avg, big = ob.split_aabbs_preset(ctx, tris, 0, with_stats=True)
+    _, _, ravg, rbig = O.presplit(tris, 0)
+    assert np.float32(avg).tobytes() == np.float32(ravg).tobytes(), (avg, ravg)
+    assert np.float32(big).tobytes() == np.float32(rbig).tobytes()
