@@ -443,14 +443,14 @@ def main():
     log(f"{len(prim_rays)} primary + {len(bounce)} bounce rays")
 
     if args.pmc_probe:
-        # Counter runs (rocprofv3 --pmc): one copy kernel of known size (the counters' scale), then one primary and
-        # one bounce traversal launch.
-        import torch
-        x = torch.ones(PMC_COPY_BYTES // 4, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        y.copy_(x)
-        torch.cuda.synchronize()
+        # Counter runs (rocprofv3 --pmc): one device-to-device copy of known size (a blit kernel: the counters'
+        # scale), then one primary and one bounce traversal launch.
+        src = ob.DeviceArray(ctx, np.uint8, PMC_COPY_BYTES)
+        dst = ob.DeviceArray(ctx, np.uint8, PMC_COPY_BYTES)
+        ob.native.check(lib.obvhs_memset_device(ctx.handle, src.ptr, 1, PMC_COPY_BYTES), ctx.handle)
+        ctx.synchronize()
+        ob.native.check(lib.obvhs_memcpy_dtod(ctx.handle, dst.ptr, src.ptr, PMC_COPY_BYTES), ctx.handle)
+        ctx.synchronize()
         trace(d_prim, len(prim_rays), hits)
         trace(d_bounce, len(bounce), bhits)
         ctx.synchronize()

@@ -101,6 +101,7 @@ obvhs_status obvhs_device_free(obvhs_context* ctx, void* dptr);
 obvhs_status obvhs_memcpy_htod(obvhs_context* ctx, void* dst, const void* src, uint64_t bytes); /* sync */
 obvhs_status obvhs_memcpy_dtoh(obvhs_context* ctx, void* dst, const void* src, uint64_t bytes); /* sync */
 obvhs_status obvhs_memset_device(obvhs_context* ctx, void* dst, int value, uint64_t bytes);      /* async */
+obvhs_status obvhs_memcpy_dtod(obvhs_context* ctx, void* dst, const void* src, uint64_t bytes);  /* async */
 /* Keep each build's Morton codes and sort permutation for obvhs_bvh2_copy_sort_trace (parity tests). */
 obvhs_status obvhs_context_set_trace(obvhs_context* ctx, int enable);
 /* Stream timer: records a start/stop event on the context stream; obvhs_timer_stop synchronises. */
@@ -236,6 +237,18 @@ obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch
                                            const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
                                            const uint32_t* d_ray_mesh, uint32_t n, uint8_t* d_miss);
 obvhs_status obvhs_batch_destroy(obvhs_batch* b);
+/* ---- TLAS over the meshes of a batch ----
+ * A CwBvh whose primitives are the meshes' bounds (each mesh's BVH2 root box; every mesh must be non-empty), built
+ * like obvhs_build_cwbvh_from_tris over boxes with the given parameters (pre-splits and reinsertion ignored). */
+obvhs_status obvhs_batch_build_tlas(obvhs_context* ctx, const obvhs_batch* b, const obvhs_build_params* params,
+                                    obvhs_cwbvh** out);
+/* CwBvh::ray_traverse_tlas_blas (cwbvh/mod.rs:341-491) with the batch's meshes as BLASes and Triangle::intersect
+ * on d_bvh_tris (obvhs_batch_reorder_triangles): RayHit::primitive_id is the primitive slot inside the BLAS,
+ * RayHit::geometry_id the BLAS (mesh) index.  Follows the crate's function as written, including that when several
+ * TLAS leaves of one node are hit only the first one (highest bit) is entered. */
+obvhs_status obvhs_batch_tlas_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh* tlas, const obvhs_batch* b,
+                                           const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays, uint32_t n,
+                                           obvhs_ray_hit* d_hits);
 
 /* ---- serving (resident BVHs, continuous batching) ----
  * No single reference counterpart: a server keeps scenes (CWBVH + triangles) resident in HBM, coalesces ray

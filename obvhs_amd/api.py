@@ -471,6 +471,22 @@ class CwBvhBatch:
                                                   miss.ptr), self.ctx.handle)
         return miss.to_host().astype(bool)
 
+    def build_tlas(self, config):
+        """A CwBvh over the meshes' bounds (the TLAS of ray_traverse_tlas_blas, cwbvh/mod.rs:341-491)."""
+        out = C.c_void_p()
+        p = config.to_c()
+        check(lib().obvhs_batch_build_tlas(self.ctx.handle, self.handle, C.byref(p), C.byref(out)), self.ctx.handle)
+        return CwBvh(self.ctx, out)
+
+    def tlas_ray_traverse(self, tlas, rays, bvh_tris):
+        """CwBvh::ray_traverse_tlas_blas with this batch's meshes as BLASes: RayHit[n] with geometry_id = mesh."""
+        dr, n = _as_device(self.ctx, rays, RAY)
+        dt, _ = _as_device(self.ctx, bvh_tris, TRIANGLE)
+        hits = DeviceArray(self.ctx, RAY_HIT, n)
+        check(lib().obvhs_batch_tlas_ray_traverse(self.ctx.handle, tlas.handle, self.handle, dt.ptr, dr.ptr, n,
+                                                  hits.ptr), self.ctx.handle)
+        return hits.to_host()
+
     def close(self):
         if self.handle:
             lib().obvhs_batch_destroy(self.handle)

@@ -450,6 +450,17 @@ __global__ void __launch_bounds__(T256) k_b_iota_local(uint32_t* __restrict__ pr
     prims[g] = g - offsets[mesh_of(offsets, m, g)];
 }
 
+// Root box of every mesh's BVH2 (the BLAS bounds a TLAS is built over), as Aabb records.
+__global__ void __launch_bounds__(T256) k_b_root_boxes(const Node32* __restrict__ nodes,
+                                                       const MeshDesc* __restrict__ meshes, uint32_t m,
+                                                       float4* __restrict__ out) {
+    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
+    if (k >= m) return;
+    const Node32 r = load_node(&nodes[meshes[k].node_base]);
+    out[2 * (size_t)k] = make_float4(r.minx, r.miny, r.minz, 0.0f);
+    out[2 * (size_t)k + 1] = make_float4(r.maxx, r.maxy, r.maxz, 0.0f);
+}
+
 }  // namespace
 
 #define OBVHS_TRY(expr, where)                                      \
@@ -686,6 +697,68 @@ obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch
                                           (uint32_t)b->meshes.size(), d_ray_mesh,
                                           d_bvh_tris, d_rays, n, 1, nullptr, d_miss, ts, ctx->stream);
     if (e != hipSuccess) return set_hip_error(ctx, e, "batch traverse miss");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_batch_build_tlas(obvhs_context* ctx, const obvhs_batch* b, const obvhs_build_params* params,
+                                    obvhs_cwbvh** out) {
+    if (!ctx || !b || !params || !out) return OBVHS_ERR_INVALID_ARG;
+    *out = nullptr;
+    const uint32_t m = (uint32_t)b->meshes.size();
+    for (const MeshDesc& md : b->meshes)
+        if (md.prim_count == 0u) return OBVHS_ERR_INVALID_ARG;  // an empty BLAS has no bounds
+    if (hipSetDevice(ctx->device) != hipSuccess) return OBVHS_ERR_DEVICE;
+    DevBuf boxes;
+    hipError_t e = boxes.ensure((size_t)std::max(m, 1u) * 32);
+    if (e != hipSuccess) return set_hip_error(ctx, e, "alloc tlas boxes");
+    if (m) {
+        k_b_root_boxes<<<(m + T256 - 1) / T256, T256, 0, ctx->stream>>>(b->bvh2_nodes.as<Node32>(),
+                                                                       b->mesh_table.as<MeshDesc>(), m,
+                                                                       boxes.as<float4>());
+        e = hipGetLastError();
+        if (e != hipSuccess) {
+            boxes.release();
+            return set_hip_error(ctx, e, "tlas boxes");
+        }
+    }
+    obvhs_cwbvh* cw = new (std::nothrow) obvhs_cwbvh();
+    if (!cw) {
+        boxes.release();
+        return OBVHS_ERR_OOM;
+    }
+    uint32_t mp = params->max_prims_per_leaf;
+    mp = mp < 1u ? 1u : (mp > 3u ? 3u : mp);
+    obvhs_bvh2* b2 = nullptr;
+    obvhs_status s = obvhs_ploc_build_aabbs(ctx, boxes.as<obvhs_aabb>(), nullptr, m, params->ploc_search_distance,
+                                            params->sort_precision, params->search_depth_threshold, nullptr, &b2);
+    if (s == OBVHS_OK) {
+        cw->owned_bvh2 = b2;
+        s = cwbvh_convert(ctx, b2, mp, true, cw);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    boxes.release();
+    if (s != OBVHS_OK) {
+        obvhs_cwbvh_destroy(cw);
+        return s;
+    }
+    *out = cw;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_batch_tlas_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh* tlas, const obvhs_batch* b,
+                                           const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays, uint32_t n,
+                                           obvhs_ray_hit* d_hits) {
+    if (!ctx || !tlas || !b || (n && (!d_rays || !d_hits))) return OBVHS_ERR_INVALID_ARG;
+    if (b->n_total && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
+    TravScratch ts;
+    if (n) {
+        const obvhs_status ss = traverse_scratch(ctx, ts);
+        if (ss != OBVHS_OK) return ss;
+    }
+    hipError_t e = launch_traverse_tlas(tlas->nodes.as<CwNode>(), tlas->node_count, tlas->prim_indices.as<uint32_t>(),
+                                        tlas->prim_count, b->cw_nodes.as<CwNode>(), b->mesh_table.as<MeshDesc>(),
+                                        (uint32_t)b->meshes.size(), d_bvh_tris, d_rays, n, d_hits, ts, ctx->stream);
+    if (e != hipSuccess) return set_hip_error(ctx, e, "tlas traverse");
     return OBVHS_OK;
 }
 

@@ -168,6 +168,13 @@ obvhs_status obvhs_memset_device(obvhs_context* ctx, void* dst, int value, uint6
     return OBVHS_OK;
 }
 
+obvhs_status obvhs_memcpy_dtod(obvhs_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return OBVHS_ERR_INVALID_ARG;
+    if (!bytes) return OBVHS_OK;
+    CTX_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream), "memcpy dtod");
+    return OBVHS_OK;
+}
+
 obvhs_status obvhs_timer_start(obvhs_context* ctx) {
     if (!ctx) return OBVHS_ERR_INVALID_ARG;
     CTX_TRY(hipEventRecord(ctx->ev_start, ctx->stream), "timer start");

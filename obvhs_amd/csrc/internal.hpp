@@ -154,6 +154,12 @@ hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, u
                                   uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss, const TravScratch& ts,
                                   hipStream_t stream);
 
+// CwBvh::ray_traverse_tlas_blas with the meshes of a batched build as BLASes.
+hipError_t launch_traverse_tlas(const CwNode* tlas, uint32_t tlas_count, const uint32_t* tlas_prims,
+                                uint32_t tlas_prim_count, const CwNode* blas_nodes, const MeshDesc* meshes,
+                                uint32_t mesh_count, const obvhs_triangle* tris, const obvhs_ray* rays, uint32_t n,
+                                obvhs_ray_hit* hits, const TravScratch& ts, hipStream_t stream);
+
 // One request's share of a coalesced traversal launch: rays [begin, begin + count) of the launch are rays[0..count)
 // of this request, traced against one CWBVH, results to out (RayHit[] in closest mode, u8[] in miss mode).
 struct TraceSegment {

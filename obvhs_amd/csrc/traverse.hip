@@ -30,6 +30,7 @@ namespace {
 constexpr int TRAV_THREADS = 64;
 constexpr int STACK_SIZE = 32;  // TRAVERSAL_STACK_SIZE, cwbvh/mod.rs:60
 constexpr int LDS_STACK = TRAV_LDS_STACK;  // entries kept in LDS; STACK_SIZE - LDS_STACK spill to HBM
+constexpr uint32_t XCDS = 8;               // accelerator complex dies of an MI355X, each with its own L2
 constexpr float NODE_EPSILON = 0.0001f;  // cwbvh/node.rs:82
 
 struct RayData {
@@ -137,7 +138,11 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     const uint32_t lanes = gridDim.x * TRAV_THREADS;
     const uint32_t gid = blockIdx.x * TRAV_THREADS + lane;
     uint2* my_spill = spill + gid;  // entry k >= LDS_STACK at my_spill[(k - LDS_STACK) * lanes]
-    for (uint32_t i = gid; i < n; i += lanes) {
+    // Workgroups are dispatched to the XCDs round-robin; each XCD takes a contiguous eighth of the rays so that
+    // neighbouring (coherent) rays share that XCD's L2.  The grid is a multiple of XCDS blocks.
+    const uint32_t xcd = blockIdx.x % XCDS, xblock = blockIdx.x / XCDS, xblocks = gridDim.x / XCDS;
+    const uint32_t r_begin = (uint32_t)(((uint64_t)n * xcd) / XCDS), r_end = (uint32_t)(((uint64_t)n * (xcd + 1)) / XCDS);
+    for (uint32_t i = r_begin + xblock * TRAV_THREADS + lane; i < r_end; i += xblocks * TRAV_THREADS) {
     const uint4* nodes = nodes_in;
     const float4* tris = tris_in;
     const float4* rays = rays_in;
@@ -261,13 +266,133 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     }  // rays of this lane
 }
 
+// CwBvh::ray_traverse_tlas_blas (cwbvh/mod.rs:341-491): one stack for both levels, TLAS primitives name BLASes
+// (meshes of a batched build) through the TLAS's primitive_indices, the closure is Triangle::intersect on the
+// mesh's triangles in its primitive order.  Restated as written, including that a TLAS primitive group popped after
+// a BLAS finishes is treated as a node group and dropped (mod.rs:396-401, 469).  Hits report the BLAS-local
+// primitive slot and the BLAS index (RayHit::geometry_id).
+__global__ void __launch_bounds__(TRAV_THREADS)
+    k_traverse_tlas(const uint4* __restrict__ tlas, uint32_t tlas_count, const uint32_t* __restrict__ tlas_prims,
+                    uint32_t tlas_prim_count, const uint4* __restrict__ blas_nodes, const MeshDesc* __restrict__ meshes,
+                    uint32_t mesh_count, const float4* __restrict__ tris, const float4* __restrict__ rays, uint32_t n,
+                    obvhs_ray_hit* __restrict__ hits, uint2* __restrict__ spill) {
+    __shared__ uint2 stack[LDS_STACK][TRAV_THREADS];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t lanes = gridDim.x * TRAV_THREADS;
+    const uint32_t gid = blockIdx.x * TRAV_THREADS + lane;
+    uint2* my_spill = spill + gid;
+    const uint32_t xcd = blockIdx.x % XCDS, xblock = blockIdx.x / XCDS, xblocks = gridDim.x / XCDS;
+    const uint32_t r_begin = (uint32_t)(((uint64_t)n * xcd) / XCDS), r_end = (uint32_t)(((uint64_t)n * (xcd + 1)) / XCDS);
+    for (uint32_t i = r_begin + xblock * TRAV_THREADS + lane; i < r_end; i += xblocks * TRAV_THREADS) {
+        const float4 ra = rays[4 * (size_t)i], rb = rays[4 * (size_t)i + 1], rc = rays[4 * (size_t)i + 2],
+                     rd = rays[4 * (size_t)i + 3];
+        RayData r{ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, rc.x, rc.y, rc.z, rd.x, rd.y};
+        const uint32_t oct = octant_inv4(r.dx, r.dy, r.dz);
+        uint32_t hit_id = INVALID32, hit_geom = INVALID32;
+        int sp = 0;  // StackStack::index
+        uint32_t tlas_stack_size = INVALID32, mesh = INVALID32;
+        const uint4* nodes = tlas;
+        uint32_t node_count = tlas_count;
+        const float4* mtris = tris;
+        uint32_t mprims = 0;
+        uint2 cur = tlas_count == 0u ? make_uint2(0u, 0u) : make_uint2(0u, 0x80000000u);
+        for (;;) {
+            uint2 prim = make_uint2(0u, 0u);
+            if ((cur.y & 0xff000000u) != 0u) {
+                const uint32_t hits_imask = cur.y;
+                const uint32_t off = firstbithigh(hits_imask);
+                const uint32_t base = cur.x;
+                cur.y &= ~(1u << off);
+                if ((cur.y & 0xff000000u) != 0u) {
+                    if (sp < LDS_STACK) stack[sp][lane] = cur;
+                    else my_spill[(size_t)(sp - LDS_STACK) * lanes] = cur;
+                    sp = min(sp + 1, STACK_SIZE - 1);
+                }
+                const uint32_t slot_index = (off - 24u) ^ (oct & 0xffu);
+                const uint32_t rel = (uint32_t)__popc(hits_imask & ~(0xffffffffu << slot_index));
+                const uint32_t child = base + rel;
+                if (child >= node_count) break;  // malformed input only
+                const uint4* np = nodes + (size_t)child * 5;
+                const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3], q4 = np[4];
+                const uint32_t hm = intersect_node(q0, q1, q2, q3, q4, r, r.tmax, oct);
+                cur = make_uint2(q1.x, (hm & 0xff000000u) | ((q0.w >> 24) & 0xffu));
+                prim = make_uint2(q1.y, hm & 0x00ffffffu);
+            } else {
+                cur = make_uint2(0u, 0u);
+            }
+            while (prim.y != 0u) {
+                const uint32_t local = firstbithigh(prim.y);
+                prim.y &= ~(1u << local);
+                const uint32_t global = prim.x + local;
+                if (tlas_stack_size == INVALID32) {  // in the TLAS: the primitive is a BLAS
+                    if (prim.y != 0u) {
+                        if (sp < LDS_STACK) stack[sp][lane] = prim;
+                        else my_spill[(size_t)(sp - LDS_STACK) * lanes] = prim;
+                        sp = min(sp + 1, STACK_SIZE - 1);
+                    }
+                    if ((cur.y & 0xff000000u) != 0u) {
+                        if (sp < LDS_STACK) stack[sp][lane] = cur;
+                        else my_spill[(size_t)(sp - LDS_STACK) * lanes] = cur;
+                        sp = min(sp + 1, STACK_SIZE - 1);
+                    }
+                    tlas_stack_size = (uint32_t)sp;
+                    const uint32_t b = global < tlas_prim_count ? tlas_prims[global] : INVALID32;
+                    if (b < mesh_count) {
+                        const MeshDesc md = meshes[b];
+                        nodes = blas_nodes + (size_t)md.cw_base * 5;
+                        node_count = md.cw_count;
+                        mtris = tris + (size_t)md.prim_base * 3;
+                        mprims = md.prim_count;
+                        mesh = b;
+                        cur = node_count == 0u ? make_uint2(0u, 0u) : make_uint2(0u, 0x80000000u);
+                    } else {  // malformed input only
+                        cur = make_uint2(0u, 0u);
+                    }
+                    break;
+                }
+                if (global < mprims) {  // in a BLAS: Triangle::intersect with the ray's current tmax
+                    const float4 v0 = mtris[3 * (size_t)global], v1 = mtris[3 * (size_t)global + 1],
+                                 v2 = mtris[3 * (size_t)global + 2];
+                    const float t = tri_intersect(v0, v1, v2, r, r.tmin, r.tmax);
+                    if (t < r.tmax) {
+                        hit_id = global;
+                        hit_geom = mesh;
+                        r.tmax = t;
+                    }
+                }
+            }
+            if ((cur.y & 0xff000000u) == 0u) {
+                if (sp == 0) break;
+                if ((uint32_t)sp == tlas_stack_size) {  // the BLAS is done: back to the TLAS
+                    tlas_stack_size = INVALID32;
+                    mesh = INVALID32;
+                    nodes = tlas;
+                    node_count = tlas_count;
+                }
+                sp -= 1;
+                cur = sp < LDS_STACK ? stack[sp][lane] : my_spill[(size_t)(sp - LDS_STACK) * lanes];
+            }
+        }
+        reinterpret_cast<uint4*>(hits)[i] = make_uint4(hit_id, hit_geom, INVALID32,
+                                                       __float_as_uint(hit_id != INVALID32 ? r.tmax : INFINITY));
+    }
+}
+
 }  // namespace
+
+// Persistent grid: enough blocks for the rays, at most the resident capacity, a multiple of XCDS.
+static uint32_t trav_blocks(uint32_t n, const TravScratch& ts) {
+    const uint32_t want = (n + TRAV_THREADS - 1) / TRAV_THREADS;
+    const uint32_t cap = ts.lanes / TRAV_THREADS / XCDS * XCDS;
+    const uint32_t b = std::min<uint32_t>((want + XCDS - 1) / XCDS * XCDS, cap);
+    return b < XCDS ? XCDS : b;
+}
 
 hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
                            const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,
                            uint32_t* node_visits, uint32_t* tri_tests, const TravScratch& ts, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t blocks = std::min<uint32_t>((n + TRAV_THREADS - 1) / TRAV_THREADS, ts.lanes / TRAV_THREADS);
+    const uint32_t blocks = trav_blocks(n, ts);
     const uint4* nd = reinterpret_cast<const uint4*>(nodes);
     const float4* tr = reinterpret_cast<const float4*>(tris);
     const float4* ry = reinterpret_cast<const float4*>(rays);
@@ -293,7 +418,7 @@ hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, u
                                   uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss, const TravScratch& ts,
                                   hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint32_t blocks = std::min<uint32_t>((n + TRAV_THREADS - 1) / TRAV_THREADS, ts.lanes / TRAV_THREADS);
+    const uint32_t blocks = trav_blocks(n, ts);
     const uint4* nd = reinterpret_cast<const uint4*>(nodes);
     const float4* tr = reinterpret_cast<const float4*>(tris);
     const float4* ry = reinterpret_cast<const float4*>(rays);
@@ -312,7 +437,7 @@ hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, u
 hipError_t launch_traverse_segments(const TraceSegment* segs, uint32_t nseg, uint32_t n, int mode,
                                     const TravScratch& ts, hipStream_t stream) {
     if (n == 0 || nseg == 0) return hipSuccess;
-    const uint32_t blocks = std::min<uint32_t>((n + TRAV_THREADS - 1) / TRAV_THREADS, ts.lanes / TRAV_THREADS);
+    const uint32_t blocks = trav_blocks(n, ts);
     uint2* sp = static_cast<uint2*>(ts.spill);
     if (mode == 0)
         k_traverse<0, false, SRC_SEGMENTS><<<blocks, TRAV_THREADS, 0, stream>>>(
@@ -336,6 +461,23 @@ obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts) {
     ts.spill = ctx->scratch.trav_spill.ptr;
     ts.lanes = ctx->trav_lanes;
     return OBVHS_OK;
+}
+
+}  // namespace obvhs
+
+namespace obvhs {
+
+hipError_t launch_traverse_tlas(const CwNode* tlas, uint32_t tlas_count, const uint32_t* tlas_prims,
+                                uint32_t tlas_prim_count, const CwNode* blas_nodes, const MeshDesc* meshes,
+                                uint32_t mesh_count, const obvhs_triangle* tris, const obvhs_ray* rays, uint32_t n,
+                                obvhs_ray_hit* hits, const TravScratch& ts, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = trav_blocks(n, ts);
+    k_traverse_tlas<<<blocks, TRAV_THREADS, 0, stream>>>(
+        reinterpret_cast<const uint4*>(tlas), tlas_count, tlas_prims, tlas_prim_count,
+        reinterpret_cast<const uint4*>(blas_nodes), meshes, mesh_count, reinterpret_cast<const float4*>(tris),
+        reinterpret_cast<const float4*>(rays), n, hits, static_cast<uint2*>(ts.spill));
+    return hipGetLastError();
 }
 
 }  // namespace obvhs

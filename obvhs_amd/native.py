@@ -86,6 +86,7 @@ SIGNATURES = {
     "obvhs_memcpy_htod": (INT, [P, P, P, U64]),
     "obvhs_memcpy_dtoh": (INT, [P, P, P, U64]),
     "obvhs_memset_device": (INT, [P, P, INT, U64]),
+    "obvhs_memcpy_dtod": (INT, [P, P, P, U64]),
     "obvhs_timer_start": (INT, [P]),
     "obvhs_timer_stop": (INT, [P, C.POINTER(C.c_float)]),
     "obvhs_ploc_build_aabbs": (INT, [P, P, P, U32, U32, U32, U32, P, C.POINTER(P)]),
@@ -118,6 +119,8 @@ SIGNATURES = {
     "obvhs_batch_ray_traverse": (INT, [P, P, P, P, P, U32, P]),
     "obvhs_batch_ray_traverse_miss": (INT, [P, P, P, P, P, U32, P]),
     "obvhs_batch_destroy": (INT, [P]),
+    "obvhs_batch_build_tlas": (INT, [P, P, C.POINTER(BuildParams), C.POINTER(P)]),
+    "obvhs_batch_tlas_ray_traverse": (INT, [P, P, P, P, P, U32, P]),
     "obvhs_build_cwbvh_from_tris_ex": (INT, [P, P, U32, C.POINTER(BuildParams), C.POINTER(BuildExt), P,
                                              C.POINTER(P), C.POINTER(BuildStats)]),
     "obvhs_cwbvh_bvh2": (INT, [P, C.POINTER(P)]),

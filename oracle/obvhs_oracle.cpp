@@ -1208,6 +1208,103 @@ void refit_all(std::vector<Bvh2Node>& nodes) {
     }
 }
 
+// cwbvh/mod.rs:341-491 (CwBvh::ray_traverse_tlas_blas) with the intersection closure fixed to Triangle::intersect
+// on the BLAS's triangles in primitive_indices order.  One shared StackStack<UVec2, 32> for both levels; TLAS
+// primitives are BLAS indices through the TLAS's primitive_indices.  Restated as written, including that a TLAS
+// primitive group popped after a BLAS finishes is treated as a node group and dropped (mod.rs:396-401, 469).
+struct BlasRef {
+    const CwBvhNode* nodes;
+    uint32_t nnodes;
+    const Triangle* tris;
+};
+
+inline void traverse_tlas_one(const CwBvhNode* tlas, uint32_t tlas_nnodes, const uint32_t* tlas_prims,
+                              const BlasRef* blas, uint32_t nblas, Ray ray, RayHit* hit) {
+    struct G {
+        uint32_t x, y;
+    };
+    const int STACK = 32;
+    G stack[STACK];
+    int index = 0;  // StackStack::index
+    auto push = [&](G v) {
+        stack[index] = v;
+        index = std::min(index + 1, STACK - 1);
+    };
+    auto pop_fast = [&]() {
+        index = index > 0 ? index - 1 : 0;
+        return stack[index];
+    };
+    uint32_t tlas_stack_size = INVALID32;
+    uint32_t current_mesh = INVALID32;
+    const CwBvhNode* nodes = tlas;
+    uint32_t nnodes = tlas_nnodes;
+    const uint32_t oct_inv4 = ray_get_octant_inv4(ray.direction);
+    G current = tlas_nnodes == 0 ? G{0, 0} : G{0, 0x80000000u};
+    RayHit h{INVALID32, INVALID32, INVALID32, INFINITY};
+    for (;;) {
+        G prim{0, 0};
+        if ((current.y & 0xff000000u) != 0) {
+            const uint32_t hits_imask = current.y;
+            const uint32_t off = firstbithigh(hits_imask);
+            const uint32_t base = current.x;
+            current.y &= ~(1u << off);
+            if ((current.y & 0xff000000u) != 0) push(current);
+            const uint32_t slot_index = (off - 24) ^ (oct_inv4 & 0xffu);
+            const uint32_t rel = (uint32_t)__builtin_popcount(hits_imask & ~(0xffffffffu << slot_index));
+            const uint32_t child = base + rel;
+            if (child >= nnodes) break;  // out of bounds in the reference (panic); never for valid inputs
+            const CwBvhNode& nd = nodes[child];
+            const uint32_t hitmask = intersect_node(nd, ray, ray.tmax, oct_inv4);
+            current = G{nd.child_base_idx, (hitmask & 0xff000000u) | (uint32_t)nd.imask};
+            prim = G{nd.primitive_base_idx, hitmask & 0x00ffffffu};
+        } else {
+            current = G{0, 0};
+        }
+        while (prim.y != 0) {
+            if (tlas_stack_size == INVALID32) {
+                const uint32_t local = firstbithigh(prim.y);
+                prim.y &= ~(1u << local);
+                const uint32_t global = prim.x + local;
+                if (prim.y != 0) push(prim);
+                if ((current.y & 0xff000000u) != 0) push(current);
+                tlas_stack_size = (uint32_t)index;
+                const uint32_t blas_index = tlas_prims[global];
+                if (blas_index >= nblas) {  // out of bounds in the reference; never for valid inputs
+                    current = G{0, 0};
+                    break;
+                }
+                nodes = blas[blas_index].nodes;
+                nnodes = blas[blas_index].nnodes;
+                current_mesh = blas_index;
+                current = nnodes == 0 ? G{0, 0} : G{0, 0x80000000u};
+                break;
+            } else {
+                const uint32_t local = firstbithigh(prim.y);
+                prim.y &= ~(1u << local);
+                const uint32_t global = prim.x + local;
+                const float t = tri_intersect(blas[current_mesh].tris[global], ray, ray.tmin, ray.tmax);
+                if (t < ray.tmax) {
+                    h.primitive_id = global;
+                    h.geometry_id = current_mesh;
+                    ray.tmax = t;
+                }
+            }
+        }
+        if ((current.y & 0xff000000u) == 0) {
+            if (index == 0) break;
+            if ((uint32_t)index == tlas_stack_size) {
+                tlas_stack_size = INVALID32;
+                current_mesh = INVALID32;
+                nodes = tlas;
+                nnodes = tlas_nnodes;
+            }
+            current = pop_fast();
+        }
+    }
+    if (h.primitive_id != INVALID32) h.t = ray.tmax;
+    *hit = h;
+}
+
 // ===============================================================================================================
 // C ABI for the tests (ctypes)
 // ===============================================================================================================
@@ -1378,6 +1475,19 @@ void orc_cwbvh_traverse(const CwBvhNode* nodes, uint64_t nnodes, const Triangle*
         });
     }
     for (auto& th : pool) th.join();
+}
+
+// ray_traverse_tlas_blas for every ray.  BLAS i: nodes[blas_node_offsets[i] .. + blas_node_counts[i]), triangles
+// tris[blas_tri_offsets[i] ..] in that BLAS's primitive_indices order.
+void orc_tlas_traverse(const CwBvhNode* tlas, uint64_t tlas_nnodes, const uint32_t* tlas_prims,
+                       const CwBvhNode* blas_nodes, const uint32_t* blas_node_offsets,
+                       const uint32_t* blas_node_counts, const Triangle* tris, const uint32_t* blas_tri_offsets,
+                       uint32_t nblas, const Ray* rays, uint64_t nrays, RayHit* out) {
+    std::vector<BlasRef> refs(nblas);
+    for (uint32_t i = 0; i < nblas; ++i)
+        refs[i] = BlasRef{blas_nodes + blas_node_offsets[i], blas_node_counts[i], tris + blas_tri_offsets[i]};
+    for (uint64_t r = 0; r < nrays; ++r)
+        traverse_tlas_one(tlas, (uint32_t)tlas_nnodes, tlas_prims, refs.data(), nblas, rays[r], &out[r]);
 }
 
 // Brute force: min t over all triangles with Triangle::intersect; ties keep the lowest index.

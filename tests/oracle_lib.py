@@ -68,6 +68,7 @@ def lib():
         L.orc_find_reinsertion.argtypes = [P, C.c_uint64, C.c_uint32, P, P, P]
         L.orc_reinsert_node.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32]
         L.orc_refit_all.argtypes = [P, C.c_uint64]
+        L.orc_tlas_traverse.argtypes = [P, C.c_uint64, P, P, P, P, P, P, C.c_uint32, P, C.c_uint64, P]
         _lib = L
     return _lib
 
@@ -228,3 +229,20 @@ def build_cwbvh(tris, search_distance=14, search_depth_threshold=0, max_prims_pe
         b["nodes"] = reinsertion(b["nodes"], reinsertion_batch_ratio, ratio_sequence)
     cw, prims, _ = bvh2_to_cwbvh(b["nodes"], b["prim_indices"], max(1, min(3, max_prims_per_leaf)), True)
     return b, cw, prims
+
+
+def tlas_traverse(tlas_nodes, tlas_prims, blas_list, rays):
+    """CwBvh::ray_traverse_tlas_blas (cwbvh/mod.rs:341-491) for every ray.  blas_list: [(cw_nodes, bvh_tris)]."""
+    from obvhs_amd.types import CWBVH_NODE, RAY_HIT, TRIANGLE
+    nodes = np.concatenate([b[0] for b in blas_list]) if blas_list else np.zeros(0, dtype=CWBVH_NODE)
+    tris = np.concatenate([b[1] for b in blas_list]) if blas_list else np.zeros(0, dtype=TRIANGLE)
+    ncount = np.array([len(b[0]) for b in blas_list], dtype=np.uint32)
+    tcount = np.array([len(b[1]) for b in blas_list], dtype=np.uint32)
+    noff = np.concatenate([[0], np.cumsum(ncount)[:-1]]).astype(np.uint32)
+    toff = np.concatenate([[0], np.cumsum(tcount)[:-1]]).astype(np.uint32)
+    out = np.zeros(len(rays), dtype=RAY_HIT)
+    lib().orc_tlas_traverse(_ptr(np.ascontiguousarray(tlas_nodes)), len(tlas_nodes),
+                            _ptr(np.ascontiguousarray(tlas_prims, dtype=np.uint32)), _ptr(nodes), _ptr(noff),
+                            _ptr(ncount), _ptr(tris), _ptr(toff), len(blas_list),
+                            _ptr(np.ascontiguousarray(rays)), len(rays), _ptr(out))
+    return out

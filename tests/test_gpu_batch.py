@@ -188,3 +188,46 @@ def test_batch_full_icosphere_workload_sampled(ctx):
     seg = np.repeat(np.arange(4096), counts)
     perm_ok[offsets[seg] + prims] = True
     assert perm_ok.all()
+
+
+@pytest.mark.parametrize("tlas_mp", [1, 3])
+def test_tlas_over_batch_matches_oracle(ctx, tlas_mp):
+    """CwBvh::ray_traverse_tlas_blas (cwbvh/mod.rs:341-491) with the batch's meshes as BLASes."""
+    print(f"[case] tlas mp={tlas_mp}", flush=True)
+    rng = np.random.default_rng(8)
+    meshes = []
+    for i in range(24):
+        t = scenes.soup(int(rng.integers(50, 3000)), seed=200 + i)
+        off = rng.uniform(-1.5, 1.5, 3).astype(np.float32)  # overlapping bounds: several leaves per node get hit
+        for v in ("v0", "v1", "v2"):
+            t[v][:, :3] += off
+        meshes.append(t)
+    tris, offsets = concat(meshes)
+    params = ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3)
+    batch = ob.build_cwbvh_batch(ctx, tris, offsets, params)
+    tlas = batch.build_tlas(ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=tlas_mp))
+    roots = []
+    blas = []
+    nodes, prims = batch.all_nodes(), batch.all_primitive_indices()
+    for i, m in enumerate(meshes):
+        b, rcw, rprims = O.build_cwbvh(m, 14, 0, 3)
+        roots.append(b["nodes"][0])
+        blas.append((rcw, m[rprims]))
+    boxes = ob.aabbs_from_minmax(np.array([r["min"][:3] for r in roots]), np.array([r["max"][:3] for r in roots]))
+    t = O.ploc_build(boxes, 14, 0)
+    rtlas, rtprims, _ = O.bvh2_to_cwbvh(t["nodes"], t["prim_indices"], tlas_mp, True)
+    assert_same_bytes(tlas.nodes, rtlas, "TLAS nodes")
+    np.testing.assert_array_equal(tlas.primitive_indices, rtprims)
+    n = 20000
+    o = rng.uniform(-3, 4, (n, 3))
+    pick = rng.integers(0, len(tris), n)  # aimed at triangle centroids of random meshes
+    c = (tris["v0"][pick, :3].astype(np.float64) + tris["v1"][pick, :3] + tris["v2"][pick, :3]) / 3.0
+    d = c - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = ob.make_rays(o.astype(np.float32), d.astype(np.float32), 0.0, 1e30)
+    hits = batch.tlas_ray_traverse(tlas, rays, batch.reorder_triangles(tris))
+    ref = O.tlas_traverse(rtlas, rtprims, blas, rays)
+    np.testing.assert_array_equal(hits["primitive_id"], ref["primitive_id"])
+    np.testing.assert_array_equal(hits["geometry_id"], ref["geometry_id"])
+    np.testing.assert_array_equal(hits["t"].view(np.uint32), ref["t"].view(np.uint32))
+    assert np.isfinite(ref["t"]).mean() > 0.05

@@ -1,0 +1,104 @@
+"""Pins of the oracle's TLAS/BLAS traversal (CwBvh::ray_traverse_tlas_blas, cwbvh/mod.rs:341-491).
+
+* with one BLAS the two-level walk equals the plain traversal of that BLAS (same hits, geometry_id 0);
+* with meshes far apart and rays that enter only one mesh's bounds, hits equal brute force over all triangles;
+* when a ray enters two TLAS leaves of one node, the crate's function enters only the first (the remaining leaf's
+  primitive group is popped as a node group and dropped, mod.rs:396-401, 469): results are brute force over one of
+  the two meshes, and some rays that hit the other mesh report a miss."""
+import numpy as np
+
+import oracle_lib as O
+from obvhs_amd import scenes
+from obvhs_amd.types import aabbs_from_minmax, make_rays, triangles_from_vertices
+
+
+def blas_of(tris, mp=3):
+    b, cw, prims = O.build_cwbvh(tris, 14, 0, mp)
+    return b, cw, prims, tris[prims]
+
+
+def tlas_over(roots, mp=1):
+    boxes = aabbs_from_minmax(np.array([r["min"][:3] for r in roots]), np.array([r["max"][:3] for r in roots]))
+    t = O.ploc_build(boxes, 14, 0)
+    cw, prims, _ = O.bvh2_to_cwbvh(t["nodes"], t["prim_indices"], mp, True)
+    return cw, prims
+
+
+def test_single_blas_equals_plain_traversal():
+    tris = scenes.soup(3000, seed=2)
+    b, cw, prims, bt = blas_of(tris)
+    tlas, tprims = tlas_over([b["nodes"][0]])
+    rays = scenes.pinhole_rays(48, 48, eye=(0.5, 0.5, -1.5))
+    rays = np.concatenate([rays, make_rays(np.full((500, 3), -1.0), np.random.default_rng(1).uniform(0.2, 1, (500, 3)))])
+    h = O.tlas_traverse(tlas, tprims, [(cw, bt)], rays)
+    p = O.traverse(cw, bt, rays)
+    np.testing.assert_array_equal(h["primitive_id"], p["primitive_id"])
+    np.testing.assert_array_equal(h["t"].view(np.uint32), p["t"].view(np.uint32))
+    hit = p["primitive_id"] != 0xFFFFFFFF
+    assert hit.any() and (h["geometry_id"][hit] == 0).all() and (h["geometry_id"][~hit] == 0xFFFFFFFF).all()
+
+
+def separated_meshes(k=8):
+    meshes = []
+    for i in range(k):
+        t = scenes.soup(400 + 50 * i, seed=10 + i)
+        for v in ("v0", "v1", "v2"):
+            t[v][:, 0] += np.float32(10.0 * i)
+        meshes.append(t)
+    return meshes
+
+
+def test_separated_meshes_equal_brute_force():
+    meshes = separated_meshes()
+    parts = [blas_of(m) for m in meshes]
+    tlas, tprims = tlas_over([p[0]["nodes"][0] for p in parts], mp=1)
+    rng = np.random.default_rng(4)
+    origins, dirs, target = [], [], []
+    for i, m in enumerate(meshes):  # rays along +z into mesh i's footprint only
+        c = (m["v0"][:, :3].astype(np.float64) + m["v1"][:, :3] + m["v2"][:, :3]) / 3.0
+        pick = rng.integers(0, len(m), 200)
+        o = c[pick].copy()
+        o[:, 2] = -2.0
+        origins.append(o)
+        dirs.append(np.tile([0.0, 0.0, 1.0], (200, 1)))
+        target.append(np.full(200, i))
+    rays = make_rays(np.concatenate(origins), np.concatenate(dirs))
+    target = np.concatenate(target)
+    h = O.tlas_traverse(tlas, tprims, [(p[1], p[3]) for p in parts], rays)
+    all_tris = np.concatenate(meshes)
+    bf = O.brute_force(all_tris, rays)
+    np.testing.assert_array_equal(h["t"].view(np.uint32), bf["t"].view(np.uint32))
+    hit = np.isfinite(bf["t"])
+    assert hit.mean() > 0.5
+    assert (h["geometry_id"][hit] == target[hit]).all()
+    offs = np.concatenate([[0], np.cumsum([len(m) for m in meshes])])
+    for r in np.nonzero(hit)[0][:300]:
+        g = h["geometry_id"][r]
+        assert parts[g][2][h["primitive_id"][r]] + offs[g] == bf["primitive_id"][r]
+
+
+def test_second_leaf_of_a_node_is_dropped():
+    # two stacked sheets of small triangles: A at z = 0, B at z = 1, same xy footprint; rays along +z cross both
+    a = scenes.soup(800, seed=31)
+    b = scenes.soup(800, seed=32)
+    for t, z in ((a, 0.0), (b, 1.0)):
+        for v in ("v0", "v1", "v2"):
+            t[v][:, 2] = np.float32(z) + (t[v][:, 2] - np.float32(0.5)) * np.float32(0.001)
+    parts = [blas_of(a), blas_of(b)]
+    tlas, tprims = tlas_over([p[0]["nodes"][0] for p in parts], mp=1)
+    assert len(tlas) == 1  # one node holding both leaves
+    rng = np.random.default_rng(6)
+    xy = rng.uniform(0.05, 0.95, (4000, 2))
+    rays = make_rays(np.column_stack([xy, np.full(4000, -1.0)]), np.tile([0.0, 0.0, 1.0], (4000, 1)))
+    h = O.tlas_traverse(tlas, tprims, [(p[1], p[3]) for p in parts], rays)
+    entered = []
+    for g, (m, part) in enumerate(zip((a, b), parts)):
+        only = O.traverse(part[1], part[3], rays)
+        entered.append(only)
+    # every result is the traversal of exactly one of the two meshes (the leaf entered first)
+    same_a = (h["t"].view(np.uint32) == entered[0]["t"].view(np.uint32))
+    same_b = (h["t"].view(np.uint32) == entered[1]["t"].view(np.uint32))
+    assert (same_a | same_b).all()
+    full = O.brute_force(np.concatenate([a, b]), rays)
+    dropped = np.isfinite(full["t"]) & ~np.isfinite(h["t"])
+    assert dropped.any()  # rays that hit the other sheet only were not traced into it
