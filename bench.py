@@ -51,6 +51,47 @@ def traffic_per_launch():
         return None
 
 
+def write_traffic(fetch_csv, write_csv):
+    """Turns the two `--pmc-probe` counter runs (FETCH_SIZE, then WRITE_SIZE; rocprofv3 CSV, KiB per dispatch) into
+    TRAFFIC_FILE.  Each counter is scaled by what it reads for the copy of PMC_COPY_BYTES that the probe runs first
+    (that copy reads and writes exactly PMC_COPY_BYTES); the primary and bounce launches are the first two
+    traversal dispatches after it."""
+    import csv
+
+    def per_dispatch(path, counter):
+        rows = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                d = int(r["Dispatch_Id"])
+                name, val = r["Kernel_Name"], float(r["Counter_Value"])
+                prev = rows.get(d, (name, 0.0))
+                rows[d] = (name, prev[1] + val)  # summed over agents/dimensions if split
+        return sorted((d, n, v) for d, (n, v) in rows.items())
+
+    out = {"copy_bytes": PMC_COPY_BYTES}
+    for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
+        rows = per_dispatch(path, counter)
+        copies = [(d, v) for d, n, v in rows if "copyBuffer" in n]
+        if not copies:
+            raise SystemExit(f"{path}: no copy dispatch")
+        copy_d, copy_kib = max(copies, key=lambda t: t[1])
+        trav = [(d, v) for d, n, v in rows if d > copy_d and "k_traverse" in n][:2]
+        if len(trav) != 2:
+            raise SystemExit(f"{path}: expected two traversal dispatches after the copy")
+        scale = PMC_COPY_BYTES / (copy_kib * 1024.0)
+        out[key] = {"copy_kib": copy_kib, "scale": scale, "primary_kib": trav[0][1], "bounce_kib": trav[1][1]}
+    for launch in ("primary", "bounce"):
+        fb = out["fetch"][f"{launch}_kib"] * 1024.0 * out["fetch"]["scale"]
+        wb = out["write"][f"{launch}_kib"] * 1024.0 * out["write"]["scale"]
+        out[launch] = {"bytes": int(round(fb + wb)), "fetch_bytes": int(round(fb)), "write_bytes": int(round(wb))}
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps(out))
+
+
 def cpu_baseline(tris, rays_all, threads, sample_rays):
     """Times the oracle's traversal (all `threads` host threads) and its serial build on this host."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -385,6 +426,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="one copy kernel + one traversal per batch, then exit")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the HBM traffic file from two --pmc-probe counter runs, then exit")
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 20)
     ap.add_argument("--batch-meshes", type=int, default=4096, help="0 skips the batched leg")
     ap.add_argument("--batch-rays-per-mesh", type=int, default=1024)
@@ -397,6 +440,9 @@ def main():
     ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--serve-update-streams", type=int, default=3)
     args = ap.parse_args()
+    if args.traffic_from:
+        write_traffic(*args.traffic_from)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "design-only, see the design notes)"}))
