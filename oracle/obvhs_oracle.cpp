@@ -98,6 +98,7 @@ enum {
     MUT_REINS_PUSH_ORDER = 12,  // reinsertion search pushes the right child before the left one
     MUT_REINS_NO_CONFLICT = 13,  // reinsertions applied without the conflict check
     MUT_REFIT_FULL = 14,       // refit_from_fast never stops early
+    MUT_REBUILD_SLOT_PARITY = 15,  // partial rebuild checks the right (even) slot of a pair for being free
 };
 int g_mut = MUT_NONE;
 
@@ -252,32 +253,33 @@ struct PlocTrace {
     uint32_t max_offset_abs;   // largest |merge offset| seen (diagnostic)
 };
 
-// ploc/mod.rs:169-251 (build_ploc) + 258-503 (build_ploc_from_leaves, REBUILD = false) + 771-847 (sort)
-int ploc_build(const Aabb* aabbs, const uint32_t* indices, size_t n, size_t sd, size_t search_depth_threshold,
-               bool basic_search, Bvh2Node* out_nodes, uint32_t* out_prim_indices, PlocTrace* tr) {
-    if (sd < 1 || sd > 32) return ORC_ERR_INVALID_ARG;
-    for (size_t i = 0; i < n; ++i) out_prim_indices[i] = indices[i];  // reset_for_reuse (bvh2/mod.rs:106-121)
-    if (tr) {
-        tr->num_iters = 0;
-        tr->depth = 0;
-        tr->max_offset_abs = 0;
-    }
-    if (n == 0) return ORC_OK;
+constexpr uint32_t NODE_INVALID_BIT = 0x80000000u;  // Bvh2Node::set_invalid (bvh2/node.rs:137-151)
+inline bool node_valid(const Bvh2Node& n) { return (n.prim_count & NODE_INVALID_BIT) == 0u; }
 
-    std::vector<Bvh2Node> current(n), next(n);
-    Aabb total = aabb_empty();
-    for (size_t i = 0; i < n; ++i) {  // ploc/mod.rs:187-194, 233-243
-        Aabb a = aabbs[i];
-        a.min.w = a.max.w = 0.0f;
-        if (has_nan(a)) return ORC_ERR_NAN_INPUT;
-        aabb_extend(total, a.min);
-        aabb_extend(total, a.max);
-        current[i] = Bvh2Node{a, 1u, indices[i], 0u, 0u};
-    }
-
+// ploc/mod.rs:258-503 (build_ploc_from_leaves) + 771-847 (sort) over the clusters in `current`.
+//   tie      : sort key that orders equal Morton codes (the reference's sorts are unstable there); null = position
+//              in `current`.
+//   rebuild  : the REBUILD = true placement (ploc/mod.rs:449-462): every merge scans down from the array end for
+//              the next sibling pair whose left slot is invalid.  Otherwise children fill nodes[2n-3 ..] downwards
+//              and the array is exactly 2n-1 long.
+// `nodes` has nodes_len entries; out_depth receives the pass count.
+int ploc_from_leaves(std::vector<Bvh2Node>& current, const uint32_t* tie, const Aabb& total, size_t sd,
+                     size_t search_depth_threshold, bool basic_search, bool rebuild, Bvh2Node* out_nodes,
+                     size_t nodes_len, PlocTrace* tr, size_t* out_depth) {
+    const size_t n = current.size();
+    if (out_depth) *out_depth = 0;
+    if (n == 0) return ORC_OK;  // ploc/mod.rs:268-270
+    std::vector<Bvh2Node> next(n);
     const size_t nodes_count = 2 * n - 1;  // ploc/mod.rs:274
-    size_t insert_index = nodes_count;     // ploc/mod.rs:283-284
-    for (size_t i = 0; i < nodes_count; ++i) out_nodes[i] = Bvh2Node{};
+    size_t insert_index;
+    if (rebuild) {  // ploc/mod.rs:276-282
+        if (nodes_len == 0) return ORC_OK;
+        if (nodes_len < nodes_count) return ORC_ERR_CAPACITY;
+        insert_index = nodes_len - 1;
+    } else {
+        if (nodes_len != nodes_count) return ORC_ERR_CAPACITY;
+        insert_index = nodes_count;  // ploc/mod.rs:283-284
+    }
 
     // ploc/mod.rs:287-288: scale = 1 / diagonal (f64), offset = -min * scale
     double diag[3] = {(double)(total.max.x - total.min.x), (double)(total.max.y - total.min.y),
@@ -297,10 +299,12 @@ int ploc_build(const Aabb* aabbs, const uint32_t* indices, size_t n, size_t sd, 
         mortons[i] = Morton64{morton_encode_u64_unorm(px, py, pz), (uint64_t)i};
         if (tr && tr->codes) tr->codes[i] = mortons[i].code;
     }
-    // ploc/mod.rs:811-827. The reference uses unstable sorts; ties are ordered by original index here.
+    // ploc/mod.rs:811-827. The reference uses unstable sorts; ties are ordered by the tie key here (the original
+    // index for a build, the node slot for a partial rebuild).
     const bool ties_desc = g_mut == MUT_SORT_TIES_DESC;
-    std::sort(mortons.begin(), mortons.end(), [ties_desc](const Morton64& a, const Morton64& b) {
-        return a.code < b.code || (a.code == b.code && (ties_desc ? a.index > b.index : a.index < b.index));
+    std::sort(mortons.begin(), mortons.end(), [ties_desc, tie](const Morton64& a, const Morton64& b) {
+        const uint64_t ta = tie ? tie[a.index] : a.index, tb = tie ? tie[b.index] : b.index;
+        return a.code < b.code || (a.code == b.code && (ties_desc ? ta > tb : ta < tb));
     });
     // ploc/mod.rs:829-846 remap, then swap (ploc/mod.rs:305)
     for (size_t k = 0; k < n; ++k) {
@@ -364,13 +368,29 @@ int ploc_build(const Aabb* aabbs, const uint32_t* indices, size_t n, size_t sd, 
             Bvh2Node left = current[index];
             Bvh2Node right = current[best_index];
             if (g_mut == MUT_PLOC_SWAP_CHILDREN) std::swap(left, right);
-            if (insert_index < 2) return ORC_ERR_PLOC_SELF_MERGE;  // reference panics out of bounds here
-            insert_index -= 2;
-            size_t first_child = insert_index;
-            if (g_mut == MUT_PLOC_SLOT_ORDER) {
-                first_child = ascending_slot;
-                ascending_slot += 2;
+            size_t first_child;
+            if (rebuild) {
+                // ploc/mod.rs:449-462: the left slot of the next free sibling pair, scanning down.
+                for (;;) {
+                    if (insert_index < 2) return ORC_ERR_CAPACITY;  // reference panics out of bounds here
+                    const size_t left_slot = g_mut == MUT_REBUILD_SLOT_PARITY ? insert_index : insert_index - 1;
+                    if (!node_valid(out_nodes[left_slot])) {
+                        first_child = left_slot;
+                        insert_index -= 2;
+                        break;
+                    }
+                    insert_index -= 2;
+                }
+            } else {
+                if (insert_index < 2) return ORC_ERR_PLOC_SELF_MERGE;  // reference panics out of bounds here
+                insert_index -= 2;
+                first_child = insert_index;
+                if (g_mut == MUT_PLOC_SLOT_ORDER) {
+                    first_child = ascending_slot;
+                    ascending_slot += 2;
+                }
             }
+            if (first_child + 1 >= nodes_len) return ORC_ERR_CAPACITY;
             out_nodes[first_child] = left;
             out_nodes[first_child + 1] = right;
             Aabb u = aabb_union(left.aabb, right.aabb);
@@ -386,10 +406,39 @@ int ploc_build(const Aabb* aabbs, const uint32_t* indices, size_t n, size_t sd, 
         next_nodes_idx = 0;
         depth += 1;
     }
-    if (insert_index != 1) return ORC_ERR_PLOC_SELF_MERGE;  // ploc/mod.rs:496 debug_assert
-    out_nodes[0] = current[0];                              // ploc/mod.rs:499
+    if (!rebuild && insert_index != 1) return ORC_ERR_PLOC_SELF_MERGE;  // ploc/mod.rs:495-497 debug_assert
+    out_nodes[0] = current[0];                                          // ploc/mod.rs:499
     if (tr) tr->depth = (uint32_t)depth;
+    if (out_depth) *out_depth = depth;
     return ORC_OK;
+}
+
+// ploc/mod.rs:169-251 (build_ploc): leaves from the boxes, scene bounds, then build_ploc_from_leaves.
+int ploc_build(const Aabb* aabbs, const uint32_t* indices, size_t n, size_t sd, size_t search_depth_threshold,
+               bool basic_search, Bvh2Node* out_nodes, uint32_t* out_prim_indices, PlocTrace* tr) {
+    if (sd < 1 || sd > 32) return ORC_ERR_INVALID_ARG;
+    for (size_t i = 0; i < n; ++i) out_prim_indices[i] = indices[i];  // reset_for_reuse (bvh2/mod.rs:106-121)
+    if (tr) {
+        tr->num_iters = 0;
+        tr->depth = 0;
+        tr->max_offset_abs = 0;
+    }
+    if (n == 0) return ORC_OK;
+
+    std::vector<Bvh2Node> current(n);
+    Aabb total = aabb_empty();
+    for (size_t i = 0; i < n; ++i) {  // ploc/mod.rs:187-194, 233-243
+        Aabb a = aabbs[i];
+        a.min.w = a.max.w = 0.0f;
+        if (has_nan(a)) return ORC_ERR_NAN_INPUT;
+        aabb_extend(total, a.min);
+        aabb_extend(total, a.max);
+        current[i] = Bvh2Node{a, 1u, indices[i], 0u, 0u};
+    }
+    const size_t nodes_count = 2 * n - 1;
+    for (size_t i = 0; i < nodes_count; ++i) out_nodes[i] = Bvh2Node{};
+    return ploc_from_leaves(current, nullptr, total, sd, search_depth_threshold, basic_search, false, out_nodes,
+                            nodes_count, tr, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1305,6 +1354,97 @@ inline void traverse_tlas_one(const CwBvhNode* tlas, uint32_t tlas_nnodes, const
     *hit = h;
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Partial / full rebuild (ploc/rebuild.rs) and primitive -> leaf map (bvh2/mod.rs:647-663)
+// ---------------------------------------------------------------------------------------------------------------
+
+// ploc/rebuild.rs:12-42 (compute_rebuild_path_flags): flag each given leaf and its ancestors, stopping at the first
+// ancestor already flagged.  Nothing is written for a tree of fewer than two nodes.
+int rebuild_path_flags(const std::vector<Bvh2Node>& nodes, const uint32_t* leaves, size_t n_leaves, uint8_t* flags) {
+    if (nodes.size() < 2) return ORC_OK;
+    std::vector<uint32_t> parents;
+    compute_parents(nodes, parents);
+    std::fill(flags, flags + nodes.size(), (uint8_t)0);
+    for (size_t k = 0; k < n_leaves; ++k) {
+        size_t index = leaves[k];
+        if (index >= nodes.size()) return ORC_ERR_INVALID_ARG;  // reference: index out of bounds
+        flags[index] = 1;
+        size_t steps = 0;
+        while (index > 0) {
+            index = parents[index];
+            if (flags[index]) break;
+            flags[index] = 1;
+            if (++steps > nodes.size()) return ORC_ERR_INVALID_ARG;  // not a tree
+        }
+    }
+    return ORC_OK;
+}
+
+// ploc/rebuild.rs:98-134 (partial_rebuild) + 136-183 (rebuild_from_leaves::<true>): the root's subtree is walked
+// top-down with a stack; a node that is not flagged, or is a leaf, becomes a cluster (its subtree stays where it
+// is), a flagged inner node is opened.  Every visited slot is marked invalid, and build_ploc_from_leaves::<REBUILD>
+// refills exactly those slots.  The scene box is the root's stored box.  Clusters with equal Morton codes are
+// ordered by their old slot.
+int partial_rebuild(std::vector<Bvh2Node>& nodes, const uint8_t* flags, size_t sd, size_t sdt, size_t* out_depth) {
+    if (out_depth) *out_depth = 0;
+    if (nodes.size() < 2) return ORC_OK;
+    std::vector<Bvh2Node> current;
+    std::vector<uint32_t> slot_of;
+    std::vector<uint32_t> stack{nodes[0].first_index};
+    size_t visited = 0;
+    while (!stack.empty()) {
+        const uint32_t left = stack.back();
+        stack.pop_back();
+        for (size_t node_index : {(size_t)left, (size_t)left + 1}) {
+            if (node_index >= nodes.size() || ++visited > nodes.size()) return ORC_ERR_INVALID_ARG;
+            Bvh2Node& node = nodes[node_index];
+            if (!flags[node_index] || node.prim_count != 0u) {
+                current.push_back(node);
+                slot_of.push_back((uint32_t)node_index);
+            } else {
+                stack.push_back(node.first_index);
+            }
+            node.prim_count |= NODE_INVALID_BIT;
+        }
+    }
+    const Aabb total = nodes[0].aabb;  // rebuild.rs:154
+    return ploc_from_leaves(current, slot_of.data(), total, sd, sdt, false, true, nodes.data(), nodes.size(),
+                            nullptr, out_depth);
+}
+
+// ploc/rebuild.rs:53-79 (full_rebuild) + rebuild_from_leaves::<false>: every leaf, in slot order, rebuilt with the
+// root's stored box as the scene box.  The node array becomes 2k-1 long for k leaves.
+int full_rebuild(std::vector<Bvh2Node>& nodes, size_t sd, size_t sdt, size_t* out_depth) {
+    if (out_depth) *out_depth = 0;
+    if (nodes.size() < 2) return ORC_OK;
+    std::vector<Bvh2Node> current;
+    for (const Bvh2Node& node : nodes)
+        if (node.prim_count != 0u) current.push_back(node);
+    const Aabb total = nodes[0].aabb;
+    if (current.empty()) return ORC_OK;
+    nodes.resize(2 * current.size() - 1);
+    return ploc_from_leaves(current, nullptr, total, sd, sdt, false, false, nodes.data(), nodes.size(), nullptr,
+                            out_depth);
+}
+
+// bvh2/mod.rs:647-663 (compute_primitives_to_nodes): leaf slot of every primitive, INVALID where none.
+int primitives_to_nodes(const Bvh2Node* nodes, size_t nnodes, const uint32_t* prim_indices, size_t nprims,
+                        uint32_t* out) {
+    std::fill(out, out + nprims, INVALID32);
+    for (size_t node_id = 0; node_id < nnodes; ++node_id) {
+        const Bvh2Node& node = nodes[node_id];
+        if (node.prim_count == 0u) continue;
+        const uint64_t start = node.first_index, end = start + node.prim_count;
+        for (uint64_t k = start; k < end; ++k) {
+            if (k >= nprims) return ORC_ERR_INVALID_ARG;
+            const uint32_t prim_id = prim_indices[k];
+            if (prim_id >= nprims) return ORC_ERR_INVALID_ARG;
+            out[prim_id] = (uint32_t)node_id;
+        }
+    }
+    return ORC_OK;
+}
+
 // ===============================================================================================================
 // C ABI for the tests (ctypes)
 // ===============================================================================================================
@@ -1433,6 +1573,43 @@ void orc_reinsert_node(Bvh2Node* nodes, uint64_t nnodes, uint32_t from, uint32_t
     compute_parents(v, parents);
     reinsert_node(v, parents, from, to);
     std::memcpy(nodes, v.data(), nnodes * sizeof(Bvh2Node));
+}
+
+// compute_rebuild_path_flags (ploc/rebuild.rs:12-42); flags has nnodes bytes.
+int orc_rebuild_path_flags(const Bvh2Node* nodes, uint64_t nnodes, const uint32_t* leaves, uint64_t n_leaves,
+                           uint8_t* flags) {
+    std::vector<Bvh2Node> v(nodes, nodes + nnodes);
+    return rebuild_path_flags(v, leaves, (size_t)n_leaves, flags);
+}
+
+// PlocBuilder::partial_rebuild(bvh, |i| flags[i] != 0, search_distance, U64, search_depth_threshold) in place.
+int orc_partial_rebuild(Bvh2Node* nodes, uint64_t nnodes, const uint8_t* flags, uint32_t search_distance,
+                        uint32_t search_depth_threshold, uint32_t* out_depth) {
+    std::vector<Bvh2Node> v(nodes, nodes + nnodes);
+    size_t depth = 0;
+    int rc = partial_rebuild(v, flags, search_distance, search_depth_threshold, &depth);
+    std::memcpy(nodes, v.data(), nnodes * sizeof(Bvh2Node));
+    if (out_depth) *out_depth = (uint32_t)depth;
+    return rc;
+}
+
+// PlocBuilder::full_rebuild(bvh, search_distance, U64, search_depth_threshold) in place; *out_nnodes receives the
+// new node count (2k-1 for k leaves, equal to nnodes for a well-formed tree).
+int orc_full_rebuild(Bvh2Node* nodes, uint64_t nnodes, uint32_t search_distance, uint32_t search_depth_threshold,
+                     uint64_t* out_nnodes, uint32_t* out_depth) {
+    std::vector<Bvh2Node> v(nodes, nodes + nnodes);
+    size_t depth = 0;
+    int rc = full_rebuild(v, search_distance, search_depth_threshold, &depth);
+    if (v.size() > nnodes) return ORC_ERR_CAPACITY;
+    std::memcpy(nodes, v.data(), v.size() * sizeof(Bvh2Node));
+    if (out_nnodes) *out_nnodes = v.size();
+    if (out_depth) *out_depth = (uint32_t)depth;
+    return rc;
+}
+
+int orc_primitives_to_nodes(const Bvh2Node* nodes, uint64_t nnodes, const uint32_t* prim_indices, uint64_t nprims,
+                            uint32_t* out) {
+    return primitives_to_nodes(nodes, (size_t)nnodes, prim_indices, (size_t)nprims, out);
 }
 
 // Bvh2::refit_all (bvh2/mod.rs:527-569) in place.

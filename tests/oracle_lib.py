@@ -69,6 +69,14 @@ def lib():
         L.orc_reinsert_node.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32]
         L.orc_refit_all.argtypes = [P, C.c_uint64]
         L.orc_tlas_traverse.argtypes = [P, C.c_uint64, P, P, P, P, P, P, C.c_uint32, P, C.c_uint64, P]
+        L.orc_rebuild_path_flags.restype = C.c_int
+        L.orc_rebuild_path_flags.argtypes = [P, C.c_uint64, P, C.c_uint64, P]
+        L.orc_partial_rebuild.restype = C.c_int
+        L.orc_partial_rebuild.argtypes = [P, C.c_uint64, P, C.c_uint32, C.c_uint32, P]
+        L.orc_full_rebuild.restype = C.c_int
+        L.orc_full_rebuild.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, P, P]
+        L.orc_primitives_to_nodes.restype = C.c_int
+        L.orc_primitives_to_nodes.argtypes = [P, C.c_uint64, P, C.c_uint64, P]
         _lib = L
     return _lib
 
@@ -213,6 +221,43 @@ def refit_all(nodes):
     out = np.ascontiguousarray(nodes).copy()
     lib().orc_refit_all(_ptr(out), len(out))
     return out
+
+
+def rebuild_path_flags(nodes, leaves):
+    """compute_rebuild_path_flags: one byte per node."""
+    nodes = np.ascontiguousarray(nodes)
+    leaves = np.ascontiguousarray(leaves, dtype=np.uint32)
+    flags = np.zeros(len(nodes), np.uint8)
+    rc = lib().orc_rebuild_path_flags(_ptr(nodes), len(nodes), _ptr(leaves), len(leaves), _ptr(flags))
+    return rc, flags
+
+
+def partial_rebuild(nodes, flags, search_distance=1, search_depth_threshold=0):
+    """PlocBuilder::partial_rebuild on a copy; returns (rc, nodes, pass count)."""
+    out = np.ascontiguousarray(nodes).copy()
+    flags = np.ascontiguousarray(flags, dtype=np.uint8)
+    assert len(flags) == len(out)
+    depth = C.c_uint32()
+    rc = lib().orc_partial_rebuild(_ptr(out), len(out), _ptr(flags), search_distance, search_depth_threshold,
+                                   C.byref(depth))
+    return rc, out, depth.value
+
+
+def full_rebuild(nodes, search_distance=1, search_depth_threshold=0):
+    """PlocBuilder::full_rebuild on a copy; returns (rc, nodes, pass count)."""
+    out = np.ascontiguousarray(nodes).copy()
+    nn, depth = C.c_uint64(), C.c_uint32()
+    rc = lib().orc_full_rebuild(_ptr(out), len(out), search_distance, search_depth_threshold, C.byref(nn),
+                                C.byref(depth))
+    return rc, out[:nn.value] if rc == 0 else out, depth.value
+
+
+def primitives_to_nodes(nodes, prim_indices):
+    prim_indices = np.ascontiguousarray(prim_indices, dtype=np.uint32)
+    out = np.zeros(len(prim_indices), np.uint32)
+    rc = lib().orc_primitives_to_nodes(_ptr(np.ascontiguousarray(nodes)), len(nodes), _ptr(prim_indices),
+                                       len(prim_indices), _ptr(out))
+    return rc, out
 
 
 def build_cwbvh(tris, search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3, pre_split=False,
