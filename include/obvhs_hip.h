@@ -171,6 +171,36 @@ obvhs_status obvhs_bvh2_refit_all(obvhs_context* ctx, obvhs_bvh2* bvh);
 /* The refit example of Bvh2::refit_all (bvh2/mod.rs:506-526): every one-primitive leaf takes the box of its
  * triangle in d_tris (original order, device), then refit_all.  For BVH2s built without pre-splits. */
 obvhs_status obvhs_bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh, const obvhs_triangle* d_tris);
+/* ---- BVH2 from its parts, leaf updates and rebuilds (ploc/rebuild.rs) ----
+ * A Bvh2 from host arrays (the crate's Bvh2 fields nodes / primitive_indices are public, bvh2/mod.rs:31-85): 48-byte
+ * nodes (bvh2/node.rs:38-66), node_count of them, and prim_count primitive indices.  Inner nodes must point inside
+ * the array and leaves inside primitive_indices (OBVHS_ERR_INVALID_ARG otherwise). */
+obvhs_status obvhs_bvh2_create_from_host(obvhs_context* ctx, const obvhs_bvh2_node* nodes, uint32_t node_count,
+                                         const uint32_t* prim_indices, uint32_t prim_count, obvhs_bvh2** out);
+/* Bvh2Node::set_aabb (bvh2/node.rs:116-125) for count nodes: d_aabbs[i] becomes the box of node d_node_ids[i]
+ * (device arrays; ids distinct).  Nothing above them is refit, as examples/physics.rs:433-439 does before a
+ * partial rebuild. */
+obvhs_status obvhs_bvh2_set_node_aabbs(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_node_ids,
+                                       const obvhs_aabb* d_aabbs, uint32_t count);
+/* Bvh2::compute_primitives_to_nodes (bvh2/mod.rs:647-663): d_out[p] (prim_count entries, device) = slot of the leaf
+ * holding primitive p, INVALID (0xffffffff) if none; the highest slot when several leaves hold it. */
+obvhs_status obvhs_bvh2_primitives_to_nodes(obvhs_context* ctx, const obvhs_bvh2* bvh, uint32_t* d_out);
+/* compute_rebuild_path_flags (ploc/rebuild.rs:12-42): d_flags (node_count bytes, device) = 1 for every given leaf
+ * and each of its ancestors, 0 elsewhere; untouched for a BVH of fewer than two nodes.  Parents are computed here
+ * (the crate requires init_parents_if_uninit first). */
+obvhs_status obvhs_bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_leaf_ids,
+                                           uint32_t count, uint8_t* d_flags);
+/* PlocBuilder::partial_rebuild(bvh, |i| d_flags[i] != 0, search_distance, SortPrecision::U64,
+ * search_depth_threshold) (ploc/rebuild.rs:98-134): the flagged inner nodes under the root are opened, the leaves
+ * and unflagged subtrees they expose are rebuilt into a new top with PLOC, and the new nodes go to the freed sibling
+ * pairs, highest first.  Subtrees that were not opened keep their slots.  Clusters with equal Morton codes are
+ * ordered by their old slot (the crate's unstable sort leaves that order open). */
+obvhs_status obvhs_bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,
+                                        uint32_t search_distance, uint32_t search_depth_threshold);
+/* PlocBuilder::full_rebuild(bvh, search_distance, SortPrecision::U64, search_depth_threshold)
+ * (ploc/rebuild.rs:53-79): every leaf, in slot order, rebuilt with PLOC; the root's stored box is the scene box. */
+obvhs_status obvhs_bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance,
+                                     uint32_t search_depth_threshold);
 /* The pre-split leaf set of build_cwbvh_from_tris with pre_split = true (cwbvh/builder.rs:26-55 +
  * split_aabbs_preset, splits.rs:16-33): writes out_count boxes and triangle indices to caller device arrays of
  * out_capacity entries (OBVHS_ERR_CAPACITY with *out_count set when they do not fit).  max_total as

@@ -256,6 +256,33 @@ class Bvh2:
         check(lib().obvhs_bvh2_refit_from_triangles(self.ctx.handle, self.handle, d.ptr), self.ctx.handle)
         return self
 
+    @classmethod
+    def from_host(cls, ctx, nodes, primitive_indices):
+        """A device Bvh2 from its parts: Bvh2Node[] (48-byte layout) and primitive_indices (bvh2/mod.rs:31-85)."""
+        nodes = np.ascontiguousarray(nodes, dtype=BVH2_NODE)
+        prims = np.ascontiguousarray(primitive_indices, dtype=np.uint32)
+        out = C.c_void_p()
+        check(lib().obvhs_bvh2_create_from_host(ctx.handle, nodes.ctypes.data_as(C.c_void_p), len(nodes),
+                                                prims.ctypes.data_as(C.c_void_p), len(prims), C.byref(out)),
+              ctx.handle)
+        return cls(ctx, out)
+
+    def set_node_aabbs(self, node_ids, aabbs):
+        """Bvh2Node::set_aabb (bvh2/node.rs:116-125) for each (id, box); nothing above is refit."""
+        di, n = _as_device(self.ctx, node_ids if isinstance(node_ids, DeviceArray)
+                           else np.asarray(node_ids, dtype=np.uint32), np.uint32)
+        da, m = _as_device(self.ctx, aabbs, AABB)
+        if n != m:
+            raise ValueError("one box per node id")
+        check(lib().obvhs_bvh2_set_node_aabbs(self.ctx.handle, self.handle, di.ptr, da.ptr, n), self.ctx.handle)
+        return self
+
+    def primitives_to_nodes(self):
+        """Bvh2::compute_primitives_to_nodes (bvh2/mod.rs:647-663) as a device u32 array (INVALID where none)."""
+        out = DeviceArray(self.ctx, np.uint32, self._info()[1])
+        check(lib().obvhs_bvh2_primitives_to_nodes(self.ctx.handle, self.handle, out.ptr), self.ctx.handle)
+        return out
+
     def close(self):
         if self.handle and self.owned:
             lib().obvhs_bvh2_destroy(self.handle)
@@ -299,6 +326,51 @@ class PlocBuilder:
         if bvh is not None:
             return bvh
         return Bvh2(self.ctx, out)
+
+    def full_rebuild(self, bvh, search_distance, sort_precision=SortPrecision.U64, search_depth_threshold=0):
+        """ploc/rebuild.rs:53-79: rebuild `bvh` from its current leaves."""
+        sd = PlocSearchDistance.from_u32(int(search_distance))
+        _check_u64(sort_precision)
+        check(lib().obvhs_bvh2_full_rebuild(self.ctx.handle, bvh.handle, int(sd), int(search_depth_threshold)),
+              self.ctx.handle)
+        return bvh
+
+    def partial_rebuild(self, bvh, should_remove, search_distance, sort_precision=SortPrecision.U64,
+                        search_depth_threshold=0):
+        """ploc/rebuild.rs:98-134.  `should_remove` is one flag per node (a device u8 array such as
+        compute_rebuild_path_flags returns, or a host bool/u8 array): the closure |i| flags[i]."""
+        sd = PlocSearchDistance.from_u32(int(search_distance))
+        _check_u64(sort_precision)
+        n = bvh.node_count
+        if isinstance(should_remove, DeviceArray):
+            flags = should_remove
+            if flags.count < n:
+                raise ValueError("one flag per node")
+        else:
+            host = np.ascontiguousarray(np.asarray(should_remove).astype(np.uint8))
+            if host.size != n:
+                raise ValueError("one flag per node")
+            flags = DeviceArray.from_host(self.ctx, host if n else np.zeros(1, np.uint8))
+        check(lib().obvhs_bvh2_partial_rebuild(self.ctx.handle, bvh.handle, flags.ptr, int(sd),
+                                               int(search_depth_threshold)), self.ctx.handle)
+        return bvh
+
+
+def _check_u64(sort_precision):
+    if SortPrecision(sort_precision) != SortPrecision.U64:
+        raise ValueError("only SortPrecision::U64 is implemented")
+
+
+def compute_rebuild_path_flags(bvh, leaves):
+    """ploc/rebuild.rs:12-42: a device u8 array, 1 for each given leaf and its ancestors."""
+    ctx = bvh.ctx
+    di, n = _as_device(ctx, leaves if isinstance(leaves, DeviceArray) else np.asarray(leaves, dtype=np.uint32),
+                       np.uint32)
+    flags = DeviceArray(ctx, np.uint8, bvh.node_count)
+    check(lib().obvhs_memset_device(ctx.handle, flags.ptr, 0, flags.nbytes), ctx.handle)
+    check(lib().obvhs_bvh2_rebuild_path_flags(ctx.handle, bvh.handle, di.ptr if n else None, n, flags.ptr),
+          ctx.handle)
+    return flags
 
 
 class CwBvh:

@@ -395,6 +395,79 @@ obvhs_status obvhs_bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh
     return bvh2_refit_from_triangles(ctx, bvh, d_tris);
 }
 
+obvhs_status obvhs_bvh2_create_from_host(obvhs_context* ctx, const obvhs_bvh2_node* nodes, uint32_t node_count,
+                                         const uint32_t* prim_indices, uint32_t prim_count, obvhs_bvh2** out) {
+    if (!ctx || !out || (node_count && !nodes) || (prim_count && !prim_indices)) return OBVHS_ERR_INVALID_ARG;
+    std::vector<Node32> tmp(node_count);
+    for (uint32_t i = 0; i < node_count; ++i) {
+        const obvhs_bvh2_node& a = nodes[i];
+        if (a.prim_count == 0u && (uint64_t)a.first_index + 1u >= node_count) return OBVHS_ERR_INVALID_ARG;
+        if (a.prim_count != 0u && (uint64_t)a.first_index + (a.prim_count & 0x7fffffffu) > prim_count)
+            return OBVHS_ERR_INVALID_ARG;
+        Node32& o = tmp[i];
+        o.minx = a.aabb.min.x;
+        o.miny = a.aabb.min.y;
+        o.minz = a.aabb.min.z;
+        o.maxx = a.aabb.max.x;
+        o.maxy = a.aabb.max.y;
+        o.maxz = a.aabb.max.z;
+        o.prim_count = a.prim_count;
+        o.first_index = a.first_index;
+    }
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    obvhs_bvh2* b = new (std::nothrow) obvhs_bvh2();
+    if (!b) return OBVHS_ERR_OOM;
+    b->node_count = node_count;
+    b->prim_count = prim_count;
+    b->pass_ordered = false;  // node order unknown: bottom-up sweeps use level lists
+    hipError_t e = b->nodes.ensure((size_t)node_count * sizeof(Node32) + 32);
+    if (e == hipSuccess) e = b->prim_indices.ensure((size_t)prim_count * 4 + 4);
+    if (e == hipSuccess && node_count)
+        e = hipMemcpy(b->nodes.ptr, tmp.data(), tmp.size() * sizeof(Node32), hipMemcpyHostToDevice);
+    if (e == hipSuccess && prim_count)
+        e = hipMemcpy(b->prim_indices.ptr, prim_indices, (size_t)prim_count * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        obvhs_bvh2_destroy(b);
+        return set_hip_error(ctx, e, "create bvh2 from host");
+    }
+    *out = b;
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_bvh2_set_node_aabbs(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_node_ids,
+                                       const obvhs_aabb* d_aabbs, uint32_t count) {
+    if (!ctx || !bvh || (count && (!d_node_ids || !d_aabbs))) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    return bvh2_set_node_aabbs(ctx, bvh, d_node_ids, d_aabbs, count);
+}
+
+obvhs_status obvhs_bvh2_primitives_to_nodes(obvhs_context* ctx, const obvhs_bvh2* bvh, uint32_t* d_out) {
+    if (!ctx || !bvh || (bvh->prim_count && !d_out)) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    return bvh2_primitives_to_nodes(ctx, bvh, d_out);
+}
+
+obvhs_status obvhs_bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_leaf_ids,
+                                           uint32_t count, uint8_t* d_flags) {
+    if (!ctx || !bvh || (count && !d_leaf_ids) || (bvh->node_count && !d_flags)) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    return bvh2_rebuild_path_flags(ctx, bvh, d_leaf_ids, count, d_flags);
+}
+
+obvhs_status obvhs_bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,
+                                        uint32_t search_distance, uint32_t search_depth_threshold) {
+    if (!ctx || !bvh || (bvh->node_count && !d_flags)) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    return bvh2_partial_rebuild(ctx, bvh, d_flags, search_distance, search_depth_threshold);
+}
+
+obvhs_status obvhs_bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance,
+                                     uint32_t search_depth_threshold) {
+    if (!ctx || !bvh) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(hipSetDevice(ctx->device), "set device");
+    return bvh2_full_rebuild(ctx, bvh, search_distance, search_depth_threshold);
+}
+
 obvhs_status obvhs_split_aabbs_preset(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
                                       uint64_t max_total, obvhs_aabb* d_out_aabbs, uint32_t* d_out_indices,
                                       uint32_t out_capacity, uint32_t* out_count, float* out_avg_half_area,

@@ -58,6 +58,7 @@ struct QualityScratch {
     DevBuf keys, vals, rkeys, rvals, rfrom, rto, conf, state, stack;  // candidates / reinsertions
     DevBuf sort_scratch;
     DevBuf split_aabbs, split_indices, ha, cand_a, cand_b, halves, flags, sums;  // pre-splits
+    DevBuf marks, kept, freed;                                                   // rebuilds
 };
 
 }  // namespace obvhs
@@ -115,6 +116,8 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identi
                             void* scratch, hipStream_t stream);
 
 // ploc.hip
+obvhs_status ploc_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_src_slots, uint32_t k,
+                          const uint32_t* d_slot_map, uint32_t search_distance, uint32_t search_depth_threshold);
 obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
                         const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
                         uint32_t search_depth_threshold, uint32_t fused_max_prims, obvhs_bvh2* bvh);
@@ -176,12 +179,22 @@ hipError_t launch_traverse_segments(const TraceSegment* segs, uint32_t nseg, uin
                                     const TravScratch& ts, hipStream_t stream);
 
 // quality.hip
-obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32_t>& meta_host, uint32_t& levels);
+obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32_t>& meta_host, uint32_t& levels,
+                         const uint8_t* expand);
 obvhs_status cost_table_levels(obvhs_context* ctx, obvhs_bvh2* bvh, const DpView& dp, uint32_t* d_err);
 obvhs_status bvh2_refit_all(obvhs_context* ctx, obvhs_bvh2* bvh);
 obvhs_status bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh, const obvhs_triangle* d_tris);
 obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_size_ratio, const float* seq,
                               uint32_t seq_len);
+obvhs_status bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_leaves, uint32_t count,
+                                     uint8_t* d_flags);
+obvhs_status bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,
+                                  uint32_t search_distance, uint32_t search_depth_threshold);
+obvhs_status bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance,
+                               uint32_t search_depth_threshold);
+obvhs_status bvh2_set_node_aabbs(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_ids,
+                                 const obvhs_aabb* d_aabbs, uint32_t count);
+obvhs_status bvh2_primitives_to_nodes(obvhs_context* ctx, const obvhs_bvh2* bvh, uint32_t* d_out);
 obvhs_status presplit_leaves(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n, uint64_t max_total,
                              uint32_t& out_n, float* out_avg, float* out_largest);
 

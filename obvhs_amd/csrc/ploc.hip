@@ -16,6 +16,11 @@
 //                    clusters resident in LDS and writes the root (ploc/mod.rs:499)
 // Pass state (cluster count, insert index, depth) lives on the device so passes are enqueued without host syncs;
 // the host only reads the count every few passes to shrink the grids.
+//
+// ploc_rebuild runs the same passes over clusters taken from an existing tree (rebuild_from_leaves,
+// ploc/rebuild.rs:136-183).  For a partial rebuild the children of the g-th merge (counted over all passes in the
+// serial loop's order) go to the g-th freed sibling pair from the top (`slot_map`, the REBUILD placement of
+// ploc/mod.rs:449-462); the plain placement is the special case slot_map[g] = 2n-3-2g.
 #include "common.hpp"
 #include "cwdp.hpp"
 #include "device_util.hpp"
@@ -156,6 +161,33 @@ __global__ void __launch_bounds__(T256) k_leaf_init_aabbs(const float4* __restri
 }
 
 // ploc/mod.rs:287-288 and 782-785; morton.rs:55-58.
+// rebuild_from_leaves takes the root's stored box as the scene box (ploc/rebuild.rs:154).
+__global__ void k_bounds_from_node(Misc* misc, const Node32* __restrict__ nodes) {
+    if (threadIdx.x == 0) {
+        const Node32 r = load_node(&nodes[0]);
+        misc->bounds[0] = f32_order_key(r.minx);
+        misc->bounds[1] = f32_order_key(r.miny);
+        misc->bounds[2] = f32_order_key(r.minz);
+        misc->bounds[3] = f32_order_key(r.maxx);
+        misc->bounds[4] = f32_order_key(r.maxy);
+        misc->bounds[5] = f32_order_key(r.maxz);
+    }
+}
+
+// Child slot of a merge whose plain-placement left slot is v (see the header).  Returns false (and flags the error)
+// when v is outside the map, which a well-formed pass sequence never produces.
+__device__ __forceinline__ bool child_slot(uint32_t v, const uint32_t* __restrict__ slot_map, uint32_t vtop,
+                                           uint32_t map_len, uint32_t& first) {
+    if (!slot_map) {
+        first = v;
+        return true;
+    }
+    const uint32_t g = (vtop - v) >> 1;
+    if (v > vtop || g >= map_len) return false;
+    first = slot_map[g];
+    return true;
+}
+
 __global__ void __launch_bounds__(T256) k_morton(const Node32* __restrict__ leaves, uint32_t n,
                                                  const Misc* __restrict__ misc, uint64_t* __restrict__ keys) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
@@ -350,7 +382,9 @@ __global__ void __launch_bounds__(1024) k_ploc_scan(uint2* __restrict__ blocksum
 __global__ void __launch_bounds__(T256) k_ploc_compact(const Node32* __restrict__ cur, Node32* __restrict__ next,
                                                        const int8_t* __restrict__ merge, Misc* misc, uint32_t slot,
                                                        const uint2* __restrict__ offsets,
-                                                       Node32* __restrict__ bvh_nodes, DpView dp, uint32_t fused) {
+                                                       Node32* __restrict__ bvh_nodes, DpView dp, uint32_t fused,
+                                                       const uint32_t* __restrict__ slot_map, uint32_t vtop,
+                                                       uint32_t map_len) {
     __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
     const PlocState st = misc->state[slot];
     const uint32_t count = st.count;
@@ -377,7 +411,11 @@ __global__ void __launch_bounds__(T256) k_ploc_compact(const Node32* __restrict_
     if (mrg) {
         const uint32_t s = pm + (uint32_t)__popcll(bm & lt);
         const Node32 c = load_node(&cur[b]);
-        const uint32_t first = st.insert - 2u * (s + 1u);
+        uint32_t first;
+        if (!child_slot(st.insert - 2u * (s + 1u), slot_map, vtop, map_len, first)) {
+            atomicOr(&misc->err, DERR_CAPACITY);
+            return;
+        }
         store_node(&bvh_nodes[first], a);
         store_node(&bvh_nodes[first + 1], c);
         const Box u = box_union(node_box(a), node_box(c));
@@ -396,7 +434,9 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
                                                             uint32_t slot, uint32_t first_pass, uint32_t r,
                                                             uint32_t sd_is_one, uint32_t depth_threshold,
                                                             Node32* __restrict__ bvh_nodes,
-                                                            uint32_t* __restrict__ hist, DpView dp, uint32_t fused) {
+                                                            uint32_t* __restrict__ hist, DpView dp, uint32_t fused,
+                                                            const uint32_t* __restrict__ slot_map, uint32_t vtop,
+                                                            uint32_t map_len) {
     __shared__ float lx[TAIL_MAX], ly[TAIL_MAX], lz[TAIL_MAX], hx[TAIL_MAX], hy[TAIL_MAX], hz[TAIL_MAX];
     __shared__ uint32_t pc[TAIL_MAX], fi[TAIL_MAX];
     __shared__ int8_t mg[TAIL_MAX];
@@ -427,6 +467,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
     unsigned long long csum = st.cluster_sum;
     uint32_t pass = first_pass;
     bool self_merge = false;
+    bool bad_slot = false;
     uint32_t dp_err = 0u;
     while (count > 1) {
         csum += count;
@@ -518,11 +559,15 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
             Box outb = va[q];
             uint32_t opc = vpc[q], ofi = vfi[q];
             if (mrg[q]) {
-                const uint32_t first = insert - 2u * (pm + 1u);
-                const Node32 lnode = make_node(va[q], vpc[q], vfi[q]);
-                const Node32 rnode = make_node(vb[q], vbpc[q], vbfi[q]);
-                store_node(&bvh_nodes[first], lnode);
-                store_node(&bvh_nodes[first + 1], rnode);
+                uint32_t first = 0u;
+                if (child_slot(insert - 2u * (pm + 1u), slot_map, vtop, map_len, first)) {
+                    const Node32 lnode = make_node(va[q], vpc[q], vfi[q]);
+                    const Node32 rnode = make_node(vb[q], vbpc[q], vbfi[q]);
+                    store_node(&bvh_nodes[first], lnode);
+                    store_node(&bvh_nodes[first + 1], rnode);
+                } else {
+                    bad_slot = true;
+                }
                 outb = box_union(va[q], vb[q]);
                 if (fused) mlist[pm] = first;
                 opc = 0u;
@@ -560,6 +605,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
         }
     }
     if (self_merge) atomicOr(&misc->err, DERR_SELF_MERGE);
+    if (bad_slot) atomicOr(&misc->err, DERR_CAPACITY);
     if (dp_err) atomicOr(&misc->err, dp_err);
     if (tid == 0) {
         store_node(&bvh_nodes[0], make_node(Box{lx[0], ly[0], lz[0], hx[0], hy[0], hz[0]}, pc[0], fi[0]));
@@ -595,6 +641,61 @@ void DevBuf::release() {
         hipError_t _e = (expr);                                             \
         if (_e != hipSuccess) return set_hip_error(ctx, _e, where);         \
     } while (0)
+
+// The PLOC passes from the sorted clusters in `cur` to the root (ploc/mod.rs:327-499): grid passes while more than
+// TAIL_MAX clusters remain, then the one-workgroup tail.  `fin` receives the final pass state.
+static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uint32_t n, uint32_t r,
+                               uint32_t search_depth_threshold, Node32* bvh_nodes, uint32_t* hist, DpView dp,
+                               uint32_t fused, const uint32_t* slot_map, uint32_t vtop, uint32_t map_len,
+                               PlocState& fin) {
+    hipStream_t st = ctx->stream;
+    BuildScratch& S = ctx->scratch;
+    Misc* misc = S.misc.as<Misc>();
+    int8_t* merge = S.merge.as<int8_t>();
+    uint2* blocksums = S.blocksums.as<uint2>();
+    uint32_t known = n;
+    uint32_t pass = 0;
+    uint32_t slot = 0;
+    PlocState* hstate = static_cast<PlocState*>(ctx->pinned);
+    while (known > TAIL_MAX) {
+        const uint32_t blocks = (known + T256 - 1) / T256;
+        for (int k = 0; k < PASSES_PER_SYNC; ++k) {
+            const uint32_t dist1 = (r == 1 || pass < search_depth_threshold) ? 1u : 0u;
+            k_ploc_search<<<blocks, T256, 0, st>>>(cur, misc, slot, r, dist1, merge);
+            k_ploc_count<<<blocks, T256, 0, st>>>(merge, misc, slot, blocksums);
+            k_ploc_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, slot, hist);
+            k_ploc_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, bvh_nodes, dp, fused,
+                                                   slot_map, vtop, map_len);
+            std::swap(cur, nxt);
+            slot ^= 1u;
+            pass++;
+        }
+        OBVHS_TRY(hipGetLastError(), "ploc pass launch");
+        uint32_t* herr_loop = reinterpret_cast<uint32_t*>(hstate + 1);
+        OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
+                  "read ploc state");
+        OBVHS_TRY(hipMemcpyAsync(herr_loop, &misc->err, 4, hipMemcpyDeviceToHost, st), "read err");
+        OBVHS_TRY(hipStreamSynchronize(st), "ploc sync");
+        if (*herr_loop) return device_error_to_status(*herr_loop);
+        // Every pass merges at least one pair unless costs are NaN; a batch without progress would never end.
+        if (hstate->count >= known) return OBVHS_ERR_PLOC_SELF_MERGE;
+        known = hstate->count;
+        // The device depth counts real passes only; later enqueued passes may have been no-ops.
+        pass = hstate->depth;
+    }
+    k_ploc_tail<<<1, TAIL_THREADS, 0, st>>>(cur, misc, slot, pass, r, r == 1 ? 1u : 0u, search_depth_threshold,
+                                            bvh_nodes, hist, dp, fused, slot_map, vtop, map_len);
+    OBVHS_TRY(hipGetLastError(), "ploc tail launch");
+    uint32_t* herr = reinterpret_cast<uint32_t*>(hstate + 1);
+    OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
+              "read ploc state");
+    OBVHS_TRY(hipMemcpyAsync(herr, &misc->err, 4, hipMemcpyDeviceToHost, st), "read err");
+    OBVHS_TRY(hipStreamSynchronize(st), "ploc final sync");
+    if (*herr) return device_error_to_status(*herr);
+    if (hstate->insert != 1u) return OBVHS_ERR_PLOC_SELF_MERGE;  // ploc/mod.rs:495-497
+    fin = *hstate;
+    return OBVHS_OK;
+}
 
 obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
                         const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
@@ -671,53 +772,69 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
     k_gather<<<nblk, T256, 0, st>>>(leaves, vals, n, cur);
     OBVHS_TRY(hipGetLastError(), "gather launch");
 
-    const uint32_t r = search_distance;
-    int8_t* merge = S.merge.as<int8_t>();
-    uint2* blocksums = S.blocksums.as<uint2>();
-    uint32_t known = n;
-    uint32_t pass = 0;
-    uint32_t slot = 0;
-    PlocState* hstate = static_cast<PlocState*>(ctx->pinned);
-    while (known > TAIL_MAX) {
-        const uint32_t blocks = (known + T256 - 1) / T256;
-        for (int k = 0; k < PASSES_PER_SYNC; ++k) {
-            const uint32_t dist1 = (r == 1 || pass < search_depth_threshold) ? 1u : 0u;
-            k_ploc_search<<<blocks, T256, 0, st>>>(cur, misc, slot, r, dist1, merge);
-            k_ploc_count<<<blocks, T256, 0, st>>>(merge, misc, slot, blocksums);
-            k_ploc_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, slot, hist);
-            k_ploc_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, bvh_nodes, dp, fused);
-            std::swap(cur, nxt);
-            slot ^= 1u;
-            pass++;
-        }
-        OBVHS_TRY(hipGetLastError(), "ploc pass launch");
-        uint32_t* herr_loop = reinterpret_cast<uint32_t*>(hstate + 1);
-        OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
-                  "read ploc state");
-        OBVHS_TRY(hipMemcpyAsync(herr_loop, &misc->err, 4, hipMemcpyDeviceToHost, st), "read err");
-        OBVHS_TRY(hipStreamSynchronize(st), "ploc sync");
-        if (*herr_loop) return device_error_to_status(*herr_loop);
-        // Every pass merges at least one pair unless costs are NaN; a batch without progress would never end.
-        if (hstate->count >= known) return OBVHS_ERR_PLOC_SELF_MERGE;
-        known = hstate->count;
-        // The device depth counts real passes only; later enqueued passes may have been no-ops.
-        pass = hstate->depth;
-    }
-    k_ploc_tail<<<1, TAIL_THREADS, 0, st>>>(cur, misc, slot, pass, r, r == 1 ? 1u : 0u, search_depth_threshold,
-                                            bvh_nodes, hist, dp, fused);
-    OBVHS_TRY(hipGetLastError(), "ploc tail launch");
-    uint32_t* herr = reinterpret_cast<uint32_t*>(hstate + 1);
-    OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
-              "read ploc state");
-    OBVHS_TRY(hipMemcpyAsync(herr, &misc->err, 4, hipMemcpyDeviceToHost, st), "read err");
     if (!d_indices) OBVHS_TRY(launch_iota(bvh->prim_indices.as<uint32_t>(), n, st), "iota");
-    OBVHS_TRY(hipStreamSynchronize(st), "ploc final sync");
-    if (*herr) return device_error_to_status(*herr);
-    if (hstate->insert != 1u) return OBVHS_ERR_PLOC_SELF_MERGE;  // ploc/mod.rs:496
-    bvh->ploc_iterations = hstate->depth;
-    bvh->cluster_sum = hstate->cluster_sum;
-    bvh->max_depth = hstate->depth + 1 > 96 ? hstate->depth + 1 : 96;  // ploc/mod.rs:501
+    PlocState fin{};
+    obvhs_status ps = run_passes(ctx, cur, nxt, n, search_distance, search_depth_threshold, bvh_nodes, hist, dp,
+                                 fused, nullptr, 0u, 0u, fin);
+    if (ps != OBVHS_OK) return ps;
+    bvh->ploc_iterations = fin.depth;
+    bvh->cluster_sum = fin.cluster_sum;
+    bvh->max_depth = fin.depth + 1 > 96 ? fin.depth + 1 : 96;  // ploc/mod.rs:501
     bvh->dp_max_prims = fused_max_prims;
+    return OBVHS_OK;
+}
+
+// rebuild_from_leaves (ploc/rebuild.rs:136-183) over the nodes at d_src_slots[0..k) of bvh->nodes, listed in slot
+// order (the order that breaks Morton-code ties).  slot_map = NULL: full rebuild, the tree becomes 2k-1 nodes with
+// the plain placement (children after parents).  Otherwise slot_map[g] (k-1 entries, descending) is the left slot
+// of the g-th freed sibling pair; every other slot keeps its node.  The scene box is the root's stored box.
+obvhs_status ploc_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_src_slots, uint32_t k,
+                          const uint32_t* d_slot_map, uint32_t search_distance, uint32_t search_depth_threshold) {
+    if (!(search_distance == 1 || search_distance == 2 || search_distance == 6 || search_distance == 14 ||
+          search_distance == 24 || search_distance == 32))
+        return OBVHS_ERR_INVALID_ARG;
+    if (k == 0) return OBVHS_OK;
+    if (!d_slot_map && 2ull * k - 1ull > bvh->node_count) return OBVHS_ERR_INVALID_ARG;
+    hipStream_t st = ctx->stream;
+    BuildScratch& S = ctx->scratch;
+    OBVHS_TRY(bvh->hist.ensure((size_t)(k + 1) * 4), "alloc pass history");
+    OBVHS_TRY(S.leaves.ensure((size_t)k * sizeof(Node32)), "alloc leaves");
+    OBVHS_TRY(S.clusters_a.ensure((size_t)k * sizeof(Node32)), "alloc clusters");
+    OBVHS_TRY(S.clusters_b.ensure((size_t)k * sizeof(Node32)), "alloc clusters");
+    OBVHS_TRY(S.keys.ensure((size_t)k * 8), "alloc keys");
+    OBVHS_TRY(S.vals.ensure((size_t)k * 4), "alloc vals");
+    OBVHS_TRY(S.merge.ensure((size_t)k + 64), "alloc merge");
+    const uint32_t nblk = (k + T256 - 1) / T256;
+    OBVHS_TRY(S.blocksums.ensure((size_t)nblk * sizeof(uint2) + 64 + (size_t)nblk * 6 * 4), "alloc blocksums");
+    OBVHS_TRY(S.sort_scratch.ensure(radix_sort_scratch_bytes(k)), "alloc sort scratch");
+    OBVHS_TRY(S.misc.ensure(sizeof(Misc)), "alloc misc");
+    Misc* misc = S.misc.as<Misc>();
+    Node32* leaves = S.leaves.as<Node32>();
+    Node32* bvh_nodes = bvh->nodes.as<Node32>();
+    uint64_t* keys = S.keys.as<uint64_t>();
+    uint32_t* vals = S.vals.as<uint32_t>();
+    uint32_t* hist = bvh->hist.as<uint32_t>();
+    // The clusters are copied out first: merges may overwrite the freed slots they came from.
+    k_gather<<<nblk, T256, 0, st>>>(bvh_nodes, d_src_slots, k, leaves);
+    k_init_misc<<<1, 64, 0, st>>>(misc, k, hist);
+    k_bounds_from_node<<<1, 64, 0, st>>>(misc, bvh_nodes);
+    k_morton<<<nblk, T256, 0, st>>>(leaves, k, misc, keys);
+    OBVHS_TRY(hipGetLastError(), "rebuild morton launch");
+    OBVHS_TRY(radix_sort_pairs(keys, vals, true, k, 63, S.sort_scratch.ptr, st), "radix sort");
+    Node32* cur = S.clusters_a.as<Node32>();
+    Node32* nxt = S.clusters_b.as<Node32>();
+    k_gather<<<nblk, T256, 0, st>>>(leaves, vals, k, cur);
+    OBVHS_TRY(hipGetLastError(), "gather launch");
+    PlocState fin{};
+    obvhs_status ps = run_passes(ctx, cur, nxt, k, search_distance, search_depth_threshold, bvh_nodes, hist,
+                                 DpView{}, 0u, d_slot_map, 2u * k - 3u, k - 1u, fin);
+    if (ps != OBVHS_OK) return ps;
+    if (!d_slot_map) bvh->node_count = 2u * k - 1u;  // ploc/mod.rs:283-284 (nodes.resize)
+    bvh->pass_ordered = d_slot_map == nullptr;    // ploc/mod.rs:502
+    bvh->ploc_iterations = fin.depth;
+    bvh->cluster_sum = fin.cluster_sum;
+    bvh->max_depth = fin.depth + 1 > 96 ? fin.depth + 1 : 96;  // ploc/mod.rs:501
+    bvh->dp_max_prims = 0;                                      // boxes changed: any cost table is stale
     return OBVHS_OK;
 }
 

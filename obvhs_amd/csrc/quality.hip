@@ -59,9 +59,18 @@ __global__ void __launch_bounds__(T256) k_levels_init(uint32_t* __restrict__ met
 // reserves its output with one atomic (the order inside a level does not matter).  A node whose children would fall
 // outside the node array, or more than n_nodes entries in total (not a tree), sets the error word meta[err_word]
 // instead of writing.
+// Children of an inner node sit at (first, first + 1) inside the array.  The partial rebuild's slot reuse also needs
+// left children at odd slots (Bvh2Node::is_left_sibling, bvh2/node.rs:154-156), as every builder lays them out.
+__device__ __forceinline__ bool child_pair_ok(uint32_t first, uint32_t n_nodes, bool odd_left) {
+    return first != 0u && first < n_nodes - 1u && (!odd_left || (first & 1u) != 0u);
+}
+
+// With `expand` set, only the root and the inner nodes x with expand[x] != 0 are opened (the top-down walk of
+// PlocBuilder::partial_rebuild, ploc/rebuild.rs:113-126); the lists then hold the slots that walk visits.
 __global__ void __launch_bounds__(T256) k_levels_expand(const Node32* __restrict__ nodes, uint32_t n_nodes,
                                                         uint32_t* __restrict__ list, uint32_t* __restrict__ meta,
-                                                        uint32_t level, uint32_t err_word) {
+                                                        uint32_t level, uint32_t err_word,
+                                                        const uint8_t* __restrict__ expand) {
     __shared__ uint32_t wcount[T256 / 64];
     __shared__ uint32_t block_base;
     const uint32_t s = meta[2 * level], c = meta[2 * level + 1];
@@ -75,9 +84,10 @@ __global__ void __launch_bounds__(T256) k_levels_expand(const Node32* __restrict
     // pass 1: count the block's internal nodes
     uint32_t mine = 0;
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += T256) {
-        const Node32 nd = load_node(&nodes[list[s + i]]);
-        if (nd.prim_count == 0u) {
-            if (nd.first_index == 0u || nd.first_index >= n_nodes - 1u) atomicOr(&meta[err_word], 1u);
+        const uint32_t x = list[s + i];
+        const Node32 nd = load_node(&nodes[x]);
+        if (nd.prim_count == 0u && (!expand || x == 0u || expand[x] != 0u)) {
+            if (!child_pair_ok(nd.first_index, n_nodes, expand != nullptr)) atomicOr(&meta[err_word], 1u);
             else mine++;
         }
     }
@@ -99,9 +109,11 @@ __global__ void __launch_bounds__(T256) k_levels_expand(const Node32* __restrict
         bool inner = false;
         uint32_t first = 0u;
         if (i < b1) {
-            const Node32 nd = load_node(&nodes[list[s + i]]);
+            const uint32_t x = list[s + i];
+            const Node32 nd = load_node(&nodes[x]);
             first = nd.first_index;
-            inner = nd.prim_count == 0u && first != 0u && first < n_nodes - 1u;
+            inner = nd.prim_count == 0u && (!expand || x == 0u || expand[x] != 0u) &&
+                    child_pair_ok(first, n_nodes, expand != nullptr);
         }
         const uint64_t m = __ballot(inner);
         __syncthreads();
@@ -185,7 +197,7 @@ __global__ void __launch_bounds__(T256) k_parents(const Node32* __restrict__ nod
     if (i == 0) parents[0] = 0u;
     if (i >= n) return;
     const Node32 nd = load_node(&nodes[i]);
-    if (nd.prim_count == 0u) {
+    if (nd.prim_count == 0u && nd.first_index + 1u < n) {
         parents[nd.first_index] = i;
         parents[nd.first_index + 1] = i;
     }
@@ -744,6 +756,162 @@ __global__ void __launch_bounds__(T256) k_split_keep_write(const uint32_t* __res
     if (f) cand[r] = ext[k];
 }
 
+
+// ---- partial / full rebuild (ploc/rebuild.rs) ----
+
+// compute_rebuild_path_flags (ploc/rebuild.rs:12-42), one lane per given leaf: flag it and walk up, stopping at the
+// first ancestor that is already flagged.  The lane that set a flag always carries on upwards from it, so every
+// ancestor ends up flagged whatever order the lanes run in (the result is the set of ancestors-or-self).
+__global__ void __launch_bounds__(T256) k_path_flags(const uint32_t* __restrict__ parents, uint32_t n_nodes,
+                                                     const uint32_t* __restrict__ leaves, uint32_t count,
+                                                     uint8_t* __restrict__ flags, uint32_t* __restrict__ err) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= count) return;
+    uint32_t index = leaves[i];
+    if (index >= n_nodes) {
+        atomicOr(err, 1u);  // the reference indexes out of bounds here
+        return;
+    }
+    flags[index] = 1u;
+    for (uint32_t steps = 0; index > 0u; ++steps) {
+        index = parents[index];
+        if (index >= n_nodes || steps > n_nodes) {
+            atomicOr(err, 2u);  // not a tree
+            return;
+        }
+        if (flags[index] != 0u) break;
+        flags[index] = 1u;
+    }
+}
+
+// Slot marks of a rebuild: bit 0 = the slot is freed (visited by the walk), bit 1 = its node becomes a cluster.
+// Partial: the visited slots are level-list entries 1..count (the root stays); kept = a leaf or not flagged
+// (ploc/rebuild.rs:118-123).
+__global__ void __launch_bounds__(T256) k_rb_mark_partial(const Node32* __restrict__ nodes,
+                                                          const uint32_t* __restrict__ list, uint32_t count,
+                                                          const uint8_t* __restrict__ flags,
+                                                          uint8_t* __restrict__ marks) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t x = list[1u + i];
+    const Node32 nd = load_node(&nodes[x]);
+    const bool kept = nd.prim_count != 0u || flags[x] == 0u;
+    marks[x] = kept ? 3u : 1u;
+}
+// Full: every leaf (Bvh2Node::is_leaf, ploc/rebuild.rs:67-71) is a cluster.
+__global__ void __launch_bounds__(T256) k_rb_mark_full(const Node32* __restrict__ nodes, uint32_t n,
+                                                       uint8_t* __restrict__ marks) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= n) return;
+    marks[i] = load_node(&nodes[i]).prim_count != 0u ? 2u : 0u;
+}
+
+// Two order-preserving selections over the slots in one pass: clusters (ascending slot) and freed left slots (odd
+// slots, listed in descending order).  sums[b] / sums[nblk + b] are the per-block counts, then their prefixes.
+__device__ __forceinline__ void rb_flags(const uint8_t* marks, uint32_t i, uint32_t n, bool& kept, bool& freed) {
+    const uint32_t m = i < n ? marks[i] : 0u;
+    kept = (m & 2u) != 0u;
+    freed = (m & 1u) != 0u && (i & 1u) != 0u;
+}
+__global__ void __launch_bounds__(T256) k_rb_count(const uint8_t* __restrict__ marks, uint32_t n,
+                                                   uint32_t* __restrict__ sums, uint32_t nblk) {
+    __shared__ uint32_t wk[T256 / 64], wf[T256 / 64];
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    bool kept, freed;
+    rb_flags(marks, i, n, kept, freed);
+    const uint32_t ck = (uint32_t)__popcll(__ballot(kept)), cf = (uint32_t)__popcll(__ballot(freed));
+    if ((threadIdx.x & 63u) == 0u) {
+        wk[threadIdx.x >> 6] = ck;
+        wf[threadIdx.x >> 6] = cf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t a = 0, b = 0;
+        for (int w = 0; w < T256 / 64; ++w) {
+            a += wk[w];
+            b += wf[w];
+        }
+        sums[blockIdx.x] = a;
+        sums[nblk + blockIdx.x] = b;
+    }
+}
+__global__ void __launch_bounds__(T256) k_rb_write(const uint8_t* __restrict__ marks, uint32_t n,
+                                                   const uint32_t* __restrict__ sums, uint32_t nblk,
+                                                   const uint32_t* __restrict__ totals,
+                                                   uint32_t* __restrict__ kept_list, uint32_t* __restrict__ free_desc) {
+    __shared__ uint32_t wk[T256 / 64], wf[T256 / 64];
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    bool kept, freed;
+    rb_flags(marks, i, n, kept, freed);
+    const uint64_t mk = __ballot(kept), mf = __ballot(freed);
+    if ((threadIdx.x & 63u) == 0u) {
+        wk[threadIdx.x >> 6] = (uint32_t)__popcll(mk);
+        wf[threadIdx.x >> 6] = (uint32_t)__popcll(mf);
+    }
+    __syncthreads();
+    uint32_t rk = sums[blockIdx.x], rf = sums[nblk + blockIdx.x];
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) {
+        rk += wk[w];
+        rf += wf[w];
+    }
+    rk += (uint32_t)__popcll(mk & lanemask_lt());
+    rf += (uint32_t)__popcll(mf & lanemask_lt());
+    if (kept) kept_list[rk] = i;
+    if (freed) free_desc[totals[1] - 1u - rf] = i;
+}
+
+// Bvh2Node::set_aabb (bvh2/node.rs:116-125) for a list of nodes.
+__global__ void __launch_bounds__(T256) k_set_node_aabbs(Node32* __restrict__ nodes, uint32_t n_nodes,
+                                                         const uint32_t* __restrict__ ids,
+                                                         const float4* __restrict__ aabbs, uint32_t count,
+                                                         uint32_t* __restrict__ err) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t id = ids[i];
+    if (id >= n_nodes) {
+        atomicOr(err, 1u);
+        return;
+    }
+    const float4 lo = aabbs[2 * (size_t)i], hi = aabbs[2 * (size_t)i + 1];
+    Node32 nd = load_node(&nodes[id]);
+    nd.minx = lo.x;
+    nd.miny = lo.y;
+    nd.minz = lo.z;
+    nd.maxx = hi.x;
+    nd.maxy = hi.y;
+    nd.maxz = hi.z;
+    store_node(&nodes[id], nd);
+}
+
+// compute_primitives_to_nodes (bvh2/mod.rs:647-663): out[prim] = slot of the leaf holding it.  The reference's
+// serial loop leaves the highest such slot when a primitive sits in several leaves (pre-splits): atomicMax.
+__global__ void __launch_bounds__(T256) k_prims_to_nodes(const Node32* __restrict__ nodes, uint32_t n_nodes,
+                                                         const uint32_t* __restrict__ prim_indices,
+                                                         uint32_t n_prims, uint32_t* __restrict__ out,
+                                                         uint32_t* __restrict__ err) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= n_nodes) return;
+    const Node32 nd = load_node(&nodes[i]);
+    if (nd.prim_count == 0u) return;
+    const uint64_t end = (uint64_t)nd.first_index + nd.prim_count;
+    if (end > n_prims) {
+        atomicOr(err, 1u);
+        return;
+    }
+    for (uint32_t k = nd.first_index; k < (uint32_t)end; ++k) {
+        const uint32_t prim = prim_indices[k];
+        if (prim >= n_prims) {
+            atomicOr(err, 1u);
+            continue;
+        }
+        atomicMax(&out[prim], i + 1u);  // 0 = no leaf yet
+    }
+}
+__global__ void __launch_bounds__(T256) k_p2n_finish(uint32_t* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i < n) out[i] = out[i] == 0u ? 0xffffffffu : out[i] - 1u;  // INVALID where no leaf holds it
+}
+
 }  // namespace
 
 #define OBVHS_TRY(expr, where)                                      \
@@ -754,7 +922,9 @@ __global__ void __launch_bounds__(T256) k_split_keep_write(const uint32_t* __res
 
 // Top-down level lists of the BVH2 in bvh->nodes.  On return meta_host[2L], meta_host[2L+1] = start, count of
 // level L for L < levels.
-obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32_t>& meta_host, uint32_t& levels) {
+// With `expand` set, the lists hold what the partial rebuild's top-down walk visits (see k_levels_expand).
+obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32_t>& meta_host, uint32_t& levels,
+                         const uint8_t* expand) {
     hipStream_t st = ctx->stream;
     const uint32_t N = bvh->node_count;
     levels = 0;
@@ -774,7 +944,7 @@ obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32
         for (uint32_t q = 0; q < LEVELS_PER_SYNC && level + q < MAX_TREE_LEVELS; ++q) {
             const uint64_t bound = (level + q) < 31 ? std::min<uint64_t>(1ull << (level + q), N) : N;
             const uint32_t g = (uint32_t)std::min<uint64_t>((bound + T256 - 1) / T256, 2048);
-            k_levels_expand<<<g, T256, 0, st>>>(nodes, N, list, meta, level + q, words - 1);
+            k_levels_expand<<<g, T256, 0, st>>>(nodes, N, list, meta, level + q, words - 1, expand);
         }
         OBVHS_TRY(hipGetLastError(), "levels launch");
         level = std::min(level + LEVELS_PER_SYNC, MAX_TREE_LEVELS);
@@ -794,7 +964,7 @@ obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32
         total += meta_host[2 * l + 1];
         levels = l + 1;
     }
-    if (total != N) return OBVHS_ERR_CAPACITY;  // not a tree: some node unreachable or reached twice
+    if (!expand && total != N) return OBVHS_ERR_CAPACITY;  // not a tree: some node unreachable or reached twice
     return OBVHS_OK;
 }
 
@@ -815,7 +985,7 @@ static obvhs_status refit_levels(obvhs_context* ctx, obvhs_bvh2* bvh, const std:
 obvhs_status cost_table_levels(obvhs_context* ctx, obvhs_bvh2* bvh, const DpView& dp, uint32_t* d_err) {
     std::vector<uint32_t> meta;
     uint32_t levels = 0;
-    obvhs_status s = tree_levels(ctx, bvh, meta, levels);
+    obvhs_status s = tree_levels(ctx, bvh, meta, levels, nullptr);
     if (s != OBVHS_OK) return s;
     hipStream_t st = ctx->stream;
     const uint32_t* list = ctx->quality.level_list.as<uint32_t>();
@@ -831,7 +1001,7 @@ obvhs_status cost_table_levels(obvhs_context* ctx, obvhs_bvh2* bvh, const DpView
 obvhs_status bvh2_refit_all(obvhs_context* ctx, obvhs_bvh2* bvh) {
     std::vector<uint32_t> meta;
     uint32_t levels = 0;
-    obvhs_status s = tree_levels(ctx, bvh, meta, levels);
+    obvhs_status s = tree_levels(ctx, bvh, meta, levels, nullptr);
     if (s != OBVHS_OK) return s;
     s = refit_levels(ctx, bvh, meta, levels);
     bvh->dp_max_prims = 0;  // boxes changed: the cost table is stale
@@ -854,6 +1024,143 @@ obvhs_status bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh, cons
     OBVHS_TRY(hipStreamSynchronize(st), "leaf refit sync");
     if (h[0]) return device_error_to_status(h[0]);
     return bvh2_refit_all(ctx, bvh);
+}
+
+// ---- partial / full rebuild (ploc/rebuild.rs) ----
+
+obvhs_status bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_leaves, uint32_t count,
+                                     uint8_t* d_flags) {
+    const uint32_t N = bvh->node_count;
+    if (N < 2) return OBVHS_OK;  // rebuild.rs:17-19
+    hipStream_t st = ctx->stream;
+    QualityScratch& Q = ctx->quality;
+    OBVHS_TRY(Q.parents.ensure((size_t)N * 4), "alloc parents");
+    OBVHS_TRY(Q.misc.ensure(4096), "alloc quality misc");
+    uint32_t* err = Q.misc.as<uint32_t>();
+    uint32_t* parents = Q.parents.as<uint32_t>();
+    k_fill_u32<<<1, 64, 0, st>>>(err, 0u, 1);
+    OBVHS_TRY(hipMemsetAsync(d_flags, 0, N, st), "clear flags");
+    k_parents<<<(N + T256 - 1) / T256, T256, 0, st>>>(bvh->nodes.as<Node32>(), N, parents);
+    if (count) k_path_flags<<<(count + T256 - 1) / T256, T256, 0, st>>>(parents, N, d_leaves, count, d_flags, err);
+    OBVHS_TRY(hipGetLastError(), "path flags launch");
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    OBVHS_TRY(hipMemcpyAsync(h, err, 4, hipMemcpyDeviceToHost, st), "read path flags err");
+    OBVHS_TRY(hipStreamSynchronize(st), "path flags sync");
+    return h[0] ? OBVHS_ERR_INVALID_ARG : OBVHS_OK;
+}
+
+// Cluster list (ascending slot) and freed left slots (descending) from the slot marks; returns their sizes.
+static obvhs_status rebuild_collect(obvhs_context* ctx, uint32_t N, uint32_t& kept, uint32_t& freed) {
+    hipStream_t st = ctx->stream;
+    QualityScratch& Q = ctx->quality;
+    const uint32_t nblk = (N + T256 - 1) / T256;
+    OBVHS_TRY(Q.sums.ensure((size_t)nblk * 8 + 16), "alloc rebuild sums");
+    OBVHS_TRY(Q.misc.ensure(4096), "alloc quality misc");
+    uint32_t* sums = Q.sums.as<uint32_t>();
+    uint32_t* totals = Q.misc.as<uint32_t>() + 8;
+    const uint8_t* marks = Q.marks.as<uint8_t>();
+    k_rb_count<<<nblk, T256, 0, st>>>(marks, N, sums, nblk);
+    k_scan_blocks<<<1, 1024, 0, st>>>(sums, nblk, totals);
+    k_scan_blocks<<<1, 1024, 0, st>>>(sums + nblk, nblk, totals + 1);
+    OBVHS_TRY(hipGetLastError(), "rebuild count launch");
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    OBVHS_TRY(hipMemcpyAsync(h, totals, 8, hipMemcpyDeviceToHost, st), "read rebuild counts");
+    OBVHS_TRY(hipStreamSynchronize(st), "rebuild count sync");
+    kept = h[0];
+    freed = h[1];
+    OBVHS_TRY(Q.kept.ensure((size_t)kept * 4 + 4), "alloc cluster list");
+    OBVHS_TRY(Q.freed.ensure((size_t)freed * 4 + 4), "alloc free list");
+    k_rb_write<<<nblk, T256, 0, st>>>(marks, N, sums, nblk, totals, Q.kept.as<uint32_t>(), Q.freed.as<uint32_t>());
+    OBVHS_TRY(hipGetLastError(), "rebuild write launch");
+    return OBVHS_OK;
+}
+
+static bool valid_search_distance(uint32_t r) {
+    return r == 1 || r == 2 || r == 6 || r == 14 || r == 24 || r == 32;  // ploc/mod.rs:550-561
+}
+
+// PlocBuilder::partial_rebuild (ploc/rebuild.rs:98-134) with should_remove(i) = d_flags[i] != 0.
+obvhs_status bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,
+                                  uint32_t search_distance, uint32_t search_depth_threshold) {
+    if (!valid_search_distance(search_distance)) return OBVHS_ERR_INVALID_ARG;
+    const uint32_t N = bvh->node_count;
+    if (N < 2) return OBVHS_OK;  // rebuild.rs:106-108
+    std::vector<uint32_t> meta;
+    uint32_t levels = 0;
+    obvhs_status s = tree_levels(ctx, bvh, meta, levels, d_flags);
+    if (s == OBVHS_ERR_CAPACITY) return OBVHS_ERR_INVALID_ARG;  // not a tree, or a left child at an even slot
+    if (s != OBVHS_OK) return s;
+    uint64_t total = 0;
+    for (uint32_t l = 0; l < levels; ++l) total += meta[2 * l + 1];
+    const uint32_t visited = (uint32_t)(total - 1);  // the root is not visited
+    hipStream_t st = ctx->stream;
+    QualityScratch& Q = ctx->quality;
+    OBVHS_TRY(Q.marks.ensure(N), "alloc slot marks");
+    OBVHS_TRY(hipMemsetAsync(Q.marks.ptr, 0, N, st), "clear slot marks");
+    if (visited)
+        k_rb_mark_partial<<<(visited + T256 - 1) / T256, T256, 0, st>>>(
+            bvh->nodes.as<Node32>(), Q.level_list.as<uint32_t>(), visited, d_flags, Q.marks.as<uint8_t>());
+    OBVHS_TRY(hipGetLastError(), "rebuild mark launch");
+    uint32_t k = 0, f = 0;
+    s = rebuild_collect(ctx, N, k, f);
+    if (s != OBVHS_OK) return s;
+    // The visited slots form whole sibling pairs at (odd, odd + 1), one pair per merge to come.
+    if (k < 2 || visited != 2 * k - 2 || f != k - 1) return OBVHS_ERR_INVALID_ARG;
+    return ploc_rebuild(ctx, bvh, Q.kept.as<uint32_t>(), k, Q.freed.as<uint32_t>(), search_distance,
+                        search_depth_threshold);
+}
+
+// PlocBuilder::full_rebuild (ploc/rebuild.rs:53-79).
+obvhs_status bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance,
+                               uint32_t search_depth_threshold) {
+    if (!valid_search_distance(search_distance)) return OBVHS_ERR_INVALID_ARG;
+    const uint32_t N = bvh->node_count;
+    if (N < 2) return OBVHS_OK;  // rebuild.rs:60-62
+    hipStream_t st = ctx->stream;
+    QualityScratch& Q = ctx->quality;
+    OBVHS_TRY(Q.marks.ensure(N), "alloc slot marks");
+    k_rb_mark_full<<<(N + T256 - 1) / T256, T256, 0, st>>>(bvh->nodes.as<Node32>(), N, Q.marks.as<uint8_t>());
+    OBVHS_TRY(hipGetLastError(), "rebuild mark launch");
+    uint32_t k = 0, f = 0;
+    obvhs_status s = rebuild_collect(ctx, N, k, f);
+    if (s != OBVHS_OK) return s;
+    return ploc_rebuild(ctx, bvh, Q.kept.as<uint32_t>(), k, nullptr, search_distance, search_depth_threshold);
+}
+
+obvhs_status bvh2_set_node_aabbs(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_ids,
+                                 const obvhs_aabb* d_aabbs, uint32_t count) {
+    if (count == 0) return OBVHS_OK;
+    hipStream_t st = ctx->stream;
+    OBVHS_TRY(ctx->quality.misc.ensure(4096), "alloc quality misc");
+    uint32_t* err = ctx->quality.misc.as<uint32_t>();
+    k_fill_u32<<<1, 64, 0, st>>>(err, 0u, 1);
+    k_set_node_aabbs<<<(count + T256 - 1) / T256, T256, 0, st>>>(
+        bvh->nodes.as<Node32>(), bvh->node_count, d_ids, reinterpret_cast<const float4*>(d_aabbs), count, err);
+    OBVHS_TRY(hipGetLastError(), "set aabbs launch");
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    OBVHS_TRY(hipMemcpyAsync(h, err, 4, hipMemcpyDeviceToHost, st), "read set aabbs err");
+    OBVHS_TRY(hipStreamSynchronize(st), "set aabbs sync");
+    bvh->dp_max_prims = 0;
+    return h[0] ? OBVHS_ERR_INVALID_ARG : OBVHS_OK;
+}
+
+obvhs_status bvh2_primitives_to_nodes(obvhs_context* ctx, const obvhs_bvh2* bvh, uint32_t* d_out) {
+    const uint32_t P = bvh->prim_count, N = bvh->node_count;
+    if (P == 0) return OBVHS_OK;
+    hipStream_t st = ctx->stream;
+    OBVHS_TRY(ctx->quality.misc.ensure(4096), "alloc quality misc");
+    uint32_t* err = ctx->quality.misc.as<uint32_t>();
+    k_fill_u32<<<1, 64, 0, st>>>(err, 0u, 1);
+    k_fill_u32<<<std::min((P + T256 - 1) / T256, 4096u), T256, 0, st>>>(d_out, 0u, P);
+    if (N)
+        k_prims_to_nodes<<<(N + T256 - 1) / T256, T256, 0, st>>>(bvh->nodes.as<Node32>(), N,
+                                                                 bvh->prim_indices.as<uint32_t>(), P, d_out, err);
+    k_p2n_finish<<<(P + T256 - 1) / T256, T256, 0, st>>>(d_out, P);
+    OBVHS_TRY(hipGetLastError(), "primitives to nodes launch");
+    uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
+    OBVHS_TRY(hipMemcpyAsync(h, err, 4, hipMemcpyDeviceToHost, st), "read primitives to nodes err");
+    OBVHS_TRY(hipStreamSynchronize(st), "primitives to nodes sync");
+    return h[0] ? OBVHS_ERR_INVALID_ARG : OBVHS_OK;
 }
 
 // ReinsertionOptimizer::run (reinsertion.rs:40-61, 101-119).
@@ -893,7 +1200,7 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
 
     std::vector<uint32_t> meta;
     uint32_t levels = 0;
-    obvhs_status s = tree_levels(ctx, bvh, meta, levels);
+    obvhs_status s = tree_levels(ctx, bvh, meta, levels, nullptr);
     if (s != OBVHS_OK) return s;
     for (uint32_t si = 0; si < seq_len; ++si) {
         // reinsertion.rs:112-117 in f32, `as usize` saturating
@@ -963,7 +1270,7 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
         }
         k_reins_apply<<<pblk, T256, 0, st>>>(Q.conf.as<uint32_t>(), Q.state.as<uint8_t>(), misc, nodes);
         OBVHS_TRY(hipGetLastError(), "apply launch");
-        s = tree_levels(ctx, bvh, meta, levels);
+        s = tree_levels(ctx, bvh, meta, levels, nullptr);
         if (s != OBVHS_OK) return s;
         s = refit_levels(ctx, bvh, meta, levels);
         if (s != OBVHS_OK) return s;
