@@ -206,6 +206,71 @@ def bench_batch(args, ob, ctx):
             "cwbvh_nodes": info["total_nodes"], "hit_fraction": round(float(np.isfinite(batch_hits["t"]).mean()), 4)}
 
 
+def bench_rebuild(args, ob, ctx, tris, d_tris, fresh_build_ms):
+    """Partial rebuild as examples/physics.rs:426-448 runs it, on the heightfield scene: a fraction of the triangles
+    moves, their leaves take the new boxes (Bvh2Node::set_aabb), the paths to the root are flagged
+    (compute_rebuild_path_flags), the flagged top is rebuilt with PLOC (partial_rebuild) and the CWBVH re-emitted.
+    Compared with full_rebuild + re-emit of the same BVH2 and with a fresh build_cwbvh_from_tris."""
+    import ctypes as C
+    from obvhs_amd import scenes
+    from obvhs_amd.test_util import hash_noise
+    lib = ob.native.lib()
+    n = len(tris)
+    moved_ids = np.nonzero(hash_noise(np.arange(n, dtype=np.uint32), 5, 21) < args.rebuild_frac)[0]
+    moved = scenes.jitter_triangles(tris[moved_ids], seed=8, frac=0.002)
+    verts = np.stack([moved["v0"][:, :3], moved["v1"][:, :3], moved["v2"][:, :3]])
+    boxes = np.zeros(len(moved_ids), ob.AABB)
+    boxes["min"][:, :3] = verts.min(axis=0)
+    boxes["max"][:, :3] = verts.max(axis=0)
+    sd = 14
+    base = ob.PlocBuilder(ctx).build(sd, d_tris)
+    base_nodes, prims = base.nodes, base.primitive_indices
+    leaf_of = base.primitives_to_nodes().to_host()
+    d_ids = ctx.upload(leaf_of[moved_ids].astype(np.uint32))
+    d_boxes = ctx.upload(boxes)
+    flags = ob.DeviceArray(ctx, np.uint8, len(base_nodes))
+    k = len(moved_ids)
+
+    def fresh():
+        b = ob.Bvh2.from_host(ctx, base_nodes, prims)
+        ctx.synchronize()
+        return b
+
+    part_ms, part_emit_ms, full_emit_ms = [], [], []
+    for _ in range(max(2, args.build_reps)):
+        b = fresh()
+        t0 = time.perf_counter()
+        ob.native.check(lib.obvhs_bvh2_set_node_aabbs(ctx.handle, b.handle, d_ids.ptr, d_boxes.ptr, k), ctx.handle)
+        ob.native.check(lib.obvhs_bvh2_rebuild_path_flags(ctx.handle, b.handle, d_ids.ptr, k, flags.ptr), ctx.handle)
+        ob.native.check(lib.obvhs_bvh2_partial_rebuild(ctx.handle, b.handle, flags.ptr, sd, 0), ctx.handle)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        cw = ob.bvh2_to_cwbvh(b, 3, True, False)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        part_ms.append((t1 - t0) * 1e3)
+        part_emit_ms.append((t2 - t0) * 1e3)
+        cw.close()
+        b.close()
+        b = fresh()
+        t0 = time.perf_counter()
+        ob.native.check(lib.obvhs_bvh2_set_node_aabbs(ctx.handle, b.handle, d_ids.ptr, d_boxes.ptr, k), ctx.handle)
+        ob.native.check(lib.obvhs_bvh2_full_rebuild(ctx.handle, b.handle, sd, 0), ctx.handle)
+        cw = ob.bvh2_to_cwbvh(b, 3, True, False)
+        ctx.synchronize()
+        full_emit_ms.append((time.perf_counter() - t0) * 1e3)
+        cw.close()
+        b.close()
+    flagged = int(flags.to_host().astype(np.int64).sum())
+    med = lambda v: round(float(np.median(v)), 3)
+    return {"workload": f"heightfield {n} tris, PLOC r={sd}; {k} triangles ({args.rebuild_frac:.1%}) moved, their "
+                        f"leaves re-boxed, {flagged} of {len(base_nodes)} nodes flagged",
+            "partial_rebuild_ms": med(part_ms), "partial_rebuild_and_emit_ms": med(part_emit_ms),
+            "full_rebuild_and_emit_ms": med(full_emit_ms), "fresh_build_ms": round(fresh_build_ms, 3),
+            "timing": "host wall clock per call sequence, synchronized (includes the host reads of level and "
+                      "cluster counts)"}
+
+
 def bench_large(args, ob, ctx):
     """Large scene with the quality passes: a 2896 x 2896 heightfield (16,773,632 triangles) plus 500,000 long thin
     triangles; spatial pre-splits capped at 1.3x the triangle count and reinsertion at ratio 0.05 with the ratio
@@ -431,6 +496,7 @@ def main():
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 20)
     ap.add_argument("--batch-meshes", type=int, default=4096, help="0 skips the batched leg")
     ap.add_argument("--batch-rays-per-mesh", type=int, default=1024)
+    ap.add_argument("--rebuild-frac", type=float, default=0.01, help="0 skips the partial-rebuild leg")
     ap.add_argument("--large-grid", type=int, default=2896, help="0 skips the large-scene leg")
     ap.add_argument("--large-thin", type=int, default=500000)
     ap.add_argument("--large-rays", type=int, default=1 << 22)
@@ -562,6 +628,8 @@ def main():
                   "all_ms": [round(x, 3) for x in build_ms], "ploc_passes": int(stats.ploc_iterations),
                   "ploc_cluster_sum": int(stats.ploc_cluster_sum), "cwbvh_nodes": int(stats.cwbvh_node_count)},
     }
+    if args.rebuild_frac > 0:
+        out["rebuild"] = bench_rebuild(args, ob, ctx, tris, d_tris, build_med)
     if args.batch_meshes > 0:
         out["batched"] = bench_batch(args, ob, ctx)
     if args.large_grid > 0:
