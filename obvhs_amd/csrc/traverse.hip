@@ -137,9 +137,11 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     const uint32_t lanes = gridDim.x * TRAV_THREADS;
     const uint32_t gid = blockIdx.x * TRAV_THREADS + lane;
     uint2* my_spill = spill + gid;  // entry k >= LDS_STACK at my_spill[(k - LDS_STACK) * lanes]
-    // Rays are strided over the persistent grid: consecutive blocks take consecutive 64-ray spans.  Giving each
-    // XCD a contiguous eighth of the rays instead (so coherent rays share one L2) measured about 25% slower on the
-    // 1.29M-triangle heightfield workload.
+    // Rays are strided over the persistent grid: consecutive blocks take consecutive 64-ray spans.  Measured
+    // alternatives on the 1.29M-triangle heightfield workload: giving each XCD a contiguous eighth of the rays (so
+    // coherent rays share one L2) was about 25% slower; waves fetching further 64-ray spans from a counter (one
+    // device-scope atomic per span) made the primary launch 1.8x slower; a copy of the nodes at a 128-byte stride
+    // (one cache line per node) was 7% slower.
     for (uint32_t i = gid; i < n; i += lanes) {
     const uint4* nodes = nodes_in;
     const float4* tris = tris_in;
