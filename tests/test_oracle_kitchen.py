@@ -1,0 +1,63 @@
+"""The crate's known-answer render (examples/obj_cwbvh.rs:142-182) through the oracle: the kitchen asset built with
+each of the five presets the crate's test uses, one closest-hit ray per pixel of the 32 x 18 camera, the hash of
+the hit normals equal to 1343358762.  Brute force over all triangles gives the same hash, so the known answer pins
+the oracle's hits (closest t, strict <, Triangle::intersect) independently of any BVH."""
+import numpy as np
+import pytest
+
+import kitchen_scene as K
+import oracle_lib as O
+from obvhs_amd import BvhBuildParams
+
+PRESETS = {
+    "fastest": BvhBuildParams.fastest_build,
+    "fast": BvhBuildParams.fast_build,
+    "medium": BvhBuildParams.medium_build,
+    "slow": BvhBuildParams.slow_build,
+    "very_slow": BvhBuildParams.very_slow_build,
+}
+
+
+@pytest.fixture(scope="module")
+def scene():
+    tris = K.triangles()
+    rays, dirs = K.camera_rays(32)
+    return tris, rays, dirs
+
+
+def test_brute_force_hash(scene):
+    tris, rays, dirs = scene
+    assert len(tris) == 56939 and len(rays) == 32 * 18
+    hits = O.brute_force(tris, rays)
+    assert K.render_hash(tris, hits, dirs) == K.KNOWN_HASH
+
+
+@pytest.mark.parametrize("preset", list(PRESETS))
+def test_preset_hash(scene, preset):
+    """obj_cwbvh.rs:160-181.  SortPrecision::U128 (slow, very_slow) runs with U64 keys: the BVH differs from the
+    crate's there, the closest hits and therefore the hash do not."""
+    tris, rays, dirs = scene
+    p = PRESETS[preset]()
+    _, cw, prims = O.build_cwbvh(tris, int(p.ploc_search_distance), p.search_depth_threshold, p.max_prims_per_leaf,
+                                 pre_split=p.pre_split, reinsertion_batch_ratio=p.reinsertion_batch_ratio)
+    bvh_tris = tris[prims]
+    hits = O.traverse(cw, bvh_tris, rays, 0)
+    assert K.render_hash(bvh_tris, hits, dirs) == K.KNOWN_HASH
+    bf = O.brute_force(tris, rays)
+    hit = bf["primitive_id"] != 0xFFFFFFFF
+    np.testing.assert_array_equal(hits["primitive_id"] != 0xFFFFFFFF, hit)
+    np.testing.assert_array_equal(hits["t"][hit].view(np.uint32), bf["t"][hit].view(np.uint32))
+
+
+def test_hash_detects_wrong_normals_and_hits(scene):
+    """The check has teeth: a sign flip of one normal component, or every seventh pixel reporting the next
+    triangle, changes the hash."""
+    tris, rays, dirs = scene
+    hits = O.brute_force(tris, rays)
+    flipped = K.normals(tris, hits, dirs)
+    flipped[:, 0] = -flipped[:, 0]
+    h = int(np.bitwise_xor.reduce(K.uhash(flipped.reshape(-1).view(np.uint32)), initial=np.uint32(0)))
+    assert h != K.KNOWN_HASH
+    wrong = hits.copy()
+    wrong["primitive_id"][::7] = (wrong["primitive_id"][::7] + 1) % len(tris)
+    assert K.render_hash(tris, wrong, dirs) != K.KNOWN_HASH
