@@ -511,7 +511,7 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
     OBVHS_TRY(B->bvh2_nodes.ensure((size_t)2 * n * sizeof(Node32)), "alloc bvh2 nodes");
     OBVHS_TRY(B->bvh2_prims.ensure((size_t)n * 4), "alloc bvh2 prims");
     const size_t k = n;
-    OBVHS_TRY(B->dp.ensure(k * (28 + 8 + 28 + 4 + 4 + 8 + 4) + 1024), "alloc cost table");
+    OBVHS_TRY(B->dp.ensure(dp_table_bytes(k)), "alloc cost table");
     char* mb = S.misc.as<char>();
     SegState* seg[2] = {reinterpret_cast<SegState*>(mb), reinterpret_cast<SegState*>(mb + seg_bytes)};
     uint32_t* bounds = reinterpret_cast<uint32_t*>(mb + 2 * seg_bytes);
@@ -525,17 +525,7 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
     int8_t* merge = S.merge.as<int8_t>();
     uint2* blocksums = S.blocksums.as<uint2>();
     Node32* nodes = B->bvh2_nodes.as<Node32>();
-    char* dpb = B->dp.as<char>();
-    DpView dp;
-    dp.cost = reinterpret_cast<float*>(dpb);
-    dp.meta = reinterpret_cast<uint2*>(dpb + k * 28);
-    dp.sub = reinterpret_cast<uint32_t*>(dpb + k * 36);
-    dp.nprims = reinterpret_cast<uint32_t*>(dpb + k * 64);
-    dp.cwsize = reinterpret_cast<uint32_t*>(dpb + k * 68);
-    dp.subdepth = reinterpret_cast<uint2*>(dpb + k * 72);
-    dp.cwdepth = reinterpret_cast<uint32_t*>(dpb + k * 80);
-    dp.max_prims = max_prims;
-    dp.kbase = 0u;
+    DpView dp = dp_view_at(B->dp.as<char>(), k, max_prims);
 
     const uint32_t mblk = (m + T256 - 1) / T256;
     k_b_init<<<mblk, T256, 0, st>>>(bounds, offsets, m, seg[0], misc, n);

@@ -138,15 +138,18 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     }
     Box cb[8];
     uint32_t ckind = 0u;   // 2 bits per child: its decision-0 kind
+    uint32_t cleafpc[8];   // prim_count of a child that is a BVH2 leaf, 0 for an internal child
     uint32_t cfirst[8];    // first_index of each child
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         cb[c] = Box{0, 0, 0, 0, 0, 0};
         cfirst[c] = 0u;
+        cleafpc[c] = 0u;
         if (c < cc) {
             const Node32 cn = load_node(&nodes[ch[c]]);
             cb[c] = node_box(cn);
             cfirst[c] = cn.first_index;
+            cleafpc[c] = cn.prim_count;
             ckind |= node_kind(dp, cn, 0) << (2 * c);
         }
     }
@@ -240,10 +243,13 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
 #pragma unroll
         for (int q = 1; q < 8; ++q)
             if (q == c) b = cb[q];
-        uint32_t ci = ch[0];
+        uint32_t cf = cfirst[0], cpc = cleafpc[0];
 #pragma unroll
         for (int q = 1; q < 8; ++q)
-            if (q == c) ci = ch[q];
+            if (q == c) {
+                cf = cfirst[q];
+                cpc = cleafpc[q];
+            }
         float lo, hi;
         lo = floorf(((b.lx - o.p[0]) - 0.0f) * rx);
         hi = ceilf(((b.hx - o.p[0]) + 0.0f) * rx);
@@ -259,27 +265,14 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
         o.child_max_z[s] = (uint8_t)smin(smax(hi, 0.0f), 255.0f);
         const uint32_t kind = (ckind >> (2 * c)) & 3u;
         if (kind == KIND_LEAF) {
-            // count_primitives: left-first DFS of the (at most 3-leaf) subtree.
-            uint32_t stk[8];
-            int sp = 0;
-            stk[sp++] = ci;
-            uint32_t pc = 0;
-            while (sp > 0) {
-                const uint32_t x = stk[--sp];
-                const Node32 xn = load_node(&nodes[x]);
-                if (xn.prim_count != 0u) {
-                    const uint32_t dst = it.b + num_primitives + pc;
-                    if (dst < prim_cap) out_prims[dst] = bvh2_prims[xn.first_index];
-                    pc += xn.prim_count;
-                } else {
-                    if (sp + 2 > 8) {
-                        atomicOr(&misc->err, DERR_CW_PRIMCOUNT);
-                        return;
-                    }
-                    stk[sp++] = xn.first_index + 1;
-                    stk[sp++] = xn.first_index;
-                }
-            }
+            // count_primitives (bvh2_to_cwbvh.rs:197-211): a BVH2 leaf appends its first primitive and counts
+            // prim_count; a subtree of at most 3 leaves appends them left first, as recorded in the cost table.
+            const uint4 pl = cpc != 0u ? make_uint4(cf, 0u, 0u, 1u) : dp.plist[dp_key(dp, cf)];
+            const uint32_t pc = cpc != 0u ? cpc : pl.w;
+            const uint32_t dst = it.b + num_primitives;
+            if (pl.w >= 1u && dst < prim_cap) out_prims[dst] = bvh2_prims[pl.x];
+            if (pl.w >= 2u && dst + 1u < prim_cap) out_prims[dst + 1u] = bvh2_prims[pl.y];
+            if (pl.w >= 3u && dst + 2u < prim_cap) out_prims[dst + 2u] = bvh2_prims[pl.z];
             uint32_t unary;
             if (pc == 1u) unary = 0x20u;
             else if (pc == 2u) unary = 0x60u;
@@ -450,23 +443,12 @@ hipError_t upload_log2_table() {
 
 hipError_t bvh_dp_alloc(obvhs_bvh2* bvh) {
     const size_t k = bvh->prim_count > 1 ? bvh->prim_count - 1 : 1;
-    return bvh->dp.ensure(k * (28 + 8 + 28 + 4 + 4 + 8 + 4) + 1024);
+    return bvh->dp.ensure(dp_table_bytes(k));
 }
 
 DpView bvh_dp_view(const obvhs_bvh2* bvh, uint32_t max_prims) {
     const size_t k = bvh->prim_count > 1 ? bvh->prim_count - 1 : 1;
-    char* base = bvh->dp.as<char>();
-    DpView v;
-    v.cost = reinterpret_cast<float*>(base);
-    v.meta = reinterpret_cast<uint2*>(base + k * 28);
-    v.sub = reinterpret_cast<uint32_t*>(base + k * 36);
-    v.nprims = reinterpret_cast<uint32_t*>(base + k * 64);
-    v.cwsize = reinterpret_cast<uint32_t*>(base + k * 68);
-    v.subdepth = reinterpret_cast<uint2*>(base + k * 72);
-    v.cwdepth = reinterpret_cast<uint32_t*>(base + k * 80);
-    v.max_prims = max_prims;
-    v.kbase = 0u;
-    return v;
+    return dp_view_at(bvh->dp.as<char>(), k, max_prims);
 }
 
 obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_prims_per_leaf,

@@ -35,6 +35,7 @@ struct DpView {
     uint32_t* cwsize;   // [k] 1 + sub[0], or 0 when the node's CWBVH subtree cannot be built
     uint2* subdepth;    // [k] seven bytes: CWBVH depth below the children get_children(node, i) produces
     uint32_t* cwdepth;  // [k] 1 + subdepth[0]: levels of the CWBVH subtree the node roots
+    uint4* plist;       // [k] for a subtree of at most 3 primitives: its leaves' first_index, left first, and count
     uint32_t max_prims; // max_prims_per_leaf the table was evaluated for
     uint32_t kbase;     // key offset of the BVH2 being processed (batched builds: the mesh's first primitive)
 };
@@ -49,6 +50,24 @@ struct MeshDesc {
     uint32_t cw_depth;
     uint32_t pad[2];
 };
+
+// Table layout for k keys: the seven-decision arrays (84 bytes per key), then the 16-byte leaf lists.
+__host__ __device__ __forceinline__ size_t dp_plist_offset(size_t k) { return (k * 84 + 15) & ~(size_t)15; }
+__host__ __device__ __forceinline__ size_t dp_table_bytes(size_t k) { return dp_plist_offset(k) + k * 16 + 1024; }
+__host__ __device__ __forceinline__ DpView dp_view_at(char* base, size_t k, uint32_t max_prims) {
+    DpView v;
+    v.cost = reinterpret_cast<float*>(base);
+    v.meta = reinterpret_cast<uint2*>(base + k * 28);
+    v.sub = reinterpret_cast<uint32_t*>(base + k * 36);
+    v.nprims = reinterpret_cast<uint32_t*>(base + k * 64);
+    v.cwsize = reinterpret_cast<uint32_t*>(base + k * 68);
+    v.subdepth = reinterpret_cast<uint2*>(base + k * 72);
+    v.cwdepth = reinterpret_cast<uint32_t*>(base + k * 80);
+    v.plist = reinterpret_cast<uint4*>(base + dp_plist_offset(k));
+    v.max_prims = max_prims;
+    v.kbase = 0u;
+    return v;
+}
 
 __host__ __device__ __forceinline__ uint32_t dkey(uint32_t first_index) { return first_index >> 1; }
 __device__ __forceinline__ size_t dp_key(const DpView& dp, uint32_t first_index) {
@@ -206,6 +225,15 @@ __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box
     dp.nprims[key] = num_primitives;
     dp.cwsize[key] = sub[0] == SUB_INVALID ? 0u : 1u + sub[0];
     dp.cwdepth[key] = 1u + sd[0];
+    if (num_primitives <= 3u) {
+        // The leaves count_primitives (bvh2_to_cwbvh.rs:197-211) visits, left subtree first.  Each child holds at
+        // least one leaf, so the left list has one or two entries.
+        uint4 a = make_uint4(l.first_index, 0u, 0u, 1u), c = make_uint4(r.first_index, 0u, 0u, 1u);
+        if (!L.leaf) a = dp.plist[L.key];
+        if (!R.leaf) c = dp.plist[R.key];
+        dp.plist[key] = a.w >= 2u ? make_uint4(a.x, a.y, c.x, num_primitives)
+                                  : make_uint4(a.x, c.x, c.y, num_primitives);
+    }
     return 0u;
 }
 
