@@ -45,7 +45,7 @@ def check_build(ctx, tris, sd=14, sdt=0, max_prims=3):
     return bvh2, cw, rcw, rprims
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 17, 100, 1000, 4096, 4097, 9000, 40000])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 17, 100, 1000, 2048, 2049, 4096, 4097, 9000, 40000])
 def test_build_soup_sizes(ctx, n):
     check_build(ctx, scenes.soup(n, seed=n))
 
