@@ -37,6 +37,7 @@ constexpr int T256 = 256;
 constexpr uint32_t LEVELS_PER_SYNC = 16;
 constexpr uint32_t MIS_ROUNDS_PER_SYNC = 8;
 constexpr uint32_t FIND_THREADS = 1u << 18;  // concurrent find_reinsertion lanes (each owns a stack column)
+constexpr uint32_t FIND_LDS = 8;             // top entries of each find_reinsertion stack kept in LDS
 
 __device__ __forceinline__ uint32_t sibling_of(uint32_t id) { return (id & 1u) ? id + 1u : id - 1u; }
 __device__ __forceinline__ uint32_t left_sibling_of(uint32_t id) { return (id & 1u) ? id : id - 1u; }
@@ -224,8 +225,30 @@ __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restr
                                                            uint2* __restrict__ stack, uint32_t sd,
                                                            uint32_t* __restrict__ r_from, uint32_t* __restrict__ r_to,
                                                            uint64_t* __restrict__ keys, ReinsMisc* misc) {
+    // The DFS stack of each lane: its top FIND_LDS entries in an LDS ring, older entries in the lane's HBM column
+    // (entry k at stack[k * nthreads + tid]).  Entries [lo, sp) are in the ring, [0, lo) in HBM: a push into a full
+    // ring first writes the ring's oldest entry out (off the walk's critical path), a pop of an empty ring reads
+    // HBM.  The walk's pops then cost an LDS access instead of an L2 round trip.
+    __shared__ uint2 ring[FIND_LDS][T256];
     const uint32_t tid = blockIdx.x * T256 + threadIdx.x;
     const uint32_t nthreads = gridDim.x * T256;
+    uint32_t sp = 0, lo = 0;
+    auto push = [&](uint2 v) {
+        if (sp - lo == FIND_LDS) {
+            stack[(size_t)lo * nthreads + tid] = ring[lo % FIND_LDS][threadIdx.x];
+            ++lo;
+        }
+        ring[sp % FIND_LDS][threadIdx.x] = v;
+        ++sp;
+    };
+    auto pop = [&]() -> uint2 {
+        --sp;
+        if (sp < lo) {
+            lo = sp;
+            return stack[(size_t)sp * nthreads + tid];
+        }
+        return ring[sp % FIND_LDS][threadIdx.x];
+    };
     for (uint32_t t = tid; t < count; t += nthreads) {
         const uint32_t node_id = cand[t] + 1u;  // candidates were numbered from node 1
         uint32_t best_to = 0u;
@@ -242,16 +265,15 @@ __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restr
         uint32_t climb = 0, steps = 0;
         const uint32_t step_budget = 2u * n_nodes + 64u;
         for (;;) {
-            uint32_t sp = 0;
-            stack[(size_t)sp * nthreads + tid] = make_uint2(__float_as_uint(area_diff), sib);
-            sp = 1;
+            sp = 0;
+            lo = 0;
+            push(make_uint2(__float_as_uint(area_diff), sib));
             while (sp > 0) {
-                --sp;
                 if (++steps > step_budget) {
                     overflow = true;
                     break;
                 }
-                const uint2 top = stack[(size_t)sp * nthreads + tid];
+                const uint2 top = pop();
                 const float top_area = __uint_as_float(top.x);
                 if (top_area - node_area <= best_area) continue;
                 const Node32 dst = load_node(&nodes[top.y]);
@@ -268,10 +290,8 @@ __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restr
                         break;
                     }
                     const float child_area = reinsert_area + half_area(db);
-                    stack[(size_t)sp * nthreads + tid] = make_uint2(__float_as_uint(child_area), dst.first_index);
-                    stack[(size_t)(sp + 1) * nthreads + tid] =
-                        make_uint2(__float_as_uint(child_area), dst.first_index + 1u);
-                    sp += 2;
+                    push(make_uint2(__float_as_uint(child_area), dst.first_index));
+                    push(make_uint2(__float_as_uint(child_area), dst.first_index + 1u));
                 }
             }
             if (overflow) break;
