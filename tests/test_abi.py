@@ -1,5 +1,7 @@
-"""The C-ABI library on the CPU: it loads, exports every entry point include/obvhs_hip.h declares, and its
-host-side CWBVH exponent table equals what the host libm's log2f gives (bvh2_to_cwbvh.rs:85-88).  No GPU calls."""
+"""The C-ABI library on the CPU: it loads, exports every entry point include/obvhs_hip.h declares, the header is
+plain C (a C11 program compiles against it with every warning an error, links the library and calls its host-only
+entry points), and its host-side CWBVH exponent table equals what the host libm's log2f gives
+(bvh2_to_cwbvh.rs:85-88).  No GPU calls."""
 import ctypes
 import os
 import re
@@ -39,6 +41,42 @@ def test_status_strings():
     lib = native.lib()
     for code in native.STATUS:
         assert lib.obvhs_status_string(code)
+
+
+C_PROGRAM = r"""
+#include <stdio.h>
+#include <string.h>
+#include "obvhs_hip.h"
+int main(void) {
+    uint32_t table[256];
+    obvhs_status s = obvhs_log2_threshold_table(table);
+    if (s != OBVHS_OK) return 1;
+    if (strlen(obvhs_status_string(OBVHS_ERR_INVALID_ARG)) == 0) return 2;
+    printf("%u %u\n", table[127], table[200]);
+    return 0;
+}
+"""
+
+
+def test_header_is_c_and_links(tmp_path):
+    import shutil
+    import subprocess
+    from obvhs_amd import native
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "abi.c"
+    src.write_text(C_PROGRAM)
+    exe = tmp_path / "abi"
+    libdir = os.path.dirname(native.LIB_PATH)
+    r = subprocess.run([gcc, "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic", "-I", os.path.join(REPO, "include"),
+                        str(src), "-L", libdir, "-lobvhs_hip", "-Wl,-rpath," + libdir, "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    t127, t200 = map(int, r.stdout.split())
+    assert t127 == 0 and t200 > 0
 
 
 def f32_from_bits(b):
