@@ -327,38 +327,43 @@ __global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, 
     }
 }
 
-// Exclusive (emit, merge) prefix at cluster index i, from the scanned block offsets.
-__device__ __forceinline__ uint2 b_prefix_at(const int8_t* __restrict__ merge, const uint32_t* __restrict__ cmesh,
-                                             const SegState* __restrict__ seg, const uint2* __restrict__ offsets,
-                                             uint32_t i, uint32_t total, const BatchMisc* misc) {
+// Exclusive (emit, merge) prefix at cluster index i: the block's scanned offset plus the block's clusters before i,
+// counted by a whole wave, 64 clusters per step (one ballot each).
+__device__ __forceinline__ uint2 b_prefix_at_wave(const int8_t* __restrict__ merge,
+                                                  const uint32_t* __restrict__ cmesh,
+                                                  const SegState* __restrict__ seg, const uint2* __restrict__ offsets,
+                                                  uint32_t i, uint32_t total, const BatchMisc* misc, uint32_t lane) {
     if (i >= total) return make_uint2(misc->totals[0], misc->totals[1]);
     const uint32_t blk = i / T256;
     uint2 p = offsets[blk];
-    for (uint32_t j = blk * T256; j < i; ++j) {
-        bool emit, mrg;
+    for (uint32_t j0 = blk * T256; j0 < i; j0 += 64u) {
+        const uint32_t j = j0 + lane;
+        bool emit = false, mrg = false;
         uint32_t b;
-        b_classify(merge, cmesh, seg, j, total, emit, mrg, b);
-        p.x += emit ? 1u : 0u;
-        p.y += mrg ? 1u : 0u;
+        if (j < i) b_classify(merge, cmesh, seg, j, total, emit, mrg, b);
+        p.x += (uint32_t)__popcll(__ballot(emit));
+        p.y += (uint32_t)__popcll(__ballot(mrg));
     }
     return p;
 }
 
-// Per mesh: the segment of the next pass, its insert index and this pass's merge base.
+// Per mesh (one wave each): the segment of the next pass, its insert index and this pass's merge base.
 __global__ void __launch_bounds__(T256) k_b_segments(const int8_t* __restrict__ merge,
                                                      const uint32_t* __restrict__ cmesh, SegState* __restrict__ seg,
                                                      SegState* __restrict__ seg_next, const uint2* __restrict__ offsets,
                                                      uint32_t m, BatchMisc* misc) {
-    const uint32_t k = blockIdx.x * T256 + threadIdx.x;
-    if (k >= m) return;
+    const uint32_t k = (blockIdx.x * T256 + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (k >= m) return;  // wave-uniform
     const SegState sg = seg[k];
     if (sg.count == 0u) {
-        seg_next[k] = sg;
+        if (lane == 0u) seg_next[k] = sg;
         return;
     }
     const uint32_t total = misc->total;
-    const uint2 head = b_prefix_at(merge, cmesh, seg, offsets, sg.start, total, misc);
-    const uint2 end = b_prefix_at(merge, cmesh, seg, offsets, sg.start + sg.count, total, misc);
+    const uint2 head = b_prefix_at_wave(merge, cmesh, seg, offsets, sg.start, total, misc, lane);
+    const uint2 end = b_prefix_at_wave(merge, cmesh, seg, offsets, sg.start + sg.count, total, misc, lane);
+    if (lane != 0u) return;
     const uint32_t merges = end.y - head.y;
     seg[k].mbase = head.y;
     SegState nx{head.x, end.x - head.x, sg.insert - 2u * merges, 0u};
@@ -557,7 +562,8 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
             k_b_search<<<blocks, T256, 0, st>>>(cur, cmesh[cm], seg[cs], misc, r, dist1, merge);
             k_b_count<<<blocks, T256, 0, st>>>(merge, cmesh[cm], seg[cs], misc, blocksums);
             k_b_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc);
-            k_b_segments<<<mblk, T256, 0, st>>>(merge, cmesh[cm], seg[cs], seg[cs ^ 1], blocksums, m, misc);
+            k_b_segments<<<(m + 3u) / 4u, T256, 0, st>>>(merge, cmesh[cm], seg[cs], seg[cs ^ 1], blocksums, m,
+                                                          misc);
             k_b_compact<<<blocks, T256, 0, st>>>(cur, nxt, cmesh[cm], cmesh[cm ^ 1], merge, seg[cs], misc, blocksums,
                                                  d_meshes, nodes, dp);
             k_b_after_pass<<<1, 64, 0, st>>>(misc);
