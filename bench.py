@@ -146,6 +146,16 @@ def bench_batch(args, ob, ctx):
     ctx.synchronize()
     trace_s = (time.perf_counter() - t0) / args.steps
     batch_hits = hits.to_host()
+    info = batch.info()
+    result = {"workload": f"{m} icospheres (log-uniform 64..65536 tris, rounded down to 20*4^s) = {n_tris} tris; "
+                          f"{args.batch_rays_per_mesh} rays per mesh = {n_rays} closest-hit rays",
+              "build_ms": round(build_ms, 3), "build_mtris_per_s": round(n_tris / build_ms / 1e3, 2),
+              "trace_mrays_per_s": round(n_rays / trace_s / 1e6, 2), "trace_ms": round(trace_s * 1e3, 4),
+              "ploc_passes": info["ploc_passes"], "cwbvh_nodes": info["total_nodes"],
+              "hit_fraction": round(float(np.isfinite(batch_hits["t"]).mean()), 4)}
+    if args.batch_no_plain:
+        batch.close()
+        return result
 
     # Plain form: per-mesh handles, builds and traversal launches through the single-mesh entry points.
     log("plain per-mesh builds")
@@ -191,19 +201,14 @@ def bench_batch(args, ob, ctx):
     plain_trace_s = (time.perf_counter() - t0) / max(1, args.steps // 4)
     for h in handles:
         lib.obvhs_cwbvh_destroy(h)
-    info = batch.info()
     batch.close()
-    return {"workload": f"{m} icospheres (log-uniform 64..65536 tris, rounded down to 20*4^s) = {n_tris} tris; "
-                        f"{rpm} rays per mesh = {n_rays} closest-hit rays",
-            "build_ms": round(build_ms, 3), "build_mtris_per_s": round(n_tris / build_ms / 1e3, 2),
-            "trace_mrays_per_s": round(n_rays / trace_s / 1e6, 2), "trace_ms": round(trace_s * 1e3, 4),
-            "plain_build_ms": round(plain_build_ms, 3),
-            "plain_build_mtris_per_s": round(n_tris / plain_build_ms / 1e3, 2),
-            "plain_trace_mrays_per_s": round(n_rays / plain_trace_s / 1e6, 2),
-            "trace_speedup_vs_plain": round(plain_trace_s / trace_s, 2),
-            "build_speedup_vs_plain": round(plain_build_ms / build_ms, 2),
-            "hits_identical_to_plain": same, "ploc_passes": info["ploc_passes"],
-            "cwbvh_nodes": info["total_nodes"], "hit_fraction": round(float(np.isfinite(batch_hits["t"]).mean()), 4)}
+    result.update({"plain_build_ms": round(plain_build_ms, 3),
+                   "plain_build_mtris_per_s": round(n_tris / plain_build_ms / 1e3, 2),
+                   "plain_trace_mrays_per_s": round(n_rays / plain_trace_s / 1e6, 2),
+                   "trace_speedup_vs_plain": round(plain_trace_s / trace_s, 2),
+                   "build_speedup_vs_plain": round(plain_build_ms / build_ms, 2),
+                   "hits_identical_to_plain": same})
+    return result
 
 
 def bench_rebuild(args, ob, ctx, tris, d_tris, fresh_build_ms):
@@ -496,6 +501,7 @@ def main():
     ap.add_argument("--cpu-sample-rays", type=int, default=1 << 20)
     ap.add_argument("--batch-meshes", type=int, default=4096, help="0 skips the batched leg")
     ap.add_argument("--batch-rays-per-mesh", type=int, default=1024)
+    ap.add_argument("--batch-no-plain", action="store_true", help="skip the per-mesh comparison of the batched leg")
     ap.add_argument("--rebuild-frac", type=float, default=0.01, help="0 skips the partial-rebuild leg")
     ap.add_argument("--large-grid", type=int, default=2896, help="0 skips the large-scene leg")
     ap.add_argument("--large-thin", type=int, default=500000)
