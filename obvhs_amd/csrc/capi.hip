@@ -110,6 +110,8 @@ obvhs_status obvhs_context_destroy(obvhs_context* ctx) {
     S.sort_scratch.release();
     S.misc.release();
     S.cw_scratch.release();
+    S.trav_spill.release();
+    ctx->quality.release_all();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,12 @@ struct QualityScratch {
     DevBuf sort_scratch;
     DevBuf split_aabbs, split_indices, ha, cand_a, cand_b, halves, flags, sums;  // pre-splits
     DevBuf marks, kept, freed;                                                   // rebuilds
+    void release_all() {
+        for (DevBuf* d : {&level_list, &level_meta, &parents, &owner, &touched, &misc, &keys, &vals, &rkeys, &rvals,
+                          &rfrom, &rto, &conf, &state, &stack, &sort_scratch, &split_aabbs, &split_indices, &ha,
+                          &cand_a, &cand_b, &halves, &flags, &sums, &marks, &kept, &freed})
+            d->release();
+    }
 };
 
 }  // namespace obvhs
