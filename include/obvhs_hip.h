@@ -42,7 +42,8 @@ typedef struct obvhs_ray {                                                      
     float tmin, tmax;
     float _pad[2];
 } obvhs_ray;
-typedef struct obvhs_ray_hit { uint32_t primitive_id, geometry_id, instance_id; float t; } obvhs_ray_hit; /* ray.rs:63-70 */
+typedef struct obvhs_ray_hit { uint32_t primitive_id, geometry_id, instance_id; float t; } obvhs_ray_hit;
+/* ray.rs:63-70 */
 typedef struct obvhs_bvh2_node {                                                    /* bvh2/node.rs:38-66, 48 B */
     obvhs_aabb aabb;
     uint32_t prim_count, first_index, meta1, meta2;

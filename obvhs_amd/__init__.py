@@ -8,7 +8,7 @@ from .types import (AABB, BVH2_NODE, BVH2_NODE_SMALL, CWBVH_NODE, INVALID_ID, RA
                     aabbs_from_minmax, make_rays, ray_hit_none, triangles_from_vertices)
 
 __all__ = ["Bvh2", "BvhBuildParams", "Context", "CwBvh", "CwBvhBatch", "DeviceArray", "ObvhsError", "PlocBuilder",
-           "PlocSearchDistance", "Server", "SortPrecision", "build_cwbvh_batch", "build_cwbvh_from_tris", "bvh2_to_cwbvh",
-           "compute_rebuild_path_flags", "log2_threshold_table", "split_aabbs_preset",
+           "PlocSearchDistance", "Server", "SortPrecision", "build_cwbvh_batch", "build_cwbvh_from_tris",
+           "bvh2_to_cwbvh", "compute_rebuild_path_flags", "log2_threshold_table", "split_aabbs_preset",
            "AABB", "BVH2_NODE", "BVH2_NODE_SMALL", "CWBVH_NODE", "INVALID_ID", "RAY", "RAY_HIT", "TRIANGLE",
            "aabbs_from_minmax", "make_rays", "ray_hit_none", "triangles_from_vertices"]

@@ -702,5 +702,6 @@ def log2_threshold_table():
     return out
 
 
-__all__ = ["Server", "CwBvhBatch", "build_cwbvh_batch", "split_aabbs_preset", "PlocSearchDistance", "SortPrecision", "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder",
+__all__ = ["Server", "CwBvhBatch", "build_cwbvh_batch", "split_aabbs_preset", "PlocSearchDistance", "SortPrecision",
+           "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder", "compute_rebuild_path_flags",
            "CwBvh", "bvh2_to_cwbvh", "build_cwbvh_from_tris", "log2_threshold_table", "ObvhsError"]

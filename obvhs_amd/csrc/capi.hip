@@ -553,8 +553,8 @@ obvhs_status obvhs_cwbvh_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh* bvh
         const obvhs_status ss = traverse_scratch(ctx, ts);
         if (ss != OBVHS_OK) return ss;
     }
-    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0, d_hits, nullptr,
-                            nullptr, nullptr, ts, ctx->stream),
+    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0,
+                            d_hits, nullptr, nullptr, nullptr, ts, ctx->stream),
             "traverse");
     return OBVHS_OK;
 }
@@ -569,8 +569,8 @@ obvhs_status obvhs_cwbvh_ray_traverse_miss(obvhs_context* ctx, const obvhs_cwbvh
         const obvhs_status ss = traverse_scratch(ctx, ts);
         if (ss != OBVHS_OK) return ss;
     }
-    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 1, nullptr, d_miss,
-                            nullptr, nullptr, ts, ctx->stream),
+    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 1,
+                            nullptr, d_miss, nullptr, nullptr, ts, ctx->stream),
             "traverse miss");
     return OBVHS_OK;
 }
@@ -586,8 +586,8 @@ obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cw
         const obvhs_status ss = traverse_scratch(ctx, ts);
         if (ss != OBVHS_OK) return ss;
     }
-    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0, d_hits, nullptr,
-                            d_node_visits, d_tri_tests, ts, ctx->stream),
+    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0,
+                            d_hits, nullptr, d_node_visits, d_tri_tests, ts, ctx->stream),
             "traverse counted");
     return OBVHS_OK;
 }

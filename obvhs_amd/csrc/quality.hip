@@ -145,7 +145,8 @@ __global__ void __launch_bounds__(T256) k_refit_level(Node32* __restrict__ nodes
     const uint32_t x = list[start + i];
     Node32 nd = load_node(&nodes[x]);
     if (nd.prim_count != 0u) return;
-    const Box u = box_union(node_box(load_node(&nodes[nd.first_index])), node_box(load_node(&nodes[nd.first_index + 1])));
+    const Box u =
+        box_union(node_box(load_node(&nodes[nd.first_index])), node_box(load_node(&nodes[nd.first_index + 1])));
     store_node(&nodes[x], make_node(u, 0u, nd.first_index));
 }
 
@@ -444,7 +445,8 @@ __device__ __forceinline__ void box_extend(Box& b, float x, float y, float z) {
 }
 // aabb.rs:98-103 (self = the clipped triangle box, other = the halved leaf box)
 __device__ __forceinline__ Box box_intersection(const Box& a, const Box& b) {
-    return Box{smax(a.lx, b.lx), smax(a.ly, b.ly), smax(a.lz, b.lz), smin(a.hx, b.hx), smin(a.hy, b.hy), smin(a.hz, b.hz)};
+    return Box{smax(a.lx, b.lx), smax(a.ly, b.ly), smax(a.lz, b.lz),
+               smin(a.hx, b.hx), smin(a.hy, b.hy), smin(a.hz, b.hz)};
 }
 // aabb.rs:125-134
 __device__ __forceinline__ uint32_t largest_axis(const Box& a) {
@@ -544,7 +546,8 @@ __global__ void __launch_bounds__(SUM_THREADS) k_split_avg(const float* __restri
         const uint32_t bits = __float_as_uint(s);
         const uint32_t be_raw = (bits >> 23) & 0xffu;
         const uint32_t be = be_raw == 0u ? 1u : be_raw;  // subnormals share the first normal binade's ulp
-        const unsigned long long m0 = zero ? 0ull : (unsigned long long)((bits & 0x7fffffu) | (be_raw ? 0x800000u : 0u));
+        const unsigned long long m0 =
+            zero ? 0ull : (unsigned long long)((bits & 0x7fffffu) | (be_raw ? 0x800000u : 0u));
         const double inv_u = ldexp(1.0, 150 - (int)be);
         unsigned long long tsum = 0ull;
 #pragma unroll

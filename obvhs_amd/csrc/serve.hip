@@ -12,10 +12,10 @@
 //     budget allows, and launches one traversal per mode over every ready request at the head of its scene's FIFO
 //     (each request a segment of the launch: its own BVH, rays and outputs) while fewer than max_inflight launches
 //     are on the trace stream;
-//   * update threads (one context and stream each) run refits and rebuilds, several scenes at a time.  An update writes the scene's
-//     other version slot, after the trace stream's last launch that read that slot (event wait), and becomes the
-//     scene's current version when it completes; traces submitted after it wait for it, traces submitted before it
-//     keep reading the old slot.
+//   * update threads (one context and stream each) run refits and rebuilds, several scenes at a time.  An update
+//     writes the scene's other version slot, after the trace stream's last launch that read that slot (event wait),
+//     and becomes the scene's current version when it completes; traces submitted after it wait for it, traces
+//     submitted before it keep reading the old slot.
 #include <string.h>
 
 #include <algorithm>
@@ -189,7 +189,8 @@ obvhs_status run_update(obvhs_server* S, obvhs_context* ctx, Scene* sc, Req* r, 
             if (!dst.cw->owned_bvh2) return OBVHS_ERR_OOM;
         }
         obvhs_bvh2* d = dst.cw->owned_bvh2;
-        if (d->nodes.ensure(b->nodes.bytes) != hipSuccess || d->prim_indices.ensure(b->prim_indices.bytes) != hipSuccess ||
+        if (d->nodes.ensure(b->nodes.bytes) != hipSuccess ||
+            d->prim_indices.ensure(b->prim_indices.bytes) != hipSuccess ||
             d->hist.ensure(b->hist.bytes ? b->hist.bytes : 4) != hipSuccess)
             return OBVHS_ERR_OOM;
         if (hipMemcpyAsync(d->nodes.ptr, b->nodes.ptr, (size_t)b->node_count * 32, hipMemcpyDeviceToDevice, st) !=
@@ -600,7 +601,8 @@ obvhs_status obvhs_server_replay(obvhs_server* S, const obvhs_server_request* li
         const Clock::time_point due = start + std::chrono::nanoseconds(q.arrival_ns);
         while (Clock::now() < due) {
             const auto left = due - Clock::now();
-            if (left > std::chrono::microseconds(200)) std::this_thread::sleep_for(left - std::chrono::microseconds(100));
+            if (left > std::chrono::microseconds(200))
+                std::this_thread::sleep_for(left - std::chrono::microseconds(100));
         }
         if (q.kind == OBVHS_REQ_TRACE_CLOSEST || q.kind == OBVHS_REQ_TRACE_MISS)
             sub[i] = obvhs_server_submit_trace(S, q.scene, q.kind, static_cast<const obvhs_ray*>(q.rays), q.n, q.out,

@@ -449,7 +449,8 @@ hipError_t launch_traverse_segments(const TraceSegment* segs, uint32_t nseg, uin
 obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts) {
     if (ctx->trav_lanes == 0) {
         hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return set_hip_error(ctx, hipGetLastError(), "device properties");
+        if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess)
+            return set_hip_error(ctx, hipGetLastError(), "device properties");
         ctx->trav_lanes = (uint32_t)std::max(1, prop.multiProcessorCount) * 32u * TRAV_THREADS;
     }
     const size_t bytes = (size_t)ctx->trav_lanes * (STACK_SIZE - LDS_STACK) * sizeof(uint2);

@@ -69,8 +69,9 @@ def test_header_is_c_and_links(tmp_path):
     src.write_text(C_PROGRAM)
     exe = tmp_path / "abi"
     libdir = os.path.dirname(native.LIB_PATH)
-    r = subprocess.run([gcc, "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic", "-I", os.path.join(REPO, "include"),
-                        str(src), "-L", libdir, "-lobvhs_hip", "-Wl,-rpath," + libdir, "-o", str(exe)],
+    r = subprocess.run([gcc, "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic",
+                        "-I", os.path.join(REPO, "include"), str(src), "-L", libdir, "-lobvhs_hip",
+                        "-Wl,-rpath," + libdir, "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)

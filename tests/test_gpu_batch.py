@@ -150,7 +150,8 @@ def test_batch_reuse_and_out_of_range_mesh_ids(ctx):
 def test_batch_invalid_arguments(ctx):
     tris, offsets = concat([scenes.soup(10), scenes.soup(10)])
     with pytest.raises(ob.ObvhsError) as e:
-        ob.build_cwbvh_batch(ctx, tris, offsets, ob.BvhBuildParams(ploc_search_distance=14, reinsertion_batch_ratio=0.1))
+        ob.build_cwbvh_batch(ctx, tris, offsets,
+                             ob.BvhBuildParams(ploc_search_distance=14, reinsertion_batch_ratio=0.1))
     assert e.value.name == "OBVHS_ERR_INVALID_ARG"
     with pytest.raises(ValueError):
         ob.build_cwbvh_batch(ctx, tris, offsets[:-1], ob.BvhBuildParams())

@@ -29,7 +29,8 @@ def test_single_blas_equals_plain_traversal():
     b, cw, prims, bt = blas_of(tris)
     tlas, tprims = tlas_over([b["nodes"][0]])
     rays = scenes.pinhole_rays(48, 48, eye=(0.5, 0.5, -1.5))
-    rays = np.concatenate([rays, make_rays(np.full((500, 3), -1.0), np.random.default_rng(1).uniform(0.2, 1, (500, 3)))])
+    far = make_rays(np.full((500, 3), -1.0), np.random.default_rng(1).uniform(0.2, 1, (500, 3)))
+    rays = np.concatenate([rays, far])
     h = O.tlas_traverse(tlas, tprims, [(cw, bt)], rays)
     p = O.traverse(cw, bt, rays)
     np.testing.assert_array_equal(h["primitive_id"], p["primitive_id"])
