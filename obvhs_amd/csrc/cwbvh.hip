@@ -125,19 +125,19 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     const Box nb = node_box(nd);
 
     uint32_t ch[8];
+    uint32_t ckind = 0u;  // 2 bits per child: its decision-0 kind
     int cc;
     if (nd.prim_count != 0u) {  // a single-leaf BVH2: the root is its own only child
         ch[0] = it.bvh2;
         cc = 1;
     } else {
-        cc = get_children_keyed(nodes, dp, nd.first_index, node_decision(dp, nd.first_index, 0), ch);
+        cc = get_children_kids(dp, nd.first_index, node_decision(dp, nd.first_index, 0), ch, &ckind);
     }
     if (cc < 0) {
         atomicOr(&misc->err, cc == -2 ? DERR_CW_DISTRIBUTE : DERR_CAPACITY);
         return;
     }
     Box cb[8];
-    uint32_t ckind = 0u;   // 2 bits per child: its decision-0 kind
     uint32_t cleafpc[8];   // prim_count of a child that is a BVH2 leaf, 0 for an internal child
     uint32_t cfirst[8];    // first_index of each child
 #pragma unroll
@@ -150,7 +150,6 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
             cb[c] = node_box(cn);
             cfirst[c] = cn.first_index;
             cleafpc[c] = cn.prim_count;
-            ckind |= node_kind(dp, cn, 0) << (2 * c);
         }
     }
     // order_children (bvh2_to_cwbvh.rs:402-467): greedy octant assignment, child-major / slot-minor, strict <.

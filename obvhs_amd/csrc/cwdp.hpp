@@ -25,6 +25,7 @@ constexpr uint32_t KIND_LEAF = 0, KIND_INTERNAL = 1, KIND_DISTRIBUTE = 2;  // bv
 constexpr uint32_t DIDX_INVALID = 7u;      // distribute index "INVALID" (u8::MAX in the reference) in 3 bits
 constexpr uint32_t SUB_INVALID = 0xffffffffu;
 constexpr float PRIM_COST = 0.3f;  // bvh2_to_cwbvh.rs:30
+constexpr uint32_t KID_LEAF = 0x80000000u;  // leaf flag in DpView::kids
 
 // Decision byte: kind (2 bits) | distribute_left << 2 (3 bits) | distribute_right << 5 (3 bits).
 struct DpView {
@@ -36,6 +37,7 @@ struct DpView {
     uint2* subdepth;    // [k] seven bytes: CWBVH depth below the children get_children(node, i) produces
     uint32_t* cwdepth;  // [k] 1 + subdepth[0]: levels of the CWBVH subtree the node roots
     uint4* plist;       // [k] for a subtree of at most 3 primitives: its leaves' first_index, left first, and count
+    uint2* kids;        // [k] the children's first_index (bit 31 set for a leaf), left then right
     uint32_t max_prims; // max_prims_per_leaf the table was evaluated for
     uint32_t kbase;     // key offset of the BVH2 being processed (batched builds: the mesh's first primitive)
 };
@@ -51,9 +53,9 @@ struct MeshDesc {
     uint32_t pad[2];
 };
 
-// Table layout for k keys: the seven-decision arrays (84 bytes per key), then the 16-byte leaf lists.
+// Table layout for k keys: the seven-decision arrays (84 bytes per key), the 16-byte leaf lists, the children.
 __host__ __device__ __forceinline__ size_t dp_plist_offset(size_t k) { return (k * 84 + 15) & ~(size_t)15; }
-__host__ __device__ __forceinline__ size_t dp_table_bytes(size_t k) { return dp_plist_offset(k) + k * 16 + 1024; }
+__host__ __device__ __forceinline__ size_t dp_table_bytes(size_t k) { return dp_plist_offset(k) + k * 24 + 1024; }
 __host__ __device__ __forceinline__ DpView dp_view_at(char* base, size_t k, uint32_t max_prims) {
     DpView v;
     v.cost = reinterpret_cast<float*>(base);
@@ -64,6 +66,7 @@ __host__ __device__ __forceinline__ DpView dp_view_at(char* base, size_t k, uint
     v.subdepth = reinterpret_cast<uint2*>(base + k * 72);
     v.cwdepth = reinterpret_cast<uint32_t*>(base + k * 80);
     v.plist = reinterpret_cast<uint4*>(base + dp_plist_offset(k));
+    v.kids = reinterpret_cast<uint2*>(base + dp_plist_offset(k) + k * 16);
     v.max_prims = max_prims;
     v.kbase = 0u;
     return v;
@@ -225,6 +228,7 @@ __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box
     dp.nprims[key] = num_primitives;
     dp.cwsize[key] = sub[0] == SUB_INVALID ? 0u : 1u + sub[0];
     dp.cwdepth[key] = 1u + sd[0];
+    dp.kids[key] = make_uint2(l.first_index | (L.leaf ? KID_LEAF : 0u), r.first_index | (R.leaf ? KID_LEAF : 0u));
     if (num_primitives <= 3u) {
         // The leaves count_primitives (bvh2_to_cwbvh.rs:197-211) visits, left subtree first.  Each child holds at
         // least one leaf, so the left list has one or two entries.
@@ -237,51 +241,60 @@ __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box
     return 0u;
 }
 
-// get_children(node, 0) for an internal node whose first child slot is `first` and whose decision 0 is `meta0`.
-// Children are appended left-first like the reference's recursion.  Returns the count, -1 when more than 8
-// children would be produced, -2 on a malformed decision (distribute index invalid).
-__device__ __forceinline__ int get_children_keyed(const Node32* __restrict__ nodes, const DpView& dp,
-                                                  uint32_t first, uint32_t meta0, uint32_t* children) {
-    uint32_t stk_node[8];
-    uint32_t stk_i[8];
+// get_children(node, 0) from the cost table alone: the node's children are the slots first and first + 1 and their
+// first_index values are in its `kids` entry, so each level of the walk loads one entry (decisions and children
+// together) instead of a node and then its decisions.  Children are appended left-first like the reference's
+// recursion (bvh2_to_cwbvh.rs:346-397).  Outputs per child: its slot, and its decision-0 kind in 2 bits of *kinds.
+// Returns the count, -1 when more than 8 children would be produced, -2 on a malformed decision (distribute index
+// invalid).
+__device__ __forceinline__ int get_children_kids(const DpView& dp, uint32_t first, uint32_t meta0, uint32_t* children,
+                                                 uint32_t* kinds) {
+    uint32_t stk_slot[8], stk_info[8], stk_i[8];
     int sp = 0;
     int cc = 0;
-    uint32_t dl = dleft(meta0), dr = dright(meta0);
+    uint32_t kb = 0u;
+    const uint32_t dl = dleft(meta0), dr = dright(meta0);
     if (dl == DIDX_INVALID || dr == DIDX_INVALID) return -2;
-    {
-        const Node32 r = load_node(&nodes[first + 1]);
-        const Node32 l = load_node(&nodes[first]);
-        stk_node[sp] = first + 1;
-        stk_i[sp] = node_kind(dp, r, dr) == KIND_DISTRIBUTE ? dr : 7u;
-        sp++;
-        stk_node[sp] = first;
-        stk_i[sp] = node_kind(dp, l, dl) == KIND_DISTRIBUTE ? dl : 7u;
-        sp++;
-    }
+    const uint2 k0 = dp.kids[dp_key(dp, first)];
+    stk_slot[0] = first + 1u;
+    stk_info[0] = k0.y;
+    stk_i[0] = dr;
+    stk_slot[1] = first;
+    stk_info[1] = k0.x;
+    stk_i[1] = dl;
+    sp = 2;
     while (sp > 0) {
         sp--;
-        const uint32_t x = stk_node[sp], i = stk_i[sp];
-        if (i == 7u) {
-            if (cc >= 8) return -1;
-            children[cc++] = x;
-            continue;
+        const uint32_t x = stk_slot[sp], info = stk_info[sp], i = stk_i[sp];
+        uint32_t kind0 = KIND_LEAF;
+        if (!(info & KID_LEAF)) {  // a leaf is never DISTRIBUTE
+            const uint32_t f = info & ~KID_LEAF;
+            const size_t key = dp_key(dp, f);
+            const uint2 m = dp.meta[key];
+            const uint2 kx = dp.kids[key];  // loaded with the decisions; used when the child is distributed
+            const uint32_t b = dbyte(m, i);
+            if (dkind(b) == KIND_DISTRIBUTE) {
+                const uint32_t cl = dleft(b), cr = dright(b);
+                if (cl == DIDX_INVALID || cr == DIDX_INVALID) return -2;
+                if (sp + 2 > 8) return -1;
+                stk_slot[sp] = f + 1u;
+                stk_info[sp] = kx.y;
+                stk_i[sp] = cr;
+                sp++;
+                stk_slot[sp] = f;
+                stk_info[sp] = kx.x;
+                stk_i[sp] = cl;
+                sp++;
+                continue;
+            }
+            kind0 = dkind(dbyte(m, 0u));
         }
-        const Node32 nx = load_node(&nodes[x]);  // internal (a leaf is never DISTRIBUTE)
-        const uint32_t m = node_decision(dp, nx.first_index, i);
-        const uint32_t f = nx.first_index;
-        dl = dleft(m);
-        dr = dright(m);
-        if (dl == DIDX_INVALID || dr == DIDX_INVALID) return -2;
-        if (sp + 2 > 8) return -1;
-        const Node32 r = load_node(&nodes[f + 1]);
-        const Node32 l = load_node(&nodes[f]);
-        stk_node[sp] = f + 1;
-        stk_i[sp] = node_kind(dp, r, dr) == KIND_DISTRIBUTE ? dr : 7u;
-        sp++;
-        stk_node[sp] = f;
-        stk_i[sp] = node_kind(dp, l, dl) == KIND_DISTRIBUTE ? dl : 7u;
-        sp++;
+        if (cc >= 8) return -1;
+        children[cc] = x;
+        kb |= kind0 << (2 * cc);
+        cc++;
     }
+    *kinds = kb;
     return cc;
 }
 
