@@ -14,7 +14,9 @@
 // cwsize = 1 + sub[0] is the size of the CWBVH subtree the node roots if it becomes a CWBVH node -- what the
 // top-down pass needs to number nodes in the reference's pre-order (bvh2_to_cwbvh.rs:172-191).  The same
 // recurrence with max instead of + gives the depth of that CWBVH subtree, so the top-down pass knows its level
-// count without reading anything back.
+// count without reading anything back.  Each entry also keeps the children's first_index words and, for a subtree
+// of at most 3 primitives, its leaf list: the top-down pass walks get_children and collects a leaf child's
+// primitives from the table, one dependent load per level instead of a node and then its decisions.
 #pragma once
 
 #include "common.hpp"
