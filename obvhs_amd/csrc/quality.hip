@@ -216,10 +216,13 @@ __global__ void __launch_bounds__(T256) k_cand_keys(const Node32* __restrict__ n
 struct ReinsMisc {
     uint32_t err;
     uint32_t positive;               // reinsertions with area_diff > 0
+    uint32_t next;                   // candidates handed out beyond the grid's first span
     uint32_t undecided[MIS_ROUNDS_PER_SYNC];
 };
 
-// reinsertion.rs:247-340, one lane per candidate (grid-stride), stack column `tid` of an [sd][nthreads] array.
+// reinsertion.rs:247-340, one lane per candidate, stack column `tid` of an [sd][nthreads] array.  The grid first
+// takes candidates [0, nthreads) (wave w the span [64w, 64w + 64)); a wave that finishes its span fetches the next
+// from a counter, so waves whose walks are short take more candidates (one atomic per 64 walks).
 __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restrict__ nodes, uint32_t n_nodes,
                                                            const uint32_t* __restrict__ parents,
                                                            const uint32_t* __restrict__ cand, uint32_t count,
@@ -250,79 +253,87 @@ __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restr
         }
         return ring[sp % FIND_LDS][threadIdx.x];
     };
-    for (uint32_t t = tid; t < count; t += nthreads) {
-        const uint32_t node_id = cand[t] + 1u;  // candidates were numbered from node 1
-        uint32_t best_to = 0u;
-        float best_area = 0.0f;
-        const Box aabb = node_box(load_node(&nodes[node_id]));
-        const float node_area = half_area(aabb);
-        const uint32_t parent_id = parents[node_id];
-        float area_diff = half_area(node_box(load_node(&nodes[parent_id])));
-        uint32_t sib = sibling_of(node_id);
-        Box pivot_bbox = node_box(load_node(&nodes[sib]));
-        uint32_t pivot_id = parent_id;
-        bool overflow = false;
-        // walks of a valid tree end long before these budgets; they only bound a malformed one
-        uint32_t climb = 0, steps = 0;
-        const uint32_t step_budget = 2u * n_nodes + 64u;
-        for (;;) {
-            sp = 0;
-            lo = 0;
-            push(make_uint2(__float_as_uint(area_diff), sib));
-            while (sp > 0) {
-                if (++steps > step_budget) {
-                    overflow = true;
-                    break;
-                }
-                const uint2 top = pop();
-                const float top_area = __uint_as_float(top.x);
-                if (top_area - node_area <= best_area) continue;
-                const Node32 dst = load_node(&nodes[top.y]);
-                const Box db = node_box(dst);
-                const float merged_area = half_area(box_union(db, aabb));
-                const float reinsert_area = top_area - merged_area;
-                if (reinsert_area > best_area) {
-                    best_to = top.y;
-                    best_area = reinsert_area;
-                }
-                if (dst.prim_count == 0u) {
-                    if (sp + 2 > sd) {
+    uint32_t span = tid & ~63u;
+    for (;;) {
+        if (span >= count) break;  // wave-uniform
+        const uint32_t t = span + (threadIdx.x & 63u);
+        if (t < count) {
+            const uint32_t node_id = cand[t] + 1u;  // candidates were numbered from node 1
+            uint32_t best_to = 0u;
+            float best_area = 0.0f;
+            const Box aabb = node_box(load_node(&nodes[node_id]));
+            const float node_area = half_area(aabb);
+            const uint32_t parent_id = parents[node_id];
+            float area_diff = half_area(node_box(load_node(&nodes[parent_id])));
+            uint32_t sib = sibling_of(node_id);
+            Box pivot_bbox = node_box(load_node(&nodes[sib]));
+            uint32_t pivot_id = parent_id;
+            bool overflow = false;
+            // walks of a valid tree end long before these budgets; they only bound a malformed one
+            uint32_t climb = 0, steps = 0;
+            const uint32_t step_budget = 2u * n_nodes + 64u;
+            for (;;) {
+                sp = 0;
+                lo = 0;
+                push(make_uint2(__float_as_uint(area_diff), sib));
+                while (sp > 0) {
+                    if (++steps > step_budget) {
                         overflow = true;
                         break;
                     }
-                    const float child_area = reinsert_area + half_area(db);
-                    push(make_uint2(__float_as_uint(child_area), dst.first_index));
-                    push(make_uint2(__float_as_uint(child_area), dst.first_index + 1u));
+                    const uint2 top = pop();
+                    const float top_area = __uint_as_float(top.x);
+                    if (top_area - node_area <= best_area) continue;
+                    const Node32 dst = load_node(&nodes[top.y]);
+                    const Box db = node_box(dst);
+                    const float merged_area = half_area(box_union(db, aabb));
+                    const float reinsert_area = top_area - merged_area;
+                    if (reinsert_area > best_area) {
+                        best_to = top.y;
+                        best_area = reinsert_area;
+                    }
+                    if (dst.prim_count == 0u) {
+                        if (sp + 2 > sd) {
+                            overflow = true;
+                            break;
+                        }
+                        const float child_area = reinsert_area + half_area(db);
+                        push(make_uint2(__float_as_uint(child_area), dst.first_index));
+                        push(make_uint2(__float_as_uint(child_area), dst.first_index + 1u));
+                    }
                 }
+                if (overflow) break;
+                if (pivot_id != parent_id) {
+                    pivot_bbox = box_union(pivot_bbox, node_box(load_node(&nodes[sib])));
+                    area_diff += half_area(node_box(load_node(&nodes[pivot_id]))) - half_area(pivot_bbox);
+                }
+                if (pivot_id == 0u) break;
+                if (++climb > sd) {
+                    overflow = true;
+                    break;
+                }
+                sib = sibling_of(pivot_id);
+                pivot_id = parents[pivot_id];
             }
-            if (overflow) break;
-            if (pivot_id != parent_id) {
-                pivot_bbox = box_union(pivot_bbox, node_box(load_node(&nodes[sib])));
-                area_diff += half_area(node_box(load_node(&nodes[pivot_id]))) - half_area(pivot_bbox);
+            if (overflow) atomicOr(&misc->err, DERR_CAPACITY);
+            uint32_t from = node_id;
+            if (best_to == sibling_of(node_id) || best_to == parent_id) {
+                from = 0u;
+                best_to = 0u;
+                best_area = 0.0f;
             }
-            if (pivot_id == 0u) break;
-            if (++climb > sd) {
-                overflow = true;
-                break;
-            }
-            sib = sibling_of(pivot_id);
-            pivot_id = parents[pivot_id];
-        }
-        if (overflow) atomicOr(&misc->err, DERR_CAPACITY);
-        uint32_t from = node_id;
-        if (best_to == sibling_of(node_id) || best_to == parent_id) {
-            from = 0u;
-            best_to = 0u;
-            best_area = 0.0f;
-        }
-        r_from[t] = from;
-        r_to[t] = best_to;
-        const bool pos = best_area > 0.0f;
-        keys[t] = pos ? (uint64_t)(~f32_order_key(best_area)) : 0xffffffffull;
-        const uint64_t m = __ballot(pos);
-        if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1) && m)
-            atomicAdd(&misc->positive, (uint32_t)__popcll(m));
-    }
+            r_from[t] = from;
+            r_to[t] = best_to;
+            const bool pos = best_area > 0.0f;
+            keys[t] = pos ? (uint64_t)(~f32_order_key(best_area)) : 0xffffffffull;
+            const uint64_t m = __ballot(pos);
+            if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1) && m)
+                atomicAdd(&misc->positive, (uint32_t)__popcll(m));
+        }  // t < count
+        uint32_t nx = 0u;
+        if ((threadIdx.x & 63u) == 0u) nx = nthreads + atomicAdd(&misc->next, 64u);
+        span = (uint32_t)__shfl((int)nx, 0);
+    }  // spans of this wave
 }
 
 constexpr uint32_t ST_UNDECIDED = 0u, ST_ACCEPTED_NEW = 1u, ST_ACCEPTED = 2u, ST_REJECTED = 3u;
