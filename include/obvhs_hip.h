@@ -110,10 +110,10 @@ obvhs_status obvhs_timer_start(obvhs_context* ctx);
 obvhs_status obvhs_timer_stop(obvhs_context* ctx, float* ms);
 
 /* ---- PLOC BVH2 build ----
- * Replaces PlocBuilder::build / build_with_bvh (ploc/mod.rs:95-113, 129-159) for T = Aabb, with
- * SortPrecision::U64.  d_aabbs: n primitive AABBs (device).  d_indices: the `indices` vector (device), or NULL
- * for 0..n.  `reuse` may be a handle from an earlier build whose allocations are reused (build_with_bvh);
- * pass NULL to allocate a new handle. */
+ * Replaces PlocBuilder::build / build_with_bvh (ploc/mod.rs:95-113, 129-159) for T = Aabb.  sort_precision 64
+ * (SortPrecision::U64, 63-bit keys) or 128 (U128, 126-bit keys).  d_aabbs: n primitive AABBs (device).
+ * d_indices: the `indices` vector (device), or NULL for 0..n.  `reuse` may be a handle from an earlier build
+ * whose allocations are reused (build_with_bvh); pass NULL to allocate a new handle. */
 obvhs_status obvhs_ploc_build_aabbs(obvhs_context* ctx, const obvhs_aabb* d_aabbs, const uint32_t* d_indices,
                                     uint32_t n, uint32_t search_distance, uint32_t sort_precision,
                                     uint32_t search_depth_threshold, obvhs_bvh2* reuse, obvhs_bvh2** out);
@@ -128,7 +128,8 @@ obvhs_status obvhs_bvh2_copy_nodes(obvhs_context* ctx, const obvhs_bvh2* bvh, ob
 obvhs_status obvhs_bvh2_copy_nodes_small(obvhs_context* ctx, const obvhs_bvh2* bvh, obvhs_bvh2_node_small* host_out);
 obvhs_status obvhs_bvh2_copy_primitive_indices(obvhs_context* ctx, const obvhs_bvh2* bvh, uint32_t* host_out);
 /* Debug/parity outputs of the last build into this handle: Morton codes in original leaf order and the sorted
- * permutation (sorted position -> leaf index), ploc/mod.rs:782-846. */
+ * permutation (sorted position -> leaf index), ploc/mod.rs:782-846.  host_codes holds prim_count u64 after a U64
+ * build and 2 * prim_count u64 after a U128 build (each code as its low word, then its high word). */
 obvhs_status obvhs_bvh2_copy_sort_trace(obvhs_context* ctx, const obvhs_bvh2* bvh, uint64_t* host_codes,
                                         uint32_t* host_sorted_index);
 obvhs_status obvhs_bvh2_destroy(obvhs_bvh2* bvh);
@@ -191,17 +192,18 @@ obvhs_status obvhs_bvh2_primitives_to_nodes(obvhs_context* ctx, const obvhs_bvh2
  * (the crate requires init_parents_if_uninit first). */
 obvhs_status obvhs_bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_leaf_ids,
                                            uint32_t count, uint8_t* d_flags);
-/* PlocBuilder::partial_rebuild(bvh, |i| d_flags[i] != 0, search_distance, SortPrecision::U64,
- * search_depth_threshold) (ploc/rebuild.rs:98-134): the flagged inner nodes under the root are opened, the leaves
- * and unflagged subtrees they expose are rebuilt into a new top with PLOC, and the new nodes go to the freed sibling
- * pairs, highest first.  Subtrees that were not opened keep their slots.  Clusters with equal Morton codes are
+/* PlocBuilder::partial_rebuild(bvh, |i| d_flags[i] != 0, search_distance, sort_precision,
+ * search_depth_threshold) (ploc/rebuild.rs:98-134), sort_precision 64 or 128: the flagged inner nodes under the
+ * root are opened, the leaves and unflagged subtrees they expose are rebuilt into a new top with PLOC, and the new
+ * nodes go to the freed sibling pairs, highest first.  Subtrees that were not opened keep their slots.  Clusters with equal Morton codes are
  * ordered by their old slot (the crate's unstable sort leaves that order open). */
 obvhs_status obvhs_bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,
-                                        uint32_t search_distance, uint32_t search_depth_threshold);
-/* PlocBuilder::full_rebuild(bvh, search_distance, SortPrecision::U64, search_depth_threshold)
+                                        uint32_t search_distance, uint32_t sort_precision,
+                                        uint32_t search_depth_threshold);
+/* PlocBuilder::full_rebuild(bvh, search_distance, sort_precision, search_depth_threshold)
  * (ploc/rebuild.rs:53-79): every leaf, in slot order, rebuilt with PLOC; the root's stored box is the scene box. */
 obvhs_status obvhs_bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance,
-                                     uint32_t search_depth_threshold);
+                                     uint32_t sort_precision, uint32_t search_depth_threshold);
 /* The pre-split leaf set of build_cwbvh_from_tris with pre_split = true (cwbvh/builder.rs:26-55 +
  * split_aabbs_preset, splits.rs:16-33): writes out_count boxes and triangle indices to caller device arrays of
  * out_capacity entries (OBVHS_ERR_CAPACITY with *out_count set when they do not fit).  max_total as
@@ -244,7 +246,8 @@ obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cw
 /* ---- batched build / traversal (many independent meshes in shared launches) ----
  * No single reference entry point: equivalent to calling build_cwbvh_from_tris (cwbvh/builder.rs:20-85) once per
  * mesh, with identical per-mesh results.  d_tris: concatenated triangles (device); tri_offsets: host array of
- * mesh_count + 1 prefix offsets (tri_offsets[0] == 0).  Pre-splits and reinsertion are not supported. */
+ * mesh_count + 1 prefix offsets (tri_offsets[0] == 0).  Pre-splits, reinsertion and SortPrecision::U128 are not
+ * supported (OBVHS_ERR_INVALID_ARG). */
 obvhs_status obvhs_batch_build_cwbvh(obvhs_context* ctx, const obvhs_triangle* d_tris, const uint32_t* tri_offsets,
                                      uint32_t mesh_count, const obvhs_build_params* params, obvhs_batch* reuse,
                                      obvhs_batch** out);

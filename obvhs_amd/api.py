@@ -190,6 +190,7 @@ class Bvh2:
         self.handle = handle
         self.owned = owned
         self._keepalive = keepalive  # the CwBvh owning a borrowed BVH2
+        self.sort_precision = SortPrecision.U64  # key width of the last PlocBuilder build into this handle
 
     def _info(self):
         nc, pc, md = C.c_uint32(), C.c_uint32(), C.c_uint32()
@@ -229,9 +230,11 @@ class Bvh2:
         return out
 
     def sort_trace(self):
-        """(Morton codes in leaf order, sorted position -> leaf index) of the last build (context trace on)."""
+        """(Morton codes in leaf order, sorted position -> leaf index) of the last build (context trace on).
+        The codes are u64[n] after a SortPrecision::U64 build and u64[n, 2] (low, high words) after U128."""
         n = self._info()[1]
-        codes = np.zeros(n, dtype=np.uint64)
+        wide = self.sort_precision == SortPrecision.U128
+        codes = np.zeros((n, 2) if wide else n, dtype=np.uint64)
         order = np.zeros(n, dtype=np.uint32)
         check(lib().obvhs_bvh2_copy_sort_trace(self.ctx.handle, self.handle, codes.ctypes.data_as(C.c_void_p),
                                                order.ctypes.data_as(C.c_void_p)), self.ctx.handle)
@@ -321,18 +324,19 @@ class PlocBuilder:
         out = C.c_void_p()
         fn = lib().obvhs_ploc_build_triangles if is_tris else lib().obvhs_ploc_build_aabbs
         reuse = bvh.handle if bvh is not None else None
-        check(fn(self.ctx.handle, d.ptr, di.ptr if di is not None else None, n, int(sd), int(sort_precision),
+        sp = SortPrecision(sort_precision)
+        check(fn(self.ctx.handle, d.ptr, di.ptr if di is not None else None, n, int(sd), int(sp),
                  int(search_depth_threshold), reuse, C.byref(out)), self.ctx.handle)
-        if bvh is not None:
-            return bvh
-        return Bvh2(self.ctx, out)
+        if bvh is None:
+            bvh = Bvh2(self.ctx, out)
+        bvh.sort_precision = sp
+        return bvh
 
     def full_rebuild(self, bvh, search_distance, sort_precision=SortPrecision.U64, search_depth_threshold=0):
         """ploc/rebuild.rs:53-79: rebuild `bvh` from its current leaves."""
         sd = PlocSearchDistance.from_u32(int(search_distance))
-        _check_u64(sort_precision)
-        check(lib().obvhs_bvh2_full_rebuild(self.ctx.handle, bvh.handle, int(sd), int(search_depth_threshold)),
-              self.ctx.handle)
+        check(lib().obvhs_bvh2_full_rebuild(self.ctx.handle, bvh.handle, int(sd), int(SortPrecision(sort_precision)),
+                                            int(search_depth_threshold)), self.ctx.handle)
         return bvh
 
     def partial_rebuild(self, bvh, should_remove, search_distance, sort_precision=SortPrecision.U64,
@@ -340,7 +344,7 @@ class PlocBuilder:
         """ploc/rebuild.rs:98-134.  `should_remove` is one flag per node (a device u8 array such as
         compute_rebuild_path_flags returns, or a host bool/u8 array): the closure |i| flags[i]."""
         sd = PlocSearchDistance.from_u32(int(search_distance))
-        _check_u64(sort_precision)
+        sp = SortPrecision(sort_precision)
         n = bvh.node_count
         if isinstance(should_remove, DeviceArray):
             flags = should_remove
@@ -351,14 +355,9 @@ class PlocBuilder:
             if host.size != n:
                 raise ValueError("one flag per node")
             flags = DeviceArray.from_host(self.ctx, host if n else np.zeros(1, np.uint8))
-        check(lib().obvhs_bvh2_partial_rebuild(self.ctx.handle, bvh.handle, flags.ptr, int(sd),
+        check(lib().obvhs_bvh2_partial_rebuild(self.ctx.handle, bvh.handle, flags.ptr, int(sd), int(sp),
                                                int(search_depth_threshold)), self.ctx.handle)
         return bvh
-
-
-def _check_u64(sort_precision):
-    if SortPrecision(sort_precision) != SortPrecision.U64:
-        raise ValueError("only SortPrecision::U64 is implemented")
 
 
 def compute_rebuild_path_flags(bvh, leaves):

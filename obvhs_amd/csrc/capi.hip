@@ -104,6 +104,7 @@ obvhs_status obvhs_context_destroy(obvhs_context* ctx) {
     S.clusters_a.release();
     S.clusters_b.release();
     S.keys.release();
+    S.keys_hi.release();
     S.vals.release();
     S.merge.release();
     S.blocksums.release();
@@ -202,12 +203,12 @@ static obvhs_status ploc_entry(obvhs_context* ctx, const obvhs_triangle* d_tris,
     if (!ctx || !out) return OBVHS_ERR_INVALID_ARG;
     if (n > 0 && !d_tris && !d_aabbs) return OBVHS_ERR_INVALID_ARG;
     if (n >= (1u << 31)) return OBVHS_ERR_INVALID_ARG;
-    if (sort_precision != 64u) return OBVHS_ERR_INVALID_ARG;  // SortPrecision::U64 only in this version
+    if (sort_precision != 64u && sort_precision != 128u) return OBVHS_ERR_INVALID_ARG;  // SortPrecision
     CTX_TRY(hipSetDevice(ctx->device), "set device");
     obvhs_bvh2* bvh = reuse ? reuse : new (std::nothrow) obvhs_bvh2();
     if (!bvh) return OBVHS_ERR_OOM;
-    obvhs_status s = ploc_build(ctx, d_tris, d_aabbs, d_indices, n, search_distance, search_depth_threshold,
-                                fused_max_prims, bvh);
+    obvhs_status s = ploc_build(ctx, d_tris, d_aabbs, d_indices, n, search_distance, sort_precision == 128u,
+                                search_depth_threshold, fused_max_prims, bvh);
     if (s != OBVHS_OK) {
         if (!reuse) obvhs_bvh2_destroy(bvh);
         *out = nullptr;
@@ -276,7 +277,16 @@ obvhs_status obvhs_bvh2_copy_sort_trace(obvhs_context* ctx, const obvhs_bvh2* bv
     if (!bvh->prim_count) return OBVHS_OK;
     if (!bvh->codes.ptr || !bvh->sorted_index.ptr) return OBVHS_ERR_INVALID_ARG;  // build without trace
     obvhs_status s = OBVHS_OK;
-    if (host_codes) s = obvhs_memcpy_dtoh(ctx, host_codes, bvh->codes.ptr, (size_t)bvh->prim_count * 8);
+    const size_t n = bvh->prim_count;
+    if (host_codes && bvh->code_words == 1u) s = obvhs_memcpy_dtoh(ctx, host_codes, bvh->codes.ptr, n * 8);
+    if (host_codes && bvh->code_words == 2u) {  // device: low words, then high words; host: (low, high) per leaf
+        std::vector<uint64_t> words(2 * n);
+        s = obvhs_memcpy_dtoh(ctx, words.data(), bvh->codes.ptr, n * 16);
+        for (size_t i = 0; s == OBVHS_OK && i < n; ++i) {
+            host_codes[2 * i] = words[i];
+            host_codes[2 * i + 1] = words[n + i];
+        }
+    }
     if (s == OBVHS_OK && host_sorted_index)
         s = obvhs_memcpy_dtoh(ctx, host_sorted_index, bvh->sorted_index.ptr, (size_t)bvh->prim_count * 4);
     return s;
@@ -457,17 +467,20 @@ obvhs_status obvhs_bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, 
 }
 
 obvhs_status obvhs_bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,
-                                        uint32_t search_distance, uint32_t search_depth_threshold) {
+                                        uint32_t search_distance, uint32_t sort_precision,
+                                        uint32_t search_depth_threshold) {
     if (!ctx || !bvh || (bvh->node_count && !d_flags)) return OBVHS_ERR_INVALID_ARG;
+    if (sort_precision != 64u && sort_precision != 128u) return OBVHS_ERR_INVALID_ARG;
     CTX_TRY(hipSetDevice(ctx->device), "set device");
-    return bvh2_partial_rebuild(ctx, bvh, d_flags, search_distance, search_depth_threshold);
+    return bvh2_partial_rebuild(ctx, bvh, d_flags, search_distance, sort_precision == 128u, search_depth_threshold);
 }
 
 obvhs_status obvhs_bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance,
-                                     uint32_t search_depth_threshold) {
+                                     uint32_t sort_precision, uint32_t search_depth_threshold) {
     if (!ctx || !bvh) return OBVHS_ERR_INVALID_ARG;
+    if (sort_precision != 64u && sort_precision != 128u) return OBVHS_ERR_INVALID_ARG;
     CTX_TRY(hipSetDevice(ctx->device), "set device");
-    return bvh2_full_rebuild(ctx, bvh, search_distance, search_depth_threshold);
+    return bvh2_full_rebuild(ctx, bvh, search_distance, sort_precision == 128u, search_depth_threshold);
 }
 
 obvhs_status obvhs_split_aabbs_preset(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,

@@ -46,6 +46,29 @@ __device__ __forceinline__ uint32_t f64_as_u32(double v) {
     return (uint32_t)v;
 }
 
+// Rust `f64 as u64` (saturating, NaN -> 0).
+__device__ __forceinline__ uint64_t f64_as_u64(double v) {
+    if (!(v >= 1.0)) return 0u;
+    if (v >= 18446744073709551616.0) return ~0ull;
+    return (uint64_t)v;
+}
+
+// 126-bit Morton code of the unit-cube point (px, py, pz) (morton_encode_u128_unorm, morton.rs:63-89) as its low
+// and high 64-bit words.  Each 42-bit coordinate is x = xh * 2^21 + xl; spreading puts bit i of x at 3i, so xl
+// fills bits [0, 63) and xh bits [63, 126): the code is code64(xl, yl, zl) + code64(xh, yh, zh) * 2^63.
+__device__ __forceinline__ void morton128_of_point(double px, double py, double pz, uint64_t& lo, uint64_t& hi) {
+    const double s = 4398046511104.0;  // 2^42
+    const uint64_t m = 0x3ffffffffffull;
+    const uint64_t x = f64_as_u64(px * s) & m, y = f64_as_u64(py * s) & m, z = f64_as_u64(pz * s) & m;
+    // split_by_3_u64 keeps the low 21 bits of its argument
+    const uint64_t a = split_by_3_u64((uint32_t)x) | split_by_3_u64((uint32_t)y) << 1 |
+                       split_by_3_u64((uint32_t)z) << 2;
+    const uint64_t b = split_by_3_u64((uint32_t)(x >> 21)) | split_by_3_u64((uint32_t)(y >> 21)) << 1 |
+                       split_by_3_u64((uint32_t)(z >> 21)) << 2;
+    lo = a | b << 63;
+    hi = b >> 1;
+}
+
 // Morton code of a leaf box under the scene normalisation (ploc/mod.rs:287-288, 782-785; morton.rs:55-58).
 __device__ __forceinline__ uint64_t morton_of_box(const Box& b, double sx, double sy, double sz, double ox, double oy,
                                                   double oz) {

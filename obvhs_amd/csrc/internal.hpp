@@ -41,6 +41,7 @@ struct BuildScratch {
     DevBuf clusters_a;  // Node32[n]
     DevBuf clusters_b;  // Node32[n]
     DevBuf keys;        // u64[n]
+    DevBuf keys_hi;     // u64[n] high words of SortPrecision::U128 codes
     DevBuf vals;        // u32[n]
     DevBuf merge;       // i8[n]
     DevBuf blocksums;   // uint2[n/256+1]
@@ -86,7 +87,8 @@ struct obvhs_context {
 struct obvhs_bvh2 {
     obvhs::DevBuf nodes;           // Node32[2n-1]
     obvhs::DevBuf prim_indices;    // u32[n]
-    obvhs::DevBuf codes;           // u64[n] Morton codes (original leaf order), for parity tests
+    obvhs::DevBuf codes;           // u64[n * code_words] Morton codes (original leaf order), for parity tests
+    uint32_t code_words = 1;       // 2 after a SortPrecision::U128 build: codes = low words, then high words
     obvhs::DevBuf sorted_index;    // u32[n]
     uint32_t node_count = 0;
     uint32_t prim_count = 0;
@@ -124,9 +126,10 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identi
 
 // ploc.hip
 obvhs_status ploc_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_src_slots, uint32_t k,
-                          const uint32_t* d_slot_map, uint32_t search_distance, uint32_t search_depth_threshold);
+                          const uint32_t* d_slot_map, uint32_t search_distance, bool u128,
+                          uint32_t search_depth_threshold);
 obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
-                        const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
+                        const uint32_t* d_indices, uint32_t n, uint32_t search_distance, bool u128,
                         uint32_t search_depth_threshold, uint32_t fused_max_prims, obvhs_bvh2* bvh);
 struct DpView;
 hipError_t bvh_dp_alloc(obvhs_bvh2* bvh);
@@ -196,8 +199,8 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
 obvhs_status bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_leaves, uint32_t count,
                                      uint8_t* d_flags);
 obvhs_status bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,
-                                  uint32_t search_distance, uint32_t search_depth_threshold);
-obvhs_status bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance,
+                                  uint32_t search_distance, bool u128, uint32_t search_depth_threshold);
+obvhs_status bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance, bool u128,
                                uint32_t search_depth_threshold);
 obvhs_status bvh2_set_node_aabbs(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_ids,
                                  const obvhs_aabb* d_aabbs, uint32_t count);

@@ -5,8 +5,8 @@
 //   k_leaf_init_*  : leaf Bvh2Node per primitive (ploc/mod.rs:187-243) + scene bounds (atomic min/max on
 //                    order-preserving keys; the zero sign of the bounds cannot change a Morton code)
 //   k_morton       : 63-bit Morton code of the f32 centre normalised in f64 (ploc/mod.rs:287-288, 782-785;
-//                    morton.rs:35-58)
-//   radix sort     : stable (sort.hip)
+//                    morton.rs:35-58), or the 126-bit code of SortPrecision::U128 (morton.rs:63-89) as two words
+//   radix sort     : stable (sort.hip); a U128 key is sorted by its low word, then stably by its high word
 //   k_gather       : sorted leaves become the first cluster array (ploc/mod.rs:829-846)
 //   per pass       : k_ploc_search (ploc/mod.rs:329-419; window in LDS), k_ploc_count + k_ploc_scan (prefix
 //                    counts of emitted clusters and of merges), k_ploc_compact (ploc/mod.rs:420-487: the parent
@@ -188,8 +188,10 @@ __device__ __forceinline__ bool child_slot(uint32_t v, const uint32_t* __restric
     return true;
 }
 
+// keys_hi = NULL: U64 codes into keys.  Otherwise U128 codes, low words into keys and high words into keys_hi.
 __global__ void __launch_bounds__(T256) k_morton(const Node32* __restrict__ leaves, uint32_t n,
-                                                 const Misc* __restrict__ misc, uint64_t* __restrict__ keys) {
+                                                 const Misc* __restrict__ misc, uint64_t* __restrict__ keys,
+                                                 uint64_t* __restrict__ keys_hi) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     if (i >= n) return;
     const float mnx = f32_from_order_key(misc->bounds[0]), mny = f32_from_order_key(misc->bounds[1]),
@@ -201,9 +203,21 @@ __global__ void __launch_bounds__(T256) k_morton(const Node32* __restrict__ leav
     Node32 l = load_node(&leaves[i]);
     const float cx = (l.maxx + l.minx) * 0.5f, cy = (l.maxy + l.miny) * 0.5f, cz = (l.maxz + l.minz) * 0.5f;
     const double px = (double)cx * sx + ox, py = (double)cy * sy + oy, pz = (double)cz * sz + oz;
+    if (keys_hi) {
+        morton128_of_point(px, py, pz, keys[i], keys_hi[i]);
+        return;
+    }
     const double s = 2097152.0;
     keys[i] = split_by_3_u64(f64_as_u32(px * s)) | split_by_3_u64(f64_as_u32(py * s)) << 1 |
               split_by_3_u64(f64_as_u32(pz * s)) << 2;
+}
+
+__global__ void __launch_bounds__(T256) k_gather_u64(const uint64_t* __restrict__ src,
+                                                     const uint32_t* __restrict__ order, uint32_t n,
+                                                     uint64_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= n) return;
+    out[i] = src[order[i]];
 }
 
 __global__ void __launch_bounds__(T256) k_gather(const Node32* __restrict__ leaves,
@@ -642,6 +656,25 @@ void DevBuf::release() {
         if (_e != hipSuccess) return set_hip_error(ctx, _e, where);         \
     } while (0)
 
+// Sorts the leaf indices vals[0..n) (identity on entry) by their Morton codes, ties in leaf order
+// (ploc/mod.rs:811-827).  U64: one sort of the 63-bit codes in keys.  U128 (keys_hi != NULL): LSD over the whole
+// 126-bit key as a stable sort of the low words followed by a stable sort of the high words (62 bits) gathered in
+// that order; keys ends up holding the sorted high words.
+static obvhs_status sort_codes(obvhs_context* ctx, uint64_t* keys, const uint64_t* keys_hi, uint32_t* vals,
+                               uint32_t n) {
+    hipStream_t st = ctx->stream;
+    void* scratch = ctx->scratch.sort_scratch.ptr;
+    if (!keys_hi) {
+        OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 63, scratch, st), "radix sort");
+        return OBVHS_OK;
+    }
+    OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 64, scratch, st), "radix sort (low words)");
+    k_gather_u64<<<(n + T256 - 1) / T256, T256, 0, st>>>(keys_hi, vals, n, keys);
+    OBVHS_TRY(hipGetLastError(), "gather high words");
+    OBVHS_TRY(radix_sort_pairs(keys, vals, false, n, 62, scratch, st), "radix sort (high words)");
+    return OBVHS_OK;
+}
+
 // The PLOC passes from the sorted clusters in `cur` to the root (ploc/mod.rs:327-499): grid passes while more than
 // TAIL_MAX clusters remain, then the one-workgroup tail.  `fin` receives the final pass state.
 static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uint32_t n, uint32_t r,
@@ -698,7 +731,7 @@ static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uin
 }
 
 obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const obvhs_aabb* d_aabbs,
-                        const uint32_t* d_indices, uint32_t n, uint32_t search_distance,
+                        const uint32_t* d_indices, uint32_t n, uint32_t search_distance, bool u128,
                         uint32_t search_depth_threshold, uint32_t fused_max_prims, obvhs_bvh2* bvh) {
     if (!(search_distance == 1 || search_distance == 2 || search_distance == 6 || search_distance == 14 ||
           search_distance == 24 || search_distance == 32))
@@ -726,6 +759,7 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
     OBVHS_TRY(S.clusters_a.ensure((size_t)n * sizeof(Node32)), "alloc clusters");
     OBVHS_TRY(S.clusters_b.ensure((size_t)n * sizeof(Node32)), "alloc clusters");
     OBVHS_TRY(S.keys.ensure((size_t)n * 8), "alloc keys");
+    if (u128) OBVHS_TRY(S.keys_hi.ensure((size_t)n * 8), "alloc high key words");
     OBVHS_TRY(S.vals.ensure((size_t)n * 4), "alloc vals");
     OBVHS_TRY(S.merge.ensure((size_t)n + 64), "alloc merge");
     const uint32_t nblk = (n + T256 - 1) / T256;
@@ -736,6 +770,7 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
     Node32* leaves = S.leaves.as<Node32>();
     Node32* bvh_nodes = bvh->nodes.as<Node32>();
     uint64_t* keys = S.keys.as<uint64_t>();
+    uint64_t* keys_hi = u128 ? S.keys_hi.as<uint64_t>() : nullptr;
     uint32_t* vals = S.vals.as<uint32_t>();
     uint32_t* hist = bvh->hist.as<uint32_t>();
     // per-block bound partials share the blocksums allocation (they are dead before the first pass)
@@ -755,13 +790,19 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
                                                  misc, partials);
     }
     k_reduce_bounds<<<1, 1024, 0, st>>>(partials, nblk, misc);
-    k_morton<<<nblk, T256, 0, st>>>(leaves, n, misc, keys);
+    k_morton<<<nblk, T256, 0, st>>>(leaves, n, misc, keys, keys_hi);
     OBVHS_TRY(hipGetLastError(), "leaf/morton launch");
-    if (ctx->keep_trace) {
-        OBVHS_TRY(bvh->codes.ensure((size_t)n * 8), "alloc codes");
+    bvh->code_words = u128 ? 2u : 1u;
+    if (ctx->keep_trace) {  // low words, then the high words of U128 codes
+        OBVHS_TRY(bvh->codes.ensure((size_t)n * 8 * bvh->code_words), "alloc codes");
         OBVHS_TRY(hipMemcpyAsync(bvh->codes.ptr, keys, (size_t)n * 8, hipMemcpyDeviceToDevice, st), "copy codes");
+        if (u128)
+            OBVHS_TRY(hipMemcpyAsync(bvh->codes.as<uint64_t>() + n, keys_hi, (size_t)n * 8, hipMemcpyDeviceToDevice,
+                                     st),
+                      "copy codes");
     }
-    OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 63, S.sort_scratch.ptr, st), "radix sort");
+    obvhs_status ss = sort_codes(ctx, keys, keys_hi, vals, n);
+    if (ss != OBVHS_OK) return ss;
     if (ctx->keep_trace) {
         OBVHS_TRY(bvh->sorted_index.ensure((size_t)n * 4), "alloc sorted index");
         OBVHS_TRY(hipMemcpyAsync(bvh->sorted_index.ptr, vals, (size_t)n * 4, hipMemcpyDeviceToDevice, st),
@@ -789,7 +830,8 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
 // the plain placement (children after parents).  Otherwise slot_map[g] (k-1 entries, descending) is the left slot
 // of the g-th freed sibling pair; every other slot keeps its node.  The scene box is the root's stored box.
 obvhs_status ploc_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_src_slots, uint32_t k,
-                          const uint32_t* d_slot_map, uint32_t search_distance, uint32_t search_depth_threshold) {
+                          const uint32_t* d_slot_map, uint32_t search_distance, bool u128,
+                          uint32_t search_depth_threshold) {
     if (!(search_distance == 1 || search_distance == 2 || search_distance == 6 || search_distance == 14 ||
           search_distance == 24 || search_distance == 32))
         return OBVHS_ERR_INVALID_ARG;
@@ -802,6 +844,7 @@ obvhs_status ploc_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d
     OBVHS_TRY(S.clusters_a.ensure((size_t)k * sizeof(Node32)), "alloc clusters");
     OBVHS_TRY(S.clusters_b.ensure((size_t)k * sizeof(Node32)), "alloc clusters");
     OBVHS_TRY(S.keys.ensure((size_t)k * 8), "alloc keys");
+    if (u128) OBVHS_TRY(S.keys_hi.ensure((size_t)k * 8), "alloc high key words");
     OBVHS_TRY(S.vals.ensure((size_t)k * 4), "alloc vals");
     OBVHS_TRY(S.merge.ensure((size_t)k + 64), "alloc merge");
     const uint32_t nblk = (k + T256 - 1) / T256;
@@ -812,15 +855,17 @@ obvhs_status ploc_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d
     Node32* leaves = S.leaves.as<Node32>();
     Node32* bvh_nodes = bvh->nodes.as<Node32>();
     uint64_t* keys = S.keys.as<uint64_t>();
+    uint64_t* keys_hi = u128 ? S.keys_hi.as<uint64_t>() : nullptr;
     uint32_t* vals = S.vals.as<uint32_t>();
     uint32_t* hist = bvh->hist.as<uint32_t>();
     // The clusters are copied out first: merges may overwrite the freed slots they came from.
     k_gather<<<nblk, T256, 0, st>>>(bvh_nodes, d_src_slots, k, leaves);
     k_init_misc<<<1, 64, 0, st>>>(misc, k, hist);
     k_bounds_from_node<<<1, 64, 0, st>>>(misc, bvh_nodes);
-    k_morton<<<nblk, T256, 0, st>>>(leaves, k, misc, keys);
+    k_morton<<<nblk, T256, 0, st>>>(leaves, k, misc, keys, keys_hi);
     OBVHS_TRY(hipGetLastError(), "rebuild morton launch");
-    OBVHS_TRY(radix_sort_pairs(keys, vals, true, k, 63, S.sort_scratch.ptr, st), "radix sort");
+    obvhs_status ss = sort_codes(ctx, keys, keys_hi, vals, k);
+    if (ss != OBVHS_OK) return ss;
     Node32* cur = S.clusters_a.as<Node32>();
     Node32* nxt = S.clusters_b.as<Node32>();
     k_gather<<<nblk, T256, 0, st>>>(leaves, vals, k, cur);

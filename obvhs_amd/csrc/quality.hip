@@ -1115,7 +1115,7 @@ static bool valid_search_distance(uint32_t r) {
 
 // PlocBuilder::partial_rebuild (ploc/rebuild.rs:98-134) with should_remove(i) = d_flags[i] != 0.
 obvhs_status bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,
-                                  uint32_t search_distance, uint32_t search_depth_threshold) {
+                                  uint32_t search_distance, bool u128, uint32_t search_depth_threshold) {
     if (!valid_search_distance(search_distance)) return OBVHS_ERR_INVALID_ARG;
     const uint32_t N = bvh->node_count;
     if (N < 2) return OBVHS_OK;  // rebuild.rs:106-108
@@ -1140,12 +1140,12 @@ obvhs_status bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uin
     if (s != OBVHS_OK) return s;
     // The visited slots form whole sibling pairs at (odd, odd + 1), one pair per merge to come.
     if (k < 2 || visited != 2 * k - 2 || f != k - 1) return OBVHS_ERR_INVALID_ARG;
-    return ploc_rebuild(ctx, bvh, Q.kept.as<uint32_t>(), k, Q.freed.as<uint32_t>(), search_distance,
+    return ploc_rebuild(ctx, bvh, Q.kept.as<uint32_t>(), k, Q.freed.as<uint32_t>(), search_distance, u128,
                         search_depth_threshold);
 }
 
 // PlocBuilder::full_rebuild (ploc/rebuild.rs:53-79).
-obvhs_status bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance,
+obvhs_status bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t search_distance, bool u128,
                                uint32_t search_depth_threshold) {
     if (!valid_search_distance(search_distance)) return OBVHS_ERR_INVALID_ARG;
     const uint32_t N = bvh->node_count;
@@ -1158,7 +1158,8 @@ obvhs_status bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32_t sea
     uint32_t k = 0, f = 0;
     obvhs_status s = rebuild_collect(ctx, N, k, f);
     if (s != OBVHS_OK) return s;
-    return ploc_rebuild(ctx, bvh, Q.kept.as<uint32_t>(), k, nullptr, search_distance, search_depth_threshold);
+    return ploc_rebuild(ctx, bvh, Q.kept.as<uint32_t>(), k, nullptr, search_distance, u128,
+                        search_depth_threshold);
 }
 
 obvhs_status bvh2_set_node_aabbs(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_ids,

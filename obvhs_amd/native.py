@@ -131,8 +131,8 @@ SIGNATURES = {
     "obvhs_bvh2_set_node_aabbs": (INT, [P, P, P, P, U32]),
     "obvhs_bvh2_primitives_to_nodes": (INT, [P, P, P]),
     "obvhs_bvh2_rebuild_path_flags": (INT, [P, P, P, U32, P]),
-    "obvhs_bvh2_partial_rebuild": (INT, [P, P, P, U32, U32]),
-    "obvhs_bvh2_full_rebuild": (INT, [P, P, U32, U32]),
+    "obvhs_bvh2_partial_rebuild": (INT, [P, P, P, U32, U32, U32]),
+    "obvhs_bvh2_full_rebuild": (INT, [P, P, U32, U32, U32]),
     "obvhs_split_aabbs_preset": (INT, [P, P, U32, U64, P, P, U32, PU32, C.POINTER(C.c_float),
                                        C.POINTER(C.c_float)]),
     "obvhs_server_create": (INT, [INT, C.POINTER(ServerConfig), C.POINTER(P)]),

@@ -183,6 +183,35 @@ inline uint64_t morton_encode_u64_unorm(double px, double py, double pz) {
     return morton_encode_u64(f64_as_u32(px * s), f64_as_u32(py * s), f64_as_u32(pz * s));
 }
 
+typedef unsigned __int128 uint128;
+inline uint128 u128_from(uint64_t hi, uint64_t lo) { return (uint128)hi << 64 | (uint128)lo; }
+// morton.rs:66-75 (42 bits per axis; each mask is the reference's 128-bit constant as high and low words)
+inline uint128 split_by_3_u128(uint64_t a) {
+    uint128 x = (uint128)a & 0x3ffffffffffull;
+    x = (x | x << 64) & u128_from(0x3ff00000000ull, 0xffffffffull);
+    x = (x | x << 32) & u128_from(0x3ff00000000ull, 0xffff00000000ffffull);
+    x = (x | x << 16) & u128_from(0x30000ff0000ff00ull, 0xff0000ff0000ffull);
+    x = (x | x << 8) & u128_from(0x300f00f00f00f00ull, 0xf00f00f00f00f00full);
+    x = (x | x << 4) & u128_from(0x30c30c30c30c30cull, 0x30c30c30c30c30c3ull);
+    x = (x | x << 2) & u128_from(0x924924924924924ull, 0x9249249249249249ull);
+    return x;
+}
+// morton.rs:79-81
+inline uint128 morton_encode_u128(uint64_t x, uint64_t y, uint64_t z) {
+    return split_by_3_u128(x) | split_by_3_u128(y) << 1 | split_by_3_u128(z) << 2;
+}
+// Rust `f64 as u64`: saturating, NaN -> 0.
+inline uint64_t f64_as_u64(double v) {
+    if (!(v >= 1.0)) return 0u;
+    if (v >= 18446744073709551616.0) return ~0ull;
+    return (uint64_t)v;
+}
+// morton.rs:85-88
+inline uint128 morton_encode_u128_unorm(double px, double py, double pz) {
+    const double s = (double)(1ull << 42);
+    return morton_encode_u128(f64_as_u64(px * s), f64_as_u64(py * s), f64_as_u64(pz * s));
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // PLOC (ploc/mod.rs)
 // ---------------------------------------------------------------------------------------------------------------
@@ -238,13 +267,16 @@ struct SearchCache {
     }
 };
 
-struct Morton64 {
-    uint64_t code;
+// Morton64 (ploc/mod.rs:705-709) and Morton128 (ploc/mod.rs:686-692) as one record: U64 keys leave `hi` zero.
+struct MortonKey {
+    uint64_t hi;
+    uint64_t lo;
     uint64_t index;
 };
 
 struct PlocTrace {
-    uint64_t* codes;           // [n] Morton codes in original leaf order (optional)
+    uint64_t* codes;           // [n] Morton codes (the low 64 bits for U128) in original leaf order (optional)
+    uint64_t* codes_hi;        // [n] high 64 bits of the U128 codes (optional; untouched for U64)
     uint32_t* sorted_index;    // [n] sorted position -> original leaf index (optional)
     uint32_t* iter_counts;     // [cap] cluster count at the start of each iteration (optional)
     uint32_t iter_cap;
@@ -262,9 +294,10 @@ inline bool node_valid(const Bvh2Node& n) { return (n.prim_count & NODE_INVALID_
 //   rebuild  : the REBUILD = true placement (ploc/mod.rs:449-462): every merge scans down from the array end for
 //              the next sibling pair whose left slot is invalid.  Otherwise children fill nodes[2n-3 ..] downwards
 //              and the array is exactly 2n-1 long.
+//   u128     : SortPrecision::U128 (126-bit keys, morton_encode_u128_unorm) instead of U64.
 // `nodes` has nodes_len entries; out_depth receives the pass count.
 int ploc_from_leaves(std::vector<Bvh2Node>& current, const uint32_t* tie, const Aabb& total, size_t sd,
-                     size_t search_depth_threshold, bool basic_search, bool rebuild, Bvh2Node* out_nodes,
+                     size_t search_depth_threshold, bool basic_search, bool rebuild, bool u128, Bvh2Node* out_nodes,
                      size_t nodes_len, PlocTrace* tr, size_t* out_depth) {
     const size_t n = current.size();
     if (out_depth) *out_depth = 0;
@@ -289,22 +322,29 @@ int ploc_from_leaves(std::vector<Bvh2Node>& current, const uint32_t* tie, const 
                         -(double)total.min.z * scale[2]};
 
     // ploc/mod.rs:782-809 gen_mort
-    std::vector<Morton64> mortons(n);
+    std::vector<MortonKey> mortons(n);
     for (size_t i = 0; i < n; ++i) {
         const Aabb& a = current[i].aabb;
         float cx = (a.max.x + a.min.x) * 0.5f, cy = (a.max.y + a.min.y) * 0.5f, cz = (a.max.z + a.min.z) * 0.5f;
         double px = (double)cx * scale[0] + offset[0];
         double py = (double)cy * scale[1] + offset[1];
         double pz = (double)cz * scale[2] + offset[2];
-        mortons[i] = Morton64{morton_encode_u64_unorm(px, py, pz), (uint64_t)i};
-        if (tr && tr->codes) tr->codes[i] = mortons[i].code;
+        if (u128) {  // Morton128::new (ploc/mod.rs:727-733)
+            const uint128 c = morton_encode_u128_unorm(px, py, pz);
+            mortons[i] = MortonKey{(uint64_t)(c >> 64), (uint64_t)c, (uint64_t)i};
+            if (tr && tr->codes_hi) tr->codes_hi[i] = mortons[i].hi;
+        } else {     // Morton64::new (ploc/mod.rs:749-755)
+            mortons[i] = MortonKey{0u, morton_encode_u64_unorm(px, py, pz), (uint64_t)i};
+        }
+        if (tr && tr->codes) tr->codes[i] = mortons[i].lo;
     }
     // ploc/mod.rs:811-827. The reference uses unstable sorts; ties are ordered by the tie key here (the original
     // index for a build, the node slot for a partial rebuild).
     const bool ties_desc = g_mut == MUT_SORT_TIES_DESC;
-    std::sort(mortons.begin(), mortons.end(), [ties_desc, tie](const Morton64& a, const Morton64& b) {
+    std::sort(mortons.begin(), mortons.end(), [ties_desc, tie](const MortonKey& a, const MortonKey& b) {
         const uint64_t ta = tie ? tie[a.index] : a.index, tb = tie ? tie[b.index] : b.index;
-        return a.code < b.code || (a.code == b.code && (ties_desc ? ta > tb : ta < tb));
+        if (a.hi != b.hi) return a.hi < b.hi;
+        return a.lo < b.lo || (a.lo == b.lo && (ties_desc ? ta > tb : ta < tb));
     });
     // ploc/mod.rs:829-846 remap, then swap (ploc/mod.rs:305)
     for (size_t k = 0; k < n; ++k) {
@@ -415,7 +455,7 @@ int ploc_from_leaves(std::vector<Bvh2Node>& current, const uint32_t* tie, const 
 
 // ploc/mod.rs:169-251 (build_ploc): leaves from the boxes, scene bounds, then build_ploc_from_leaves.
 int ploc_build(const Aabb* aabbs, const uint32_t* indices, size_t n, size_t sd, size_t search_depth_threshold,
-               bool basic_search, Bvh2Node* out_nodes, uint32_t* out_prim_indices, PlocTrace* tr) {
+               bool basic_search, bool u128, Bvh2Node* out_nodes, uint32_t* out_prim_indices, PlocTrace* tr) {
     if (sd < 1 || sd > 32) return ORC_ERR_INVALID_ARG;
     for (size_t i = 0; i < n; ++i) out_prim_indices[i] = indices[i];  // reset_for_reuse (bvh2/mod.rs:106-121)
     if (tr) {
@@ -437,8 +477,8 @@ int ploc_build(const Aabb* aabbs, const uint32_t* indices, size_t n, size_t sd, 
     }
     const size_t nodes_count = 2 * n - 1;
     for (size_t i = 0; i < nodes_count; ++i) out_nodes[i] = Bvh2Node{};
-    return ploc_from_leaves(current, nullptr, total, sd, search_depth_threshold, basic_search, false, out_nodes,
-                            nodes_count, tr, nullptr);
+    return ploc_from_leaves(current, nullptr, total, sd, search_depth_threshold, basic_search, false, u128,
+                            out_nodes, nodes_count, tr, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1385,7 +1425,8 @@ int rebuild_path_flags(const std::vector<Bvh2Node>& nodes, const uint32_t* leave
 // is), a flagged inner node is opened.  Every visited slot is marked invalid, and build_ploc_from_leaves::<REBUILD>
 // refills exactly those slots.  The scene box is the root's stored box.  Clusters with equal Morton codes are
 // ordered by their old slot.
-int partial_rebuild(std::vector<Bvh2Node>& nodes, const uint8_t* flags, size_t sd, size_t sdt, size_t* out_depth) {
+int partial_rebuild(std::vector<Bvh2Node>& nodes, const uint8_t* flags, size_t sd, bool u128, size_t sdt,
+                    size_t* out_depth) {
     if (out_depth) *out_depth = 0;
     if (nodes.size() < 2) return ORC_OK;
     std::vector<Bvh2Node> current;
@@ -1408,13 +1449,13 @@ int partial_rebuild(std::vector<Bvh2Node>& nodes, const uint8_t* flags, size_t s
         }
     }
     const Aabb total = nodes[0].aabb;  // rebuild.rs:154
-    return ploc_from_leaves(current, slot_of.data(), total, sd, sdt, false, true, nodes.data(), nodes.size(),
+    return ploc_from_leaves(current, slot_of.data(), total, sd, sdt, false, true, u128, nodes.data(), nodes.size(),
                             nullptr, out_depth);
 }
 
 // ploc/rebuild.rs:53-79 (full_rebuild) + rebuild_from_leaves::<false>: every leaf, in slot order, rebuilt with the
 // root's stored box as the scene box.  The node array becomes 2k-1 long for k leaves.
-int full_rebuild(std::vector<Bvh2Node>& nodes, size_t sd, size_t sdt, size_t* out_depth) {
+int full_rebuild(std::vector<Bvh2Node>& nodes, size_t sd, bool u128, size_t sdt, size_t* out_depth) {
     if (out_depth) *out_depth = 0;
     if (nodes.size() < 2) return ORC_OK;
     std::vector<Bvh2Node> current;
@@ -1423,8 +1464,8 @@ int full_rebuild(std::vector<Bvh2Node>& nodes, size_t sd, size_t sdt, size_t* ou
     const Aabb total = nodes[0].aabb;
     if (current.empty()) return ORC_OK;
     nodes.resize(2 * current.size() - 1);
-    return ploc_from_leaves(current, nullptr, total, sd, sdt, false, false, nodes.data(), nodes.size(), nullptr,
-                            out_depth);
+    return ploc_from_leaves(current, nullptr, total, sd, sdt, false, false, u128, nodes.data(), nodes.size(),
+                            nullptr, out_depth);
 }
 
 // bvh2/mod.rs:647-663 (compute_primitives_to_nodes): leaf slot of every primitive, INVALID where none.
@@ -1468,20 +1509,33 @@ void orc_tri_aabbs(const Triangle* tris, uint64_t n, Aabb* out) {
 
 uint64_t orc_morton_encode_u64_unorm(double x, double y, double z) { return morton_encode_u64_unorm(x, y, z); }
 uint64_t orc_split_by_3_u64(uint32_t a) { return split_by_3_u64(a); }
+// 128-bit results as {low 64 bits, high 64 bits}.
+void orc_split_by_3_u128(uint64_t a, uint64_t* out) {
+    const uint128 c = split_by_3_u128(a);
+    out[0] = (uint64_t)c;
+    out[1] = (uint64_t)(c >> 64);
+}
+void orc_morton_encode_u128_unorm(double x, double y, double z, uint64_t* out) {
+    const uint128 c = morton_encode_u128_unorm(x, y, z);
+    out[0] = (uint64_t)c;
+    out[1] = (uint64_t)(c >> 64);
+}
 uint32_t orc_f64_as_u32(double v) { return f64_as_u32(v); }
 float orc_half_area(const Aabb* a) { return half_area(*a); }
 
-// Full PLOC build (ploc/mod.rs:95-113 with indices 0..n supplied by the caller).
-// out_nodes: 2n-1 Bvh2Node (48 B layout), out_prim_indices: n.
+// Full PLOC build (ploc/mod.rs:95-113 with indices 0..n supplied by the caller); sort_precision 64 or 128.
+// out_nodes: 2n-1 Bvh2Node (48 B layout), out_prim_indices: n.  out_codes gets the U64 codes, or the low 64 bits
+// of the U128 codes with the high 64 bits in out_codes_hi.
 // Trace outputs may be null.  Returns an ORC_* code.
 int orc_ploc_build(const Aabb* aabbs, const uint32_t* indices, uint64_t n, uint32_t search_distance,
                    uint32_t search_depth_threshold, int basic_search, Bvh2Node* out_nodes,
                    uint32_t* out_prim_indices, uint64_t* out_codes, uint32_t* out_sorted_index,
                    uint32_t* out_iter_counts, uint32_t iter_cap, uint32_t* out_num_iters, uint32_t* out_depth,
-                   uint32_t* out_max_offset) {
-    PlocTrace tr{out_codes, out_sorted_index, out_iter_counts, iter_cap, 0, 0, 0};
+                   uint32_t* out_max_offset, uint32_t sort_precision, uint64_t* out_codes_hi) {
+    if (sort_precision != 64u && sort_precision != 128u) return ORC_ERR_INVALID_ARG;
+    PlocTrace tr{out_codes, out_codes_hi, out_sorted_index, out_iter_counts, iter_cap, 0, 0, 0};
     int rc = ploc_build(aabbs, indices, (size_t)n, search_distance, search_depth_threshold, basic_search != 0,
-                        out_nodes, out_prim_indices, &tr);
+                        sort_precision == 128u, out_nodes, out_prim_indices, &tr);
     if (out_num_iters) *out_num_iters = tr.num_iters;
     if (out_depth) *out_depth = tr.depth;
     if (out_max_offset) *out_max_offset = tr.max_offset_abs;
@@ -1582,24 +1636,27 @@ int orc_rebuild_path_flags(const Bvh2Node* nodes, uint64_t nnodes, const uint32_
     return rebuild_path_flags(v, leaves, (size_t)n_leaves, flags);
 }
 
-// PlocBuilder::partial_rebuild(bvh, |i| flags[i] != 0, search_distance, U64, search_depth_threshold) in place.
+// PlocBuilder::partial_rebuild(bvh, |i| flags[i] != 0, search_distance, sort_precision (64 or 128),
+// search_depth_threshold) in place.
 int orc_partial_rebuild(Bvh2Node* nodes, uint64_t nnodes, const uint8_t* flags, uint32_t search_distance,
-                        uint32_t search_depth_threshold, uint32_t* out_depth) {
+                        uint32_t search_depth_threshold, uint32_t* out_depth, uint32_t sort_precision) {
+    if (sort_precision != 64u && sort_precision != 128u) return ORC_ERR_INVALID_ARG;
     std::vector<Bvh2Node> v(nodes, nodes + nnodes);
     size_t depth = 0;
-    int rc = partial_rebuild(v, flags, search_distance, search_depth_threshold, &depth);
+    int rc = partial_rebuild(v, flags, search_distance, sort_precision == 128u, search_depth_threshold, &depth);
     std::memcpy(nodes, v.data(), nnodes * sizeof(Bvh2Node));
     if (out_depth) *out_depth = (uint32_t)depth;
     return rc;
 }
 
-// PlocBuilder::full_rebuild(bvh, search_distance, U64, search_depth_threshold) in place; *out_nnodes receives the
-// new node count (2k-1 for k leaves, equal to nnodes for a well-formed tree).
+// PlocBuilder::full_rebuild(bvh, search_distance, sort_precision (64 or 128), search_depth_threshold) in place;
+// *out_nnodes receives the new node count (2k-1 for k leaves, equal to nnodes for a well-formed tree).
 int orc_full_rebuild(Bvh2Node* nodes, uint64_t nnodes, uint32_t search_distance, uint32_t search_depth_threshold,
-                     uint64_t* out_nnodes, uint32_t* out_depth) {
+                     uint64_t* out_nnodes, uint32_t* out_depth, uint32_t sort_precision) {
+    if (sort_precision != 64u && sort_precision != 128u) return ORC_ERR_INVALID_ARG;
     std::vector<Bvh2Node> v(nodes, nodes + nnodes);
     size_t depth = 0;
-    int rc = full_rebuild(v, search_distance, search_depth_threshold, &depth);
+    int rc = full_rebuild(v, search_distance, sort_precision == 128u, search_depth_threshold, &depth);
     if (v.size() > nnodes) return ORC_ERR_CAPACITY;
     std::memcpy(nodes, v.data(), v.size() * sizeof(Bvh2Node));
     if (out_nnodes) *out_nnodes = v.size();

@@ -41,7 +41,9 @@ def lib():
         L.orc_half_area.argtypes = [P]
         L.orc_ploc_build.restype = C.c_int
         L.orc_ploc_build.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, P, P, P, P, P, C.c_uint32,
-                                     P, P, P]
+                                     P, P, P, C.c_uint32, P]
+        L.orc_split_by_3_u128.argtypes = [C.c_uint64, P]
+        L.orc_morton_encode_u128_unorm.argtypes = [C.c_double, C.c_double, C.c_double, P]
         L.orc_bvh2_to_cwbvh.restype = C.c_int
         L.orc_bvh2_to_cwbvh.argtypes = [P, C.c_uint64, P, C.c_uint64, C.c_uint32, C.c_int, P, C.c_uint64, P,
                                         C.c_uint64, P, P, P]
@@ -72,9 +74,9 @@ def lib():
         L.orc_rebuild_path_flags.restype = C.c_int
         L.orc_rebuild_path_flags.argtypes = [P, C.c_uint64, P, C.c_uint64, P]
         L.orc_partial_rebuild.restype = C.c_int
-        L.orc_partial_rebuild.argtypes = [P, C.c_uint64, P, C.c_uint32, C.c_uint32, P]
+        L.orc_partial_rebuild.argtypes = [P, C.c_uint64, P, C.c_uint32, C.c_uint32, P, C.c_uint32]
         L.orc_full_rebuild.restype = C.c_int
-        L.orc_full_rebuild.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, P, P]
+        L.orc_full_rebuild.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
         L.orc_primitives_to_nodes.restype = C.c_int
         L.orc_primitives_to_nodes.argtypes = [P, C.c_uint64, P, C.c_uint64, P]
         _lib = L
@@ -87,6 +89,20 @@ class OracleError(RuntimeError):
         super().__init__(f"oracle error code {code}")
 
 
+def split_by_3_u128(a):
+    """split_by_3_u128 (morton.rs:66-75) as a Python int."""
+    out = np.zeros(2, np.uint64)
+    lib().orc_split_by_3_u128(int(a), _ptr(out))
+    return int(out[0]) | int(out[1]) << 64
+
+
+def morton_encode_u128_unorm(x, y, z):
+    """morton_encode_u128_unorm (morton.rs:85-88) as a Python int."""
+    out = np.zeros(2, np.uint64)
+    lib().orc_morton_encode_u128_unorm(float(x), float(y), float(z), _ptr(out))
+    return int(out[0]) | int(out[1]) << 64
+
+
 def tri_aabbs(tris):
     from obvhs_amd.types import AABB
     out = np.zeros(len(tris), dtype=AABB)
@@ -94,8 +110,10 @@ def tri_aabbs(tris):
     return out
 
 
-def ploc_build(aabbs, search_distance=14, search_depth_threshold=0, indices=None, basic_search=False):
-    """Returns dict(nodes (Bvh2Node[2n-1]), prim_indices, codes, sorted_index, iter_counts, depth, max_offset)."""
+def ploc_build(aabbs, search_distance=14, search_depth_threshold=0, indices=None, basic_search=False,
+               sort_precision=64):
+    """Returns dict(nodes (Bvh2Node[2n-1]), prim_indices, codes, sorted_index, iter_counts, depth, max_offset);
+    codes are u64[n] for SortPrecision::U64 and u64[n, 2] (low, high words) for U128."""
     from obvhs_amd.types import BVH2_NODE
     n = len(aabbs)
     aabbs = np.ascontiguousarray(aabbs)
@@ -103,15 +121,18 @@ def ploc_build(aabbs, search_distance=14, search_depth_threshold=0, indices=None
     nodes = np.zeros(max(2 * n - 1, 0), dtype=BVH2_NODE)
     prims = np.zeros(n, dtype=np.uint32)
     codes = np.zeros(n, dtype=np.uint64)
+    codes_hi = np.zeros(n, dtype=np.uint64)
     order = np.zeros(n, dtype=np.uint32)
     cap = 1 << 20
     iters = np.zeros(cap, dtype=np.uint32)
     ni, depth, maxoff = C.c_uint32(), C.c_uint32(), C.c_uint32()
     rc = lib().orc_ploc_build(_ptr(aabbs), _ptr(idx), n, search_distance, search_depth_threshold,
                               int(basic_search), _ptr(nodes), _ptr(prims), _ptr(codes), _ptr(order), _ptr(iters),
-                              cap, C.byref(ni), C.byref(depth), C.byref(maxoff))
+                              cap, C.byref(ni), C.byref(depth), C.byref(maxoff), sort_precision, _ptr(codes_hi))
     if rc != 0:
         raise OracleError(rc)
+    if sort_precision == 128:
+        codes = np.stack([codes, codes_hi], axis=1)
     return dict(nodes=nodes, prim_indices=prims, codes=codes, sorted_index=order,
                 iter_counts=iters[:min(ni.value, cap)].copy(), depth=depth.value, max_offset=maxoff.value)
 
@@ -232,23 +253,23 @@ def rebuild_path_flags(nodes, leaves):
     return rc, flags
 
 
-def partial_rebuild(nodes, flags, search_distance=1, search_depth_threshold=0):
+def partial_rebuild(nodes, flags, search_distance=1, search_depth_threshold=0, sort_precision=64):
     """PlocBuilder::partial_rebuild on a copy; returns (rc, nodes, pass count)."""
     out = np.ascontiguousarray(nodes).copy()
     flags = np.ascontiguousarray(flags, dtype=np.uint8)
     assert len(flags) == len(out)
     depth = C.c_uint32()
     rc = lib().orc_partial_rebuild(_ptr(out), len(out), _ptr(flags), search_distance, search_depth_threshold,
-                                   C.byref(depth))
+                                   C.byref(depth), sort_precision)
     return rc, out, depth.value
 
 
-def full_rebuild(nodes, search_distance=1, search_depth_threshold=0):
+def full_rebuild(nodes, search_distance=1, search_depth_threshold=0, sort_precision=64):
     """PlocBuilder::full_rebuild on a copy; returns (rc, nodes, pass count)."""
     out = np.ascontiguousarray(nodes).copy()
     nn, depth = C.c_uint64(), C.c_uint32()
     rc = lib().orc_full_rebuild(_ptr(out), len(out), search_distance, search_depth_threshold, C.byref(nn),
-                                C.byref(depth))
+                                C.byref(depth), sort_precision)
     return rc, out[:nn.value] if rc == 0 else out, depth.value
 
 
@@ -261,7 +282,7 @@ def primitives_to_nodes(nodes, prim_indices):
 
 
 def build_cwbvh(tris, search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3, pre_split=False,
-                reinsertion_batch_ratio=0.0, ratio_sequence=None, split_max_total=0):
+                reinsertion_batch_ratio=0.0, ratio_sequence=None, split_max_total=0, sort_precision=64):
     """Oracle build_cwbvh_from_tris (cwbvh/builder.rs:20-85): [pre-splits] -> PLOC -> [reinsertion] ->
     bvh2_to_cwbvh(.., clamp(max_prims, 1, 3), true, false).  Returns (bvh2 dict, cw nodes, cw primitive indices);
     the dict's "nodes" are the BVH2 after reinsertion."""
@@ -269,7 +290,7 @@ def build_cwbvh(tris, search_distance=14, search_depth_threshold=0, max_prims_pe
         aabbs, idx, _, _ = presplit(tris, split_max_total)
     else:
         aabbs, idx = tri_aabbs(tris), None
-    b = ploc_build(aabbs, search_distance, search_depth_threshold, indices=idx)
+    b = ploc_build(aabbs, search_distance, search_depth_threshold, indices=idx, sort_precision=sort_precision)
     if reinsertion_batch_ratio > 0.0 and len(b["nodes"]):
         b["nodes"] = reinsertion(b["nodes"], reinsertion_batch_ratio, ratio_sequence)
     cw, prims, _ = bvh2_to_cwbvh(b["nodes"], b["prim_indices"], max(1, min(3, max_prims_per_leaf)), True)

@@ -41,14 +41,14 @@ def test_kitchen_known_answer(ctx, preset):
     print(f"[case] kitchen {preset}", flush=True)
     tris = K.triangles()
     rays, dirs = K.camera_rays(32)
-    p = PRESETS[preset]()
-    p.sort_precision = ob.SortPrecision.U64  # the only key width here; closest hits do not depend on it
+    p = PRESETS[preset]()  # slow and very_slow sort with SortPrecision::U128
     cw = ob.build_cwbvh_from_tris(ctx, tris, p)
     bvh_tris = cw.reorder_triangles(tris).to_host()
     hits = cw.ray_traverse(rays, bvh_tris)
     assert K.render_hash(bvh_tris, hits, dirs) == K.KNOWN_HASH
     _, rcw, rprims = O.build_cwbvh(tris, int(p.ploc_search_distance), p.search_depth_threshold, p.max_prims_per_leaf,
-                                   pre_split=p.pre_split, reinsertion_batch_ratio=p.reinsertion_batch_ratio)
+                                   pre_split=p.pre_split, reinsertion_batch_ratio=p.reinsertion_batch_ratio,
+                                   sort_precision=int(p.sort_precision))
     if p.reinsertion_batch_ratio > 0:
         same_nodes_up_to_zero_sign(cw.nodes, rcw, f"CWBVH nodes ({preset})")
     else:

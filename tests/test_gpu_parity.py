@@ -8,12 +8,13 @@ import pytest
 import obvhs_amd as ob
 import oracle_lib as O
 from obvhs_amd import scenes
+from obvhs_amd.types import triangles_from_vertices
 
 pytestmark = pytest.mark.gpu
 
 
-def gpu_build(ctx, tris, sd=14, sdt=0, max_prims=3):
-    bvh2 = ob.PlocBuilder(ctx).build(sd, tris, None, ob.SortPrecision.U64, sdt)
+def gpu_build(ctx, tris, sd=14, sdt=0, max_prims=3, sp=64):
+    bvh2 = ob.PlocBuilder(ctx).build(sd, tris, None, ob.SortPrecision(sp), sdt)
     cw = ob.bvh2_to_cwbvh(bvh2, max_prims, True, False)
     return bvh2, cw
 
@@ -27,11 +28,11 @@ def assert_same_bytes(a, b, what):
     assert len(bad) == 0, f"{what}: {len(bad)} of {len(a)} records differ, first at {bad[:5]}"
 
 
-def check_build(ctx, tris, sd=14, sdt=0, max_prims=3):
-    print(f"[case] build n={len(tris)} sd={sd} sdt={sdt} max_prims={max_prims}", flush=True)
+def check_build(ctx, tris, sd=14, sdt=0, max_prims=3, sp=64):
+    print(f"[case] build n={len(tris)} sd={sd} sdt={sdt} max_prims={max_prims} sort=U{sp}", flush=True)
     aabbs = O.tri_aabbs(tris)
-    ref = O.ploc_build(aabbs, sd, sdt)
-    bvh2, cw = gpu_build(ctx, tris, sd, sdt, max_prims)
+    ref = O.ploc_build(aabbs, sd, sdt, sort_precision=sp)
+    bvh2, cw = gpu_build(ctx, tris, sd, sdt, max_prims, sp)
     if len(tris):
         codes, order = bvh2.sort_trace()
         np.testing.assert_array_equal(codes, ref["codes"], err_msg="Morton codes")
@@ -48,6 +49,27 @@ def check_build(ctx, tris, sd=14, sdt=0, max_prims=3):
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 17, 100, 1000, 2048, 2049, 4096, 4097, 9000, 40000])
 def test_build_soup_sizes(ctx, n):
     check_build(ctx, scenes.soup(n, seed=n))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 2049, 9000, 40000])
+@pytest.mark.parametrize("sd", [1, 14])
+def test_build_u128(ctx, n, sd):
+    """SortPrecision::U128 (ploc/mod.rs:664-683): 126-bit codes, sorted permutation, nodes and CWBVH."""
+    check_build(ctx, scenes.soup(n, seed=n + 5), sd, 0, 3, 128)
+
+
+def test_build_u128_orders_below_the_u64_resolution(ctx):
+    """Groups of tiny triangles inside one 2^-21 Morton cell: U64 keys tie (leaf order), U128 keys do not."""
+    rng = np.random.default_rng(8)
+    cells = rng.random((700, 3)) * 0.9 + 0.05
+    c = (np.repeat(cells, 16, axis=0) + rng.random((700 * 16, 3)) * 2.0 ** -24).astype(np.float32)
+    c = np.concatenate([c, np.array([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], np.float32)])  # the unit scene box
+    d = np.float32(2.0 ** -30)
+    tris = triangles_from_vertices(c, c + np.array([d, 0, 0], np.float32), c + np.array([0, d, 0], np.float32))
+    check_build(ctx, tris, 14, 0, 3, 128)
+    a = ob.PlocBuilder(ctx).build(14, tris, None, ob.SortPrecision.U64, 0).sort_trace()[1]
+    b = ob.PlocBuilder(ctx).build(14, tris, None, ob.SortPrecision.U128, 0).sort_trace()[1]
+    assert a.tolist() != b.tolist()
 
 
 @pytest.mark.parametrize("sd", [1, 2, 6, 14, 24, 32])

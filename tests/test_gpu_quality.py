@@ -78,15 +78,15 @@ PRESETS = {
 @pytest.mark.parametrize("preset", list(PRESETS))
 @pytest.mark.parametrize("scene", ["soup5k", "thin"])
 def test_presets_match_oracle(ctx, preset, scene):
-    """The crate's presets (lib.rs:243-306); SortPrecision::U128 presets run with U64 (the only key width here)."""
+    """The crate's presets (lib.rs:243-306), each with its own SortPrecision (U128 for slow and very_slow)."""
     print(f"[case] preset {preset} {scene}", flush=True)
     tris = SCENES[scene]()
     p = PRESETS[preset]()
-    p.sort_precision = ob.SortPrecision.U64
     cw = ob.build_cwbvh_from_tris(ctx, tris, p)
     _, rcw, rprims = O.build_cwbvh(tris, int(p.ploc_search_distance), p.search_depth_threshold,
                                    p.max_prims_per_leaf, pre_split=p.pre_split,
-                                   reinsertion_batch_ratio=p.reinsertion_batch_ratio)
+                                   reinsertion_batch_ratio=p.reinsertion_batch_ratio,
+                                   sort_precision=int(p.sort_precision))
     same(cw.nodes, rcw, f"CWBVH nodes ({preset})")
     np.testing.assert_array_equal(cw.primitive_indices, rprims)
     rays = scenes.pinhole_rays(64, 64, eye=(0.5, 0.45, -0.35), target=(0.5, 0.0, 0.55))

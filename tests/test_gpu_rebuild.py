@@ -84,6 +84,26 @@ def test_partial_rebuild_matches_oracle(ctx, scene, frac, sd, sdt, reins):
 
 
 @pytest.mark.parametrize("scene", list(SCENES))
+@pytest.mark.parametrize("frac", [0.05, 1.0])
+def test_rebuilds_u128_match_oracle(ctx, scene, frac):
+    """partial_rebuild and full_rebuild with SortPrecision::U128."""
+    print(f"[case] U128 rebuilds {scene} frac={frac}", flush=True)
+    tris = SCENES[scene]()
+    bvh = ob.PlocBuilder(ctx).build(14, tris, None, ob.SortPrecision.U128)
+    before = bvh.nodes
+    flags = ob.compute_rebuild_path_flags(bvh, pick_leaves(before, frac, seed=11))
+    ob.PlocBuilder(ctx).partial_rebuild(bvh, flags, 14, ob.SortPrecision.U128, 0)
+    rc, want, _ = O.partial_rebuild(before, flags.to_host(), 14, 0, sort_precision=128)
+    assert rc == 0
+    same(bvh.nodes, want, "BVH2 after a U128 partial rebuild")
+    before = bvh.nodes
+    ob.PlocBuilder(ctx).full_rebuild(bvh, 6, ob.SortPrecision.U128, 2)
+    rc, want, _ = O.full_rebuild(before, 6, 2, sort_precision=128)
+    assert rc == 0
+    same(bvh.nodes, want, "BVH2 after a U128 full rebuild")
+
+
+@pytest.mark.parametrize("scene", list(SCENES))
 @pytest.mark.parametrize("sd,sdt", [(1, 0), (14, 0), (24, 3)])
 def test_full_rebuild_matches_oracle(ctx, scene, sd, sdt):
     print(f"[case] full rebuild {scene} sd={sd} sdt={sdt}", flush=True)

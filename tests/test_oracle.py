@@ -3,10 +3,13 @@
 Oracle functions and what pins each of them here:
   morton_encode_u64_unorm (morton.rs:35-58)   closed form (naive per-bit interleave), Rust `as u32` saturation,
                                               the 1.0 -> 0 wrap; planted axis swap is caught
+  morton_encode_u128_unorm (morton.rs:63-89)  the same for 42 bits per axis (`as u64`), and its top 63 bits equal
+                                              the U64 code for the same point
   half_area (aabb.rs:151-154)                 aabb.rs:314-317 known answer + xy+yz+zx closed form; planted wrong
                                               term is caught
   sort (ploc/mod.rs:811-827)                  sortedness + stability (tie rule of this project; the reference's
-                                              unstable tie order is not reproducible -- parity unpinned for ties)
+                                              unstable tie order is not reproducible -- parity unpinned for ties);
+                                              U128 orders points that share a U64 code by their finer bits
   PLOC search / merge (ploc/mod.rs:327-503)   cached search == find_best_node_basic brute force (ploc/mod.rs:508);
                                               hand-derived trees for tie-only and two-merge inputs pin the `<=`
                                               rule, the left/right orientation and the child-slot order; Bvh2
@@ -74,6 +77,41 @@ def test_morton_planted_axis_swap_is_caught(mutation):
     mutation(2)
     p = [(v + 0.5) / (1 << 21) for v in (5, 9, 3)]
     assert O.lib().orc_morton_encode_u64_unorm(*p) != naive_morton(5, 9, 3)
+
+
+def naive_morton128(x, y, z):
+    code = 0
+    for bit in range(42):
+        code |= ((x >> bit) & 1) << (3 * bit)
+        code |= ((y >> bit) & 1) << (3 * bit + 1)
+        code |= ((z >> bit) & 1) << (3 * bit + 2)
+    return code
+
+
+def test_morton128_closed_form():
+    rng = np.random.default_rng(11)
+    for bit in range(42):  # every mask stage moves each bit to 3 * bit
+        assert O.split_by_3_u128(1 << bit) == 1 << (3 * bit)
+    assert O.split_by_3_u128((1 << 64) - 1) == naive_morton128((1 << 42) - 1, 0, 0)  # bits above 42 are dropped
+    for _ in range(2000):
+        x, y, z = (int(v) for v in rng.integers(0, 1 << 42, 3))
+        assert O.split_by_3_u128(x) == naive_morton128(x, 0, 0)
+        p = [(v + 0.5) / (1 << 42) for v in (x, y, z)]
+        assert O.morton_encode_u128_unorm(*p) == naive_morton128(x, y, z)
+
+
+def test_morton128_special_cases_and_u64_prefix():
+    enc = O.morton_encode_u128_unorm
+    assert enc(float("nan"), 0.0, 0.0) == 0                             # NaN as u64 -> 0
+    assert enc(-0.3, -5.0, -1e30) == 0                                  # negative saturates to 0
+    assert enc(1.0, 1.0, 1.0) == 0                                      # 2^42 & 0x3ffffffffff wraps to 0
+    assert enc(1e30, 0.0, 0.0) == naive_morton128((1 << 42) - 1, 0, 0)  # u64::MAX & 0x3ffffffffff
+    # floor(p * 2^42) >> 21 == floor(p * 2^21) (scaling by a power of two is exact), so the U64 code is the top 63
+    # bits of the 126-bit U128 code, special cases included
+    rng = np.random.default_rng(12)
+    pts = np.concatenate([rng.random((3000, 3)), [[1.0, 0.5, 0.0], [1e30, -1.0, 0.25], [0.0, 0.0, 0.0]]])
+    for p in pts:
+        assert enc(*p) >> 63 == O.lib().orc_morton_encode_u64_unorm(*p)
 
 
 def _box(lo, hi):
@@ -150,6 +188,28 @@ def test_sort_sorted_and_stable(mutation):
     r2 = O.ploc_build(aabbs, 14, 0)
     o2 = r2["sorted_index"]
     assert not np.all(o2[1:][same] > o2[:-1][same])
+
+
+def test_sort_u128_orders_below_the_u64_resolution():
+    # 48 groups of 8 points that share a U64 code (all within one 2^-21 cell of the unit scene box) but have
+    # distinct U128 codes, listed in descending fine order, plus exact duplicates for the tie rule
+    rng = np.random.default_rng(6)
+    cells = rng.integers(1, (1 << 21) - 2, (48, 3))
+    fine = np.array([[7 - k, (k * 3) % 8, 0] for k in range(8)]) * 2.0 ** -26
+    pts = (np.repeat(cells, 8, axis=0) + 0.25) / 2.0 ** 21 + np.tile(fine, (48, 1))
+    pts = np.concatenate([pts, pts[:40], [[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]]])  # duplicates; scene box = unit cube
+    aabbs = aabbs_from_minmax(pts, pts)
+    r64, r128 = O.ploc_build(aabbs, 14, 0), O.ploc_build(aabbs, 14, 0, sort_precision=128)
+    c = r128["codes"]
+    keys = [(int(h) << 64 | int(lo), i) for i, (lo, h) in enumerate(c)]
+    assert r128["sorted_index"].tolist() == [i for _, i in sorted(keys)]  # ascending code, ties by leaf index
+    for i in range(0, 384, 97):
+        p = pts[i].astype(np.float32).astype(np.float64)  # the centre of a point box is the f32 point
+        assert keys[i][0] == O.morton_encode_u128_unorm(*p)
+    assert np.array_equal((c[:, 1] << np.uint64(1)) | (c[:, 0] >> np.uint64(63)), r64["codes"])  # code128 >> 63
+    assert r64["sorted_index"].tolist() != r128["sorted_index"].tolist()
+    rc, _ = O.bvh2_validate(r128["nodes"], r128["prim_indices"], aabbs, tight_fit=True, ordered=True)
+    assert rc == 0
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 1000, 20000])

@@ -34,12 +34,12 @@ def test_brute_force_hash(scene):
 
 @pytest.mark.parametrize("preset", list(PRESETS))
 def test_preset_hash(scene, preset):
-    """obj_cwbvh.rs:160-181.  SortPrecision::U128 (slow, very_slow) runs with U64 keys: the BVH differs from the
-    crate's there, the closest hits and therefore the hash do not."""
+    """obj_cwbvh.rs:160-181, each preset with its own SortPrecision (U128 for slow and very_slow)."""
     tris, rays, dirs = scene
     p = PRESETS[preset]()
     _, cw, prims = O.build_cwbvh(tris, int(p.ploc_search_distance), p.search_depth_threshold, p.max_prims_per_leaf,
-                                 pre_split=p.pre_split, reinsertion_batch_ratio=p.reinsertion_batch_ratio)
+                                 pre_split=p.pre_split, reinsertion_batch_ratio=p.reinsertion_batch_ratio,
+                                 sort_precision=int(p.sort_precision))
     bvh_tris = tris[prims]
     hits = O.traverse(cw, bvh_tris, rays, 0)
     assert K.render_hash(bvh_tris, hits, dirs) == K.KNOWN_HASH
