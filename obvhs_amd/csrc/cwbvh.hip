@@ -8,16 +8,19 @@
 //                earlier pass, so ranges processed in pass order see finished children (no atomics, no fences).
 //                Alongside the seven decisions each node stores the size of the CWBVH subtree it roots, which
 //                lets the top-down pass number nodes exactly like the reference's pre-order DFS.
-//   k_cw_emit  : one launch per CWBVH level.  Each node computes its children (get_children, order_children),
-//                its exponents and quantised child boxes, writes the 80-byte node and its leaf primitive indices,
-//                and hands its internal children their DFS-derived node index, child base and primitive base
-//                (convert_to_cwbvh_impl, bvh2_to_cwbvh.rs:75-193; count_primitives 197-211).
+//   k_cw_emit  : one launch per CWBVH level, 8 lanes per node.  The group walks get_children, each lane loads one
+//                child, the octant assignment (order_children) runs as a per-round 8-lane argmin, then lane s
+//                quantises the child in slot s; prefix sums over the slots give the leaf primitive offsets and the
+//                internal children's DFS-derived node index, child base and primitive base, and the 80-byte node
+//                is assembled with lane shuffles (convert_to_cwbvh_impl, bvh2_to_cwbvh.rs:75-193;
+//                count_primitives 197-211).
 // The exponent exp2(ceil(log2(x))) (bvh2_to_cwbvh.rs:85-88) goes through the host libm's log2f in the reference;
 // a 256-entry table of per-exponent mantissa thresholds, generated from that same log2f when the context is
 // created, reproduces it with integer compares.
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -99,6 +102,254 @@ __device__ __forceinline__ uint32_t cw_exponent(float ext) {
     return k >= 255u ? 255u : k;
 }
 
+// Group helpers: the 8 lanes r = 0..7 of an aligned lane group work on one CWBVH node (lane r first owns the
+// node's child r, then slot r).
+__device__ __forceinline__ uint32_t g8_or(uint32_t v) {
+    v |= (uint32_t)__shfl_xor((int)v, 1, 8);
+    v |= (uint32_t)__shfl_xor((int)v, 2, 8);
+    v |= (uint32_t)__shfl_xor((int)v, 4, 8);
+    return v;
+}
+__device__ __forceinline__ uint32_t g8_inclusive_sum(uint32_t v, uint32_t r) {
+#pragma unroll
+    for (uint32_t d = 1; d < 8; d <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)v, d, 8);
+        if (r >= d) v += t;
+    }
+    return v;
+}
+// One 8-byte child array of the node from one byte per slot lane: (bytes of slots 0-3, bytes of slots 4-7).
+__device__ __forceinline__ uint2 g8_bytes(uint32_t byte, uint32_t r) {
+    uint32_t w = byte << (8u * (r & 3u));
+    w |= (uint32_t)__shfl_xor((int)w, 1, 8);
+    w |= (uint32_t)__shfl_xor((int)w, 2, 8);
+    const uint32_t x = (uint32_t)__shfl_xor((int)w, 4, 8);
+    return r < 4u ? make_uint2(w, x) : make_uint2(x, w);
+}
+__device__ __forceinline__ uint32_t quantise_lo(float b, float p, float rcp) {
+    return (uint32_t)(uint8_t)smin(smax(floorf(((b - p) - 0.0f) * rcp), 0.0f), 255.0f);
+}
+__device__ __forceinline__ uint32_t quantise_hi(float b, float p, float rcp) {
+    return (uint32_t)(uint8_t)smin(smax(ceilf(((b - p) + 0.0f) * rcp), 0.0f), 255.0f);
+}
+
+constexpr uint32_t EMIT_MAX_BLOCKS = 2048;   // 8 waves per SIMD on 256 CUs; larger levels loop
+constexpr uint32_t EMIT1_MAX_BLOCKS = 8192;  // one-lane form: 2M nodes per sweep
+// Batched conversions with at least this many roots use the one-lane form: every level is that wide, so the
+// 8-lane form's shorter per-node latency no longer pays for its 8x instruction count per node.
+constexpr uint32_t EMIT_ONE_LANE_ROOTS = 512;
+
+// What one node hands to the next level: its internal child count (0 on error), and per slot lane whether the
+// slot holds an internal child, its rank among them and its work item.
+struct EmitOut {
+    uint32_t num_internal;
+    uint32_t isint;
+    uint32_t rank;
+    CwItem item;
+};
+
+// One CWBVH node by the 8-lane group of lane r (every lane of the group calls with the same t).
+__device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, const uint32_t* __restrict__ bvh2_prims,
+                                          const DpView& dp, const CwItem* __restrict__ in_items, uint32_t t,
+                                          uint32_t r, uint32_t level, uint32_t order_children,
+                                          CwNode* __restrict__ out, uint32_t* __restrict__ out_prims, uint32_t cw_cap,
+                                          uint32_t prim_cap, CwMisc* misc, const MeshDesc* __restrict__ meshes,
+                                          EmitOut& eo) {
+    const CwItem it = in_items[t];
+    const Node32* nds = nodes;
+    const uint32_t* bprims = bvh2_prims;
+    CwNode* onodes = out;
+    uint32_t* oprims = out_prims;
+    uint32_t ccap = cw_cap, pcap = prim_cap;
+    DpView d = dp;
+    if (meshes) {  // batched conversions: every array is the item's mesh's slice
+        const MeshDesc md = meshes[it.mesh];
+        nds += md.node_base;
+        bprims += md.prim_base;
+        onodes += md.cw_base;
+        oprims += md.prim_base;
+        ccap = md.cw_count;
+        pcap = md.prim_count;
+        d.kbase = md.prim_base;
+    }
+    const Node32 nd = load_node(&nds[it.bvh2]);
+    const Box nb = node_box(nd);
+
+    // get_children (bvh2_to_cwbvh.rs:346-397), the same walk in every lane of the group
+    uint32_t ch[8];
+    uint32_t ckind = 0u;  // 2 bits per child: its decision-0 kind
+    int cc;
+    if (nd.prim_count != 0u) {  // a single-leaf BVH2: the root is its own only child
+        ch[0] = it.bvh2;
+        cc = 1;
+    } else {
+        cc = get_children_kids(d, nd.first_index, node_decision(d, nd.first_index, 0), ch, &ckind);
+    }
+    if (cc < 0) {
+        if (r == 0u) atomicOr(&misc->err, cc == -2 ? DERR_CW_DISTRIBUTE : DERR_CAPACITY);
+        return;
+    }
+    // lane r: child r
+    const bool has = (int)r < cc;
+    uint32_t my_ch = ch[0];
+#pragma unroll
+    for (int q = 1; q < 8; ++q)
+        if ((uint32_t)q == r) my_ch = ch[q];
+    Box cb{0, 0, 0, 0, 0, 0};
+    uint32_t cfirst = 0u, cpc = 0u;
+    if (has) {
+        const Node32 cn = load_node(&nds[my_ch]);
+        cb = node_box(cn);
+        cfirst = cn.first_index;
+        cpc = cn.prim_count;
+    }
+    const uint32_t ckd = (ckind >> (2u * r)) & 3u;
+
+    // order_children (bvh2_to_cwbvh.rs:402-467): greedy octant assignment, the first strict minimum in
+    // child-major / slot-minor order per round.  Lane r finds its own first minimum over the free slots; the
+    // group keeps the lowest cost, ties to the lowest child.
+    int my_slot = has ? (int)r : -1;
+    if (order_children) {
+        const float px = (nb.hx + nb.lx) * 0.5f, py = (nb.hy + nb.ly) * 0.5f, pz = (nb.hz + nb.lz) * 0.5f;
+        const float vx = (cb.hx + cb.lx) * 0.5f - px, vy = (cb.hy + cb.ly) * 0.5f - py,
+                    vz = (cb.hz + cb.lz) * 0.5f - pz;
+        const float FMAX = 3.40282347e+38f;
+        uint32_t filled = 0u;
+        bool assigned = false;
+        my_slot = -1;
+        for (int round = 0; round < 8; ++round) {
+            float bc = FMAX;
+            int bs = -1;
+            if (has && !assigned) {
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    const float dx = (s & 4) ? -1.0f : 1.0f, dy = (s & 2) ? -1.0f : 1.0f,
+                                dz = (s & 1) ? -1.0f : 1.0f;
+                    const float v = (dx * vx + dy * vy) + dz * vz;  // exact for d in {+-1}^3
+                    if (!((filled >> s) & 1u) && v < bc) {
+                        bc = v;
+                        bs = s;
+                    }
+                }
+            }
+            int bl = (int)r;
+#pragma unroll
+            for (int m = 1; m < 8; m <<= 1) {
+                const float oc = __shfl_xor(bc, m, 8);
+                const int os = __shfl_xor(bs, m, 8), ol = __shfl_xor(bl, m, 8);
+                if (os >= 0 && (bs < 0 || oc < bc || (oc == bc && ol < bl))) {
+                    bc = oc;
+                    bs = os;
+                    bl = ol;
+                }
+            }
+            if (bs < 0) break;  // the same (bs, bl) in all 8 lanes
+            filled |= 1u << bs;
+            if ((uint32_t)bl == r) {
+                assigned = true;
+                my_slot = bs;
+            }
+        }
+        if (g8_or(has && my_slot < 0 ? 1u : 0u)) {
+            if (r == 0u) atomicOr(&misc->err, DERR_CW_ORDER);
+            return;
+        }
+    }
+
+    // lane r: slot r (the child placed there, if any)
+    int src = -1;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (__shfl(my_slot, q, 8) == (int)r) src = q;
+    const bool occ = src >= 0;
+    const int sl = occ ? src : (int)r;
+    const Box sb{__shfl(cb.lx, sl, 8), __shfl(cb.ly, sl, 8), __shfl(cb.lz, sl, 8),
+                 __shfl(cb.hx, sl, 8), __shfl(cb.hy, sl, 8), __shfl(cb.hz, sl, 8)};
+    const uint32_t s_first = (uint32_t)__shfl((int)cfirst, sl, 8), s_pc = (uint32_t)__shfl((int)cpc, sl, 8);
+    const uint32_t s_kind = (uint32_t)__shfl((int)ckd, sl, 8), s_ch = (uint32_t)__shfl((int)my_ch, sl, 8);
+
+    // Leaves append primitives in slot order (bvh2_to_cwbvh.rs:117-170; count_primitives 197-211): a BVH2 leaf
+    // its first primitive with prim_count, a subtree of at most 3 leaves its leaves left first, as recorded in
+    // the cost table.
+    uint32_t lpc = 0u, isint = 0u, bad = 0u;
+    uint4 pl = make_uint4(0u, 0u, 0u, 0u);
+    if (occ) {
+        if (s_kind == KIND_LEAF) {
+            pl = s_pc != 0u ? make_uint4(s_first, 0u, 0u, 1u) : d.rec[dp_key(d, s_first)].plist;
+            lpc = s_pc != 0u ? s_pc : pl.w;
+            if (lpc < 1u || lpc > 3u) bad |= DERR_CW_PRIMCOUNT;
+        } else if (s_kind == KIND_INTERNAL) {
+            isint = 1u;
+        } else {
+            bad |= DERR_CW_DISTRIBUTE;
+        }
+    }
+    const uint32_t packed = (bad ? 0u : lpc) | isint << 8;
+    const uint32_t incl = g8_inclusive_sum(packed, r);
+    const uint32_t tot = (uint32_t)__shfl((int)incl, 7, 8);
+    const uint32_t prim_off = (incl - packed) & 0xffu, int_rank = (incl - packed) >> 8;
+    const uint32_t num_primitives = tot & 0xffu, num_internal = tot >> 8;
+    const uint32_t imask = g8_or(isint << r);
+    if (it.cw_index >= ccap) bad |= DERR_CAPACITY;
+    if (num_internal != 0u && level + 1 >= MAX_CW_LEVELS) bad |= DERR_CW_DEPTH;
+    bad = g8_or(bad);
+    if (bad) {
+        if (r == 0u) atomicOr(&misc->err, bad);
+        return;
+    }
+    if (lpc != 0u) {
+        const uint32_t dst = it.b + prim_off;
+        if (dst < pcap) oprims[dst] = bprims[pl.x];
+        if (lpc >= 2u && dst + 1u < pcap) oprims[dst + 1u] = bprims[pl.y];
+        if (lpc >= 3u && dst + 2u < pcap) oprims[dst + 2u] = bprims[pl.z];
+    }
+
+    // The 80-byte node (bvh2_to_cwbvh.rs:85-170): exponents, quantised child boxes, meta bytes.
+    const uint32_t ex = cw_exponent(nb.hx - nb.lx), ey = cw_exponent(nb.hy - nb.ly), ez = cw_exponent(nb.hz - nb.lz);
+    const float rx = 1.0f / __uint_as_float(ex << 23), ry = 1.0f / __uint_as_float(ey << 23),
+                rz = 1.0f / __uint_as_float(ez << 23);
+    uint32_t meta = 0u, qlx = 0u, qhx = 0u, qly = 0u, qhy = 0u, qlz = 0u, qhz = 0u;
+    if (occ) {
+        qlx = quantise_lo(sb.lx, nb.lx, rx);
+        qhx = quantise_hi(sb.hx, nb.lx, rx);
+        qly = quantise_lo(sb.ly, nb.ly, ry);
+        qhy = quantise_hi(sb.hy, nb.ly, ry);
+        qlz = quantise_lo(sb.lz, nb.lz, rz);
+        qhz = quantise_hi(sb.hz, nb.lz, rz);
+        meta = isint ? ((24u + r) | 0x20u) : (prim_off | (lpc == 1u ? 0x20u : (lpc == 2u ? 0x60u : 0xe0u)));
+    }
+    const uint2 bm = g8_bytes(meta, r), blx = g8_bytes(qlx, r), bhx = g8_bytes(qhx, r), bly = g8_bytes(qly, r),
+                bhy = g8_bytes(qhy, r), blz = g8_bytes(qlz, r), bhz = g8_bytes(qhz, r);
+    const uint32_t child_base = 1u + it.a;
+    uint4 chunk;
+    if (r == 0u)
+        chunk = make_uint4(__float_as_uint(nb.lx), __float_as_uint(nb.ly), __float_as_uint(nb.lz),
+                           ex | ey << 8 | ez << 16 | imask << 24);
+    else if (r == 1u)
+        chunk = make_uint4(child_base, it.b, bm.x, bm.y);
+    else if (r == 2u)
+        chunk = make_uint4(blx.x, blx.y, bhx.x, bhx.y);
+    else if (r == 3u)
+        chunk = make_uint4(bly.x, bly.y, bhy.x, bhy.y);
+    else
+        chunk = make_uint4(blz.x, blz.y, bhz.x, bhz.y);
+    if (r < 5u) reinterpret_cast<uint4*>(&onodes[it.cw_index])[r] = chunk;
+
+    // Children's DFS numbering (pre-order, child blocks appended at visit time, bvh2_to_cwbvh.rs:172-191); the
+    // caller places the items.
+    if (num_internal == 0u) return;
+    const uint32_t csz = isint ? d.rec[dp_key(d, s_first)].cwsize - 1u : 0u;
+    const uint32_t cnp = isint ? d.rec[dp_key(d, s_first)].nprims : 0u;
+    const uint32_t a_ex = g8_inclusive_sum(csz, r) - csz, b_ex = g8_inclusive_sum(cnp, r) - cnp;
+    eo.num_internal = num_internal;
+    eo.isint = isint;
+    eo.rank = int_rank;
+    eo.item = CwItem{child_base + int_rank, s_ch, it.a + num_internal + a_ex, it.b + num_primitives + b_ex, it.mesh};
+}
+
+// One CWBVH level.  Each iteration of the block-uniform loop gives every 8-lane group one node; the groups' next-level
+// items are counted in LDS and placed with one global atomic per block (a counter shared by the whole level would
+// serialise one atomic per node).
 __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nodes,
                                                   const uint32_t* __restrict__ bvh2_prims,
                                                   DpView dp,
@@ -106,12 +357,44 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
                                                   uint32_t level, uint32_t order_children, CwNode* __restrict__ out,
                                                   uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
                                                   CwMisc* misc, const MeshDesc* __restrict__ meshes) {
-    const uint32_t t = blockIdx.x * T256 + threadIdx.x;
+    constexpr uint32_t GROUPS = T256 / 8;
+    __shared__ uint32_t s_count, s_base;
+    const uint32_t r = threadIdx.x & 7u;
     const uint32_t count = misc->level_count[level];
-    if (t >= count) return;
-    const CwItem it = in_items[t];
-    // Batched conversions: every array is the item's mesh's slice.
-    if (meshes) {
+    for (uint32_t first = blockIdx.x * GROUPS; first < count; first += gridDim.x * GROUPS) {
+        const uint32_t t = first + (threadIdx.x >> 3);
+        EmitOut eo;
+        eo.num_internal = 0u;
+        eo.isint = 0u;
+        eo.rank = 0u;
+        if (t < count)
+            emit_node(nodes, bvh2_prims, dp, in_items, t, r, level, order_children, out, out_prims, cw_cap, prim_cap,
+                      misc, meshes, eo);
+        if (threadIdx.x == 0) s_count = 0u;
+        __syncthreads();
+        uint32_t off = 0u;
+        if (r == 0u && eo.num_internal != 0u) off = atomicAdd(&s_count, eo.num_internal);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_count != 0u) s_base = atomicAdd(&misc->level_count[level + 1], s_count);
+        __syncthreads();
+        off = (uint32_t)__shfl((int)off, 0, 8);
+        if (eo.isint) out_items[s_base + off + eo.rank] = eo.item;
+        __syncthreads();  // s_count and s_base are reused by the next iteration
+    }
+}
+
+// One CWBVH node by one lane (emit_node1) -- the form for levels many thousands of nodes wide (batched conversions),
+// where one lane per node issues the fewest instructions per node.  Outputs what the caller needs to place the
+// node's internal children: their count, slots, BVH2 slots and first_index words, and the item's running bases.
+__device__ __forceinline__ void emit_node1(const Node32* __restrict__ nodes, const uint32_t* __restrict__ bvh2_prims,
+                                           DpView& dp, const CwItem* __restrict__ in_items, uint32_t t,
+                                           uint32_t level, uint32_t order_children, CwNode* __restrict__ out,
+                                           uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
+                                           CwMisc* misc, const MeshDesc* __restrict__ meshes, CwItem& it,
+                                           uint32_t* ch, uint32_t* cfirst, int* slot_child, uint32_t* internal_slots,
+                                           uint32_t& num_internal, uint32_t& num_primitives, uint32_t& child_base) {
+    it = in_items[t];
+    if (meshes) {  // batched conversions: every array is the item's mesh's slice
         const MeshDesc md = meshes[it.mesh];
         nodes += md.node_base;
         bvh2_prims += md.prim_base;
@@ -124,7 +407,6 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     const Node32 nd = load_node(&nodes[it.bvh2]);
     const Box nb = node_box(nd);
 
-    uint32_t ch[8];
     uint32_t ckind = 0u;  // 2 bits per child: its decision-0 kind
     int cc;
     if (nd.prim_count != 0u) {  // a single-leaf BVH2: the root is its own only child
@@ -139,7 +421,6 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     }
     Box cb[8];
     uint32_t cleafpc[8];   // prim_count of a child that is a BVH2 leaf, 0 for an internal child
-    uint32_t cfirst[8];    // first_index of each child
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         cb[c] = Box{0, 0, 0, 0, 0, 0};
@@ -204,7 +485,6 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
         }
     }
     // slots[s] = child index placed in slot s (or -1)
-    int slot_child[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         slot_child[s] = -1;
@@ -224,8 +504,8 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     const float rx = 1.0f / __uint_as_float(ex << 23), ry = 1.0f / __uint_as_float(ey << 23),
                 rz = 1.0f / __uint_as_float(ez << 23);
     uint32_t imask = 0u;
-    uint32_t num_internal = 0u, num_primitives = 0u;
-    uint32_t internal_slots[8];
+    num_internal = 0u;
+    num_primitives = 0u;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         o.child_meta[s] = 0;
@@ -233,7 +513,7 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
         o.child_min_y[s] = o.child_max_y[s] = 0;
         o.child_min_z[s] = o.child_max_z[s] = 0;
     }
-    const uint32_t child_base = 1u + it.a;
+    child_base = 1u + it.a;
     // First sweep: leaves append primitives in slot order before any recursion (bvh2_to_cwbvh.rs:117-170).
     for (int s = 0; s < 8; ++s) {
         const int c = slot_child[s];
@@ -266,7 +546,7 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
         if (kind == KIND_LEAF) {
             // count_primitives (bvh2_to_cwbvh.rs:197-211): a BVH2 leaf appends its first primitive and counts
             // prim_count; a subtree of at most 3 leaves appends them left first, as recorded in the cost table.
-            const uint4 pl = cpc != 0u ? make_uint4(cf, 0u, 0u, 1u) : dp.plist[dp_key(dp, cf)];
+            const uint4 pl = cpc != 0u ? make_uint4(cf, 0u, 0u, 1u) : dp.rec[dp_key(dp, cf)].plist;
             const uint32_t pc = cpc != 0u ? cpc : pl.w;
             const uint32_t dst = it.b + num_primitives;
             if (pl.w >= 1u && dst < prim_cap) out_prims[dst] = bvh2_prims[pl.x];
@@ -278,6 +558,7 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
             else if (pc == 3u) unary = 0xe0u;
             else {
                 atomicOr(&misc->err, DERR_CW_PRIMCOUNT);
+                num_internal = 0u;
                 return;
             }
             o.child_meta[s] = (uint8_t)(num_primitives | unary);
@@ -288,6 +569,7 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
             internal_slots[num_internal++] = (uint32_t)s;
         } else {
             atomicOr(&misc->err, DERR_CW_DISTRIBUTE);
+            num_internal = 0u;
             return;
         }
     }
@@ -302,28 +584,59 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     } else {
         atomicOr(&misc->err, DERR_CAPACITY);
     }
-    if (num_internal == 0u) return;
-    if (level + 1 >= MAX_CW_LEVELS) {
+    if (num_internal != 0u && level + 1 >= MAX_CW_LEVELS) {
         atomicOr(&misc->err, DERR_CW_DEPTH);
-        return;
+        num_internal = 0u;
     }
-    // Children's DFS numbering (pre-order, child blocks appended at visit time, bvh2_to_cwbvh.rs:172-191).
-    const uint32_t base = atomicAdd(&misc->level_count[level + 1], num_internal);
-    uint32_t a_run = it.a + num_internal;
-    uint32_t b_run = it.b + num_primitives;
-    for (uint32_t j = 0; j < num_internal; ++j) {
-        const uint32_t s = internal_slots[j];
-        const int c = slot_child[s];
-        uint32_t ci = ch[0], cf = cfirst[0];
+}
+
+__global__ void __launch_bounds__(T256) k_cw_emit1(const Node32* __restrict__ nodes,
+                                                   const uint32_t* __restrict__ bvh2_prims,
+                                                   DpView dp,
+                                                   const CwItem* __restrict__ in_items, CwItem* __restrict__ out_items,
+                                                   uint32_t level, uint32_t order_children, CwNode* __restrict__ out,
+                                                   uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
+                                                   CwMisc* misc, const MeshDesc* __restrict__ meshes) {
+    __shared__ uint32_t s_count, s_base;
+    const uint32_t count = misc->level_count[level];
+    for (uint32_t first = blockIdx.x * T256; first < count; first += gridDim.x * T256) {  // block-uniform
+        const uint32_t t = first + threadIdx.x;
+        CwItem it{};
+        DpView d = dp;
+        uint32_t ch[8], cfirst[8], internal_slots[8];
+        int slot_child[8];
+        uint32_t num_internal = 0u, num_primitives = 0u, child_base = 0u;
+        if (t < count)
+            emit_node1(nodes, bvh2_prims, d, in_items, t, level, order_children, out, out_prims, cw_cap, prim_cap,
+                       misc, meshes, it, ch, cfirst, slot_child, internal_slots, num_internal, num_primitives,
+                       child_base);
+        // Place the next level's items: counted in LDS, one global atomic per block.
+        if (threadIdx.x == 0) s_count = 0u;
+        __syncthreads();
+        uint32_t base = 0u;
+        if (num_internal != 0u) base = atomicAdd(&s_count, num_internal);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_count != 0u) s_base = atomicAdd(&misc->level_count[level + 1], s_count);
+        __syncthreads();
+        base += s_base;
+        // Children's DFS numbering (pre-order, child blocks appended at visit time, bvh2_to_cwbvh.rs:172-191).
+        uint32_t a_run = it.a + num_internal;
+        uint32_t b_run = it.b + num_primitives;
+        for (uint32_t j = 0; j < num_internal; ++j) {
+            const uint32_t s = internal_slots[j];
+            const int c = slot_child[s];
+            uint32_t ci = ch[0], cf = cfirst[0];
 #pragma unroll
-        for (int q = 1; q < 8; ++q)
-            if (q == c) {
-                ci = ch[q];
-                cf = cfirst[q];
-            }
-        out_items[base + j] = CwItem{child_base + j, ci, a_run, b_run, it.mesh};
-        a_run += dp.cwsize[dp_key(dp, cf)] - 1u;
-        b_run += dp.nprims[dp_key(dp, cf)];
+            for (int q = 1; q < 8; ++q)
+                if (q == c) {
+                    ci = ch[q];
+                    cf = cfirst[q];
+                }
+            out_items[base + j] = CwItem{child_base + j, ci, a_run, b_run, it.mesh};
+            a_run += d.rec[dp_key(d, cf)].cwsize - 1u;
+            b_run += d.rec[dp_key(d, cf)].nprims;
+        }
+        __syncthreads();  // s_count and s_base are reused by the next iteration
     }
 }
 
@@ -355,8 +668,8 @@ __global__ void __launch_bounds__(T256) k_b_cw_sizes(const Node32* __restrict__ 
             md.cw_depth = 1u;
         } else {
             dp.kbase = md.prim_base;
-            md.cw_count = dp.cwsize[dp_key(dp, root.first_index)];
-            md.cw_depth = dp.cwdepth[dp_key(dp, root.first_index)];
+            md.cw_count = dp.rec[dp_key(dp, root.first_index)].cwsize;
+            md.cw_depth = dp.rec[dp_key(dp, root.first_index)].cwdepth;
         }
     }
     meshes[i] = md;
@@ -507,9 +820,9 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
     memcpy(&root, h, sizeof(Node32));
     uint32_t ncw = 1, levels = 1;
     if (root.prim_count == 0u) {
-        OBVHS_TRY(hipMemcpyAsync(h, dp.cwsize + dkey(root.first_index), 4, hipMemcpyDeviceToHost, st),
+        OBVHS_TRY(hipMemcpyAsync(h, &dp.rec[dkey(root.first_index)].cwsize, 4, hipMemcpyDeviceToHost, st),
                   "read cw size");
-        OBVHS_TRY(hipMemcpyAsync(h + 2, dp.cwdepth + dkey(root.first_index), 4, hipMemcpyDeviceToHost, st),
+        OBVHS_TRY(hipMemcpyAsync(h + 2, &dp.rec[dkey(root.first_index)].cwdepth, 4, hipMemcpyDeviceToHost, st),
                   "read cw depth");
     }
     OBVHS_TRY(hipMemcpyAsync(h + 1, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cw err");
@@ -527,8 +840,8 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
     // One launch per level, no host round trips: a level holds at most min(8^level, ncw) nodes.
     uint64_t bound = 1;
     for (uint32_t level = 0; level < levels; ++level) {
-        const uint32_t cap = (uint32_t)(bound < ncw ? bound : ncw);
-        const uint32_t g = (cap + T256 - 1) / T256;
+        const uint64_t cap = bound < ncw ? bound : ncw;
+        const uint32_t g = (uint32_t)std::min<uint64_t>((cap * 8 + T256 - 1) / T256, EMIT_MAX_BLOCKS);
         k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2->prim_indices.as<uint32_t>(), dp, items[level & 1],
                                       items[(level + 1) & 1], level, order_children ? 1u : 0u,
                                       out->nodes.as<CwNode>(), out->prim_indices.as<uint32_t>(), ncw, n, misc, nullptr);
@@ -595,12 +908,19 @@ obvhs_status cwbvh_convert_batch(obvhs_context* ctx, const Node32* nodes, const 
     k_b_cw_roots<<<(m + T256 - 1) / T256, T256, 0, st>>>(d_meshes, m, items[0], misc);
     uint64_t bound = active;
     for (uint32_t level = 0; level < levels; ++level) {
-        const uint32_t capl = (uint32_t)(bound < total ? bound : total);
-        const uint32_t g = (capl + T256 - 1) / T256;
-        if (g > 0)
+        const uint64_t capl = bound < total ? bound : total;
+        if (capl == 0) continue;
+        if (active >= EMIT_ONE_LANE_ROOTS) {  // every level at least `active` nodes wide: one lane per node
+            const uint32_t g = (uint32_t)std::min<uint64_t>((capl + T256 - 1) / T256, EMIT1_MAX_BLOCKS);
+            k_cw_emit1<<<g, T256, 0, st>>>(nodes, bvh2_prims, dp, items[level & 1], items[(level + 1) & 1], level,
+                                           order_children ? 1u : 0u, out_nodes.as<CwNode>(),
+                                           out_prims.as<uint32_t>(), 0u, 0u, misc, d_meshes);
+        } else {
+            const uint32_t g = (uint32_t)std::min<uint64_t>((capl * 8 + T256 - 1) / T256, EMIT_MAX_BLOCKS);
             k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2_prims, dp, items[level & 1], items[(level + 1) & 1], level,
                                           order_children ? 1u : 0u, out_nodes.as<CwNode>(),
                                           out_prims.as<uint32_t>(), 0u, 0u, misc, d_meshes);
+        }
         bound *= 8;
     }
     OBVHS_TRY(hipGetLastError(), "batch cwbvh emit launch");

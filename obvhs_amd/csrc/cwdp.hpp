@@ -29,17 +29,26 @@ constexpr uint32_t SUB_INVALID = 0xffffffffu;
 constexpr float PRIM_COST = 0.3f;  // bvh2_to_cwbvh.rs:30
 constexpr uint32_t KID_LEAF = 0x80000000u;  // leaf flag in DpView::kids
 
+// One table entry, a 128-byte cache line: every read of a node's entry (a parent's evaluation, a step of the
+// get_children walk, the emit of a leaf list or a child's sizes) touches a single line.
 // Decision byte: kind (2 bits) | distribute_left << 2 (3 bits) | distribute_right << 5 (3 bits).
+struct __align__(128) DpRec {
+    float cost[7];
+    uint32_t nprims;    // primitives under the node
+    uint32_t sub[7];
+    uint32_t cwsize;    // 1 + sub[0], or 0 when the node's CWBVH subtree cannot be built
+    uint2 meta;         // seven decision bytes (byte 7 unused)
+    uint2 subdepth;     // seven bytes: CWBVH depth below the children get_children(node, i) produces
+    uint2 kids;         // the children's first_index (bit 31 set for a leaf), left then right
+    uint32_t cwdepth;   // 1 + subdepth[0]: levels of the CWBVH subtree the node roots
+    uint32_t pad0;
+    uint4 plist;        // for a subtree of at most 3 primitives: its leaves' first_index, left first, and count
+    uint32_t pad1[4];
+};
+static_assert(sizeof(DpRec) == 128, "one cache line per entry");
+
 struct DpView {
-    float* cost;        // [k][7]
-    uint2* meta;        // [k] seven decision bytes (byte 7 unused)
-    uint32_t* sub;      // [k][7]
-    uint32_t* nprims;   // [k] primitives under the node
-    uint32_t* cwsize;   // [k] 1 + sub[0], or 0 when the node's CWBVH subtree cannot be built
-    uint2* subdepth;    // [k] seven bytes: CWBVH depth below the children get_children(node, i) produces
-    uint32_t* cwdepth;  // [k] 1 + subdepth[0]: levels of the CWBVH subtree the node roots
-    uint4* plist;       // [k] for a subtree of at most 3 primitives: its leaves' first_index, left first, and count
-    uint2* kids;        // [k] the children's first_index (bit 31 set for a leaf), left then right
+    DpRec* rec;         // [k]
     uint32_t max_prims; // max_prims_per_leaf the table was evaluated for
     uint32_t kbase;     // key offset of the BVH2 being processed (batched builds: the mesh's first primitive)
 };
@@ -55,20 +64,12 @@ struct MeshDesc {
     uint32_t pad[2];
 };
 
-// Table layout for k keys: the seven-decision arrays (84 bytes per key), the 16-byte leaf lists, the children.
-__host__ __device__ __forceinline__ size_t dp_plist_offset(size_t k) { return (k * 84 + 15) & ~(size_t)15; }
-__host__ __device__ __forceinline__ size_t dp_table_bytes(size_t k) { return dp_plist_offset(k) + k * 24 + 1024; }
+// Table of k entries (the base must be 128-byte aligned; device allocations are 256-byte aligned).
+__host__ __device__ __forceinline__ size_t dp_table_bytes(size_t k) { return k * sizeof(DpRec) + 1024; }
 __host__ __device__ __forceinline__ DpView dp_view_at(char* base, size_t k, uint32_t max_prims) {
+    (void)k;
     DpView v;
-    v.cost = reinterpret_cast<float*>(base);
-    v.meta = reinterpret_cast<uint2*>(base + k * 28);
-    v.sub = reinterpret_cast<uint32_t*>(base + k * 36);
-    v.nprims = reinterpret_cast<uint32_t*>(base + k * 64);
-    v.cwsize = reinterpret_cast<uint32_t*>(base + k * 68);
-    v.subdepth = reinterpret_cast<uint2*>(base + k * 72);
-    v.cwdepth = reinterpret_cast<uint32_t*>(base + k * 80);
-    v.plist = reinterpret_cast<uint4*>(base + dp_plist_offset(k));
-    v.kids = reinterpret_cast<uint2*>(base + dp_plist_offset(k) + k * 16);
+    v.rec = reinterpret_cast<DpRec*>(base);
     v.max_prims = max_prims;
     v.kbase = 0u;
     return v;
@@ -88,7 +89,7 @@ __device__ __forceinline__ uint32_t dpack(uint32_t k, uint32_t l, uint32_t r) { 
 
 // Decision i of an internal node's entry.
 __device__ __forceinline__ uint32_t node_decision(const DpView& dp, uint32_t first_index, uint32_t i) {
-    return dbyte(dp.meta[dp_key(dp, first_index)], i);
+    return dbyte(dp.rec[dp_key(dp, first_index)].meta, i);
 }
 __device__ __forceinline__ uint32_t node_kind(const DpView& dp, const Node32& nx, uint32_t i) {
     return nx.prim_count != 0u ? KIND_LEAF : dkind(node_decision(dp, nx.first_index, i));
@@ -121,13 +122,22 @@ __device__ __forceinline__ void load_child_dp(const DpView& dp, const Node32& c,
         o.cwdepth = 1u;
     } else {
         o.key = dp_key(dp, c.first_index);
-#pragma unroll
-        for (int k = 0; k < 7; ++k) o.cost[k] = dp.cost[o.key * 7 + k];
-        o.meta = dp.meta[o.key];
-        o.subdepth = dp.subdepth[o.key];
-        o.nprims = dp.nprims[o.key];
-        o.cwsize = dp.cwsize[o.key];
-        o.cwdepth = dp.cwdepth[o.key];
+        const DpRec& e = dp.rec[o.key];
+        const float4 c0 = *reinterpret_cast<const float4*>(&e.cost[0]);
+        const float4 c1 = *reinterpret_cast<const float4*>(&e.cost[4]);  // cost[4..6], nprims
+        const uint4 m = *reinterpret_cast<const uint4*>(&e.meta);         // meta, subdepth
+        o.cost[0] = c0.x;
+        o.cost[1] = c0.y;
+        o.cost[2] = c0.z;
+        o.cost[3] = c0.w;
+        o.cost[4] = c1.x;
+        o.cost[5] = c1.y;
+        o.cost[6] = c1.z;
+        o.nprims = __float_as_uint(c1.w);
+        o.meta = make_uint2(m.x, m.y);
+        o.subdepth = make_uint2(m.z, m.w);
+        o.cwsize = e.cwsize;
+        o.cwdepth = e.cwdepth;
     }
 }
 
@@ -135,7 +145,7 @@ __device__ __forceinline__ void load_child_dp(const DpView& dp, const Node32& c,
 // the contributions of its distributed descendants when it is DISTRIBUTE, nothing when it is a leaf child.
 __device__ __forceinline__ uint32_t child_contribution(const DpView& dp, const ChildDp& c, uint32_t d) {
     if (c.leaf) return 0u;
-    if (dkind(dbyte(c.meta, d)) == KIND_DISTRIBUTE) return dp.sub[c.key * 7 + d];
+    if (dkind(dbyte(c.meta, d)) == KIND_DISTRIBUTE) return dp.rec[c.key].sub[d];
     if (dkind(dbyte(c.meta, 0u)) == KIND_INTERNAL) return c.cwsize == 0u ? SUB_INVALID : c.cwsize;
     return 0u;
 }
@@ -218,28 +228,27 @@ __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box
             sd[i] = min(255u, max(child_depth(L, dl), child_depth(R, dr)));
         }
     }
-    const size_t key = dp_key(dp, first);
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        dp.cost[key * 7 + k] = cost[k];
-        dp.sub[key * 7 + k] = sub[k];
-    }
-    dp.meta[key] = make_uint2(meta[0] | meta[1] << 8 | meta[2] << 16 | meta[3] << 24,
-                              meta[4] | meta[5] << 8 | meta[6] << 16);
-    dp.subdepth[key] = make_uint2(sd[0] | sd[1] << 8 | sd[2] << 16 | sd[3] << 24, sd[4] | sd[5] << 8 | sd[6] << 16);
-    dp.nprims[key] = num_primitives;
-    dp.cwsize[key] = sub[0] == SUB_INVALID ? 0u : 1u + sub[0];
-    dp.cwdepth[key] = 1u + sd[0];
-    dp.kids[key] = make_uint2(l.first_index | (L.leaf ? KID_LEAF : 0u), r.first_index | (R.leaf ? KID_LEAF : 0u));
+    uint4 pl = make_uint4(0u, 0u, 0u, 0u);
     if (num_primitives <= 3u) {
         // The leaves count_primitives (bvh2_to_cwbvh.rs:197-211) visits, left subtree first.  Each child holds at
         // least one leaf, so the left list has one or two entries.
         uint4 a = make_uint4(l.first_index, 0u, 0u, 1u), c = make_uint4(r.first_index, 0u, 0u, 1u);
-        if (!L.leaf) a = dp.plist[L.key];
-        if (!R.leaf) c = dp.plist[R.key];
-        dp.plist[key] = a.w >= 2u ? make_uint4(a.x, a.y, c.x, num_primitives)
-                                  : make_uint4(a.x, c.x, c.y, num_primitives);
+        if (!L.leaf) a = dp.rec[L.key].plist;
+        if (!R.leaf) c = dp.rec[R.key].plist;
+        pl = a.w >= 2u ? make_uint4(a.x, a.y, c.x, num_primitives) : make_uint4(a.x, c.x, c.y, num_primitives);
     }
+    // The whole entry in seven 16-byte stores.
+    uint4* e = reinterpret_cast<uint4*>(&dp.rec[dp_key(dp, first)]);
+    e[0] = make_uint4(__float_as_uint(cost[0]), __float_as_uint(cost[1]), __float_as_uint(cost[2]),
+                      __float_as_uint(cost[3]));
+    e[1] = make_uint4(__float_as_uint(cost[4]), __float_as_uint(cost[5]), __float_as_uint(cost[6]), num_primitives);
+    e[2] = make_uint4(sub[0], sub[1], sub[2], sub[3]);
+    e[3] = make_uint4(sub[4], sub[5], sub[6], sub[0] == SUB_INVALID ? 0u : 1u + sub[0]);
+    e[4] = make_uint4(meta[0] | meta[1] << 8 | meta[2] << 16 | meta[3] << 24, meta[4] | meta[5] << 8 | meta[6] << 16,
+                      sd[0] | sd[1] << 8 | sd[2] << 16 | sd[3] << 24, sd[4] | sd[5] << 8 | sd[6] << 16);
+    e[5] = make_uint4(l.first_index | (L.leaf ? KID_LEAF : 0u), r.first_index | (R.leaf ? KID_LEAF : 0u), 1u + sd[0],
+                      0u);
+    e[6] = pl;
     return 0u;
 }
 
@@ -257,7 +266,7 @@ __device__ __forceinline__ int get_children_kids(const DpView& dp, uint32_t firs
     uint32_t kb = 0u;
     const uint32_t dl = dleft(meta0), dr = dright(meta0);
     if (dl == DIDX_INVALID || dr == DIDX_INVALID) return -2;
-    const uint2 k0 = dp.kids[dp_key(dp, first)];
+    const uint2 k0 = dp.rec[dp_key(dp, first)].kids;
     stk_slot[0] = first + 1u;
     stk_info[0] = k0.y;
     stk_i[0] = dr;
@@ -272,8 +281,8 @@ __device__ __forceinline__ int get_children_kids(const DpView& dp, uint32_t firs
         if (!(info & KID_LEAF)) {  // a leaf is never DISTRIBUTE
             const uint32_t f = info & ~KID_LEAF;
             const size_t key = dp_key(dp, f);
-            const uint2 m = dp.meta[key];
-            const uint2 kx = dp.kids[key];  // loaded with the decisions; used when the child is distributed
+            const uint2 m = dp.rec[key].meta;
+            const uint2 kx = dp.rec[key].kids;  // same line as the decisions; used when the child is distributed
             const uint32_t b = dbyte(m, i);
             if (dkind(b) == KIND_DISTRIBUTE) {
                 const uint32_t cl = dleft(b), cr = dright(b);
