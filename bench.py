@@ -12,11 +12,18 @@ with the algorithmic bytes of every ray (40 B of ray/hit + 80 B per node test + 
 from an instrumented run of the same kernel) divided by its event-timed duration.  The CPU baseline is the C++
 restatement of the crate's CPU path (oracle/), timed on this host's cores in this process.
 
+Before anything is timed, the context's tuning table is filled on the workload itself (obvhs_amd/csrc/tune.hip): the
+radix sort tile for the build's key count and the traversal LDS stack depth on the step's rays.  Neither changes a
+result (the sort order and the hits are identical for every choice); "tuning" reports the choices, and --no-tune
+keeps the defaults (8 keys per thread, 8 LDS entries).
+
 The "batched" object reports the batched form on 4,096 seeded icospheres (~20 M triangles, 1,024 rays each): one
 shared build and one traversal launch per step, next to the plain form (a build_cwbvh_from_tris call and a
-traversal launch per mesh) on the same inputs, with a check that both give identical hits.  The "large" object
-reports the 17.3 M-triangle scene built with pre-splits and reinsertion, traced with 4M incoherent closest-hit rays
-and 4M shadow rays.
+traversal launch per mesh) on the same inputs, with a check that both give identical hits.  Its meshes are built with
+the PLOC search distance the tuner picks per size class from tree costs, the smallest whose tree is within 1% of
+distance 14's (PlocSearchDistance.Auto); the time with distance 14 for every mesh is reported beside it.  The
+"large" object reports the 17.3 M-triangle scene built with pre-splits and reinsertion, traced with 4M incoherent
+closest-hit rays and 4M shadow rays.
 """
 import argparse
 import json
@@ -109,6 +116,34 @@ def cpu_baseline(tris, rays_all, threads, sample_rays):
     return dict(build_s=build_s, trace_s=trace_s, n_rays=len(sample), n_tris=len(tris))
 
 
+def autotune_batch(ob, ctx, tris, offsets, rays, ray_mesh):
+    """The tuning run before the batched leg (tune.hip): the radix sort tile for the batch's key count; the PLOC
+    search distance of each size class from the class's median-size mesh (the smallest distance whose tree cost is
+    within 1% of distance 14's, the preset; Auto builds then resolve it per mesh); the traversal LDS depth on the
+    mesh nearest the mean size, whose class the batched traversal uses, with its rays repeated 16 times."""
+    t0 = time.perf_counter()
+    counts = np.diff(offsets).astype(np.int64)
+    sort_items = ob.tune_sort(ctx, int(counts.sum()))
+    radius = {}
+    for c in range(4):
+        idx = np.nonzero([(ob.size_class(k) == c and k >= 2) for k in counts])[0]
+        if len(idx):
+            i = int(idx[np.argsort(counts[idx])[len(idx) // 2]])
+            r, costs = ob.tune_ploc_radius(ctx, tris[offsets[i]:offsets[i + 1]], 3, 0.01, baseline=14)
+            radius[str(c)] = {"distance": int(r), "mesh_tris": int(counts[i]),
+                              "cost_by_distance": {str(d): round(float(v), 6)
+                                                   for d, v in zip((1, 2, 6, 14, 24, 32), costs)}}
+    i = int(np.argmin(np.abs(counts - counts.mean())))
+    sub = np.ascontiguousarray(tris[offsets[i]:offsets[i + 1]])
+    cw = ob.build_cwbvh_from_tris(ctx, sub, ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3))
+    sub_rays = np.tile(rays[ray_mesh == i], 16)
+    depth = ob.tune_traversal(ctx, cw, sub_rays, cw.reorder_triangles(sub))
+    cw.close()
+    return {"sort_items": sort_items, "ploc_search_distance_by_size_class": radius,
+            "traversal_lds_depth": depth, "traversal_size_class": ob.size_class(int(counts[i])),
+            "tuning_s": round(time.perf_counter() - t0, 2)}
+
+
 def bench_batch(args, ob, ctx):
     """Batched form: 4,096 icospheres (log-uniform 64..65,536 triangles rounded down to 20*4^s) built in shared
     launches and traced with 1,024 rays per mesh in one launch per step; beside it the plain form on the same
@@ -121,15 +156,24 @@ def bench_batch(args, ob, ctx):
     tris, offsets, rays, ray_mesh = scenes.icosphere_batch(m, seed=0, rays_per_mesh=args.batch_rays_per_mesh)
     n_tris, n_rays = len(tris), len(rays)
     d_tris, d_rays, d_mesh = ctx.upload(tris), ctx.upload(rays), ctx.upload(ray_mesh)
-    params = ob.BvhBuildParams(ploc_search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3)
+    tuned = None if args.no_tune else autotune_batch(ob, ctx, tris, offsets, rays, ray_mesh)
+    fixed = ob.BvhBuildParams(ploc_search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3)
+    params = ob.BvhBuildParams(ploc_search_distance=ob.PlocSearchDistance.Auto, search_depth_threshold=0,
+                               max_prims_per_leaf=3)
     log(f"batched build: {m} meshes, {n_tris} triangles")
-    batch = ob.build_cwbvh_batch(ctx, d_tris, offsets, params)
-    ms = []
-    for _ in range(max(1, args.build_reps)):
-        ctx.timer_start()
-        ob.build_cwbvh_batch(ctx, d_tris, offsets, params, reuse=batch)
-        ms.append(ctx.timer_stop())
-    build_ms = float(np.median(ms))
+
+    def time_build(p):
+        bt = ob.build_cwbvh_batch(ctx, d_tris, offsets, p)
+        ms = []
+        for _ in range(max(1, args.build_reps)):
+            ctx.timer_start()
+            ob.build_cwbvh_batch(ctx, d_tris, offsets, p, reuse=bt)
+            ms.append(ctx.timer_stop())
+        return bt, float(np.median(ms))
+
+    batch14, build_ms_r14 = time_build(fixed)
+    batch14.close()
+    batch, build_ms = time_build(params)
     d_bvh_tris = batch.reorder_triangles(d_tris)
     hits = ob.DeviceArray(ctx, ob.RAY_HIT, n_rays)
 
@@ -150,6 +194,7 @@ def bench_batch(args, ob, ctx):
     result = {"workload": f"{m} icospheres (log-uniform 64..65536 tris, rounded down to 20*4^s) = {n_tris} tris; "
                           f"{args.batch_rays_per_mesh} rays per mesh = {n_rays} closest-hit rays",
               "build_ms": round(build_ms, 3), "build_mtris_per_s": round(n_tris / build_ms / 1e3, 2),
+              "build_ms_search_distance_14": round(build_ms_r14, 3), "tuning": tuned,
               "trace_mrays_per_s": round(n_rays / trace_s / 1e6, 2), "trace_ms": round(trace_s * 1e3, 4),
               "ploc_passes": info["ploc_passes"], "cwbvh_nodes": info["total_nodes"],
               "hit_fraction": round(float(np.isfinite(batch_hits["t"]).mean()), 4)}
@@ -247,7 +292,7 @@ def bench_rebuild(args, ob, ctx, tris, d_tris, fresh_build_ms):
         t0 = time.perf_counter()
         ob.native.check(lib.obvhs_bvh2_set_node_aabbs(ctx.handle, b.handle, d_ids.ptr, d_boxes.ptr, k), ctx.handle)
         ob.native.check(lib.obvhs_bvh2_rebuild_path_flags(ctx.handle, b.handle, d_ids.ptr, k, flags.ptr), ctx.handle)
-        ob.native.check(lib.obvhs_bvh2_partial_rebuild(ctx.handle, b.handle, flags.ptr, sd, 0), ctx.handle)
+        ob.native.check(lib.obvhs_bvh2_partial_rebuild(ctx.handle, b.handle, flags.ptr, sd, 64, 0), ctx.handle)
         ctx.synchronize()
         t1 = time.perf_counter()
         cw = ob.bvh2_to_cwbvh(b, 3, True, False)
@@ -260,7 +305,7 @@ def bench_rebuild(args, ob, ctx, tris, d_tris, fresh_build_ms):
         b = fresh()
         t0 = time.perf_counter()
         ob.native.check(lib.obvhs_bvh2_set_node_aabbs(ctx.handle, b.handle, d_ids.ptr, d_boxes.ptr, k), ctx.handle)
-        ob.native.check(lib.obvhs_bvh2_full_rebuild(ctx.handle, b.handle, sd, 0), ctx.handle)
+        ob.native.check(lib.obvhs_bvh2_full_rebuild(ctx.handle, b.handle, sd, 64, 0), ctx.handle)
         cw = ob.bvh2_to_cwbvh(b, 3, True, False)
         ctx.synchronize()
         full_emit_ms.append((time.perf_counter() - t0) * 1e3)
@@ -495,6 +540,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tuning table (no tuning runs)")
     ap.add_argument("--pmc-probe", action="store_true", help="one copy kernel + one traversal per batch, then exit")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the HBM traffic file from two --pmc-probe counter runs, then exit")
@@ -530,6 +576,9 @@ def main():
     d_tris = ctx.upload(tris)
     params = ob.BvhBuildParams(ploc_search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3)
 
+    tuning = {}
+    if not args.no_tune:  # the radix sort tile for this key count (tune.hip; the order is the same for every tile)
+        tuning["sort_items"] = ob.tune_sort(ctx, n_tris)
     log(f"building CWBVH over {n_tris} triangles")
     cw = ob.build_cwbvh_from_tris(ctx, d_tris, params)
     ctx.synchronize()
@@ -559,6 +608,8 @@ def main():
     bhits = ob.DeviceArray(ctx, ob.RAY_HIT, max(1, len(bounce)))
     n_rays = len(prim_rays) + len(bounce)
     log(f"{len(prim_rays)} primary + {len(bounce)} bounce rays")
+    if not args.no_tune:  # the traversal LDS stack depth on this workload's rays (the hits are the same for all)
+        tuning["traversal_lds_depth"] = ob.tune_traversal(ctx, cw, np.concatenate([prim_rays, bounce]), d_bvh_tris)
 
     if args.pmc_probe:
         # Counter runs (rocprofv3 --pmc): one device-to-device copy of known size (a blit kernel: the counters'
@@ -633,6 +684,7 @@ def main():
         "build": {"ms": round(build_med, 3), "mtris_per_s": round(n_tris / build_med / 1e3, 2),
                   "all_ms": [round(x, 3) for x in build_ms], "ploc_passes": int(stats.ploc_iterations),
                   "ploc_cluster_sum": int(stats.ploc_cluster_sum), "cwbvh_nodes": int(stats.cwbvh_node_count)},
+        "tuning": tuning,
     }
     if args.rebuild_frac > 0:
         out["rebuild"] = bench_rebuild(args, ob, ctx, tris, d_tris, build_med)

@@ -65,7 +65,9 @@ typedef struct obvhs_cwbvh_node {                                               
 } obvhs_cwbvh_node;
 
 /* BvhBuildParams (lib.rs:207-230).  ploc_search_distance is the numeric distance {1,2,6,14,24,32}
- * (PlocSearchDistance, ploc/mod.rs:534-562); sort_precision 64 or 128 (SortPrecision, ploc/mod.rs:657-661). */
+ * (PlocSearchDistance, ploc/mod.rs:534-562), or 0 for the distance of the context's tuning table (below);
+ * sort_precision 64 or 128 (SortPrecision, ploc/mod.rs:657-661).  Every search_distance argument below takes 0
+ * the same way. */
 typedef struct obvhs_build_params {
     uint32_t pre_split;
     uint32_t ploc_search_distance;
@@ -105,6 +107,38 @@ obvhs_status obvhs_memset_device(obvhs_context* ctx, void* dst, int value, uint6
 obvhs_status obvhs_memcpy_dtod(obvhs_context* ctx, void* dst, const void* src, uint64_t bytes);  /* async */
 /* Keep each build's Morton codes and sort permutation for obvhs_bvh2_copy_sort_trace (parity tests). */
 obvhs_status obvhs_context_set_trace(obvhs_context* ctx, int enable);
+
+/* ---- tuning table (no reference counterpart: choices of this implementation, per context) ----
+ * Size class of n primitives (or sort keys): 0 below 4,096, 1 below 65,536, 2 below 1,048,576, else 3.
+ *   ploc_radius[c]    : the PlocSearchDistance a build with search distance 0 ("auto") uses for a BVH of class c;
+ *                       batched builds resolve it per mesh.  Changes the tree (each result equals the build with
+ *                       that distance).  Default 14.
+ *   sort_items[c]     : keys per thread of the radix sort's tiles (4, 8 or 16) for a sort of class c.  Default 8.
+ *   trav_lds_depth[c] : traversal stack entries kept in LDS (4, 8 or 16) for a BVH of class c.  Default 8.
+ * The last two never change a result, only the speed.  obvhs_tune_* fill entries by measurement. */
+typedef struct obvhs_tune_table {
+    uint32_t ploc_radius[4];
+    uint32_t sort_items[4];
+    uint32_t trav_lds_depth[4];
+} obvhs_tune_table;
+uint32_t obvhs_size_class(uint64_t n);
+obvhs_status obvhs_tune_get(const obvhs_context* ctx, obvhs_tune_table* out);
+obvhs_status obvhs_tune_set(obvhs_context* ctx, const obvhs_tune_table* table); /* rejects values out of range */
+/* Times the radix sort of n seeded 63-bit keys with each tile size and stores the fastest in sort_items[class(n)]. */
+obvhs_status obvhs_tune_sort(obvhs_context* ctx, uint32_t n, uint32_t* out_items);
+/* Times closest-hit (any_hit = 0) or occlusion traversal of the given rays with each LDS depth and stores the
+ * fastest in trav_lds_depth[class(prim count)]. */
+obvhs_status obvhs_tune_traversal(obvhs_context* ctx, const obvhs_cwbvh* bvh, const obvhs_triangle* d_bvh_tris,
+                                  const obvhs_ray* d_rays, uint32_t n, int any_hit, uint32_t* out_depth);
+/* Builds the triangles with every PlocSearchDistance (1, 2, 6, 14, 24, 32), evaluates each tree's CWBVH cost for
+ * max_prims_per_leaf (the conversion's cost model, calculate_cost bvh2_to_cwbvh.rs:220-344, at the root), and stores
+ * in ploc_radius[class(n)] the smallest distance whose cost is within `tolerance` (relative) of the reference cost:
+ * the cost at distance `baseline` (one of the six: the fastest build as good as that distance's tree), or with
+ * baseline 0 the lowest of the six (the best tree).  The choice depends on the costs only, not on timing, so it is
+ * reproducible.  out_costs (6 entries) may be NULL. */
+obvhs_status obvhs_tune_ploc_radius(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
+                                    uint32_t max_prims_per_leaf, uint32_t baseline, float tolerance,
+                                    uint32_t* out_radius, float* out_costs);
 /* Stream timer: records a start/stop event on the context stream; obvhs_timer_stop synchronises. */
 obvhs_status obvhs_timer_start(obvhs_context* ctx);
 obvhs_status obvhs_timer_stop(obvhs_context* ctx, float* ms);
@@ -195,8 +229,8 @@ obvhs_status obvhs_bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, 
 /* PlocBuilder::partial_rebuild(bvh, |i| d_flags[i] != 0, search_distance, sort_precision,
  * search_depth_threshold) (ploc/rebuild.rs:98-134), sort_precision 64 or 128: the flagged inner nodes under the
  * root are opened, the leaves and unflagged subtrees they expose are rebuilt into a new top with PLOC, and the new
- * nodes go to the freed sibling pairs, highest first.  Subtrees that were not opened keep their slots.  Clusters with equal Morton codes are
- * ordered by their old slot (the crate's unstable sort leaves that order open). */
+ * nodes go to the freed sibling pairs, highest first.  Subtrees that were not opened keep their slots.  Clusters
+ * with equal Morton codes are ordered by their old slot (the crate's unstable sort leaves that order open). */
 obvhs_status obvhs_bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,
                                         uint32_t search_distance, uint32_t sort_precision,
                                         uint32_t search_depth_threshold);

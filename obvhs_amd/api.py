@@ -17,7 +17,10 @@ from .types import (AABB, BVH2_NODE, BVH2_NODE_SMALL, CWBVH_NODE, RAY, RAY_HIT, 
 
 
 class PlocSearchDistance(enum.IntEnum):
-    """ploc/mod.rs:534-562"""
+    """ploc/mod.rs:534-562.  Auto (0) is not a distance of the crate: builds given it use the distance of the
+    context's tuning table for the BVH's size class (see tune_ploc_radius); from_u32 keeps the crate's conversion
+    and rejects 0."""
+    Auto = 0
     Minimum = 1
     VeryLow = 2
     Low = 6
@@ -28,9 +31,16 @@ class PlocSearchDistance(enum.IntEnum):
     @classmethod
     def from_u32(cls, v):
         try:
+            if int(v) == 0:
+                raise ValueError
             return cls(int(v))
         except ValueError:
             raise ValueError(f"Invalid value for PlocSearchDistance: {v}") from None  # ploc/mod.rs:559
+
+    @classmethod
+    def for_build(cls, v):
+        """A builder argument: a distance of the crate, or Auto."""
+        return cls.Auto if int(v) == 0 else cls.from_u32(v)
 
 
 class SortPrecision(enum.IntEnum):
@@ -46,7 +56,7 @@ class BvhBuildParams:
                  reinsertion_batch_ratio=0.0, post_collapse_reinsertion_batch_ratio_multiplier=0.0,
                  sort_precision=SortPrecision.U64, max_prims_per_leaf=1, collapse_traversal_cost=1.0):
         self.pre_split = bool(pre_split)
-        self.ploc_search_distance = PlocSearchDistance(ploc_search_distance)
+        self.ploc_search_distance = PlocSearchDistance(ploc_search_distance)  # Auto (0) allowed
         self.search_depth_threshold = int(search_depth_threshold)
         self.reinsertion_batch_ratio = float(reinsertion_batch_ratio)
         self.post_collapse_reinsertion_batch_ratio_multiplier = float(post_collapse_reinsertion_batch_ratio_multiplier)
@@ -117,6 +127,20 @@ class Context:
 
     def set_trace(self, enable=True):
         check(lib().obvhs_context_set_trace(self.handle, int(bool(enable))), self.handle)
+
+    def tune_table(self):
+        """The context's tuning table: per size class (see size_class) the PLOC distance of Auto builds, the radix
+        sort tile (keys per thread) and the traversal LDS stack depth."""
+        t = native.TuneTable()
+        check(lib().obvhs_tune_get(self.handle, C.byref(t)), self.handle)
+        return {"ploc_radius": list(t.ploc_radius), "sort_items": list(t.sort_items),
+                "trav_lds_depth": list(t.trav_lds_depth)}
+
+    def set_tune_table(self, table):
+        t = native.TuneTable()
+        for k in ("ploc_radius", "sort_items", "trav_lds_depth"):
+            getattr(t, k)[:] = [int(v) for v in table[k]]
+        check(lib().obvhs_tune_set(self.handle, C.byref(t)), self.handle)
 
     def upload(self, array):
         return DeviceArray.from_host(self, array)
@@ -310,7 +334,7 @@ class PlocBuilder:
 
     def build_with_bvh(self, bvh, search_distance, aabbs, indices=None, sort_precision=SortPrecision.U64,
                        search_depth_threshold=0):
-        sd = PlocSearchDistance.from_u32(int(search_distance))
+        sd = PlocSearchDistance.for_build(search_distance)
         is_tris = (isinstance(aabbs, DeviceArray) and aabbs.dtype == TRIANGLE) or \
             (isinstance(aabbs, np.ndarray) and aabbs.dtype == TRIANGLE)
         dt = TRIANGLE if is_tris else AABB
@@ -334,7 +358,7 @@ class PlocBuilder:
 
     def full_rebuild(self, bvh, search_distance, sort_precision=SortPrecision.U64, search_depth_threshold=0):
         """ploc/rebuild.rs:53-79: rebuild `bvh` from its current leaves."""
-        sd = PlocSearchDistance.from_u32(int(search_distance))
+        sd = PlocSearchDistance.for_build(search_distance)
         check(lib().obvhs_bvh2_full_rebuild(self.ctx.handle, bvh.handle, int(sd), int(SortPrecision(sort_precision)),
                                             int(search_depth_threshold)), self.ctx.handle)
         return bvh
@@ -343,7 +367,7 @@ class PlocBuilder:
                         search_depth_threshold=0):
         """ploc/rebuild.rs:98-134.  `should_remove` is one flag per node (a device u8 array such as
         compute_rebuild_path_flags returns, or a host bool/u8 array): the closure |i| flags[i]."""
-        sd = PlocSearchDistance.from_u32(int(search_distance))
+        sd = PlocSearchDistance.for_build(search_distance)
         sp = SortPrecision(sort_precision)
         n = bvh.node_count
         if isinstance(should_remove, DeviceArray):
@@ -701,6 +725,42 @@ def log2_threshold_table():
     return out
 
 
+def size_class(n):
+    """Size class of n primitives or keys: 0 below 4,096, 1 below 65,536, 2 below 1,048,576, else 3."""
+    return int(lib().obvhs_size_class(int(n)))
+
+
+def tune_sort(ctx, n):
+    """Times the radix sort of n keys with each tile size; stores and returns the fastest (keys per thread)."""
+    out = C.c_uint32()
+    check(lib().obvhs_tune_sort(ctx.handle, int(n), C.byref(out)), ctx.handle)
+    return out.value
+
+
+def tune_traversal(ctx, cwbvh, rays, bvh_tris, any_hit=False):
+    """Times traversal of the rays with each LDS stack depth; stores and returns the fastest for the BVH's class."""
+    dr, n = _as_device(ctx, rays, RAY)
+    dt, _ = _as_device(ctx, bvh_tris, TRIANGLE)
+    out = C.c_uint32()
+    check(lib().obvhs_tune_traversal(ctx.handle, cwbvh.handle, dt.ptr, dr.ptr, n, int(bool(any_hit)),
+                                     C.byref(out)), ctx.handle)
+    return out.value
+
+
+def tune_ploc_radius(ctx, triangles, max_prims_per_leaf=3, tolerance=0.01, baseline=PlocSearchDistance.Medium):
+    """Builds the triangles with every PlocSearchDistance and picks the smallest whose CWBVH cost is within
+    `tolerance` of the cost at `baseline` (Auto/0: of the lowest cost); stores it for the size class (Auto builds
+    use it).  Returns (distance, costs of the six distances)."""
+    d, n = _as_device(ctx, triangles, TRIANGLE)
+    out = C.c_uint32()
+    costs = np.zeros(6, np.float32)
+    check(lib().obvhs_tune_ploc_radius(ctx.handle, d.ptr, n, int(max_prims_per_leaf), int(baseline),
+                                       float(tolerance), C.byref(out), costs.ctypes.data_as(C.c_void_p)),
+          ctx.handle)
+    return PlocSearchDistance(out.value), costs
+
+
 __all__ = ["Server", "CwBvhBatch", "build_cwbvh_batch", "split_aabbs_preset", "PlocSearchDistance", "SortPrecision",
            "BvhBuildParams", "Context", "DeviceArray", "Bvh2", "PlocBuilder", "compute_rebuild_path_flags",
-           "CwBvh", "bvh2_to_cwbvh", "build_cwbvh_from_tris", "log2_threshold_table", "ObvhsError"]
+           "CwBvh", "bvh2_to_cwbvh", "build_cwbvh_from_tris", "log2_threshold_table", "ObvhsError", "size_class",
+           "tune_sort", "tune_traversal", "tune_ploc_radius"]

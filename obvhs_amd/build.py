@@ -9,7 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libobvhs_hip.so")
-SOURCES = ["capi.hip", "sort.hip", "ploc.hip", "cwbvh.hip", "traverse.hip", "batch.hip", "quality.hip", "serve.hip"]
+SOURCES = ["capi.hip", "sort.hip", "ploc.hip", "cwbvh.hip", "traverse.hip", "batch.hip", "quality.hip", "serve.hip",
+           "tune.hip"]
 
 # Bit-exactness with the CPU path: no FMA contraction, IEEE f32 div/sqrt, denormals kept (HIP default).
 # Packed FP32 instructions are disabled for correctness on this hardware pool.

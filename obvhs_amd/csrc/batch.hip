@@ -173,16 +173,19 @@ __global__ void __launch_bounds__(T256) k_b_gather(const Node32* __restrict__ le
 }
 
 // Neighbour pass (ploc/mod.rs:329-419) with windows clipped to the cluster's mesh segment.
+// Each mesh searches with its own distance (MeshDesc::radius, at most rmax, the width of the LDS window); the
+// distance-1 rule applies to a mesh whose distance is 1 and to every mesh during the first passes (before_sdt).
 __global__ void __launch_bounds__(T256) k_b_search(const Node32* __restrict__ cur,
                                                    const uint32_t* __restrict__ cmesh,
                                                    const SegState* __restrict__ seg, BatchMisc* misc,
-                                                   uint32_t r, uint32_t dist1, int8_t* __restrict__ merge) {
+                                                   const MeshDesc* __restrict__ meshes, uint32_t rmax,
+                                                   uint32_t before_sdt, int8_t* __restrict__ merge) {
     __shared__ float s[6][T256 + 2 * MAX_RADIUS];
     const uint32_t total = misc->total;
     const uint32_t base = blockIdx.x * T256;
     if (base >= total) return;
-    const int wstart = (int)base - (int)r;
-    const int wlen = T256 + 2 * (int)r;
+    const int wstart = (int)base - (int)rmax;
+    const int wlen = T256 + 2 * (int)rmax;
     for (int w = threadIdx.x; w < wlen; w += T256) {
         const int g = wstart + w;
         if (g >= 0 && (uint32_t)g < total) {
@@ -198,10 +201,13 @@ __global__ void __launch_bounds__(T256) k_b_search(const Node32* __restrict__ cu
     __syncthreads();
     const uint32_t i = base + threadIdx.x;
     if (i >= total) return;
-    const SegState sg = seg[cmesh[i]];
+    const uint32_t mi = cmesh[i];
+    const SegState sg = seg[mi];
     if (sg.count <= 1u) return;
+    const uint32_t r = meshes[mi].radius;
+    const bool dist1 = r == 1u || before_sdt != 0u;
     const uint32_t li = i - sg.start;
-    const int lw = (int)threadIdx.x + (int)r;
+    const int lw = (int)threadIdx.x + (int)rmax;
     const Box me{s[0][lw], s[1][lw], s[2][lw], s[3][lw], s[4][lw], s[5][lw]};
     int8_t mv;
     if (dist1) {
@@ -480,12 +486,15 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
     const uint32_t n = h_offsets[m];
     B->n_total = n;
     B->meshes.assign(m, MeshDesc{});
+    uint32_t rmax = 1u;  // the widest search window of any mesh
     for (uint32_t k = 0; k < m; ++k) {
         if (h_offsets[k + 1] < h_offsets[k]) return OBVHS_ERR_INVALID_ARG;
         MeshDesc& md = B->meshes[k];
         md.prim_base = h_offsets[k];
         md.node_base = 2u * h_offsets[k];
         md.prim_count = h_offsets[k + 1] - h_offsets[k];
+        md.radius = r != 0u ? r : radius_for(ctx, md.prim_count);
+        rmax = std::max(rmax, md.radius);
     }
     OBVHS_TRY(B->tri_offsets.ensure((size_t)(m + 1) * 4), "alloc offsets");
     OBVHS_TRY(B->mesh_table.ensure((size_t)m * sizeof(MeshDesc)), "alloc mesh table");
@@ -538,11 +547,12 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
                                          misc);
     k_b_morton<<<nblk, T256, 0, st>>>(leaves, offsets, m, n, bounds, keys);
     OBVHS_TRY(hipGetLastError(), "batch leaf launch");
-    OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 63, S.sort_scratch.ptr, st), "batch morton sort");
+    const int items = sort_items_for(ctx, n);
+    OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 63, S.sort_scratch.ptr, st, items), "batch morton sort");
     k_b_mesh_keys<<<nblk, T256, 0, st>>>(vals, offsets, m, n, keys);
     int mbits = 1;
     while (mbits < 32 && (1ull << mbits) < m) mbits++;
-    OBVHS_TRY(radix_sort_pairs(keys, vals, false, n, mbits, S.sort_scratch.ptr, st), "batch mesh sort");
+    OBVHS_TRY(radix_sort_pairs(keys, vals, false, n, mbits, S.sort_scratch.ptr, st, items), "batch mesh sort");
     Node32* cur = S.clusters_a.as<Node32>();
     Node32* nxt = S.clusters_b.as<Node32>();
     k_b_gather<<<nblk, T256, 0, st>>>(leaves, vals, offsets, m, n, cur, cmesh[0]);
@@ -557,9 +567,9 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
     for (;;) {
         const uint32_t blocks = (known + T256 - 1) / T256;
         for (int q = 0; q < PASSES_PER_SYNC; ++q) {
-            const uint32_t dist1 = (r == 1 || pass < sdt) ? 1u : 0u;
             k_b_clear_active<<<1, 64, 0, st>>>(misc);
-            k_b_search<<<blocks, T256, 0, st>>>(cur, cmesh[cm], seg[cs], misc, r, dist1, merge);
+            k_b_search<<<blocks, T256, 0, st>>>(cur, cmesh[cm], seg[cs], misc, d_meshes, rmax,
+                                                pass < sdt ? 1u : 0u, merge);
             k_b_count<<<blocks, T256, 0, st>>>(merge, cmesh[cm], seg[cs], misc, blocksums);
             k_b_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc);
             k_b_segments<<<(m + 3u) / 4u, T256, 0, st>>>(merge, cmesh[cm], seg[cs], seg[cs ^ 1], blocksums, m,
@@ -603,8 +613,8 @@ obvhs_status obvhs_batch_build_cwbvh(obvhs_context* ctx, const obvhs_triangle* d
     if (!ctx || !params || !out || !tri_offsets) return OBVHS_ERR_INVALID_ARG;
     if (params->pre_split || params->reinsertion_batch_ratio > 0.0f || params->sort_precision != 64u)
         return OBVHS_ERR_INVALID_ARG;
-    const uint32_t r = params->ploc_search_distance;
-    if (!(r == 1 || r == 2 || r == 6 || r == 14 || r == 24 || r == 32)) return OBVHS_ERR_INVALID_ARG;
+    const uint32_t r = params->ploc_search_distance;  // 0: per mesh from the tuning table
+    if (!(r == 0 || r == 1 || r == 2 || r == 6 || r == 14 || r == 24 || r == 32)) return OBVHS_ERR_INVALID_ARG;
     if (tri_offsets[0] != 0u || tri_offsets[mesh_count] >= (1u << 30)) return OBVHS_ERR_INVALID_ARG;
     if (tri_offsets[mesh_count] > 0 && !d_tris) return OBVHS_ERR_INVALID_ARG;
     uint32_t mp = params->max_prims_per_leaf;
@@ -673,6 +683,7 @@ obvhs_status obvhs_batch_ray_traverse(obvhs_context* ctx, const obvhs_batch* b, 
         const obvhs_status ss = traverse_scratch(ctx, ts);
         if (ss != OBVHS_OK) return ss;
     }
+    ts.lds_depth = lds_depth_for(ctx, b->meshes.empty() ? 0u : b->n_total / b->meshes.size());  // mean mesh
     hipError_t e = launch_traverse_meshes(b->cw_nodes.as<CwNode>(), b->mesh_table.as<MeshDesc>(),
                                           (uint32_t)b->meshes.size(), d_ray_mesh,
                                           d_bvh_tris, d_rays, n, 0, d_hits, nullptr, ts, ctx->stream);
@@ -689,6 +700,7 @@ obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch
         const obvhs_status ss = traverse_scratch(ctx, ts);
         if (ss != OBVHS_OK) return ss;
     }
+    ts.lds_depth = lds_depth_for(ctx, b->meshes.empty() ? 0u : b->n_total / b->meshes.size());  // mean mesh
     hipError_t e = launch_traverse_meshes(b->cw_nodes.as<CwNode>(), b->mesh_table.as<MeshDesc>(),
                                           (uint32_t)b->meshes.size(), d_ray_mesh,
                                           d_bvh_tris, d_rays, n, 1, nullptr, d_miss, ts, ctx->stream);

@@ -204,6 +204,7 @@ static obvhs_status ploc_entry(obvhs_context* ctx, const obvhs_triangle* d_tris,
     if (n > 0 && !d_tris && !d_aabbs) return OBVHS_ERR_INVALID_ARG;
     if (n >= (1u << 31)) return OBVHS_ERR_INVALID_ARG;
     if (sort_precision != 64u && sort_precision != 128u) return OBVHS_ERR_INVALID_ARG;  // SortPrecision
+    if (search_distance == 0u) search_distance = radius_for(ctx, n);  // "auto": the tuning table's distance
     CTX_TRY(hipSetDevice(ctx->device), "set device");
     obvhs_bvh2* bvh = reuse ? reuse : new (std::nothrow) obvhs_bvh2();
     if (!bvh) return OBVHS_ERR_OOM;
@@ -471,6 +472,7 @@ obvhs_status obvhs_bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, con
                                         uint32_t search_depth_threshold) {
     if (!ctx || !bvh || (bvh->node_count && !d_flags)) return OBVHS_ERR_INVALID_ARG;
     if (sort_precision != 64u && sort_precision != 128u) return OBVHS_ERR_INVALID_ARG;
+    if (search_distance == 0u) search_distance = radius_for(ctx, bvh->prim_count);
     CTX_TRY(hipSetDevice(ctx->device), "set device");
     return bvh2_partial_rebuild(ctx, bvh, d_flags, search_distance, sort_precision == 128u, search_depth_threshold);
 }
@@ -479,6 +481,7 @@ obvhs_status obvhs_bvh2_full_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, uint32
                                      uint32_t sort_precision, uint32_t search_depth_threshold) {
     if (!ctx || !bvh) return OBVHS_ERR_INVALID_ARG;
     if (sort_precision != 64u && sort_precision != 128u) return OBVHS_ERR_INVALID_ARG;
+    if (search_distance == 0u) search_distance = radius_for(ctx, bvh->prim_count);
     CTX_TRY(hipSetDevice(ctx->device), "set device");
     return bvh2_full_rebuild(ctx, bvh, search_distance, sort_precision == 128u, search_depth_threshold);
 }
@@ -566,6 +569,7 @@ obvhs_status obvhs_cwbvh_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh* bvh
         const obvhs_status ss = traverse_scratch(ctx, ts);
         if (ss != OBVHS_OK) return ss;
     }
+    ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
     CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0,
                             d_hits, nullptr, nullptr, nullptr, ts, ctx->stream),
             "traverse");
@@ -582,6 +586,7 @@ obvhs_status obvhs_cwbvh_ray_traverse_miss(obvhs_context* ctx, const obvhs_cwbvh
         const obvhs_status ss = traverse_scratch(ctx, ts);
         if (ss != OBVHS_OK) return ss;
     }
+    ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
     CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 1,
                             nullptr, d_miss, nullptr, nullptr, ts, ctx->stream),
             "traverse miss");
@@ -599,6 +604,7 @@ obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cw
         const obvhs_status ss = traverse_scratch(ctx, ts);
         if (ss != OBVHS_OK) return ss;
     }
+    ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
     CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0,
                             d_hits, nullptr, d_node_visits, d_tri_tests, ts, ctx->stream),
             "traverse counted");

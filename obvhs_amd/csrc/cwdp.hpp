@@ -61,7 +61,8 @@ struct MeshDesc {
     uint32_t cw_base;     // first CWBVH node
     uint32_t cw_count;
     uint32_t cw_depth;
-    uint32_t pad[2];
+    uint32_t radius;      // the mesh's PLOC search distance
+    uint32_t pad;
 };
 
 // Table of k entries (the base must be 128-byte aligned; device allocations are 256-byte aligned).

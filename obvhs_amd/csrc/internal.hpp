@@ -81,6 +81,8 @@ struct obvhs_context {
     void* pinned = nullptr;  // small pinned host staging area
     bool keep_trace = false; // keep Morton codes and the sort permutation of each build (parity tests)
     uint32_t trav_lanes = 0;  // persistent traversal grid size (0 = not yet queried)
+    obvhs_tune_table tune{{14u, 14u, 14u, 14u}, {8u, 8u, 8u, 8u}, {8u, 8u, 8u, 8u}};
+    obvhs::DevBuf tune_scratch;  // working memory of the tuning runs
     std::string last_error;
 };
 
@@ -122,7 +124,12 @@ hipError_t launch_iota(uint32_t* p, uint32_t n, hipStream_t stream);
 // sort.hip
 size_t radix_sort_scratch_bytes(uint32_t n);
 hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identity, uint32_t n, int bits,
-                            void* scratch, hipStream_t stream);
+                            void* scratch, hipStream_t stream, int items);
+// Size class of n primitives or keys (the tuning table's index).
+inline uint32_t size_class(uint64_t n) { return n < 4096u ? 0u : n < 65536u ? 1u : n < 1048576u ? 2u : 3u; }
+inline int sort_items_for(const obvhs_context* ctx, uint64_t n) { return (int)ctx->tune.sort_items[size_class(n)]; }
+inline uint32_t lds_depth_for(const obvhs_context* ctx, uint64_t n) { return ctx->tune.trav_lds_depth[size_class(n)]; }
+inline uint32_t radius_for(const obvhs_context* ctx, uint64_t n) { return ctx->tune.ploc_radius[size_class(n)]; }
 
 // ploc.hip
 obvhs_status ploc_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_src_slots, uint32_t k,
@@ -156,6 +163,7 @@ hipError_t launch_reorder_triangles(const obvhs_triangle* tris, const uint32_t* 
 struct TravScratch {
     void* spill = nullptr;
     uint32_t lanes = 0;
+    uint32_t lds_depth = 8;  // stack entries in LDS for the launch (4, 8 or 16); results do not depend on it
 };
 obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts);
 hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,

@@ -664,14 +664,15 @@ static obvhs_status sort_codes(obvhs_context* ctx, uint64_t* keys, const uint64_
                                uint32_t n) {
     hipStream_t st = ctx->stream;
     void* scratch = ctx->scratch.sort_scratch.ptr;
+    const int items = sort_items_for(ctx, n);
     if (!keys_hi) {
-        OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 63, scratch, st), "radix sort");
+        OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 63, scratch, st, items), "radix sort");
         return OBVHS_OK;
     }
-    OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 64, scratch, st), "radix sort (low words)");
+    OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 64, scratch, st, items), "radix sort (low words)");
     k_gather_u64<<<(n + T256 - 1) / T256, T256, 0, st>>>(keys_hi, vals, n, keys);
     OBVHS_TRY(hipGetLastError(), "gather high words");
-    OBVHS_TRY(radix_sort_pairs(keys, vals, false, n, 62, scratch, st), "radix sort (high words)");
+    OBVHS_TRY(radix_sort_pairs(keys, vals, false, n, 62, scratch, st, items), "radix sort (high words)");
     return OBVHS_OK;
 }
 

@@ -1267,14 +1267,14 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
         k_cand_keys<<<(ncand + T256 - 1) / T256, T256, 0, st>>>(nodes, take, Q.keys.as<uint64_t>());
         OBVHS_TRY(hipGetLastError(), "candidate launch");
         OBVHS_TRY(radix_sort_pairs(Q.keys.as<uint64_t>(), Q.vals.as<uint32_t>(), true, ncand, 32,
-                                   Q.sort_scratch.ptr, st),
+                                   Q.sort_scratch.ptr, st, sort_items_for(ctx, ncand)),
                   "candidate sort");
         k_find_reinsertion<<<fthreads / T256, T256, 0, st>>>(nodes, N, parents, Q.vals.as<uint32_t>(), count,
                                                               Q.stack.as<uint2>(), sd, Q.rfrom.as<uint32_t>(),
                                                               Q.rto.as<uint32_t>(), Q.rkeys.as<uint64_t>(), misc);
         OBVHS_TRY(hipGetLastError(), "find reinsertion launch");
         OBVHS_TRY(radix_sort_pairs(Q.rkeys.as<uint64_t>(), Q.rvals.as<uint32_t>(), true, count, 32,
-                                   Q.sort_scratch.ptr, st),
+                                   Q.sort_scratch.ptr, st, sort_items_for(ctx, count)),
                   "reinsertion sort");
         OBVHS_TRY(hipMemcpyAsync(h, misc, 8, hipMemcpyDeviceToHost, st), "read reinsertion count");
         OBVHS_TRY(hipStreamSynchronize(st), "reinsertion count sync");

@@ -280,7 +280,7 @@ void launch_batch(obvhs_server* S, std::vector<Req*>& ready) {
         const uint32_t slot = S->seg_slot;
         S->seg_slot = (S->seg_slot + 1) % (2 * S->cfg.max_inflight + 2);
         TraceSegment* hs = S->seg_host + (size_t)slot * MAX_SEGMENTS;
-        uint32_t nseg = 0, total = 0;
+        uint32_t nseg = 0, total = 0, largest = 0;
         for (Req* r : ready) {
             if ((r->kind == REQ_TRACE_MISS) != (mode == 1)) continue;
             Scene* sc = S->scenes[r->scene];
@@ -292,6 +292,7 @@ void launch_batch(obvhs_server* S, std::vector<Req*>& ready) {
             sg.out = r->out;
             sg.node_count = v.cw ? v.cw->node_count : 0u;
             sg.prim_count = v.cw ? v.cw->prim_count : 0u;
+            largest = std::max(largest, sg.prim_count);
             sg.begin = total;
             sg.count = r->n;
             hs[nseg++] = sg;
@@ -304,7 +305,9 @@ void launch_batch(obvhs_server* S, std::vector<Req*>& ready) {
         if (nseg == 0) continue;
         TraceSegment* ds = S->seg_dev.as<TraceSegment>() + (size_t)slot * MAX_SEGMENTS;
         hipError_t e = hipMemcpyAsync(ds, hs, (size_t)nseg * sizeof(TraceSegment), hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = launch_traverse_segments(ds, nseg, total, mode, S->trav, st);
+        TravScratch ts = S->trav;
+        ts.lds_depth = lds_depth_for(S->tctx, largest);  // the depth tuned for the launch's largest BVH
+        if (e == hipSuccess) e = launch_traverse_segments(ds, nseg, total, mode, ts, st);
         if (e != hipSuccess) {
             for (Req* r : ready)
                 if ((r->kind == REQ_TRACE_MISS) == (mode == 1)) r->status = OBVHS_ERR_DEVICE;

@@ -9,7 +9,8 @@
 //   scan      : one workgroup per digit, exclusive scan of counts[digit][*] offset by the digits below it;
 //   downsweep : wave-level multisplit ranking (eight 64-lane ballots per item) gives each key its stable rank
 //               inside the tile, the tile is reordered in LDS and written out coalesced per digit run.
-// Tiles are 2048 keys (256 threads x 8 items); each wave owns a contiguous 512-key segment.
+// Tiles are 256 threads x ITEMS keys (ITEMS = 4, 8 or 16, the sort tile the context's tuning table picks per size
+// class; 8 by default); each wave owns a contiguous 64 x ITEMS key segment.  Every tile size gives the same order.
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -18,8 +19,7 @@ namespace obvhs {
 namespace {
 
 constexpr int SORT_THREADS = 256;
-constexpr int SORT_ITEMS = 8;
-constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+constexpr int SORT_MIN_ITEMS = 4;  // the scratch is sized for the smallest tile (the most tiles)
 constexpr int SORT_WAVES = SORT_THREADS / 64;
 
 __global__ void __launch_bounds__(SORT_THREADS) k_zero_u32(uint32_t* p, uint32_t n) {
@@ -27,16 +27,18 @@ __global__ void __launch_bounds__(SORT_THREADS) k_zero_u32(uint32_t* p, uint32_t
     if (i < n) p[i] = 0u;
 }
 
+template <int ITEMS>
 __global__ void __launch_bounds__(SORT_THREADS)
     k_radix_upsweep(const uint64_t* __restrict__ keys, uint32_t n, uint32_t shift, uint32_t num_tiles,
                     uint32_t* __restrict__ counts, uint32_t* __restrict__ totals) {
+    constexpr int SORT_TILE = SORT_THREADS * ITEMS;
     __shared__ uint32_t hist[256];
     const uint32_t tid = threadIdx.x;
     hist[tid] = 0u;
     __syncthreads();
     const uint32_t base = blockIdx.x * SORT_TILE;
 #pragma unroll
-    for (int j = 0; j < SORT_ITEMS; ++j) {
+    for (int j = 0; j < ITEMS; ++j) {
         uint32_t idx = base + j * SORT_THREADS + tid;
         if (idx < n) {
             uint32_t d = (uint32_t)(keys[idx] >> shift) & 0xffu;
@@ -90,10 +92,12 @@ __global__ void __launch_bounds__(SORT_THREADS)
     (void)dummy;
 }
 
+template <int ITEMS>
 __global__ void __launch_bounds__(SORT_THREADS)
     k_radix_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                       uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t shift,
                       const uint32_t* __restrict__ offsets, uint32_t num_tiles) {
+    constexpr int SORT_TILE = SORT_THREADS * ITEMS;
     __shared__ uint32_t whist[SORT_WAVES][256];
     __shared__ uint32_t digit_start[256];
     __shared__ uint32_t gbase[256];
@@ -112,13 +116,13 @@ __global__ void __launch_bounds__(SORT_THREADS)
     for (int w = 0; w < SORT_WAVES; ++w) whist[w][tid] = 0u;
     __syncthreads();
 
-    const uint32_t seg = tile_base + wave * (64 * SORT_ITEMS);
-    uint64_t k[SORT_ITEMS];
-    uint32_t v[SORT_ITEMS];
-    uint32_t rank[SORT_ITEMS];
+    const uint32_t seg = tile_base + wave * (64 * ITEMS);
+    uint64_t k[ITEMS];
+    uint32_t v[ITEMS];
+    uint32_t rank[ITEMS];
     const uint64_t lt = lanemask_lt();
 #pragma unroll
-    for (int j = 0; j < SORT_ITEMS; ++j) {
+    for (int j = 0; j < ITEMS; ++j) {
         uint32_t idx = seg + j * 64 + lane;
         bool valid = idx < n;
         k[j] = valid ? kin[idx] : 0ull;
@@ -154,7 +158,7 @@ __global__ void __launch_bounds__(SORT_THREADS)
     gbase[tid] = offsets[(size_t)tid * num_tiles + tile];
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < SORT_ITEMS; ++j) {
+    for (int j = 0; j < ITEMS; ++j) {
         uint32_t idx = seg + j * 64 + lane;
         if (idx < n) {
             uint32_t d = (uint32_t)(k[j] >> shift) & 0xffu;
@@ -176,7 +180,8 @@ __global__ void __launch_bounds__(SORT_THREADS)
 }  // namespace
 
 size_t radix_sort_scratch_bytes(uint32_t n) {
-    uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    constexpr uint32_t MIN_TILE = SORT_THREADS * SORT_MIN_ITEMS;
+    uint32_t tiles = (n + MIN_TILE - 1) / MIN_TILE;
     size_t b = 0;
     b += (size_t)n * 8;                     // alternate keys
     b += (size_t)n * 4;                     // alternate values
@@ -187,10 +192,11 @@ size_t radix_sort_scratch_bytes(uint32_t n) {
 
 // Sorts keys[0..n) ascending (stable) with values.  On entry values may be null (identity 0..n).  `bits` is the
 // number of low key bits that can be non-zero (63 for Morton64).  Result lands in keys/values (even pass count)
-// or is copied back.  scratch must hold radix_sort_scratch_bytes(n).
-hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identity, uint32_t n, int bits,
-                            void* scratch, hipStream_t stream) {
-    if (n == 0) return hipSuccess;
+// or is copied back.  scratch must hold radix_sort_scratch_bytes(n).  items: keys per thread of a tile (4, 8, 16).
+template <int ITEMS>
+static hipError_t radix_sort_impl(uint64_t* keys, uint32_t* values, bool values_identity, uint32_t n, int bits,
+                                  void* scratch, hipStream_t stream) {
+    constexpr int SORT_TILE = SORT_THREADS * ITEMS;
     uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
     char* p = static_cast<char*>(scratch);
     auto take = [&](size_t bytes) {
@@ -210,10 +216,12 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identi
     uint32_t* vout = valt;
     for (int pass = 0; pass < passes; ++pass) {
         uint32_t shift = (uint32_t)(pass * 8);
-        k_radix_upsweep<<<tiles, SORT_THREADS, 0, stream>>>(kin, n, shift, tiles, counts, totals + pass * 256);
+        k_radix_upsweep<ITEMS><<<tiles, SORT_THREADS, 0, stream>>>(kin, n, shift, tiles, counts,
+                                                                   totals + pass * 256);
         k_radix_scan<<<256, SORT_THREADS, 0, stream>>>(counts, totals + pass * 256, tiles);
         const uint32_t* vsrc = (pass == 0 && values_identity) ? nullptr : vin;
-        k_radix_downsweep<<<tiles, SORT_THREADS, 0, stream>>>(kin, vsrc, kout, vout, n, shift, counts, tiles);
+        k_radix_downsweep<ITEMS><<<tiles, SORT_THREADS, 0, stream>>>(kin, vsrc, kout, vout, n, shift, counts,
+                                                                     tiles);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
@@ -225,6 +233,16 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identi
         e = hipMemcpyAsync(values, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, stream);
     }
     return e;
+}
+
+hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identity, uint32_t n, int bits,
+                            void* scratch, hipStream_t stream, int items) {
+    if (n == 0) return hipSuccess;
+    switch (items) {
+        case 4: return radix_sort_impl<4>(keys, values, values_identity, n, bits, scratch, stream);
+        case 16: return radix_sort_impl<16>(keys, values, values_identity, n, bits, scratch, stream);
+        default: return radix_sort_impl<8>(keys, values, values_identity, n, bits, scratch, stream);
+    }
 }
 
 }  // namespace obvhs

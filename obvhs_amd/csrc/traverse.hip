@@ -8,9 +8,10 @@
 //     slot 31, a pop of the empty stack ends traversal;
 //   * the node test is the SSE path of cwbvh/simd.rs:17-100 (mul then add, SSE min/max, tmin clamped to 1e-4,
 //     inclusive compare) and the primitive test is Triangle::intersect (triangle.rs:35-76).
-// Stack storage: the bottom LDS_STACK entries of each lane's stack live in LDS (one 64-lane column per wave,
-// entry-major), deeper entries in a per-lane column of an HBM spill array; a short LDS footprint keeps 8 waves per
-// SIMD resident, which the latency-bound walk needs.  Launches are persistent: a fixed grid of lanes (the resident
+// Stack storage: the bottom LDS entries of each lane's stack live in LDS (one 64-lane column per wave, entry-major;
+// 4, 8 or 16 entries, the depth the context's tuning table holds for the BVH's size class, 8 by default), deeper
+// entries in a per-lane column of an HBM spill array; a short LDS footprint keeps 8 waves per SIMD resident, which
+// the latency-bound walk needs.  Launches are persistent: a fixed grid of lanes (the resident
 // capacity) walks the rays with a stride, so each lane owns one spill column.  Nodes are fetched as five 16-byte
 // loads, triangles (in primitive_indices order) as three.
 #include <algorithm>
@@ -124,7 +125,7 @@ __device__ __forceinline__ float tri_intersect(const float4& v0, const float4& v
 // segments that each name their own BVH, ray array and output array (SRC_SEGMENTS, the serving path).
 constexpr int SRC_PLAIN = 0, SRC_MESHES = 1, SRC_SEGMENTS = 2;
 
-template <int MODE, bool COUNT, int SRC = SRC_PLAIN>
+template <int MODE, bool COUNT, int SRC = SRC_PLAIN, int LDS = LDS_STACK>
 __global__ void __launch_bounds__(TRAV_THREADS)
     k_traverse(const uint4* __restrict__ nodes_in, uint32_t node_count_in, uint32_t prim_count_in,
                const float4* __restrict__ tris_in, const float4* __restrict__ rays_in, uint32_t n,
@@ -132,11 +133,11 @@ __global__ void __launch_bounds__(TRAV_THREADS)
                uint32_t* __restrict__ tri_tests, const MeshDesc* __restrict__ meshes,
                const uint32_t* __restrict__ ray_mesh, const TraceSegment* __restrict__ segs,
                uint2* __restrict__ spill) {
-    __shared__ uint2 stack[LDS_STACK][TRAV_THREADS];
+    __shared__ uint2 stack[LDS][TRAV_THREADS];
     const uint32_t lane = threadIdx.x;
     const uint32_t lanes = gridDim.x * TRAV_THREADS;
     const uint32_t gid = blockIdx.x * TRAV_THREADS + lane;
-    uint2* my_spill = spill + gid;  // entry k >= LDS_STACK at my_spill[(k - LDS_STACK) * lanes]
+    uint2* my_spill = spill + gid;  // entry k >= LDS at my_spill[(k - LDS) * lanes]
     // Rays are strided over the persistent grid: consecutive blocks take consecutive 64-ray spans.  Measured
     // alternatives on the 1.29M-triangle heightfield workload: giving each XCD a contiguous eighth of the rays (so
     // coherent rays share one L2) was about 25% slower; waves fetching further 64-ray spans from a counter (one
@@ -224,8 +225,8 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             const uint32_t base = cur.x;
             cur.y &= ~(1u << off);
             if ((cur.y & 0xff000000u) != 0u) {
-                if (sp < LDS_STACK) stack[sp][lane] = cur;
-                else my_spill[(size_t)(sp - LDS_STACK) * lanes] = cur;
+                if (sp < LDS) stack[sp][lane] = cur;
+                else my_spill[(size_t)(sp - LDS) * lanes] = cur;
                 sp = min(sp + 1, STACK_SIZE - 1);
             }
             const uint32_t slot_index = (off - 24u) ^ (oct & 0xffu);
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
         if (prim.y == 0u && (cur.y & 0xff000000u) == 0u) {
             if (sp == 0) break;
             sp -= 1;
-            cur = sp < LDS_STACK ? stack[sp][lane] : my_spill[(size_t)(sp - LDS_STACK) * lanes];
+            cur = sp < LDS ? stack[sp][lane] : my_spill[(size_t)(sp - LDS) * lanes];
         }
     }
     if (MODE == 0) {
@@ -378,6 +379,17 @@ __global__ void __launch_bounds__(TRAV_THREADS)
 
 }  // namespace
 
+// Launches k_traverse<MODE, COUNT, SRC, depth> with the LDS stack depth the launch was tuned for (4, 8 or 16).
+template <int MODE, bool COUNT, int SRC, class... A>
+static void launch_depth(uint32_t depth, uint32_t blocks, hipStream_t stream, A... args) {
+    if (depth == 4u)
+        k_traverse<MODE, COUNT, SRC, 4><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+    else if (depth == 16u)
+        k_traverse<MODE, COUNT, SRC, 16><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+    else
+        k_traverse<MODE, COUNT, SRC, 8><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+}
+
 // Persistent grid: enough blocks for the rays, at most the resident capacity (the spill buffer's lane count).
 static uint32_t trav_blocks(uint32_t n, const TravScratch& ts) {
     const uint32_t want = (n + TRAV_THREADS - 1) / TRAV_THREADS;
@@ -394,18 +406,19 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
     const float4* tr = reinterpret_cast<const float4*>(tris);
     const float4* ry = reinterpret_cast<const float4*>(rays);
     uint2* sp = static_cast<uint2*>(ts.spill);
+    const MeshDesc* no_meshes = nullptr;
+    const uint32_t* no_map = nullptr;
+    const TraceSegment* no_segs = nullptr;
     if (mode == 0) {
         if (node_visits)
-            k_traverse<0, true><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss,
-                                                                     node_visits, tri_tests, nullptr, nullptr,
-                                                                     nullptr, sp);
+            launch_depth<0, true, SRC_PLAIN>(ts.lds_depth, blocks, stream, nd, node_count, prim_count, tr, ry, n,
+                                             hits, miss, node_visits, tri_tests, no_meshes, no_map, no_segs, sp);
         else
-            k_traverse<0, false><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits,
-                                                                      miss, nullptr, nullptr, nullptr, nullptr,
-                                                                      nullptr, sp);
+            launch_depth<0, false, SRC_PLAIN>(ts.lds_depth, blocks, stream, nd, node_count, prim_count, tr, ry, n,
+                                              hits, miss, node_visits, tri_tests, no_meshes, no_map, no_segs, sp);
     } else {
-        k_traverse<1, false><<<blocks, TRAV_THREADS, 0, stream>>>(nd, node_count, prim_count, tr, ry, n, hits, miss,
-                                                                  nullptr, nullptr, nullptr, nullptr, nullptr, sp);
+        launch_depth<1, false, SRC_PLAIN>(ts.lds_depth, blocks, stream, nd, node_count, prim_count, tr, ry, n, hits,
+                                          miss, node_visits, tri_tests, no_meshes, no_map, no_segs, sp);
     }
     return hipGetLastError();
 }
@@ -420,14 +433,14 @@ hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, u
     const float4* tr = reinterpret_cast<const float4*>(tris);
     const float4* ry = reinterpret_cast<const float4*>(rays);
     uint2* sp = static_cast<uint2*>(ts.spill);
+    uint32_t* no_counts = nullptr;
+    const TraceSegment* no_segs = nullptr;
     if (mode == 0)
-        k_traverse<0, false, SRC_MESHES><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits,
-                                                                              miss, nullptr, nullptr, meshes,
-                                                                              ray_mesh, nullptr, sp);
+        launch_depth<0, false, SRC_MESHES>(ts.lds_depth, blocks, stream, nd, mesh_count, 0u, tr, ry, n, hits, miss,
+                                           no_counts, no_counts, meshes, ray_mesh, no_segs, sp);
     else
-        k_traverse<1, false, SRC_MESHES><<<blocks, TRAV_THREADS, 0, stream>>>(nd, mesh_count, 0u, tr, ry, n, hits,
-                                                                              miss, nullptr, nullptr, meshes,
-                                                                              ray_mesh, nullptr, sp);
+        launch_depth<1, false, SRC_MESHES>(ts.lds_depth, blocks, stream, nd, mesh_count, 0u, tr, ry, n, hits, miss,
+                                           no_counts, no_counts, meshes, ray_mesh, no_segs, sp);
     return hipGetLastError();
 }
 
@@ -436,12 +449,19 @@ hipError_t launch_traverse_segments(const TraceSegment* segs, uint32_t nseg, uin
     if (n == 0 || nseg == 0) return hipSuccess;
     const uint32_t blocks = trav_blocks(n, ts);
     uint2* sp = static_cast<uint2*>(ts.spill);
+    const uint4* no_nodes = nullptr;
+    const float4* no_f4 = nullptr;
+    obvhs_ray_hit* no_hits = nullptr;
+    uint8_t* no_miss = nullptr;
+    uint32_t* no_counts = nullptr;
+    const MeshDesc* no_meshes = nullptr;
+    const uint32_t* no_map = nullptr;
     if (mode == 0)
-        k_traverse<0, false, SRC_SEGMENTS><<<blocks, TRAV_THREADS, 0, stream>>>(
-            nullptr, nseg, 0u, nullptr, nullptr, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, segs, sp);
+        launch_depth<0, false, SRC_SEGMENTS>(ts.lds_depth, blocks, stream, no_nodes, nseg, 0u, no_f4, no_f4, n,
+                                             no_hits, no_miss, no_counts, no_counts, no_meshes, no_map, segs, sp);
     else
-        k_traverse<1, false, SRC_SEGMENTS><<<blocks, TRAV_THREADS, 0, stream>>>(
-            nullptr, nseg, 0u, nullptr, nullptr, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, segs, sp);
+        launch_depth<1, false, SRC_SEGMENTS>(ts.lds_depth, blocks, stream, no_nodes, nseg, 0u, no_f4, no_f4, n,
+                                             no_hits, no_miss, no_counts, no_counts, no_meshes, no_map, segs, sp);
     return hipGetLastError();
 }
 
@@ -453,7 +473,8 @@ obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts) {
             return set_hip_error(ctx, hipGetLastError(), "device properties");
         ctx->trav_lanes = (uint32_t)std::max(1, prop.multiProcessorCount) * 32u * TRAV_THREADS;
     }
-    const size_t bytes = (size_t)ctx->trav_lanes * (STACK_SIZE - LDS_STACK) * sizeof(uint2);
+    // columns for the shallowest LDS depth a launch may use (4): every depth's spill fits
+    const size_t bytes = (size_t)ctx->trav_lanes * (STACK_SIZE - 4) * sizeof(uint2);
     const hipError_t e = ctx->scratch.trav_spill.ensure(bytes);
     if (e != hipSuccess) return set_hip_error(ctx, e, "alloc traversal spill");
     ts.spill = ctx->scratch.trav_spill.ptr;

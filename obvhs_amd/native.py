@@ -42,6 +42,11 @@ class BuildExt(C.Structure):
     _fields_ = [("split_max_total", C.c_uint64), ("ratio_sequence", C.c_void_p), ("ratio_sequence_len", C.c_uint32)]
 
 
+class TuneTable(C.Structure):
+    """obvhs_tune_table: per size class PLOC distance for "auto" builds, sort tile, traversal LDS stack depth."""
+    _fields_ = [("ploc_radius", C.c_uint32 * 4), ("sort_items", C.c_uint32 * 4), ("trav_lds_depth", C.c_uint32 * 4)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_rays", C.c_uint32), ("max_inflight", C.c_uint32),
                 ("update_streams", C.c_uint32), ("pad", C.c_uint32)]
@@ -80,6 +85,12 @@ SIGNATURES = {
     "obvhs_context_destroy": (INT, [P]),
     "obvhs_context_synchronize": (INT, [P]),
     "obvhs_context_set_trace": (INT, [P, INT]),
+    "obvhs_size_class": (U32, [C.c_uint64]),
+    "obvhs_tune_get": (INT, [P, P]),
+    "obvhs_tune_set": (INT, [P, P]),
+    "obvhs_tune_sort": (INT, [P, U32, P]),
+    "obvhs_tune_traversal": (INT, [P, P, P, P, U32, INT, P]),
+    "obvhs_tune_ploc_radius": (INT, [P, P, U32, U32, U32, C.c_float, P, P]),
     "obvhs_last_error_message": (C.c_char_p, [P]),
     "obvhs_device_malloc": (INT, [P, U64, C.POINTER(P)]),
     "obvhs_device_free": (INT, [P, P]),
