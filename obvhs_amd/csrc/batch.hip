@@ -680,7 +680,7 @@ obvhs_status obvhs_batch_ray_traverse(obvhs_context* ctx, const obvhs_batch* b, 
     if (!ctx || !b || (n && (!d_rays || !d_ray_mesh || !d_hits))) return OBVHS_ERR_INVALID_ARG;
     TravScratch ts;
     if (n) {
-        const obvhs_status ss = traverse_scratch(ctx, ts);
+        const obvhs_status ss = traverse_scratch(ctx, ts, n);
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, b->meshes.empty() ? 0u : b->n_total / b->meshes.size());  // mean mesh
@@ -697,7 +697,7 @@ obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch
     if (!ctx || !b || (n && (!d_rays || !d_ray_mesh || !d_miss))) return OBVHS_ERR_INVALID_ARG;
     TravScratch ts;
     if (n) {
-        const obvhs_status ss = traverse_scratch(ctx, ts);
+        const obvhs_status ss = traverse_scratch(ctx, ts, n);
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, b->meshes.empty() ? 0u : b->n_total / b->meshes.size());  // mean mesh
@@ -760,7 +760,7 @@ obvhs_status obvhs_batch_tlas_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh
     if (b->n_total && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
     TravScratch ts;
     if (n) {
-        const obvhs_status ss = traverse_scratch(ctx, ts);
+        const obvhs_status ss = traverse_scratch(ctx, ts, n);
         if (ss != OBVHS_OK) return ss;
     }
     hipError_t e = launch_traverse_tlas(tlas->nodes.as<CwNode>(), tlas->node_count, tlas->prim_indices.as<uint32_t>(),

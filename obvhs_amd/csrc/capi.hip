@@ -566,7 +566,7 @@ obvhs_status obvhs_cwbvh_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh* bvh
     if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
     TravScratch ts;
     if (n) {
-        const obvhs_status ss = traverse_scratch(ctx, ts);
+        const obvhs_status ss = traverse_scratch(ctx, ts, n);
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
@@ -583,7 +583,7 @@ obvhs_status obvhs_cwbvh_ray_traverse_miss(obvhs_context* ctx, const obvhs_cwbvh
     if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
     TravScratch ts;
     if (n) {
-        const obvhs_status ss = traverse_scratch(ctx, ts);
+        const obvhs_status ss = traverse_scratch(ctx, ts, n);
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
@@ -601,7 +601,7 @@ obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cw
     if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
     TravScratch ts;
     if (n) {
-        const obvhs_status ss = traverse_scratch(ctx, ts);
+        const obvhs_status ss = traverse_scratch(ctx, ts, n);
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);

@@ -165,7 +165,8 @@ struct TravScratch {
     uint32_t lanes = 0;
     uint32_t lds_depth = 8;  // stack entries in LDS for the launch (4, 8 or 16); results do not depend on it
 };
-obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts);
+// Spill columns (and the lane count) for a traversal launch of n rays.
+obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts, uint64_t n);
 hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
                            const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,
                            uint32_t* node_visits, uint32_t* tri_tests, const TravScratch& ts, hipStream_t stream);

@@ -430,7 +430,7 @@ obvhs_status obvhs_server_create(int device, const obvhs_server_config* cfg, obv
     }
     const size_t nseg = (size_t)(2 * S->cfg.max_inflight + 2) * MAX_SEGMENTS;
     if (s == OBVHS_OK && S->seg_dev.ensure(nseg * sizeof(TraceSegment)) != hipSuccess) s = OBVHS_ERR_OOM;
-    if (s == OBVHS_OK) s = traverse_scratch(S->tctx, S->trav);
+    if (s == OBVHS_OK) s = traverse_scratch(S->tctx, S->trav, S->cfg.max_batch_rays);
     if (s == OBVHS_OK &&
         hipHostMalloc(reinterpret_cast<void**>(&S->seg_host), nseg * sizeof(TraceSegment), hipHostMallocDefault) !=
             hipSuccess)

@@ -11,9 +11,9 @@
 // Stack storage: the bottom LDS entries of each lane's stack live in LDS (one 64-lane column per wave, entry-major;
 // 4, 8 or 16 entries, the depth the context's tuning table holds for the BVH's size class, 8 by default), deeper
 // entries in a per-lane column of an HBM spill array; a short LDS footprint keeps 8 waves per SIMD resident, which
-// the latency-bound walk needs.  Launches are persistent: a fixed grid of lanes (the resident
-// capacity) walks the rays with a stride, so each lane owns one spill column.  Nodes are fetched as five 16-byte
-// loads, triangles (in primitive_indices order) as three.
+// the latency-bound walk needs.  A launch has one lane per ray (up to 16M lanes; beyond that the lanes stride over
+// the rays) and each lane owns one spill column.  Nodes are fetched as five 16-byte loads, triangles (in
+// primitive_indices order) as three.
 #include <algorithm>
 
 #include "common.hpp"
@@ -32,6 +32,7 @@ constexpr int TRAV_THREADS = 64;
 constexpr int STACK_SIZE = 32;  // TRAVERSAL_STACK_SIZE, cwbvh/mod.rs:60
 constexpr int LDS_STACK = TRAV_LDS_STACK;  // entries kept in LDS; STACK_SIZE - LDS_STACK spill to HBM
 constexpr float NODE_EPSILON = 0.0001f;  // cwbvh/node.rs:82
+constexpr uint64_t TRAV_MAX_LANES = 1u << 24;  // spill columns (224 B each) for at most 16M lanes: 3.8 GB
 
 struct RayData {
     float ox, oy, oz;
@@ -138,11 +139,11 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     const uint32_t lanes = gridDim.x * TRAV_THREADS;
     const uint32_t gid = blockIdx.x * TRAV_THREADS + lane;
     uint2* my_spill = spill + gid;  // entry k >= LDS at my_spill[(k - LDS) * lanes]
-    // Rays are strided over the persistent grid: consecutive blocks take consecutive 64-ray spans.  Measured
-    // alternatives on the 1.29M-triangle heightfield workload: giving each XCD a contiguous eighth of the rays (so
-    // coherent rays share one L2) was about 25% slower; waves fetching further 64-ray spans from a counter (one
-    // device-scope atomic per span) made the primary launch 1.8x slower; a copy of the nodes at a 128-byte stride
-    // (one cache line per node) was 7% slower.
+    // Consecutive blocks take consecutive 64-ray spans.  Measured alternatives on the 1.29M-triangle heightfield
+    // workload (with a grid of the resident capacity striding over the rays): giving each XCD a contiguous eighth of
+    // the rays (so coherent rays share one L2) was about 25% slower; waves fetching further 64-ray spans from a
+    // counter (one device-scope atomic per span) made the primary launch 1.8x slower; a copy of the nodes at a
+    // 128-byte stride (one cache line per node) was 7% slower.
     for (uint32_t i = gid; i < n; i += lanes) {
     const uint4* nodes = nodes_in;
     const float4* tris = tris_in;
@@ -390,7 +391,7 @@ static void launch_depth(uint32_t depth, uint32_t blocks, hipStream_t stream, A.
         k_traverse<MODE, COUNT, SRC, 8><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
 }
 
-// Persistent grid: enough blocks for the rays, at most the resident capacity (the spill buffer's lane count).
+// Grid: one lane per ray, at most the spill buffer's lane count (lanes then stride over the rays).
 static uint32_t trav_blocks(uint32_t n, const TravScratch& ts) {
     const uint32_t want = (n + TRAV_THREADS - 1) / TRAV_THREADS;
     const uint32_t b = std::min<uint32_t>(want, ts.lanes / TRAV_THREADS);
@@ -465,20 +466,27 @@ hipError_t launch_traverse_segments(const TraceSegment* segs, uint32_t nseg, uin
     return hipGetLastError();
 }
 
-// Spill columns for every lane of a persistent traversal grid: the device's compute units x 32 resident waves.
-obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts) {
+// Spill columns for a launch of n rays: one lane per ray up to TRAV_MAX_LANES lanes, at least the device's resident
+// capacity (compute units x 32 waves).  One ray per lane lets the dispatcher keep the CUs full as waves finish;
+// measured on the 1.29M-triangle heightfield, 2.07M primary rays took 0.300 ms against 0.364 ms on a persistent
+// grid of the resident capacity walking the rays with a stride (incoherent bounce rays: no change).  Launches of
+// more rays stride over TRAV_MAX_LANES lanes.  The spill costs capacity, not bandwidth: only stacks deeper than
+// the LDS part touch it.
+obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts, uint64_t n) {
     if (ctx->trav_lanes == 0) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess)
             return set_hip_error(ctx, hipGetLastError(), "device properties");
         ctx->trav_lanes = (uint32_t)std::max(1, prop.multiProcessorCount) * 32u * TRAV_THREADS;
     }
+    const uint64_t want = (std::min<uint64_t>(n, TRAV_MAX_LANES) + TRAV_THREADS - 1) / TRAV_THREADS * TRAV_THREADS;
+    const uint32_t lanes = (uint32_t)std::max<uint64_t>(ctx->trav_lanes, want);
     // columns for the shallowest LDS depth a launch may use (4): every depth's spill fits
-    const size_t bytes = (size_t)ctx->trav_lanes * (STACK_SIZE - 4) * sizeof(uint2);
+    const size_t bytes = (size_t)lanes * (STACK_SIZE - 4) * sizeof(uint2);
     const hipError_t e = ctx->scratch.trav_spill.ensure(bytes);
     if (e != hipSuccess) return set_hip_error(ctx, e, "alloc traversal spill");
     ts.spill = ctx->scratch.trav_spill.ptr;
-    ts.lanes = ctx->trav_lanes;
+    ts.lanes = lanes;
     return OBVHS_OK;
 }
 

@@ -137,7 +137,7 @@ obvhs_status obvhs_tune_traversal(obvhs_context* ctx, const obvhs_cwbvh* bvh, co
     if (!ctx || !bvh || n == 0 || !d_rays || (bvh->node_count && !d_bvh_tris)) return OBVHS_ERR_INVALID_ARG;
     TUNE_TRY(hipSetDevice(ctx->device), "set device");
     TravScratch ts;
-    const obvhs_status ss = traverse_scratch(ctx, ts);
+    const obvhs_status ss = traverse_scratch(ctx, ts, n);
     if (ss != OBVHS_OK) return ss;
     TUNE_TRY(ctx->tune_scratch.ensure((size_t)n * sizeof(obvhs_ray_hit)), "alloc tuning scratch");
     obvhs_ray_hit* hits = ctx->tune_scratch.as<obvhs_ray_hit>();
