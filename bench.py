@@ -14,8 +14,9 @@ restatement of the crate's CPU path (oracle/), timed on this host's cores in thi
 
 Before anything is timed, the context's tuning table is filled on the workload itself (obvhs_amd/csrc/tune.hip): the
 radix sort tile for the build's key count and the traversal LDS stack depth on the step's rays.  Neither changes a
-result (the sort order and the hits are identical for every choice); "tuning" reports the choices, and --no-tune
-keeps the defaults (8 keys per thread, 8 LDS entries).
+result (the sort order and the hits are identical for every choice); "tuning" reports the choices.  --no-tune keeps
+the defaults (8 keys per thread, 8 LDS entries); --sort-items / --lds-depth set a choice without measuring it (the
+profiling runs repeat the measured choice so that their kernels are the benchmark's).
 
 The "batched" object reports the batched form on 4,096 seeded icospheres (~20 M triangles, 1,024 rays each): one
 shared build and one traversal launch per step, next to the plain form (a build_cwbvh_from_tris call and a
@@ -541,6 +542,10 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tuning table (no tuning runs)")
+    ap.add_argument("--sort-items", type=int, default=0,
+                    help="set the sort tile (4, 8, 16) for every size class instead of measuring it")
+    ap.add_argument("--lds-depth", type=int, default=0,
+                    help="set the traversal LDS depth (4, 8, 16) for every size class instead of measuring it")
     ap.add_argument("--pmc-probe", action="store_true", help="one copy kernel + one traversal per batch, then exit")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the HBM traffic file from two --pmc-probe counter runs, then exit")
@@ -577,8 +582,17 @@ def main():
     params = ob.BvhBuildParams(ploc_search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3)
 
     tuning = {}
-    if not args.no_tune:  # the radix sort tile for this key count (tune.hip; the order is the same for every tile)
-        tuning["sort_items"] = ob.tune_sort(ctx, n_tris)
+    if args.sort_items or args.lds_depth:  # fixed entries (profiling runs repeat a measured choice)
+        t = ctx.tune_table()
+        if args.sort_items:
+            t["sort_items"] = [args.sort_items] * 4
+            tuning["sort_items"] = args.sort_items
+        if args.lds_depth:
+            t["trav_lds_depth"] = [args.lds_depth] * 4
+            tuning["traversal_lds_depth"] = args.lds_depth
+        ctx.set_tune_table(t)
+    if not args.no_tune and not args.sort_items:  # the radix sort tile for this key count (the order is the same
+        tuning["sort_items"] = ob.tune_sort(ctx, n_tris)  # for every tile)
     log(f"building CWBVH over {n_tris} triangles")
     cw = ob.build_cwbvh_from_tris(ctx, d_tris, params)
     ctx.synchronize()
@@ -608,8 +622,9 @@ def main():
     bhits = ob.DeviceArray(ctx, ob.RAY_HIT, max(1, len(bounce)))
     n_rays = len(prim_rays) + len(bounce)
     log(f"{len(prim_rays)} primary + {len(bounce)} bounce rays")
-    if not args.no_tune:  # the traversal LDS stack depth on this workload's rays (the hits are the same for all)
-        tuning["traversal_lds_depth"] = ob.tune_traversal(ctx, cw, np.concatenate([prim_rays, bounce]), d_bvh_tris)
+    if not args.no_tune and not args.lds_depth:  # the traversal LDS stack depth on this workload's rays (the hits
+        tuning["traversal_lds_depth"] = ob.tune_traversal(ctx, cw, np.concatenate([prim_rays, bounce]),
+                                                          d_bvh_tris)  # are the same for every depth)
 
     if args.pmc_probe:
         # Counter runs (rocprofv3 --pmc): one device-to-device copy of known size (a blit kernel: the counters'
