@@ -93,7 +93,9 @@ typedef struct obvhs_build_stats {
     uint64_t ploc_cluster_sum;    /* sum of the cluster counts over all PLOC passes */
 } obvhs_build_stats;
 
-/* ---- context, memory, timing (no reference counterpart: the crate is CPU-only) ---- */
+/* ---- context, memory, timing (no reference counterpart: the crate is CPU-only) ----
+ * A context keeps grow-only working memory until it is destroyed: build scratch for its largest build and 224 bytes
+ * of traversal stack spill per ray of its largest traversal launch (at most 16M rays' worth, 3.8 GB). */
 const char* obvhs_status_string(obvhs_status s);
 obvhs_status obvhs_context_create(int device, void* hip_stream, obvhs_context** out);
 obvhs_status obvhs_context_destroy(obvhs_context* ctx);
