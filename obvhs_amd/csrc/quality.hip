@@ -432,6 +432,8 @@ struct SplitMisc {
     uint32_t kept;         // candidates kept by the filter
     uint32_t err;
     uint32_t pad;
+    float sum;         // the running half-area sum at element sum_pos
+    uint32_t sum_pos;
 };
 
 __device__ __forceinline__ float vget(const float4& v, uint32_t d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
@@ -518,16 +520,26 @@ __global__ void __launch_bounds__(T256) k_split_prepare(const float4* __restrict
     if (__lane_id() == 0) atomicMax(&misc->largest_key, k);
 }
 
-// The crate's running sum `avg_area += half_area` in triangle order (cwbvh/builder.rs:34-41), reproduced exactly by
-// one workgroup without a sequential walk.  While the running sum s stays in one binade (ulp u), the f32 step
-// fl(s + x) equals s + u * round(x / u) for every x >= 0 -- unless x / u has fraction exactly 1/2 (a tie, decided by
-// the parity of s / u) or s + x leaves the binade.  So each tile's increments round(x / u) are prefix-summed as
-// integers; the first tie, binade crossing or non-finite value is then added with one IEEE f32 add (the sequential
-// step itself, from the exact s before it) and the scan resumes after it in the new binade.  The split thresholds
-// of split_aabbs_preset follow.
+// The crate's running sum `avg_area += half_area` in triangle order (cwbvh/builder.rs:34-41), reproduced exactly
+// without a sequential walk.  While the running sum s stays in one binade (ulp u), the f32 step fl(s + x) equals
+// s + u * round(x / u) for every x >= 0 -- unless x / u has fraction exactly 1/2 (a tie, decided by the parity of
+// s / u) or s + x leaves the binade.  So the increments round(x / u) are prefix-summed as integers; the first tie,
+// binade crossing or non-finite value is then added with one IEEE f32 add (the sequential step itself, from the
+// exact s before it) and the scan resumes after it.  Stops crowd the start of the sum (every binade crossing while s
+// is small) and are rare after it, so one workgroup scans the first SUM_SERIAL_SPAN elements tile by tile
+// (k_sum_serial), and the rest goes in passes over all CUs (k_sum_blocks: per-block increment totals; k_sum_pass:
+// one workgroup that jumps over the blocks without a stop, and steps through ties in place, since a tie keeps the
+// binade and so the block totals).  A pass ends at a binade crossing, and the host starts the next one.  The split
+// thresholds of split_aabbs_preset follow.
 constexpr int SUM_THREADS = 1024;
-constexpr int SUM_ITEMS = 16;  // 4 measured 4 ms slower per 17.3M-triangle presplit (more tile iterations)
+constexpr int SUM_ITEMS = 16;  // serial tile: 16K elements
 constexpr uint32_t SUM_TILE = SUM_THREADS * SUM_ITEMS;
+constexpr uint32_t SUM_CAP = 1u << 24;
+constexpr uint32_t SUM_NONE = 0xffffffffu;
+constexpr uint32_t SUM_SERIAL_SPAN = 1u << 18;
+constexpr int PASS_ITEMS = 16;
+constexpr uint32_t PASS_BLOCK = T256 * PASS_ITEMS;  // elements per k_sum_blocks workgroup
+constexpr int PASS_TILE_ITEMS = PASS_BLOCK / SUM_THREADS;
 
 __device__ __forceinline__ unsigned long long wave_inclusive_scan_u64(unsigned long long v) {
     const int lane = (int)__lane_id();
@@ -539,100 +551,334 @@ __device__ __forceinline__ unsigned long long wave_inclusive_scan_u64(unsigned l
     return v;
 }
 
-__global__ void __launch_bounds__(SUM_THREADS) k_split_avg(const float* __restrict__ ha, uint32_t n,
-                                                          SplitMisc* misc) {
-    __shared__ unsigned long long wtot[SUM_THREADS / 64];
-    __shared__ uint32_t first_stop;
-    __shared__ float s_next;
-    __shared__ uint32_t pos_next;
+// round(x / u) for the ulp u = 2^(be - 150) of the running sum's binade and a finite x >= 0 given by its bits, in
+// integer arithmetic; `tie` is set when x / u has fraction exactly 1/2.  Results of 2^24 and more are clamped to
+// 2^24: one such step alone carries the sum out of the binade, which stops the scan at it.
+__device__ __forceinline__ uint32_t ulp_steps(uint32_t xb, uint32_t be, bool& tie) {
+    const uint32_t ex = xb >> 23;
+    const uint32_t mx = (xb & 0x7fffffu) | (ex != 0u ? 0x800000u : 0u);
+    const int d = (int)be - (int)(ex != 0u ? ex : 1u);  // x / u = mx / 2^d
+    tie = false;
+    if (d < 0) return SUM_CAP;  // x is normal here (mx >= 2^23), so x / u >= 2^24
+    if (d == 0) return mx;
+    if (d > 24) return 0u;  // x / u < 2^24 / 2^25
+    const uint32_t half = 1u << (d - 1);
+    const uint32_t rem = mx & ((half << 1) - 1u);
+    tie = rem == half;
+    return (mx >> d) + (rem > half ? 1u : 0u);
+}
+
+// The running sum as the integer scan sees it: s = (m0 + k) * u after k more ulps.
+struct SumMode {
+    float s;
+    bool zero;  // s == 0: the first non-zero element starts the sum
+    bool scan;  // finite s > 0: the integer scan applies; otherwise every element that changes s is a stop
+    uint32_t be, m0;
+};
+__device__ __forceinline__ SumMode sum_mode(float s) {
+    const uint32_t bits = __float_as_uint(s);
+    const uint32_t be_raw = (bits >> 23) & 0xffu;
+    SumMode m;
+    m.s = s;
+    m.zero = s == 0.0f;
+    m.scan = isfinite(s) && s > 0.0f;
+    m.be = be_raw == 0u ? 1u : be_raw;  // subnormals share the first normal binade's ulp
+    m.m0 = m.zero ? 0u : (bits & 0x7fffffu) | (be_raw ? 0x800000u : 0u);
+    return m;
+}
+// x's increment in ulps (0 for a stop) and whether x stops the integer scan.
+__device__ __forceinline__ bool sum_rule(const SumMode& md, float x, uint32_t& q) {
+    const uint32_t xb = __float_as_uint(x);
+    bool tie;
+    const uint32_t steps = ulp_steps(xb, md.be, tie);
+    bool st;
+    if (md.zero)
+        st = (xb & 0x7fffffffu) != 0u;  // 0 + x = x: the first non-zero value starts the sum
+    else if (!md.scan)
+        st = !(md.s + x == md.s);  // e.g. an infinite sum stays itself until it turns NaN
+    else
+        st = xb >= 0x7f800000u || tie;  // a non-finite or negative x, or a tie: one IEEE add
+    q = !md.scan || st ? 0u : steps;
+    return st;
+}
+__device__ __forceinline__ bool sum_crosses(const SumMode& md, unsigned long long k) {
+    return md.scan && md.m0 + k >= SUM_CAP;
+}
+__device__ __forceinline__ float sum_value(const SumMode& md, unsigned long long k) {  // k below SUM_CAP - m0
+    if (!md.scan) return md.s;
+    return ldexpf((float)(md.m0 + k), (int)md.be - 150);
+}
+
+struct SumShared {
+    unsigned long long wtot[SUM_THREADS / 64];
+    unsigned long long total;
+    uint32_t first;
+    float s_before, s_after;
+};
+struct SumStep {
+    uint32_t idx;              // first stop in the tile, SUM_NONE for none
+    unsigned long long total;  // no stop: base + the tile's increments
+    float s_before, s_after;   // a stop: the exact sum before it and the IEEE step that adds it
+};
+
+// One block-wide step over `len` elements (thread tid holds elements tid * ITEMS + r; get(e) reads element e) from
+// `base` ulps above md.m0: the first stop and the sums around it, or the total.  Ends with a barrier, after which
+// the shared state may be reused.
+template <int ITEMS, class Get>
+__device__ __forceinline__ SumStep sum_step(const SumMode& md, unsigned long long base, uint32_t len, Get get,
+                                            SumShared& sh) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    float s = 0.0f;  // the running sum, identical in every thread
-    uint32_t pos = 0;
-    while (pos < n && !isnan(s)) {
-        float x[SUM_ITEMS];
-        unsigned long long a[SUM_ITEMS];
-        bool stop[SUM_ITEMS];
-        const bool zero = s == 0.0f;
-        const bool finite = isfinite(s);
-        const uint32_t bits = __float_as_uint(s);
-        const uint32_t be_raw = (bits >> 23) & 0xffu;
-        const uint32_t be = be_raw == 0u ? 1u : be_raw;  // subnormals share the first normal binade's ulp
-        const unsigned long long m0 =
-            zero ? 0ull : (unsigned long long)((bits & 0x7fffffu) | (be_raw ? 0x800000u : 0u));
-        const double inv_u = ldexp(1.0, 150 - (int)be);
-        unsigned long long tsum = 0ull;
+    uint32_t q[ITEMS];
+    uint32_t stops = 0u;  // bit r: element r stops the integer scan
+    uint32_t tsum = 0u;   // at most ITEMS * 2^24
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const uint32_t e = tid * ITEMS + (uint32_t)r;
+        q[r] = 0u;
+        if (e < len && sum_rule(md, get(e), q[r])) stops |= 1u << r;
+        tsum += q[r];
+    }
+    // block-wide exclusive prefix of the thread totals (element order)
+    const unsigned long long incl = wave_inclusive_scan_u64(tsum);
+    if (tid == 0) sh.first = SUM_NONE;
+    if (lane == 63u) sh.wtot[wave] = incl;
+    __syncthreads();
+    unsigned long long excl = base + incl - tsum;
+    for (uint32_t w = 0; w < wave; ++w) excl += sh.wtot[w];
+    unsigned long long run = excl;
+    uint32_t my_stop = SUM_NONE;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        run += q[r];
+        if ((((stops >> r) & 1u) || sum_crosses(md, run)) && my_stop == SUM_NONE) my_stop = tid * ITEMS + (uint32_t)r;
+    }
+    if (my_stop != SUM_NONE) atomicMin(&sh.first, my_stop);
+    __syncthreads();
+    const uint32_t idx = sh.first;
+    if (idx == SUM_NONE) {
+        if (tid == SUM_THREADS - 1) sh.total = run;  // below SUM_CAP - m0: no crossing
+    } else if (idx / ITEMS == tid) {
+        // every increment before the first stop is exact (a clamped one would cross, so stop, itself)
+        const int r0 = (int)(idx % ITEMS);
+        unsigned long long before = excl;
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r)
+            if (r < r0) before += q[r];
+        const float sb = sum_value(md, before);
+        sh.s_before = sb;
+        sh.s_after = sb + get(idx);  // the sequential step
+    }
+    __syncthreads();
+    SumStep out;
+    out.idx = idx;
+    out.total = sh.total;
+    out.s_before = sh.s_before;
+    out.s_after = sh.s_after;
+    __syncthreads();
+    return out;
+}
+
+__device__ __forceinline__ void split_finish(SplitMisc* misc, float s, uint32_t n) {
+    const float avg = s / (float)n;
+    const float largest = f32_from_order_key(misc->largest_key);
+    misc->avg = avg;
+    misc->thresh_low = avg * 3.0f;                                       // splits.rs:27
+    misc->thresh_high = fmaxf(avg * 4.0f, avg * 0.9f + largest * 0.1f);  // splits.rs:28
+}
+
+// From the state (misc->sum, misc->sum_pos), 16K-element tiles until the position reaches `limit` (or the sum turns
+// NaN).  The tiles go through LDS, one pad word after every SUM_ITEMS elements: filled by lane-contiguous loads and
+// read back as each thread's SUM_ITEMS consecutive elements (thread stride SUM_ITEMS + 1 words); the next tile is
+// loaded while the current one is scanned.
+__global__ void __launch_bounds__(SUM_THREADS) k_sum_serial(const float* __restrict__ ha, uint32_t n, uint32_t limit,
+                                                           SplitMisc* misc) {
+    __shared__ float tile[SUM_TILE + SUM_TILE / SUM_ITEMS];
+    __shared__ SumShared sh;
+    const uint32_t tid = threadIdx.x;
+    float nxt[SUM_ITEMS];
+    auto load_tile = [&](uint32_t base) {
 #pragma unroll
         for (int r = 0; r < SUM_ITEMS; ++r) {
-            const uint32_t k = pos + tid * SUM_ITEMS + (uint32_t)r;
-            x[r] = k < n ? ha[k] : 0.0f;
-            a[r] = 0ull;
-            stop[r] = false;
-            if (k >= n) continue;
-            if (zero) {
-                stop[r] = x[r] != 0.0f;  // 0 + x = x: the first non-zero value starts the sum
-            } else if (!finite || !isfinite(x[r])) {
-                stop[r] = true;
-            } else {
-                const double t = (double)x[r] * inv_u;  // exact: a power-of-two scale of an f32
-                const double fl = floor(t);
-                const double fr = t - fl;
-                if (fr == 0.5) {
-                    stop[r] = true;  // tie: rounding depends on the parity of s / u
-                } else {
-                    const double q = fl + (fr > 0.5 ? 1.0 : 0.0);
-                    a[r] = q >= 1099511627776.0 ? 1099511627776ull : (unsigned long long)q;  // >= 2^40 crosses anyway
-                }
-            }
-            tsum += a[r];
+            const uint64_t k = (uint64_t)base + (uint32_t)r * SUM_THREADS + tid;
+            nxt[r] = k < n ? ha[k] : 0.0f;
         }
-        // block-wide exclusive prefix of the thread totals (element k = tid * SUM_ITEMS + r, in order)
-        const unsigned long long incl = wave_inclusive_scan_u64(tsum);
-        if (tid == 0) first_stop = SUM_TILE;
-        if (lane == 63u) wtot[wave] = incl;
-        __syncthreads();
-        unsigned long long excl = incl - tsum;
-        for (uint32_t w = 0; w < wave; ++w) excl += wtot[w];
-        unsigned long long run = excl;
-        uint32_t my_stop = SUM_TILE;
+    };
+    auto store_tile = [&]() {
 #pragma unroll
         for (int r = 0; r < SUM_ITEMS; ++r) {
-            run += a[r];
-            const bool cross = !zero && m0 + run >= (1ull << 24);  // the sum reaches the next binade
-            if ((stop[r] || cross) && my_stop == SUM_TILE) my_stop = tid * SUM_ITEMS + (uint32_t)r;
+            const uint32_t e = (uint32_t)r * SUM_THREADS + tid;
+            tile[e + e / SUM_ITEMS] = nxt[r];
         }
-        if (my_stop != SUM_TILE) atomicMin(&first_stop, my_stop);
-        __syncthreads();
-        const uint32_t idx = first_stop;
+    };
+    float s = misc->sum;  // the running sum, identical in every thread
+    uint32_t pos = misc->sum_pos;
+    if (pos < n) {
+        load_tile(pos);
+        store_tile();
+    }
+    __syncthreads();
+    while (pos < limit && !isnan(s)) {
         const uint32_t tile_len = min(SUM_TILE, n - pos);
-        if (idx == SUM_TILE) {
-            if (tid == SUM_THREADS - 1) {  // owns the tile's total
-                s_next = zero ? 0.0f : ldexpf((float)(m0 + run), (int)be - 150);
-                pos_next = pos + tile_len;
-            }
-        } else if (idx / SUM_ITEMS == tid) {
-            const int r0 = (int)(idx % SUM_ITEMS);
-            unsigned long long before = excl;
-            float xi = x[0];
-#pragma unroll
-            for (int r = 0; r < SUM_ITEMS; ++r) {
-                if (r < r0) before += a[r];
-                if (r == r0) xi = x[r];
-            }
-            const float sb = zero ? 0.0f : ldexpf((float)(m0 + before), (int)be - 150);  // exact s before element idx
-            s_next = sb + xi;                                                            // the sequential step
-            pos_next = pos + idx + 1u;
+        if (tile_len < n - pos) load_tile(pos + SUM_TILE);  // the next tile when this one has no stop
+        const SumMode md = sum_mode(s);
+        const SumStep o =
+            sum_step<SUM_ITEMS>(md, 0ull, tile_len, [&](uint32_t e) { return tile[e + e / SUM_ITEMS]; }, sh);
+        const uint32_t prev = pos;
+        if (o.idx == SUM_NONE) {
+            s = sum_value(md, o.total);
+            pos += tile_len;
+        } else {
+            s = o.s_after;
+            pos += o.idx + 1u;
         }
-        __syncthreads();
-        s = s_next;
-        pos = pos_next;
+        if (pos < n) {
+            if (pos != prev + SUM_TILE) load_tile(pos);  // a stop moved the window: load it afresh
+            store_tile();
+        }
         __syncthreads();
     }
     if (tid == 0) {
-        const float avg = s / (float)n;
-        const float largest = f32_from_order_key(misc->largest_key);
-        misc->avg = avg;
-        misc->thresh_low = avg * 3.0f;                                       // splits.rs:27
-        misc->thresh_high = fmaxf(avg * 4.0f, avg * 0.9f + largest * 0.1f);  // splits.rs:28
+        misc->sum = s;
+        misc->sum_pos = pos;
+        if (pos >= n || isnan(s)) split_finish(misc, s, n);
+    }
+}
+
+// Per-block increments and whether the block holds a stop, for elements [pos, n) from the running sum s.
+__global__ void __launch_bounds__(T256) k_sum_blocks(const float* __restrict__ ha, uint32_t n, uint32_t pos, float s,
+                                                     unsigned long long* __restrict__ bsum,
+                                                     uint32_t* __restrict__ bstop) {
+    __shared__ unsigned long long ws[T256 / 64];
+    __shared__ uint32_t wf[T256 / 64];
+    const uint32_t tid = threadIdx.x;
+    const SumMode md = sum_mode(s);
+    const uint32_t base = pos + blockIdx.x * PASS_BLOCK;
+    uint32_t tsum = 0u, first = SUM_NONE;
+#pragma unroll
+    for (int r = 0; r < PASS_ITEMS; ++r) {
+        const uint32_t e = (uint32_t)r * T256 + tid;  // lane-contiguous
+        if ((uint64_t)base + e < n) {
+            uint32_t q;
+            if (sum_rule(md, ha[base + e], q)) first = min(first, e);
+            tsum += q;
+        }
+    }
+    unsigned long long v = tsum;
+    for (int off = 32; off >= 1; off >>= 1) {
+        v += __shfl_xor(v, off);
+        first = min(first, (uint32_t)__shfl_xor((int)first, off));
+    }
+    if ((tid & 63u) == 0u) {
+        ws[tid >> 6] = v;
+        wf[tid >> 6] = first;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long t = 0ull;
+        uint32_t f = SUM_NONE;
+        for (int w = 0; w < T256 / 64; ++w) {
+            t += ws[w];
+            f = min(f, wf[w]);
+        }
+        bsum[blockIdx.x] = t;
+        bstop[blockIdx.x] = f;
+    }
+}
+
+// One pass from (pos, s) over the k_sum_blocks results of its nb blocks: block totals carry the sum over blocks
+// without a stop; the block with the next stop is scanned element by element; a tie keeps the binade, so the pass
+// goes on after it; any other stop ends the pass with the new state.
+__global__ void __launch_bounds__(SUM_THREADS) k_sum_pass(const float* __restrict__ ha, uint32_t n, uint32_t pos,
+                                                         float s, uint32_t nb,
+                                                         const unsigned long long* __restrict__ bsum,
+                                                         const uint32_t* __restrict__ bstop, SplitMisc* misc) {
+    __shared__ SumShared sh;
+    __shared__ uint32_t s_block;
+    __shared__ unsigned long long s_pre;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const SumMode md = sum_mode(s);
+    unsigned long long acc = 0ull;  // ulps above md.m0 at element `at`
+    uint32_t b = 0, at = pos;       // block b holds element `at`
+    float s_end = s;
+    uint32_t pos_end = n;
+    bool scan_block = true;  // block b needs an element scan from `at`
+    for (;;) {
+        if (scan_block) {
+            const uint32_t bend = (uint32_t)min((uint64_t)n, (uint64_t)pos + (uint64_t)(b + 1u) * PASS_BLOCK);
+            const uint32_t start = at;
+            const SumStep o = sum_step<PASS_TILE_ITEMS>(md, acc, bend - start,
+                                                        [&](uint32_t e) { return ha[start + e]; }, sh);
+            if (o.idx == SUM_NONE) {
+                acc = o.total;
+                at = bend;
+                ++b;
+            } else {
+                const float sa = o.s_after;
+                at = start + o.idx + 1u;
+                // a tie inside the binade: the block totals still hold; anything else ends the pass
+                const uint32_t eb = __float_as_uint(s) >> 23;  // sign and exponent of the pass's sum
+                const bool stay = md.scan && !(sa < o.s_before) && (__float_as_uint(sa) >> 23) == eb;
+                if (!stay) {
+                    s_end = sa;
+                    pos_end = at;
+                    break;
+                }
+                acc = ((__float_as_uint(sa) & 0x7fffffu) | (eb != 0u ? 0x800000u : 0u)) - md.m0;  // same binade
+                if (at == bend) ++b;
+            }
+            if (at >= n) {
+                s_end = sum_value(md, acc);
+                pos_end = n;
+                break;
+            }
+            scan_block = at != pos + b * PASS_BLOCK;  // inside a block after a tie: scan the rest of it
+            if (scan_block) continue;
+        }
+        // block b starts at `at`: find the first block from b with a flagged stop or a crossing
+        const uint32_t rem = nb - b;
+        const uint32_t c = (rem + SUM_THREADS - 1u) / SUM_THREADS;
+        const uint32_t k0 = b + min(rem, tid * c), k1 = b + min(rem, (tid + 1u) * c);
+        unsigned long long tsum = 0ull;
+        for (uint32_t k = k0; k < k1; ++k) tsum += bsum[k];
+        const unsigned long long incl = wave_inclusive_scan_u64(tsum);
+        if (tid == 0) s_block = SUM_NONE;
+        if (lane == 63u) sh.wtot[wave] = incl;
+        __syncthreads();
+        unsigned long long run = acc + incl - tsum;
+        for (uint32_t w = 0; w < wave; ++w) run += sh.wtot[w];
+        uint32_t mine = SUM_NONE;
+        for (uint32_t k = k0; k < k1; ++k) {
+            if (bstop[k] != SUM_NONE || sum_crosses(md, run + bsum[k])) {
+                mine = k;
+                break;
+            }
+            run += bsum[k];
+        }
+        if (mine != SUM_NONE) atomicMin(&s_block, mine);
+        __syncthreads();
+        const uint32_t found = s_block;
+        if (found == SUM_NONE) {
+            if (tid == SUM_THREADS - 1) s_pre = run;  // every remaining block: the final total
+        } else if (mine == found) {
+            s_pre = run;  // the sum before block `found`
+        }
+        __syncthreads();
+        const unsigned long long pre = s_pre;
+        __syncthreads();
+        if (found == SUM_NONE) {
+            s_end = sum_value(md, pre);
+            pos_end = n;
+            break;
+        }
+        acc = pre;
+        b = found;
+        at = pos + b * PASS_BLOCK;
+        scan_block = true;
+    }
+    if (tid == 0) {
+        misc->sum = s_end;
+        misc->sum_pos = pos_end;
+        if (pos_end >= n || isnan(s_end)) split_finish(misc, s_end, n);
     }
 }
 
@@ -1334,11 +1580,40 @@ obvhs_status presplit_leaves(obvhs_context* ctx, const obvhs_triangle* d_tris, u
     uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
     const float4* tris = reinterpret_cast<const float4*>(d_tris);
     const uint32_t nblk = (n + T256 - 1) / T256;
-    OBVHS_TRY(Q.sums.ensure((size_t)(nblk + 1) * 4 + 64), "alloc block sums");
+    const uint32_t max_nb = (n + PASS_BLOCK - 1) / PASS_BLOCK;  // the sum's passes use the block sums first
+    OBVHS_TRY(Q.sums.ensure(std::max<size_t>((size_t)(nblk + 1) * 4, (size_t)max_nb * 12) + 64), "alloc block sums");
     k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(misc), 0u, sizeof(SplitMisc) / 4);
     k_split_prepare<<<nblk, T256, 0, st>>>(tris, n, Q.split_aabbs.as<float4>(), Q.split_indices.as<uint32_t>(),
                                            Q.ha.as<float>(), misc);
-    k_split_avg<<<1, SUM_THREADS, 0, st>>>(Q.ha.as<float>(), n, misc);
+    {  // the exact running sum: one workgroup over its first SUM_SERIAL_SPAN elements, then passes over all CUs
+        k_sum_serial<<<1, SUM_THREADS, 0, st>>>(Q.ha.as<float>(), n, std::min(n, SUM_SERIAL_SPAN), misc);
+        OBVHS_TRY(hipGetLastError(), "sum launch");
+        unsigned long long* bsum = Q.sums.as<unsigned long long>();
+        uint32_t* bstop = reinterpret_cast<uint32_t*>(bsum + max_nb);
+        uint32_t start = 0;
+        bool last_pass = false;
+        while (n > SUM_SERIAL_SPAN) {
+            OBVHS_TRY(hipMemcpyAsync(h, &misc->sum, 8, hipMemcpyDeviceToHost, st), "read sum");
+            OBVHS_TRY(hipStreamSynchronize(st), "sum sync");
+            float sum;
+            uint32_t pos;
+            memcpy(&sum, h, 4);
+            memcpy(&pos, h + 1, 4);
+            if (pos >= n || std::isnan(sum)) break;
+            const bool dense = last_pass && pos - start < SUM_SERIAL_SPAN / 4;  // the last pass ended soon
+            start = pos;
+            if (dense) {
+                k_sum_serial<<<1, SUM_THREADS, 0, st>>>(
+                    Q.ha.as<float>(), n, (uint32_t)std::min<uint64_t>(n, (uint64_t)pos + SUM_SERIAL_SPAN), misc);
+            } else {
+                const uint32_t nb = (n - pos + PASS_BLOCK - 1) / PASS_BLOCK;
+                k_sum_blocks<<<nb, T256, 0, st>>>(Q.ha.as<float>(), n, pos, sum, bsum, bstop);
+                k_sum_pass<<<1, SUM_THREADS, 0, st>>>(Q.ha.as<float>(), n, pos, sum, nb, bsum, bstop, misc);
+            }
+            last_pass = !dense;
+            OBVHS_TRY(hipGetLastError(), "sum pass launch");
+        }
+    }
     // candidate lists hold distinct leaf indices (at most cap); the extended list at most twice that
     OBVHS_TRY(Q.cand_a.ensure(cap * 4), "alloc candidates");
     OBVHS_TRY(Q.cand_b.ensure(cap * 8), "alloc extended candidates");

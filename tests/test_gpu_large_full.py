@@ -27,6 +27,14 @@ def test_large_scene_full_size_properties(ctx):
     cap = int(len(tris) * 1.3)
     seq = [1.0, 1.0 / 3.0]
     d_tris = ctx.upload(tris)
+    # the pre-split average is the crate's one-at-a-time f32 sum of the leaf half areas (cwbvh/builder.rs:34-41)
+    v = [tris[k][:, :3] for k in ("v0", "v1", "v2")]
+    ext = np.maximum(np.maximum(v[0], v[1]), v[2]) - np.minimum(np.minimum(v[0], v[1]), v[2])
+    ha = (ext[:, 0] * ext[:, 1] + ext[:, 1] * ext[:, 2]) + ext[:, 2] * ext[:, 0]
+    seq_avg = np.add.accumulate(ha, dtype=np.float32)[-1] / np.float32(len(ha))
+    _, _, avg, _ = ob.split_aabbs_preset(ctx, d_tris, cap, with_stats=True)
+    assert np.float32(avg).tobytes() == np.float32(seq_avg).tobytes(), (avg, seq_avg)
+    del v, ext, ha
     a = ob.build_cwbvh_from_tris(ctx, d_tris, quality, split_max_total=cap, ratio_sequence=seq)
     na, pa = a.nodes, a.primitive_indices
     assert len(pa) > len(tris)  # the splits added leaves

@@ -193,6 +193,14 @@ def _sum_cases():
         "subnormal": np.concatenate([np.full(5000, np.float32(1e-45)), np.full(3, np.float32(1e-38))]),
         "powers_of_two": 2.0 ** rng.integers(-40, 20, 50000),
         "one_value": [0.25],
+        # the sum reaches 2.0 at element 16384, the last of the 16K tile that starts after the first element
+        "stop_on_tile_end": np.concatenate([[1.0], np.full(49252, 2.0 ** -14)]),
+        "two_full_tiles": rng.uniform(0, 1, 32768),
+        # past the first 2^18 elements the sum runs in passes over all CUs; from s = 300000 (ulp 2^-5), 2^-6 is a tie
+        "passes_with_ties": np.concatenate([np.ones(300000), np.where(rng.random(400000) < 1e-4, 2.0 ** -6,
+                                                                      rng.uniform(0, 0.1, 400000))]),
+        "passes_cross_binades": np.concatenate([np.ones(300000), rng.uniform(0, 4, 600000)]),
+        "late_overflow": np.concatenate([np.ones(300000), [3e38, 3e38], np.ones(100000)]),
     }
 
 
