@@ -499,25 +499,35 @@ __device__ __forceinline__ void store_box(float4* __restrict__ a, uint32_t i, co
     a[2 * (size_t)i + 1] = make_float4(b.hx, b.hy, b.hz, 0.0f);
 }
 
-// Leaf boxes (triangle.rs:28-30), indices 0..n, half areas, largest half area.
+// Leaf boxes (triangle.rs:28-30), indices 0..n, half areas, largest half area.  Grid-stride over at most
+// PREPARE_MAX_BLOCKS workgroups, so the largest half area takes one global atomic per workgroup (one per wave made
+// the launch wait on about 270K atomics to one address at 17M triangles).
+constexpr uint32_t PREPARE_MAX_BLOCKS = 2048;
 __global__ void __launch_bounds__(T256) k_split_prepare(const float4* __restrict__ tris, uint32_t n,
                                                         float4* __restrict__ aabbs, uint32_t* __restrict__ indices,
                                                         float* __restrict__ ha, SplitMisc* misc) {
-    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
-    float h = 0.0f;
-    if (i < n) {
+    __shared__ uint32_t wmax[T256 / 64];
+    uint32_t k = 0u;
+    for (uint32_t i = blockIdx.x * T256 + threadIdx.x; i < n; i += gridDim.x * T256) {
         const float4 v0 = tris[3 * (size_t)i], v1 = tris[3 * (size_t)i + 1], v2 = tris[3 * (size_t)i + 2];
         Box b{v0.x, v0.y, v0.z, v0.x, v0.y, v0.z};
         box_extend(b, v1.x, v1.y, v1.z);
         box_extend(b, v2.x, v2.y, v2.z);
         store_box(aabbs, i, b);
         indices[i] = i;
-        h = half_area(b);
+        const float h = half_area(b);
         ha[i] = h;
+        // f32::max keeps the larger value; half areas of triangle boxes are never negative, so the order key's max
+        // is it
+        k = max(k, f32_order_key(h));
     }
-    // f32::max keeps the larger value; half areas of triangle boxes are never negative, so the order key's max is it
-    const uint32_t k = wave_max_u32(i < n ? f32_order_key(h) : 0u);
-    if (__lane_id() == 0) atomicMax(&misc->largest_key, k);
+    k = wave_max_u32(k);
+    if (__lane_id() == 0) wmax[threadIdx.x >> 6] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < T256 / 64; ++w) k = max(k, wmax[w]);
+        atomicMax(&misc->largest_key, k);
+    }
 }
 
 // The crate's running sum `avg_area += half_area` in triangle order (cwbvh/builder.rs:34-41), reproduced exactly
@@ -1583,7 +1593,7 @@ obvhs_status presplit_leaves(obvhs_context* ctx, const obvhs_triangle* d_tris, u
     const uint32_t max_nb = (n + PASS_BLOCK - 1) / PASS_BLOCK;  // the sum's passes use the block sums first
     OBVHS_TRY(Q.sums.ensure(std::max<size_t>((size_t)(nblk + 1) * 4, (size_t)max_nb * 12) + 64), "alloc block sums");
     k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(misc), 0u, sizeof(SplitMisc) / 4);
-    k_split_prepare<<<nblk, T256, 0, st>>>(tris, n, Q.split_aabbs.as<float4>(), Q.split_indices.as<uint32_t>(),
+    k_split_prepare<<<std::min(nblk, PREPARE_MAX_BLOCKS), T256, 0, st>>>(tris, n, Q.split_aabbs.as<float4>(), Q.split_indices.as<uint32_t>(),
                                            Q.ha.as<float>(), misc);
     {  // the exact running sum: one workgroup over its first SUM_SERIAL_SPAN elements, then passes over all CUs
         k_sum_serial<<<1, SUM_THREADS, 0, st>>>(Q.ha.as<float>(), n, std::min(n, SUM_SERIAL_SPAN), misc);
