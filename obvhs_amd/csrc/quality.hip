@@ -275,13 +275,17 @@ __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restr
             for (;;) {
                 sp = 0;
                 lo = 0;
-                push(make_uint2(__float_as_uint(area_diff), sib));
-                while (sp > 0) {
+                // The second child of an expanded node is the next entry popped, so it is kept in registers
+                // (`held`) instead of pushed and popped: the same visit order without an LDS round trip.
+                uint2 top = make_uint2(__float_as_uint(area_diff), sib);
+                bool held = true;
+                while (held || sp > 0) {
                     if (++steps > step_budget) {
                         overflow = true;
                         break;
                     }
-                    const uint2 top = pop();
+                    if (!held) top = pop();
+                    held = false;
                     const float top_area = __uint_as_float(top.x);
                     if (top_area - node_area <= best_area) continue;
                     const Node32 dst = load_node(&nodes[top.y]);
@@ -292,14 +296,16 @@ __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restr
                         best_to = top.y;
                         best_area = reinsert_area;
                     }
-                    if (dst.prim_count == 0u) {
+                    const float child_area = reinsert_area + half_area(db);
+                    // children whose bound already fails would fail it when popped (best_area only grows)
+                    if (dst.prim_count == 0u && !(child_area - node_area <= best_area)) {
                         if (sp + 2 > sd) {
                             overflow = true;
                             break;
                         }
-                        const float child_area = reinsert_area + half_area(db);
                         push(make_uint2(__float_as_uint(child_area), dst.first_index));
-                        push(make_uint2(__float_as_uint(child_area), dst.first_index + 1u));
+                        top = make_uint2(__float_as_uint(child_area), dst.first_index + 1u);
+                        held = true;
                     }
                 }
                 if (overflow) break;
