@@ -365,7 +365,7 @@ obvhs_status obvhs_build_cwbvh_from_tris_ex(obvhs_context* ctx, const obvhs_tria
     if (s == OBVHS_OK) cw->owned_bvh2 = b2;
     if (s == OBVHS_OK && reinsert)  // cwbvh/builder.rs:73
         s = bvh2_reinsertion(ctx, b2, params->reinsertion_batch_ratio, ext ? ext->ratio_sequence : nullptr,
-                             ext ? ext->ratio_sequence_len : 0u);
+                             ext ? ext->ratio_sequence_len : 0u, mp);
     if (s == OBVHS_OK) s = cwbvh_convert(ctx, b2, mp, true, cw);
     if (s != OBVHS_OK) {
         if (!reuse) obvhs_cwbvh_destroy(cw);

@@ -203,8 +203,10 @@ obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32
 obvhs_status cost_table_levels(obvhs_context* ctx, obvhs_bvh2* bvh, const DpView& dp, uint32_t* d_err);
 obvhs_status bvh2_refit_all(obvhs_context* ctx, obvhs_bvh2* bvh);
 obvhs_status bvh2_refit_from_triangles(obvhs_context* ctx, obvhs_bvh2* bvh, const obvhs_triangle* d_tris);
+// dp_max_prims != 0: also evaluates the conversion's cost table for that leaf size over the final tree (fused with
+// the last refit), so the conversion that follows skips its own pass over the levels.
 obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_size_ratio, const float* seq,
-                              uint32_t seq_len);
+                              uint32_t seq_len, uint32_t dp_max_prims = 0u);
 obvhs_status bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, const uint32_t* d_leaves, uint32_t count,
                                      uint8_t* d_flags);
 obvhs_status bvh2_partial_rebuild(obvhs_context* ctx, obvhs_bvh2* bvh, const uint8_t* d_flags,

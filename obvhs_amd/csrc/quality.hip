@@ -150,6 +150,28 @@ __global__ void __launch_bounds__(T256) k_refit_level(Node32* __restrict__ nodes
     store_node(&nodes[x], make_node(u, 0u, nd.first_index));
 }
 
+// The last refit of the reinsertion passes fused with the conversion's cost table (k_refit_level, then k_dp_level,
+// in one pass): the same union of the two children, in the same operand order, and the same table evaluation.
+template <bool REFIT>
+__global__ void __launch_bounds__(T256) k_refit_dp_level(Node32* __restrict__ nodes, DpView dp,
+                                                         const uint32_t* __restrict__ list, uint32_t start,
+                                                         uint32_t count, uint32_t* __restrict__ err) {
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t x = list[start + i];
+    const Node32 nx = load_node(&nodes[x]);
+    if (nx.prim_count != 0u) return;
+    const Node32 l = load_node(&nodes[nx.first_index]);
+    const Node32 r = load_node(&nodes[nx.first_index + 1]);
+    Box b = node_box(nx);
+    if (REFIT) {
+        b = box_union(node_box(l), node_box(r));
+        store_node(&nodes[x], make_node(b, 0u, nx.first_index));
+    }
+    const uint32_t e = dp_evaluate(dp, b, nx.first_index, l, r);
+    if (e) atomicOr(err, e);
+}
+
 __global__ void __launch_bounds__(T256) k_dp_level(const Node32* __restrict__ nodes, DpView dp,
                                                    const uint32_t* __restrict__ list, uint32_t start, uint32_t count,
                                                    uint32_t* __restrict__ err) {
@@ -1462,7 +1484,7 @@ obvhs_status bvh2_primitives_to_nodes(obvhs_context* ctx, const obvhs_bvh2* bvh,
 
 // ReinsertionOptimizer::run (reinsertion.rs:40-61, 101-119).
 obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_size_ratio, const float* seq,
-                              uint32_t seq_len) {
+                              uint32_t seq_len, uint32_t dp_max_prims) {
     const uint32_t N = bvh->node_count;
     if (N == 0 || !(batch_size_ratio > 0.0f)) return OBVHS_OK;
     hipStream_t st = ctx->stream;
@@ -1499,7 +1521,16 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
     uint32_t levels = 0;
     obvhs_status s = tree_levels(ctx, bvh, meta, levels, nullptr);
     if (s != OBVHS_OK) return s;
+    // After a step's moves the level lists are rebuilt at once (the next step's stack bound is the depth) and the
+    // refit waits for the next step, which reads the boxes; the last one runs after the loop, fused with the
+    // conversion's cost table when one is asked for.
+    bool refit_pending = false;
     for (uint32_t si = 0; si < seq_len; ++si) {
+        if (refit_pending) {
+            s = refit_levels(ctx, bvh, meta, levels);
+            if (s != OBVHS_OK) return s;
+            refit_pending = false;
+        }
         // reinsertion.rs:112-117 in f32, `as usize` saturating
         const float prod = ((float)N * batch_size_ratio) * seq[si];
         uint64_t batch = prod >= 1.8446744e19f ? UINT64_MAX : (prod > 0.0f ? (uint64_t)prod : 0ull);
@@ -1569,11 +1600,32 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
         OBVHS_TRY(hipGetLastError(), "apply launch");
         s = tree_levels(ctx, bvh, meta, levels, nullptr);
         if (s != OBVHS_OK) return s;
-        s = refit_levels(ctx, bvh, meta, levels);
-        if (s != OBVHS_OK) return s;
+        refit_pending = true;
     }
     bvh->pass_ordered = false;  // children are no longer stored in PLOC pass order (reinsertion.rs:102)
     bvh->dp_max_prims = 0;
+    if (dp_max_prims != 0u && bvh->prim_count > 1) {
+        OBVHS_TRY(bvh_dp_alloc(bvh), "alloc cost table");
+        const DpView dp = bvh_dp_view(bvh, dp_max_prims);
+        k_fill_u32<<<1, 64, 0, st>>>(&misc->err, 0u, 1);
+        for (uint32_t l = levels; l-- > 0;) {
+            const uint32_t b = meta[2 * l], c = meta[2 * l + 1];
+            if (refit_pending)
+                k_refit_dp_level<true><<<(c + T256 - 1) / T256, T256, 0, st>>>(nodes, dp, Q.level_list.as<uint32_t>(),
+                                                                              b, c, &misc->err);
+            else
+                k_refit_dp_level<false><<<(c + T256 - 1) / T256, T256, 0, st>>>(nodes, dp, Q.level_list.as<uint32_t>(),
+                                                                               b, c, &misc->err);
+        }
+        OBVHS_TRY(hipGetLastError(), "refit and cost table launch");
+        OBVHS_TRY(hipMemcpyAsync(h, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cost table error");
+        OBVHS_TRY(hipStreamSynchronize(st), "cost table sync");
+        if (h[0]) return device_error_to_status(h[0]);
+        bvh->dp_max_prims = dp_max_prims;
+    } else if (refit_pending) {
+        s = refit_levels(ctx, bvh, meta, levels);
+        if (s != OBVHS_OK) return s;
+    }
     return OBVHS_OK;
 }
 
