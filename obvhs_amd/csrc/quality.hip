@@ -37,6 +37,7 @@ constexpr int T256 = 256;
 constexpr uint32_t LEVELS_PER_SYNC = 16;
 constexpr uint32_t MIS_ROUNDS_PER_SYNC = 8;
 constexpr uint32_t FIND_THREADS = 1u << 19;  // concurrent find_reinsertion lanes (each owns a stack column)
+constexpr uint32_t FIND_REFILL = 16;  // idle lanes of a wave that take new candidates together
 constexpr uint32_t FIND_LDS = 8;  // top stack entries in LDS: 16 KB per block, 32 waves per CU (16 entries: 16% slower)
 
 __device__ __forceinline__ uint32_t sibling_of(uint32_t id) { return (id & 1u) ? id + 1u : id - 1u; }
@@ -242,9 +243,9 @@ struct ReinsMisc {
     uint32_t undecided[MIS_ROUNDS_PER_SYNC];
 };
 
-// reinsertion.rs:247-340, one lane per candidate, stack column `tid` of an [sd][nthreads] array.  The grid first
-// takes candidates [0, nthreads) (wave w the span [64w, 64w + 64)); a wave that finishes its span fetches the next
-// from a counter, so waves whose walks are short take more candidates (one atomic per 64 walks).
+// reinsertion.rs:247-340, one lane per candidate walk, stack column `tid` of an [sd][nthreads] array; a walk's
+// steps are the crate's in the crate's order, one per loop iteration (start, search step, or climb to the next
+// pivot), so lanes of a wave refill with new candidates independently.
 __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restrict__ nodes, uint32_t n_nodes,
                                                            const uint32_t* __restrict__ parents,
                                                            const uint32_t* __restrict__ cand, uint32_t count,
@@ -275,41 +276,93 @@ __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restr
         }
         return ring[sp % FIND_LDS][threadIdx.x];
     };
-    uint32_t span = tid & ~63u;
+    // Each lane walks one candidate at a time: the grid first takes candidates [0, nthreads) (lane tid the
+    // candidate tid); a lane that finishes waits until FIND_REFILL lanes of its wave are idle (or none is walking),
+    // and the wave then takes that many further candidates from the counter with one atomic.  Lanes walk
+    // independently, so a long walk no longer holds the other 63 lanes idle until it ends.
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t step_budget = 2u * n_nodes + 64u;  // walks of a valid tree end long before these budgets
+    uint32_t t = tid;
+    bool done = t >= count, live = !done, fresh = live;
+    uint32_t node_id = 0u, parent_id = 0u, sib = 0u, pivot_id = 0u, best_to = 0u, climb = 0u, steps = 0u;
+    float node_area = 0.0f, area_diff = 0.0f, best_area = 0.0f;
+    Box aabb{}, pivot_bbox{};
+    bool overflow = false, held = false;
+    uint2 top = make_uint2(0u, 0u);
     for (;;) {
-        if (span >= count) break;  // wave-uniform
-        const uint32_t t = span + (threadIdx.x & 63u);
-        if (t < count) {
-            const uint32_t node_id = cand[t] + 1u;  // candidates were numbered from node 1
-            uint32_t best_to = 0u;
-            float best_area = 0.0f;
-            const Box aabb = node_box(load_node(&nodes[node_id]));
-            const float node_area = half_area(aabb);
-            const uint32_t parent_id = parents[node_id];
-            float area_diff = half_area(node_box(load_node(&nodes[parent_id])));
-            uint32_t sib = sibling_of(node_id);
-            Box pivot_bbox = node_box(load_node(&nodes[sib]));
-            uint32_t pivot_id = parent_id;
-            bool overflow = false;
-            // walks of a valid tree end long before these budgets; they only bound a malformed one
-            uint32_t climb = 0, steps = 0;
-            const uint32_t step_budget = 2u * n_nodes + 64u;
-            for (;;) {
+        const uint64_t want = __ballot(!live && !done);
+        const uint64_t walking = __ballot(live);
+        if (want != 0ull && ((uint32_t)__popcll(want) >= FIND_REFILL || walking == 0ull)) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)want) - 1u;
+            uint32_t base = 0u;
+            if (lane == leader) base = nthreads + atomicAdd(&misc->next, (uint32_t)__popcll(want));
+            base = (uint32_t)__shfl((int)base, (int)leader);
+            if (!live && !done) {
+                t = base + (uint32_t)__popcll(want & lanemask_lt());
+                if (t < count) {
+                    live = true;
+                    fresh = true;
+                } else {
+                    done = true;
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) break;  // every lane of the wave is done (wave-uniform)
+        bool fin = false;
+        if (live) {
+            if (fresh) {  // the candidate's start (reinsertion.rs:250-262)
+                fresh = false;
+                node_id = cand[t] + 1u;  // candidates were numbered from node 1
+                best_to = 0u;
+                best_area = 0.0f;
+                aabb = node_box(load_node(&nodes[node_id]));
+                node_area = half_area(aabb);
+                parent_id = parents[node_id];
+                area_diff = half_area(node_box(load_node(&nodes[parent_id])));
+                sib = sibling_of(node_id);
+                pivot_bbox = node_box(load_node(&nodes[sib]));
+                pivot_id = parent_id;
+                overflow = false;
+                climb = 0u;
+                steps = 0u;
                 sp = 0;
                 lo = 0;
                 // The second child of an expanded node is the next entry popped, so it is kept in registers
                 // (`held`) instead of pushed and popped: the same visit order without an LDS round trip.
-                uint2 top = make_uint2(__float_as_uint(area_diff), sib);
-                bool held = true;
-                while (held || sp > 0) {
-                    if (++steps > step_budget) {
-                        overflow = true;
-                        break;
+                top = make_uint2(__float_as_uint(area_diff), sib);
+                held = true;
+            }
+            if (!held && sp == 0) {
+                // this level's search is over: climb to the next pivot (or end the walk)
+                fin = overflow;
+                if (!fin) {
+                    if (pivot_id != parent_id) {
+                        pivot_bbox = box_union(pivot_bbox, node_box(load_node(&nodes[sib])));
+                        area_diff += half_area(node_box(load_node(&nodes[pivot_id]))) - half_area(pivot_bbox);
                     }
-                    if (!held) top = pop();
-                    held = false;
-                    const float top_area = __uint_as_float(top.x);
-                    if (top_area - node_area <= best_area) continue;
+                    if (pivot_id == 0u) {
+                        fin = true;
+                    } else if (++climb > sd) {
+                        overflow = true;
+                        fin = true;
+                    } else {
+                        sib = sibling_of(pivot_id);
+                        pivot_id = parents[pivot_id];
+                        sp = 0;
+                        lo = 0;
+                        top = make_uint2(__float_as_uint(area_diff), sib);
+                        held = true;
+                    }
+                }
+            } else if (++steps > step_budget) {
+                overflow = true;  // ends the walk at the next iteration
+                held = false;
+                sp = 0;
+            } else {
+                if (!held) top = pop();
+                held = false;
+                const float top_area = __uint_as_float(top.x);
+                if (!(top_area - node_area <= best_area)) {
                     const Node32 dst = load_node(&nodes[top.y]);
                     const Box db = node_box(dst);
                     const float merged_area = half_area(box_union(db, aabb));
@@ -323,26 +376,19 @@ __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restr
                     if (dst.prim_count == 0u && !(child_area - node_area <= best_area)) {
                         if (sp + 2 > sd) {
                             overflow = true;
-                            break;
+                            held = false;
+                            sp = 0;
+                        } else {
+                            push(make_uint2(__float_as_uint(child_area), dst.first_index));
+                            top = make_uint2(__float_as_uint(child_area), dst.first_index + 1u);
+                            held = true;
                         }
-                        push(make_uint2(__float_as_uint(child_area), dst.first_index));
-                        top = make_uint2(__float_as_uint(child_area), dst.first_index + 1u);
-                        held = true;
                     }
                 }
-                if (overflow) break;
-                if (pivot_id != parent_id) {
-                    pivot_bbox = box_union(pivot_bbox, node_box(load_node(&nodes[sib])));
-                    area_diff += half_area(node_box(load_node(&nodes[pivot_id]))) - half_area(pivot_bbox);
-                }
-                if (pivot_id == 0u) break;
-                if (++climb > sd) {
-                    overflow = true;
-                    break;
-                }
-                sib = sibling_of(pivot_id);
-                pivot_id = parents[pivot_id];
             }
+        }
+        bool pos = false;
+        if (fin) {
             if (overflow) atomicOr(&misc->err, DERR_CAPACITY);
             uint32_t from = node_id;
             if (best_to == sibling_of(node_id) || best_to == parent_id) {
@@ -352,16 +398,13 @@ __global__ void __launch_bounds__(T256) k_find_reinsertion(const Node32* __restr
             }
             r_from[t] = from;
             r_to[t] = best_to;
-            const bool pos = best_area > 0.0f;
+            pos = best_area > 0.0f;
             keys[t] = pos ? (uint64_t)(~f32_order_key(best_area)) : 0xffffffffull;
-            const uint64_t m = __ballot(pos);
-            if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)__ballot(true)) - 1) && m)
-                atomicAdd(&misc->positive, (uint32_t)__popcll(m));
-        }  // t < count
-        uint32_t nx = 0u;
-        if ((threadIdx.x & 63u) == 0u) nx = nthreads + atomicAdd(&misc->next, 64u);
-        span = (uint32_t)__shfl((int)nx, 0);
-    }  // spans of this wave
+            live = false;
+        }
+        const uint64_t m = __ballot(pos);
+        if (m != 0ull && lane == (uint32_t)__ffsll((unsigned long long)m) - 1u) atomicAdd(&misc->positive, (uint32_t)__popcll(m));
+    }
 }
 
 constexpr uint32_t ST_UNDECIDED = 0u, ST_ACCEPTED_NEW = 1u, ST_ACCEPTED = 2u, ST_REJECTED = 3u;
