@@ -121,7 +121,8 @@ def autotune_batch(ob, ctx, tris, offsets, rays, ray_mesh):
     """The tuning run before the batched leg (tune.hip): the radix sort tile for the batch's key count; the PLOC
     search distance of each size class from the class's median-size mesh (the smallest distance whose tree cost is
     within 1% of distance 14's, the preset; Auto builds then resolve it per mesh); the traversal LDS depth on the
-    mesh nearest the mean size, whose class the batched traversal uses, with its rays repeated 16 times."""
+    mesh nearest the mean size with its rays repeated 16 times (bench_batch then re-times the depth on the batched
+    launch itself, whose class is the mean mesh size's)."""
     t0 = time.perf_counter()
     counts = np.diff(offsets).astype(np.int64)
     sort_items = ob.tune_sort(ctx, int(counts.sum()))
@@ -182,6 +183,29 @@ def bench_batch(args, ob, ctx):
         ob.native.check(lib.obvhs_batch_ray_traverse(ctx.handle, batch.handle, d_bvh_tris.ptr, d_rays.ptr,
                                                      d_mesh.ptr, n_rays, hits.ptr), ctx.handle)
 
+    if tuned is not None and not args.lds_depth:
+        # The batched launch takes the LDS depth of its mean mesh size's class: time the launch itself at each
+        # candidate depth (one warm-up and three timed launches each) and keep the fastest for the timed steps.
+        cls = ob.size_class(n_tris // m)
+        by_depth = {}
+        for depth in (4, 8, 16):
+            t = ctx.tune_table()
+            t["trav_lds_depth"][cls] = depth
+            ctx.set_tune_table(t)
+            trace_batch()
+            ms = []
+            for _ in range(3):
+                ctx.timer_start()
+                trace_batch()
+                ms.append(ctx.timer_stop())
+            by_depth[depth] = min(ms)
+        best = min(by_depth, key=by_depth.get)
+        t = ctx.tune_table()
+        t["trav_lds_depth"][cls] = best
+        ctx.set_tune_table(t)
+        tuned["traversal_lds_depth"] = best
+        tuned["traversal_size_class"] = cls
+        tuned["trace_ms_by_lds_depth"] = {str(k): round(v, 4) for k, v in by_depth.items()}
     for _ in range(args.warmup):
         trace_batch()
     ctx.synchronize()
