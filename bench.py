@@ -389,6 +389,12 @@ def bench_large(args, ob, ctx):
         ob.native.check(lib.obvhs_cwbvh_ray_traverse_miss(ctx.handle, cw.handle, d_bvh_tris.ptr, d_shadow.ptr, ns,
                                                           miss.ptr), ctx.handle)
 
+    tuned = None
+    if not args.no_tune and not args.lds_depth and args.trav_chunk < 0:
+        # this scene's rays are incoherent: the depth and launch form measured on its closest-hit rays
+        depth = ob.tune_traversal(ctx, cw, d_rays, d_bvh_tris)
+        tuned = {"traversal_lds_depth": depth, "traversal_chunked": ctx.tune_table()["trav_chunk"][ob.size_class(n)]}
+
     def timed(fn):
         for _ in range(args.warmup):
             fn()
@@ -408,7 +414,8 @@ def bench_large(args, ob, ctx):
             "build_mtris_per_s": round(n / build_ms / 1e3, 2), "build_ms_without_quality_passes": round(plain_ms, 3),
             "closest_mrays_per_s": round(nr / tc / 1e6, 2), "anyhit_mrays_per_s": round(ns / ta / 1e6, 2),
             "closest_hit_fraction": round(float(np.isfinite(h["t"]).mean()), 4),
-            "shadowed_fraction": round(float((m == 0).mean()), 4), "cwbvh_nodes": int(st.cwbvh_node_count)}
+            "shadowed_fraction": round(float((m == 0).mean()), 4), "cwbvh_nodes": int(st.cwbvh_node_count),
+            "tuning": tuned}
 
 
 def bench_serve(args, ob, ctx):
@@ -570,6 +577,9 @@ def main():
                     help="set the sort tile (4, 8, 16) for every size class instead of measuring it")
     ap.add_argument("--lds-depth", type=int, default=0,
                     help="set the traversal LDS depth (4, 8, 16) for every size class instead of measuring it")
+    ap.add_argument("--trav-chunk", type=int, default=-1, choices=(-1, 0, 1),
+                    help="set chunked single-BVH traversal launches (0 or 1) for every size class instead of "
+                         "measuring it (-1)")
     ap.add_argument("--pmc-probe", action="store_true", help="one copy kernel + one traversal per batch, then exit")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the HBM traffic file from two --pmc-probe counter runs, then exit")
@@ -606,8 +616,11 @@ def main():
     params = ob.BvhBuildParams(ploc_search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3)
 
     tuning = {}
-    if args.sort_items or args.lds_depth:  # fixed entries (profiling runs repeat a measured choice)
+    if args.sort_items or args.lds_depth or args.trav_chunk >= 0:  # fixed entries (profiling runs repeat a choice)
         t = ctx.tune_table()
+        if args.trav_chunk >= 0:
+            t["trav_chunk"] = [args.trav_chunk] * 4
+            tuning["traversal_chunked"] = args.trav_chunk
         if args.sort_items:
             t["sort_items"] = [args.sort_items] * 4
             tuning["sort_items"] = args.sort_items
@@ -646,9 +659,11 @@ def main():
     bhits = ob.DeviceArray(ctx, ob.RAY_HIT, max(1, len(bounce)))
     n_rays = len(prim_rays) + len(bounce)
     log(f"{len(prim_rays)} primary + {len(bounce)} bounce rays")
-    if not args.no_tune and not args.lds_depth:  # the traversal LDS stack depth on this workload's rays (the hits
-        tuning["traversal_lds_depth"] = ob.tune_traversal(ctx, cw, np.concatenate([prim_rays, bounce]),
-                                                          d_bvh_tris)  # are the same for every depth)
+    if not args.no_tune and not args.lds_depth and args.trav_chunk < 0:
+        # the traversal LDS stack depth and launch form on the primary rays, the launch they can change (the hits are
+        # the same for every choice; the bounce launch holds too few rays per lane for chunks)
+        tuning["traversal_lds_depth"] = ob.tune_traversal(ctx, cw, prim_rays, d_bvh_tris)
+        tuning["traversal_chunked"] = ctx.tune_table()["trav_chunk"][ob.size_class(n_tris)]
 
     if args.pmc_probe:
         # Counter runs (rocprofv3 --pmc): one device-to-device copy of known size (a blit kernel: the counters'

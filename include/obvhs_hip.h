@@ -117,19 +117,23 @@ obvhs_status obvhs_context_set_trace(obvhs_context* ctx, int enable);
  *                       that distance).  Default 14.
  *   sort_items[c]     : keys per thread of the radix sort's tiles (4, 8 or 16) for a sort of class c.  Default 8.
  *   trav_lds_depth[c] : traversal stack entries kept in LDS (4, 8 or 16) for a BVH of class c.  Default 8.
- * The last two never change a result, only the speed.  obvhs_tune_* fill entries by measurement. */
+ *   trav_chunk[c]     : 1: a single-BVH traversal launch of at least two rays per resident lane gives each wave
+ *                       interleaved rows of rays and refills a lane as soon as its ray ends (faster for incoherent
+ *                       rays, slower for coherent ones); 0: one lane per ray.  Default 0.
+ * The last three never change a result, only the speed.  obvhs_tune_* fill entries by measurement. */
 typedef struct obvhs_tune_table {
     uint32_t ploc_radius[4];
     uint32_t sort_items[4];
     uint32_t trav_lds_depth[4];
+    uint32_t trav_chunk[4];
 } obvhs_tune_table;
 uint32_t obvhs_size_class(uint64_t n);
 obvhs_status obvhs_tune_get(const obvhs_context* ctx, obvhs_tune_table* out);
 obvhs_status obvhs_tune_set(obvhs_context* ctx, const obvhs_tune_table* table); /* rejects values out of range */
 /* Times the radix sort of n seeded 63-bit keys with each tile size and stores the fastest in sort_items[class(n)]. */
 obvhs_status obvhs_tune_sort(obvhs_context* ctx, uint32_t n, uint32_t* out_items);
-/* Times closest-hit (any_hit = 0) or occlusion traversal of the given rays with each LDS depth and stores the
- * fastest in trav_lds_depth[class(prim count)]. */
+/* Times closest-hit (any_hit = 0) or occlusion traversal of the given rays with each LDS depth, one lane per ray
+ * and chunked, and stores the fastest in trav_lds_depth[class(prim count)] and trav_chunk[class(prim count)]. */
 obvhs_status obvhs_tune_traversal(obvhs_context* ctx, const obvhs_cwbvh* bvh, const obvhs_triangle* d_bvh_tris,
                                   const obvhs_ray* d_rays, uint32_t n, int any_hit, uint32_t* out_depth);
 /* Builds the triangles with every PlocSearchDistance (1, 2, 6, 14, 24, 32), evaluates each tree's CWBVH cost for

@@ -130,16 +130,17 @@ class Context:
 
     def tune_table(self):
         """The context's tuning table: per size class (see size_class) the PLOC distance of Auto builds, the radix
-        sort tile (keys per thread) and the traversal LDS stack depth."""
+        sort tile (keys per thread), the traversal LDS stack depth and whether large single-BVH traversal launches
+        run as wave-owned ray rows with lanes refilled (trav_chunk, 0 or 1)."""
         t = native.TuneTable()
         check(lib().obvhs_tune_get(self.handle, C.byref(t)), self.handle)
         return {"ploc_radius": list(t.ploc_radius), "sort_items": list(t.sort_items),
-                "trav_lds_depth": list(t.trav_lds_depth)}
+                "trav_lds_depth": list(t.trav_lds_depth), "trav_chunk": list(t.trav_chunk)}
 
     def set_tune_table(self, table):
         t = native.TuneTable()
-        for k in ("ploc_radius", "sort_items", "trav_lds_depth"):
-            getattr(t, k)[:] = [int(v) for v in table[k]]
+        for k in ("ploc_radius", "sort_items", "trav_lds_depth", "trav_chunk"):
+            getattr(t, k)[:] = [int(v) for v in table.get(k, [0] * 4)]
         check(lib().obvhs_tune_set(self.handle, C.byref(t)), self.handle)
 
     def upload(self, array):
@@ -738,7 +739,8 @@ def tune_sort(ctx, n):
 
 
 def tune_traversal(ctx, cwbvh, rays, bvh_tris, any_hit=False):
-    """Times traversal of the rays with each LDS stack depth; stores and returns the fastest for the BVH's class."""
+    """Times traversal of the rays with each LDS stack depth, one lane per ray and chunked; stores the fastest
+    for the BVH's class and returns its depth (the chunk choice is in ctx.tune_table()["trav_chunk"])."""
     dr, n = _as_device(ctx, rays, RAY)
     dt, _ = _as_device(ctx, bvh_tris, TRIANGLE)
     out = C.c_uint32()

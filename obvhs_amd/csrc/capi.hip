@@ -570,6 +570,7 @@ obvhs_status obvhs_cwbvh_ray_traverse(obvhs_context* ctx, const obvhs_cwbvh* bvh
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
+    ts.chunked = chunk_for(ctx, bvh->prim_count);
     CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0,
                             d_hits, nullptr, nullptr, nullptr, ts, ctx->stream),
             "traverse");
@@ -587,6 +588,7 @@ obvhs_status obvhs_cwbvh_ray_traverse_miss(obvhs_context* ctx, const obvhs_cwbvh
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
+    ts.chunked = chunk_for(ctx, bvh->prim_count);
     CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 1,
                             nullptr, d_miss, nullptr, nullptr, ts, ctx->stream),
             "traverse miss");
@@ -605,6 +607,7 @@ obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cw
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
+    ts.chunked = chunk_for(ctx, bvh->prim_count);
     CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, 0,
                             d_hits, nullptr, d_node_visits, d_tri_tests, ts, ctx->stream),
             "traverse counted");

@@ -81,7 +81,7 @@ struct obvhs_context {
     void* pinned = nullptr;  // small pinned host staging area
     bool keep_trace = false; // keep Morton codes and the sort permutation of each build (parity tests)
     uint32_t trav_lanes = 0;  // persistent traversal grid size (0 = not yet queried)
-    obvhs_tune_table tune{{14u, 14u, 14u, 14u}, {8u, 8u, 8u, 8u}, {8u, 8u, 8u, 8u}};
+    obvhs_tune_table tune{{14u, 14u, 14u, 14u}, {8u, 8u, 8u, 8u}, {8u, 8u, 8u, 8u}, {0u, 0u, 0u, 0u}};
     obvhs::DevBuf tune_scratch;  // working memory of the tuning runs
     std::string last_error;
 };
@@ -129,6 +129,7 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* values, bool values_identi
 inline uint32_t size_class(uint64_t n) { return n < 4096u ? 0u : n < 65536u ? 1u : n < 1048576u ? 2u : 3u; }
 inline int sort_items_for(const obvhs_context* ctx, uint64_t n) { return (int)ctx->tune.sort_items[size_class(n)]; }
 inline uint32_t lds_depth_for(const obvhs_context* ctx, uint64_t n) { return ctx->tune.trav_lds_depth[size_class(n)]; }
+inline bool chunk_for(const obvhs_context* ctx, uint64_t n) { return ctx->tune.trav_chunk[size_class(n)] != 0u; }
 inline uint32_t radius_for(const obvhs_context* ctx, uint64_t n) { return ctx->tune.ploc_radius[size_class(n)]; }
 
 // ploc.hip
@@ -163,7 +164,9 @@ hipError_t launch_reorder_triangles(const obvhs_triangle* tris, const uint32_t* 
 struct TravScratch {
     void* spill = nullptr;
     uint32_t lanes = 0;
+    uint32_t resident = 0;   // lanes the device keeps resident (compute units x 32 waves x 64)
     uint32_t lds_depth = 8;  // stack entries in LDS for the launch (4, 8 or 16); results do not depend on it
+    bool chunked = false;    // wave-owned ray rows with lanes refilled (k_traverse_chunk); results do not depend on it
 };
 // Spill columns (and the lane count) for a traversal launch of n rays.
 obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts, uint64_t n);

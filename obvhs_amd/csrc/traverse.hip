@@ -278,6 +278,136 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     }  // rays of this lane
 }
 
+// k_traverse for one BVH when a launch holds several rays per resident lane: each 64-lane workgroup (one wave)
+// owns per_wave / 64 rows of 64 rays, rows blockIdx.x, blockIdx.x + W, blockIdx.x + 2W, ... (W workgroups, so the
+// waves sweep the rays together, as a one-ray-per-lane grid does), and a lane whose ray ends takes the wave's next
+// ray at the next step (a wave-uniform count of the rays handed out, no atomics) instead of idling until the wave's
+// longest ray ends.  The steps of every ray are k_traverse's, so the hits are identical.
+template <int MODE, int LDS>
+__global__ void __launch_bounds__(TRAV_THREADS)
+    k_traverse_chunk(const uint4* __restrict__ nodes, uint32_t node_count, uint32_t prim_count,
+                     const float4* __restrict__ tris, const float4* __restrict__ rays, uint32_t n,
+                     obvhs_ray_hit* __restrict__ hits, uint8_t* __restrict__ miss_out, uint2* __restrict__ spill,
+                     uint32_t per_wave) {
+    __shared__ uint2 stack[LDS][TRAV_THREADS];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t lanes = gridDim.x * TRAV_THREADS;
+    uint2* my_spill = spill + blockIdx.x * TRAV_THREADS + lane;  // entry k >= LDS at my_spill[(k - LDS) * lanes]
+    uint32_t handed = 0u;  // rays of the chunk handed out so far (the same in every lane)
+    bool live = false;
+    uint32_t i = 0u;
+    RayData r{};
+    uint32_t oct = 0u, hit_id = INVALID32;
+    float tmax = 0.0f, hit_t = INFINITY;
+    bool is_miss = true;
+    int sp = 0;
+    uint2 cur = make_uint2(0u, 0u), prim = make_uint2(0u, 0u);
+    for (;;) {
+        const uint64_t want = __ballot(!live);
+        if (want != 0ull) {
+            if (!live) {
+                // the h-th ray handed out in this wave: row h / 64 of the wave's rows blockIdx.x, blockIdx.x + W, ...
+                const uint32_t h = handed + (uint32_t)__popcll(want & lanemask_lt());
+                const uint64_t k = ((uint64_t)(h / TRAV_THREADS) * gridDim.x + blockIdx.x) * TRAV_THREADS +
+                                   h % TRAV_THREADS;
+                if (h < per_wave && k < n) {
+                    i = (uint32_t)k;
+                    const float4 ra = rays[4 * (size_t)i], rb = rays[4 * (size_t)i + 1], rc = rays[4 * (size_t)i + 2],
+                                 rd = rays[4 * (size_t)i + 3];
+                    r = RayData{ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, rc.x, rc.y, rc.z, rd.x, rd.y};
+                    oct = octant_inv4(r.dx, r.dy, r.dz);
+                    tmax = r.tmax;
+                    hit_id = INVALID32;
+                    hit_t = INFINITY;
+                    is_miss = true;
+                    sp = 0;
+                    cur = node_count == 0u ? make_uint2(0u, 0u) : make_uint2(0u, 0x80000000u);
+                    prim = make_uint2(0u, 0u);
+                    live = true;
+                }
+            }
+            handed = min(handed + (uint32_t)__popcll(want), per_wave);
+        }
+        if (__ballot(live) == 0ull) break;  // the chunk is done (wave-uniform)
+        if (!live) continue;
+        // one step of k_traverse's loop
+        bool done = false;
+        const bool node_next = (cur.y & 0xff000000u) != 0u;
+        uint32_t child = 0u;
+        uint4 q0, q1, q2, q3, q4;
+        if (node_next) {
+            const uint32_t slot_index = (firstbithigh(cur.y) - 24u) ^ (oct & 0xffu);
+            child = cur.x + (uint32_t)__popc(cur.y & ~(0xffffffffu << slot_index));
+            if (child < node_count) {
+                const uint4* np = nodes + (size_t)child * 5;
+                q0 = np[0];
+                q1 = np[1];
+                q2 = np[2];
+                q3 = np[3];
+                q4 = np[4];
+            }
+        }
+        while (prim.y != 0u) {
+            const uint32_t local = firstbithigh(prim.y);
+            prim.y &= ~(1u << local);
+            const uint32_t pid = prim.x + local;
+            if (pid >= prim_count) continue;  // malformed input only: never index past the triangle buffer
+            const float4 v0 = tris[3 * (size_t)pid], v1 = tris[3 * (size_t)pid + 1], v2 = tris[3 * (size_t)pid + 2];
+            if (MODE == 0) {
+                const float t = tri_intersect(v0, v1, v2, r, r.tmin, tmax);
+                if (t < tmax) {
+                    hit_id = pid;
+                    hit_t = t;
+                    tmax = t;
+                }
+            } else {
+                const float t = tri_intersect(v0, v1, v2, r, r.tmin, r.tmax);
+                if (t < r.tmax) {
+                    is_miss = false;
+                    done = true;
+                    break;
+                }
+            }
+        }
+        if (!done) {
+            prim = make_uint2(0u, 0u);
+            if (node_next) {
+                cur.y &= ~(1u << firstbithigh(cur.y));
+                if ((cur.y & 0xff000000u) != 0u) {
+                    if (sp < LDS) stack[sp][lane] = cur;
+                    else my_spill[(size_t)(sp - LDS) * lanes] = cur;
+                    sp = min(sp + 1, STACK_SIZE - 1);
+                }
+                if (child >= node_count) {
+                    done = true;  // malformed input only: never index past the node buffer
+                } else {
+                    const uint32_t hm = intersect_node(q0, q1, q2, q3, q4, r, tmax, oct);
+                    const uint32_t imask = (q0.w >> 24) & 0xffu;
+                    cur = make_uint2(q1.x, (hm & 0xff000000u) | imask);
+                    prim = make_uint2(q1.y, hm & 0x00ffffffu);
+                }
+            } else {
+                cur = make_uint2(0u, 0u);
+            }
+            if (!done && prim.y == 0u && (cur.y & 0xff000000u) == 0u) {
+                if (sp == 0) {
+                    done = true;
+                } else {
+                    sp -= 1;
+                    cur = sp < LDS ? stack[sp][lane] : my_spill[(size_t)(sp - LDS) * lanes];
+                }
+            }
+        }
+        if (done) {
+            if (MODE == 0)
+                reinterpret_cast<uint4*>(hits)[i] = make_uint4(hit_id, INVALID32, INVALID32, __float_as_uint(hit_t));
+            else
+                miss_out[i] = is_miss ? 1 : 0;
+            live = false;
+        }
+    }
+}
+
 // CwBvh::ray_traverse_tlas_blas (cwbvh/mod.rs:341-491): one stack for both levels, TLAS primitives name BLASes
 // (meshes of a batched build) through the TLAS's primitive_indices, the closure is Triangle::intersect on the
 // mesh's triangles in its primitive order.  Restated as written, including that a TLAS primitive group popped after
@@ -401,6 +531,18 @@ static void launch_depth(uint32_t depth, uint32_t blocks, hipStream_t stream, A.
         k_traverse<MODE, COUNT, SRC, 8><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
 }
 
+// k_traverse_chunk<MODE, depth> with the launch's LDS stack depth.
+constexpr uint32_t CHUNK_MAX_RAYS_PER_LANE = 16;
+template <int MODE, class... A>
+static void launch_chunk(uint32_t depth, uint32_t blocks, hipStream_t stream, A... args) {
+    if (depth == 4u)
+        k_traverse_chunk<MODE, 4><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+    else if (depth == 16u)
+        k_traverse_chunk<MODE, 16><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+    else
+        k_traverse_chunk<MODE, 8><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+}
+
 // Grid: one lane per ray, at most the spill buffer's lane count (lanes then stride over the rays).
 static uint32_t trav_blocks(uint32_t n, const TravScratch& ts) {
     const uint32_t want = (n + TRAV_THREADS - 1) / TRAV_THREADS;
@@ -420,6 +562,21 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
     const MeshDesc* no_meshes = nullptr;
     const uint32_t* no_map = nullptr;
     const TraceSegment* no_segs = nullptr;
+    // Chunked launches (when the tuning table asks for them: incoherent rays) need at least two rays per resident
+    // lane; each lane then gets that many, refilled in the wave.
+    const uint32_t per_lane =
+        ts.chunked && ts.resident ? std::min<uint32_t>(n / ts.resident, CHUNK_MAX_RAYS_PER_LANE) : 0u;
+    const uint32_t per_wave = per_lane * TRAV_THREADS;
+    const uint64_t cblocks = per_wave ? ((uint64_t)n + per_wave - 1) / per_wave : 0u;
+    if (per_lane >= 2u && !node_visits && cblocks * TRAV_THREADS <= ts.lanes) {  // a spill column per lane
+        if (mode == 0)
+            launch_chunk<0>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits, miss, sp,
+                            per_wave);
+        else
+            launch_chunk<1>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits, miss, sp,
+                            per_wave);
+        return hipGetLastError();
+    }
     if (mode == 0) {
         if (node_visits)
             launch_depth<0, true, SRC_PLAIN>(ts.lds_depth, blocks, stream, nd, node_count, prim_count, tr, ry, n,
@@ -497,6 +654,7 @@ obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts, uint64_t n) {
     if (e != hipSuccess) return set_hip_error(ctx, e, "alloc traversal spill");
     ts.spill = ctx->scratch.trav_spill.ptr;
     ts.lanes = lanes;
+    ts.resident = ctx->trav_lanes;
     return OBVHS_OK;
 }
 

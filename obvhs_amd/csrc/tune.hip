@@ -4,6 +4,8 @@
 // 1: below 65,536, 2: below 1,048,576, 3: larger):
 //   sort_items     : keys per thread of the radix sort's tiles (sort.hip) -- the order never changes;
 //   trav_lds_depth : traversal stack entries kept in LDS (traverse.hip) -- the hits never change;
+//   trav_chunk     : single-BVH launches as wave-owned ray rows with lanes refilled (traverse.hip) -- the hits
+//                    never change;
 //   ploc_radius    : the PLOC search distance of builds that ask for distance 0 ("auto").  This one changes the
 //                    tree, so it is chosen from tree costs only: the smallest distance whose CWBVH cost (the
 //                    conversion's own cost model at the root, bvh2_to_cwbvh.rs:220-344) is within a tolerance of a
@@ -89,7 +91,7 @@ obvhs_status obvhs_tune_set(obvhs_context* ctx, const obvhs_tune_table* table) {
     if (!ctx || !table) return OBVHS_ERR_INVALID_ARG;
     for (int c = 0; c < 4; ++c)
         if (!valid_radius(table->ploc_radius[c]) || !valid_tile(table->sort_items[c]) ||
-            !valid_tile(table->trav_lds_depth[c]))
+            !valid_tile(table->trav_lds_depth[c]) || table->trav_chunk[c] > 1u)
             return OBVHS_ERR_INVALID_ARG;
     ctx->tune = *table;
     return OBVHS_OK;
@@ -143,9 +145,12 @@ obvhs_status obvhs_tune_traversal(obvhs_context* ctx, const obvhs_cwbvh* bvh, co
     obvhs_ray_hit* hits = ctx->tune_scratch.as<obvhs_ray_hit>();
     uint8_t* miss = ctx->tune_scratch.as<uint8_t>();
     uint32_t best = 8u;
+    bool best_chunked = false;
     float best_ms = INFINITY;
+    for (int chunked = 0; chunked < 2; ++chunked)
     for (uint32_t depth : CANDIDATE_DEPTHS) {
         ts.lds_depth = depth;
+        ts.chunked = chunked != 0;
         float ms = 0.0f;
         const obvhs_status s = time_min(
             ctx,
@@ -161,9 +166,11 @@ obvhs_status obvhs_tune_traversal(obvhs_context* ctx, const obvhs_cwbvh* bvh, co
         if (ms < best_ms) {
             best_ms = ms;
             best = depth;
+            best_chunked = chunked != 0;
         }
     }
     ctx->tune.trav_lds_depth[size_class(bvh->prim_count)] = best;
+    ctx->tune.trav_chunk[size_class(bvh->prim_count)] = best_chunked ? 1u : 0u;
     if (out_depth) *out_depth = best;
     return OBVHS_OK;
 }

@@ -43,8 +43,10 @@ class BuildExt(C.Structure):
 
 
 class TuneTable(C.Structure):
-    """obvhs_tune_table: per size class PLOC distance for "auto" builds, sort tile, traversal LDS stack depth."""
-    _fields_ = [("ploc_radius", C.c_uint32 * 4), ("sort_items", C.c_uint32 * 4), ("trav_lds_depth", C.c_uint32 * 4)]
+    """obvhs_tune_table: per size class PLOC distance for "auto" builds, sort tile, traversal LDS stack depth,
+    chunked traversal launches."""
+    _fields_ = [("ploc_radius", C.c_uint32 * 4), ("sort_items", C.c_uint32 * 4), ("trav_lds_depth", C.c_uint32 * 4),
+                ("trav_chunk", C.c_uint32 * 4)]
 
 
 class ServerConfig(C.Structure):

@@ -8,12 +8,14 @@ fixed order, so a missing barrier or an order-dependent write shows up as a run-
   second context give identical CWBVH nodes and primitive indices;
 * a batched build of mixed-size meshes run twice gives identical nodes, and equals the single builds;
 * closest-hit and occlusion traversal give identical results when repeated and when the rays are split into uneven
-  launches (every ray then runs on a different lane and spill column);
+  launches (every ray then runs on a different lane and spill column), also between launches large enough for
+  wave-owned ray chunks and the one-ray-per-lane launches of their pieces;
 * partial rebuilds and refits repeated from the same starting tree give identical trees."""
 import numpy as np
 import pytest
 
 import obvhs_amd as ob
+import oracle_lib as O
 from obvhs_amd import scenes
 from obvhs_amd.test_util import hash_noise
 
@@ -97,3 +99,39 @@ def test_rebuilds_and_refits_repeat_bit_for_bit(ctx):
         b.refit_from_triangles(moved)
         outs.append((rebuilt, b.nodes.tobytes()))
     assert outs[1] == outs[0] and outs[2] == outs[0]
+
+
+def test_chunked_launches_match_one_ray_per_lane(ctx):
+    """With the tuning table's trav_chunk set, launches of at least two rays per resident lane run as wave-owned
+    ray rows with lanes refilled (k_traverse_chunk); smaller launches still run one ray per lane (k_traverse).  The
+    same rays traced both ways give identical hits and occlusion flags, and a strided sample equals the oracle."""
+    saved = ctx.tune_table()
+    t = ctx.tune_table()
+    t["trav_chunk"] = [1] * 4
+    ctx.set_tune_table(t)
+    try:
+        _chunked_vs_pieces(ctx)
+    finally:
+        ctx.set_tune_table(saved)
+
+
+def _chunked_vs_pieces(ctx):
+    tris, rays = scenes.heightfield_workload(256, 2000, 1920, 1080, seed=4)
+    cw = ob.build_cwbvh_from_tris(ctx, tris, ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3))
+    bt = cw.reorder_triangles(tris)
+    shadow = np.concatenate([rays, rays])  # 4.1M rays: four or more per resident lane on any current device
+    for name, rs in (("primary", rays), ("doubled", shadow)):
+        print(f"[case] chunked launch {name} n={len(rs)}", flush=True)
+        whole = cw.ray_traverse(ctx.upload(rs), bt)
+        cuts = list(range(0, len(rs), 500000)) + [len(rs)]
+        parts = [cw.ray_traverse(ctx.upload(np.ascontiguousarray(rs[a:b])), bt) for a, b in zip(cuts[:-1], cuts[1:])]
+        assert np.concatenate(parts).tobytes() == whole.tobytes()
+        m_whole = cw.ray_traverse_miss(ctx.upload(rs), bt)
+        m_parts = [cw.ray_traverse_miss(ctx.upload(np.ascontiguousarray(rs[a:b])), bt)
+                   for a, b in zip(cuts[:-1], cuts[1:])]
+        assert np.array_equal(np.concatenate(m_parts), m_whole)
+    sample = np.ascontiguousarray(rays[::997])
+    ref = O.traverse(cw.nodes, tris[cw.primitive_indices], sample, 0, threads=8)
+    got = cw.ray_traverse(ctx.upload(rays), bt)[::997]
+    assert got.tobytes() == ref.tobytes()
+    assert 0.2 < np.isfinite(got["t"]).mean() < 1.0

@@ -18,7 +18,7 @@ from obvhs_amd import scenes
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = {"ploc_radius": [14] * 4, "sort_items": [8] * 4, "trav_lds_depth": [8] * 4}
+DEFAULT = {"ploc_radius": [14] * 4, "sort_items": [8] * 4, "trav_lds_depth": [8] * 4, "trav_chunk": [0] * 4}
 
 
 @pytest.fixture
@@ -42,6 +42,8 @@ def test_table_roundtrip_and_validation(fresh):
         with_table(fresh, sort_items=5)
     with pytest.raises(ob.ObvhsError):
         with_table(fresh, ploc_radius=3)
+    with pytest.raises(ob.ObvhsError):
+        with_table(fresh, trav_chunk=2)
     assert fresh.tune_table() == DEFAULT
 
 
@@ -80,6 +82,7 @@ def test_traversal_depths_give_identical_hits(fresh):
     d = ob.tune_traversal(fresh, cw, d_rays, bt)
     assert d in (4, 8, 16)
     assert fresh.tune_table()["trav_lds_depth"][ob.size_class(len(tris))] == d
+    assert fresh.tune_table()["trav_chunk"][ob.size_class(len(tris))] in (0, 1)
     assert cw.ray_traverse(d_rays, bt).tobytes() == ref_h.tobytes()
 
 
