@@ -184,13 +184,13 @@ def bench_batch(args, ob, ctx):
                                                      d_mesh.ptr, n_rays, hits.ptr), ctx.handle)
 
     if tuned is not None and not args.lds_depth:
-        # The batched launch takes the LDS depth of its mean mesh size's class: time the launch itself at each
-        # candidate depth (one warm-up and three timed launches each) and keep the fastest for the timed steps.
+        # The batched launch takes the LDS depth and launch form of its mean mesh size's class: time the launch
+        # itself with each (one warm-up and three timed launches each) and keep the fastest for the timed steps.
         cls = ob.size_class(n_tris // m)
         by_depth = {}
-        for depth in (4, 8, 16):
+        for depth in [(d, c) for c in (0, 1) for d in (4, 8, 16)]:
             t = ctx.tune_table()
-            t["trav_lds_depth"][cls] = depth
+            t["trav_lds_depth"][cls], t["trav_chunk"][cls] = depth
             ctx.set_tune_table(t)
             trace_batch()
             ms = []
@@ -201,11 +201,11 @@ def bench_batch(args, ob, ctx):
             by_depth[depth] = min(ms)
         best = min(by_depth, key=by_depth.get)
         t = ctx.tune_table()
-        t["trav_lds_depth"][cls] = best
+        t["trav_lds_depth"][cls], t["trav_chunk"][cls] = best
         ctx.set_tune_table(t)
-        tuned["traversal_lds_depth"] = best
+        tuned["traversal_lds_depth"], tuned["traversal_chunked"] = best
         tuned["traversal_size_class"] = cls
-        tuned["trace_ms_by_lds_depth"] = {str(k): round(v, 4) for k, v in by_depth.items()}
+        tuned["trace_ms_by_lds_depth_and_chunked"] = {f"{d}/{c}": round(v, 4) for (d, c), v in by_depth.items()}
     for _ in range(args.warmup):
         trace_batch()
     ctx.synchronize()

@@ -685,6 +685,7 @@ obvhs_status obvhs_batch_ray_traverse(obvhs_context* ctx, const obvhs_batch* b, 
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, b->meshes.empty() ? 0u : b->n_total / b->meshes.size());  // mean mesh
+    ts.chunked = chunk_for(ctx, b->meshes.empty() ? 0u : b->n_total / b->meshes.size());
     hipError_t e = launch_traverse_meshes(b->cw_nodes.as<CwNode>(), b->mesh_table.as<MeshDesc>(),
                                           (uint32_t)b->meshes.size(), d_ray_mesh,
                                           d_bvh_tris, d_rays, n, 0, d_hits, nullptr, ts, ctx->stream);
@@ -702,6 +703,7 @@ obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch
         if (ss != OBVHS_OK) return ss;
     }
     ts.lds_depth = lds_depth_for(ctx, b->meshes.empty() ? 0u : b->n_total / b->meshes.size());  // mean mesh
+    ts.chunked = chunk_for(ctx, b->meshes.empty() ? 0u : b->n_total / b->meshes.size());
     hipError_t e = launch_traverse_meshes(b->cw_nodes.as<CwNode>(), b->mesh_table.as<MeshDesc>(),
                                           (uint32_t)b->meshes.size(), d_ray_mesh,
                                           d_bvh_tris, d_rays, n, 1, nullptr, d_miss, ts, ctx->stream);

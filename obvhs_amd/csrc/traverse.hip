@@ -283,12 +283,14 @@ __global__ void __launch_bounds__(TRAV_THREADS)
 // waves sweep the rays together, as a one-ray-per-lane grid does), and a lane whose ray ends takes the wave's next
 // ray at the next step (a wave-uniform count of the rays handed out, no atomics) instead of idling until the wave's
 // longest ray ends.  The steps of every ray are k_traverse's, so the hits are identical.
-template <int MODE, int LDS>
+// SRC_MESHES: as in k_traverse, each ray's mesh (ray_mesh) selects its CWBVH and triangle slices; node_count_in is
+// then the mesh count.
+template <int MODE, int LDS, int SRC = SRC_PLAIN>
 __global__ void __launch_bounds__(TRAV_THREADS)
-    k_traverse_chunk(const uint4* __restrict__ nodes, uint32_t node_count, uint32_t prim_count,
-                     const float4* __restrict__ tris, const float4* __restrict__ rays, uint32_t n,
+    k_traverse_chunk(const uint4* __restrict__ nodes_in, uint32_t node_count_in, uint32_t prim_count_in,
+                     const float4* __restrict__ tris_in, const float4* __restrict__ rays, uint32_t n,
                      obvhs_ray_hit* __restrict__ hits, uint8_t* __restrict__ miss_out, uint2* __restrict__ spill,
-                     uint32_t per_wave) {
+                     uint32_t per_wave, const MeshDesc* __restrict__ meshes, const uint32_t* __restrict__ ray_mesh) {
     __shared__ uint2 stack[LDS][TRAV_THREADS];
     const uint32_t lane = threadIdx.x;
     const uint32_t lanes = gridDim.x * TRAV_THREADS;
@@ -302,6 +304,9 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     bool is_miss = true;
     int sp = 0;
     uint2 cur = make_uint2(0u, 0u), prim = make_uint2(0u, 0u);
+    const uint4* nodes = nodes_in;
+    const float4* tris = tris_in;
+    uint32_t node_count = node_count_in, prim_count = prim_count_in;
     for (;;) {
         const uint64_t want = __ballot(!live);
         if (want != 0ull) {
@@ -312,6 +317,20 @@ __global__ void __launch_bounds__(TRAV_THREADS)
                                    h % TRAV_THREADS;
                 if (h < per_wave && k < n) {
                     i = (uint32_t)k;
+                    if (SRC == SRC_MESHES) {  // the ray's mesh; an out-of-range id sees an empty BVH (a miss)
+                        const uint32_t mi = ray_mesh[i];
+                        nodes = nodes_in;
+                        tris = tris_in;
+                        node_count = 0u;
+                        prim_count = 0u;
+                        if (mi < node_count_in) {
+                            const MeshDesc md = meshes[mi];
+                            nodes += (size_t)md.cw_base * 5;
+                            tris += (size_t)md.prim_base * 3;
+                            node_count = md.cw_count;
+                            prim_count = md.prim_count;
+                        }
+                    }
                     const float4 ra = rays[4 * (size_t)i], rb = rays[4 * (size_t)i + 1], rc = rays[4 * (size_t)i + 2],
                                  rd = rays[4 * (size_t)i + 3];
                     r = RayData{ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, rc.x, rc.y, rc.z, rd.x, rd.y};
@@ -533,14 +552,24 @@ static void launch_depth(uint32_t depth, uint32_t blocks, hipStream_t stream, A.
 
 // k_traverse_chunk<MODE, depth> with the launch's LDS stack depth.
 constexpr uint32_t CHUNK_MAX_RAYS_PER_LANE = 16;
-template <int MODE, class... A>
+template <int MODE, int SRC, class... A>
 static void launch_chunk(uint32_t depth, uint32_t blocks, hipStream_t stream, A... args) {
     if (depth == 4u)
-        k_traverse_chunk<MODE, 4><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+        k_traverse_chunk<MODE, 4, SRC><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
     else if (depth == 16u)
-        k_traverse_chunk<MODE, 16><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+        k_traverse_chunk<MODE, 16, SRC><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
     else
-        k_traverse_chunk<MODE, 8><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+        k_traverse_chunk<MODE, 8, SRC><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+}
+
+// Rows of rays per lane for a chunked launch of n rays (0: one lane per ray), with a spill column per lane.
+static uint32_t chunk_rows(uint32_t n, const TravScratch& ts) {
+    const uint32_t per_lane =
+        ts.chunked && ts.resident ? std::min<uint32_t>(n / ts.resident, CHUNK_MAX_RAYS_PER_LANE) : 0u;
+    if (per_lane < 2u) return 0u;
+    const uint64_t per_wave = (uint64_t)per_lane * TRAV_THREADS;
+    const uint64_t blocks = ((uint64_t)n + per_wave - 1) / per_wave;
+    return blocks * TRAV_THREADS <= ts.lanes ? per_lane : 0u;
 }
 
 // Grid: one lane per ray, at most the spill buffer's lane count (lanes then stride over the rays).
@@ -564,17 +593,16 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
     const TraceSegment* no_segs = nullptr;
     // Chunked launches (when the tuning table asks for them: incoherent rays) need at least two rays per resident
     // lane; each lane then gets that many, refilled in the wave.
-    const uint32_t per_lane =
-        ts.chunked && ts.resident ? std::min<uint32_t>(n / ts.resident, CHUNK_MAX_RAYS_PER_LANE) : 0u;
-    const uint32_t per_wave = per_lane * TRAV_THREADS;
-    const uint64_t cblocks = per_wave ? ((uint64_t)n + per_wave - 1) / per_wave : 0u;
-    if (per_lane >= 2u && !node_visits && cblocks * TRAV_THREADS <= ts.lanes) {  // a spill column per lane
+    const uint32_t rows = node_visits ? 0u : chunk_rows(n, ts);
+    if (rows) {
+        const uint32_t per_wave = rows * TRAV_THREADS;
+        const uint32_t cblocks = (uint32_t)(((uint64_t)n + per_wave - 1) / per_wave);
         if (mode == 0)
-            launch_chunk<0>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits, miss, sp,
-                            per_wave);
+            launch_chunk<0, SRC_PLAIN>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits,
+                                       miss, sp, per_wave, no_meshes, no_map);
         else
-            launch_chunk<1>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits, miss, sp,
-                            per_wave);
+            launch_chunk<1, SRC_PLAIN>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits,
+                                       miss, sp, per_wave, no_meshes, no_map);
         return hipGetLastError();
     }
     if (mode == 0) {
@@ -603,6 +631,18 @@ hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, u
     uint2* sp = static_cast<uint2*>(ts.spill);
     uint32_t* no_counts = nullptr;
     const TraceSegment* no_segs = nullptr;
+    const uint32_t rows = chunk_rows(n, ts);
+    if (rows) {
+        const uint32_t per_wave = rows * TRAV_THREADS;
+        const uint32_t cblocks = (uint32_t)(((uint64_t)n + per_wave - 1) / per_wave);
+        if (mode == 0)
+            launch_chunk<0, SRC_MESHES>(ts.lds_depth, cblocks, stream, nd, mesh_count, 0u, tr, ry, n, hits, miss, sp,
+                                        per_wave, meshes, ray_mesh);
+        else
+            launch_chunk<1, SRC_MESHES>(ts.lds_depth, cblocks, stream, nd, mesh_count, 0u, tr, ry, n, hits, miss, sp,
+                                        per_wave, meshes, ray_mesh);
+        return hipGetLastError();
+    }
     if (mode == 0)
         launch_depth<0, false, SRC_MESHES>(ts.lds_depth, blocks, stream, nd, mesh_count, 0u, tr, ry, n, hits, miss,
                                            no_counts, no_counts, meshes, ray_mesh, no_segs, sp);

@@ -115,6 +115,31 @@ def test_chunked_launches_match_one_ray_per_lane(ctx):
         ctx.set_tune_table(saved)
 
 
+def test_chunked_batched_launch_matches_one_ray_per_lane(ctx):
+    """The batched traversal's chunked launch (each refilled ray selects its mesh) gives the hits of the
+    one-ray-per-lane launch."""
+    meshes = [scenes.soup(s, seed=20 + i) for i, s in enumerate([64, 3000, 20000, 500, 9000, 1, 0, 40000])]
+    offsets = np.zeros(len(meshes) + 1, dtype=np.uint32)
+    offsets[1:] = np.cumsum([len(m) for m in meshes])
+    tris = np.concatenate(meshes)
+    batch = ob.build_cwbvh_batch(ctx, tris, offsets, ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3))
+    bt = batch.reorder_triangles(tris)
+    rays = scenes.pinhole_rays(1920, 1100, eye=(0.5, 0.5, -1.5), target=(0.5, 0.5, 0.5))
+    ray_mesh = (hash_noise(np.arange(len(rays), dtype=np.uint32), 3, 9) * (len(meshes) + 1)).astype(np.uint32)
+    print(f"[case] chunked batched launch n={len(rays)}", flush=True)
+    saved = ctx.tune_table()
+    try:
+        outs = []
+        for chunk in (0, 1):
+            t = ctx.tune_table()
+            t["trav_chunk"] = [chunk] * 4
+            ctx.set_tune_table(t)
+            outs.append(batch.ray_traverse(rays, ray_mesh, bt).tobytes())
+    finally:
+        ctx.set_tune_table(saved)
+    assert outs[0] == outs[1]
+
+
 def _chunked_vs_pieces(ctx):
     tris, rays = scenes.heightfield_workload(256, 2000, 1920, 1080, seed=4)
     cw = ob.build_cwbvh_from_tris(ctx, tris, ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3))
