@@ -660,9 +660,9 @@ def main():
     n_rays = len(prim_rays) + len(bounce)
     log(f"{len(prim_rays)} primary + {len(bounce)} bounce rays")
     if not args.no_tune and not args.lds_depth and args.trav_chunk < 0:
-        # the traversal LDS stack depth and launch form on the primary rays, the launch they can change (the hits are
-        # the same for every choice; the bounce launch holds too few rays per lane for chunks)
-        tuning["traversal_lds_depth"] = ob.tune_traversal(ctx, cw, prim_rays, d_bvh_tris)
+        # the traversal LDS stack depth and launch form on this workload's rays, primary and bounce together (the
+        # hits are the same for every choice)
+        tuning["traversal_lds_depth"] = ob.tune_traversal(ctx, cw, np.concatenate([prim_rays, bounce]), d_bvh_tris)
         tuning["traversal_chunked"] = ctx.tune_table()["trav_chunk"][ob.size_class(n_tris)]
 
     if args.pmc_probe:
