@@ -13,8 +13,10 @@
 //                  nodes of each reinsertion -- identical accept decisions, because a conflict node's parent can
 //                  only have changed through an accepted reinsertion that already touched it; the accepted moves
 //                  (reinsertion.rs:342-382) applied in parallel (their node sets are disjoint); then one full
-//                  refit, which leaves every internal box equal to the union of its children exactly like the
-//                  crate's refit_from_fast sequence does.
+//                  refit, which leaves every internal box equal to the union of its children, as the crate's
+//                  refit_from_fast sequence does.  Where that sequence stops early a box keeps its old bits, which
+//                  differ from the union's only in the sign of a zero; so when a leaf holds a -0 coordinate the
+//                  accepted moves and their walks run as the crate runs them, one after another (k_reins_replay).
 //   pre-splits   : cwbvh/builder.rs:26-55 + splits.rs:16-158.  Largest and average half area (the average summed in
 //                  triangle order by one wave, as the crate sums it); per iteration every candidate's best of 11
 //                  split planes in parallel, appended right halves numbered by an order-preserving scan, the
@@ -215,9 +217,9 @@ __global__ void __launch_bounds__(T256) k_leaf_refit(Node32* __restrict__ nodes,
 // reinsertion
 // ---------------------------------------------------------------------------------------------------------------
 
-// bvh2/mod.rs:586-609
+// bvh2/mod.rs:586-609.  With `negzero` set, also raises *negzero when a leaf box holds a -0 coordinate.
 __global__ void __launch_bounds__(T256) k_parents(const Node32* __restrict__ nodes, uint32_t n,
-                                                  uint32_t* __restrict__ parents) {
+                                                  uint32_t* __restrict__ parents, uint32_t* __restrict__ negzero) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     if (i == 0) parents[0] = 0u;
     if (i >= n) return;
@@ -225,6 +227,11 @@ __global__ void __launch_bounds__(T256) k_parents(const Node32* __restrict__ nod
     if (nd.prim_count == 0u && nd.first_index + 1u < n) {
         parents[nd.first_index] = i;
         parents[nd.first_index + 1] = i;
+    } else if (negzero && nd.prim_count != 0u) {
+        const uint32_t nz = 0x80000000u;
+        if (__float_as_uint(nd.minx) == nz || __float_as_uint(nd.miny) == nz || __float_as_uint(nd.minz) == nz ||
+            __float_as_uint(nd.maxx) == nz || __float_as_uint(nd.maxy) == nz || __float_as_uint(nd.maxz) == nz)
+            atomicOr(negzero, 1u);
     }
 }
 
@@ -239,6 +246,7 @@ __global__ void __launch_bounds__(T256) k_cand_keys(const Node32* __restrict__ n
 struct ReinsMisc {
     uint32_t err;
     uint32_t positive;               // reinsertions with area_diff > 0
+    uint32_t negzero;                // a leaf box holds a -0 coordinate (k_parents)
     uint32_t next;                   // candidates handed out beyond the grid's first span
     uint32_t undecided[MIS_ROUNDS_PER_SYNC];
 };
@@ -484,6 +492,62 @@ __global__ void __launch_bounds__(T256) k_reins_apply(const uint32_t* __restrict
     store_node(&nodes[to], make_node(node_box(dst_node), 0u, left_sibling_of(from)));
     store_node(&nodes[sib], dst_node);
     store_node(&nodes[parent_id], sibling_node);
+}
+
+__device__ __forceinline__ bool box_value_eq(const Box& a, const Box& b) {
+    return a.lx == b.lx && a.ly == b.ly && a.lz == b.lz && a.hx == b.hx && a.hy == b.hy && a.hz == b.hz;
+}
+
+// refit_from_fast (bvh2/mod.rs:722-751, release build): every internal box on the way up from `index` is set to the
+// union of its children, except that the walk returns, without writing, at the second box whose recomputed value
+// equals the stored one (by value: -0 == +0).
+__device__ void refit_from_fast_walk(Node32* __restrict__ nodes, const uint32_t* __restrict__ parents,
+                                     uint32_t index) {
+    uint32_t same = 0;
+    for (;;) {
+        const Node32 nd = load_node(&nodes[index]);
+        if (nd.prim_count == 0u) {
+            const Box u = box_union(node_box(load_node(&nodes[nd.first_index])),
+                                    node_box(load_node(&nodes[nd.first_index + 1])));
+            if (box_value_eq(node_box(nd), u) && ++same == 2u) return;
+            store_node(&nodes[index], make_node(u, 0u, nd.first_index));
+        }
+        if (index == 0u) break;
+        index = parents[index];
+    }
+}
+
+// The crate's apply loop for one step (reinsertion.rs:175-195, 342-382), one accepted move after another in
+// priority order, each followed by its two refit_from_fast walks, with the parents kept as reinsert_node keeps them.
+// Where a walk stops early, boxes above keep the bits they had, which differ from a full refit's only in the sign
+// of a zero coordinate; the host runs this instead of k_reins_apply + a full refit when a leaf holds a -0.  One lane.
+__global__ void __launch_bounds__(64) k_reins_replay(const uint32_t* __restrict__ conf,
+                                                     const uint8_t* __restrict__ state, const ReinsMisc* misc,
+                                                     Node32* __restrict__ nodes, uint32_t* __restrict__ parents) {
+    if (threadIdx.x != 0u) return;
+    const uint32_t npos = misc->positive;
+    for (uint32_t k = 0; k < npos; ++k) {
+        if (state[k] != ST_ACCEPTED) continue;
+        const uint32_t to = conf[5 * (size_t)k + 0], from = conf[5 * (size_t)k + 1];
+        const uint32_t sib = conf[5 * (size_t)k + 2], parent_id = conf[5 * (size_t)k + 4];
+        const Node32 sibling_node = load_node(&nodes[sib]);
+        const Node32 dst_node = load_node(&nodes[to]);
+        store_node(&nodes[to], make_node(node_box(dst_node), 0u, left_sibling_of(from)));
+        store_node(&nodes[sib], dst_node);
+        store_node(&nodes[parent_id], sibling_node);
+        if (dst_node.prim_count == 0u) {
+            parents[dst_node.first_index] = sib;
+            parents[dst_node.first_index + 1] = sib;
+        }
+        if (sibling_node.prim_count == 0u) {
+            parents[sibling_node.first_index] = parent_id;
+            parents[sibling_node.first_index + 1] = parent_id;
+        }
+        parents[sib] = to;
+        parents[from] = to;
+        refit_from_fast_walk(nodes, parents, to);
+        refit_from_fast_walk(nodes, parents, parent_id);
+    }
 }
 
 __global__ void __launch_bounds__(T256) k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
@@ -1401,7 +1465,7 @@ obvhs_status bvh2_rebuild_path_flags(obvhs_context* ctx, obvhs_bvh2* bvh, const 
     uint32_t* parents = Q.parents.as<uint32_t>();
     k_fill_u32<<<1, 64, 0, st>>>(err, 0u, 1);
     OBVHS_TRY(hipMemsetAsync(d_flags, 0, N, st), "clear flags");
-    k_parents<<<(N + T256 - 1) / T256, T256, 0, st>>>(bvh->nodes.as<Node32>(), N, parents);
+    k_parents<<<(N + T256 - 1) / T256, T256, 0, st>>>(bvh->nodes.as<Node32>(), N, parents, nullptr);
     if (count) k_path_flags<<<(count + T256 - 1) / T256, T256, 0, st>>>(parents, N, d_leaves, count, d_flags, err);
     OBVHS_TRY(hipGetLastError(), "path flags launch");
     uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
@@ -1599,7 +1663,7 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
         OBVHS_TRY(Q.stack.ensure((size_t)fthreads * sd * 8), "alloc reinsertion stacks");
 
         k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(misc), 0u, sizeof(ReinsMisc) / 4);
-        k_parents<<<nblk, T256, 0, st>>>(nodes, N, parents);
+        k_parents<<<nblk, T256, 0, st>>>(nodes, N, parents, &misc->negzero);
         k_cand_keys<<<(ncand + T256 - 1) / T256, T256, 0, st>>>(nodes, take, Q.keys.as<uint64_t>());
         OBVHS_TRY(hipGetLastError(), "candidate launch");
         OBVHS_TRY(radix_sort_pairs(Q.keys.as<uint64_t>(), Q.vals.as<uint32_t>(), true, ncand, 32,
@@ -1612,10 +1676,11 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
         OBVHS_TRY(radix_sort_pairs(Q.rkeys.as<uint64_t>(), Q.rvals.as<uint32_t>(), true, count, 32,
                                    Q.sort_scratch.ptr, st, sort_items_for(ctx, count)),
                   "reinsertion sort");
-        OBVHS_TRY(hipMemcpyAsync(h, misc, 8, hipMemcpyDeviceToHost, st), "read reinsertion count");
+        OBVHS_TRY(hipMemcpyAsync(h, misc, 12, hipMemcpyDeviceToHost, st), "read reinsertion count");
         OBVHS_TRY(hipStreamSynchronize(st), "reinsertion count sync");
         if (h[0]) return device_error_to_status(h[0]);
         const uint32_t npos = h[1];
+        const bool replay = h[2] != 0u;
         if (npos == 0) continue;
         const uint32_t pblk = (npos + T256 - 1) / T256;
         k_fill_u32<<<std::min(nblk, 4096u), T256, 0, st>>>(reinterpret_cast<uint32_t*>(touched), 0u,
@@ -1639,11 +1704,18 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
             OBVHS_TRY(hipStreamSynchronize(st), "conflict rounds sync");
             if (h[0] == 0u) break;
         }
-        k_reins_apply<<<pblk, T256, 0, st>>>(Q.conf.as<uint32_t>(), Q.state.as<uint8_t>(), misc, nodes);
+        if (replay) {
+            // leaves hold a -0: the crate's walks decide which zero bits each box keeps, so they run as written
+            k_reins_replay<<<1, 64, 0, st>>>(Q.conf.as<uint32_t>(), Q.state.as<uint8_t>(), misc, nodes, parents);
+        } else {
+            // every zero coordinate is +0, so a value-equal box has equal bits and one full refit (deferred to the
+            // next step or fused with the cost table) leaves exactly the boxes the crate's walks leave
+            k_reins_apply<<<pblk, T256, 0, st>>>(Q.conf.as<uint32_t>(), Q.state.as<uint8_t>(), misc, nodes);
+        }
         OBVHS_TRY(hipGetLastError(), "apply launch");
         s = tree_levels(ctx, bvh, meta, levels, nullptr);
         if (s != OBVHS_OK) return s;
-        refit_pending = true;
+        refit_pending = !replay;
     }
     bvh->pass_ordered = false;  // children are no longer stored in PLOC pass order (reinsertion.rs:102)
     bvh->dp_max_prims = 0;

@@ -38,6 +38,24 @@ def soup(n, seed=0):
     return triangles_from_vertices(*verts)
 
 
+def signed_zero_soup(n, seed=0):
+    """n triangles with vertices on a 1/4 lattice of [-1, 1]^3 (many coordinates exactly zero), each zero
+    coordinate given a seeded sign: boxes that are equal in value but differ in the sign of a zero, the inputs on
+    which refit_from_fast's early exit (bvh2/mod.rs:722-751) decides which bits a box keeps."""
+    i = np.arange(n, dtype=np.uint32)
+    q = np.float32(0.25)
+    c = np.stack([np.floor(hash_noise(i, a, seed) * np.float32(9.0)) - np.float32(4.0) for a in range(3)], axis=1)
+    verts = []
+    for k in range(3):
+        off = np.stack([np.floor(hash_noise(i, 3 + 3 * k + a, seed) * np.float32(3.0)) - np.float32(1.0)
+                        for a in range(3)], axis=1)
+        v = ((c + off) * q).astype(np.float32)
+        neg = np.stack([hash_noise(i, 12 + 3 * k + a, seed) < np.float32(0.5) for a in range(3)], axis=1)
+        v = np.where((v == 0) & neg, np.float32(-0.0), np.where(v == 0, np.float32(0.0), v)).astype(np.float32)
+        verts.append(v)
+    return triangles_from_vertices(*verts)
+
+
 def look_at_basis(eye, target, up=(0.0, 1.0, 0.0)):
     eye = np.asarray(eye, dtype=np.float64)
     fwd = np.asarray(target, dtype=np.float64) - eye

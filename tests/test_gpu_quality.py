@@ -32,6 +32,8 @@ SCENES = {
     "soup5k": lambda: scenes.soup(5000, seed=3),
     "heightfield": lambda: hf_scene(64, 200, 1),
     "thin": lambda: np.concatenate([hf_scene(40, 50, 2), scenes.thin_triangles(400, seed=2)]),
+    # exact +0 and -0 coordinates: refit_from_fast's early exit decides which zero bits a box keeps
+    "signed_zero": lambda: scenes.signed_zero_soup(3000, seed=1),
 }
 
 

@@ -11,7 +11,11 @@ What pins them (none of these re-types the oracle's formulas):
     refit everything, re-sum), and no other legal target decreases them by more (brute force over all targets);
   * the optimizer: trees stay valid and tight, the summed half area never grows, a dropped conflict check breaks
     the tree, rays hit what brute force hits;
-  * refit_all: after a seeded vertex jitter every internal box is the union of its children (tight-fit validator).
+  * refit_all: after a seeded vertex jitter every internal box is the union of its children (tight-fit validator);
+  * refit_from_fast's early exit (bvh2/mod.rs:722-751): a hand-derived move on a tree of boxes with exact +0 and
+    -0 coordinates, where the second walk stops below the root and the root keeps a zero sign that a walk to the
+    root would change (caught when the walks never stop early); and a seeded +0/-0 soup whose optimized tree
+    differs from the never-stopping variant, so the GPU parity case on that soup exercises the early exit.
 Tie orders of the reference's unstable sorts (candidates by area, reinsertions by area_diff) stay unpinned: the
 oracle orders ties by node index / candidate rank."""
 from fractions import Fraction
@@ -21,7 +25,7 @@ import pytest
 
 import oracle_lib as O
 from obvhs_amd import scenes, test_util
-from obvhs_amd.types import AABB, make_rays, triangles_from_vertices
+from obvhs_amd.types import AABB, BVH2_NODE, make_rays, triangles_from_vertices
 
 
 @pytest.fixture
@@ -222,3 +226,61 @@ def test_refit_all_after_jitter():
     assert O.bvh2_validate(refit, b["prim_indices"], ja, True, ordered=False)[0] == 0
     untouched = refit["prim_count"] != 0
     assert refit[untouched].tobytes() == nodes[untouched].tobytes()
+
+
+def _node(lo, hi, prim_count, first_index):
+    n = np.zeros(1, dtype=BVH2_NODE)
+    n["min"][0, :3] = np.array(lo, dtype=np.float32)
+    n["max"][0, :3] = np.array(hi, dtype=np.float32)
+    n["prim_count"], n["first_index"] = prim_count, first_index
+    return n[0]
+
+
+def test_refit_from_fast_early_exit_keeps_zero_sign(mutation):
+    """root(A(C(E, F), D), B) with E, D, F all at x = 0 (D's zero is -0.0): moving D (node 4) next to E (node 5)
+    turns node 5 into X(E, D) and node 1 into C'(X, F).  Hand derivation with min(a, b) = a < b ? a : b (the second
+    operand on a tie):
+      walk from 5: X = union(E, D) equals E's box in value (same = 1, written: min.x = -0 from D); node 1's union
+                   (X, F) equals C's box in value (same = 2): returns before writing node 1;
+      walk from 1: node 1 = union(X, F), min.x = +0 from F (same = 1, written); the root's union equals its box in
+                   value (same = 2): the root keeps min.x = -0, which it took from D when A = union(C, D) was built.
+    A walk to the root would store union(C', B), min.x = +0."""
+    nz, pz = np.float32(-0.0), np.float32(0.0)
+    E = ((pz, 0, 0), (1, 1, 1))
+    D = ((nz, 0.25, 0.25), (0.5, 0.5, 0.5))
+    F = ((pz, 2, 2), (3, 3, 3))
+    B = ((5, 5, 5), (6, 6, 6))
+    # boxes of the original tree as a build computes them (second operand on ties)
+    C = ((pz, 0, 0), (3, 3, 3))          # union(E, F): min.x from F (+0)
+    A = ((nz, 0, 0), (3, 3, 3))          # union(C, D): min.x from D (-0)
+    R = ((nz, 0, 0), (6, 6, 6))          # union(A, B)
+    nodes = np.array([_node(*R, 0, 1), _node(*A, 0, 3), _node(*B, 1, 3), _node(*C, 0, 5), _node(*D, 1, 2),
+                      _node(*E, 1, 0), _node(*F, 1, 1)], dtype=BVH2_NODE)
+    X = ((nz, 0, 0), (1, 1, 1))          # union(E, D): min.x from D (-0)
+    Cp = ((pz, 0, 0), (3, 3, 3))         # union(X, F): min.x from F (+0)
+    want = np.array([_node(*R, 0, 1), _node(*Cp, 0, 5), _node(*B, 1, 3), _node(*E, 1, 0), _node(*D, 1, 2),
+                     _node(*X, 0, 3), _node(*F, 1, 1)], dtype=BVH2_NODE)
+    got = O.reinsert_node(nodes, 4, 5)
+    assert got.tobytes() == want.tobytes()
+    assert got["min"][0, 0].view(np.uint32) == 0x80000000
+    mutation(14)  # walks that never stop early
+    full = O.reinsert_node(nodes, 4, 5)
+    mutation(0)
+    assert full["min"][0, 0].view(np.uint32) == 0
+    assert full.tobytes() != want.tobytes()
+
+
+@pytest.mark.parametrize("ratio,seq", [(0.05, [1.0, 1.0 / 3.0]), (0.2, None), (1.0, [1.0])])
+def test_signed_zero_soup_exercises_early_exit(mutation, ratio, seq):
+    tris = scenes.signed_zero_soup(3000, seed=1)
+    aabbs = O.tri_aabbs(tris)
+    b = O.ploc_build(aabbs, 14, 0)
+    crate = O.reinsertion(b["nodes"], ratio, seq)
+    mutation(14)
+    full = O.reinsertion(b["nodes"], ratio, seq)
+    mutation(0)
+    assert crate.tobytes() != full.tobytes()
+    # the difference is the sign of zero coordinates only: the values are the same, the tree valid and tight
+    np.testing.assert_array_equal(crate["min"], full["min"])
+    np.testing.assert_array_equal(crate["max"], full["max"])
+    assert O.bvh2_validate(crate, b["prim_indices"], aabbs, True, ordered=False)[0] == 0
