@@ -99,6 +99,11 @@ enum {
     MUT_REINS_NO_CONFLICT = 13,  // reinsertions applied without the conflict check
     MUT_REFIT_FULL = 14,       // refit_from_fast never stops early
     MUT_REBUILD_SLOT_PARITY = 15,  // partial rebuild checks the right (even) slot of a pair for being free
+    MUT_CW_DIST_LE = 16,       // `<=` instead of `<` in the conversion's distribute searches
+    MUT_CW_LEAF_LE = 17,       // `cost_leaf <= cost_internal` (a tie makes a leaf)
+    MUT_CW_LUT_FLIP_X = 18,    // direction LUT with the x sign flipped
+    MUT_CW_SLOT_LE = 19,       // `<=` instead of `<` in the greedy slot assignment (the last minimum wins)
+    MUT_CW_SLOT_MAJOR = 20,    // greedy slot assignment scanned slot-major (slot outer, child inner)
 };
 int g_mut = MUT_NONE;
 
@@ -536,7 +541,7 @@ struct Converter {
             uint8_t dl = INVALID8, dr = INVALID8;
             for (int k = 0; k < 7; ++k) {
                 float c = decisions[f7 + k].cost + decisions[n7 + 6 - k].cost;
-                if (c < cost_distribute) {
+                if (g_mut == MUT_CW_DIST_LE ? c <= cost_distribute : c < cost_distribute) {
                     cost_distribute = c;
                     dl = (uint8_t)k;
                     dr = (uint8_t)(6 - k);
@@ -544,7 +549,7 @@ struct Converter {
             }
             float cost_internal = cost_distribute + ha;
             Decision& d = decisions[dec];
-            if (cost_leaf < cost_internal) {
+            if (g_mut == MUT_CW_LEAF_LE ? cost_leaf <= cost_internal : cost_leaf < cost_internal) {
                 d.kind = KIND_LEAF;
                 d.cost = cost_leaf;
             } else {
@@ -560,7 +565,7 @@ struct Converter {
             uint8_t dl = INVALID8, dr = INVALID8;
             for (int k = 0; k < i; ++k) {
                 float c = decisions[f7 + k].cost + decisions[n7 + i - k - 1].cost;
-                if (c < cost_distribute) {
+                if (g_mut == MUT_CW_DIST_LE ? c <= cost_distribute : c < cost_distribute) {
                     cost_distribute = c;
                     dl = (uint8_t)k;
                     dr = (uint8_t)(i - k - 1);
@@ -655,14 +660,27 @@ struct Converter {
         for (;;) {
             float min_cost = FMAX;
             size_t min_slot = INVALID32, min_index = INVALID32;
-            for (uint32_t c = 0; c < child_count; ++c) {
-                if (assignment[c] == INVALID32) {
-                    for (int s = 0; s < 8; ++s) {
-                        float cst = cost[c][s];
-                        if (!slot_filled[s] && cst < min_cost) {
-                            min_cost = cst;
+            if (g_mut == MUT_CW_SLOT_MAJOR) {
+                for (int s = 0; s < 8; ++s) {
+                    if (slot_filled[s]) continue;
+                    for (uint32_t c = 0; c < child_count; ++c) {
+                        if (assignment[c] == INVALID32 && cost[c][s] < min_cost) {
+                            min_cost = cost[c][s];
                             min_slot = s;
                             min_index = c;
+                        }
+                    }
+                }
+            } else {
+                for (uint32_t c = 0; c < child_count; ++c) {
+                    if (assignment[c] == INVALID32) {
+                        for (int s = 0; s < 8; ++s) {
+                            float cst = cost[c][s];
+                            if (!slot_filled[s] && (g_mut == MUT_CW_SLOT_LE ? cst <= min_cost : cst < min_cost)) {
+                                min_cost = cst;
+                                min_slot = s;
+                                min_index = c;
+                            }
                         }
                     }
                 }
@@ -1558,7 +1576,8 @@ int orc_bvh2_to_cwbvh(const Bvh2Node* nodes, uint64_t nnodes, const uint32_t* pr
     cv.bvh2_prim_indices = prim_indices;
     cv.order_children = order_children != 0;
     for (int s = 0; s < 8; ++s)  // bvh2_to_cwbvh.rs:40-50
-        cv.lut[s] = v3((s & 4) ? -1.0f : 1.0f, (s & 2) ? -1.0f : 1.0f, (s & 1) ? -1.0f : 1.0f);
+        cv.lut[s] = v3(((s & 4) != 0) != (g_mut == MUT_CW_LUT_FLIP_X) ? -1.0f : 1.0f, (s & 2) ? -1.0f : 1.0f,
+                       (s & 1) ? -1.0f : 1.0f);
     cv.decisions.assign((size_t)nnodes * 7, Decision{});
     cv.out.reserve((size_t)nprims);
     cv.out.push_back(CwBvhNode{});
@@ -1575,6 +1594,26 @@ int orc_bvh2_to_cwbvh(const Bvh2Node* nodes, uint64_t nnodes, const uint32_t* pr
     if (out_cw_to_bvh2) std::memcpy(out_cw_to_bvh2, cv.cw_to_bvh2.data(), cv.out.size() * sizeof(uint32_t));
     *out_num_nodes = cv.out.size();
     *out_num_prims = cv.prims.size();
+    return ORC_OK;
+}
+
+// The conversion's cost table (calculate_cost, bvh2_to_cwbvh.rs:220-344) for each BVH2 node and i = 0..6:
+// cost, kind (0 LEAF, 1 INTERNAL, 2 DISTRIBUTE), distribute_left, distribute_right; arrays of 7 * nnodes.
+int orc_cw_cost_table(const Bvh2Node* nodes, uint64_t nnodes, uint32_t max_prims_per_leaf, float* costs,
+                      uint8_t* kinds, uint8_t* dist_left, uint8_t* dist_right) {
+    if (nnodes == 0) return ORC_OK;
+    Converter cv;
+    cv.nodes = nodes;
+    cv.nnodes = (size_t)nnodes;
+    cv.decisions.assign((size_t)nnodes * 7, Decision{});
+    cv.calculate_cost(max_prims_per_leaf);
+    if (cv.err) return cv.err;
+    for (size_t i = 0; i < (size_t)nnodes * 7; ++i) {
+        costs[i] = cv.decisions[i].cost;
+        kinds[i] = (uint8_t)cv.decisions[i].kind;
+        dist_left[i] = cv.decisions[i].distribute_left;
+        dist_right[i] = cv.decisions[i].distribute_right;
+    }
     return ORC_OK;
 }
 

@@ -77,6 +77,8 @@ def lib():
         L.orc_partial_rebuild.argtypes = [P, C.c_uint64, P, C.c_uint32, C.c_uint32, P, C.c_uint32]
         L.orc_full_rebuild.restype = C.c_int
         L.orc_full_rebuild.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
+        L.orc_cw_cost_table.restype = C.c_int
+        L.orc_cw_cost_table.argtypes = [P, C.c_uint64, C.c_uint32, P, P, P, P]
         L.orc_primitives_to_nodes.restype = C.c_int
         L.orc_primitives_to_nodes.argtypes = [P, C.c_uint64, P, C.c_uint64, P]
         _lib = L
@@ -152,6 +154,19 @@ def bvh2_to_cwbvh(nodes, prim_indices, max_prims_per_leaf=3, order_children=True
     if rc != 0:
         raise OracleError(rc)
     return out[:nnodes.value].copy(), prims[:nprims.value].copy(), cw2b[:nnodes.value].copy()
+
+
+def cw_cost_table(nodes, max_prims_per_leaf=3):
+    """The conversion's cost table (bvh2_to_cwbvh.rs:220-344): cost, kind (0 LEAF, 1 INTERNAL, 2 DISTRIBUTE),
+    distribute_left, distribute_right, each of shape (len(nodes), 7)."""
+    nn = len(nodes)
+    cost = np.zeros(nn * 7, dtype=np.float32)
+    kind, dl, dr = (np.zeros(nn * 7, dtype=np.uint8) for _ in range(3))
+    rc = lib().orc_cw_cost_table(_ptr(np.ascontiguousarray(nodes)), nn, max_prims_per_leaf, _ptr(cost), _ptr(kind),
+                                 _ptr(dl), _ptr(dr))
+    if rc != 0:
+        raise OracleError(rc)
+    return tuple(a.reshape(nn, 7) for a in (cost, kind, dl, dr))
 
 
 def traverse(cw_nodes, bvh_tris, rays, mode=0, threads=1, counts=False):

@@ -17,8 +17,9 @@ Oracle functions and what pins each of them here:
   bvh2_to_cwbvh (bvh2_to_cwbvh.rs)            CwBvh::validate invariants (cwbvh/mod.rs:747-908), quantised child
                                               boxes contain the exact child boxes except at the 0/255 clamp, and
                                               traversal equals brute force; planted floor-for-ceil is caught.  The
-                                              exact DP tie choices and octant slot order are pinned by the
-                                              restatement only (no reference test fixes them): parity unpinned there.
+                                              cost DP against an exhaustive enumeration of collapses, its tie rules
+                                              and the octant slot assignment on hand-derived cases, and
+                                              safe_inverse: tests/test_oracle_cwbvh.py.
   Triangle::intersect (triangle.rs:35-76)     exact known answers, the -0.0 sign-bit miss (triangle.rs:60)
   node test + traverse! (simd.rs, macro)      brute-force equality, the 1e-4 tmin clamp, highest-bit-first
                                               primitive order on exact ties
