@@ -107,6 +107,65 @@ enum {
 };
 int g_mut = MUT_NONE;
 
+// Threads of the multithreaded build (the CPU baseline's counterpart of the crate's `parallel` feature); 1 runs the
+// serial restatement.  Every parallel section below mirrors a rayon section of the crate, with its size threshold,
+// and produces the same result as the serial code.
+int g_threads = 1;
+
+// Runs fn(begin, end) over [0, n) split into `parts` contiguous chunks, one thread each.
+template <class F>
+void parallel_chunks(size_t n, int parts, F&& fn) {
+    if (parts <= 1 || n < 2) {
+        fn((size_t)0, n);
+        return;
+    }
+    const size_t chunk = (n + (size_t)parts - 1) / (size_t)parts;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < parts; ++t) {
+        const size_t b = std::min(n, (size_t)t * chunk), e = std::min(n, b + chunk);
+        if (b < e) pool.emplace_back([&fn, b, e] { fn(b, e); });
+    }
+    for (auto& th : pool) th.join();
+}
+
+// Sorts [first, last) with `less` on g_threads threads: chunks sorted in parallel, then merged pairwise in parallel
+// rounds.  `less` is a strict total order here (ties broken by a unique key), so the result equals std::sort's.
+template <class T, class L>
+void parallel_sort(std::vector<T>& v, L less) {
+    const size_t n = v.size();
+    const int parts = g_threads;
+    if (parts <= 1 || n < 4096) {
+        std::sort(v.begin(), v.end(), less);
+        return;
+    }
+    const size_t chunk = (n + (size_t)parts - 1) / (size_t)parts;
+    std::vector<size_t> bounds;
+    for (size_t b = 0; b < n; b += chunk) bounds.push_back(b);
+    bounds.push_back(n);
+    {
+        std::vector<std::thread> pool;
+        for (size_t k = 0; k + 1 < bounds.size(); ++k)
+            pool.emplace_back([&, k] { std::sort(v.begin() + bounds[k], v.begin() + bounds[k + 1], less); });
+        for (auto& th : pool) th.join();
+    }
+    std::vector<T> tmp(n);
+    while (bounds.size() > 2) {
+        std::vector<size_t> nb;
+        std::vector<std::thread> pool;
+        for (size_t k = 0; k + 1 < bounds.size(); k += 2) {
+            const size_t a = bounds[k], m = bounds[k + 1], e = k + 2 < bounds.size() ? bounds[k + 2] : m;
+            nb.push_back(a);
+            pool.emplace_back([&, a, m, e] {
+                std::merge(v.begin() + a, v.begin() + m, v.begin() + m, v.begin() + e, tmp.begin() + a, less);
+            });
+        }
+        nb.push_back(n);
+        for (auto& th : pool) th.join();
+        std::swap(v, tmp);
+        bounds.swap(nb);
+    }
+}
+
 // glam Vec3A::min / max on x86 SSE (aabb.rs:84-89 via glam).
 inline float smin(float a, float b) { return a < b ? a : b; }
 inline float smax(float a, float b) { return a > b ? a : b; }
@@ -272,6 +331,28 @@ struct SearchCache {
     }
 };
 
+// find_best_node_parallel for the first search-distance clusters of a chunk: the areas towards lower indices are
+// computed (no earlier cluster of this chunk cached them), the ones towards higher indices cached as
+// find_best_node caches them; the same `<=` rule in the same order.
+int8_t find_best_node_basic_fill(size_t index, const Bvh2Node* nodes, size_t count, SearchCache& cache) {
+    const size_t sd = cache.sd;
+    size_t best_node = index;
+    float best_cost = INFINITY;
+    size_t begin = index - std::min(sd, index);
+    size_t end = std::min(index + sd + 1, count);
+    const Aabb& our = nodes[index].aabb;
+    for (size_t other = begin; other < end; ++other) {
+        if (other == index) continue;
+        float cost = half_area(aabb_union(our, nodes[other].aabb));
+        if (other > index) cache.front(index, other) = cost;
+        if (cost <= best_cost) {
+            best_node = other;
+            best_cost = cost;
+        }
+    }
+    return (int8_t)((int64_t)best_node - (int64_t)index);
+}
+
 // Morton64 (ploc/mod.rs:705-709) and Morton128 (ploc/mod.rs:686-692) as one record: U64 keys leave `hi` zero.
 struct MortonKey {
     uint64_t hi;
@@ -328,7 +409,8 @@ int ploc_from_leaves(std::vector<Bvh2Node>& current, const uint32_t* tie, const 
 
     // ploc/mod.rs:782-809 gen_mort
     std::vector<MortonKey> mortons(n);
-    for (size_t i = 0; i < n; ++i) {
+    parallel_chunks(n, n > 100000 ? g_threads : 1, [&](size_t i0, size_t i1) {
+    for (size_t i = i0; i < i1; ++i) {
         const Aabb& a = current[i].aabb;
         float cx = (a.max.x + a.min.x) * 0.5f, cy = (a.max.y + a.min.y) * 0.5f, cz = (a.max.z + a.min.z) * 0.5f;
         double px = (double)cx * scale[0] + offset[0];
@@ -343,10 +425,12 @@ int ploc_from_leaves(std::vector<Bvh2Node>& current, const uint32_t* tie, const 
         }
         if (tr && tr->codes) tr->codes[i] = mortons[i].lo;
     }
+    });
     // ploc/mod.rs:811-827. The reference uses unstable sorts; ties are ordered by the tie key here (the original
-    // index for a build, the node slot for a partial rebuild).
+    // index for a build, the node slot for a partial rebuild).  Multithreaded: par_sort_unstable_by_key
+    // (ploc/mod.rs:814-815) as a parallel merge sort, with the same total order.
     const bool ties_desc = g_mut == MUT_SORT_TIES_DESC;
-    std::sort(mortons.begin(), mortons.end(), [ties_desc, tie](const MortonKey& a, const MortonKey& b) {
+    parallel_sort(mortons, [ties_desc, tie](const MortonKey& a, const MortonKey& b) {
         const uint64_t ta = tie ? tie[a.index] : a.index, tb = tie ? tie[b.index] : b.index;
         if (a.hi != b.hi) return a.hi < b.hi;
         return a.lo < b.lo || (a.lo == b.lo && (ties_desc ? ta > tb : ta < tb));
@@ -368,13 +452,27 @@ int ploc_from_leaves(std::vector<Bvh2Node>& current, const uint32_t* tie, const 
         if (tr) tr->num_iters++;
         if (sd == 1 || depth < search_depth_threshold) {
             // ploc/mod.rs:329-382 distance-1 rule
-            float last_cost = INFINITY;
-            for (size_t i = 0; i + 1 < count; ++i) {
-                float cost = half_area(aabb_union(current[i].aabb, current[i + 1].aabb));
-                merge[i] = last_cost < cost ? -1 : 1;
-                last_cost = cost;
-            }
+            // (multithreaded above 300k clusters, ploc/mod.rs:340-375: each chunk starts from its left
+            // neighbour's cost)
+            parallel_chunks(count - 1, count >= 300000 ? g_threads : 1, [&](size_t i0, size_t i1) {
+                float last_cost = i0 == 0 ? INFINITY
+                                          : half_area(aabb_union(current[i0 - 1].aabb, current[i0].aabb));
+                for (size_t i = i0; i < i1; ++i) {
+                    float cost = half_area(aabb_union(current[i].aabb, current[i + 1].aabb));
+                    merge[i] = last_cost < cost ? -1 : 1;
+                    last_cost = cost;
+                }
+            });
             merge[count - 1] = -1;
+        } else if (g_threads > 1 && count >= 4000 && !basic_search) {
+            // ploc/mod.rs:389-417: chunks in parallel, each with its own cache; the first sd clusters of a chunk
+            // compute the areas the cache would have held (find_best_node_parallel), so results are the serial ones
+            parallel_chunks(count, g_threads, [&](size_t i0, size_t i1) {
+                SearchCache local(sd);
+                for (size_t i = i0; i < i1; ++i)
+                    merge[i] = i - i0 <= sd ? find_best_node_basic_fill(i, current.data(), count, local)
+                                           : local.find_best_node(i, current.data(), count);
+            });
         } else {
             // ploc/mod.rs:384-387
             for (size_t i = 0; i < count; ++i) {
@@ -472,13 +570,29 @@ int ploc_build(const Aabb* aabbs, const uint32_t* indices, size_t n, size_t sd, 
 
     std::vector<Bvh2Node> current(n);
     Aabb total = aabb_empty();
-    for (size_t i = 0; i < n; ++i) {  // ploc/mod.rs:187-194, 233-243
-        Aabb a = aabbs[i];
-        a.min.w = a.max.w = 0.0f;
-        if (has_nan(a)) return ORC_ERR_NAN_INPUT;
-        aabb_extend(total, a.min);
-        aabb_extend(total, a.max);
-        current[i] = Bvh2Node{a, 1u, indices[i], 0u, 0u};
+    // ploc/mod.rs:187-194, 233-243 (multithreaded above 500k leaves, ploc/mod.rs:199-231: per-chunk bounds, then
+    // their union in chunk order)
+    const int parts = n >= 500000 ? g_threads : 1;
+    std::vector<Aabb> part_total((size_t)std::max(parts, 1), aabb_empty());
+    std::vector<uint8_t> part_nan((size_t)std::max(parts, 1), 0);
+    const size_t chunk = (n + (size_t)std::max(parts, 1) - 1) / (size_t)std::max(parts, 1);
+    parallel_chunks(n, parts, [&](size_t i0, size_t i1) {
+        const size_t k = i0 / chunk;
+        Aabb t = aabb_empty();
+        for (size_t i = i0; i < i1; ++i) {
+            Aabb a = aabbs[i];
+            a.min.w = a.max.w = 0.0f;
+            if (has_nan(a)) part_nan[k] = 1;
+            aabb_extend(t, a.min);
+            aabb_extend(t, a.max);
+            current[i] = Bvh2Node{a, 1u, indices[i], 0u, 0u};
+        }
+        part_total[k] = t;
+    });
+    for (size_t k = 0; k < part_total.size(); ++k) {
+        if (part_nan[k]) return ORC_ERR_NAN_INPUT;
+        aabb_extend(total, part_total[k].min);
+        aabb_extend(total, part_total[k].max);
     }
     const size_t nodes_count = 2 * n - 1;
     for (size_t i = 0; i < nodes_count; ++i) out_nodes[i] = Bvh2Node{};
@@ -1276,9 +1390,21 @@ void reinsertion_run(std::vector<Bvh2Node>& nodes, float batch_size_ratio, const
         const size_t count = node_count - 1;
         std::fill(touched.begin(), touched.end(), 0);
         reins.clear();
-        for (size_t i = 0; i < count && i < cand.size(); ++i) {
-            Reinsertion r = find_reinsertion(nodes, parents, cand[i].second, stack);
-            if (r.area_diff > 0.0f) reins.push_back(r);
+        if (g_threads > 1) {
+            // reinsertion.rs:145-155: every candidate's search in parallel (par_iter), then the positive ones
+            const size_t m = std::min(count, cand.size());
+            std::vector<Reinsertion> all(m);
+            parallel_chunks(m, g_threads, [&](size_t i0, size_t i1) {
+                std::vector<std::pair<float, uint32_t>> st;
+                for (size_t i = i0; i < i1; ++i) all[i] = find_reinsertion(nodes, parents, cand[i].second, st);
+            });
+            for (const Reinsertion& r : all)
+                if (r.area_diff > 0.0f) reins.push_back(r);
+        } else {
+            for (size_t i = 0; i < count && i < cand.size(); ++i) {
+                Reinsertion r = find_reinsertion(nodes, parents, cand[i].second, stack);
+                if (r.area_diff > 0.0f) reins.push_back(r);
+            }
         }
         std::stable_sort(reins.begin(), reins.end(),
                          [](const Reinsertion& a, const Reinsertion& b) { return a.area_diff > b.area_diff; });
@@ -1950,6 +2076,8 @@ double orc_now() {
 
 extern "C" {
 void orc_set_mutation(int m) { g_mut = m; }
+// Threads of the multithreaded build (1: the serial restatement).
+void orc_set_threads(int t) { g_threads = t < 1 ? 1 : t; }
 // Host libm log2f (the function the reference's f32::log2 calls on linux-gnu), for the exponent-table tests.
 float orc_log2f(float x) { return log2f(x); }
 }

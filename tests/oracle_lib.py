@@ -63,6 +63,7 @@ def lib():
         L.orc_log2f.restype = C.c_float
         L.orc_log2f.argtypes = [C.c_float]
         L.orc_set_mutation.argtypes = [C.c_int]
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_presplit.restype = C.c_int
         L.orc_presplit.argtypes = [P, C.c_uint64, C.c_uint64, P, P, C.c_uint64, P, P, P]
         L.orc_split_triangle.argtypes = [P, C.c_uint32, C.c_float, P, P]
@@ -327,3 +328,17 @@ def tlas_traverse(tlas_nodes, tlas_prims, blas_list, rays):
                             _ptr(ncount), _ptr(tris), _ptr(toff), len(blas_list),
                             _ptr(np.ascontiguousarray(rays)), len(rays), _ptr(out))
     return out
+
+
+class threads:
+    """Context manager: the oracle's builds run multithreaded (the crate's `parallel` sections) inside it."""
+
+    def __init__(self, n):
+        self.n = n
+
+    def __enter__(self):
+        lib().orc_set_threads(self.n)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_threads(1)

@@ -284,3 +284,26 @@ def test_signed_zero_soup_exercises_early_exit(mutation, ratio, seq):
     np.testing.assert_array_equal(crate["min"], full["min"])
     np.testing.assert_array_equal(crate["max"], full["max"])
     assert O.bvh2_validate(crate, b["prim_indices"], aabbs, True, ordered=False)[0] == 0
+
+
+def test_multithreaded_build_equals_serial():
+    """The multithreaded oracle build (the CPU baseline's counterpart of the crate's rayon sections: leaf bounds
+    above 500K leaves, Morton codes above 100K, the parallel sort, the distance-1 rule above 300K clusters, the
+    chunked search with per-chunk caches from 4,000 clusters, parallel reinsertion searches) gives the serial
+    build's bytes."""
+    tris = np.concatenate([scenes.heightfield_scene(520, 2000, seed=5), scenes.thin_triangles(3000, seed=5)])
+    assert len(tris) > 500_000
+    aabbs = O.tri_aabbs(tris)
+    ref = O.ploc_build(aabbs, 14, 0)
+    ref1 = O.ploc_build(aabbs, 14, 3)  # distance-1 rule for the first three passes
+    with O.threads(8):
+        mt = O.ploc_build(aabbs, 14, 0)
+        mt1 = O.ploc_build(aabbs, 14, 3)
+    assert mt["nodes"].tobytes() == ref["nodes"].tobytes()
+    assert mt1["nodes"].tobytes() == ref1["nodes"].tobytes()
+    small = scenes.heightfield_scene(64, 200, seed=6)
+    b = O.ploc_build(O.tri_aabbs(small), 14, 0)
+    want = O.reinsertion(b["nodes"], 0.2, [1.0, 1.0 / 3.0])
+    with O.threads(8):
+        got = O.reinsertion(b["nodes"], 0.2, [1.0, 1.0 / 3.0])
+    assert got.tobytes() == want.tobytes()
