@@ -1,30 +1,26 @@
 """Benchmark of the obvhs hot path on one MI355X.
 
-Workload (heightfield scene, closest hit): a 724 x 724 grid over [0,1]^2 (2 triangles per cell, height = sum of 8
-seeded sines) plus 20,000 seeded 12-triangle boxes = 1,288,352 f32 triangles; PLOC BVH2 (search distance 14, no
-pre-splits) -> CWBVH8; 1920 x 1080 coherent primary rays plus one cosine-weighted bounce ray per primary hit.
+Headline workload (the largest single-GPU configuration, closest hit): the large scene -- a 2896 x 2896 heightfield
+(16,773,632 triangles) plus 500,000 seeded long thin triangles = 17,273,632 f32 triangles; spatial pre-splits capped
+at 1.3x the triangle count and two reinsertion passes (ratio 0.05, sequence [1, 1/3]) -> CWBVH8 (max 3 primitives
+per leaf); 4,194,304 incoherent closest-hit rays (uniform sphere origins, uniform directions).
 
-A step is one closest-hit trace of the whole ray batch (primary + bounce) against the device-resident CWBVH, with
-rays, triangles and hit buffers resident in HBM.  value = rays per step * steps / wall time of the timed steps.
-The PLOC -> CWBVH build is timed in the same run (device-resident triangles to device-resident CWBVH + primitive
-indices) and reported beside it as Mtris/s.  The roofline entry is for the traversal kernel (the dominant kernel),
-with the algorithmic bytes of every ray (40 B of ray/hit + 80 B per node test + 36 B per triangle test, counts taken
-from an instrumented run of the same kernel) divided by its event-timed duration.  The CPU baseline is the C++
-restatement of the crate's CPU path (oracle/), timed on this host's cores in this process.
+A step is one closest-hit trace of the 4M rays against the device-resident CWBVH, with rays, triangles and hit
+buffers resident in HBM.  value = rays per step * steps / wall time of the timed steps.  Before the timed steps the
+traversal's LDS stack depth and launch form (one lane per ray, or wave-owned ray rows with lanes refilled) are
+chosen by timing the step itself with each choice (none changes a hit).  The roofline entry is for the step's only
+kernel: algorithmic bytes per launch (40 B of ray and hit + 80 B per node test + 36 B per triangle test, counts
+from an instrumented launch of the same traversal, which visits the same nodes and triangles) divided by its
+HIP-event duration on the context stream; "traffic" is the HBM bytes per launch that rocprofv3's FETCH_SIZE and
+WRITE_SIZE counters read for the same launch (profiles/), scaled by their reading for a copy of known size.
 
-Before anything is timed, the context's tuning table is filled on the workload itself (obvhs_amd/csrc/tune.hip): the
-radix sort tile for the build's key count and the traversal LDS stack depth on the step's rays.  Neither changes a
-result (the sort order and the hits are identical for every choice); "tuning" reports the choices.  --no-tune keeps
-the defaults (8 keys per thread, 8 LDS entries); --sort-items / --lds-depth set a choice without measuring it (the
-profiling runs repeat the measured choice so that their kernels are the benchmark's).
-
-The "batched" object reports the batched form on 4,096 seeded icospheres (~20 M triangles, 1,024 rays each): one
-shared build and one traversal launch per step, next to the plain form (a build_cwbvh_from_tris call and a
-traversal launch per mesh) on the same inputs, with a check that both give identical hits.  Its meshes are built with
-the PLOC search distance the tuner picks per size class from tree costs, the smallest whose tree is within 1% of
-distance 14's (PlocSearchDistance.Auto); the time with distance 14 for every mesh is reported beside it.  The
-"large" object reports the 17.3 M-triangle scene built with pre-splits and reinsertion, traced with 4M incoherent
-closest-hit rays and 4M shadow rays.
+Beside the headline, in the same run: the large scene's build (device triangles -> device CWBVH + primitive indices,
+Mtris/s over the primitives after pre-splits) and any-hit shadow rays; the heightfield scene (1,288,352 triangles,
+1080p primary + bounce rays) with its build and a physics-style partial rebuild; the plain 65,536-triangle soup the
+crate's CPU path runs (build + 256 x 256 primary rays, GPU and CPU); the batched form on 4,096 icospheres; serving.
+The CPU baseline is the C++ restatement of the crate's CPU path (oracle/), timed on this host's cores in this
+process: traversal of the large scene's CWBVH with all its rays, and serial and multithreaded builds of the
+large-scene recipe at 1/16 scale.
 """
 import argparse
 import json
@@ -39,9 +35,11 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 PMC_COPY_BYTES = 256 << 20
-# HBM bytes per traversal launch from the FETCH_SIZE / WRITE_SIZE counter runs, scaled by the copy kernel's reading
-# (profiles/heightfield_traversal_hbm_traffic.json); null until those runs exist.
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "heightfield_traversal_hbm_traffic.json")
+# HBM bytes per traversal launch from the FETCH_SIZE / WRITE_SIZE counter runs (bench.py --pmc-probe under
+# rocprofv3 --pmc, then --traffic-from), scaled by the copy kernel's reading; null until those runs exist.
+TRAFFIC_FILES = {"large": os.path.join(REPO, "profiles", "large_17p3M_closest_hit_hbm_traffic.json"),
+                 "heightfield": os.path.join(REPO, "profiles", "heightfield_traversal_hbm_traffic.json")}
+TRAFFIC_LAUNCHES = {"large": ("closest",), "heightfield": ("primary", "bounce")}
 METRIC = "Mrays/s CWBVH closest-hit; Mtris/s PLOC build; p50/p99 batch latency"
 
 
@@ -49,20 +47,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_per_launch():
-    """Measured HBM bytes per closest-hit launch (mean of the primary and bounce launches), or None."""
+def traffic_per_launch(leg):
+    """Measured HBM bytes per traversal launch of a leg (mean over its launches), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILES[leg]) as f:
             t = json.load(f)
-        return int(round((t["primary"]["bytes"] + t["bounce"]["bytes"]) / 2))
+        return int(round(np.mean([t[k]["bytes"] for k in TRAFFIC_LAUNCHES[leg]])))
     except (OSError, KeyError, ValueError):
         return None
 
 
-def write_traffic(fetch_csv, write_csv):
+def write_traffic(leg, fetch_csv, write_csv):
     """Turns the two `--pmc-probe` counter runs (FETCH_SIZE, then WRITE_SIZE; rocprofv3 CSV, KiB per dispatch) into
-    TRAFFIC_FILE.  Each counter is scaled by what it reads for the copy of PMC_COPY_BYTES that the probe runs first
-    (that copy reads and writes exactly PMC_COPY_BYTES); the primary and bounce launches are the first two
+    the leg's traffic file.  Each counter is scaled by what it reads for the copy of PMC_COPY_BYTES that the probe
+    runs first (that copy reads and writes exactly PMC_COPY_BYTES); the leg's traversal launches are the first
     traversal dispatches after it."""
     import csv
 
@@ -78,6 +76,7 @@ def write_traffic(fetch_csv, write_csv):
                 rows[d] = (name, prev[1] + val)  # summed over agents/dimensions if split
         return sorted((d, n, v) for d, (n, v) in rows.items())
 
+    launches = TRAFFIC_LAUNCHES[leg]
     out = {"copy_bytes": PMC_COPY_BYTES}
     for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
         rows = per_dispatch(path, counter)
@@ -85,36 +84,127 @@ def write_traffic(fetch_csv, write_csv):
         if not copies:
             raise SystemExit(f"{path}: no copy dispatch")
         copy_d, copy_kib = max(copies, key=lambda t: t[1])
-        trav = [(d, v) for d, n, v in rows if d > copy_d and "k_traverse" in n][:2]
-        if len(trav) != 2:
-            raise SystemExit(f"{path}: expected two traversal dispatches after the copy")
+        trav = [(d, n, v) for d, n, v in rows if d > copy_d and "k_traverse" in n][:len(launches)]
+        if len(trav) != len(launches):
+            raise SystemExit(f"{path}: expected {len(launches)} traversal dispatches after the copy")
         scale = PMC_COPY_BYTES / (copy_kib * 1024.0)
-        out[key] = {"copy_kib": copy_kib, "scale": scale, "primary_kib": trav[0][1], "bounce_kib": trav[1][1]}
-    for launch in ("primary", "bounce"):
+        out[key] = {"copy_kib": copy_kib, "scale": scale}
+        for name, (_, kname, v) in zip(launches, trav):
+            out[key][f"{name}_kib"] = v
+            out[key][f"{name}_kernel"] = kname
+    for launch in launches:
         fb = out["fetch"][f"{launch}_kib"] * 1024.0 * out["fetch"]["scale"]
         wb = out["write"][f"{launch}_kib"] * 1024.0 * out["write"]["scale"]
         out[launch] = {"bytes": int(round(fb + wb)), "fetch_bytes": int(round(fb)), "write_bytes": int(round(wb))}
-    with open(TRAFFIC_FILE, "w") as f:
+    with open(TRAFFIC_FILES[leg], "w") as f:
         json.dump(out, f, indent=1)
         f.write("\n")
     print(json.dumps(out))
 
 
-def cpu_baseline(tris, rays_all, threads, sample_rays):
-    """Times the oracle's traversal (all `threads` host threads) and its serial build on this host."""
+def pmc_copy(ob, ctx):
+    """One device-to-device copy of PMC_COPY_BYTES (a blit kernel: the counters' scale) before the probed launches."""
+    lib = ob.native.lib()
+    src = ob.DeviceArray(ctx, np.uint8, PMC_COPY_BYTES)
+    dst = ob.DeviceArray(ctx, np.uint8, PMC_COPY_BYTES)
+    ob.native.check(lib.obvhs_memset_device(ctx.handle, src.ptr, 1, PMC_COPY_BYTES), ctx.handle)
+    ctx.synchronize()
+    ob.native.check(lib.obvhs_memcpy_dtod(ctx.handle, dst.ptr, src.ptr, PMC_COPY_BYTES), ctx.handle)
+    ctx.synchronize()
+
+
+def tune_step(ob, ctx, n_prims, step, reps=3):
+    """Chooses the traversal LDS stack depth (4, 8, 16) and launch form (one lane per ray / chunked) for the size
+    class of n_prims by timing the timed step itself with each (one warm-up, best of `reps`), and keeps the
+    fastest in the context's tuning table.  No choice changes a hit."""
+    cls = ob.size_class(n_prims)
+    ms_by = {}
+    for chunk in (0, 1):
+        for depth in (4, 8, 16):
+            t = ctx.tune_table()
+            t["trav_lds_depth"][cls], t["trav_chunk"][cls] = depth, chunk
+            ctx.set_tune_table(t)
+            step()
+            ms = []
+            for _ in range(reps):
+                ctx.timer_start()
+                step()
+                ms.append(ctx.timer_stop())
+            ms_by[(depth, chunk)] = min(ms)
+    best = min(ms_by, key=ms_by.get)
+    t = ctx.tune_table()
+    t["trav_lds_depth"][cls], t["trav_chunk"][cls] = best
+    ctx.set_tune_table(t)
+    return {"traversal_lds_depth": best[0], "traversal_chunked": best[1], "size_class": cls,
+            "step_ms_by_lds_depth_and_chunked": {f"{d}/{c}": round(v, 4) for (d, c), v in ms_by.items()}}
+
+
+def kernel_name(ctx, ob, n_prims, mode="closest"):
+    t = ctx.tune_table()
+    cls = ob.size_class(n_prims)
+    form = "k_traverse_chunk" if t["trav_chunk"][cls] else "k_traverse"
+    return f"{form}<{mode}, lds {t['trav_lds_depth'][cls]}>"
+
+
+def roofline(algo_bytes_per_launch, kernel_ms, traffic, kernel, **extra):
+    """Roofline object of a traversal kernel: achieved algorithmic GB/s against the HBM peak; with the measured
+    traffic, what the kernel actually moves (the limiter: HBM traffic far below peak = latency-bound)."""
+    kernel_s = kernel_ms * 1e-3
+    achieved = algo_bytes_per_launch / kernel_s / 1e9
+    r = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+         "avg_kernel_ms": round(kernel_ms, 4), "algorithmic_bytes_per_launch": int(algo_bytes_per_launch)}
+    if traffic:
+        hbm = traffic / kernel_s / 1e9
+        r["measured_hbm_gbs"] = round(hbm, 1)
+        r["limiter"] = (f"latency: the counted HBM traffic moves {hbm:.0f} GB/s ({hbm / HBM_PEAK_GBS:.2f} of peak); "
+                        "node and triangle reads mostly hit L2/MALL" if hbm < 0.5 * HBM_PEAK_GBS
+                        else f"HBM bandwidth: {hbm:.0f} GB/s counted")
+    else:
+        r["limiter"] = "unmeasured (no counter run for this launch yet)"
+    r.update(extra)
+    return r
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on, at most 16 (one GPU's share of the
+    host on the benchmark machines)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def oracle():
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib as O
+    return O
+
+
+def cpu_build_baseline(args, threads):
+    """Serial and multithreaded oracle builds (the crate's default features and its `parallel` sections) of the
+    large-scene recipe at 1/16 scale: a 724 x 724 heightfield + 31,250 thin triangles, pre-splits capped at 1.3n,
+    reinsertion 0.05 x [1, 1/3] -> CWBVH."""
+    from obvhs_amd import scenes
+    O = oracle()
+    grid, thin = max(2, args.large_grid // 4), max(1, args.large_thin // 16)
+    tris = np.concatenate([scenes.heightfield_scene(grid, 0, 0), scenes.thin_triangles(thin, 0)])
+    kw = dict(pre_split=True, reinsertion_batch_ratio=0.05, ratio_sequence=[1.0, 1.0 / 3.0],
+              split_max_total=int(len(tris) * 1.3))
     t0 = time.perf_counter()
-    b, cw, prims = O.build_cwbvh(tris, 14, 0, 3)
-    build_s = time.perf_counter() - t0
-    bvh_tris = tris[prims]
-    idx = np.linspace(0, len(rays_all) - 1, num=min(sample_rays, len(rays_all))).astype(np.int64)
-    sample = np.ascontiguousarray(rays_all[idx])
-    O.traverse(cw, bvh_tris, sample[:4096], 0, threads=threads)  # warm-up
-    t0 = time.perf_counter()
-    O.traverse(cw, bvh_tris, sample, 0, threads=threads)
-    trace_s = time.perf_counter() - t0
-    return dict(build_s=build_s, trace_s=trace_s, n_rays=len(sample), n_tris=len(tris))
+    b, cw, _ = O.build_cwbvh(tris, 14, 0, 3, **kw)
+    serial_s = time.perf_counter() - t0
+    with O.threads(threads):
+        t0 = time.perf_counter()
+        b2, cw2, _ = O.build_cwbvh(tris, 14, 0, 3, **kw)
+        mt_s = time.perf_counter() - t0
+    leaves = len(b["prim_indices"])
+    return {"sample": f"large-scene recipe at 1/16 scale: heightfield {grid}x{grid}x2 + {thin} thin = {len(tris)} "
+                      f"tris ({leaves} leaves after pre-splits), pre-splits 1.3n, reinsertion 0.05 x [1, 1/3]",
+            "serial_s": round(serial_s, 3), "serial_mtris_per_s": round(leaves / serial_s / 1e6, 3),
+            "mt_s": round(mt_s, 3), "mt_mtris_per_s": round(leaves / mt_s / 1e6, 3), "mt_threads": threads,
+            "mt_equals_serial": bool(cw2.tobytes() == cw.tobytes())}
 
 
 def autotune_batch(ob, ctx, tris, offsets, rays, ray_mesh):
@@ -347,9 +437,9 @@ def bench_rebuild(args, ob, ctx, tris, d_tris, fresh_build_ms):
 
 
 def bench_large(args, ob, ctx):
-    """Large scene with the quality passes: a 2896 x 2896 heightfield (16,773,632 triangles) plus 500,000 long thin
-    triangles; spatial pre-splits capped at 1.3x the triangle count and reinsertion at ratio 0.05 with the ratio
-    sequence [1, 1/3] (two optimizer iterations); 4M incoherent closest-hit rays and 4M shadow (any-hit) rays."""
+    """The headline: the large scene (heightfield 2896 x 2896 + 500,000 thin triangles, pre-splits capped at 1.3n,
+    reinsertion 0.05 x [1, 1/3]) traced with 4M incoherent closest-hit rays per step; its build and 4M any-hit
+    shadow rays beside it; the oracle's traversal of the same CWBVH on the host cores."""
     from obvhs_amd import scenes
     log("generating the large scene")
     tris, rays, shadow, light = scenes.large_workload(args.large_grid, args.large_thin, args.large_rays,
@@ -361,8 +451,12 @@ def bench_large(args, ob, ctx):
                                reinsertion_batch_ratio=0.05, max_prims_per_leaf=3)
     cap = int(n * 1.3)
     seq = [1.0, 1.0 / 3.0]
+    tuning = {}
     log(f"large build over {n} triangles")
     cw = ob.build_cwbvh_from_tris(ctx, d_tris, params, split_max_total=cap, ratio_sequence=seq)
+    leaves = int(cw.stats.prim_count)
+    if not args.no_tune and not args.sort_items:  # the radix sort tile for this key count (same order for all)
+        tuning["sort_items"] = ob.tune_sort(ctx, leaves)
     ms = []
     for _ in range(max(1, min(args.build_reps, 3))):
         ctx.timer_start()
@@ -370,11 +464,12 @@ def bench_large(args, ob, ctx):
         ms.append(ctx.timer_stop())
     build_ms = float(np.median(ms))
     # the same build without the quality passes, for reference
-    plain_cw = ob.build_cwbvh_from_tris(ctx, d_tris, ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3))
+    plain_params = ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3)
+    plain_cw = ob.build_cwbvh_from_tris(ctx, d_tris, plain_params)
     ctx.timer_start()
-    ob.build_cwbvh_from_tris(ctx, d_tris, ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3),
-                             reuse=plain_cw)
+    ob.build_cwbvh_from_tris(ctx, d_tris, plain_params, reuse=plain_cw)
     plain_ms = ctx.timer_stop()
+    plain_cw.close()
     d_bvh_tris = cw.reorder_triangles(d_tris)
     lib = ob.native.lib()
     nr, ns = len(rays), len(shadow)
@@ -389,33 +484,249 @@ def bench_large(args, ob, ctx):
         ob.native.check(lib.obvhs_cwbvh_ray_traverse_miss(ctx.handle, cw.handle, d_bvh_tris.ptr, d_shadow.ptr, ns,
                                                           miss.ptr), ctx.handle)
 
-    tuned = None
     if not args.no_tune and not args.lds_depth and args.trav_chunk < 0:
-        # this scene's rays are incoherent: the depth and launch form measured on its closest-hit rays
-        depth = ob.tune_traversal(ctx, cw, d_rays, d_bvh_tris)
-        tuned = {"traversal_lds_depth": depth, "traversal_chunked": ctx.tune_table()["trav_chunk"][ob.size_class(n)]}
-
-    def timed(fn):
-        for _ in range(args.warmup):
-            fn()
+        tuning["closest"] = tune_step(ob, ctx, leaves, closest)
+    if args.pmc_probe == "large":
+        pmc_copy(ob, ctx)
+        closest()
         ctx.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            fn()
-        ctx.synchronize()
-        return (time.perf_counter() - t0) / args.steps
+        print(json.dumps({"pmc_probe": "large", "copy_bytes": PMC_COPY_BYTES, "closest_rays": nr,
+                          "kernel": kernel_name(ctx, ob, leaves)}), flush=True)
+        return None
+    # per-ray node / triangle test counts for the byte model (instrumented launch, not timed)
+    _, nv, tv = cw.ray_traverse(d_rays, d_bvh_tris, counted=True)
+    node_tests, tri_tests = int(nv.sum()), int(tv.sum())
+    algo_bytes = 40 * nr + 80 * node_tests + 36 * tri_tests
 
-    tc, ta = timed(closest), timed(anyhit)
+    for _ in range(args.warmup):
+        closest()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        closest()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    # kernel durations: HIP events around each single-kernel launch on the context stream
+    kms = []
+    for _ in range(max(3, min(args.steps, 10))):
+        ctx.timer_start()
+        closest()
+        kms.append(ctx.timer_stop())
+    kernel_ms = float(np.mean(kms))
+    closest_kernel = kernel_name(ctx, ob, leaves)
+
+    for _ in range(args.warmup):
+        anyhit()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        anyhit()
+    ctx.synchronize()
+    ta = (time.perf_counter() - t0) / args.steps
     h, m = hits.to_host(), miss.to_host()
+
+    # the watertight mode (an extra primitive test, reported separately): the same rays and launch choices
+    wt_hits = ob.DeviceArray(ctx, ob.RAY_HIT, nr)
+
+    def closest_wt():
+        ob.native.check(lib.obvhs_cwbvh_ray_traverse_watertight(ctx.handle, cw.handle, d_bvh_tris.ptr, d_rays.ptr,
+                                                                nr, wt_hits.ptr), ctx.handle)
+
+    for _ in range(args.warmup):
+        closest_wt()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        closest_wt()
+    ctx.synchronize()
+    tw = (time.perf_counter() - t0) / args.steps
+    hw = wt_hits.to_host()
     st = cw.stats
-    return {"workload": f"heightfield {args.large_grid}x{args.large_grid}x2 + {args.large_thin} thin = {n} tris; "
-                        f"pre-splits capped at 1.3n, reinsertion 0.05 x [1, 1/3]; {nr} closest-hit + {ns} any-hit rays",
-            "leaves_after_splits": int(st.prim_count), "build_ms": round(build_ms, 3),
-            "build_mtris_per_s": round(n / build_ms / 1e3, 2), "build_ms_without_quality_passes": round(plain_ms, 3),
-            "closest_mrays_per_s": round(nr / tc / 1e6, 2), "anyhit_mrays_per_s": round(ns / ta / 1e6, 2),
-            "closest_hit_fraction": round(float(np.isfinite(h["t"]).mean()), 4),
-            "shadowed_fraction": round(float((m == 0).mean()), 4), "cwbvh_nodes": int(st.cwbvh_node_count),
-            "tuning": tuned}
+    result = {
+        "value": nr * args.steps / wall / 1e6, "wall": wall,
+        "workload": f"large scene: heightfield {args.large_grid}x{args.large_grid}x2 + {args.large_thin} thin = {n} "
+                    f"tris ({leaves} leaves after pre-splits capped at 1.3n), reinsertion 0.05 x [1, 1/3]; "
+                    f"{nr} incoherent closest-hit rays per step",
+        "roofline": roofline(algo_bytes, kernel_ms, traffic_per_launch("large"), closest_kernel,
+                             node_tests=node_tests, tri_tests=tri_tests,
+                             bytes_model="40 B per ray + 80 B per node test + 36 B per triangle test"),
+        "large": {"triangles": n, "leaves_after_splits": leaves, "build_ms": round(build_ms, 3),
+                  "build_mtris_per_s": round(leaves / build_ms / 1e3, 2),
+                  "build_mtris_per_s_input_triangles": round(n / build_ms / 1e3, 2),
+                  "build_ms_without_quality_passes": round(plain_ms, 3),
+                  "closest_mrays_per_s": round(nr * args.steps / wall / 1e6, 2),
+                  "anyhit_mrays_per_s": round(ns / ta / 1e6, 2), "anyhit_rays": ns,
+                  "closest_watertight_mrays_per_s": round(nr / tw / 1e6, 2),
+                  "watertight_hits_differing_from_crate_test": int((hw["primitive_id"] != h["primitive_id"]).sum()),
+                  "closest_hit_fraction": round(float(np.isfinite(h["t"]).mean()), 4),
+                  "shadowed_fraction": round(float((m == 0).mean()), 4), "cwbvh_nodes": int(st.cwbvh_node_count),
+                  "tuning": tuning},
+    }
+    if not args.no_cpu_baseline:
+        threads = cpu_threads()
+        log(f"cpu baseline: oracle traversal of the large CWBVH, {threads} threads")
+        O = oracle()
+        cw_host, bvh_tris_host = cw.nodes, d_bvh_tris.to_host()
+        sample = rays[:: max(1, args.cpu_ray_stride)]
+        O.traverse(cw_host, bvh_tris_host, sample[:4096], 0, threads=threads)  # warm-up
+        t0 = time.perf_counter()
+        cpu_hits = O.traverse(cw_host, bvh_tris_host, sample, 0, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        gpu_same = bool(cpu_hits.tobytes() == h[:: max(1, args.cpu_ray_stride)].tobytes())
+        result["cpu_trace"] = {"value": len(sample) / cpu_s / 1e6, "threads": threads, "rays": len(sample),
+                               "seconds": cpu_s, "hits_equal_gpu": gpu_same}
+    cw.close()
+    return result
+
+
+def bench_heightfield(args, ob, ctx):
+    """The heightfield scene: 724 x 724 grid + 20,000 boxes = 1,288,352 triangles, PLOC r=14 -> CWBVH; a step is the
+    1080p primary launch then the bounce launch (closest hit); the build timed beside it."""
+    from obvhs_amd import scenes
+    log("generating the heightfield scene")
+    tris, prim_rays = scenes.heightfield_workload(args.grid, args.boxes, args.width, args.height, seed=0)
+    n_tris = len(tris)
+    d_tris = ctx.upload(tris)
+    params = ob.BvhBuildParams(ploc_search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3)
+    tuning = {}
+    if not args.no_tune and not args.sort_items:
+        tuning["sort_items"] = ob.tune_sort(ctx, n_tris)
+    log(f"building CWBVH over {n_tris} triangles")
+    cw = ob.build_cwbvh_from_tris(ctx, d_tris, params)
+    ctx.synchronize()
+    build_ms = []
+    for _ in range(max(1, args.build_reps)):
+        ctx.timer_start()
+        ob.build_cwbvh_from_tris(ctx, d_tris, params, reuse=cw)
+        build_ms.append(ctx.timer_stop())
+    stats = cw.stats
+    build_med = float(np.median(build_ms))
+    d_bvh_tris = cw.reorder_triangles(d_tris)
+    bvh_tris_host = d_bvh_tris.to_host()
+    d_prim = ctx.upload(prim_rays)
+    hits = ob.DeviceArray(ctx, ob.RAY_HIT, len(prim_rays))
+    lib = ob.native.lib()
+
+    def trace(d_rays, n, out):
+        ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, cw.handle, d_bvh_tris.ptr, d_rays.ptr, n, out.ptr),
+                        ctx.handle)
+
+    trace(d_prim, len(prim_rays), hits)
+    prim_hits = hits.to_host()
+    bounce = scenes.bounce_rays(prim_rays, prim_hits, bvh_tris_host, seed=1)
+    d_bounce = ctx.upload(bounce)
+    bhits = ob.DeviceArray(ctx, ob.RAY_HIT, max(1, len(bounce)))
+    n_rays = len(prim_rays) + len(bounce)
+    log(f"{len(prim_rays)} primary + {len(bounce)} bounce rays")
+
+    def step():
+        trace(d_prim, len(prim_rays), hits)
+        trace(d_bounce, len(bounce), bhits)
+
+    if not args.no_tune and not args.lds_depth and args.trav_chunk < 0:
+        tuning["step"] = tune_step(ob, ctx, n_tris, step)
+    if args.pmc_probe == "heightfield":
+        pmc_copy(ob, ctx)
+        step()
+        ctx.synchronize()
+        print(json.dumps({"pmc_probe": "heightfield", "copy_bytes": PMC_COPY_BYTES, "primary_rays": len(prim_rays),
+                          "bounce_rays": len(bounce), "kernel": kernel_name(ctx, ob, n_tris)}), flush=True)
+        return None
+    _, nv_p, tv_p = cw.ray_traverse(d_prim, d_bvh_tris, counted=True)
+    _, nv_b, tv_b = cw.ray_traverse(d_bounce, d_bvh_tris, counted=True)
+    node_tests = int(nv_p.sum()) + int(nv_b.sum())
+    tri_tests = int(tv_p.sum()) + int(tv_b.sum())
+    algo_bytes = 40 * n_rays + 80 * node_tests + 36 * tri_tests
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    kp, kb = [], []
+    for _ in range(max(3, min(args.steps, 10))):
+        ctx.timer_start()
+        trace(d_prim, len(prim_rays), hits)
+        kp.append(ctx.timer_stop())
+        ctx.timer_start()
+        trace(d_bounce, len(bounce), bhits)
+        kb.append(ctx.timer_stop())
+    kernel_ms = float(np.mean(kp) + np.mean(kb)) / 2.0
+    out = {"workload": f"heightfield {args.grid}x{args.grid}x2 + {args.boxes} boxes = {n_tris} tris; "
+                       f"{len(prim_rays)} primary + {len(bounce)} bounce closest-hit rays per step",
+           "closest_mrays_per_s": round(n_rays * args.steps / wall / 1e6, 2),
+           "ms_per_step": round(wall / args.steps * 1e3, 4),
+           "roofline": roofline(algo_bytes / 2.0, kernel_ms, traffic_per_launch("heightfield"),
+                                kernel_name(ctx, ob, n_tris),
+                                avg_kernel_ms_by_launch={"primary": round(float(np.mean(kp)), 4),
+                                                         "bounce": round(float(np.mean(kb)), 4)},
+                                node_tests=node_tests, tri_tests=tri_tests),
+           "build": {"ms": round(build_med, 3), "mtris_per_s": round(n_tris / build_med / 1e3, 2),
+                     "all_ms": [round(x, 3) for x in build_ms], "ploc_passes": int(stats.ploc_iterations),
+                     "ploc_cluster_sum": int(stats.ploc_cluster_sum), "cwbvh_nodes": int(stats.cwbvh_node_count)},
+           "tuning": tuning}
+    if args.rebuild_frac > 0:
+        out["rebuild"] = bench_rebuild(args, ob, ctx, tris, d_tris, build_med)
+    cw.close()
+    return out
+
+
+def bench_plain(args, ob, ctx):
+    """The plain 65,536-triangle soup the crate's CPU path runs: PLOC r=14 -> CWBVH (max 3 per leaf), 256 x 256
+    pinhole rays, closest hit; GPU and the oracle (serial build, serial and multithreaded traversal)."""
+    from obvhs_amd import scenes
+    O = oracle()
+    tris, rays = scenes.plain_workload()
+    params = ob.BvhBuildParams(ploc_search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3)
+    d_tris, d_rays = ctx.upload(tris), ctx.upload(rays)
+    cw = ob.build_cwbvh_from_tris(ctx, d_tris, params)
+    ms = []
+    for _ in range(max(1, args.build_reps)):
+        ctx.timer_start()
+        ob.build_cwbvh_from_tris(ctx, d_tris, params, reuse=cw)
+        ms.append(ctx.timer_stop())
+    build_ms = float(np.median(ms))
+    d_bvh_tris = cw.reorder_triangles(d_tris)
+    hits = ob.DeviceArray(ctx, ob.RAY_HIT, len(rays))
+    lib = ob.native.lib()
+
+    def step():
+        ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, cw.handle, d_bvh_tris.ptr, d_rays.ptr, len(rays),
+                                                     hits.ptr), ctx.handle)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    trace_s = (time.perf_counter() - t0) / args.steps
+    gpu_hits = hits.to_host()
+    out = {"workload": f"soup {len(tris)} tris (PLOC r=14, max 3 per leaf); {len(rays)} pinhole closest-hit rays",
+           "gpu_build_ms": round(build_ms, 4), "gpu_build_mtris_per_s": round(len(tris) / build_ms / 1e3, 2),
+           "gpu_trace_mrays_per_s": round(len(rays) / trace_s / 1e6, 2)}
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        _, rcw, rprims = O.build_cwbvh(tris, 14, 0, 3)
+        cpu_build_s = time.perf_counter() - t0
+        bt = tris[rprims]
+        t0 = time.perf_counter()
+        cpu_hits = O.traverse(rcw, bt, rays, 0)
+        cpu_trace_s = time.perf_counter() - t0
+        threads = cpu_threads()
+        t0 = time.perf_counter()
+        O.traverse(rcw, bt, rays, 0, threads=threads)
+        cpu_mt_s = time.perf_counter() - t0
+        out.update({"cpu_build_mtris_per_s_serial": round(len(tris) / cpu_build_s / 1e6, 3),
+                    "cpu_trace_mrays_per_s_serial": round(len(rays) / cpu_trace_s / 1e6, 3),
+                    "cpu_trace_mrays_per_s_mt": round(len(rays) / cpu_mt_s / 1e6, 3), "cpu_threads": threads,
+                    "cwbvh_equal_oracle": bool(cw.nodes.tobytes() == rcw.tobytes()),
+                    "hits_equal_oracle": bool(gpu_hits.tobytes() == cpu_hits.tobytes())})
+    cw.close()
+    return out
 
 
 def bench_serve(args, ob, ctx):
@@ -504,6 +815,13 @@ def bench_serve(args, ob, ctx):
     log("serving: burst replay (capacity)")
     burst_reqs = make_requests(args.serve_requests, 0.0)
     burst = summarize(burst_reqs, server.replay(burst_reqs))
+    # rate sweep: the same request list with its Poisson arrival times scaled to each rate
+    rates = [float(x) for x in args.serve_sweep.split(",")] if args.serve_sweep else []
+    sweep_served = {}
+    for rate in rates:
+        log(f"serving: sweep at {rate:g} requests/s")
+        scaled = [dict(q, arrival_ns=int(q["arrival_ns"] * args.serve_rate / rate)) for q in reqs]
+        sweep_served[rate] = summarize(scaled, server.replay(scaled))
     st = server.stats()
     server.close()
     # plain form: the same requests one at a time through the single-request API (traces: one launch and a
@@ -533,23 +851,50 @@ def bench_serve(args, ob, ctx):
         service.append(time.perf_counter() - t0)
     traces = [(q, sv) for q, sv in zip(reqs, service) if q["kind"] == ob.api.REQ_TRACE_CLOSEST]
     plain_rays = sum(q["n"] for q, _ in traces)
-    # the plain form's latency under the same arrivals: one request at a time, first come first served
-    free_at, plain_lat = 0.0, []
-    for q, sv in zip(reqs, service):
-        a = q["arrival_ns"] * 1e-9
-        start = max(a, free_at)
-        free_at = start + sv
-        if q["kind"] == ob.api.REQ_TRACE_CLOSEST:
-            plain_lat.append((free_at - a) * 1e3)
+
+    def plain_at(scale):
+        """The plain form under the request list's arrivals scaled by `scale` (one request at a time, first come
+        first served, each taking its measured service time): trace p50 / p99 and rays per second of the span."""
+        free_at, lat, first = 0.0, [], None
+        for q, sv in zip(reqs, service):
+            a = q["arrival_ns"] * 1e-9 * scale
+            first = a if first is None else first
+            start = max(a, free_at)
+            free_at = start + sv
+            if q["kind"] == ob.api.REQ_TRACE_CLOSEST:
+                lat.append((free_at - a) * 1e3)
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mrays_per_s": round(plain_rays / (free_at - first) / 1e6, 2)}
+
+    same = plain_at(1.0)
     plain = {"mrays_per_s": round(plain_rays / sum(service) / 1e6, 2),
              "trace_only_mrays_per_s": round(plain_rays / sum(sv for _, sv in traces) / 1e6, 2),
-             "p50_ms_same_arrivals": round(float(np.percentile(plain_lat, 50)), 3),
-             "p99_ms_same_arrivals": round(float(np.percentile(plain_lat, 99)), 3)}
-    return {"workload": f"{len(sc)} resident scenes ({sum(len(t) for t, _ in sc)} tris, 100K-4M each); "
-                        f"{args.serve_requests} requests, 90% ray batches of 1K-1M rays, 10% refit/rebuild",
-            "poisson_rate_per_s": args.serve_rate, "served": poisson, "served_burst": burst, "plain": plain,
-            "served_vs_plain_mrays": round(burst["mrays_per_s"] / plain["mrays_per_s"], 2),
-            "launches": st["launches"], "resident_gb": round(st["resident_bytes"] / 2 ** 30, 2)}
+             "p50_ms_same_arrivals": same["p50_ms"], "p99_ms_same_arrivals": same["p99_ms"]}
+    out = {"workload": f"{len(sc)} resident scenes ({sum(len(t) for t, _ in sc)} tris, 100K-4M each); "
+                       f"{args.serve_requests} requests, 90% ray batches of 1K-1M rays, 10% refit/rebuild",
+           "poisson_rate_per_s": args.serve_rate, "served": poisson, "served_burst": burst, "plain": plain,
+           "served_burst_vs_plain_mrays": round(burst["mrays_per_s"] / plain["mrays_per_s"], 2),
+           "launches": st["launches"], "resident_gb": round(st["resident_bytes"] / 2 ** 30, 2)}
+    if rates:
+        slo = args.serve_slo_ms
+        rows = []
+        for rate in rates:
+            pl = plain_at(args.serve_rate / rate)
+            sv = sweep_served[rate]
+            rows.append({"rate_per_s": rate, "served_p50_ms": sv["p50_ms"], "served_p99_ms": sv["p99_ms"],
+                         "served_mrays_per_s": sv["mrays_per_s"],
+                         "served_requests_per_launch": sv["mean_requests_per_launch"],
+                         "plain_p50_ms": pl["p50_ms"], "plain_p99_ms": pl["p99_ms"],
+                         "plain_mrays_per_s": pl["mrays_per_s"]})
+        best_s = max((r["served_mrays_per_s"] for r in rows if r["served_p99_ms"] < slo), default=0.0)
+        best_p = max((r["plain_mrays_per_s"] for r in rows if r["plain_p99_ms"] < slo), default=0.0)
+        out["sweep"] = {"slo_p99_ms": slo, "rows": rows,
+                        "served_best_mrays_per_s_at_slo": best_s, "plain_best_mrays_per_s_at_slo": best_p,
+                        "served_vs_plain_at_slo": round(best_s / best_p, 2) if best_p > 0 else None,
+                        "note": "the same request list at each rate (Poisson arrival times scaled); plain = one "
+                                "request at a time, first come first served, with each request's measured service "
+                                "time"}
+    return out
 
 
 class _Slice:
@@ -580,20 +925,27 @@ def main():
     ap.add_argument("--trav-chunk", type=int, default=-1, choices=(-1, 0, 1),
                     help="set chunked single-BVH traversal launches (0 or 1) for every size class instead of "
                          "measuring it (-1)")
-    ap.add_argument("--pmc-probe", action="store_true", help="one copy kernel + one traversal per batch, then exit")
-    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
-                    help="write the HBM traffic file from two --pmc-probe counter runs, then exit")
-    ap.add_argument("--cpu-sample-rays", type=int, default=1 << 20)
+    ap.add_argument("--pmc-probe", choices=("large", "heightfield"), default=None,
+                    help="one copy kernel, then the leg's traversal launches of one step, then exit (counter runs)")
+    ap.add_argument("--traffic-from", nargs=3, metavar=("LEG", "FETCH_CSV", "WRITE_CSV"),
+                    help="write a leg's HBM traffic file from its two --pmc-probe counter runs, then exit")
+    ap.add_argument("--cpu-ray-stride", type=int, default=1,
+                    help="the CPU baseline traces every k-th ray of the large scene (1: all of them)")
     ap.add_argument("--batch-meshes", type=int, default=4096, help="0 skips the batched leg")
     ap.add_argument("--batch-rays-per-mesh", type=int, default=1024)
     ap.add_argument("--batch-no-plain", action="store_true", help="skip the per-mesh comparison of the batched leg")
     ap.add_argument("--rebuild-frac", type=float, default=0.01, help="0 skips the partial-rebuild leg")
-    ap.add_argument("--large-grid", type=int, default=2896, help="0 skips the large-scene leg")
+    ap.add_argument("--large-grid", type=int, default=2896)
     ap.add_argument("--large-thin", type=int, default=500000)
     ap.add_argument("--large-rays", type=int, default=1 << 22)
+    ap.add_argument("--no-heightfield", action="store_true", help="skip the heightfield leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain 65,536-triangle leg")
     ap.add_argument("--serve-scenes", type=int, default=64, help="0 skips the serving leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-sweep", default="250,500,1000,1500,2000,3000,4000,6000",
+                    help="request rates of the serving sweep (empty: no sweep)")
+    ap.add_argument("--serve-slo-ms", type=float, default=5.0)
     ap.add_argument("--serve-inflight", type=int, default=2)
     ap.add_argument("--serve-update-streams", type=int, default=3)
     args = ap.parse_args()
@@ -606,157 +958,68 @@ def main():
         sys.exit(2)
 
     import obvhs_amd as ob
-    from obvhs_amd import scenes
 
-    log("generating scene")
-    tris, prim_rays = scenes.heightfield_workload(args.grid, args.boxes, args.width, args.height, seed=0)
-    n_tris = len(tris)
     ctx = ob.Context(0)
-    d_tris = ctx.upload(tris)
-    params = ob.BvhBuildParams(ploc_search_distance=14, search_depth_threshold=0, max_prims_per_leaf=3)
-
-    tuning = {}
+    fixed = {}
     if args.sort_items or args.lds_depth or args.trav_chunk >= 0:  # fixed entries (profiling runs repeat a choice)
         t = ctx.tune_table()
         if args.trav_chunk >= 0:
             t["trav_chunk"] = [args.trav_chunk] * 4
-            tuning["traversal_chunked"] = args.trav_chunk
+            fixed["traversal_chunked"] = args.trav_chunk
         if args.sort_items:
             t["sort_items"] = [args.sort_items] * 4
-            tuning["sort_items"] = args.sort_items
+            fixed["sort_items"] = args.sort_items
         if args.lds_depth:
             t["trav_lds_depth"] = [args.lds_depth] * 4
-            tuning["traversal_lds_depth"] = args.lds_depth
+            fixed["traversal_lds_depth"] = args.lds_depth
         ctx.set_tune_table(t)
-    if not args.no_tune and not args.sort_items:  # the radix sort tile for this key count (the order is the same
-        tuning["sort_items"] = ob.tune_sort(ctx, n_tris)  # for every tile)
-    log(f"building CWBVH over {n_tris} triangles")
-    cw = ob.build_cwbvh_from_tris(ctx, d_tris, params)
-    ctx.synchronize()
-    # Build timing (device-resident input -> device-resident CWBVH + primitive indices).
-    build_ms = []
-    for _ in range(max(1, args.build_reps)):
-        ctx.timer_start()
-        ob.build_cwbvh_from_tris(ctx, d_tris, params, reuse=cw)
-        build_ms.append(ctx.timer_stop())
-    stats = cw.stats
-    build_med = float(np.median(build_ms))
-
-    d_bvh_tris = cw.reorder_triangles(d_tris)
-    bvh_tris_host = d_bvh_tris.to_host()
-    d_prim = ctx.upload(prim_rays)
-    hits = ob.DeviceArray(ctx, ob.RAY_HIT, len(prim_rays))
-    lib = ob.native.lib()
-
-    def trace(d_rays, n, out):
-        ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, cw.handle, d_bvh_tris.ptr, d_rays.ptr, n, out.ptr),
-                        ctx.handle)
-
-    trace(d_prim, len(prim_rays), hits)
-    prim_hits = hits.to_host()
-    bounce = scenes.bounce_rays(prim_rays, prim_hits, bvh_tris_host, seed=1)
-    d_bounce = ctx.upload(bounce)
-    bhits = ob.DeviceArray(ctx, ob.RAY_HIT, max(1, len(bounce)))
-    n_rays = len(prim_rays) + len(bounce)
-    log(f"{len(prim_rays)} primary + {len(bounce)} bounce rays")
-    if not args.no_tune and not args.lds_depth and args.trav_chunk < 0:
-        # the traversal LDS stack depth and launch form on this workload's rays, primary and bounce together (the
-        # hits are the same for every choice)
-        tuning["traversal_lds_depth"] = ob.tune_traversal(ctx, cw, np.concatenate([prim_rays, bounce]), d_bvh_tris)
-        tuning["traversal_chunked"] = ctx.tune_table()["trav_chunk"][ob.size_class(n_tris)]
-
-    if args.pmc_probe:
-        # Counter runs (rocprofv3 --pmc): one device-to-device copy of known size (a blit kernel: the counters'
-        # scale), then one primary and one bounce traversal launch.
-        src = ob.DeviceArray(ctx, np.uint8, PMC_COPY_BYTES)
-        dst = ob.DeviceArray(ctx, np.uint8, PMC_COPY_BYTES)
-        ob.native.check(lib.obvhs_memset_device(ctx.handle, src.ptr, 1, PMC_COPY_BYTES), ctx.handle)
-        ctx.synchronize()
-        ob.native.check(lib.obvhs_memcpy_dtod(ctx.handle, dst.ptr, src.ptr, PMC_COPY_BYTES), ctx.handle)
-        ctx.synchronize()
-        trace(d_prim, len(prim_rays), hits)
-        trace(d_bounce, len(bounce), bhits)
-        ctx.synchronize()
-        print(json.dumps({"pmc_probe": True, "copy_bytes": PMC_COPY_BYTES, "primary_rays": len(prim_rays),
-                          "bounce_rays": len(bounce)}), flush=True)
+    if args.pmc_probe == "heightfield":
+        bench_heightfield(args, ob, ctx)
         return
 
-    # Per-ray node/triangle test counts for the byte model (instrumented launch, not timed).
-    _, nv_p, tv_p = cw.ray_traverse(d_prim, d_bvh_tris, counted=True)
-    _, nv_b, tv_b = cw.ray_traverse(d_bounce, d_bvh_tris, counted=True)
-    node_tests = int(nv_p.sum()) + int(nv_b.sum())
-    tri_tests = int(tv_p.sum()) + int(tv_b.sum())
-    algo_bytes = 40 * n_rays + 80 * node_tests + 36 * tri_tests
-
-    def step():
-        trace(d_prim, len(prim_rays), hits)
-        trace(d_bounce, len(bounce), bhits)
-
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-
-    # Kernel durations (events around each single-kernel launch on the context stream).
-    kp, kb = [], []
-    for _ in range(max(3, min(args.steps, 10))):
-        ctx.timer_start()
-        trace(d_prim, len(prim_rays), hits)
-        kp.append(ctx.timer_stop())
-        ctx.timer_start()
-        trace(d_bounce, len(bounce), bhits)
-        kb.append(ctx.timer_stop())
-    kernel_s = (np.mean(kp) + np.mean(kb)) * 1e-3
-    achieved_gbs = algo_bytes / kernel_s / 1e9
-
-    value = n_rays * args.steps / wall / 1e6
+    head = bench_large(args, ob, ctx)
+    if head is None:  # counter probe
+        return
     out = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(head["value"], 3),
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(head["wall"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded heightfield + boxes, pinhole primary + cosine bounce rays)",
-        "config": {"workload": f"heightfield {args.grid}x{args.grid}x2 + {args.boxes} boxes = {n_tris} tris; "
-                               f"{len(prim_rays)} primary + {len(bounce)} bounce closest-hit rays",
-                   "ploc_search_distance": 14, "max_prims_per_leaf": 3},
-        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic_per_launch(),
-                     "kernel": "k_traverse<closest>", "avg_kernel_ms": {"primary": round(float(np.mean(kp)), 4),
-                                                                       "bounce": round(float(np.mean(kb)), 4)},
-                     "bytes_per_step": algo_bytes, "node_tests": node_tests, "tri_tests": tri_tests},
-        "build": {"ms": round(build_med, 3), "mtris_per_s": round(n_tris / build_med / 1e3, 2),
-                  "all_ms": [round(x, 3) for x in build_ms], "ploc_passes": int(stats.ploc_iterations),
-                  "ploc_cluster_sum": int(stats.ploc_cluster_sum), "cwbvh_nodes": int(stats.cwbvh_node_count)},
-        "tuning": tuning,
+        "data": "synthetic (seeded heightfield + long thin triangles; uniform-sphere origins and directions)",
+        "config": {"workload": head["workload"], "ploc_search_distance": 14, "max_prims_per_leaf": 3,
+                   "pre_splits": "1.3n", "reinsertion": "0.05 x [1, 1/3]"},
+        "roofline": head["roofline"],
+        "large": head["large"],
     }
-    if args.rebuild_frac > 0:
-        out["rebuild"] = bench_rebuild(args, ob, ctx, tris, d_tris, build_med)
+    if fixed:
+        out["tuning_fixed"] = fixed
+    if not args.no_heightfield:
+        out["heightfield"] = bench_heightfield(args, ob, ctx)
+    if not args.no_plain:
+        out["plain"] = bench_plain(args, ob, ctx)
     if args.batch_meshes > 0:
         out["batched"] = bench_batch(args, ob, ctx)
-    if args.large_grid > 0:
-        out["large"] = bench_large(args, ob, ctx)
     if args.serve_scenes > 0:
         out["serving"] = bench_serve(args, ob, ctx)
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
-        log(f"cpu baseline with {threads} threads")
-        cb = cpu_baseline(tris, np.concatenate([prim_rays, bounce]), threads, args.cpu_sample_rays)
-        out["cpu_baseline"] = {"value": round(cb["n_rays"] / cb["trace_s"] / 1e6, 3), "unit": "Mrays/s",
-                               "cores": threads, "kind": "port",
-                               "sample": f"{cb['n_rays']} of the {n_rays} rays (evenly strided), closest hit, "
-                                         f"SSE-semantics C++ restatement of the crate's traversal",
-                               "build_mtris_per_s_serial": round(cb["n_tris"] / cb["build_s"] / 1e6, 3)}
+        ct = head["cpu_trace"]
+        threads = cpu_threads()
+        log(f"cpu baseline: oracle builds, serial and {threads} threads")
+        cb = cpu_build_baseline(args, threads)
+        out["cpu_baseline"] = {
+            "value": round(ct["value"], 3), "unit": "Mrays/s", "cores": ct["threads"], "kind": "port",
+            "sample": f"all {ct['rays']} closest-hit rays of the headline step (large scene), traced through the "
+                      f"GPU-built CWBVH by the oracle (C++ restatement of the crate's traversal, SSE semantics) on "
+                      f"{ct['threads']} host threads in {ct['seconds']:.2f} s; hits equal the GPU's: "
+                      f"{ct['hits_equal_gpu']}",
+            "build": cb}
     print(json.dumps(out), flush=True)
 
 

@@ -282,6 +282,18 @@ obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cw
                                               const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays, uint32_t n,
                                               obvhs_ray_hit* d_hits, uint32_t* d_node_visits,
                                               uint32_t* d_tri_tests);
+/* Watertight mode (no counterpart in the reference: an extra primitive test, reported beside the crate's).  The
+ * traversal of obvhs_cwbvh_ray_traverse / _miss with the triangle test of Woop, Benthin and Wald, "Watertight
+ * Ray/Triangle Intersection" (JCGT 2013) in place of Triangle::intersect (triangle.rs:35-76): shear onto the ray's
+ * dominant axis, 2D edge functions recomputed in f64 when one is exactly 0, no culling, t in [tmin, tmax].  A ray
+ * through an edge or vertex shared by several triangles hits at least one of them; hit distances can differ from
+ * Triangle::intersect's in the last bits. */
+obvhs_status obvhs_cwbvh_ray_traverse_watertight(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                                 const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
+                                                 uint32_t n, obvhs_ray_hit* d_hits);
+obvhs_status obvhs_cwbvh_ray_traverse_miss_watertight(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                                      const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
+                                                      uint32_t n, uint8_t* d_miss);
 
 /* ---- batched build / traversal (many independent meshes in shared launches) ----
  * No single reference entry point: equivalent to calling build_cwbvh_from_tris (cwbvh/builder.rs:20-85) once per

@@ -468,6 +468,21 @@ class CwBvh:
               self.ctx.handle)
         return miss.to_host().astype(bool)
 
+    def ray_traverse_watertight(self, rays, bvh_tris, miss=False):
+        """The same traversal with the watertight triangle test (an extra mode beside Triangle::intersect; no
+        counterpart in the crate): RayHit[n], or with miss=True the occlusion flags (True where nothing was hit)."""
+        dr, n = _as_device(self.ctx, rays, RAY)
+        dt, _ = _as_device(self.ctx, bvh_tris, TRIANGLE)
+        if miss:
+            out = DeviceArray(self.ctx, np.uint8, n)
+            check(lib().obvhs_cwbvh_ray_traverse_miss_watertight(self.ctx.handle, self.handle, dt.ptr, dr.ptr, n,
+                                                                 out.ptr), self.ctx.handle)
+            return out.to_host().astype(bool)
+        hits = DeviceArray(self.ctx, RAY_HIT, n)
+        check(lib().obvhs_cwbvh_ray_traverse_watertight(self.ctx.handle, self.handle, dt.ptr, dr.ptr, n, hits.ptr),
+              self.ctx.handle)
+        return hits.to_host()
+
     @property
     def bvh2(self):
         """The BVH2 this CwBvh was built from by build_cwbvh_from_tris (borrowed; None for conversions)."""

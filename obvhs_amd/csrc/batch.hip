@@ -679,6 +679,7 @@ obvhs_status obvhs_batch_ray_traverse(obvhs_context* ctx, const obvhs_batch* b, 
                                       const obvhs_ray* d_rays, const uint32_t* d_ray_mesh, uint32_t n,
                                       obvhs_ray_hit* d_hits) {
     if (!ctx || !b || (n && (!d_rays || !d_ray_mesh || !d_hits))) return OBVHS_ERR_INVALID_ARG;
+    if (b->n_total && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
     TravScratch ts;
     if (n) {
         const obvhs_status ss = traverse_scratch(ctx, ts, n);
@@ -697,6 +698,7 @@ obvhs_status obvhs_batch_ray_traverse_miss(obvhs_context* ctx, const obvhs_batch
                                            const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
                                            const uint32_t* d_ray_mesh, uint32_t n, uint8_t* d_miss) {
     if (!ctx || !b || (n && (!d_rays || !d_ray_mesh || !d_miss))) return OBVHS_ERR_INVALID_ARG;
+    if (b->n_total && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
     TravScratch ts;
     if (n) {
         const obvhs_status ss = traverse_scratch(ctx, ts, n);

@@ -614,6 +614,38 @@ obvhs_status obvhs_cwbvh_ray_traverse_counted(obvhs_context* ctx, const obvhs_cw
     return OBVHS_OK;
 }
 
+// The watertight primitive test in place of Triangle::intersect (traverse.hip: tri_intersect_wt), closest hit
+// (mode 2) or occlusion (mode 3); the traversal itself is ray_traverse / ray_traverse_miss's.
+static obvhs_status traverse_watertight(obvhs_context* ctx, const obvhs_cwbvh* bvh, const obvhs_triangle* d_bvh_tris,
+                                        const obvhs_ray* d_rays, uint32_t n, int mode, obvhs_ray_hit* d_hits,
+                                        uint8_t* d_miss) {
+    if (!ctx || !bvh || (n && (!d_rays || (mode == 2 ? !d_hits : !d_miss)))) return OBVHS_ERR_INVALID_ARG;
+    if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
+    TravScratch ts;
+    if (n) {
+        const obvhs_status ss = traverse_scratch(ctx, ts, n);
+        if (ss != OBVHS_OK) return ss;
+    }
+    ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
+    ts.chunked = chunk_for(ctx, bvh->prim_count);
+    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris, d_rays, n, mode,
+                            d_hits, d_miss, nullptr, nullptr, ts, ctx->stream),
+            "traverse watertight");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_cwbvh_ray_traverse_watertight(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                                 const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
+                                                 uint32_t n, obvhs_ray_hit* d_hits) {
+    return traverse_watertight(ctx, bvh, d_bvh_tris, d_rays, n, 2, d_hits, nullptr);
+}
+
+obvhs_status obvhs_cwbvh_ray_traverse_miss_watertight(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                                      const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
+                                                      uint32_t n, uint8_t* d_miss) {
+    return traverse_watertight(ctx, bvh, d_bvh_tris, d_rays, n, 3, nullptr, d_miss);
+}
+
 // The CWBVH exponent threshold table (host copy; built from the host libm on first use, no device needed).
 obvhs_status obvhs_log2_threshold_table(uint32_t* out256) {
     if (!out256) return OBVHS_ERR_INVALID_ARG;

@@ -8,14 +8,15 @@
 //
 // Threads and streams:
 //   * the scheduler thread owns all queues.  It moves submitted requests into per-scene FIFOs (submission order
-//     per scene), completes finished launches (event queries), admits requests in submission order while the HBM
-//     budget allows, and launches one traversal per mode over every ready request at the head of its scene's FIFO
-//     (each request a segment of the launch: its own BVH, rays and outputs) while fewer than max_inflight launches
-//     are on the trace stream;
+//     per scene), completes finished launches (event queries, in any order), admits requests in submission order
+//     while the HBM budget allows, and launches one traversal per mode over every ready request at the head of its
+//     scene's FIFO (each request a segment of the launch: its own BVH, rays and outputs) on a free trace stream:
+//     max_inflight trace contexts (stream + traversal spill columns each), so a launch that starts while a large
+//     one runs shares the GPU with it instead of waiting for it to finish;
 //   * update threads (one context and stream each) run refits and rebuilds, several scenes at a time.  An update
-//     writes the scene's other version slot, after the trace stream's last launch that read that slot (event wait),
-//     and becomes the scene's current version when it completes; traces submitted after it wait for it, traces
-//     submitted before it keep reading the old slot.
+//     writes the scene's other version slot, after every trace stream's last launch that read that slot (event
+//     waits), and becomes the scene's current version when it completes; traces submitted after it wait for it,
+//     traces submitted before it keep reading the old slot.
 #include <string.h>
 
 #include <algorithm>
@@ -61,13 +62,15 @@ struct Req {
     int64_t t_submit = 0, t_start = 0, t_done = 0;
 };
 
+constexpr uint32_t MAX_TRACE_LANES = 8;  // trace contexts (streams) at most: max_inflight is capped to this
+
 struct Version {
     obvhs_cwbvh* cw = nullptr;  // owns its BVH2
     obvhs::DevBuf tris;         // triangles in input order
     obvhs::DevBuf bvh_tris;     // triangles in primitive_indices order
     uint32_t n = 0;
-    hipEvent_t last_use = nullptr;  // recorded on the trace stream after each launch reading this slot
-    bool used = false;
+    hipEvent_t last_use[MAX_TRACE_LANES] = {};  // per trace stream: recorded after each launch reading this slot
+    bool used[MAX_TRACE_LANES] = {};
     uint64_t bytes = 0;
 };
 
@@ -83,6 +86,7 @@ struct Scene {
 
 struct Batch {
     uint64_t id = 0;
+    uint32_t lane = 0;  // trace context (stream) it runs on
     hipEvent_t done = nullptr;
     std::vector<Req*> reqs;
 };
@@ -109,7 +113,8 @@ uint64_t version_bytes(const Version& v) {
 struct obvhs_server {
     int device = 0;
     obvhs_server_config cfg{};
-    obvhs_context* tctx = nullptr;  // trace stream
+    std::vector<obvhs_context*> tctx;  // trace contexts: one stream + traversal spill columns per in-flight launch
+    std::vector<uint32_t> free_lanes;  // trace contexts without a launch in flight
     std::vector<obvhs_context*> uctx;  // one context (stream + build scratch) per updater thread, then one for
                                        // add_scene
     Clock::time_point t0 = Clock::now();
@@ -129,10 +134,9 @@ struct obvhs_server {
     std::deque<Req*> order;  // admitted-or-waiting requests in submission order (admission is FIFO)
     uint64_t resident = 0, inflight_bytes = 0;
     uint64_t next_batch = 1;
-    obvhs::DevBuf seg_dev;   // TraceSegment ring: max_inflight * 2 launches * max_segments
+    obvhs::DevBuf seg_dev;   // TraceSegment tables: 2 per trace context (one per mode) * max_segments
     obvhs::TraceSegment* seg_host = nullptr;
-    uint32_t seg_slot = 0;
-    obvhs::TravScratch trav;  // spill columns of the trace context's persistent traversal grid
+    std::vector<obvhs::TravScratch> trav;  // spill columns of each trace context
 
     // update thread
     std::deque<Req*> updates;
@@ -162,8 +166,8 @@ obvhs_status run_update(obvhs_server* S, obvhs_context* ctx, Scene* sc, Req* r, 
     hipStream_t st = ctx->stream;
     Version& dst = sc->v[slot];
     Version& src = sc->v[sc->cur];
-    if (dst.used) {  // the trace stream's last launch that read this slot must finish first
-        if (hipStreamWaitEvent(st, dst.last_use, 0) != hipSuccess) return OBVHS_ERR_DEVICE;
+    for (uint32_t k = 0; k < MAX_TRACE_LANES; ++k) {  // every trace stream's last launch that read this slot first
+        if (dst.used[k] && hipStreamWaitEvent(st, dst.last_use[k], 0) != hipSuccess) return OBVHS_ERR_DEVICE;
     }
     // a refit keeps the triangle count of the version it refits (checked when it runs: earlier rebuilds may change it)
     if (r->kind == REQ_REFIT && r->n != src.n) return OBVHS_ERR_INVALID_ARG;
@@ -264,7 +268,10 @@ void admit(obvhs_server* S) {
 }
 
 void launch_batch(obvhs_server* S, std::vector<Req*>& ready) {
-    hipStream_t st = S->tctx->stream;
+    const uint32_t lane = S->free_lanes.back();
+    S->free_lanes.pop_back();
+    obvhs_context* tc = S->tctx[lane];
+    hipStream_t st = tc->stream;
     Batch* b = nullptr;
     if (!S->free_batches.empty()) {
         b = S->free_batches.back();
@@ -274,11 +281,11 @@ void launch_batch(obvhs_server* S, std::vector<Req*>& ready) {
         (void)hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
     }
     b->id = S->next_batch++;
+    b->lane = lane;
     b->reqs = ready;
     const int64_t t = now_ns(S->t0);
     for (int mode = 0; mode < 2; ++mode) {
-        const uint32_t slot = S->seg_slot;
-        S->seg_slot = (S->seg_slot + 1) % (2 * S->cfg.max_inflight + 2);
+        const uint32_t slot = 2 * lane + (uint32_t)mode;  // this lane's table for the mode: free while it is idle
         TraceSegment* hs = S->seg_host + (size_t)slot * MAX_SEGMENTS;
         uint32_t nseg = 0, total = 0, largest = 0;
         for (Req* r : ready) {
@@ -305,8 +312,8 @@ void launch_batch(obvhs_server* S, std::vector<Req*>& ready) {
         if (nseg == 0) continue;
         TraceSegment* ds = S->seg_dev.as<TraceSegment>() + (size_t)slot * MAX_SEGMENTS;
         hipError_t e = hipMemcpyAsync(ds, hs, (size_t)nseg * sizeof(TraceSegment), hipMemcpyHostToDevice, st);
-        TravScratch ts = S->trav;
-        ts.lds_depth = lds_depth_for(S->tctx, largest);  // the depth tuned for the launch's largest BVH
+        TravScratch ts = S->trav[lane];
+        ts.lds_depth = lds_depth_for(tc, largest);  // the depth tuned for the launch's largest BVH
         if (e == hipSuccess) e = launch_traverse_segments(ds, nseg, total, mode, ts, st);
         if (e != hipSuccess) {
             for (Req* r : ready)
@@ -318,8 +325,8 @@ void launch_batch(obvhs_server* S, std::vector<Req*>& ready) {
     }
     for (Req* r : ready) {  // slots read by this launch
         Version& v = S->scenes[r->scene]->v[S->scenes[r->scene]->cur];
-        (void)hipEventRecord(v.last_use, st);
-        v.used = true;
+        (void)hipEventRecord(v.last_use[lane], st);
+        v.used[lane] = true;
     }
     (void)hipEventRecord(b->done, st);
     S->inflight.push_back(b);
@@ -338,15 +345,20 @@ void scheduler_main(obvhs_server* S) {
             S->order.push_back(r);
             progress = true;
         }
-        // finished launches
-        while (!S->inflight.empty() && hipEventQuery(S->inflight.front()->done) == hipSuccess) {
-            Batch* b = S->inflight.front();
-            S->inflight.pop_front();
+        // finished launches (the trace streams finish in any order)
+        for (size_t i = 0; i < S->inflight.size();) {
+            Batch* b = S->inflight[i];
+            if (hipEventQuery(b->done) != hipSuccess) {
+                ++i;
+                continue;
+            }
+            S->inflight.erase(S->inflight.begin() + (std::ptrdiff_t)i);
             for (Req* r : b->reqs) {
                 S->inflight_bytes -= std::min(S->inflight_bytes, r->bytes);
                 complete(S, r, r->status);
             }
             b->reqs.clear();
+            S->free_lanes.push_back(b->lane);
             S->free_batches.push_back(b);
             progress = true;
             S->cv_done.notify_all();
@@ -376,7 +388,7 @@ void scheduler_main(obvhs_server* S) {
             }
         }
         // one coalesced launch over every ready trace at the head of its scene's FIFO
-        if (S->inflight.size() < S->cfg.max_inflight) {
+        if (!S->free_lanes.empty()) {
             ready.clear();
             uint64_t rays = 0;
             for (Scene* sc : S->scenes) {
@@ -420,17 +432,26 @@ obvhs_status obvhs_server_create(int device, const obvhs_server_config* cfg, obv
     S->device = device;
     S->cfg.hbm_budget_bytes = cfg && cfg->hbm_budget_bytes ? cfg->hbm_budget_bytes : (200ull << 30);
     S->cfg.max_batch_rays = cfg && cfg->max_batch_rays ? cfg->max_batch_rays : (1u << 23);
-    S->cfg.max_inflight = cfg && cfg->max_inflight ? std::min(cfg->max_inflight, 8u) : 2u;
-    obvhs_status s = obvhs_context_create(device, nullptr, &S->tctx);
+    S->cfg.max_inflight = cfg && cfg->max_inflight ? std::min(cfg->max_inflight, MAX_TRACE_LANES) : 2u;
+    obvhs_status s = OBVHS_OK;
+    S->trav.resize(S->cfg.max_inflight);
+    for (uint32_t k = 0; s == OBVHS_OK && k < S->cfg.max_inflight; ++k) {
+        obvhs_context* c = nullptr;
+        s = obvhs_context_create(device, nullptr, &c);
+        if (s == OBVHS_OK) {
+            S->tctx.push_back(c);
+            s = traverse_scratch(c, S->trav[k], S->cfg.max_batch_rays);
+            S->free_lanes.push_back(S->cfg.max_inflight - 1 - k);
+        }
+    }
     S->cfg.update_streams = cfg && cfg->update_streams ? std::min(cfg->update_streams, 4u) : 2u;
     for (uint32_t u = 0; s == OBVHS_OK && u <= S->cfg.update_streams; ++u) {
         obvhs_context* c = nullptr;
         s = obvhs_context_create(device, nullptr, &c);
         if (s == OBVHS_OK) S->uctx.push_back(c);
     }
-    const size_t nseg = (size_t)(2 * S->cfg.max_inflight + 2) * MAX_SEGMENTS;
+    const size_t nseg = (size_t)(2 * S->cfg.max_inflight) * MAX_SEGMENTS;
     if (s == OBVHS_OK && S->seg_dev.ensure(nseg * sizeof(TraceSegment)) != hipSuccess) s = OBVHS_ERR_OOM;
-    if (s == OBVHS_OK) s = traverse_scratch(S->tctx, S->trav, S->cfg.max_batch_rays);
     if (s == OBVHS_OK &&
         hipHostMalloc(reinterpret_cast<void**>(&S->seg_host), nseg * sizeof(TraceSegment), hipHostMallocDefault) !=
             hipSuccess)
@@ -457,7 +478,7 @@ obvhs_status obvhs_server_destroy(obvhs_server* S) {
     for (std::thread& t : S->updaters)
         if (t.joinable()) t.join();
     (void)hipSetDevice(S->device);
-    if (S->tctx) (void)hipStreamSynchronize(S->tctx->stream);
+    for (obvhs_context* c : S->tctx) (void)hipStreamSynchronize(c->stream);
     for (obvhs_context* c : S->uctx) (void)hipStreamSynchronize(c->stream);
     for (Batch* b : S->inflight) {
         if (b->done) (void)hipEventDestroy(b->done);
@@ -472,14 +493,15 @@ obvhs_status obvhs_server_destroy(obvhs_server* S) {
             if (v.cw) obvhs_cwbvh_destroy(v.cw);
             v.tris.release();
             v.bvh_tris.release();
-            if (v.last_use) (void)hipEventDestroy(v.last_use);
+            for (hipEvent_t& e : v.last_use)
+                if (e) (void)hipEventDestroy(e);
         }
         delete sc;
     }
     for (auto& kv : S->reqs) delete kv.second;
     S->seg_dev.release();
     if (S->seg_host) (void)hipHostFree(S->seg_host);
-    obvhs_context_destroy(S->tctx);
+    for (obvhs_context* c : S->tctx) obvhs_context_destroy(c);
     for (obvhs_context* c : S->uctx) obvhs_context_destroy(c);
     delete S;
     return OBVHS_OK;
@@ -497,7 +519,8 @@ obvhs_status obvhs_server_add_scene(obvhs_server* S, const obvhs_triangle* d_tri
     r.tris = d_tris;
     r.n = n;
     for (Version& v : sc->v)
-        if (hipEventCreateWithFlags(&v.last_use, hipEventDisableTiming) != hipSuccess) return OBVHS_ERR_DEVICE;
+        for (hipEvent_t& e : v.last_use)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return OBVHS_ERR_DEVICE;
     // the initial version is built like a rebuild into slot 0 on the update context
     obvhs_status s;
     {
@@ -512,7 +535,8 @@ obvhs_status obvhs_server_add_scene(obvhs_server* S, const obvhs_triangle* d_tri
             if (v.cw) obvhs_cwbvh_destroy(v.cw);
             v.tris.release();
             v.bvh_tris.release();
-            (void)hipEventDestroy(v.last_use);
+            for (hipEvent_t& e : v.last_use)
+                if (e) (void)hipEventDestroy(e);
         }
         delete sc;
         return s;

@@ -53,6 +53,10 @@ __device__ __forceinline__ float byte_f(uint32_t w, int b) { return (float)((w >
 // CwBvhNode::intersect_ray_simd (cwbvh/simd.rs:17-100) with get_child_and_index_bits (cwbvh/node.rs:207-231).
 // Node words: q0 = {p.x, p.y, p.z, e|imask}, q1 = {child_base, prim_base, meta lo, meta hi},
 // q2 = {min_x lo/hi, max_x lo/hi}, q3 = {min_y, max_y}, q4 = {min_z, max_z}.
+// ROBUST (the watertight mode only): each quantised child box widened by one quantum per side and the slab interval
+// by a relative 2^-20, so rounding in the quantisation and in the slab arithmetic cannot cull a child the ray
+// touches (the crate's test is exact only in exact arithmetic).
+template <bool ROBUST = false>
 __device__ __forceinline__ uint32_t intersect_node(const uint4& q0, const uint4& q1, const uint4& q2,
                                                    const uint4& q3, const uint4& q4, const RayData& r,
                                                    float ray_tmax, uint32_t oct_inv4) {
@@ -80,9 +84,10 @@ __device__ __forceinline__ uint32_t intersect_node(const uint4& q0, const uint4&
         const uint32_t wminx = h ? q2.y : q2.x, wmaxx = h ? q2.w : q2.z;
         const uint32_t wminy = h ? q3.y : q3.x, wmaxy = h ? q3.w : q3.z;
         const uint32_t wminz = h ? q4.y : q4.x, wmaxz = h ? q4.w : q4.z;
-        const float qlx = byte_f(wminx, b), qhx = byte_f(wmaxx, b);
-        const float qly = byte_f(wminy, b), qhy = byte_f(wmaxy, b);
-        const float qlz = byte_f(wminz, b), qhz = byte_f(wmaxz, b);
+        const float pad = ROBUST ? 1.0f : 0.0f;
+        const float qlx = byte_f(wminx, b) - pad, qhx = byte_f(wmaxx, b) + pad;
+        const float qly = byte_f(wminy, b) - pad, qhy = byte_f(wmaxy, b) + pad;
+        const float qlz = byte_f(wminz, b) - pad, qhz = byte_f(wmaxz, b) + pad;
         const float xmin = rdx ? qhx : qlx, xmax = rdx ? qlx : qhx;
         const float ymin = rdy ? qhy : qly, ymax = rdy ? qly : qhy;
         const float zmin = rdz ? qhz : qlz, zmax = rdz ? qlz : qhz;
@@ -91,6 +96,11 @@ __device__ __forceinline__ uint32_t intersect_node(const uint4& q0, const uint4&
         const float tminz = zmin * adz + aoz, tmaxz = zmax * adz + aoz;
         float tmin = smax(tminx, smax(tminy, tminz));
         float tmax = smin(tmaxx, smin(tmaxy, tmaxz));
+        if (ROBUST) {
+            const float k = 9.5367431640625e-07f;  // 2^-20
+            tmin = tmin - fabsf(tmin) * k;
+            tmax = tmax + fabsf(tmax) * k;
+        }
         tmin = smax(tmin, NODE_EPSILON);
         tmax = smin(tmax, ray_tmax);
         if (tmin <= tmax) {
@@ -120,6 +130,76 @@ __device__ __forceinline__ float tri_intersect(const float4& v0, const float4& v
         if (t >= tmin && t <= tmax) return t;
     }
     return INFINITY;
+}
+
+// ---- watertight mode (no counterpart in the reference: an extra primitive test beside Triangle::intersect) ----
+// Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1) 2013: the ray's dominant axis kz, the other
+// two axes kx, ky (swapped when d[kz] < 0, which keeps the winding) and the shear constants are computed once per
+// ray; per triangle the vertices are translated to the ray origin, sheared onto the plane z = 0 and tested with 2D
+// edge functions U, V, W, recomputed in f64 when one is exactly 0 (f32 products are exact in f64), so a ray through
+// an edge or a vertex shared by several triangles hits at least one of them.  No culling; t = T / det in
+// [tmin, tmax] inclusive, as Triangle::intersect's range test.
+struct WtRay {
+    uint32_t kx, ky, kz;
+    float sx, sy, sz;
+};
+
+__device__ __forceinline__ float sel3(float x, float y, float z, uint32_t k) { return k == 0u ? x : (k == 1u ? y : z); }
+
+__device__ __forceinline__ WtRay wt_ray(const RayData& r) {
+    const float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
+    WtRay w;
+    w.kz = (ax >= ay && ax >= az) ? 0u : (ay >= az ? 1u : 2u);
+    w.kx = w.kz == 2u ? 0u : w.kz + 1u;
+    w.ky = w.kx == 2u ? 0u : w.kx + 1u;
+    const float dz = sel3(r.dx, r.dy, r.dz, w.kz);
+    if (dz < 0.0f) {
+        const uint32_t t = w.kx;
+        w.kx = w.ky;
+        w.ky = t;
+    }
+    w.sx = sel3(r.dx, r.dy, r.dz, w.kx) / dz;
+    w.sy = sel3(r.dx, r.dy, r.dz, w.ky) / dz;
+    w.sz = 1.0f / dz;
+    return w;
+}
+
+__device__ __forceinline__ float tri_intersect_wt(const float4& v0, const float4& v1, const float4& v2,
+                                                  const RayData& r, const WtRay& w, float tmin, float tmax) {
+    const float a0 = v0.x - r.ox, a1 = v0.y - r.oy, a2 = v0.z - r.oz;
+    const float b0 = v1.x - r.ox, b1 = v1.y - r.oy, b2 = v1.z - r.oz;
+    const float c0 = v2.x - r.ox, c1 = v2.y - r.oy, c2 = v2.z - r.oz;
+    const float akz = sel3(a0, a1, a2, w.kz), bkz = sel3(b0, b1, b2, w.kz), ckz = sel3(c0, c1, c2, w.kz);
+    const float ax = sel3(a0, a1, a2, w.kx) - w.sx * akz, ay = sel3(a0, a1, a2, w.ky) - w.sy * akz;
+    const float bx = sel3(b0, b1, b2, w.kx) - w.sx * bkz, by = sel3(b0, b1, b2, w.ky) - w.sy * bkz;
+    const float cx = sel3(c0, c1, c2, w.kx) - w.sx * ckz, cy = sel3(c0, c1, c2, w.ky) - w.sy * ckz;
+    float u = cx * by - cy * bx, v = ax * cy - ay * cx, ww = bx * ay - by * ax;
+    if (u == 0.0f || v == 0.0f || ww == 0.0f) {
+        u = (float)((double)cx * (double)by - (double)cy * (double)bx);
+        v = (float)((double)ax * (double)cy - (double)ay * (double)cx);
+        ww = (float)((double)bx * (double)ay - (double)by * (double)ax);
+    }
+    if ((u < 0.0f || v < 0.0f || ww < 0.0f) && (u > 0.0f || v > 0.0f || ww > 0.0f)) return INFINITY;
+    const float det = (u + v) + ww;
+    if (det == 0.0f) return INFINITY;
+    const float az = w.sz * akz, bz = w.sz * bkz, cz = w.sz * ckz;
+    float tt = (u * az + v * bz) + ww * cz;
+    float adet = det;
+    if (det < 0.0f) {
+        tt = -tt;
+        adet = -det;
+    }
+    const float t = tt / adet;
+    return (t >= tmin && t <= tmax) ? t : INFINITY;
+}
+
+// MODE of the traversal kernels: bit 0 = occlusion (ray_traverse_miss) instead of closest hit, bit 1 = the
+// watertight primitive test instead of Triangle::intersect.
+template <int MODE>
+__device__ __forceinline__ float prim_test(const float4& v0, const float4& v1, const float4& v2, const RayData& r,
+                                           const WtRay& w, float tmin, float tmax) {
+    if (MODE & 2) return tri_intersect_wt(v0, v1, v2, r, w, tmin, tmax);
+    return tri_intersect(v0, v1, v2, r, tmin, tmax);
 }
 
 // Where a launch's rays come from: one BVH (SRC_PLAIN), a mesh of a batched build per ray (SRC_MESHES), or request
@@ -185,6 +265,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     const float4 ra = rays[4 * ri], rb = rays[4 * ri + 1], rc = rays[4 * ri + 2], rd = rays[4 * ri + 3];
     const RayData r{ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, rc.x, rc.y, rc.z, rd.x, rd.y};
     const uint32_t oct = octant_inv4(r.dx, r.dy, r.dz);
+    const WtRay wr = (MODE & 2) ? wt_ray(r) : WtRay{};
     float tmax = r.tmax;
     uint32_t hit_id = INVALID32;
     float hit_t = INFINITY;
@@ -220,15 +301,15 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             if (pid >= prim_count) continue;  // malformed input only: never index past the triangle buffer
             const float4 v0 = tris[3 * (size_t)pid], v1 = tris[3 * (size_t)pid + 1], v2 = tris[3 * (size_t)pid + 2];
             if (COUNT) tv++;
-            if (MODE == 0) {
-                const float t = tri_intersect(v0, v1, v2, r, r.tmin, tmax);
+            if ((MODE & 1) == 0) {
+                const float t = prim_test<MODE>(v0, v1, v2, r, wr, r.tmin, tmax);
                 if (t < tmax) {
                     hit_id = pid;
                     hit_t = t;
                     tmax = t;
                 }
             } else {
-                const float t = tri_intersect(v0, v1, v2, r, r.tmin, r.tmax);
+                const float t = prim_test<MODE>(v0, v1, v2, r, wr, r.tmin, r.tmax);
                 if (t < r.tmax) {
                     is_miss = false;
                     stop = true;
@@ -247,7 +328,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             }
             if (child >= node_count) break;  // malformed input only: never index past the node buffer
             if (COUNT) nv++;
-            const uint32_t hm = intersect_node(q0, q1, q2, q3, q4, r, tmax, oct);
+            const uint32_t hm = intersect_node<(MODE & 2) != 0>(q0, q1, q2, q3, q4, r, tmax, oct);
             const uint32_t imask = (q0.w >> 24) & 0xffu;
             cur = make_uint2(q1.x, (hm & 0xff000000u) | imask);
             prim = make_uint2(q1.y, hm & 0x00ffffffu);
@@ -260,7 +341,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             cur = sp < LDS ? stack[sp][lane] : my_spill[(size_t)(sp - LDS) * lanes];
         }
     }
-    if (MODE == 0) {
+    if ((MODE & 1) == 0) {
         obvhs_ray_hit h;
         h.primitive_id = hit_id;
         h.geometry_id = INVALID32;
@@ -299,6 +380,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     bool live = false;
     uint32_t i = 0u;
     RayData r{};
+    WtRay wr{};
     uint32_t oct = 0u, hit_id = INVALID32;
     float tmax = 0.0f, hit_t = INFINITY;
     bool is_miss = true;
@@ -335,6 +417,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
                                  rd = rays[4 * (size_t)i + 3];
                     r = RayData{ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, rc.x, rc.y, rc.z, rd.x, rd.y};
                     oct = octant_inv4(r.dx, r.dy, r.dz);
+                    if (MODE & 2) wr = wt_ray(r);
                     tmax = r.tmax;
                     hit_id = INVALID32;
                     hit_t = INFINITY;
@@ -372,15 +455,15 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             const uint32_t pid = prim.x + local;
             if (pid >= prim_count) continue;  // malformed input only: never index past the triangle buffer
             const float4 v0 = tris[3 * (size_t)pid], v1 = tris[3 * (size_t)pid + 1], v2 = tris[3 * (size_t)pid + 2];
-            if (MODE == 0) {
-                const float t = tri_intersect(v0, v1, v2, r, r.tmin, tmax);
+            if ((MODE & 1) == 0) {
+                const float t = prim_test<MODE>(v0, v1, v2, r, wr, r.tmin, tmax);
                 if (t < tmax) {
                     hit_id = pid;
                     hit_t = t;
                     tmax = t;
                 }
             } else {
-                const float t = tri_intersect(v0, v1, v2, r, r.tmin, r.tmax);
+                const float t = prim_test<MODE>(v0, v1, v2, r, wr, r.tmin, r.tmax);
                 if (t < r.tmax) {
                     is_miss = false;
                     done = true;
@@ -400,7 +483,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
                 if (child >= node_count) {
                     done = true;  // malformed input only: never index past the node buffer
                 } else {
-                    const uint32_t hm = intersect_node(q0, q1, q2, q3, q4, r, tmax, oct);
+                    const uint32_t hm = intersect_node<(MODE & 2) != 0>(q0, q1, q2, q3, q4, r, tmax, oct);
                     const uint32_t imask = (q0.w >> 24) & 0xffu;
                     cur = make_uint2(q1.x, (hm & 0xff000000u) | imask);
                     prim = make_uint2(q1.y, hm & 0x00ffffffu);
@@ -418,7 +501,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             }
         }
         if (done) {
-            if (MODE == 0)
+            if ((MODE & 1) == 0)
                 reinterpret_cast<uint4*>(hits)[i] = make_uint4(hit_id, INVALID32, INVALID32, __float_as_uint(hit_t));
             else
                 miss_out[i] = is_miss ? 1 : 0;
@@ -593,29 +676,30 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
     const TraceSegment* no_segs = nullptr;
     // Chunked launches (when the tuning table asks for them: incoherent rays) need at least two rays per resident
     // lane; each lane then gets that many, refilled in the wave.
+    if (mode > 3 || (node_visits && mode != 0)) return hipErrorInvalidValue;
     const uint32_t rows = node_visits ? 0u : chunk_rows(n, ts);
     if (rows) {
         const uint32_t per_wave = rows * TRAV_THREADS;
         const uint32_t cblocks = (uint32_t)(((uint64_t)n + per_wave - 1) / per_wave);
-        if (mode == 0)
-            launch_chunk<0, SRC_PLAIN>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits,
-                                       miss, sp, per_wave, no_meshes, no_map);
-        else
-            launch_chunk<1, SRC_PLAIN>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits,
-                                       miss, sp, per_wave, no_meshes, no_map);
+#define OBVHS_CHUNK(M)                                                                                           \
+    launch_chunk<M, SRC_PLAIN>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits, miss, \
+                               sp, per_wave, no_meshes, no_map)
+        if (mode == 0) OBVHS_CHUNK(0);
+        else if (mode == 1) OBVHS_CHUNK(1);
+        else if (mode == 2) OBVHS_CHUNK(2);
+        else OBVHS_CHUNK(3);
+#undef OBVHS_CHUNK
         return hipGetLastError();
     }
-    if (mode == 0) {
-        if (node_visits)
-            launch_depth<0, true, SRC_PLAIN>(ts.lds_depth, blocks, stream, nd, node_count, prim_count, tr, ry, n,
-                                             hits, miss, node_visits, tri_tests, no_meshes, no_map, no_segs, sp);
-        else
-            launch_depth<0, false, SRC_PLAIN>(ts.lds_depth, blocks, stream, nd, node_count, prim_count, tr, ry, n,
-                                              hits, miss, node_visits, tri_tests, no_meshes, no_map, no_segs, sp);
-    } else {
-        launch_depth<1, false, SRC_PLAIN>(ts.lds_depth, blocks, stream, nd, node_count, prim_count, tr, ry, n, hits,
-                                          miss, node_visits, tri_tests, no_meshes, no_map, no_segs, sp);
-    }
+#define OBVHS_LANES(M, C)                                                                                       \
+    launch_depth<M, C, SRC_PLAIN>(ts.lds_depth, blocks, stream, nd, node_count, prim_count, tr, ry, n, hits, miss, \
+                                  node_visits, tri_tests, no_meshes, no_map, no_segs, sp)
+    if (node_visits) OBVHS_LANES(0, true);
+    else if (mode == 0) OBVHS_LANES(0, false);
+    else if (mode == 1) OBVHS_LANES(1, false);
+    else if (mode == 2) OBVHS_LANES(2, false);
+    else OBVHS_LANES(3, false);
+#undef OBVHS_LANES
     return hipGetLastError();
 }
 

@@ -104,6 +104,9 @@ enum {
     MUT_CW_LUT_FLIP_X = 18,    // direction LUT with the x sign flipped
     MUT_CW_SLOT_LE = 19,       // `<=` instead of `<` in the greedy slot assignment (the last minimum wins)
     MUT_CW_SLOT_MAJOR = 20,    // greedy slot assignment scanned slot-major (slot outer, child inner)
+    MUT_WT_MIN_AXIS = 21,      // watertight test shears onto the smallest direction axis instead of the largest
+    MUT_WT_EDGE_STRICT = 22,   // watertight test counts an edge function of exactly 0 as outside
+    MUT_WT_NODE_PLAIN = 23,    // watertight mode traverses with the crate's node test, not the widened one
 };
 int g_mut = MUT_NONE;
 
@@ -960,7 +963,11 @@ inline void get_child_and_index_bits(const CwBvhNode& n, uint32_t oct_inv4, uint
 }
 
 // cwbvh/simd.rs:17-100 (the x86 SSE path the reference takes by default)
-inline uint32_t intersect_node(const CwBvhNode& n, const Ray& ray, float ray_tmax, uint32_t oct_inv4) {
+// `robust` (the watertight mode only): each quantised child box widened by one quantum per side and the slab
+// interval by a relative 2^-20, so rounding in the quantisation and in the slab arithmetic cannot cull a child the
+// ray touches (the crate's test is exact only in exact arithmetic).
+inline uint32_t intersect_node(const CwBvhNode& n, const Ray& ray, float ray_tmax, uint32_t oct_inv4,
+                               bool robust = false) {
     V3 ext = v3(bits_to_f32((uint32_t)n.e[0] << 23), bits_to_f32((uint32_t)n.e[1] << 23),
                 bits_to_f32((uint32_t)n.e[2] << 23));
     float adx = ext.x * ray.inv_direction.x, ady = ext.y * ray.inv_direction.y, adz = ext.z * ray.inv_direction.z;
@@ -975,6 +982,10 @@ inline uint32_t intersect_node(const CwBvhNode& n, const Ray& ray, float ray_tma
         float qlx = (float)n.child_min_x[c], qhx = (float)n.child_max_x[c];
         float qly = (float)n.child_min_y[c], qhy = (float)n.child_max_y[c];
         float qlz = (float)n.child_min_z[c], qhz = (float)n.child_max_z[c];
+        if (robust) {
+            qlx -= 1.0f, qly -= 1.0f, qlz -= 1.0f;
+            qhx += 1.0f, qhy += 1.0f, qhz += 1.0f;
+        }
         float xmin = rdx ? qhx : qlx, xmax = rdx ? qlx : qhx;
         float ymin = rdy ? qhy : qly, ymax = rdy ? qly : qhy;
         float zmin = rdz ? qhz : qlz, zmax = rdz ? qlz : qhz;
@@ -983,6 +994,11 @@ inline uint32_t intersect_node(const CwBvhNode& n, const Ray& ray, float ray_tma
         float tminz = zmin * adz + aoz, tmaxz = zmax * adz + aoz;
         float tmin = smax(tminx, smax(tminy, tminz));
         float tmax = smin(tmaxx, smin(tmaxy, tmaxz));
+        if (robust) {
+            const float k = 9.5367431640625e-07f;  // 2^-20
+            tmin = tmin - std::fabs(tmin) * k;
+            tmax = tmax + std::fabs(tmax) * k;
+        }
         tmin = smax(tmin, g_mut == MUT_NODE_TMIN_ZERO ? 0.0f : NODE_EPSILON);
         tmax = smin(tmax, ray_tmax);
         if (tmin <= tmax) {
@@ -1015,13 +1031,78 @@ inline float tri_intersect(const Triangle& tri, const Ray& ray, float tmin, floa
     return INFINITY;
 }
 
+// Watertight ray/triangle test (no counterpart in the crate: the library's extra primitive test), restated from
+// Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1) 2013: dominant axis kz of |d| (ties to
+// the lower axis), kx, ky the next two cyclically and swapped when d[kz] < 0; shear Sx = d[kx]/d[kz],
+// Sy = d[ky]/d[kz], Sz = 1/d[kz]; edge functions of the sheared, translated vertices, recomputed in f64 when one is
+// exactly 0; no culling; t = T/det in [tmin, tmax].
+struct WtRay {
+    uint32_t kx, ky, kz;
+    float sx, sy, sz;
+};
+
+inline float comp3(const V3& v, uint32_t k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+inline WtRay wt_ray(const Ray& ray) {
+    const V3& d = ray.direction;
+    const float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    WtRay w;
+    if (g_mut == MUT_WT_MIN_AXIS)
+        w.kz = (ax <= ay && ax <= az) ? 0u : (ay <= az ? 1u : 2u);
+    else
+        w.kz = (ax >= ay && ax >= az) ? 0u : (ay >= az ? 1u : 2u);
+    w.kx = (w.kz + 1u) % 3u;
+    w.ky = (w.kx + 1u) % 3u;
+    const float dz = comp3(d, w.kz);
+    if (dz < 0.0f) std::swap(w.kx, w.ky);
+    w.sx = comp3(d, w.kx) / dz;
+    w.sy = comp3(d, w.ky) / dz;
+    w.sz = 1.0f / dz;
+    return w;
+}
+
+inline float tri_intersect_wt(const Triangle& tri, const Ray& ray, const WtRay& w, float tmin, float tmax) {
+    const V3 a = vsub(tri.v0, ray.origin), b = vsub(tri.v1, ray.origin), c = vsub(tri.v2, ray.origin);
+    const float az0 = comp3(a, w.kz), bz0 = comp3(b, w.kz), cz0 = comp3(c, w.kz);
+    const float ax = comp3(a, w.kx) - w.sx * az0, ay = comp3(a, w.ky) - w.sy * az0;
+    const float bx = comp3(b, w.kx) - w.sx * bz0, by = comp3(b, w.ky) - w.sy * bz0;
+    const float cx = comp3(c, w.kx) - w.sx * cz0, cy = comp3(c, w.ky) - w.sy * cz0;
+    float u = cx * by - cy * bx;
+    float v = ax * cy - ay * cx;
+    float e = bx * ay - by * ax;
+    if (u == 0.0f || v == 0.0f || e == 0.0f) {
+        const double uu = (double)cx * (double)by - (double)cy * (double)bx;
+        const double vv = (double)ax * (double)cy - (double)ay * (double)cx;
+        const double ee = (double)bx * (double)ay - (double)by * (double)ax;
+        u = (float)uu;
+        v = (float)vv;
+        e = (float)ee;
+    }
+    const bool any_neg = u < 0.0f || v < 0.0f || e < 0.0f;
+    const bool any_pos = u > 0.0f || v > 0.0f || e > 0.0f;
+    if (any_neg && any_pos) return INFINITY;
+    if (g_mut == MUT_WT_EDGE_STRICT && (u == 0.0f || v == 0.0f || e == 0.0f)) return INFINITY;
+    const float det = (u + v) + e;
+    if (det == 0.0f) return INFINITY;
+    const float az = w.sz * az0, bz = w.sz * bz0, cz = w.sz * cz0;
+    float tt = (u * az + v * bz) + e * cz;
+    float adet = det;
+    if (det < 0.0f) {
+        tt = -tt;
+        adet = -det;
+    }
+    const float t = tt / adet;
+    return (t >= tmin && t <= tmax) ? t : INFINITY;
+}
+
 struct TravStats {
     uint32_t nodes;
     uint32_t tris;
 };
 
 // cwbvh/mod.rs:169-198 (closest) and 201-225 (miss), both through traverse! (traverse_macro.rs:59-126) with a
-// StackStack<UVec2, 32> (faststack.rs:282-336).  `mode` 0 = closest hit, 1 = miss/occlusion.
+// StackStack<UVec2, 32> (faststack.rs:282-336).  `mode` 0 = closest hit, 1 = miss/occlusion; 2 and 3 the same with
+// the watertight primitive test (tri_intersect_wt) in place of Triangle::intersect.
 // Returns hit (closest) or miss flag.  Triangles are indexed by position in primitive_indices.
 inline void traverse_one(const CwBvhNode* nodes, uint32_t nnodes, const Triangle* tris, const Ray& ray, int mode,
                          RayHit* hit, uint8_t* miss, TravStats* st) {
@@ -1034,6 +1115,7 @@ inline void traverse_one(const CwBvhNode* nodes, uint32_t nnodes, const Triangle
     G current = nnodes == 0 ? G{0, 0} : G{0, 0x80000000u};
     G prim{0, 0};
     uint32_t oct_inv4 = ray_get_octant_inv4(ray.direction);
+    const WtRay wr = (mode & 2) ? wt_ray(ray) : WtRay{};
     float tmax = ray.tmax;
     RayHit h{INVALID32, INVALID32, INVALID32, INFINITY};
     bool is_miss = true;
@@ -1045,8 +1127,9 @@ inline void traverse_one(const CwBvhNode* nodes, uint32_t nnodes, const Triangle
             prim.y &= ~(1u << local);
             uint32_t pid = prim.x + local;
             n_tris++;
-            float t = tri_intersect(tris[pid], ray, ray.tmin, tmax);
-            if (mode == 0) {
+            float t = (mode & 2) ? tri_intersect_wt(tris[pid], ray, wr, ray.tmin, (mode & 1) ? ray.tmax : tmax)
+                                 : tri_intersect(tris[pid], ray, ray.tmin, tmax);
+            if ((mode & 1) == 0) {
                 if (t < tmax) {
                     h.primitive_id = pid;
                     h.t = t;
@@ -1076,7 +1159,7 @@ inline void traverse_one(const CwBvhNode* nodes, uint32_t nnodes, const Triangle
             uint32_t child = child_index_base + relative_index;
             const CwBvhNode& nd = nodes[child];
             n_nodes++;
-            uint32_t hitmask = intersect_node(nd, ray, tmax, oct_inv4);
+            uint32_t hitmask = intersect_node(nd, ray, tmax, oct_inv4, (mode & 2) != 0 && g_mut != MUT_WT_NODE_PLAIN);
             current = G{nd.child_base_idx, (hitmask & 0xff000000u) | (uint32_t)nd.imask};
             prim = G{nd.primitive_base_idx, hitmask & 0x00ffffffu};
         } else {
@@ -1903,6 +1986,27 @@ void orc_brute_force(const Triangle* tris, uint64_t ntris, const Ray* rays, uint
         }
         out[r] = h;
     }
+}
+
+// Brute force with the watertight test: the closest t over all triangles (first index on equal t).
+void orc_brute_force_wt(const Triangle* tris, uint64_t ntris, const Ray* rays, uint64_t nrays, RayHit* out) {
+    for (uint64_t r = 0; r < nrays; ++r) {
+        RayHit h{INVALID32, INVALID32, INVALID32, INFINITY};
+        const WtRay w = wt_ray(rays[r]);
+        for (uint64_t i = 0; i < ntris; ++i) {
+            float t = tri_intersect_wt(tris[i], rays[r], w, rays[r].tmin, rays[r].tmax);
+            if (t < h.t) {
+                h.t = t;
+                h.primitive_id = (uint32_t)i;
+            }
+        }
+        out[r] = h;
+    }
+}
+
+// The watertight test alone, for unit tests.
+float orc_tri_intersect_wt(const Triangle* tri, const Ray* ray) {
+    return tri_intersect_wt(*tri, *ray, wt_ray(*ray), ray->tmin, ray->tmax);
 }
 
 // The raw ray-triangle test (triangle.rs:35-76) for unit tests.

@@ -49,6 +49,9 @@ def lib():
                                         C.c_uint64, P, P, P]
         L.orc_cwbvh_traverse.argtypes = [P, C.c_uint64, P, P, C.c_uint64, C.c_int, P, P, P, P, C.c_int]
         L.orc_brute_force.argtypes = [P, C.c_uint64, P, C.c_uint64, P]
+        L.orc_brute_force_wt.argtypes = [P, C.c_uint64, P, C.c_uint64, P]
+        L.orc_tri_intersect_wt.restype = C.c_float
+        L.orc_tri_intersect_wt.argtypes = [P, P]
         L.orc_tri_intersect.restype = C.c_float
         L.orc_tri_intersect.argtypes = [P, P]
         L.orc_intersect_node.restype = C.c_uint32
@@ -173,17 +176,30 @@ def cw_cost_table(nodes, max_prims_per_leaf=3):
 def traverse(cw_nodes, bvh_tris, rays, mode=0, threads=1, counts=False):
     from obvhs_amd.types import RAY_HIT
     n = len(rays)
-    hits = np.zeros(n, dtype=RAY_HIT) if mode == 0 else None
-    miss = np.zeros(n, dtype=np.uint8) if mode == 1 else None
+    hits = np.zeros(n, dtype=RAY_HIT) if (mode & 1) == 0 else None  # modes 2, 3: the watertight test
+    miss = np.zeros(n, dtype=np.uint8) if (mode & 1) == 1 else None
     nv = np.zeros(n, dtype=np.uint32) if counts else None
     tv = np.zeros(n, dtype=np.uint32) if counts else None
     lib().orc_cwbvh_traverse(_ptr(np.ascontiguousarray(cw_nodes)), len(cw_nodes),
                              _ptr(np.ascontiguousarray(bvh_tris)), _ptr(np.ascontiguousarray(rays)), n, mode,
                              _ptr(hits), _ptr(miss), _ptr(nv), _ptr(tv), threads)
-    res = hits if mode == 0 else miss.astype(bool)
+    res = hits if (mode & 1) == 0 else miss.astype(bool)
     if counts:
         return res, nv, tv
     return res
+
+
+def brute_force_wt(tris, rays):
+    """Closest hit over all triangles with the watertight test (first index on equal t)."""
+    from obvhs_amd.types import RAY_HIT
+    out = np.zeros(len(rays), dtype=RAY_HIT)
+    lib().orc_brute_force_wt(_ptr(np.ascontiguousarray(tris)), len(tris), _ptr(np.ascontiguousarray(rays)),
+                             len(rays), _ptr(out))
+    return out
+
+
+def tri_intersect_wt(tri, ray):
+    return lib().orc_tri_intersect_wt(_ptr(np.ascontiguousarray(tri)), _ptr(np.ascontiguousarray(ray)))
 
 
 def brute_force(tris, rays):

@@ -155,6 +155,16 @@ def test_batch_invalid_arguments(ctx):
     assert e.value.name == "OBVHS_ERR_INVALID_ARG"
     with pytest.raises(ValueError):
         ob.build_cwbvh_batch(ctx, tris, offsets[:-1], ob.BvhBuildParams())
+    # a non-empty batch traced without its triangles: rejected before any launch
+    batch = ob.build_cwbvh_batch(ctx, tris, offsets, ob.BvhBuildParams(ploc_search_distance=14))
+    rays = ctx.upload(ob.make_rays([[0.5, 0.5, -1.0]] * 4, [[0, 0, 1]] * 4))
+    mesh = ctx.upload(np.zeros(4, dtype=np.uint32))
+    out = ob.DeviceArray(ctx, ob.RAY_HIT, 4)
+    miss = ob.DeviceArray(ctx, np.uint8, 4)
+    lib = ob.native.lib()
+    bad = ob.native.STATUS_CODE["OBVHS_ERR_INVALID_ARG"]
+    assert lib.obvhs_batch_ray_traverse(ctx.handle, batch.handle, None, rays.ptr, mesh.ptr, 4, out.ptr) == bad
+    assert lib.obvhs_batch_ray_traverse_miss(ctx.handle, batch.handle, None, rays.ptr, mesh.ptr, 4, miss.ptr) == bad
 
 
 def test_batch_full_icosphere_workload_sampled(ctx):

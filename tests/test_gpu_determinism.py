@@ -101,16 +101,20 @@ def test_rebuilds_and_refits_repeat_bit_for_bit(ctx):
     assert outs[1] == outs[0] and outs[2] == outs[0]
 
 
-def test_chunked_launches_match_one_ray_per_lane(ctx):
+@pytest.mark.parametrize("depth", [4, 8, 16])
+def test_chunked_launches_match_one_ray_per_lane(ctx, depth):
     """With the tuning table's trav_chunk set, launches of at least two rays per resident lane run as wave-owned
     ray rows with lanes refilled (k_traverse_chunk); smaller launches still run one ray per lane (k_traverse).  The
-    same rays traced both ways give identical hits and occlusion flags, and a strided sample equals the oracle."""
+    same rays traced both ways give identical hits and occlusion flags, for every LDS stack depth (4 puts the most
+    entries in the spill columns), on coherent primary rays and on incoherent bounce rays, and a strided sample
+    equals the oracle."""
     saved = ctx.tune_table()
     t = ctx.tune_table()
     t["trav_chunk"] = [1] * 4
+    t["trav_lds_depth"] = [depth] * 4
     ctx.set_tune_table(t)
     try:
-        _chunked_vs_pieces(ctx)
+        _chunked_vs_pieces(ctx, depth)
     finally:
         ctx.set_tune_table(saved)
 
@@ -129,24 +133,43 @@ def test_chunked_batched_launch_matches_one_ray_per_lane(ctx):
     print(f"[case] chunked batched launch n={len(rays)}", flush=True)
     saved = ctx.tune_table()
     try:
-        outs = []
+        outs, misses = [], []
         for chunk in (0, 1):
             t = ctx.tune_table()
             t["trav_chunk"] = [chunk] * 4
             ctx.set_tune_table(t)
-            outs.append(batch.ray_traverse(rays, ray_mesh, bt).tobytes())
+            outs.append(batch.ray_traverse(rays, ray_mesh, bt))
+            misses.append(batch.ray_traverse_miss(rays, ray_mesh, bt))
     finally:
         ctx.set_tune_table(saved)
-    assert outs[0] == outs[1]
+    assert outs[0].tobytes() == outs[1].tobytes()
+    assert np.array_equal(misses[0], misses[1])
+    # a strided sample of the chunked launch against the oracle, mesh by mesh
+    nodes, prims = batch.all_nodes(), batch.all_primitive_indices()
+    host_bt = bt.to_host()
+    sample = np.arange(0, len(rays), 101)
+    for i in range(len(meshes)):
+        sel = sample[ray_mesh[sample] == i]
+        if len(sel) == 0 or len(meshes[i]) == 0:
+            continue
+        no, nc, po, pc = batch.mesh_info(i)
+        mesh_bt = host_bt[offsets[i]:offsets[i + 1]]
+        ref = O.traverse(nodes[no:no + nc], mesh_bt, np.ascontiguousarray(rays[sel]), 0)
+        assert outs[1][sel].tobytes() == ref.tobytes(), f"mesh {i}"
+        refm = O.traverse(nodes[no:no + nc], mesh_bt, np.ascontiguousarray(rays[sel]), 1)
+        assert np.array_equal(misses[1][sel], refm), f"mesh {i} occlusion"
 
 
-def _chunked_vs_pieces(ctx):
+def _chunked_vs_pieces(ctx, depth):
     tris, rays = scenes.heightfield_workload(256, 2000, 1920, 1080, seed=4)
     cw = ob.build_cwbvh_from_tris(ctx, tris, ob.BvhBuildParams(ploc_search_distance=14, max_prims_per_leaf=3))
     bt = cw.reorder_triangles(tris)
     shadow = np.concatenate([rays, rays])  # 4.1M rays: four or more per resident lane on any current device
-    for name, rs in (("primary", rays), ("doubled", shadow)):
-        print(f"[case] chunked launch {name} n={len(rs)}", flush=True)
+    # incoherent bounce rays (deep stacks reach the spill columns), repeated to at least two per resident lane
+    bounce = scenes.bounce_rays(rays, cw.ray_traverse(ctx.upload(rays), bt), bt.to_host(), seed=5)
+    bounce = np.concatenate([bounce] * max(1, -(-(1 << 21) // max(1, len(bounce)))))
+    for name, rs in (("primary", rays), ("doubled", shadow), ("bounce", bounce)):
+        print(f"[case] chunked launch lds={depth} {name} n={len(rs)}", flush=True)
         whole = cw.ray_traverse(ctx.upload(rs), bt)
         cuts = list(range(0, len(rs), 500000)) + [len(rs)]
         parts = [cw.ray_traverse(ctx.upload(np.ascontiguousarray(rs[a:b])), bt) for a, b in zip(cuts[:-1], cuts[1:])]
@@ -155,8 +178,9 @@ def _chunked_vs_pieces(ctx):
         m_parts = [cw.ray_traverse_miss(ctx.upload(np.ascontiguousarray(rs[a:b])), bt)
                    for a, b in zip(cuts[:-1], cuts[1:])]
         assert np.array_equal(np.concatenate(m_parts), m_whole)
-    sample = np.ascontiguousarray(rays[::997])
-    ref = O.traverse(cw.nodes, tris[cw.primitive_indices], sample, 0, threads=8)
-    got = cw.ray_traverse(ctx.upload(rays), bt)[::997]
-    assert got.tobytes() == ref.tobytes()
-    assert 0.2 < np.isfinite(got["t"]).mean() < 1.0
+    for rs, lo in ((rays, 0.2), (bounce, 0.01)):
+        sample = np.ascontiguousarray(rs[::997])
+        ref = O.traverse(cw.nodes, tris[cw.primitive_indices], sample, 0, threads=8)
+        got = cw.ray_traverse(ctx.upload(rs), bt)[::997]
+        assert got.tobytes() == ref.tobytes()
+        assert lo < np.isfinite(got["t"]).mean() < 1.0
