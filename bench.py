@@ -113,30 +113,32 @@ def pmc_copy(ob, ctx):
     ctx.synchronize()
 
 
-def tune_step(ob, ctx, n_prims, step, reps=3):
-    """Chooses the traversal LDS stack depth (4, 8, 16) and launch form (one lane per ray / chunked) for the size
-    class of n_prims by timing the timed step itself with each (one warm-up, best of `reps`), and keeps the
-    fastest in the context's tuning table.  No choice changes a hit."""
+def tune_step(ob, ctx, n_prims, step, reps=3, layouts=("triangle",)):
+    """Chooses the traversal LDS stack depth (4, 8, 16), launch form (one lane per ray / chunked) and, when the step
+    takes one, the primitive layout (Triangle / RtTriangle rows, the same hits) for the size class of n_prims by
+    timing the timed step itself with each (one warm-up, best of `reps`); keeps the fastest depth and form in the
+    context's tuning table and returns the layout.  No choice changes a hit."""
     cls = ob.size_class(n_prims)
     ms_by = {}
-    for chunk in (0, 1):
-        for depth in (4, 8, 16):
-            t = ctx.tune_table()
-            t["trav_lds_depth"][cls], t["trav_chunk"][cls] = depth, chunk
-            ctx.set_tune_table(t)
-            step()
-            ms = []
-            for _ in range(reps):
-                ctx.timer_start()
-                step()
-                ms.append(ctx.timer_stop())
-            ms_by[(depth, chunk)] = min(ms)
+    for layout in layouts:
+        for chunk in (0, 1):
+            for depth in (4, 8, 16):
+                t = ctx.tune_table()
+                t["trav_lds_depth"][cls], t["trav_chunk"][cls] = depth, chunk
+                ctx.set_tune_table(t)
+                step(layout) if len(layouts) > 1 else step()
+                ms = []
+                for _ in range(reps):
+                    ctx.timer_start()
+                    step(layout) if len(layouts) > 1 else step()
+                    ms.append(ctx.timer_stop())
+                ms_by[(depth, chunk, layout)] = min(ms)
     best = min(ms_by, key=ms_by.get)
     t = ctx.tune_table()
-    t["trav_lds_depth"][cls], t["trav_chunk"][cls] = best
+    t["trav_lds_depth"][cls], t["trav_chunk"][cls] = best[0], best[1]
     ctx.set_tune_table(t)
-    return {"traversal_lds_depth": best[0], "traversal_chunked": best[1], "size_class": cls,
-            "step_ms_by_lds_depth_and_chunked": {f"{d}/{c}": round(v, 4) for (d, c), v in ms_by.items()}}
+    return {"traversal_lds_depth": best[0], "traversal_chunked": best[1], "layout": best[2], "size_class": cls,
+            "step_ms_by_lds_depth_chunked_layout": {f"{d}/{c}/{l}": round(v, 4) for (d, c, l), v in ms_by.items()}}
 
 
 def kernel_name(ctx, ob, n_prims, mode="closest"):
@@ -476,16 +478,29 @@ def bench_large(args, ob, ctx):
     hits = ob.DeviceArray(ctx, ob.RAY_HIT, nr)
     miss = ob.DeviceArray(ctx, np.uint8, ns)
 
-    def closest():
-        ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, cw.handle, d_bvh_tris.ptr, d_rays.ptr, nr,
-                                                     hits.ptr), ctx.handle)
+    d_rt = ob.rt_triangles(ctx, d_bvh_tris)  # RtTriangle rows (the crate's demoscene layout): the same hits
+    layout = {"value": args.layout if args.layout != "auto" else "triangle"}
+
+    def closest(which=None):
+        if (which or layout["value"]) == "rt":
+            ob.native.check(lib.obvhs_cwbvh_ray_traverse_rt(ctx.handle, cw.handle, d_rt.ptr, d_rays.ptr, nr,
+                                                            hits.ptr), ctx.handle)
+        else:
+            ob.native.check(lib.obvhs_cwbvh_ray_traverse(ctx.handle, cw.handle, d_bvh_tris.ptr, d_rays.ptr, nr,
+                                                         hits.ptr), ctx.handle)
 
     def anyhit():
-        ob.native.check(lib.obvhs_cwbvh_ray_traverse_miss(ctx.handle, cw.handle, d_bvh_tris.ptr, d_shadow.ptr, ns,
-                                                          miss.ptr), ctx.handle)
+        if layout["value"] == "rt":
+            ob.native.check(lib.obvhs_cwbvh_ray_traverse_miss_rt(ctx.handle, cw.handle, d_rt.ptr, d_shadow.ptr, ns,
+                                                                 miss.ptr), ctx.handle)
+        else:
+            ob.native.check(lib.obvhs_cwbvh_ray_traverse_miss(ctx.handle, cw.handle, d_bvh_tris.ptr, d_shadow.ptr,
+                                                              ns, miss.ptr), ctx.handle)
 
     if not args.no_tune and not args.lds_depth and args.trav_chunk < 0:
-        tuning["closest"] = tune_step(ob, ctx, leaves, closest)
+        tuning["closest"] = tune_step(ob, ctx, leaves, closest,
+                                      layouts=("triangle", "rt") if args.layout == "auto" else (layout["value"],))
+        layout["value"] = tuning["closest"]["layout"]
     if args.pmc_probe == "large":
         pmc_copy(ob, ctx)
         closest()
@@ -513,7 +528,7 @@ def bench_large(args, ob, ctx):
         closest()
         kms.append(ctx.timer_stop())
     kernel_ms = float(np.mean(kms))
-    closest_kernel = kernel_name(ctx, ob, leaves)
+    closest_kernel = kernel_name(ctx, ob, leaves, "closest, RtTriangle" if layout["value"] == "rt" else "closest")
 
     for _ in range(args.warmup):
         anyhit()
@@ -546,7 +561,7 @@ def bench_large(args, ob, ctx):
         "value": nr * args.steps / wall / 1e6, "wall": wall,
         "workload": f"large scene: heightfield {args.large_grid}x{args.large_grid}x2 + {args.large_thin} thin = {n} "
                     f"tris ({leaves} leaves after pre-splits capped at 1.3n), reinsertion 0.05 x [1, 1/3]; "
-                    f"{nr} incoherent closest-hit rays per step",
+                    f"{nr} incoherent closest-hit rays per step ({layout['value']} primitives)",
         "roofline": roofline(algo_bytes, kernel_ms, traffic_per_launch("large"), closest_kernel,
                              node_tests=node_tests, tri_tests=tri_tests,
                              bytes_model="40 B per ray + 80 B per node test + 36 B per triangle test"),
@@ -925,6 +940,8 @@ def main():
     ap.add_argument("--trav-chunk", type=int, default=-1, choices=(-1, 0, 1),
                     help="set chunked single-BVH traversal launches (0 or 1) for every size class instead of "
                          "measuring it (-1)")
+    ap.add_argument("--layout", choices=("auto", "triangle", "rt"), default="auto",
+                    help="primitive rows of the large scene's traversal: Triangle, RtTriangle, or timed (auto)")
     ap.add_argument("--pmc-probe", choices=("large", "heightfield"), default=None,
                     help="one copy kernel, then the leg's traversal launches of one step, then exit (counter runs)")
     ap.add_argument("--traffic-from", nargs=3, metavar=("LEG", "FETCH_CSV", "WRITE_CSV"),

@@ -37,6 +37,7 @@ typedef enum obvhs_status {
 typedef struct obvhs_vec3a { float x, y, z, w; } obvhs_vec3a;                       /* glam::Vec3A, 16 B */
 typedef struct obvhs_aabb { obvhs_vec3a min, max; } obvhs_aabb;                     /* aabb.rs:11-16, 32 B */
 typedef struct obvhs_triangle { obvhs_vec3a v0, v1, v2; } obvhs_triangle;           /* triangle.rs:8-13, 48 B */
+typedef struct obvhs_rt_triangle { obvhs_vec3a v0, e1, e2, ng; } obvhs_rt_triangle; /* rt_triangle.rs:147-160, 64 B */
 typedef struct obvhs_ray {                                                          /* ray.rs:15-30, 64 B */
     obvhs_vec3a origin, direction, inv_direction;
     float tmin, tmax;
@@ -294,6 +295,19 @@ obvhs_status obvhs_cwbvh_ray_traverse_watertight(obvhs_context* ctx, const obvhs
 obvhs_status obvhs_cwbvh_ray_traverse_miss_watertight(obvhs_context* ctx, const obvhs_cwbvh* bvh,
                                                       const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
                                                       uint32_t n, uint8_t* d_miss);
+/* RtTriangle primitives (rt_triangle.rs:147-239): the crate's intersection-optimised triangle (v0, e1 = v0 - v1,
+ * e2 = v2 - v0, ng = e1 x e2, one Vec3A each).  obvhs_rt_triangles_from_triangles replaces RtTriangle::from(&Triangle)
+ * (rt_triangle.rs:162-181) for n triangles; the traversals replace CwBvh::ray_traverse / ray_traverse_miss with the
+ * closure |ray, id| rt_tris[id].intersect(ray) (rt_triangle.rs:200-239), which computes the same bits as
+ * Triangle::intersect, so the hits equal obvhs_cwbvh_ray_traverse's. */
+obvhs_status obvhs_rt_triangles_from_triangles(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
+                                               obvhs_rt_triangle* d_out);
+obvhs_status obvhs_cwbvh_ray_traverse_rt(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                         const obvhs_rt_triangle* d_bvh_rt_tris, const obvhs_ray* d_rays, uint32_t n,
+                                         obvhs_ray_hit* d_hits);
+obvhs_status obvhs_cwbvh_ray_traverse_miss_rt(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                              const obvhs_rt_triangle* d_bvh_rt_tris, const obvhs_ray* d_rays,
+                                              uint32_t n, uint8_t* d_miss);
 
 /* ---- batched build / traversal (many independent meshes in shared launches) ----
  * No single reference entry point: equivalent to calling build_cwbvh_from_tris (cwbvh/builder.rs:20-85) once per

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import native
 from .native import BuildParams, BuildStats, ObvhsError, check, lib
-from .types import (AABB, BVH2_NODE, BVH2_NODE_SMALL, CWBVH_NODE, RAY, RAY_HIT, TRIANGLE)
+from .types import (AABB, BVH2_NODE, BVH2_NODE_SMALL, CWBVH_NODE, RAY, RAY_HIT, RT_TRIANGLE, TRIANGLE)
 
 
 class PlocSearchDistance(enum.IntEnum):
@@ -483,6 +483,21 @@ class CwBvh:
               self.ctx.handle)
         return hits.to_host()
 
+    def ray_traverse_rt(self, rays, bvh_rt_tris, miss=False):
+        """ray_traverse / ray_traverse_miss with RtTriangle primitives (rt_triangle.rs:200-239, the crate's
+        demoscene layout), bvh_rt_tris in primitive_indices order (rt_triangles()); the same hits as ray_traverse."""
+        dr, n = _as_device(self.ctx, rays, RAY)
+        dt, _ = _as_device(self.ctx, bvh_rt_tris, RT_TRIANGLE)
+        if miss:
+            out = DeviceArray(self.ctx, np.uint8, n)
+            check(lib().obvhs_cwbvh_ray_traverse_miss_rt(self.ctx.handle, self.handle, dt.ptr, dr.ptr, n, out.ptr),
+                  self.ctx.handle)
+            return out.to_host().astype(bool)
+        hits = DeviceArray(self.ctx, RAY_HIT, n)
+        check(lib().obvhs_cwbvh_ray_traverse_rt(self.ctx.handle, self.handle, dt.ptr, dr.ptr, n, hits.ptr),
+              self.ctx.handle)
+        return hits.to_host()
+
     @property
     def bvh2(self):
         """The BVH2 this CwBvh was built from by build_cwbvh_from_tris (borrowed; None for conversions)."""
@@ -715,6 +730,14 @@ class Server:
         check(lib().obvhs_server_get_stats(self.handle, C.byref(st)))
         return dict(launches=st.launches, rays=st.rays, segments=st.segments, resident_bytes=st.resident_bytes,
                     scenes=st.scenes)
+
+
+def rt_triangles(ctx, triangles):
+    """RtTriangle::from(&Triangle) for every triangle (rt_triangle.rs:162-181): a device RtTriangle array."""
+    d, n = _as_device(ctx, triangles, TRIANGLE)
+    out = DeviceArray(ctx, RT_TRIANGLE, n)
+    check(lib().obvhs_rt_triangles_from_triangles(ctx.handle, d.ptr, n, out.ptr), ctx.handle)
+    return out
 
 
 def split_aabbs_preset(ctx, triangles, max_total=0, with_stats=False):

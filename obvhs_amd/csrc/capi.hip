@@ -646,6 +646,45 @@ obvhs_status obvhs_cwbvh_ray_traverse_miss_watertight(obvhs_context* ctx, const 
     return traverse_watertight(ctx, bvh, d_bvh_tris, d_rays, n, 3, nullptr, d_miss);
 }
 
+obvhs_status obvhs_rt_triangles_from_triangles(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
+                                               obvhs_rt_triangle* d_out) {
+    if (!ctx || (n && (!d_tris || !d_out))) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(launch_rt_triangles(d_tris, n, d_out, ctx->stream), "rt triangles");
+    return OBVHS_OK;
+}
+
+// ray_traverse / ray_traverse_miss with RtTriangle primitives (mode 4: closest hit, 5: occlusion).
+static obvhs_status traverse_rt(obvhs_context* ctx, const obvhs_cwbvh* bvh, const obvhs_rt_triangle* d_bvh_rt_tris,
+                                const obvhs_ray* d_rays, uint32_t n, int mode, obvhs_ray_hit* d_hits,
+                                uint8_t* d_miss) {
+    if (!ctx || !bvh || (n && (!d_rays || (mode == 4 ? !d_hits : !d_miss)))) return OBVHS_ERR_INVALID_ARG;
+    if (bvh->node_count && !d_bvh_rt_tris) return OBVHS_ERR_INVALID_ARG;
+    TravScratch ts;
+    if (n) {
+        const obvhs_status ss = traverse_scratch(ctx, ts, n);
+        if (ss != OBVHS_OK) return ss;
+    }
+    ts.lds_depth = lds_depth_for(ctx, bvh->prim_count);
+    ts.chunked = chunk_for(ctx, bvh->prim_count);
+    CTX_TRY(launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count,
+                            reinterpret_cast<const obvhs_triangle*>(d_bvh_rt_tris), d_rays, n, mode, d_hits, d_miss,
+                            nullptr, nullptr, ts, ctx->stream),
+            "traverse rt");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_cwbvh_ray_traverse_rt(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                         const obvhs_rt_triangle* d_bvh_rt_tris, const obvhs_ray* d_rays, uint32_t n,
+                                         obvhs_ray_hit* d_hits) {
+    return traverse_rt(ctx, bvh, d_bvh_rt_tris, d_rays, n, 4, d_hits, nullptr);
+}
+
+obvhs_status obvhs_cwbvh_ray_traverse_miss_rt(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                              const obvhs_rt_triangle* d_bvh_rt_tris, const obvhs_ray* d_rays,
+                                              uint32_t n, uint8_t* d_miss) {
+    return traverse_rt(ctx, bvh, d_bvh_rt_tris, d_rays, n, 5, nullptr, d_miss);
+}
+
 // The CWBVH exponent threshold table (host copy; built from the host libm on first use, no device needed).
 obvhs_status obvhs_log2_threshold_table(uint32_t* out256) {
     if (!out256) return OBVHS_ERR_INVALID_ARG;

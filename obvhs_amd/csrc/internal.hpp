@@ -170,6 +170,7 @@ struct TravScratch {
 };
 // Spill columns (and the lane count) for a traversal launch of n rays.
 obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts, uint64_t n);
+hipError_t launch_rt_triangles(const obvhs_triangle* tris, uint32_t n, obvhs_rt_triangle* out, hipStream_t stream);
 hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
                            const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,
                            uint32_t* node_visits, uint32_t* tri_tests, const TravScratch& ts, hipStream_t stream);

@@ -112,12 +112,12 @@ __device__ __forceinline__ uint32_t intersect_node(const uint4& q0, const uint4&
     return hit_mask;
 }
 
-// Triangle::intersect (triangle.rs:35-76); glam dot = (x*x'+y*y')+z*z', cross unfused.
-__device__ __forceinline__ float tri_intersect(const float4& v0, const float4& v1, const float4& v2,
-                                               const RayData& r, float tmin, float tmax) {
-    const float e1x = v0.x - v1.x, e1y = v0.y - v1.y, e1z = v0.z - v1.z;
-    const float e2x = v2.x - v0.x, e2y = v2.y - v0.y, e2z = v2.z - v0.z;
-    const float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+// RtTriangle::intersect (rt_triangle.rs:200-239) on v0, e1 = v0 - v1, e2 = v2 - v0, ng = e1 x e2; glam
+// dot = (x*x'+y*y')+z*z', cross unfused.  Triangle::intersect (triangle.rs:35-76) is the same arithmetic after
+// computing e1, e2 and ng from the vertices (tri_intersect below), so both give the same bits.
+__device__ __forceinline__ float tri_intersect_rt(const float4& v0, float e1x, float e1y, float e1z, float e2x,
+                                                  float e2y, float e2z, float nx, float ny, float nz,
+                                                  const RayData& r, float tmin, float tmax) {
     const float cx = v0.x - r.ox, cy = v0.y - r.oy, cz = v0.z - r.oz;
     const float rx = r.dy * cz - r.dz * cy, ry = r.dz * cx - r.dx * cz, rz = r.dx * cy - r.dy * cx;
     const float inv_det = 1.0f / ((nx * r.dx + ny * r.dy) + nz * r.dz);
@@ -130,6 +130,23 @@ __device__ __forceinline__ float tri_intersect(const float4& v0, const float4& v
         if (t >= tmin && t <= tmax) return t;
     }
     return INFINITY;
+}
+
+// The RtTriangle fields of a Triangle (RtTriangle::new, rt_triangle.rs:170-180).
+__device__ __forceinline__ void rt_fields(const float4& v0, const float4& v1, const float4& v2, float& e1x,
+                                          float& e1y, float& e1z, float& e2x, float& e2y, float& e2z, float& nx,
+                                          float& ny, float& nz) {
+    e1x = v0.x - v1.x, e1y = v0.y - v1.y, e1z = v0.z - v1.z;
+    e2x = v2.x - v0.x, e2y = v2.y - v0.y, e2z = v2.z - v0.z;
+    nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+}
+
+// Triangle::intersect (triangle.rs:35-76).
+__device__ __forceinline__ float tri_intersect(const float4& v0, const float4& v1, const float4& v2,
+                                               const RayData& r, float tmin, float tmax) {
+    float e1x, e1y, e1z, e2x, e2y, e2z, nx, ny, nz;
+    rt_fields(v0, v1, v2, e1x, e1y, e1z, e2x, e2y, e2z, nx, ny, nz);
+    return tri_intersect_rt(v0, e1x, e1y, e1z, e2x, e2y, e2z, nx, ny, nz, r, tmin, tmax);
 }
 
 // ---- watertight mode (no counterpart in the reference: an extra primitive test beside Triangle::intersect) ----
@@ -194,10 +211,17 @@ __device__ __forceinline__ float tri_intersect_wt(const float4& v0, const float4
 }
 
 // MODE of the traversal kernels: bit 0 = occlusion (ray_traverse_miss) instead of closest hit, bit 1 = the
-// watertight primitive test instead of Triangle::intersect.
+// watertight primitive test instead of Triangle::intersect, bit 2 = the primitives are RtTriangles (four 16-byte
+// rows v0, e1, e2, ng, rt_triangle.rs:147-160) tested with RtTriangle::intersect (no watertight form).
 template <int MODE>
-__device__ __forceinline__ float prim_test(const float4& v0, const float4& v1, const float4& v2, const RayData& r,
+__device__ __forceinline__ float prim_test(const float4* __restrict__ tris, uint32_t pid, const RayData& r,
                                            const WtRay& w, float tmin, float tmax) {
+    if (MODE & 4) {
+        const float4 v0 = tris[4 * (size_t)pid], e1 = tris[4 * (size_t)pid + 1], e2 = tris[4 * (size_t)pid + 2],
+                     ng = tris[4 * (size_t)pid + 3];
+        return tri_intersect_rt(v0, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z, ng.x, ng.y, ng.z, r, tmin, tmax);
+    }
+    const float4 v0 = tris[3 * (size_t)pid], v1 = tris[3 * (size_t)pid + 1], v2 = tris[3 * (size_t)pid + 2];
     if (MODE & 2) return tri_intersect_wt(v0, v1, v2, r, w, tmin, tmax);
     return tri_intersect(v0, v1, v2, r, tmin, tmax);
 }
@@ -299,17 +323,16 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             prim.y &= ~(1u << local);
             const uint32_t pid = prim.x + local;
             if (pid >= prim_count) continue;  // malformed input only: never index past the triangle buffer
-            const float4 v0 = tris[3 * (size_t)pid], v1 = tris[3 * (size_t)pid + 1], v2 = tris[3 * (size_t)pid + 2];
             if (COUNT) tv++;
             if ((MODE & 1) == 0) {
-                const float t = prim_test<MODE>(v0, v1, v2, r, wr, r.tmin, tmax);
+                const float t = prim_test<MODE>(tris, pid, r, wr, r.tmin, tmax);
                 if (t < tmax) {
                     hit_id = pid;
                     hit_t = t;
                     tmax = t;
                 }
             } else {
-                const float t = prim_test<MODE>(v0, v1, v2, r, wr, r.tmin, r.tmax);
+                const float t = prim_test<MODE>(tris, pid, r, wr, r.tmin, r.tmax);
                 if (t < r.tmax) {
                     is_miss = false;
                     stop = true;
@@ -454,16 +477,15 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             prim.y &= ~(1u << local);
             const uint32_t pid = prim.x + local;
             if (pid >= prim_count) continue;  // malformed input only: never index past the triangle buffer
-            const float4 v0 = tris[3 * (size_t)pid], v1 = tris[3 * (size_t)pid + 1], v2 = tris[3 * (size_t)pid + 2];
             if ((MODE & 1) == 0) {
-                const float t = prim_test<MODE>(v0, v1, v2, r, wr, r.tmin, tmax);
+                const float t = prim_test<MODE>(tris, pid, r, wr, r.tmin, tmax);
                 if (t < tmax) {
                     hit_id = pid;
                     hit_t = t;
                     tmax = t;
                 }
             } else {
-                const float t = prim_test<MODE>(v0, v1, v2, r, wr, r.tmin, r.tmax);
+                const float t = prim_test<MODE>(tris, pid, r, wr, r.tmin, r.tmax);
                 if (t < r.tmax) {
                     is_miss = false;
                     done = true;
@@ -620,7 +642,29 @@ __global__ void __launch_bounds__(TRAV_THREADS)
     }
 }
 
+// RtTriangle::from(&Triangle) (rt_triangle.rs:162-181) for every triangle: v0, e1 = v0 - v1, e2 = v2 - v0,
+// ng = e1 x e2, each a Vec3A row (w = 0).
+__global__ void __launch_bounds__(256) k_rt_triangles(const float4* __restrict__ tris, uint32_t n,
+                                                      float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 v0 = tris[3 * (size_t)i], v1 = tris[3 * (size_t)i + 1], v2 = tris[3 * (size_t)i + 2];
+    float e1x, e1y, e1z, e2x, e2y, e2z, nx, ny, nz;
+    rt_fields(v0, v1, v2, e1x, e1y, e1z, e2x, e2y, e2z, nx, ny, nz);
+    out[4 * (size_t)i] = make_float4(v0.x, v0.y, v0.z, 0.0f);
+    out[4 * (size_t)i + 1] = make_float4(e1x, e1y, e1z, 0.0f);
+    out[4 * (size_t)i + 2] = make_float4(e2x, e2y, e2z, 0.0f);
+    out[4 * (size_t)i + 3] = make_float4(nx, ny, nz, 0.0f);
+}
+
 }  // namespace
+
+hipError_t launch_rt_triangles(const obvhs_triangle* tris, uint32_t n, obvhs_rt_triangle* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    k_rt_triangles<<<(n + 255) / 256, 256, 0, stream>>>(reinterpret_cast<const float4*>(tris), n,
+                                                        reinterpret_cast<float4*>(out));
+    return hipGetLastError();
+}
 
 // Launches k_traverse<MODE, COUNT, SRC, depth> with the LDS stack depth the launch was tuned for (4, 8 or 16).
 template <int MODE, bool COUNT, int SRC, class... A>
@@ -676,7 +720,7 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
     const TraceSegment* no_segs = nullptr;
     // Chunked launches (when the tuning table asks for them: incoherent rays) need at least two rays per resident
     // lane; each lane then gets that many, refilled in the wave.
-    if (mode > 3 || (node_visits && mode != 0)) return hipErrorInvalidValue;
+    if (mode > 5 || (node_visits && mode != 0)) return hipErrorInvalidValue;
     const uint32_t rows = node_visits ? 0u : chunk_rows(n, ts);
     if (rows) {
         const uint32_t per_wave = rows * TRAV_THREADS;
@@ -687,7 +731,9 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
         if (mode == 0) OBVHS_CHUNK(0);
         else if (mode == 1) OBVHS_CHUNK(1);
         else if (mode == 2) OBVHS_CHUNK(2);
-        else OBVHS_CHUNK(3);
+        else if (mode == 3) OBVHS_CHUNK(3);
+        else if (mode == 4) OBVHS_CHUNK(4);
+        else OBVHS_CHUNK(5);
 #undef OBVHS_CHUNK
         return hipGetLastError();
     }
@@ -698,7 +744,9 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
     else if (mode == 0) OBVHS_LANES(0, false);
     else if (mode == 1) OBVHS_LANES(1, false);
     else if (mode == 2) OBVHS_LANES(2, false);
-    else OBVHS_LANES(3, false);
+    else if (mode == 3) OBVHS_LANES(3, false);
+    else if (mode == 4) OBVHS_LANES(4, false);
+    else OBVHS_LANES(5, false);
 #undef OBVHS_LANES
     return hipGetLastError();
 }

@@ -124,6 +124,9 @@ SIGNATURES = {
     "obvhs_cwbvh_ray_traverse_counted": (INT, [P, P, P, P, U32, P, P, P]),
     "obvhs_cwbvh_ray_traverse_watertight": (INT, [P, P, P, P, U32, P]),
     "obvhs_cwbvh_ray_traverse_miss_watertight": (INT, [P, P, P, P, U32, P]),
+    "obvhs_rt_triangles_from_triangles": (INT, [P, P, U32, P]),
+    "obvhs_cwbvh_ray_traverse_rt": (INT, [P, P, P, P, U32, P]),
+    "obvhs_cwbvh_ray_traverse_miss_rt": (INT, [P, P, P, P, U32, P]),
     "obvhs_log2_threshold_table": (INT, [P]),
     "obvhs_batch_build_cwbvh": (INT, [P, P, P, U32, C.POINTER(BuildParams), P, C.POINTER(P)]),
     "obvhs_batch_info": (INT, [P, PU32, PU32, PU32, PU32]),

@@ -1,6 +1,7 @@
 """numpy views of the obvhs plain-data types (byte-identical to the crate's repr(C) layouts).
 
-aabb.rs:11-16 (Aabb, 32 B), triangle.rs:8-13 (Triangle, 48 B), ray.rs:15-30 (Ray, 64 B), ray.rs:63-70 (RayHit, 16 B),
+aabb.rs:11-16 (Aabb, 32 B), triangle.rs:8-13 (Triangle, 48 B), rt_triangle.rs:147-160 (RtTriangle, 64 B),
+ray.rs:15-30 (Ray, 64 B), ray.rs:63-70 (RayHit, 16 B),
 bvh2/node.rs:38-66 (Bvh2Node, 48 B), bvh2/node.rs:12-36 (small Bvh2Node, 32 B), cwbvh/node.rs:12-54 (CwBvhNode, 80 B).
 """
 import numpy as np
@@ -10,6 +11,7 @@ INVALID_ID = 0xFFFFFFFF  # ray.rs:72
 VEC3A = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("w", "<f4")])
 AABB = np.dtype([("min", "<f4", (4,)), ("max", "<f4", (4,))])
 TRIANGLE = np.dtype([("v0", "<f4", (4,)), ("v1", "<f4", (4,)), ("v2", "<f4", (4,))])
+RT_TRIANGLE = np.dtype([("v0", "<f4", (4,)), ("e1", "<f4", (4,)), ("e2", "<f4", (4,)), ("ng", "<f4", (4,))])
 RAY = np.dtype([("origin", "<f4", (4,)), ("direction", "<f4", (4,)), ("inv_direction", "<f4", (4,)),
                 ("tmin", "<f4"), ("tmax", "<f4"), ("pad", "<f4", (2,))])
 RAY_HIT = np.dtype([("primitive_id", "<u4"), ("geometry_id", "<u4"), ("instance_id", "<u4"), ("t", "<f4")])
@@ -24,6 +26,7 @@ CWBVH_NODE = np.dtype([("p", "<f4", (3,)), ("e", "u1", (3,)), ("imask", "u1"), (
                        ("child_min_z", "u1", (8,)), ("child_max_z", "u1", (8,))])
 
 assert AABB.itemsize == 32 and TRIANGLE.itemsize == 48 and RAY.itemsize == 64 and RAY_HIT.itemsize == 16
+assert RT_TRIANGLE.itemsize == 64
 assert BVH2_NODE.itemsize == 48 and BVH2_NODE_SMALL.itemsize == 32 and CWBVH_NODE.itemsize == 80
 
 F32_EPS = np.float32(1.1920929e-07)  # f32::EPSILON
