@@ -308,6 +308,16 @@ obvhs_status obvhs_cwbvh_ray_traverse_rt(obvhs_context* ctx, const obvhs_cwbvh* 
 obvhs_status obvhs_cwbvh_ray_traverse_miss_rt(obvhs_context* ctx, const obvhs_cwbvh* bvh,
                                               const obvhs_rt_triangle* d_bvh_rt_tris, const obvhs_ray* d_rays,
                                               uint32_t n, uint8_t* d_miss);
+/* Closest-hit traversal that allocates nothing (no counterpart in the reference; for callers that cannot allocate:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same traversal and hits
+ * as obvhs_cwbvh_ray_traverse, but its stack spill columns live in a static device array of the library (96 MiB per
+ * device, held until the process ends: 2^19 lanes x 24 entries x 8 B; the LDS part keeps at least 8 entries) whose
+ * address obvhs_context_create looks up; at most 2^31 rays per call (OBVHS_ERR_CAPACITY beyond), lanes stride over
+ * the rays.  Every call on a device shares that array: calls that overlap on one device (other streams or threads)
+ * race; nothing in it needs clearing between calls (a spill entry is written before it is read). */
+obvhs_status obvhs_cwbvh_ray_traverse_noalloc(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                              const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays, uint32_t n,
+                                              obvhs_ray_hit* d_hits);
 
 /* ---- batched build / traversal (many independent meshes in shared launches) ----
  * No single reference entry point: equivalent to calling build_cwbvh_from_tris (cwbvh/builder.rs:20-85) once per

@@ -91,6 +91,11 @@ obvhs_status obvhs_context_create(int device, void* hip_stream, obvhs_context** 
         obvhs_context_destroy(ctx);
         return s;
     }
+    const obvhs_status ns = noalloc_scratch_init(ctx);
+    if (ns != OBVHS_OK) {
+        obvhs_context_destroy(ctx);
+        return ns;
+    }
     *out = ctx;
     return OBVHS_OK;
 }

@@ -81,6 +81,7 @@ struct obvhs_context {
     void* pinned = nullptr;  // small pinned host staging area
     bool keep_trace = false; // keep Morton codes and the sort permutation of each build (parity tests)
     uint32_t trav_lanes = 0;  // persistent traversal grid size (0 = not yet queried)
+    void* noalloc_spill = nullptr;  // this device's address of the static traversal scratch (traverse.hip)
     obvhs_tune_table tune{{14u, 14u, 14u, 14u}, {8u, 8u, 8u, 8u}, {8u, 8u, 8u, 8u}, {0u, 0u, 0u, 0u}};
     obvhs::DevBuf tune_scratch;  // working memory of the tuning runs
     std::string last_error;
@@ -170,6 +171,7 @@ struct TravScratch {
 };
 // Spill columns (and the lane count) for a traversal launch of n rays.
 obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts, uint64_t n);
+obvhs_status noalloc_scratch_init(obvhs_context* ctx);
 hipError_t launch_rt_triangles(const obvhs_triangle* tris, uint32_t n, obvhs_rt_triangle* out, hipStream_t stream);
 hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
                            const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,

@@ -15,6 +15,7 @@
 // the rays) and each lane owns one spill column.  Nodes are fetched as five 16-byte loads, triangles (in
 // primitive_indices order) as three.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 #include "cwdp.hpp"
@@ -831,6 +832,76 @@ obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts, uint64_t n) {
 }
 
 }  // namespace obvhs
+
+// ---- closest-hit traversal that allocates nothing (obvhs_cwbvh_ray_traverse_noalloc) ----
+// Its spill columns live in this array, which the library's code object holds on every device it is loaded on until
+// the process ends: NOALLOC_LANES columns of STACK_SIZE - NOALLOC_MIN_LDS entries (8 B each) = 96 MiB.  Host code
+// never passes `scratch` itself to a kernel (on the host the name is only a stand-in for the symbol): obvhs_context_create
+// looks up each device's address of it once with hipGetSymbolAddress and every launch takes that address.
+namespace {
+constexpr uint32_t NOALLOC_LANES = 1u << 19;  // 256 CUs x 32 waves x 64 lanes: the MI355X's resident lanes
+constexpr uint32_t NOALLOC_MIN_LDS = 8;       // LDS stack entries at least (the rest of the 32 spill)
+constexpr uint64_t NOALLOC_MAX_RAYS = 1ull << 31;  // lanes stride over the rays; the ray index stays below 2^32
+constexpr size_t NOALLOC_ENTRIES = (size_t)NOALLOC_LANES * (obvhs::STACK_SIZE - NOALLOC_MIN_LDS);
+constexpr int NOALLOC_MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) uint2 scratch[NOALLOC_ENTRIES];
+
+namespace obvhs {
+
+namespace {
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+// Called by obvhs_context_create with the context's device current: the device's address of `scratch`, looked up
+// once per device and kept for every later context on it.
+obvhs_status noalloc_scratch_init(obvhs_context* ctx) {
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) return OBVHS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[ctx->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return set_hip_error(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                         "static traversal scratch address");
+        g_noalloc_ptr[ctx->device] = p;
+    }
+    ctx->noalloc_spill = g_noalloc_ptr[ctx->device];
+    if (ctx->trav_lanes == 0) {
+        hipDeviceProp_t prop;
+        const hipError_t e = hipGetDeviceProperties(&prop, ctx->device);
+        if (e != hipSuccess) return set_hip_error(ctx, e, "device properties");
+        ctx->trav_lanes = (uint32_t)std::max(1, prop.multiProcessorCount) * 32u * TRAV_THREADS;
+    }
+    return OBVHS_OK;
+}
+
+}  // namespace obvhs
+
+extern "C" obvhs_status obvhs_cwbvh_ray_traverse_noalloc(obvhs_context* ctx, const obvhs_cwbvh* bvh,
+                                                         const obvhs_triangle* d_bvh_tris, const obvhs_ray* d_rays,
+                                                         uint32_t n, obvhs_ray_hit* d_hits) {
+    using namespace obvhs;
+    if (!ctx || !bvh || (n && (!d_rays || !d_hits))) return OBVHS_ERR_INVALID_ARG;
+    if (bvh->node_count && !d_bvh_tris) return OBVHS_ERR_INVALID_ARG;
+    if ((uint64_t)n > NOALLOC_MAX_RAYS) {
+        ctx->last_error = "obvhs_cwbvh_ray_traverse_noalloc: at most 2^31 rays per call";
+        return OBVHS_ERR_CAPACITY;
+    }
+    if (!ctx->noalloc_spill) return OBVHS_ERR_DEVICE;
+    if (n == 0) return OBVHS_OK;
+    TravScratch ts;
+    ts.spill = ctx->noalloc_spill;
+    ts.lanes = NOALLOC_LANES;
+    ts.resident = std::min<uint32_t>(ctx->trav_lanes, NOALLOC_LANES);
+    ts.lds_depth = std::max<uint32_t>(lds_depth_for(ctx, bvh->prim_count), NOALLOC_MIN_LDS);
+    ts.chunked = chunk_for(ctx, bvh->prim_count);
+    const hipError_t e = launch_traverse(bvh->nodes.as<CwNode>(), bvh->node_count, bvh->prim_count, d_bvh_tris,
+                                         d_rays, n, 0, d_hits, nullptr, nullptr, nullptr, ts, ctx->stream);
+    if (e != hipSuccess) return set_hip_error(ctx, e, "traverse noalloc");
+    return OBVHS_OK;
+}
 
 namespace obvhs {
 

@@ -127,6 +127,7 @@ SIGNATURES = {
     "obvhs_rt_triangles_from_triangles": (INT, [P, P, U32, P]),
     "obvhs_cwbvh_ray_traverse_rt": (INT, [P, P, P, P, U32, P]),
     "obvhs_cwbvh_ray_traverse_miss_rt": (INT, [P, P, P, P, U32, P]),
+    "obvhs_cwbvh_ray_traverse_noalloc": (INT, [P, P, P, P, U32, P]),
     "obvhs_log2_threshold_table": (INT, [P]),
     "obvhs_batch_build_cwbvh": (INT, [P, P, P, U32, C.POINTER(BuildParams), P, C.POINTER(P)]),
     "obvhs_batch_info": (INT, [P, PU32, PU32, PU32, PU32]),
