@@ -42,25 +42,12 @@ constexpr uint32_t MAX_RADIUS = 32;
 // Passes enqueued between host reads of the cluster count (1.29M-triangle build: 4 -> 1.97 ms, 8 -> 1.92, 16 -> 1.93).
 constexpr int PASSES_PER_SYNC = 8;
 
-// Device-wide barrier state of k_ploc_mid.
-struct GridSync {
-    uint32_t arrive;
-    uint32_t gen;
-    uint32_t timeout;  // a barrier that waited too long (cannot happen with a cooperative launch): all workgroups
-    uint32_t slot;     // exit and the build reports a device error; `slot` is the final state slot
-};
-
 struct Misc {
     uint32_t bounds[6];  // order keys: min x,y,z (atomicMin) then max x,y,z (atomicMax)
     uint32_t err;
     uint32_t pad;
     PlocState state[2];
-    GridSync gs;
 };
-
-__global__ void k_grid_sync_reset(Misc* misc) {
-    if (threadIdx.x == 0) misc->gs = GridSync{0u, 0u, 0u, 0u};
-}
 
 __global__ void k_init_misc(Misc* misc, uint32_t n, uint32_t* hist) {
     if (threadIdx.x == 0) {
@@ -486,146 +473,6 @@ __global__ void __launch_bounds__(T256) k_ploc_compact(const Node32* __restrict_
                  map_len, &misc->err, we, wm);
 }
 
-// ---- persistent passes for the middle of the build ----
-// Between MID_MAX and TAIL_MAX clusters a pass is little work in four launches; k_ploc_mid runs all of these passes
-// in one cooperative launch (every workgroup resident) with device-wide barriers between the search, the counts and
-// the compaction.  Each workgroup takes tiles blockIdx.x, blockIdx.x + G, ...; after the counts every workgroup
-// scans all tiles' counts itself (at most MID_MAX / 256 of them), so the separate scan launch disappears.  The
-// passes, their order and every write are k_ploc_search / count / scan / compact's, so the tree is the same.
-constexpr uint32_t MID_MAX_TILES = (1u << 18) / T256;  // tiles a workgroup can scan (MID_MAX up to 2^18)
-constexpr uint32_t GRID_SPIN_LIMIT = 1u << 22;  // about a second of polling
-
-// Device-wide barrier: agent-scope release of this workgroup's writes, an arrival count, the last arrival bumps the
-// generation; acquire after it.  Returns false (in every thread) after a timeout.
-__device__ bool grid_sync(GridSync* g, uint32_t nblocks, uint32_t& gen, uint32_t* flag) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t my = gen;
-        const uint32_t prev = __hip_atomic_fetch_add(&g->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        bool ok = true;
-        if (prev == nblocks - 1u) {
-            __hip_atomic_store(&g->arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&g->gen, my + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&g->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my) {
-                if (__hip_atomic_load(&g->timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-                    ++spins > GRID_SPIN_LIMIT) {
-                    __hip_atomic_store(&g->timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = false;
-                    break;
-                }
-            }
-        }
-        gen = my + 1u;
-        __threadfence();
-        *flag = ok ? 1u : 0u;
-    }
-    __syncthreads();
-    return *flag != 0u;
-}
-
-__global__ void __launch_bounds__(T256) k_ploc_mid(Node32* __restrict__ buf0, Node32* __restrict__ buf1, Misc* misc,
-                                                   uint32_t slot0, uint32_t r, uint32_t sd_is_one,
-                                                   uint32_t depth_threshold, uint32_t first_pass, uint32_t stop_at,
-                                                   int8_t* __restrict__ merge, uint2* __restrict__ tilesums,
-                                                   Node32* __restrict__ bvh_nodes, uint32_t* __restrict__ hist,
-                                                   DpView dp, uint32_t fused, const uint32_t* __restrict__ slot_map,
-                                                   uint32_t vtop, uint32_t map_len, GridSync* gs) {
-    __shared__ float s[6][T256 + 2 * MAX_RADIUS];
-    __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
-    __shared__ uint2 scan[MID_MAX_TILES];
-    __shared__ uint32_t flag;
-    __shared__ uint32_t tot[2];
-    uint32_t gen = 0;
-    uint32_t slot = slot0;
-    uint32_t pass = first_pass;
-    Node32* cur = buf0;
-    Node32* nxt = buf1;
-    for (;;) {
-        const PlocState st = misc->state[slot];
-        if (st.count <= stop_at || st.count <= 1u) break;  // the same value in every workgroup
-        const uint32_t count = st.count;
-        const uint32_t ntiles = (count + T256 - 1) / T256;
-        if (ntiles > MID_MAX_TILES) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&misc->err, DERR_CAPACITY);
-            break;
-        }
-        const uint32_t dist1 = (sd_is_one || pass < depth_threshold) ? 1u : 0u;
-        for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-            search_tile(cur, count, t * T256, r, dist1, merge, s, &misc->err);
-            __syncthreads();  // s is refilled by the next tile
-        }
-        if (!grid_sync(gs, gridDim.x, gen, &flag)) return;
-        for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-            bool emit, mrg;
-            uint32_t b;
-            classify(merge, t * T256 + threadIdx.x, count, emit, mrg, b);
-            const uint32_t e = (uint32_t)__popcll(__ballot(emit));
-            const uint32_t m = (uint32_t)__popcll(__ballot(mrg));
-            const uint32_t wave = threadIdx.x >> 6;
-            if ((threadIdx.x & 63u) == 0) {
-                we[wave] = e;
-                wm[wave] = m;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t E = 0, M = 0;
-                for (int w = 0; w < T256 / 64; ++w) {
-                    E += we[w];
-                    M += wm[w];
-                }
-                tilesums[t] = make_uint2(E, M);
-            }
-            __syncthreads();
-        }
-        if (!grid_sync(gs, gridDim.x, gen, &flag)) return;
-        // every workgroup: exclusive scan of all tiles' (emit, merge) counts, serially in thread 0 (few tiles)
-        for (uint32_t t = threadIdx.x; t < ntiles; t += T256) scan[t] = tilesums[t];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t E = 0, M = 0;
-            for (uint32_t t = 0; t < ntiles; ++t) {
-                const uint2 v = scan[t];
-                scan[t] = make_uint2(E, M);
-                E += v.x;
-                M += v.y;
-            }
-            tot[0] = E;
-            tot[1] = M;
-            if (blockIdx.x == 0) {  // the next pass's state (k_ploc_scan)
-                PlocState nx = st;
-                nx.count = E;
-                nx.insert = st.insert - 2u * M;
-                nx.depth = st.depth + 1u;
-                nx.cluster_sum = st.cluster_sum + st.count;
-                hist[nx.depth] = nx.insert;
-                misc->state[slot ^ 1u] = nx;
-            }
-        }
-        __syncthreads();
-        for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-            const uint2 off = scan[t];
-            compact_tile(cur, nxt, merge, st, t * T256, off.x, off.y, bvh_nodes, dp, fused, slot_map, vtop, map_len,
-                         &misc->err, we, wm);
-            __syncthreads();  // we / wm are reused by the next tile
-        }
-        if (tot[1] == 0u) {  // no mutual pair: the reference would loop forever (reported, as the grid passes do)
-            if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&misc->err, DERR_SELF_MERGE);
-            slot ^= 1u;
-            break;
-        }
-        if (!grid_sync(gs, gridDim.x, gen, &flag)) return;
-        Node32* t = cur;
-        cur = nxt;
-        nxt = t;
-        slot ^= 1u;
-        pass += 1u;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) gs->slot = slot;
-}
-
 // Finishes the build inside one workgroup once count <= TAIL_MAX.  Clusters live in LDS as SoA.
 __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __restrict__ cur, Misc* misc,
                                                             uint32_t slot, uint32_t first_pass, uint32_t r,
@@ -859,17 +706,8 @@ static obvhs_status sort_codes(obvhs_context* ctx, uint64_t* keys, const uint64_
     return OBVHS_OK;
 }
 
-// Clusters below which the passes run persistently (k_ploc_mid); 0 runs every pass as separate launches.  The
-// environment variable OBVHS_PLOC_MID_MAX overrides it (measurements of both forms).
-static uint32_t ploc_mid_max() {
-    const char* e = getenv("OBVHS_PLOC_MID_MAX");
-    if (!e) return 0u;
-    const unsigned long v = strtoul(e, nullptr, 10);
-    return (uint32_t)std::min<unsigned long>(v, (unsigned long)MID_MAX_TILES * T256);
-}
-
 // The PLOC passes from the sorted clusters in `cur` to the root (ploc/mod.rs:327-499): grid passes while more than
-// TAIL_MAX clusters remain (persistent below ploc_mid_max() clusters), then the one-workgroup tail.  `fin`
+// TAIL_MAX clusters remain, then the one-workgroup tail.  `fin`
 // receives the final pass state.
 static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uint32_t n, uint32_t r,
                                uint32_t search_depth_threshold, Node32* bvh_nodes, uint32_t* hist, DpView dp,
@@ -884,44 +722,7 @@ static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uin
     uint32_t pass = 0;
     uint32_t slot = 0;
     PlocState* hstate = static_cast<PlocState*>(ctx->pinned);
-    const uint32_t mid_max = ploc_mid_max();
     while (known > TAIL_MAX) {
-        if (mid_max && known <= mid_max) {
-            // every remaining pass down to TAIL_MAX clusters in one cooperative launch
-            if (ctx->trav_lanes == 0) {
-                hipDeviceProp_t prop;
-                OBVHS_TRY(hipGetDeviceProperties(&prop, ctx->device), "device properties");
-                ctx->trav_lanes = (uint32_t)std::max(1, prop.multiProcessorCount) * 32u * 64u;
-            }
-            const uint32_t cus = ctx->trav_lanes / (32u * 64u);
-            const uint32_t g = std::max(1u, std::min(cus, (known + T256 - 1) / T256));
-            k_grid_sync_reset<<<1, 64, 0, st>>>(misc);
-            Node32* b0 = cur;
-            Node32* b1 = nxt;
-            uint32_t slot0 = slot, r_ = r, sd1 = r == 1 ? 1u : 0u, sdt = search_depth_threshold, p0 = pass,
-                     stop = TAIL_MAX, fu = fused, vt = vtop, ml = map_len;
-            GridSync* gs = &misc->gs;
-            void* args[] = {&b0, &b1, &misc, &slot0, &r_, &sd1, &sdt, &p0, &stop, &merge, &blocksums, &bvh_nodes,
-                            &hist, &dp, &fu, &slot_map, &vt, &ml, &gs};
-            OBVHS_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_ploc_mid), dim3(g), dim3(T256), args,
-                                                 0, st),
-                      "ploc persistent passes launch");
-            uint32_t* hgs = reinterpret_cast<uint32_t*>(hstate + 1);
-            OBVHS_TRY(hipMemcpyAsync(hgs, &misc->gs, sizeof(GridSync), hipMemcpyDeviceToHost, st), "read grid state");
-            OBVHS_TRY(hipMemcpyAsync(hgs + 4, &misc->err, 4, hipMemcpyDeviceToHost, st), "read err");
-            OBVHS_TRY(hipStreamSynchronize(st), "ploc persistent sync");
-            if (hgs[4]) return device_error_to_status(hgs[4]);
-            if (hgs[2]) return set_hip_error(ctx, hipErrorLaunchTimeOut, "ploc persistent passes barrier");
-            if (hgs[3] != slot) std::swap(cur, nxt);
-            slot = hgs[3];
-            OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
-                      "read ploc state");
-            OBVHS_TRY(hipStreamSynchronize(st), "ploc state sync");
-            if (hstate->count >= known) return OBVHS_ERR_PLOC_SELF_MERGE;
-            known = hstate->count;
-            pass = hstate->depth;
-            continue;
-        }
         const uint32_t blocks = (known + T256 - 1) / T256;
         for (int k = 0; k < PASSES_PER_SYNC; ++k) {
             const uint32_t dist1 = (r == 1 || pass < search_depth_threshold) ? 1u : 0u;

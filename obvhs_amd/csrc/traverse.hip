@@ -57,6 +57,14 @@ __device__ __forceinline__ float byte_f(uint32_t w, int b) { return (float)((w >
 // ROBUST (the watertight mode only): each quantised child box widened by one quantum per side and the slab interval
 // by a relative 2^-20, so rounding in the quantisation and in the slab arithmetic cannot cull a child the ray
 // touches (the crate's test is exact only in exact arithmetic).
+// Same hit mask as the SSE code with fewer instructions:
+//   * the crate swaps each child's converted near/far planes by the ray's direction signs; swapping the packed plane
+//     words once per node selects the same bytes;
+//   * SSE max(a, max(b, c)) is NaN exactly when c is, and otherwise the largest non-NaN of a, b, c, which IEEE
+//     max3 (v_max3_f32) also gives; only the z slab (c) needs a NaN check, after which the clamp to
+//     [1e-4, ray tmax] follows: tmin = 1e-4 when the z entry is NaN, tmax = ray tmax when the z exit is NaN (min
+//     likewise).  A NaN ray tmax fails every SSE compare; -inf stands in for it.  Results can differ from the SSE
+//     chain only in the sign of a zero, which no compare sees.
 template <bool ROBUST = false>
 __device__ __forceinline__ uint32_t intersect_node(const uint4& q0, const uint4& q1, const uint4& q2,
                                                    const uint4& q3, const uint4& q4, const RayData& r,
@@ -78,32 +86,40 @@ __device__ __forceinline__ uint32_t intersect_node(const uint4& q0, const uint4&
     const uint64_t bit_index8 = (meta8 ^ (oct_inv8 & inner_mask8)) & 0x1f1f1f1f1f1f1f1full;
     const uint64_t child_bits8 = (meta8 >> 5) & 0x0707070707070707ull;
 
+    // near (entry) and far (exit) plane words per axis and half
+    const uint32_t lx0 = rdx ? q2.z : q2.x, hx0 = rdx ? q2.x : q2.z, lx1 = rdx ? q2.w : q2.y, hx1 = rdx ? q2.y : q2.w;
+    const uint32_t ly0 = rdy ? q3.z : q3.x, hy0 = rdy ? q3.x : q3.z, ly1 = rdy ? q3.w : q3.y, hy1 = rdy ? q3.y : q3.w;
+    const uint32_t lz0 = rdz ? q4.z : q4.x, hz0 = rdz ? q4.x : q4.z, lz1 = rdz ? q4.w : q4.y, hz1 = rdz ? q4.y : q4.w;
+    // ROBUST: the near plane moves one quantum outward (down the axis for a positive direction), the far plane up
+    const float plx = ROBUST ? (rdx ? 1.0f : -1.0f) : 0.0f;
+    const float ply = ROBUST ? (rdy ? 1.0f : -1.0f) : 0.0f;
+    const float plz = ROBUST ? (rdz ? 1.0f : -1.0f) : 0.0f;
+    const float rt = ray_tmax == ray_tmax ? ray_tmax : -INFINITY;
+
     uint32_t hit_mask = 0u;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int h = c >> 2, b = c & 3;
-        const uint32_t wminx = h ? q2.y : q2.x, wmaxx = h ? q2.w : q2.z;
-        const uint32_t wminy = h ? q3.y : q3.x, wmaxy = h ? q3.w : q3.z;
-        const uint32_t wminz = h ? q4.y : q4.x, wmaxz = h ? q4.w : q4.z;
-        const float pad = ROBUST ? 1.0f : 0.0f;
-        const float qlx = byte_f(wminx, b) - pad, qhx = byte_f(wmaxx, b) + pad;
-        const float qly = byte_f(wminy, b) - pad, qhy = byte_f(wmaxy, b) + pad;
-        const float qlz = byte_f(wminz, b) - pad, qhz = byte_f(wmaxz, b) + pad;
-        const float xmin = rdx ? qhx : qlx, xmax = rdx ? qlx : qhx;
-        const float ymin = rdy ? qhy : qly, ymax = rdy ? qly : qhy;
-        const float zmin = rdz ? qhz : qlz, zmax = rdz ? qlz : qhz;
+        float xmin = byte_f(h ? lx1 : lx0, b), xmax = byte_f(h ? hx1 : hx0, b);
+        float ymin = byte_f(h ? ly1 : ly0, b), ymax = byte_f(h ? hy1 : hy0, b);
+        float zmin = byte_f(h ? lz1 : lz0, b), zmax = byte_f(h ? hz1 : hz0, b);
+        if (ROBUST) {
+            xmin += plx; xmax -= plx;
+            ymin += ply; ymax -= ply;
+            zmin += plz; zmax -= plz;
+        }
         const float tminx = xmin * adx + aox, tmaxx = xmax * adx + aox;
         const float tminy = ymin * ady + aoy, tmaxy = ymax * ady + aoy;
         const float tminz = zmin * adz + aoz, tmaxz = zmax * adz + aoz;
-        float tmin = smax(tminx, smax(tminy, tminz));
-        float tmax = smin(tmaxx, smin(tmaxy, tmaxz));
+        float tmin = fmaxf(tminx, fmaxf(tminy, tminz));
+        float tmax = fminf(tmaxx, fminf(tmaxy, tmaxz));
         if (ROBUST) {
             const float k = 9.5367431640625e-07f;  // 2^-20
             tmin = tmin - fabsf(tmin) * k;
             tmax = tmax + fabsf(tmax) * k;
         }
-        tmin = smax(tmin, NODE_EPSILON);
-        tmax = smin(tmax, ray_tmax);
+        tmin = tminz == tminz ? fmaxf(tmin, NODE_EPSILON) : NODE_EPSILON;
+        tmax = tmaxz == tmaxz ? fminf(tmax, rt) : rt;
         if (tmin <= tmax) {
             const uint32_t cb = (uint32_t)(child_bits8 >> (c * 8)) & 0xffu;
             const uint32_t bi = (uint32_t)(bit_index8 >> (c * 8)) & 0xffu;
@@ -678,26 +694,72 @@ static void launch_depth(uint32_t depth, uint32_t blocks, hipStream_t stream, A.
         k_traverse<MODE, COUNT, SRC, 8><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
 }
 
-// k_traverse_chunk<MODE, depth> with the launch's LDS stack depth.
-constexpr uint32_t CHUNK_MAX_RAYS_PER_LANE = 16;
-template <int MODE, int SRC, class... A>
-static void launch_chunk(uint32_t depth, uint32_t blocks, hipStream_t stream, A... args) {
-    if (depth == 4u)
-        k_traverse_chunk<MODE, 4, SRC><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
-    else if (depth == 16u)
-        k_traverse_chunk<MODE, 16, SRC><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
-    else
-        k_traverse_chunk<MODE, 8, SRC><<<blocks, TRAV_THREADS, 0, stream>>>(args...);
+// Chunked launches: a grid of the waves the device keeps resident for the kernel instance (its occupancy from its
+// register and LDS use, times the compute units) that the rows of 64 rays are dealt over, rows w, w + W, ... to wave
+// w, so no wave waits for a slot while others run (with a grid of 32 waves per CU and 7 resident, the last 4 per CU
+// started as the first finished and ran alone at the end).  At most CHUNK_MAX_ROWS_PER_WAVE rows per wave; larger
+// launches have more waves than slots.
+constexpr uint32_t CHUNK_MAX_ROWS_PER_WAVE = 16;
+struct ChunkPlan {
+    uint32_t blocks = 0, per_wave = 0;  // per_wave = rays (64 per row) a wave's lanes take in turn
+};
+
+template <class K>
+static uint32_t resident_waves_per_cu(K kernel) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, TRAV_THREADS, 0) != hipSuccess || nb <= 0) {
+        (void)hipGetLastError();
+        return 0u;
+    }
+    return (uint32_t)nb;
 }
 
-// Rows of rays per lane for a chunked launch of n rays (0: one lane per ray), with a spill column per lane.
-static uint32_t chunk_rows(uint32_t n, const TravScratch& ts) {
-    const uint32_t per_lane =
-        ts.chunked && ts.resident ? std::min<uint32_t>(n / ts.resident, CHUNK_MAX_RAYS_PER_LANE) : 0u;
-    if (per_lane < 2u) return 0u;
-    const uint64_t per_wave = (uint64_t)per_lane * TRAV_THREADS;
-    const uint64_t blocks = ((uint64_t)n + per_wave - 1) / per_wave;
-    return blocks * TRAV_THREADS <= ts.lanes ? per_lane : 0u;
+// The plan for k_traverse_chunk<MODE, LDS, SRC> over n rays; blocks = 0 when the launch should not be chunked (fewer
+// than two rows per resident wave, or more waves than spill columns).
+template <int MODE, int LDS, int SRC>
+static ChunkPlan chunk_plan(uint32_t n, const TravScratch& ts) {
+    static const uint32_t per_cu = resident_waves_per_cu(k_traverse_chunk<MODE, LDS, SRC>);
+    ChunkPlan p;
+    if (!ts.chunked || per_cu == 0u || ts.resident == 0u) return p;
+    const uint64_t cus = std::max<uint32_t>(1u, ts.resident / (32u * TRAV_THREADS));
+    const uint64_t waves = cus * per_cu;
+    const uint64_t rows = ((uint64_t)n + TRAV_THREADS - 1) / TRAV_THREADS;
+    if (rows < 2u * waves) return p;
+    // every resident slot takes a wave (rows dealt one at a time: the waves' row counts differ by at most one)
+    const uint64_t per = std::min<uint64_t>((rows + waves - 1) / waves, CHUNK_MAX_ROWS_PER_WAVE);
+    const uint64_t blocks = std::max<uint64_t>(waves, (rows + per - 1) / per);
+    if (blocks * TRAV_THREADS > ts.lanes) return p;
+    p.blocks = (uint32_t)blocks;
+    p.per_wave = (uint32_t)(per * TRAV_THREADS);
+    return p;
+}
+
+// k_traverse_chunk<MODE, depth, SRC> with the launch's LDS stack depth, when chunk_plan chunks the launch; false
+// (nothing launched) otherwise.
+template <int MODE, int SRC, int LDS>
+static bool launch_chunk_depth(uint32_t n, const TravScratch& ts, hipStream_t stream, const uint4* nd,
+                               uint32_t node_count, uint32_t prim_count, const float4* tr, const float4* ry,
+                               obvhs_ray_hit* hits, uint8_t* miss, uint2* sp, const MeshDesc* meshes,
+                               const uint32_t* ray_mesh) {
+    const ChunkPlan p = chunk_plan<MODE, LDS, SRC>(n, ts);
+    if (!p.blocks) return false;
+    k_traverse_chunk<MODE, LDS, SRC><<<p.blocks, TRAV_THREADS, 0, stream>>>(
+        nd, node_count, prim_count, tr, ry, n, hits, miss, sp, p.per_wave, meshes, ray_mesh);
+    return true;
+}
+
+template <int MODE, int SRC>
+static bool launch_chunk(uint32_t n, const TravScratch& ts, hipStream_t stream, const uint4* nd, uint32_t node_count,
+                         uint32_t prim_count, const float4* tr, const float4* ry, obvhs_ray_hit* hits, uint8_t* miss,
+                         uint2* sp, const MeshDesc* meshes, const uint32_t* ray_mesh) {
+    if (ts.lds_depth == 4u)
+        return launch_chunk_depth<MODE, SRC, 4>(n, ts, stream, nd, node_count, prim_count, tr, ry, hits, miss, sp,
+                                                meshes, ray_mesh);
+    if (ts.lds_depth == 16u)
+        return launch_chunk_depth<MODE, SRC, 16>(n, ts, stream, nd, node_count, prim_count, tr, ry, hits, miss, sp,
+                                                 meshes, ray_mesh);
+    return launch_chunk_depth<MODE, SRC, 8>(n, ts, stream, nd, node_count, prim_count, tr, ry, hits, miss, sp, meshes,
+                                            ray_mesh);
 }
 
 // Grid: one lane per ray, at most the spill buffer's lane count (lanes then stride over the rays).
@@ -719,16 +781,13 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
     const MeshDesc* no_meshes = nullptr;
     const uint32_t* no_map = nullptr;
     const TraceSegment* no_segs = nullptr;
-    // Chunked launches (when the tuning table asks for them: incoherent rays) need at least two rays per resident
-    // lane; each lane then gets that many, refilled in the wave.
+    // Chunked launches (when the tuning table asks for them: incoherent rays) need at least two rows per resident
+    // wave; each lane then takes that many rays in turn, refilled in the wave.
     if (mode > 5 || (node_visits && mode != 0)) return hipErrorInvalidValue;
-    const uint32_t rows = node_visits ? 0u : chunk_rows(n, ts);
-    if (rows) {
-        const uint32_t per_wave = rows * TRAV_THREADS;
-        const uint32_t cblocks = (uint32_t)(((uint64_t)n + per_wave - 1) / per_wave);
-#define OBVHS_CHUNK(M)                                                                                           \
-    launch_chunk<M, SRC_PLAIN>(ts.lds_depth, cblocks, stream, nd, node_count, prim_count, tr, ry, n, hits, miss, \
-                               sp, per_wave, no_meshes, no_map)
+    if (!node_visits) {
+        bool chunked = false;
+#define OBVHS_CHUNK(M) \
+    chunked = launch_chunk<M, SRC_PLAIN>(n, ts, stream, nd, node_count, prim_count, tr, ry, hits, miss, sp, no_meshes, no_map)
         if (mode == 0) OBVHS_CHUNK(0);
         else if (mode == 1) OBVHS_CHUNK(1);
         else if (mode == 2) OBVHS_CHUNK(2);
@@ -736,7 +795,7 @@ hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t pr
         else if (mode == 4) OBVHS_CHUNK(4);
         else OBVHS_CHUNK(5);
 #undef OBVHS_CHUNK
-        return hipGetLastError();
+        if (chunked) return hipGetLastError();
     }
 #define OBVHS_LANES(M, C)                                                                                       \
     launch_depth<M, C, SRC_PLAIN>(ts.lds_depth, blocks, stream, nd, node_count, prim_count, tr, ry, n, hits, miss, \
@@ -764,18 +823,12 @@ hipError_t launch_traverse_meshes(const CwNode* nodes, const MeshDesc* meshes, u
     uint2* sp = static_cast<uint2*>(ts.spill);
     uint32_t* no_counts = nullptr;
     const TraceSegment* no_segs = nullptr;
-    const uint32_t rows = chunk_rows(n, ts);
-    if (rows) {
-        const uint32_t per_wave = rows * TRAV_THREADS;
-        const uint32_t cblocks = (uint32_t)(((uint64_t)n + per_wave - 1) / per_wave);
-        if (mode == 0)
-            launch_chunk<0, SRC_MESHES>(ts.lds_depth, cblocks, stream, nd, mesh_count, 0u, tr, ry, n, hits, miss, sp,
-                                        per_wave, meshes, ray_mesh);
-        else
-            launch_chunk<1, SRC_MESHES>(ts.lds_depth, cblocks, stream, nd, mesh_count, 0u, tr, ry, n, hits, miss, sp,
-                                        per_wave, meshes, ray_mesh);
-        return hipGetLastError();
-    }
+    const bool chunked =
+        mode == 0 ? launch_chunk<0, SRC_MESHES>(n, ts, stream, nd, mesh_count, 0u, tr, ry, hits, miss, sp, meshes,
+                                                ray_mesh)
+                  : launch_chunk<1, SRC_MESHES>(n, ts, stream, nd, mesh_count, 0u, tr, ry, hits, miss, sp, meshes,
+                                                ray_mesh);
+    if (chunked) return hipGetLastError();
     if (mode == 0)
         launch_depth<0, false, SRC_MESHES>(ts.lds_depth, blocks, stream, nd, mesh_count, 0u, tr, ry, n, hits, miss,
                                            no_counts, no_counts, meshes, ray_mesh, no_segs, sp);

@@ -24,7 +24,7 @@ import numpy as np
 
 from .test_util import (CUBE, build_orthonormal_basis, cosine_sample_hemisphere, hash_noise, icosphere, uhash,
                         uhash2, unormf, uniform_sample_sphere)
-from .types import RAY, make_rays, triangles_from_vertices
+from .types import RAY, make_rays, safe_inverse, triangles_from_vertices
 
 
 def soup(n, seed=0):
@@ -54,6 +54,53 @@ def signed_zero_soup(n, seed=0):
         v = np.where((v == 0) & neg, np.float32(-0.0), np.where(v == 0, np.float32(0.0), v)).astype(np.float32)
         verts.append(v)
     return triangles_from_vertices(*verts)
+
+
+def slab_edge_rays(tris, planes, n, seed=0):
+    """n rays whose CWBVH node-test slab terms (cwbvh/simd.rs:17-100) meet special values, built field by field (the
+    crate's Ray fields are public, so a caller's inverse direction need not be Ray::new's safe_inverse):
+      * directions axis-aligned, in a coordinate plane with a +0 or -0 component, with a component of 1e-30 or
+        1e-38, or general;
+      * on an axis whose direction component is zero or tiny, the origin coordinate is, most of the time, that of a
+        node plane from `planes` (CWBVH node origins p): (p - o) * inv is then 0 * inf = NaN where inv is infinite;
+      * inverse directions 1/d (infinite on a zero component) or safe_inverse(d), a few with a NaN component;
+      * origins up to 3e38 out on one axis (slab terms overflow to infinity);
+      * tmax infinite, 1e30, NaN, 0, -1 or half the scene's extent; tmin 0 or 1e-3."""
+    rng = np.random.default_rng(seed)
+    v = np.concatenate([tris["v0"][:, :3], tris["v1"][:, :3], tris["v2"][:, :3]]).astype(np.float64)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    o = lo + (hi - lo) * (rng.random((n, 3)) * 1.4 - 0.2)
+    d = rng.normal(size=(n, 3))
+    kind = rng.integers(0, 6, n)
+    ax = rng.integers(0, 3, n)
+    rows = np.arange(n)
+    m = kind == 0  # axis-aligned
+    d[m] = 0.0
+    d[m, ax[m]] = rng.choice([-1.0, 1.0], int(m.sum()))
+    m = kind == 1  # in a coordinate plane; the zero's sign drawn
+    d[m, ax[m]] = np.where(rng.random(int(m.sum())) < 0.5, -0.0, 0.0)
+    m = kind == 2  # one tiny component
+    d[m, ax[m]] = rng.choice([1e-30, -1e-30, 1e-38, -1e-38], int(m.sum()))
+    m = kind == 3  # far out on one axis, aimed back at the scene
+    far = rng.choice([-3e38, 3e38, -1e30, 1e30], int(m.sum()))
+    o[m, ax[m]] = far
+    d[m, ax[m]] = -np.sign(far) * rng.uniform(1e-3, 1.0, int(m.sum()))
+    d = d.astype(np.float32)
+    p = np.asarray(planes, dtype=np.float32).reshape(-1, 3)[rng.integers(0, len(planes), n)]
+    on = (np.abs(d) < np.float32(1e-20)) & (rng.random((n, 3)) < 0.7)
+    o = np.where(on, p, o.astype(np.float32)).astype(np.float32)
+    with np.errstate(divide="ignore", over="ignore"):
+        inv = np.where(rng.random((n, 1)) < 0.6, np.float32(1.0) / d, safe_inverse(d)).astype(np.float32)
+    m = rng.random(n) < 0.02
+    inv[rows[m], ax[m]] = np.float32(np.nan)
+    out = np.zeros(n, dtype=RAY)
+    out["origin"][:, :3] = o
+    out["direction"][:, :3] = d
+    out["inv_direction"][:, :3] = inv
+    out["tmin"] = np.where(rng.random(n) < 0.8, np.float32(0.0), np.float32(1e-3))
+    tmax_choices = np.array([np.inf, 1e30, np.nan, 0.0, -1.0, 0.5 * float((hi - lo).max())], dtype=np.float32)
+    out["tmax"] = tmax_choices[rng.choice(6, n, p=[0.55, 0.2, 0.08, 0.05, 0.05, 0.07])]
+    return out
 
 
 def look_at_basis(eye, target, up=(0.0, 1.0, 0.0)):

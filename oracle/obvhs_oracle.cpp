@@ -107,6 +107,7 @@ enum {
     MUT_WT_MIN_AXIS = 21,      // watertight test shears onto the smallest direction axis instead of the largest
     MUT_WT_EDGE_STRICT = 22,   // watertight test counts an edge function of exactly 0 as outside
     MUT_WT_NODE_PLAIN = 23,    // watertight mode traverses with the crate's node test, not the widened one
+    MUT_NODE_IEEE = 24,        // node test with IEEE min/max (a NaN slab value dropped) instead of SSE min/max
 };
 int g_mut = MUT_NONE;
 
@@ -994,13 +995,22 @@ inline uint32_t intersect_node(const CwBvhNode& n, const Ray& ray, float ray_tma
         float tminz = zmin * adz + aoz, tmaxz = zmax * adz + aoz;
         float tmin = smax(tminx, smax(tminy, tminz));
         float tmax = smin(tmaxx, smin(tmaxy, tmaxz));
+        if (g_mut == MUT_NODE_IEEE) {
+            tmin = std::fmax(tminx, std::fmax(tminy, tminz));
+            tmax = std::fmin(tmaxx, std::fmin(tmaxy, tmaxz));
+        }
         if (robust) {
             const float k = 9.5367431640625e-07f;  // 2^-20
             tmin = tmin - std::fabs(tmin) * k;
             tmax = tmax + std::fabs(tmax) * k;
         }
-        tmin = smax(tmin, g_mut == MUT_NODE_TMIN_ZERO ? 0.0f : NODE_EPSILON);
-        tmax = smin(tmax, ray_tmax);
+        if (g_mut == MUT_NODE_IEEE) {
+            tmin = std::fmax(tmin, NODE_EPSILON);
+            tmax = std::fmin(tmax, ray_tmax);
+        } else {
+            tmin = smax(tmin, g_mut == MUT_NODE_TMIN_ZERO ? 0.0f : NODE_EPSILON);
+            tmax = smin(tmax, ray_tmax);
+        }
         if (tmin <= tmax) {
             uint32_t cb = (uint32_t)(child_bits8 >> (c * 8)) & 0xffu;
             uint32_t bi = (uint32_t)(bit_index8 >> (c * 8)) & 0xffu;
