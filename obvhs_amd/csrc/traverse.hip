@@ -33,6 +33,12 @@ constexpr int TRAV_THREADS = 64;
 constexpr int STACK_SIZE = 32;  // TRAVERSAL_STACK_SIZE, cwbvh/mod.rs:60
 constexpr int LDS_STACK = TRAV_LDS_STACK;  // entries kept in LDS; STACK_SIZE - LDS_STACK spill to HBM
 constexpr float NODE_EPSILON = 0.0001f;  // cwbvh/node.rs:82
+#ifndef OBVHS_TRI_PER_STEP
+#define OBVHS_TRI_PER_STEP 2
+#endif
+// Triangle tests a lane makes per step of the walk; a lane with more pending keeps them for the next steps and
+// tests its next node only once they are done, so the order of every ray's tests is the crate's.
+constexpr int TRI_PER_STEP = OBVHS_TRI_PER_STEP;
 constexpr uint64_t TRAV_MAX_LANES = 1u << 24;  // spill columns (224 B each) for at most 16M lanes: 3.8 GB
 
 struct RayData {
@@ -319,7 +325,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
         // The next node (the node group's highest remaining child) does not depend on the triangle tests below, so
         // its five loads are issued first and their latency overlaps the triangles' (heightfield: 4.00 -> 4.08
         // Grays/s; 72 VGPRs, 7 waves per SIMD).
-        const bool node_next = (cur.y & 0xff000000u) != 0u;
+        const bool node_next = (cur.y & 0xff000000u) != 0u && __popc(prim.y) <= TRI_PER_STEP;
         uint32_t child = 0u;
         uint4 q0, q1, q2, q3, q4;
         if (node_next) {
@@ -335,7 +341,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             }
         }
         bool stop = false;
-        while (prim.y != 0u) {
+        for (int k = 0; k < TRI_PER_STEP && prim.y != 0u; ++k) {
             const uint32_t local = firstbithigh(prim.y);
             prim.y &= ~(1u << local);
             const uint32_t pid = prim.x + local;
@@ -358,8 +364,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             }
         }
         if (stop) break;
-        prim = make_uint2(0u, 0u);
-        if (node_next) {
+        if (node_next) {  // the group's triangles are all tested (at most TRI_PER_STEP were pending)
             cur.y &= ~(1u << firstbithigh(cur.y));
             if ((cur.y & 0xff000000u) != 0u) {
                 if (sp < LDS) stack[sp][lane] = cur;
@@ -372,13 +377,17 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             const uint32_t imask = (q0.w >> 24) & 0xffu;
             cur = make_uint2(q1.x, (hm & 0xff000000u) | imask);
             prim = make_uint2(q1.y, hm & 0x00ffffffu);
-        } else {
-            cur = make_uint2(0u, 0u);
         }
-        if (prim.y == 0u && (cur.y & 0xff000000u) == 0u) {
-            if (sp == 0) break;
-            sp -= 1;
-            cur = sp < LDS ? stack[sp][lane] : my_spill[(size_t)(sp - LDS) * lanes];
+        // No node group left: pop the next one now, before this node's triangles are tested in the next step (the
+        // stack does not depend on them), so that step also tests a node and its five loads go out first.  The
+        // crate pops once both groups are empty; the stack's pushes and pops keep their order, so the visits do.
+        if ((cur.y & 0xff000000u) == 0u) {
+            if (sp == 0) {
+                if (prim.y == 0u) break;
+            } else {
+                sp -= 1;
+                cur = sp < LDS ? stack[sp][lane] : my_spill[(size_t)(sp - LDS) * lanes];
+            }
         }
     }
     if ((MODE & 1) == 0) {
@@ -474,7 +483,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
         if (!live) continue;
         // one step of k_traverse's loop
         bool done = false;
-        const bool node_next = (cur.y & 0xff000000u) != 0u;
+        const bool node_next = (cur.y & 0xff000000u) != 0u && __popc(prim.y) <= TRI_PER_STEP;
         uint32_t child = 0u;
         uint4 q0, q1, q2, q3, q4;
         if (node_next) {
@@ -489,7 +498,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
                 q4 = np[4];
             }
         }
-        while (prim.y != 0u) {
+        for (int k = 0; k < TRI_PER_STEP && prim.y != 0u; ++k) {
             const uint32_t local = firstbithigh(prim.y);
             prim.y &= ~(1u << local);
             const uint32_t pid = prim.x + local;
@@ -511,8 +520,7 @@ __global__ void __launch_bounds__(TRAV_THREADS)
             }
         }
         if (!done) {
-            prim = make_uint2(0u, 0u);
-            if (node_next) {
+            if (node_next) {  // the group's triangles are all tested (at most TRI_PER_STEP were pending)
                 cur.y &= ~(1u << firstbithigh(cur.y));
                 if ((cur.y & 0xff000000u) != 0u) {
                     if (sp < LDS) stack[sp][lane] = cur;
@@ -527,12 +535,10 @@ __global__ void __launch_bounds__(TRAV_THREADS)
                     cur = make_uint2(q1.x, (hm & 0xff000000u) | imask);
                     prim = make_uint2(q1.y, hm & 0x00ffffffu);
                 }
-            } else {
-                cur = make_uint2(0u, 0u);
             }
-            if (!done && prim.y == 0u && (cur.y & 0xff000000u) == 0u) {
+            if (!done && (cur.y & 0xff000000u) == 0u) {  // the early pop of k_traverse
                 if (sp == 0) {
-                    done = true;
+                    done = prim.y == 0u;
                 } else {
                     sp -= 1;
                     cur = sp < LDS ? stack[sp][lane] : my_spill[(size_t)(sp - LDS) * lanes];
