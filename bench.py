@@ -40,6 +40,8 @@ PMC_COPY_BYTES = 256 << 20
 TRAFFIC_FILES = {"large": os.path.join(REPO, "profiles", "large_17p3M_closest_hit_hbm_traffic.json"),
                  "heightfield": os.path.join(REPO, "profiles", "heightfield_traversal_hbm_traffic.json")}
 TRAFFIC_LAUNCHES = {"large": ("closest",), "heightfield": ("primary", "bounce")}
+# Issue and cache counters of a leg's dominant traversal launch (rocprofv3 --pmc runs of --pmc-probe): what limits it.
+COUNTER_FILES = {"large": os.path.join(REPO, "profiles", "large_17p3M_closest_hit_counters.json")}
 METRIC = "Mrays/s CWBVH closest-hit; Mtris/s PLOC build; p50/p99 batch latency"
 
 
@@ -148,17 +150,34 @@ def kernel_name(ctx, ob, n_prims, mode="closest"):
     return f"{form}<{mode}, lds {t['trav_lds_depth'][cls]}>"
 
 
-def roofline(algo_bytes_per_launch, kernel_ms, traffic, kernel, **extra):
-    """Roofline object of a traversal kernel: achieved algorithmic GB/s against the HBM peak; with the measured
-    traffic, what the kernel actually moves (the limiter: HBM traffic far below peak = latency-bound)."""
+def leg_counters(leg):
+    """The committed issue/cache counters of a leg's traversal launch, or None."""
+    try:
+        with open(COUNTER_FILES[leg]) as f:
+            return json.load(f)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(algo_bytes_per_launch, kernel_ms, traffic, kernel, counters=None, **extra):
+    """Roofline object of a traversal kernel: achieved algorithmic GB/s against the HBM peak (the only roofline that
+    applies: no MFMA work); with the measured traffic and counters, what actually limits the launch."""
     kernel_s = kernel_ms * 1e-3
     achieved = algo_bytes_per_launch / kernel_s / 1e9
     r = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
          "avg_kernel_ms": round(kernel_ms, 4), "algorithmic_bytes_per_launch": int(algo_bytes_per_launch)}
-    if traffic:
-        hbm = traffic / kernel_s / 1e9
+    hbm = traffic / kernel_s / 1e9 if traffic else None
+    if hbm is not None:
         r["measured_hbm_gbs"] = round(hbm, 1)
+    if counters and hbm is not None:
+        r["counters"] = {k: counters[k] for k in ("VALUBusy_pct", "VALUUtilization_pct", "OccupancyPercent",
+                                                  "l2_hit_rate") if k in counters}
+        r["limiter"] = (f"VALU issue: busy {counters['VALUBusy_pct']:.0f}% of the time with "
+                        f"{counters['VALUUtilization_pct']:.0f}% of lanes active (divergent walks, each wave's "
+                        f"drain); L2 misses move {hbm:.0f} GB/s ({hbm / HBM_PEAK_GBS:.2f} of HBM peak, L2 hit rate "
+                        f"{counters['l2_hit_rate']:.2f})")
+    elif hbm is not None:
         r["limiter"] = (f"latency: the counted HBM traffic moves {hbm:.0f} GB/s ({hbm / HBM_PEAK_GBS:.2f} of peak); "
                         "node and triangle reads mostly hit L2/MALL" if hbm < 0.5 * HBM_PEAK_GBS
                         else f"HBM bandwidth: {hbm:.0f} GB/s counted")
@@ -563,6 +582,7 @@ def bench_large(args, ob, ctx):
                     f"tris ({leaves} leaves after pre-splits capped at 1.3n), reinsertion 0.05 x [1, 1/3]; "
                     f"{nr} incoherent closest-hit rays per step ({layout['value']} primitives)",
         "roofline": roofline(algo_bytes, kernel_ms, traffic_per_launch("large"), closest_kernel,
+                             counters=leg_counters("large"),
                              node_tests=node_tests, tri_tests=tri_tests,
                              bytes_model="40 B per ray + 80 B per node test + 36 B per triangle test"),
         "large": {"triangles": n, "leaves_after_splits": leaves, "build_ms": round(build_ms, 3),
