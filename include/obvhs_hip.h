@@ -267,6 +267,15 @@ obvhs_status obvhs_cwbvh_reorder_triangles(obvhs_context* ctx, const obvhs_cwbvh
                                            obvhs_triangle* d_bvh_tris);
 obvhs_status obvhs_cwbvh_destroy(obvhs_cwbvh* bvh);
 
+/* ---- rays ----
+ * Ray::new (ray.rs:34-52) on the device for n rays: d_out[i] = {origin[i], direction[i], inv_direction, tmin, tmax}
+ * with inv_direction = safe_inverse per component (ray.rs:6-12: |x| <= f32::EPSILON -> signum(x) / EPSILON, else
+ * 1 / x, correctly rounded); the fourth lane of each vec3a and the padding are written as 0. */
+obvhs_status obvhs_rays_new(obvhs_context* ctx, const obvhs_vec3a* d_origins, const obvhs_vec3a* d_directions,
+                            uint32_t n, float tmin, float tmax, obvhs_ray* d_out);
+/* RayHit::none() (ray.rs:76-83) into n device hit records. */
+obvhs_status obvhs_ray_hits_none(obvhs_context* ctx, obvhs_ray_hit* d_hits, uint32_t n);
+
 /* ---- traversal ----
  * Replaces CwBvh::ray_traverse (cwbvh/mod.rs:169-198) with the closure |ray, id| bvh_tris[id].intersect(ray)
  * (triangle.rs:35-76), for n rays at once.  d_bvh_tris: triangles in primitive_indices order.  Each hit starts as

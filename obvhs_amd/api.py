@@ -732,6 +732,28 @@ class Server:
                     scenes=st.scenes)
 
 
+def rays_new(ctx, origins, directions, tmin=0.0, tmax=float("inf")):
+    """Ray::new (ray.rs:34-52) on the device: a device Ray array from (n, 3) or (n, 4) origins and directions
+    (host arrays are uploaded as 16-byte vec3a rows)."""
+    o = np.zeros((len(origins), 4), np.float32)
+    d = np.zeros((len(directions), 4), np.float32)
+    o[:, :3] = np.asarray(origins, np.float32)[:, :3]
+    d[:, :3] = np.asarray(directions, np.float32)[:, :3]
+    if len(o) != len(d):
+        raise ValueError("origins and directions differ in length")
+    do, dd = ctx.upload(o), ctx.upload(d)
+    out = DeviceArray(ctx, RAY, len(o))
+    check(lib().obvhs_rays_new(ctx.handle, do.ptr, dd.ptr, len(o), float(tmin), float(tmax), out.ptr), ctx.handle)
+    return out
+
+
+def ray_hits_none(ctx, n):
+    """RayHit::none() (ray.rs:76-83) for n device hit records."""
+    out = DeviceArray(ctx, RAY_HIT, n)
+    check(lib().obvhs_ray_hits_none(ctx.handle, out.ptr, n), ctx.handle)
+    return out
+
+
 def rt_triangles(ctx, triangles):
     """RtTriangle::from(&Triangle) for every triangle (rt_triangle.rs:162-181): a device RtTriangle array."""
     d, n = _as_device(ctx, triangles, TRIANGLE)

@@ -651,6 +651,19 @@ obvhs_status obvhs_cwbvh_ray_traverse_miss_watertight(obvhs_context* ctx, const 
     return traverse_watertight(ctx, bvh, d_bvh_tris, d_rays, n, 3, nullptr, d_miss);
 }
 
+obvhs_status obvhs_rays_new(obvhs_context* ctx, const obvhs_vec3a* d_origins, const obvhs_vec3a* d_directions,
+                            uint32_t n, float tmin, float tmax, obvhs_ray* d_out) {
+    if (!ctx || (n && (!d_origins || !d_directions || !d_out))) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(launch_rays_new(d_origins, d_directions, n, tmin, tmax, d_out, ctx->stream), "rays new");
+    return OBVHS_OK;
+}
+
+obvhs_status obvhs_ray_hits_none(obvhs_context* ctx, obvhs_ray_hit* d_hits, uint32_t n) {
+    if (!ctx || (n && !d_hits)) return OBVHS_ERR_INVALID_ARG;
+    CTX_TRY(launch_hits_none(d_hits, n, ctx->stream), "hits none");
+    return OBVHS_OK;
+}
+
 obvhs_status obvhs_rt_triangles_from_triangles(obvhs_context* ctx, const obvhs_triangle* d_tris, uint32_t n,
                                                obvhs_rt_triangle* d_out) {
     if (!ctx || (n && (!d_tris || !d_out))) return OBVHS_ERR_INVALID_ARG;

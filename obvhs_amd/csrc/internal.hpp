@@ -173,6 +173,9 @@ struct TravScratch {
 obvhs_status traverse_scratch(obvhs_context* ctx, TravScratch& ts, uint64_t n);
 obvhs_status noalloc_scratch_init(obvhs_context* ctx);
 hipError_t launch_rt_triangles(const obvhs_triangle* tris, uint32_t n, obvhs_rt_triangle* out, hipStream_t stream);
+hipError_t launch_rays_new(const obvhs_vec3a* origins, const obvhs_vec3a* directions, uint32_t n, float tmin,
+                           float tmax, obvhs_ray* out, hipStream_t stream);
+hipError_t launch_hits_none(obvhs_ray_hit* hits, uint32_t n, hipStream_t stream);
 hipError_t launch_traverse(const CwNode* nodes, uint32_t node_count, uint32_t prim_count, const obvhs_triangle* tris,
                            const obvhs_ray* rays, uint32_t n, int mode, obvhs_ray_hit* hits, uint8_t* miss,
                            uint32_t* node_visits, uint32_t* tri_tests, const TravScratch& ts, hipStream_t stream);

@@ -682,6 +682,45 @@ __global__ void __launch_bounds__(256) k_rt_triangles(const float4* __restrict__
 
 }  // namespace
 
+// safe_inverse (ray.rs:6-12); signum(+-0) = +-1, signum(NaN) = NaN (and |NaN| <= EPSILON is false).
+__device__ __forceinline__ float safe_inverse(float x) {
+    constexpr float EPS = 1.1920928955078125e-07f;  // f32::EPSILON = 2^-23
+    return fabsf(x) <= EPS ? copysignf(1.0f, x) / EPS : 1.0f / x;
+}
+
+// Ray::new (ray.rs:34-52), one thread per ray.
+__global__ void __launch_bounds__(256) k_rays_new(const float4* __restrict__ o, const float4* __restrict__ d,
+                                                  uint32_t n, float tmin, float tmax, float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = o[i], b = d[i];
+    out[4 * (size_t)i] = make_float4(a.x, a.y, a.z, 0.0f);
+    out[4 * (size_t)i + 1] = make_float4(b.x, b.y, b.z, 0.0f);
+    out[4 * (size_t)i + 2] = make_float4(safe_inverse(b.x), safe_inverse(b.y), safe_inverse(b.z), 0.0f);
+    out[4 * (size_t)i + 3] = make_float4(tmin, tmax, 0.0f, 0.0f);
+}
+
+// RayHit::none (ray.rs:76-83).
+__global__ void __launch_bounds__(256) k_hits_none(uint4* __restrict__ hits, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) hits[i] = make_uint4(INVALID32, INVALID32, INVALID32, __float_as_uint(INFINITY));
+}
+
+hipError_t launch_rays_new(const obvhs_vec3a* origins, const obvhs_vec3a* directions, uint32_t n, float tmin,
+                           float tmax, obvhs_ray* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    k_rays_new<<<(n + 255) / 256, 256, 0, stream>>>(reinterpret_cast<const float4*>(origins),
+                                                    reinterpret_cast<const float4*>(directions), n, tmin, tmax,
+                                                    reinterpret_cast<float4*>(out));
+    return hipGetLastError();
+}
+
+hipError_t launch_hits_none(obvhs_ray_hit* hits, uint32_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    k_hits_none<<<(n + 255) / 256, 256, 0, stream>>>(reinterpret_cast<uint4*>(hits), n);
+    return hipGetLastError();
+}
+
 hipError_t launch_rt_triangles(const obvhs_triangle* tris, uint32_t n, obvhs_rt_triangle* out, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     k_rt_triangles<<<(n + 255) / 256, 256, 0, stream>>>(reinterpret_cast<const float4*>(tris), n,

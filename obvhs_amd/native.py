@@ -124,6 +124,8 @@ SIGNATURES = {
     "obvhs_cwbvh_ray_traverse_counted": (INT, [P, P, P, P, U32, P, P, P]),
     "obvhs_cwbvh_ray_traverse_watertight": (INT, [P, P, P, P, U32, P]),
     "obvhs_cwbvh_ray_traverse_miss_watertight": (INT, [P, P, P, P, U32, P]),
+    "obvhs_rays_new": (INT, [P, P, P, U32, C.c_float, C.c_float, P]),
+    "obvhs_ray_hits_none": (INT, [P, P, U32]),
     "obvhs_rt_triangles_from_triangles": (INT, [P, P, U32, P]),
     "obvhs_cwbvh_ray_traverse_rt": (INT, [P, P, P, P, U32, P]),
     "obvhs_cwbvh_ray_traverse_miss_rt": (INT, [P, P, P, P, U32, P]),
