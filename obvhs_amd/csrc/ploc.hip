@@ -473,6 +473,65 @@ __global__ void __launch_bounds__(T256) k_ploc_compact(const Node32* __restrict_
                  map_len, &misc->err, we, wm);
 }
 
+// k_ploc_scan and k_ploc_compact in one launch for passes of at most SCAN_FUSED_BLOCKS tiles: each workgroup sums the
+// (emit, merge) counts of the tiles before its own (at most 4 per thread), and workgroup 0 also sums them all and
+// writes the next pass's state, as k_ploc_scan does.
+constexpr uint32_t SCAN_FUSED_BLOCKS = 1024;
+__global__ void __launch_bounds__(T256) k_ploc_scan_compact(const Node32* __restrict__ cur, Node32* __restrict__ next,
+                                                            const int8_t* __restrict__ merge, Misc* misc,
+                                                            uint32_t slot, const uint2* __restrict__ blocksums,
+                                                            uint32_t nblocks, Node32* __restrict__ bvh_nodes, DpView dp,
+                                                            uint32_t fused, const uint32_t* __restrict__ slot_map,
+                                                            uint32_t vtop, uint32_t map_len,
+                                                            uint32_t* __restrict__ hist) {
+    __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
+    __shared__ uint2 pre;
+    const PlocState st = misc->state[slot];
+    const uint32_t upto = blockIdx.x == 0 ? nblocks : blockIdx.x;  // workgroup 0 sums every tile (the totals)
+    uint32_t E = 0, M = 0;
+    for (uint32_t b = threadIdx.x; b < upto; b += T256) {
+        const uint2 v = blocksums[b];
+        E += v.x;
+        M += v.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        E += (uint32_t)__shfl_xor((int)E, off);
+        M += (uint32_t)__shfl_xor((int)M, off);
+    }
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) {
+        we[wave] = E;
+        wm[wave] = M;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t te = 0, tm = 0;
+        for (int w = 0; w < T256 / 64; ++w) {
+            te += we[w];
+            tm += wm[w];
+        }
+        if (blockIdx.x == 0) {
+            PlocState nx = st;
+            if (st.count > 1) {
+                nx.count = te;
+                nx.insert = st.insert - 2u * tm;
+                nx.depth = st.depth + 1u;
+                nx.cluster_sum = st.cluster_sum + st.count;
+                hist[nx.depth] = nx.insert;
+            }
+            misc->state[slot ^ 1u] = nx;
+            pre = make_uint2(0u, 0u);
+        } else {
+            pre = make_uint2(te, tm);
+        }
+    }
+    __syncthreads();
+    const uint2 off = pre;
+    __syncthreads();  // compact_tile reuses we / wm
+    compact_tile(cur, next, merge, st, blockIdx.x * T256, off.x, off.y, bvh_nodes, dp, fused, slot_map, vtop,
+                 map_len, &misc->err, we, wm);
+}
+
 // Finishes the build inside one workgroup once count <= TAIL_MAX.  Clusters live in LDS as SoA.
 __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __restrict__ cur, Misc* misc,
                                                             uint32_t slot, uint32_t first_pass, uint32_t r,
@@ -728,9 +787,14 @@ static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uin
             const uint32_t dist1 = (r == 1 || pass < search_depth_threshold) ? 1u : 0u;
             k_ploc_search<<<blocks, T256, 0, st>>>(cur, misc, slot, r, dist1, merge);
             k_ploc_count<<<blocks, T256, 0, st>>>(merge, misc, slot, blocksums);
-            k_ploc_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, slot, hist);
-            k_ploc_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, bvh_nodes, dp, fused,
-                                                   slot_map, vtop, map_len);
+            if (blocks <= SCAN_FUSED_BLOCKS) {
+                k_ploc_scan_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, blocks, bvh_nodes,
+                                                            dp, fused, slot_map, vtop, map_len, hist);
+            } else {
+                k_ploc_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, slot, hist);
+                k_ploc_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, bvh_nodes, dp, fused,
+                                                       slot_map, vtop, map_len);
+            }
             std::swap(cur, nxt);
             slot ^= 1u;
             pass++;
