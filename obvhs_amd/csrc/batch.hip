@@ -300,6 +300,8 @@ __global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, 
     const uint32_t chunk = (nblocks + 1023u) / 1024u;
     const uint32_t b0 = tid * chunk, b1 = min(nblocks, b0 + chunk);
     uint32_t E = 0, M = 0;
+    // each thread's chunk is contiguous (the prefix order); unrolled so that several loads are in flight
+#pragma unroll 8
     for (uint32_t b = b0; b < b1; ++b) {
         const uint2 v = blocksums[b];
         E += v.x;
@@ -326,6 +328,7 @@ __global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, 
     }
     __syncthreads();
     uint32_t ee = se[wave] + ie - E, em = sm[wave] + im - M;
+#pragma unroll 8
     for (uint32_t b = b0; b < b1; ++b) {
         const uint2 v = blocksums[b];
         blocksums[b] = make_uint2(ee, em);
