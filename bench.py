@@ -375,6 +375,21 @@ def bench_batch(args, ob, ctx):
     ctx.synchronize()
     plain_hits = hits.to_host()
     same = bool(plain_hits.tobytes() == batch_hits.tobytes())
+    # node and triangle test counts of every ray (the same walks as the batched launch's: identical CWBVHs and
+    # hits), for the batched trace's roofline
+    nv = ob.DeviceArray(ctx, np.uint32, n_rays)
+    tv = ob.DeviceArray(ctx, np.uint32, n_rays)
+    for i in range(m):
+        o = int(offsets[i])
+        ob.native.check(lib.obvhs_cwbvh_ray_traverse_counted(
+            ctx.handle, handles[i], C.c_void_p(base_pt + o * tri_sz), C.c_void_p(base_r + i * rpm * ray_sz), rpm,
+            C.c_void_p(base_h + i * rpm * hit_sz), C.c_void_p(nv.ptr.value + i * rpm * 4),
+            C.c_void_p(tv.ptr.value + i * rpm * 4)), ctx.handle)
+    node_tests, tri_tests = int(nv.to_host().astype(np.int64).sum()), int(tv.to_host().astype(np.int64).sum())
+    result["roofline"] = roofline(40 * n_rays + 80 * node_tests + 36 * tri_tests, trace_s * 1e3, None,
+                                  kernel_name(ctx, ob, n_tris // m), node_tests=node_tests, tri_tests=tri_tests,
+                                  bytes_model="40 B per ray + 80 B per node test + 36 B per triangle test",
+                                  timing="wall time per batched launch (one kernel), not HIP events")
     t0 = time.perf_counter()
     for _ in range(max(1, args.steps // 4)):
         trace_plain()

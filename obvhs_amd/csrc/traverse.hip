@@ -6,13 +6,16 @@
 //   * node groups and primitive groups are consumed highest bit first (firstbithigh, cwbvh/mod.rs:996-998);
 //   * the 32-entry group stack keeps StackStack semantics (faststack.rs:299-310): a push past slot 31 overwrites
 //     slot 31, a pop of the empty stack ends traversal;
-//   * the node test is the SSE path of cwbvh/simd.rs:17-100 (mul then add, SSE min/max, tmin clamped to 1e-4,
-//     inclusive compare) and the primitive test is Triangle::intersect (triangle.rs:35-76).
+//   * the node test gives the SSE path's results (cwbvh/simd.rs:17-100: mul then add, SSE min/max, tmin clamped to
+//     1e-4, inclusive compare) and the primitive test is Triangle::intersect (triangle.rs:35-76).
+// A step of a lane's walk tests at most TRI_PER_STEP pending triangles and then, once none is left, the next node;
+// an empty node group is replaced by the stack's top at the end of the step.  Each ray's sequence of tests, pushes
+// and pops is the crate's; only how lanes share a wave's steps changes.
 // Stack storage: the bottom LDS entries of each lane's stack live in LDS (one 64-lane column per wave, entry-major;
 // 4, 8 or 16 entries, the depth the context's tuning table holds for the BVH's size class, 8 by default), deeper
-// entries in a per-lane column of an HBM spill array; a short LDS footprint keeps 8 waves per SIMD resident, which
-// the latency-bound walk needs.  A launch has one lane per ray (up to 16M lanes; beyond that the lanes stride over
-// the rays) and each lane owns one spill column.  Nodes are fetched as five 16-byte loads, triangles (in
+// entries in a per-lane column of an HBM spill array, so LDS does not limit residency.  A launch has one lane per
+// ray (up to 16M lanes; beyond that the lanes stride over the rays), or, chunked, rows of rays dealt over the
+// resident waves; each lane owns one spill column.  Nodes are fetched as five 16-byte loads, triangles (in
 // primitive_indices order) as three.
 #include <algorithm>
 #include <mutex>
