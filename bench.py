@@ -998,7 +998,7 @@ def main():
     ap.add_argument("--serve-sweep", default="250,500,1000,1500,2000,3000,4000,6000",
                     help="request rates of the serving sweep (empty: no sweep)")
     ap.add_argument("--serve-slo-ms", type=float, default=5.0)
-    ap.add_argument("--serve-inflight", type=int, default=2)
+    ap.add_argument("--serve-inflight", type=int, default=3)
     ap.add_argument("--serve-update-streams", type=int, default=3)
     args = ap.parse_args()
     if args.traffic_from:

@@ -187,6 +187,7 @@ __global__ void __launch_bounds__(T256) k_b_search(const Node32* __restrict__ cu
     if (base >= total) return;
     const int wstart = (int)base - (int)rmax;
     const int wlen = T256 + 2 * (int)rmax;
+    bool nan_here = false;
     for (int w = threadIdx.x; w < wlen; w += T256) {
         const int g = wstart + w;
         if (g >= 0 && (uint32_t)g < total) {
@@ -197,9 +198,10 @@ __global__ void __launch_bounds__(T256) k_b_search(const Node32* __restrict__ cu
             s[3][w] = nd.maxx;
             s[4][w] = nd.maxy;
             s[5][w] = nd.maxz;
+            nan_here |= box_has_nan_any(node_box(nd));
         }
     }
-    __syncthreads();
+    const bool nonan = !__syncthreads_or(nan_here);  // NaN-free window: costs from union_area_nonan
     const uint32_t i = base + threadIdx.x;
     if (i >= total) return;
     const uint32_t mi = cmesh[i];
@@ -233,7 +235,7 @@ __global__ void __launch_bounds__(T256) k_b_search(const Node32* __restrict__ cu
             if (j == li) continue;
             const int w = (int)(sg.start + j) - wstart;
             const Box o{s[0][w], s[1][w], s[2][w], s[3][w], s[4][w], s[5][w]};
-            const float c = half_area(box_union(me, o));
+            const float c = nonan ? union_area_nonan(me, o) : half_area(box_union(me, o));
             if (c <= best_cost) {
                 best = j;
                 best_cost = c;

@@ -65,6 +65,20 @@ __device__ __forceinline__ float half_area(const Box& a) {
     return (dx + dy) * dz + dx * dy;
 }
 
+// half_area(box_union(a, b)) for boxes without NaN: the IEEE minimum / maximum instructions (one each) give the
+// union's components in value (only the sign of a zero may differ), so the area compares equal; for neighbour
+// searches, whose costs only meet compares.
+__device__ __forceinline__ float union_area_nonan(const Box& a, const Box& b) {
+    const Box u{__builtin_elementwise_minimum(a.lx, b.lx), __builtin_elementwise_minimum(a.ly, b.ly),
+                __builtin_elementwise_minimum(a.lz, b.lz), __builtin_elementwise_maximum(a.hx, b.hx),
+                __builtin_elementwise_maximum(a.hy, b.hy), __builtin_elementwise_maximum(a.hz, b.hz)};
+    return half_area(u);
+}
+
+__device__ __forceinline__ bool box_has_nan_any(const Box& b) {
+    return b.lx != b.lx || b.ly != b.ly || b.lz != b.lz || b.hx != b.hx || b.hy != b.hy || b.hz != b.hz;
+}
+
 __device__ __forceinline__ Box node_box(const Node32& n) { return Box{n.minx, n.miny, n.minz, n.maxx, n.maxy, n.maxz}; }
 
 __device__ __forceinline__ Node32 load_node(const Node32* p) {

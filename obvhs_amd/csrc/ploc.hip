@@ -247,6 +247,7 @@ __device__ __forceinline__ void search_tile(const Node32* __restrict__ cur, uint
                                             float (*s)[T256 + 2 * MAX_RADIUS], uint32_t* __restrict__ err) {
     const int wstart = (int)base - (int)r;
     const int wlen = T256 + 2 * (int)r;
+    bool nan_here = false;
     for (int w = threadIdx.x; w < wlen; w += T256) {
         int g = wstart + w;
         if (g >= 0 && (uint32_t)g < count) {
@@ -257,9 +258,10 @@ __device__ __forceinline__ void search_tile(const Node32* __restrict__ cur, uint
             s[3][w] = nd.maxx;
             s[4][w] = nd.maxy;
             s[5][w] = nd.maxz;
+            nan_here |= box_has_nan_any(node_box(nd));
         }
     }
-    __syncthreads();
+    const bool nonan = !__syncthreads_or(nan_here);  // NaN-free window: costs from union_area_nonan
     const uint32_t i = base + threadIdx.x;
     if (i < count) {
         const int li = (int)threadIdx.x + (int)r;
@@ -284,7 +286,9 @@ __device__ __forceinline__ void search_tile(const Node32* __restrict__ cur, uint
             float best_cost = INFINITY;
             for (uint32_t j = begin; j < end; ++j) {
                 if (j == i) continue;
-                float c = window_cost(s, me, (int)j - wstart);
+                const int o = (int)j - wstart;
+                const float c = nonan ? union_area_nonan(me, Box{s[0][o], s[1][o], s[2][o], s[3][o], s[4][o], s[5][o]})
+                                      : window_cost(s, me, o);
                 if (c <= best_cost) {
                     best = j;
                     best_cost = c;
@@ -564,7 +568,11 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
         pc[i] = nd.prim_count;
         fi[i] = nd.first_index;
     }
-    __syncthreads();
+    bool nan_here = false;
+    for (uint32_t i = tid; i < count; i += TAIL_THREADS)
+        nan_here |= box_has_nan_any(Box{lx[i], ly[i], lz[i], hx[i], hy[i], hz[i]});
+    // NaN-free clusters (merging keeps them so): search costs from union_area_nonan
+    const bool nonan = !__syncthreads_or(nan_here);
     uint32_t insert = st.insert;
     uint32_t depth = st.depth;
     unsigned long long csum = st.cluster_sum;
@@ -599,7 +607,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
                 for (uint32_t j = begin; j < end; ++j) {
                     if (j == i) continue;
                     const Box o{lx[j], ly[j], lz[j], hx[j], hy[j], hz[j]};
-                    float c = half_area(box_union(me, o));
+                    const float c = nonan ? union_area_nonan(me, o) : half_area(box_union(me, o));
                     if (c <= best_cost) {
                         best = j;
                         best_cost = c;
