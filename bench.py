@@ -41,7 +41,8 @@ TRAFFIC_FILES = {"large": os.path.join(REPO, "profiles", "large_17p3M_closest_hi
                  "heightfield": os.path.join(REPO, "profiles", "heightfield_traversal_hbm_traffic.json")}
 TRAFFIC_LAUNCHES = {"large": ("closest",), "heightfield": ("primary", "bounce")}
 # Issue and cache counters of a leg's dominant traversal launch (rocprofv3 --pmc runs of --pmc-probe): what limits it.
-COUNTER_FILES = {"large": os.path.join(REPO, "profiles", "large_17p3M_closest_hit_counters.json")}
+COUNTER_FILES = {"large": os.path.join(REPO, "profiles", "large_17p3M_closest_hit_counters.json"),
+                 "heightfield": os.path.join(REPO, "profiles", "heightfield_closest_hit_counters.json")}
 METRIC = "Mrays/s CWBVH closest-hit; Mtris/s PLOC build; p50/p99 batch latency"
 
 
@@ -174,8 +175,8 @@ def roofline(algo_bytes_per_launch, kernel_ms, traffic, kernel, counters=None, *
         r["counters"] = {k: counters[k] for k in ("VALUBusy_pct", "VALUUtilization_pct", "OccupancyPercent",
                                                   "l2_hit_rate") if k in counters}
         r["limiter"] = (f"VALU issue: busy {counters['VALUBusy_pct']:.0f}% of the time with "
-                        f"{counters['VALUUtilization_pct']:.0f}% of lanes active (divergent walks, each wave's "
-                        f"drain); L2 misses move {hbm:.0f} GB/s ({hbm / HBM_PEAK_GBS:.2f} of HBM peak, L2 hit rate "
+                        f"{counters['VALUUtilization_pct']:.0f}% of lanes active (divergent walks); L2 misses "
+                        f"move {hbm:.0f} GB/s ({hbm / HBM_PEAK_GBS:.2f} of HBM peak, L2 hit rate "
                         f"{counters['l2_hit_rate']:.2f})")
     elif hbm is not None:
         r["limiter"] = (f"latency: the counted HBM traffic moves {hbm:.0f} GB/s ({hbm / HBM_PEAK_GBS:.2f} of peak); "
@@ -709,7 +710,7 @@ def bench_heightfield(args, ob, ctx):
            "closest_mrays_per_s": round(n_rays * args.steps / wall / 1e6, 2),
            "ms_per_step": round(wall / args.steps * 1e3, 4),
            "roofline": roofline(algo_bytes / 2.0, kernel_ms, traffic_per_launch("heightfield"),
-                                kernel_name(ctx, ob, n_tris),
+                                kernel_name(ctx, ob, n_tris), counters=leg_counters("heightfield"),
                                 avg_kernel_ms_by_launch={"primary": round(float(np.mean(kp)), 4),
                                                          "bounce": round(float(np.mean(kb)), 4)},
                                 node_tests=node_tests, tri_tests=tri_tests),
