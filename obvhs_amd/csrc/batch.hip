@@ -54,11 +54,12 @@ struct SegState {
     uint32_t mbase;   // global exclusive merge count at the segment's first cluster (current pass)
 };
 
+// The cluster count is double-buffered by pass parity q: pass p reads total[p & 1] and its scan writes the next
+// pass's count into total[(p + 1) & 1], so no launch is needed between passes to move it.
 struct BatchMisc {
     uint32_t err;
-    uint32_t total;      // clusters in the current array
+    uint32_t total[2];   // clusters in the current array (by pass parity)
     uint32_t active;     // meshes with more than one cluster after the last pass
-    uint32_t pad;
     uint32_t totals[2];  // emits, merges of the pass
 };
 
@@ -78,7 +79,8 @@ __global__ void __launch_bounds__(T256) k_b_init(uint32_t* __restrict__ bounds, 
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     if (i == 0) {
         misc->err = 0u;
-        misc->total = n_total;
+        misc->total[0] = n_total;
+        misc->total[1] = 0u;
         misc->active = 0u;
     }
     if (i >= m) return;
@@ -180,9 +182,9 @@ __global__ void __launch_bounds__(T256) k_b_search(const Node32* __restrict__ cu
                                                    const uint32_t* __restrict__ cmesh,
                                                    const SegState* __restrict__ seg, BatchMisc* misc,
                                                    const MeshDesc* __restrict__ meshes, uint32_t rmax,
-                                                   uint32_t before_sdt, int8_t* __restrict__ merge) {
+                                                   uint32_t before_sdt, int8_t* __restrict__ merge, uint32_t q) {
     __shared__ float s[6][T256 + 2 * MAX_RADIUS];
-    const uint32_t total = misc->total;
+    const uint32_t total = misc->total[q];
     const uint32_t base = blockIdx.x * T256;
     if (base >= total) return;
     const int wstart = (int)base - (int)rmax;
@@ -273,12 +275,12 @@ __device__ __forceinline__ void b_classify(const int8_t* __restrict__ merge, con
 
 __global__ void __launch_bounds__(T256) k_b_count(const int8_t* __restrict__ merge, const uint32_t* __restrict__ cmesh,
                                                   const SegState* __restrict__ seg, const BatchMisc* misc,
-                                                  uint2* __restrict__ blocksums) {
+                                                  uint2* __restrict__ blocksums, uint32_t q) {
     __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     bool emit, mrg;
     uint32_t b;
-    b_classify(merge, cmesh, seg, i, misc->total, emit, mrg, b);
+    b_classify(merge, cmesh, seg, i, misc->total[q], emit, mrg, b);
     const uint32_t e = (uint32_t)__popcll(__ballot(emit)), m = (uint32_t)__popcll(__ballot(mrg));
     const uint32_t wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 0) {
@@ -296,7 +298,9 @@ __global__ void __launch_bounds__(T256) k_b_count(const int8_t* __restrict__ mer
     }
 }
 
-__global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, uint32_t nblocks, BatchMisc* misc) {
+// Also writes the next pass's cluster count (total[q ^ 1]) and clears the active-mesh counter k_b_segments fills.
+__global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, uint32_t nblocks, BatchMisc* misc,
+                                                 uint32_t q) {
     __shared__ uint32_t se[16], sm[16];
     const uint32_t tid = threadIdx.x;
     const uint32_t chunk = (nblocks + 1023u) / 1024u;
@@ -327,6 +331,8 @@ __global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, 
         }
         misc->totals[0] = re;
         misc->totals[1] = rm;
+        misc->total[q ^ 1u] = re;
+        misc->active = 0u;
     }
     __syncthreads();
     uint32_t ee = se[wave] + ie - E, em = sm[wave] + im - M;
@@ -339,31 +345,68 @@ __global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, 
     }
 }
 
-// Exclusive (emit, merge) prefix at cluster index i: the block's scanned offset plus the block's clusters before i,
-// counted by a whole wave, 64 clusters per step (one ballot each).
-__device__ __forceinline__ uint2 b_prefix_at_wave(const int8_t* __restrict__ merge,
-                                                  const uint32_t* __restrict__ cmesh,
-                                                  const SegState* __restrict__ seg, const uint2* __restrict__ offsets,
-                                                  uint32_t i, uint32_t total, const BatchMisc* misc, uint32_t lane) {
-    if (i >= total) return make_uint2(misc->totals[0], misc->totals[1]);
-    const uint32_t blk = i / T256;
-    uint2 p = offsets[blk];
-    for (uint32_t j0 = blk * T256; j0 < i; j0 += 64u) {
-        const uint32_t j = j0 + lane;
-        bool emit = false, mrg = false;
-        uint32_t b;
-        if (j < i) b_classify(merge, cmesh, seg, j, total, emit, mrg, b);
-        p.x += (uint32_t)__popcll(__ballot(emit));
-        p.y += (uint32_t)__popcll(__ballot(mrg));
+// Exclusive (emit, merge) prefixes at the segment's first cluster i0 and one past its last, i1: the scanned offset of
+// the index's block plus the block's clusters before it, counted by the whole wave 64 clusters per step (one ballot
+// each), as b_classify decides.  The up to 4 steps of both prefixes are independent, so all their loads go out in
+// two rounds (cluster mesh and merge offset; segment count and partner's offset) before the first ballot.
+__device__ __forceinline__ void b_prefix_pair(const int8_t* __restrict__ merge, const uint32_t* __restrict__ cmesh,
+                                              const SegState* __restrict__ seg, const uint2* __restrict__ offsets,
+                                              uint32_t i0, uint32_t i1, uint32_t total, const BatchMisc* misc,
+                                              uint32_t lane, uint2& p0, uint2& p1) {
+    uint32_t idx[8], cm[8];
+    int mv[8];
+    bool valid[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const uint32_t i = s < 4 ? i0 : i1;
+        const uint32_t j = (i / T256) * T256 + (uint32_t)(s & 3) * 64u + lane;
+        idx[s] = j;
+        valid[s] = i < total && j < i;
+        cm[s] = valid[s] ? cmesh[j] : 0u;
+        mv[s] = valid[s] ? (int)merge[j] : 0;
     }
-    return p;
+    uint32_t cnt[8];
+    int mb[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        cnt[s] = valid[s] ? seg[cm[s]].count : 0u;
+        // The partner's offset, read before the count says whether it matters: clamped into the array (a mesh
+        // with one cluster left keeps a stale offset from an earlier pass).
+        const int bi = min(max((int)idx[s] + mv[s], 0), (int)total - 1);
+        mb[s] = valid[s] ? (int)merge[bi] : 0;
+    }
+    p0 = i0 >= total ? make_uint2(misc->totals[0], misc->totals[1]) : offsets[i0 / T256];
+    p1 = i1 >= total ? make_uint2(misc->totals[0], misc->totals[1]) : offsets[i1 / T256];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        bool emit = false, mrg = false;
+        if (valid[s]) {
+            const int i = (int)idx[s], bi = i + mv[s];
+            if (cnt[s] <= 1u) {
+                emit = true;
+            } else if (bi + mb[s] != i || bi == i) {  // not mutual, or a self merge (b_classify)
+                emit = true;
+            } else if (bi < i) {
+                emit = true;
+                mrg = true;
+            }
+        }
+        const uint32_t ce = (uint32_t)__popcll(__ballot(emit)), cg = (uint32_t)__popcll(__ballot(mrg));
+        if (s < 4) {
+            p0.x += ce;
+            p0.y += cg;
+        } else {
+            p1.x += ce;
+            p1.y += cg;
+        }
+    }
 }
 
 // Per mesh (one wave each): the segment of the next pass, its insert index and this pass's merge base.
 __global__ void __launch_bounds__(T256) k_b_segments(const int8_t* __restrict__ merge,
                                                      const uint32_t* __restrict__ cmesh, SegState* __restrict__ seg,
                                                      SegState* __restrict__ seg_next, const uint2* __restrict__ offsets,
-                                                     uint32_t m, BatchMisc* misc) {
+                                                     uint32_t m, BatchMisc* misc, uint32_t q) {
     const uint32_t k = (blockIdx.x * T256 + threadIdx.x) >> 6;
     const uint32_t lane = threadIdx.x & 63u;
     if (k >= m) return;  // wave-uniform
@@ -372,9 +415,9 @@ __global__ void __launch_bounds__(T256) k_b_segments(const int8_t* __restrict__ 
         if (lane == 0u) seg_next[k] = sg;
         return;
     }
-    const uint32_t total = misc->total;
-    const uint2 head = b_prefix_at_wave(merge, cmesh, seg, offsets, sg.start, total, misc, lane);
-    const uint2 end = b_prefix_at_wave(merge, cmesh, seg, offsets, sg.start + sg.count, total, misc, lane);
+    const uint32_t total = misc->total[q];
+    uint2 head, end;
+    b_prefix_pair(merge, cmesh, seg, offsets, sg.start, sg.start + sg.count, total, misc, lane, head, end);
     if (lane != 0u) return;
     const uint32_t merges = end.y - head.y;
     seg[k].mbase = head.y;
@@ -389,9 +432,9 @@ __global__ void __launch_bounds__(T256) k_b_compact(const Node32* __restrict__ c
                                                     const SegState* __restrict__ seg, BatchMisc* misc,
                                                     const uint2* __restrict__ offsets,
                                                     const MeshDesc* __restrict__ meshes, Node32* __restrict__ nodes,
-                                                    DpView dp) {
+                                                    DpView dp, uint32_t q) {
     __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
-    const uint32_t total = misc->total;
+    const uint32_t total = misc->total[q];
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     bool emit, mrg;
     uint32_t b;
@@ -432,14 +475,6 @@ __global__ void __launch_bounds__(T256) k_b_compact(const Node32* __restrict__ c
     } else {
         store_node(&next[pos], a);
     }
-}
-
-__global__ void k_b_after_pass(BatchMisc* misc) {
-    if (threadIdx.x == 0) misc->total = misc->totals[0];
-}
-
-__global__ void k_b_clear_active(BatchMisc* misc) {
-    if (threadIdx.x == 0) misc->active = 0u;
 }
 
 // Roots (ploc/mod.rs:499): the mesh's last cluster goes to its node 0.
@@ -572,17 +607,16 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
     uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
     for (;;) {
         const uint32_t blocks = (known + T256 - 1) / T256;
-        for (int q = 0; q < PASSES_PER_SYNC; ++q) {
-            k_b_clear_active<<<1, 64, 0, st>>>(misc);
+        for (int rep = 0; rep < PASSES_PER_SYNC; ++rep) {
+            const uint32_t q = pass & 1u;
             k_b_search<<<blocks, T256, 0, st>>>(cur, cmesh[cm], seg[cs], misc, d_meshes, rmax,
-                                                pass < sdt ? 1u : 0u, merge);
-            k_b_count<<<blocks, T256, 0, st>>>(merge, cmesh[cm], seg[cs], misc, blocksums);
-            k_b_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc);
+                                                pass < sdt ? 1u : 0u, merge, q);
+            k_b_count<<<blocks, T256, 0, st>>>(merge, cmesh[cm], seg[cs], misc, blocksums, q);
+            k_b_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, q);
             k_b_segments<<<(m + 3u) / 4u, T256, 0, st>>>(merge, cmesh[cm], seg[cs], seg[cs ^ 1], blocksums, m,
-                                                          misc);
+                                                          misc, q);
             k_b_compact<<<blocks, T256, 0, st>>>(cur, nxt, cmesh[cm], cmesh[cm ^ 1], merge, seg[cs], misc, blocksums,
-                                                 d_meshes, nodes, dp);
-            k_b_after_pass<<<1, 64, 0, st>>>(misc);
+                                                 d_meshes, nodes, dp, q);
             std::swap(cur, nxt);
             cm ^= 1;
             cs ^= 1;
@@ -592,10 +626,11 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
         OBVHS_TRY(hipMemcpyAsync(h, misc, 16, hipMemcpyDeviceToHost, st), "read batch state");
         OBVHS_TRY(hipStreamSynchronize(st), "batch pass sync");
         if (h[0]) return device_error_to_status(h[0]);
-        if (h[2] == 0u) break;  // every mesh is down to its root
+        if (h[3] == 0u) break;  // every mesh is down to its root
         // Every active mesh merges at least one pair per pass unless costs are NaN.
-        if (h[1] >= known) return OBVHS_ERR_PLOC_SELF_MERGE;
-        known = h[1];
+        const uint32_t now = h[1 + (pass & 1u)];
+        if (now >= known) return OBVHS_ERR_PLOC_SELF_MERGE;
+        known = now;
     }
     B->passes = pass;
     k_b_roots<<<mblk, T256, 0, st>>>(cur, seg[cs], d_meshes, m, nodes, B->bvh2_prims.as<uint32_t>(), misc);

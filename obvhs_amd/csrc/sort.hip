@@ -5,13 +5,17 @@
 // oracle follows too).
 //
 // Per 8-bit digit pass (reduce-then-scan, three launches):
-//   upsweep   : per-tile digit histogram in LDS -> counts[digit][tile], plus the pass's digit totals;
-//   scan      : one workgroup per digit, exclusive scan of counts[digit][*] offset by the digits below it;
-//   downsweep : wave-level multisplit ranking (eight 64-lane ballots per item) gives each key its stable rank
-//               inside the tile, the tile is reordered in LDS and written out coalesced per digit run.
+//   upsweep   : per-tile digit histogram in LDS -> counts[digit][tile] (no global atomics);
+//   scan      : one workgroup per digit, exclusive scan of counts[digit][*] within the digit, and the digit's
+//               total;
+//   downsweep : the digit bases (an exclusive scan of the 256 totals, in every tile), then wave-level multisplit
+//               ranking (eight 64-lane ballots per item) gives each key its stable rank inside the tile; the tile's
+//               keys are reordered in LDS and written out coalesced per digit run, then its values through the same
+//               LDS (half the LDS of staging both at once, so twice the tiles per CU).
 // Tiles are 256 threads x ITEMS keys (ITEMS = 4, 8 or 16, the sort tile the context's tuning table picks per size
 // class; 8 by default); each wave owns a contiguous 64 x ITEMS key segment.  Every tile size gives the same order.
 #include "common.hpp"
+#include "device_util.hpp"
 #include "internal.hpp"
 
 namespace obvhs {
@@ -22,15 +26,10 @@ constexpr int SORT_THREADS = 256;
 constexpr int SORT_MIN_ITEMS = 4;  // the scratch is sized for the smallest tile (the most tiles)
 constexpr int SORT_WAVES = SORT_THREADS / 64;
 
-__global__ void __launch_bounds__(SORT_THREADS) k_zero_u32(uint32_t* p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
 template <int ITEMS>
 __global__ void __launch_bounds__(SORT_THREADS)
     k_radix_upsweep(const uint64_t* __restrict__ keys, uint32_t n, uint32_t shift, uint32_t num_tiles,
-                    uint32_t* __restrict__ counts, uint32_t* __restrict__ totals) {
+                    uint32_t* __restrict__ counts) {
     constexpr int SORT_TILE = SORT_THREADS * ITEMS;
     __shared__ uint32_t hist[256];
     const uint32_t tid = threadIdx.x;
@@ -46,64 +45,70 @@ __global__ void __launch_bounds__(SORT_THREADS)
         }
     }
     __syncthreads();
-    uint32_t c = hist[tid];
-    counts[tid * num_tiles + blockIdx.x] = c;
-    if (c) atomicAdd(&totals[tid], c);
+    counts[tid * num_tiles + blockIdx.x] = hist[tid];
 }
 
-// Exclusive block scan of one u32 per thread (256 threads).
-__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* tmp /*256*/, uint32_t& total) {
-    const uint32_t tid = threadIdx.x;
-    tmp[tid] = v;
-    __syncthreads();
-#pragma unroll
-    for (uint32_t off = 1; off < 256; off <<= 1) {
-        uint32_t add = tid >= off ? tmp[tid - off] : 0u;
-        __syncthreads();
-        tmp[tid] += add;
-        __syncthreads();
+// Exclusive scans over the 256 threads of two u32 values at once (wave scans, then the four wave totals through
+// LDS); ta and tb receive the totals.  tmp: 8 words of LDS.
+__device__ __forceinline__ void block_exclusive_scan2_256(uint32_t a, uint32_t b, uint32_t* tmp, uint32_t& ea,
+                                                          uint32_t& eb, uint32_t& ta, uint32_t& tb) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t ia = wave_inclusive_scan(a), ib = wave_inclusive_scan(b);
+    if (lane == 63u) {
+        tmp[wave] = ia;
+        tmp[4 + wave] = ib;
     }
-    uint32_t incl = tmp[tid];
-    total = tmp[255];
     __syncthreads();
-    return incl - v;
+    uint32_t pa = 0u, pb = 0u;
+    ta = 0u;
+    tb = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < SORT_WAVES; ++w) {
+        const uint32_t xa = tmp[w], xb = tmp[4 + w];
+        if (w < wave) {
+            pa += xa;
+            pb += xb;
+        }
+        ta += xa;
+        tb += xb;
+    }
+    ea = pa + ia - a;
+    eb = pb + ib - b;
+    __syncthreads();  // tmp is reused by the caller's next scan
 }
 
+// One workgroup per digit: exclusive scan of the digit's per-tile counts within the digit, and the digit's total.
 __global__ void __launch_bounds__(SORT_THREADS)
-    k_radix_scan(uint32_t* __restrict__ counts, const uint32_t* __restrict__ totals, uint32_t num_tiles) {
-    __shared__ uint32_t tmp[256];
+    k_radix_scan(uint32_t* __restrict__ counts, uint32_t* __restrict__ totals, uint32_t num_tiles) {
+    __shared__ uint32_t tmp[8];
     const uint32_t d = blockIdx.x;
     const uint32_t tid = threadIdx.x;
-    // digit base = sum of totals of lower digits
-    uint32_t t = tid < d ? totals[tid] : 0u;
-    uint32_t dummy;
-    uint32_t tot_lower;
-    (void)block_exclusive_scan_256(t, tmp, tot_lower);
-    uint32_t carry = tot_lower;
+    uint32_t carry = 0u;
     uint32_t* row = counts + (size_t)d * num_tiles;
-    for (uint32_t start = 0; start < num_tiles; start += 256) {
-        uint32_t i = start + tid;
-        uint32_t v = i < num_tiles ? row[i] : 0u;
-        uint32_t chunk_total;
-        uint32_t ex = block_exclusive_scan_256(v, tmp, chunk_total);
+    for (uint32_t start = 0; start < num_tiles; start += 2 * SORT_THREADS) {
+        const uint32_t i = start + 2 * tid;
+        const uint32_t v0 = i < num_tiles ? row[i] : 0u, v1 = i + 1 < num_tiles ? row[i + 1] : 0u;
+        uint32_t ex, dummy_e, chunk_total, dummy_t;
+        block_exclusive_scan2_256(v0 + v1, 0u, tmp, ex, dummy_e, chunk_total, dummy_t);
         if (i < num_tiles) row[i] = carry + ex;
+        if (i + 1 < num_tiles) row[i + 1] = carry + ex + v0;
         carry += chunk_total;
     }
-    (void)dummy;
+    if (tid == 0) totals[d] = carry;
 }
 
 template <int ITEMS>
 __global__ void __launch_bounds__(SORT_THREADS)
     k_radix_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                       uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t shift,
-                      const uint32_t* __restrict__ offsets, uint32_t num_tiles) {
+                      const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ totals,
+                      uint32_t num_tiles) {
     constexpr int SORT_TILE = SORT_THREADS * ITEMS;
     __shared__ uint32_t whist[SORT_WAVES][256];
     __shared__ uint32_t digit_start[256];
     __shared__ uint32_t gbase[256];
-    __shared__ uint32_t scan_tmp[256];
-    __shared__ uint64_t skeys[SORT_TILE];
-    __shared__ uint32_t svals[SORT_TILE];
+    __shared__ uint32_t scan_tmp[8];
+    __shared__ uint64_t stage[SORT_TILE];  // the tile's keys, then its values
 
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid >> 6;
@@ -114,6 +119,7 @@ __global__ void __launch_bounds__(SORT_THREADS)
 
 #pragma unroll
     for (int w = 0; w < SORT_WAVES; ++w) whist[w][tid] = 0u;
+    const uint32_t dtotal = totals[tid], toff = offsets[(size_t)tid * num_tiles + tile];
     __syncthreads();
 
     const uint32_t seg = tile_base + wave * (64 * ITEMS);
@@ -152,28 +158,40 @@ __global__ void __launch_bounds__(SORT_THREADS)
         whist[w][tid] = run;
         run += c;
     }
-    uint32_t tile_total;
-    uint32_t start = block_exclusive_scan_256(run, scan_tmp, tile_total);
+    uint32_t start, dbase, tile_total, n_total;
+    block_exclusive_scan2_256(run, dtotal, scan_tmp, start, dbase, tile_total, n_total);
     digit_start[tid] = start;
-    gbase[tid] = offsets[(size_t)tid * num_tiles + tile];
+    gbase[tid] = dbase + toff;
     __syncthreads();
+    uint32_t pos[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-        uint32_t idx = seg + j * 64 + lane;
-        if (idx < n) {
-            uint32_t d = (uint32_t)(k[j] >> shift) & 0xffu;
-            uint32_t pos = digit_start[d] + whist[wave][d] + rank[j];
-            skeys[pos] = k[j];
-            svals[pos] = v[j];
+        const uint32_t d = (uint32_t)(k[j] >> shift) & 0xffu;
+        pos[j] = digit_start[d] + whist[wave][d] + rank[j];
+        if (seg + j * 64 + lane < n) stage[pos[j]] = k[j];
+    }
+    __syncthreads();
+    uint32_t dst[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t s = tid + (uint32_t)j * SORT_THREADS;
+        if (s < tile_count) {
+            const uint64_t key = stage[s];
+            const uint32_t d = (uint32_t)(key >> shift) & 0xffu;
+            dst[j] = gbase[d] + (s - digit_start[d]);
+            kout[dst[j]] = key;
         }
     }
     __syncthreads();
-    for (uint32_t s = tid; s < tile_count; s += SORT_THREADS) {
-        uint64_t key = skeys[s];
-        uint32_t d = (uint32_t)(key >> shift) & 0xffu;
-        uint32_t dst = gbase[d] + (s - digit_start[d]);
-        kout[dst] = key;
-        vout[dst] = svals[s];
+    uint32_t* sv = reinterpret_cast<uint32_t*>(stage);
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+        if (seg + j * 64 + lane < n) sv[pos[j]] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t s = tid + (uint32_t)j * SORT_THREADS;
+        if (s < tile_count) vout[dst[j]] = sv[s];
     }
 }
 
@@ -186,7 +204,7 @@ size_t radix_sort_scratch_bytes(uint32_t n) {
     b += (size_t)n * 8;                     // alternate keys
     b += (size_t)n * 4;                     // alternate values
     b += (size_t)256 * tiles * 4 + 256;     // counts
-    b += 8 * 256 * 4 + 256;                 // per-pass totals
+    b += 256 * 4 + 256;                     // digit totals of the pass
     return b + 1024;
 }
 
@@ -207,21 +225,19 @@ static hipError_t radix_sort_impl(uint64_t* keys, uint32_t* values, bool values_
     uint64_t* kalt = reinterpret_cast<uint64_t*>(take((size_t)n * 8));
     uint32_t* valt = reinterpret_cast<uint32_t*>(take((size_t)n * 4));
     uint32_t* counts = reinterpret_cast<uint32_t*>(take((size_t)256 * tiles * 4));
-    uint32_t* totals = reinterpret_cast<uint32_t*>(take(8 * 256 * 4));
+    uint32_t* totals = reinterpret_cast<uint32_t*>(take(256 * 4));
     int passes = (bits + 7) / 8;
-    k_zero_u32<<<(8 * 256 + 255) / 256, 256, 0, stream>>>(totals, 8 * 256);
     uint64_t* kin = keys;
     uint32_t* vin = values;
     uint64_t* kout = kalt;
     uint32_t* vout = valt;
     for (int pass = 0; pass < passes; ++pass) {
         uint32_t shift = (uint32_t)(pass * 8);
-        k_radix_upsweep<ITEMS><<<tiles, SORT_THREADS, 0, stream>>>(kin, n, shift, tiles, counts,
-                                                                   totals + pass * 256);
-        k_radix_scan<<<256, SORT_THREADS, 0, stream>>>(counts, totals + pass * 256, tiles);
+        k_radix_upsweep<ITEMS><<<tiles, SORT_THREADS, 0, stream>>>(kin, n, shift, tiles, counts);
+        k_radix_scan<<<256, SORT_THREADS, 0, stream>>>(counts, totals, tiles);
         const uint32_t* vsrc = (pass == 0 && values_identity) ? nullptr : vin;
         k_radix_downsweep<ITEMS><<<tiles, SORT_THREADS, 0, stream>>>(kin, vsrc, kout, vout, n, shift, counts,
-                                                                     tiles);
+                                                                     totals, tiles);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
