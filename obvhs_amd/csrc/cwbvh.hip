@@ -150,12 +150,10 @@ struct EmitOut {
 
 // One CWBVH node by the 8-lane group of lane r (every lane of the group calls with the same t).
 __device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, const uint32_t* __restrict__ bvh2_prims,
-                                          const DpView& dp, const CwItem* __restrict__ in_items, uint32_t t,
-                                          uint32_t r, uint32_t level, uint32_t order_children,
-                                          CwNode* __restrict__ out, uint32_t* __restrict__ out_prims, uint32_t cw_cap,
-                                          uint32_t prim_cap, CwMisc* misc, const MeshDesc* __restrict__ meshes,
-                                          EmitOut& eo) {
-    const CwItem it = in_items[t];
+                                          const DpView& dp, const CwItem it, uint32_t r, uint32_t level,
+                                          uint32_t order_children, CwNode* __restrict__ out,
+                                          uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
+                                          CwMisc* misc, const MeshDesc* __restrict__ meshes, EmitOut& eo) {
     const Node32* nds = nodes;
     const uint32_t* bprims = bvh2_prims;
     CwNode* onodes = out;
@@ -368,8 +366,8 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
         eo.isint = 0u;
         eo.rank = 0u;
         if (t < count)
-            emit_node(nodes, bvh2_prims, dp, in_items, t, r, level, order_children, out, out_prims, cw_cap, prim_cap,
-                      misc, meshes, eo);
+            emit_node(nodes, bvh2_prims, dp, in_items[t], r, level, order_children, out, out_prims, cw_cap,
+                      prim_cap, misc, meshes, eo);
         if (threadIdx.x == 0) s_count = 0u;
         __syncthreads();
         uint32_t off = 0u;
@@ -844,7 +842,8 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
         const uint32_t g = (uint32_t)std::min<uint64_t>((cap * 8 + T256 - 1) / T256, EMIT_MAX_BLOCKS);
         k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2->prim_indices.as<uint32_t>(), dp, items[level & 1],
                                       items[(level + 1) & 1], level, order_children ? 1u : 0u,
-                                      out->nodes.as<CwNode>(), out->prim_indices.as<uint32_t>(), ncw, n, misc, nullptr);
+                                      out->nodes.as<CwNode>(), out->prim_indices.as<uint32_t>(), ncw, n, misc,
+                                      nullptr);
         bound *= 8;
     }
     OBVHS_TRY(hipGetLastError(), "cwbvh emit launch");
