@@ -48,6 +48,7 @@ struct CwItem {
 struct CwMisc {
     uint32_t err;
     uint32_t level_count[MAX_CW_LEVELS + 1];
+    uint32_t info[10];  // k_cw_info: the BVH2 root (8 words), the CWBVH node count and depth
 };
 
 __global__ void k_cw_init(CwItem* items, CwMisc* misc) {
@@ -57,6 +58,31 @@ __global__ void k_cw_init(CwItem* items, CwMisc* misc) {
         items[0] = CwItem{0u, 0u, 0u, 0u, 0u};
         misc->level_count[0] = 1u;
     }
+}
+
+// What the host needs before the emit, gathered so that one copy brings it with the error word: the BVH2 root, and
+// the size and depth of the CWBVH subtree it roots (1 and 1 for a single-leaf BVH2).
+__global__ void k_cw_info(const Node32* __restrict__ nodes, DpView dp, CwMisc* misc) {
+    if (threadIdx.x != 0) return;
+    const Node32 root = load_node(&nodes[0]);
+    const uint4* w = reinterpret_cast<const uint4*>(&root);
+    const uint4 a = w[0], b = w[1];
+    misc->info[0] = a.x;
+    misc->info[1] = a.y;
+    misc->info[2] = a.z;
+    misc->info[3] = a.w;
+    misc->info[4] = b.x;
+    misc->info[5] = b.y;
+    misc->info[6] = b.z;
+    misc->info[7] = b.w;
+    uint32_t size = 1u, depth = 1u;
+    if (root.prim_count == 0u) {
+        const DpRec& e = dp.rec[dkey(root.first_index)];
+        size = e.cwsize;
+        depth = e.cwdepth;
+    }
+    misc->info[8] = size;
+    misc->info[9] = depth;
 }
 
 __device__ __forceinline__ void dp_node(const Node32* __restrict__ nodes, const DpView& dp, uint32_t x,
@@ -811,25 +837,17 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
         bvh2->dp_max_prims = max_prims_per_leaf;
     }
     const DpView dp = bvh_dp_view(bvh2, max_prims_per_leaf);
-    // Root record, CWBVH node count and depth (the root's subtree size and levels) and device errors.
-    Node32 root;
-    OBVHS_TRY(hipMemcpyAsync(h, nodes, sizeof(Node32), hipMemcpyDeviceToHost, st), "read root");
-    OBVHS_TRY(hipStreamSynchronize(st), "root sync");
-    memcpy(&root, h, sizeof(Node32));
-    uint32_t ncw = 1, levels = 1;
-    if (root.prim_count == 0u) {
-        OBVHS_TRY(hipMemcpyAsync(h, &dp.rec[dkey(root.first_index)].cwsize, 4, hipMemcpyDeviceToHost, st),
-                  "read cw size");
-        OBVHS_TRY(hipMemcpyAsync(h + 2, &dp.rec[dkey(root.first_index)].cwdepth, 4, hipMemcpyDeviceToHost, st),
-                  "read cw depth");
-    }
-    OBVHS_TRY(hipMemcpyAsync(h + 1, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cw err");
+    // Root, CWBVH node count and depth (the root's subtree size and levels) and device errors: one copy.
+    k_cw_info<<<1, 64, 0, st>>>(nodes, dp, misc);
+    OBVHS_TRY(hipGetLastError(), "cwbvh info launch");
+    OBVHS_TRY(hipMemcpyAsync(h, misc, sizeof(CwMisc), hipMemcpyDeviceToHost, st), "read cw info");
     OBVHS_TRY(hipStreamSynchronize(st), "cwbvh dp sync");
-    if (h[1]) return device_error_to_status(h[1]);
-    if (root.prim_count == 0u) {
-        ncw = h[0];
-        levels = h[2];
-    }
+    CwMisc hm;
+    memcpy(&hm, h, sizeof(CwMisc));
+    if (hm.err) return device_error_to_status(hm.err);
+    Node32 root;
+    memcpy(&root, hm.info, sizeof(Node32));
+    const uint32_t ncw = hm.info[8], levels = hm.info[9];
     if (ncw == 0u) return OBVHS_ERR_CW_DISTRIBUTE;  // some CWBVH node's children cannot be distributed
     if (ncw > N) return OBVHS_ERR_CAPACITY;
     if (levels > MAX_CW_LEVELS) return OBVHS_ERR_DEPTH;
@@ -847,11 +865,11 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
         bound *= 8;
     }
     OBVHS_TRY(hipGetLastError(), "cwbvh emit launch");
-    OBVHS_TRY(hipMemcpyAsync(h, &misc->level_count[levels], 4, hipMemcpyDeviceToHost, st), "read level");
-    OBVHS_TRY(hipMemcpyAsync(h + 1, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cw err");
+    // the error word and the count of the level below the last (err and level_count are adjacent): one copy
+    OBVHS_TRY(hipMemcpyAsync(h, misc, (2 + (size_t)levels) * 4, hipMemcpyDeviceToHost, st), "read cw err");
     OBVHS_TRY(hipStreamSynchronize(st), "cwbvh emit sync");
-    if (h[1]) return device_error_to_status(h[1]);
-    if (h[0] != 0u) return OBVHS_ERR_DEPTH;  // deeper than the depth table said (cannot happen)
+    if (h[0]) return device_error_to_status(h[0]);
+    if (h[1 + levels] != 0u) return OBVHS_ERR_DEPTH;  // deeper than the depth table said (cannot happen)
     // total_aabb = bvh2.nodes[0].aabb (bvh2_to_cwbvh.rs:506)
     out->total_aabb.min = obvhs_vec3a{root.minx, root.miny, root.minz, 0.0f};
     out->total_aabb.max = obvhs_vec3a{root.maxx, root.maxy, root.maxz, 0.0f};
@@ -924,11 +942,10 @@ obvhs_status cwbvh_convert_batch(obvhs_context* ctx, const Node32* nodes, const 
     }
     OBVHS_TRY(hipGetLastError(), "batch cwbvh emit launch");
     uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
-    OBVHS_TRY(hipMemcpyAsync(h, &misc->level_count[levels], 4, hipMemcpyDeviceToHost, st), "read level");
-    OBVHS_TRY(hipMemcpyAsync(h + 1, &misc->err, 4, hipMemcpyDeviceToHost, st), "read cw err");
+    OBVHS_TRY(hipMemcpyAsync(h, misc, (2 + (size_t)levels) * 4, hipMemcpyDeviceToHost, st), "read cw err");
     OBVHS_TRY(hipStreamSynchronize(st), "batch cwbvh emit sync");
-    if (h[1]) return device_error_to_status(h[1]);
-    if (h[0] != 0u) return OBVHS_ERR_DEPTH;
+    if (h[0]) return device_error_to_status(h[0]);
+    if (h[1 + levels] != 0u) return OBVHS_ERR_DEPTH;
     return OBVHS_OK;
 }
 

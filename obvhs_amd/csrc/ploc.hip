@@ -21,7 +21,9 @@
 // ploc/rebuild.rs:136-183).  For a partial rebuild the children of the g-th merge (counted over all passes in the
 // serial loop's order) go to the g-th freed sibling pair from the top (`slot_map`, the REBUILD placement of
 // ploc/mod.rs:449-462); the plain placement is the special case slot_map[g] = 2n-3-2g.
+#include <stddef.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
 
@@ -776,6 +778,21 @@ static obvhs_status sort_codes(obvhs_context* ctx, uint64_t* keys, const uint64_
 // The PLOC passes from the sorted clusters in `cur` to the root (ploc/mod.rs:327-499): grid passes while more than
 // TAIL_MAX clusters remain, then the one-workgroup tail.  `fin`
 // receives the final pass state.
+// The error word and the pass state of `slot` with one copy (err, pad, state[0], state[1] are adjacent) into the
+// pinned staging area, then a stream synchronisation.
+static hipError_t read_state(const Misc* misc, uint32_t slot, void* pinned, hipStream_t st, PlocState& state,
+                             uint32_t& err) {
+    const size_t lo = offsetof(Misc, err), hi = offsetof(Misc, state) + (slot + 1u) * sizeof(PlocState);
+    hipError_t e = hipMemcpyAsync(pinned, reinterpret_cast<const char*>(misc) + lo, hi - lo, hipMemcpyDeviceToHost,
+                                  st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    const char* h = static_cast<const char*>(pinned);
+    memcpy(&err, h, 4);
+    memcpy(&state, h + (offsetof(Misc, state) + slot * sizeof(PlocState) - lo), sizeof(PlocState));
+    return hipSuccess;
+}
+
 static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uint32_t n, uint32_t r,
                                uint32_t search_depth_threshold, Node32* bvh_nodes, uint32_t* hist, DpView dp,
                                uint32_t fused, const uint32_t* slot_map, uint32_t vtop, uint32_t map_len,
@@ -788,7 +805,7 @@ static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uin
     uint32_t known = n;
     uint32_t pass = 0;
     uint32_t slot = 0;
-    PlocState* hstate = static_cast<PlocState*>(ctx->pinned);
+    PlocState hstate{};
     while (known > TAIL_MAX) {
         const uint32_t blocks = (known + T256 - 1) / T256;
         for (int k = 0; k < PASSES_PER_SYNC; ++k) {
@@ -808,29 +825,23 @@ static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uin
             pass++;
         }
         OBVHS_TRY(hipGetLastError(), "ploc pass launch");
-        uint32_t* herr_loop = reinterpret_cast<uint32_t*>(hstate + 1);
-        OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
-                  "read ploc state");
-        OBVHS_TRY(hipMemcpyAsync(herr_loop, &misc->err, 4, hipMemcpyDeviceToHost, st), "read err");
-        OBVHS_TRY(hipStreamSynchronize(st), "ploc sync");
-        if (*herr_loop) return device_error_to_status(*herr_loop);
+        uint32_t herr_loop = 0u;
+        OBVHS_TRY(read_state(misc, slot, ctx->pinned, st, hstate, herr_loop), "read ploc state");
+        if (herr_loop) return device_error_to_status(herr_loop);
         // Every pass merges at least one pair unless costs are NaN; a batch without progress would never end.
-        if (hstate->count >= known) return OBVHS_ERR_PLOC_SELF_MERGE;
-        known = hstate->count;
+        if (hstate.count >= known) return OBVHS_ERR_PLOC_SELF_MERGE;
+        known = hstate.count;
         // The device depth counts real passes only; later enqueued passes may have been no-ops.
-        pass = hstate->depth;
+        pass = hstate.depth;
     }
     k_ploc_tail<<<1, TAIL_THREADS, 0, st>>>(cur, misc, slot, pass, r, r == 1 ? 1u : 0u, search_depth_threshold,
                                             bvh_nodes, hist, dp, fused, slot_map, vtop, map_len);
     OBVHS_TRY(hipGetLastError(), "ploc tail launch");
-    uint32_t* herr = reinterpret_cast<uint32_t*>(hstate + 1);
-    OBVHS_TRY(hipMemcpyAsync(hstate, &misc->state[slot], sizeof(PlocState), hipMemcpyDeviceToHost, st),
-              "read ploc state");
-    OBVHS_TRY(hipMemcpyAsync(herr, &misc->err, 4, hipMemcpyDeviceToHost, st), "read err");
-    OBVHS_TRY(hipStreamSynchronize(st), "ploc final sync");
-    if (*herr) return device_error_to_status(*herr);
-    if (hstate->insert != 1u) return OBVHS_ERR_PLOC_SELF_MERGE;  // ploc/mod.rs:495-497
-    fin = *hstate;
+    uint32_t herr = 0u;
+    OBVHS_TRY(read_state(misc, slot, ctx->pinned, st, hstate, herr), "read final ploc state");
+    if (herr) return device_error_to_status(herr);
+    if (hstate.insert != 1u) return OBVHS_ERR_PLOC_SELF_MERGE;  // ploc/mod.rs:495-497
+    fin = hstate;
     return OBVHS_OK;
 }
 
