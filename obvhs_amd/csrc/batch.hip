@@ -59,18 +59,23 @@ struct SegState {
 struct BatchMisc {
     uint32_t err;
     uint32_t total[2];   // clusters in the current array (by pass parity)
-    uint32_t active;     // meshes with more than one cluster after the last pass
+    uint32_t active;     // 1 if a mesh has more than one cluster after the last pass, else 0
     uint32_t totals[2];  // emits, merges of the pass
 };
 
-__device__ __forceinline__ uint32_t mesh_of(const uint32_t* __restrict__ offsets, uint32_t m, uint32_t g) {
-    uint32_t lo = 0, hi = m;  // offsets[lo] <= g < offsets[hi]
+__device__ __forceinline__ uint32_t mesh_in(const uint32_t* __restrict__ offsets, uint32_t lo, uint32_t hi,
+                                            uint32_t g) {
+    // offsets[lo] <= g < offsets[hi]
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (offsets[mid] <= g) lo = mid;
         else hi = mid;
     }
     return lo;
+}
+
+__device__ __forceinline__ uint32_t mesh_of(const uint32_t* __restrict__ offsets, uint32_t m, uint32_t g) {
+    return mesh_in(offsets, 0u, m, g);
 }
 
 __global__ void __launch_bounds__(T256) k_b_init(uint32_t* __restrict__ bounds, const uint32_t* __restrict__ offsets,
@@ -93,17 +98,31 @@ __global__ void __launch_bounds__(T256) k_b_init(uint32_t* __restrict__ bounds, 
 }
 
 // Leaf boxes (triangle.rs:28-30) and per-mesh bounds from both corners (ploc/mod.rs:189-190).
-__global__ void __launch_bounds__(T256) k_b_leaf_init(const float4* __restrict__ tris,
-                                                      const uint32_t* __restrict__ offsets, uint32_t m,
-                                                      uint32_t n_total, Node32* __restrict__ leaves,
-                                                      uint32_t* __restrict__ bounds, BatchMisc* misc) {
-    const uint32_t g = blockIdx.x * T256 + threadIdx.x;
+// Also records every leaf's mesh (leaf_mesh) and its mesh-local primitive index (prims, the BVH2's
+// primitive_indices before the build).  The meshes of a block's triangles lie between those of its first and last
+// triangle, found once per block, so each thread searches only that range.  A block inside one mesh reduces its
+// bounds in LDS and updates them with one set of atomics (the updates of one mesh serialise on its six words).
+constexpr int LEAF_T = 1024;
+__global__ void __launch_bounds__(LEAF_T) k_b_leaf_init(const float4* __restrict__ tris,
+                                                        const uint32_t* __restrict__ offsets, uint32_t m,
+                                                        uint32_t n_total, Node32* __restrict__ leaves,
+                                                        uint32_t* __restrict__ bounds, BatchMisc* misc,
+                                                        uint32_t* __restrict__ leaf_mesh,
+                                                        uint32_t* __restrict__ prims) {
+    __shared__ uint32_t s_range[2];
+    __shared__ uint32_t s_k[LEAF_T / 64][6];
+    const uint32_t g = blockIdx.x * LEAF_T + threadIdx.x;
     const bool valid = g < n_total;
+    if (threadIdx.x == 0) s_range[0] = mesh_of(offsets, m, blockIdx.x * LEAF_T);  // the grid has no empty block
+    if (threadIdx.x == 64) s_range[1] = mesh_of(offsets, m, min(blockIdx.x * LEAF_T + LEAF_T, n_total) - 1u);
+    __syncthreads();
     uint32_t mesh = 0xffffffffu;
     uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
     bool nan = false;
     if (valid) {
-        mesh = mesh_of(offsets, m, g);
+        mesh = mesh_in(offsets, s_range[0], s_range[1] + 1u, g);
+        leaf_mesh[g] = mesh;
+        prims[g] = g - offsets[mesh];
         const float4 v0 = tris[3 * (size_t)g], v1 = tris[3 * (size_t)g + 1], v2 = tris[3 * (size_t)g + 2];
         Box b{v0.x, v0.y, v0.z, v0.x, v0.y, v0.z};
         b = Box{smin(b.lx, v1.x), smin(b.ly, v1.y), smin(b.lz, v1.z), smax(b.hx, v1.x), smax(b.hy, v1.y),
@@ -120,7 +139,25 @@ __global__ void __launch_bounds__(T256) k_b_leaf_init(const float4* __restrict__
         }
     }
     if (__ballot(nan) != 0ull && __lane_id() == 0) atomicOr(&misc->err, DERR_NAN_INPUT);
-    // Wave-reduce when the whole wave lies in one mesh (the common case), else per-lane atomics.
+    if (s_range[0] == s_range[1]) {  // the block lies in one mesh (block-uniform branch)
+        for (int a = 0; a < 3; ++a) {
+            k[a] = wave_min_u32(k[a]);
+            k[3 + a] = wave_max_u32(k[3 + a]);
+        }
+        const uint32_t wave = threadIdx.x >> 6;
+        if (__lane_id() == 0)
+            for (int a = 0; a < 6; ++a) s_k[wave][a] = k[a];
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const uint32_t a = threadIdx.x;
+            uint32_t v = s_k[0][a];
+            for (int w = 1; w < LEAF_T / 64; ++w) v = a < 3 ? min(v, s_k[w][a]) : max(v, s_k[w][a]);
+            if (a < 3) atomicMin(&bounds[s_range[0] * 6 + a], v);
+            else atomicMax(&bounds[s_range[0] * 6 + a], v);
+        }
+        return;
+    }
+    // Wave-reduce when the whole wave lies in one mesh, else per-lane atomics.
     const uint32_t m0 = (uint32_t)__shfl((int)mesh, 0);
     const bool uniform = __ballot(mesh == m0) == __ballot(true);
     if (uniform) {
@@ -143,11 +180,11 @@ __global__ void __launch_bounds__(T256) k_b_leaf_init(const float4* __restrict__
 }
 
 __global__ void __launch_bounds__(T256) k_b_morton(const Node32* __restrict__ leaves,
-                                                   const uint32_t* __restrict__ offsets, uint32_t m, uint32_t n_total,
+                                                   const uint32_t* __restrict__ leaf_mesh, uint32_t n_total,
                                                    const uint32_t* __restrict__ bounds, uint64_t* __restrict__ keys) {
     const uint32_t g = blockIdx.x * T256 + threadIdx.x;
     if (g >= n_total) return;
-    const uint32_t mesh = mesh_of(offsets, m, g);
+    const uint32_t mesh = leaf_mesh[g];
     const uint32_t* bk = bounds + mesh * 6;
     const float mnx = f32_from_order_key(bk[0]), mny = f32_from_order_key(bk[1]), mnz = f32_from_order_key(bk[2]);
     const float mxx = f32_from_order_key(bk[3]), mxy = f32_from_order_key(bk[4]), mxz = f32_from_order_key(bk[5]);
@@ -157,22 +194,23 @@ __global__ void __launch_bounds__(T256) k_b_morton(const Node32* __restrict__ le
 }
 
 __global__ void __launch_bounds__(T256) k_b_mesh_keys(const uint32_t* __restrict__ vals,
-                                                      const uint32_t* __restrict__ offsets, uint32_t m,
-                                                      uint32_t n_total, uint64_t* __restrict__ keys) {
+                                                      const uint32_t* __restrict__ leaf_mesh, uint32_t n_total,
+                                                      uint64_t* __restrict__ keys) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     if (i >= n_total) return;
-    keys[i] = mesh_of(offsets, m, vals[i]);
+    keys[i] = leaf_mesh[vals[i]];
 }
 
+// After the mesh sort, keys[i] is the mesh of the leaf order[i].
 __global__ void __launch_bounds__(T256) k_b_gather(const Node32* __restrict__ leaves,
                                                    const uint32_t* __restrict__ order,
-                                                   const uint32_t* __restrict__ offsets, uint32_t m, uint32_t n_total,
+                                                   const uint64_t* __restrict__ mesh_keys, uint32_t n_total,
                                                    Node32* __restrict__ out, uint32_t* __restrict__ out_mesh) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     if (i >= n_total) return;
     const uint32_t g = order[i];
     store_node(&out[i], load_node(&leaves[g]));
-    out_mesh[i] = mesh_of(offsets, m, g);
+    out_mesh[i] = (uint32_t)mesh_keys[i];
 }
 
 // Neighbour pass (ploc/mod.rs:329-419) with windows clipped to the cluster's mesh segment.
@@ -298,7 +336,7 @@ __global__ void __launch_bounds__(T256) k_b_count(const int8_t* __restrict__ mer
     }
 }
 
-// Also writes the next pass's cluster count (total[q ^ 1]) and clears the active-mesh counter k_b_segments fills.
+// Also writes the next pass's cluster count (total[q ^ 1]) and clears the active-mesh flag k_b_segments sets.
 __global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, uint32_t nblocks, BatchMisc* misc,
                                                  uint32_t q) {
     __shared__ uint32_t se[16], sm[16];
@@ -409,21 +447,27 @@ __global__ void __launch_bounds__(T256) k_b_segments(const int8_t* __restrict__ 
                                                      uint32_t m, BatchMisc* misc, uint32_t q) {
     const uint32_t k = (blockIdx.x * T256 + threadIdx.x) >> 6;
     const uint32_t lane = threadIdx.x & 63u;
-    if (k >= m) return;  // wave-uniform
-    const SegState sg = seg[k];
-    if (sg.count == 0u) {
-        if (lane == 0u) seg_next[k] = sg;
-        return;
+    bool active = false;
+    if (k < m) {  // wave-uniform
+        const SegState sg = seg[k];
+        if (sg.count == 0u) {
+            if (lane == 0u) seg_next[k] = sg;
+        } else {
+            const uint32_t total = misc->total[q];
+            uint2 head, end;
+            b_prefix_pair(merge, cmesh, seg, offsets, sg.start, sg.start + sg.count, total, misc, lane, head, end);
+            if (lane == 0u) {
+                const uint32_t merges = end.y - head.y;
+                seg[k].mbase = head.y;
+                const SegState nx{head.x, end.x - head.x, sg.insert - 2u * merges, 0u};
+                seg_next[k] = nx;
+                active = nx.count > 1u;
+            }
+        }
     }
-    const uint32_t total = misc->total[q];
-    uint2 head, end;
-    b_prefix_pair(merge, cmesh, seg, offsets, sg.start, sg.start + sg.count, total, misc, lane, head, end);
-    if (lane != 0u) return;
-    const uint32_t merges = end.y - head.y;
-    seg[k].mbase = head.y;
-    SegState nx{head.x, end.x - head.x, sg.insert - 2u * merges, 0u};
-    seg_next[k] = nx;
-    if (nx.count > 1u) atomicAdd(&misc->active, 1u);
+    // One flag write per block: the host only asks whether any mesh is still active, and 4,096 waves adding to
+    // one counter serialise on it.
+    if (__syncthreads_or(active) && threadIdx.x == 0) misc->active = 1u;
 }
 
 __global__ void __launch_bounds__(T256) k_b_compact(const Node32* __restrict__ cur, Node32* __restrict__ next,
@@ -492,14 +536,6 @@ __global__ void __launch_bounds__(T256) k_b_roots(const Node32* __restrict__ cur
         return;
     }
     store_node(&nodes[md.node_base], load_node(&cur[sg.start]));
-}
-
-__global__ void __launch_bounds__(T256) k_b_iota_local(uint32_t* __restrict__ prims,
-                                                       const uint32_t* __restrict__ offsets, uint32_t m,
-                                                       uint32_t n_total) {
-    const uint32_t g = blockIdx.x * T256 + threadIdx.x;
-    if (g >= n_total) return;
-    prims[g] = g - offsets[mesh_of(offsets, m, g)];
 }
 
 // Root box of every mesh's BVH2 (the BLAS bounds a TLAS is built over), as Aabb records.
@@ -584,20 +620,22 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
 
     const uint32_t mblk = (m + T256 - 1) / T256;
     k_b_init<<<mblk, T256, 0, st>>>(bounds, offsets, m, seg[0], misc, n);
-    k_b_leaf_init<<<nblk, T256, 0, st>>>(reinterpret_cast<const float4*>(d_tris), offsets, m, n, leaves, bounds,
-                                         misc);
-    k_b_morton<<<nblk, T256, 0, st>>>(leaves, offsets, m, n, bounds, keys);
+    // every leaf's mesh, in the second cluster-mesh buffer (free until the first pass)
+    uint32_t* leaf_mesh = cmesh[1];
+    k_b_leaf_init<<<(n + LEAF_T - 1) / LEAF_T, LEAF_T, 0, st>>>(reinterpret_cast<const float4*>(d_tris), offsets, m,
+                                                                n, leaves, bounds, misc, leaf_mesh,
+                                                                B->bvh2_prims.as<uint32_t>());
+    k_b_morton<<<nblk, T256, 0, st>>>(leaves, leaf_mesh, n, bounds, keys);
     OBVHS_TRY(hipGetLastError(), "batch leaf launch");
     const int items = sort_items_for(ctx, n);
     OBVHS_TRY(radix_sort_pairs(keys, vals, true, n, 63, S.sort_scratch.ptr, st, items), "batch morton sort");
-    k_b_mesh_keys<<<nblk, T256, 0, st>>>(vals, offsets, m, n, keys);
+    k_b_mesh_keys<<<nblk, T256, 0, st>>>(vals, leaf_mesh, n, keys);
     int mbits = 1;
     while (mbits < 32 && (1ull << mbits) < m) mbits++;
     OBVHS_TRY(radix_sort_pairs(keys, vals, false, n, mbits, S.sort_scratch.ptr, st, items), "batch mesh sort");
     Node32* cur = S.clusters_a.as<Node32>();
     Node32* nxt = S.clusters_b.as<Node32>();
-    k_b_gather<<<nblk, T256, 0, st>>>(leaves, vals, offsets, m, n, cur, cmesh[0]);
-    k_b_iota_local<<<nblk, T256, 0, st>>>(B->bvh2_prims.as<uint32_t>(), offsets, m, n);
+    k_b_gather<<<nblk, T256, 0, st>>>(leaves, vals, keys, n, cur, cmesh[0]);
     OBVHS_TRY(hipGetLastError(), "batch gather launch");
 
     uint32_t known = n;
