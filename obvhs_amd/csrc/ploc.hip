@@ -40,6 +40,7 @@ constexpr int T256 = 256;
 constexpr uint32_t TAIL_MAX = 2048;  // 4096 measured 5% slower per build (the tail's passes are LDS and ALU bound)
 constexpr int TAIL_THREADS = 1024;
 constexpr int TAIL_ITEMS = TAIL_MAX / TAIL_THREADS;
+constexpr int TAIL_SEARCH_THREADS = 512;  // the tail's search threads; the rest evaluate the previous pass's entries
 constexpr uint32_t MAX_RADIUS = 32;
 // Passes enqueued between host reads of the cluster count (1.29M-triangle build: 4 -> 1.97 ms, 8 -> 1.92, 16 -> 1.93).
 constexpr int PASSES_PER_SYNC = 8;
@@ -582,10 +583,21 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
     bool self_merge = false;
     bool bad_slot = false;
     uint32_t dp_err = 0u;
+    // Cost-table entries of a pass's merges are evaluated during the next pass's search, by the other half of the
+    // workgroup: the search is ALU-bound and the entries wait on table reads, so the two overlap.
+    uint32_t nm_prev = 0u;  // merges of the previous pass (their first child slots in mlist)
+    auto dp_entries = [&](uint32_t j0, uint32_t stride) {
+        for (uint32_t j = j0; j < nm_prev; j += stride) {
+            const uint32_t first = mlist[j];
+            const Node32 l = load_node(&bvh_nodes[first]), r = load_node(&bvh_nodes[first + 1]);
+            dp_err |= dp_evaluate(dp, box_union(node_box(l), node_box(r)), first, l, r);
+        }
+    };
     while (count > 1) {
         csum += count;
         const bool dist1 = sd_is_one || pass < depth_threshold;
-        for (uint32_t i = tid; i < count; i += TAIL_THREADS) {
+        if (tid >= TAIL_SEARCH_THREADS) dp_entries(tid - TAIL_SEARCH_THREADS, TAIL_THREADS - TAIL_SEARCH_THREADS);
+        for (uint32_t i = tid; tid < TAIL_SEARCH_THREADS && i < count; i += TAIL_SEARCH_THREADS) {
             const Box me{lx[i], ly[i], lz[i], hx[i], hy[i], hz[i]};
             int8_t m;
             if (dist1) {
@@ -698,14 +710,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
             pe++;
         }
         __syncthreads();
-        if (fused) {
-            // Cost-table entries of this pass's parents, one thread each; their children are finished.
-            for (uint32_t j = tid; j < (total >> 16); j += TAIL_THREADS) {
-                const uint32_t first = mlist[j];
-                const Node32 l = load_node(&bvh_nodes[first]), r = load_node(&bvh_nodes[first + 1]);
-                dp_err |= dp_evaluate(dp, box_union(node_box(l), node_box(r)), first, l, r);
-            }
-        }
+        nm_prev = fused ? (total >> 16) : 0u;
         count = total & 0xffffu;
         insert -= 2u * (total >> 16);
         depth += 1u;
@@ -717,6 +722,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
             break;
         }
     }
+    dp_entries(tid, TAIL_THREADS);  // the last pass's entries
     if (self_merge) atomicOr(&misc->err, DERR_SELF_MERGE);
     if (bad_slot) atomicOr(&misc->err, DERR_CAPACITY);
     if (dp_err) atomicOr(&misc->err, dp_err);
