@@ -116,6 +116,7 @@ obvhs_status obvhs_context_destroy(obvhs_context* ctx) {
     S.sort_scratch.release();
     S.misc.release();
     S.cw_scratch.release();
+    S.cw_misc.release();
     S.trav_spill.release();
     ctx->quality.release_all();
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);

@@ -45,19 +45,44 @@ struct CwItem {
     uint32_t mesh;
 };
 
+// What the 8-lane emit of a node needs of the node itself, handed down by its parent (whose walk loaded it) so
+// that the node's emit starts with its own cost-table walk: the box, first_index and prim_count, and its entry's
+// decision-0 byte and children words.  An array of these runs beside each item list of that form.
+struct CwPre {
+    float lx, ly, lz, hx, hy, hz;
+    uint32_t first, pc;
+    uint32_t meta0;   // decision byte 0 of the node's entry (internal nodes)
+    uint32_t kx, ky;  // the entry's children words (first_index | KID_LEAF)
+    uint32_t pad;
+};
+
 struct CwMisc {
     uint32_t err;
     uint32_t level_count[MAX_CW_LEVELS + 1];
     uint32_t info[10];  // k_cw_info: the BVH2 root (8 words), the CWBVH node count and depth
+    CwItem root_item;   // level 0 of a single conversion
+    CwPre root_pre;
 };
 
-__global__ void k_cw_init(CwItem* items, CwMisc* misc) {
+__global__ void k_cw_init(CwMisc* misc) {
     if (threadIdx.x == 0) {
         misc->err = 0u;
         for (uint32_t l = 0; l <= MAX_CW_LEVELS; ++l) misc->level_count[l] = 0u;
-        items[0] = CwItem{0u, 0u, 0u, 0u, 0u};
         misc->level_count[0] = 1u;
     }
+}
+
+// The CwPre of BVH2 node x (slot x of nds; its entry keyed through d).
+__device__ __forceinline__ CwPre pre_of(const Node32* __restrict__ nds, const DpView& d, uint32_t x) {
+    const Node32 n = load_node(&nds[x]);
+    CwPre p{n.minx, n.miny, n.minz, n.maxx, n.maxy, n.maxz, n.first_index, n.prim_count, 0u, 0u, 0u, 0u};
+    if (n.prim_count == 0u) {
+        const DpRec& e = d.rec[dp_key(d, n.first_index)];
+        p.meta0 = dbyte(e.meta, 0u);
+        p.kx = e.kids.x;
+        p.ky = e.kids.y;
+    }
+    return p;
 }
 
 // What the host needs before the emit, gathered so that one copy brings it with the error word: the BVH2 root, and
@@ -83,6 +108,8 @@ __global__ void k_cw_info(const Node32* __restrict__ nodes, DpView dp, CwMisc* m
     }
     misc->info[8] = size;
     misc->info[9] = depth;
+    misc->root_item = CwItem{0u, 0u, 0u, 0u, 0u};
+    misc->root_pre = pre_of(nodes, dp, 0u);
 }
 
 __device__ __forceinline__ void dp_node(const Node32* __restrict__ nodes, const DpView& dp, uint32_t x,
@@ -172,12 +199,13 @@ struct EmitOut {
     uint32_t isint;
     uint32_t rank;
     CwItem item;
+    CwPre pre;
 };
 
-// One CWBVH node by the 8-lane group of lane r (every lane of the group calls with the same t).
+// One CWBVH node by the 8-lane group of lane r (every lane of the group calls with the same item).
 __device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, const uint32_t* __restrict__ bvh2_prims,
-                                          const DpView& dp, const CwItem it, uint32_t r, uint32_t level,
-                                          uint32_t order_children, CwNode* __restrict__ out,
+                                          const DpView& dp, const CwItem it, const CwPre pre, uint32_t r,
+                                          uint32_t level, uint32_t order_children, CwNode* __restrict__ out,
                                           uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
                                           CwMisc* misc, const MeshDesc* __restrict__ meshes, EmitOut& eo) {
     const Node32* nds = nodes;
@@ -196,38 +224,97 @@ __device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, cons
         pcap = md.prim_count;
         d.kbase = md.prim_base;
     }
-    const Node32 nd = load_node(&nds[it.bvh2]);
-    const Box nb = node_box(nd);
+    const Box nb{pre.lx, pre.ly, pre.lz, pre.hx, pre.hy, pre.hz};
 
-    // get_children (bvh2_to_cwbvh.rs:346-397), the same walk in every lane of the group
-    uint32_t ch[8];
-    uint32_t ckind = 0u;  // 2 bits per child: its decision-0 kind
-    int cc;
-    if (nd.prim_count != 0u) {  // a single-leaf BVH2: the root is its own only child
-        ch[0] = it.bvh2;
-        cc = 1;
+    // get_children (bvh2_to_cwbvh.rs:346-397) breadth first over the group: lane j holds entry j, left to right, of
+    // the expansion frontier (slot, the parent's children word for it, its distribute index).  Each round every
+    // lane loads its entry's node and cost-table entry at once (one dependent load per round, not per visited
+    // entry), DISTRIBUTE entries split in place into their two children, and the final frontier read left to right
+    // is the reference's left-first recursion order.  The loads of the last round serve the rest of the emit.
+    uint32_t fx = 0u, finfo = KID_LEAF, fdi = 0u;
+    int F;
+    uint32_t werr = 0u;
+    if (pre.pc != 0u) {  // a single-leaf BVH2: the root is its own only child
+        F = 1;
+        fx = it.bvh2;
     } else {
-        cc = get_children_kids(d, nd.first_index, node_decision(d, nd.first_index, 0), ch, &ckind);
+        const uint32_t dl = dleft(pre.meta0), dr = dright(pre.meta0);
+        if (dl == DIDX_INVALID || dr == DIDX_INVALID) werr = DERR_CW_DISTRIBUTE;
+        F = 2;
+        fx = pre.first + (r == 1u ? 1u : 0u);
+        finfo = r == 1u ? pre.ky : pre.kx;
+        fdi = r == 1u ? dr : dl;
     }
-    if (cc < 0) {
-        if (r == 0u) atomicOr(&misc->err, cc == -2 ? DERR_CW_DISTRIBUTE : DERR_CAPACITY);
+    Node32 cn{0.0f, 0.0f, 0.0f, 0u, 0.0f, 0.0f, 0.0f, 0u};
+    uint2 rmeta = make_uint2(0u, 0u), rkids = make_uint2(0u, 0u);
+    uint32_t rcw = 0u, rnp = 0u;
+    uint4 rpl = make_uint4(0u, 0u, 0u, 0u);
+    while (werr == 0u) {  // at most 8 rounds: every round but the last grows the frontier, which stops at 8
+        const bool mine = (int)r < F;
+        const bool internal = mine && !(finfo & KID_LEAF);
+        if (mine) cn = load_node(&nds[fx]);
+        if (internal) {
+            const DpRec& e = d.rec[dp_key(d, finfo & ~KID_LEAF)];
+            rmeta = e.meta;
+            rkids = e.kids;
+            rcw = e.cwsize;
+            rnp = e.nprims;
+            rpl = e.plist;
+        }
+        const uint32_t b = internal ? dbyte(rmeta, fdi) : 0u;
+        const bool dist = internal && dkind(b) == KIND_DISTRIBUTE;
+        if (dist && (dleft(b) == DIDX_INVALID || dright(b) == DIDX_INVALID)) werr = DERR_CW_DISTRIBUTE;
+        werr = g8_or(werr);
+        if (werr != 0u || g8_or(dist ? 1u : 0u) == 0u) break;
+        const uint32_t w = mine ? (dist ? 2u : 1u) : 0u;
+        const uint32_t incl = g8_inclusive_sum(w, r);
+        const int F2 = __shfl((int)incl, 7, 8);
+        if (F2 > 8) {  // more than 8 children
+            werr = DERR_CAPACITY;
+            break;
+        }
+        // lane r of the new frontier: entry part of the entry of source lane q whose range [incl - w, incl) holds r
+        int src = 0;
+        uint32_t part = 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t iq = (uint32_t)__shfl((int)incl, q, 8), wq = (uint32_t)__shfl((int)w, q, 8);
+            if (r + wq >= iq && r < iq) {
+                src = q;
+                part = r + wq - iq;
+            }
+        }
+        const uint32_t sx = (uint32_t)__shfl((int)fx, src, 8), sinfo = (uint32_t)__shfl((int)finfo, src, 8);
+        const uint32_t sdi = (uint32_t)__shfl((int)fdi, src, 8), sb = (uint32_t)__shfl((int)b, src, 8);
+        const uint32_t skx = (uint32_t)__shfl((int)rkids.x, src, 8), sky = (uint32_t)__shfl((int)rkids.y, src, 8);
+        const uint32_t sw = (uint32_t)__shfl((int)w, src, 8);
+        if (sw == 2u) {  // a DISTRIBUTE entry's children: slots first, first + 1
+            fx = (sinfo & ~KID_LEAF) + part;
+            finfo = part != 0u ? sky : skx;
+            fdi = part != 0u ? dright(sb) : dleft(sb);
+        } else {
+            fx = sx;
+            finfo = sinfo;
+            fdi = sdi;
+        }
+        F = F2;
+    }
+    if (werr != 0u) {
+        if (r == 0u) atomicOr(&misc->err, werr);
         return;
     }
-    // lane r: child r
+    const int cc = F;
+    // lane r: child r (its node and, unless it is a BVH2 leaf, its table entry, from the walk's last round)
     const bool has = (int)r < cc;
-    uint32_t my_ch = ch[0];
-#pragma unroll
-    for (int q = 1; q < 8; ++q)
-        if ((uint32_t)q == r) my_ch = ch[q];
+    const uint32_t my_ch = fx;
     Box cb{0, 0, 0, 0, 0, 0};
     uint32_t cfirst = 0u, cpc = 0u;
     if (has) {
-        const Node32 cn = load_node(&nds[my_ch]);
         cb = node_box(cn);
         cfirst = cn.first_index;
         cpc = cn.prim_count;
     }
-    const uint32_t ckd = (ckind >> (2u * r)) & 3u;
+    const uint32_t ckd = (!has || (finfo & KID_LEAF)) ? KIND_LEAF : dkind(rmeta.x & 0xffu);  // decision-0 kind
 
     // order_children (bvh2_to_cwbvh.rs:402-467): greedy octant assignment, the first strict minimum in
     // child-major / slot-minor order per round.  Lane r finds its own first minimum over the free slots; the
@@ -291,6 +378,8 @@ __device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, cons
                  __shfl(cb.hx, sl, 8), __shfl(cb.hy, sl, 8), __shfl(cb.hz, sl, 8)};
     const uint32_t s_first = (uint32_t)__shfl((int)cfirst, sl, 8), s_pc = (uint32_t)__shfl((int)cpc, sl, 8);
     const uint32_t s_kind = (uint32_t)__shfl((int)ckd, sl, 8), s_ch = (uint32_t)__shfl((int)my_ch, sl, 8);
+    const uint4 s_pl = make_uint4((uint32_t)__shfl((int)rpl.x, sl, 8), (uint32_t)__shfl((int)rpl.y, sl, 8),
+                                  (uint32_t)__shfl((int)rpl.z, sl, 8), (uint32_t)__shfl((int)rpl.w, sl, 8));
 
     // Leaves append primitives in slot order (bvh2_to_cwbvh.rs:117-170; count_primitives 197-211): a BVH2 leaf
     // its first primitive with prim_count, a subtree of at most 3 leaves its leaves left first, as recorded in
@@ -299,7 +388,7 @@ __device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, cons
     uint4 pl = make_uint4(0u, 0u, 0u, 0u);
     if (occ) {
         if (s_kind == KIND_LEAF) {
-            pl = s_pc != 0u ? make_uint4(s_first, 0u, 0u, 1u) : d.rec[dp_key(d, s_first)].plist;
+            pl = s_pc != 0u ? make_uint4(s_first, 0u, 0u, 1u) : s_pl;
             lpc = s_pc != 0u ? s_pc : pl.w;
             if (lpc < 1u || lpc > 3u) bad |= DERR_CW_PRIMCOUNT;
         } else if (s_kind == KIND_INTERNAL) {
@@ -362,13 +451,17 @@ __device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, cons
     // Children's DFS numbering (pre-order, child blocks appended at visit time, bvh2_to_cwbvh.rs:172-191); the
     // caller places the items.
     if (num_internal == 0u) return;
-    const uint32_t csz = isint ? d.rec[dp_key(d, s_first)].cwsize - 1u : 0u;
-    const uint32_t cnp = isint ? d.rec[dp_key(d, s_first)].nprims : 0u;
+    const uint32_t s_cw = (uint32_t)__shfl((int)rcw, sl, 8), s_np = (uint32_t)__shfl((int)rnp, sl, 8);
+    const uint32_t s_meta0 = (uint32_t)__shfl((int)(rmeta.x & 0xffu), sl, 8);
+    const uint32_t s_kx = (uint32_t)__shfl((int)rkids.x, sl, 8), s_ky = (uint32_t)__shfl((int)rkids.y, sl, 8);
+    const uint32_t csz = isint ? s_cw - 1u : 0u;
+    const uint32_t cnp = isint ? s_np : 0u;
     const uint32_t a_ex = g8_inclusive_sum(csz, r) - csz, b_ex = g8_inclusive_sum(cnp, r) - cnp;
     eo.num_internal = num_internal;
     eo.isint = isint;
     eo.rank = int_rank;
     eo.item = CwItem{child_base + int_rank, s_ch, it.a + num_internal + a_ex, it.b + num_primitives + b_ex, it.mesh};
+    eo.pre = CwPre{sb.lx, sb.ly, sb.lz, sb.hx, sb.hy, sb.hz, s_first, s_pc, s_meta0, s_kx, s_ky, 0u};
 }
 
 // One CWBVH level.  Each iteration of the block-uniform loop gives every 8-lane group one node; the groups' next-level
@@ -378,6 +471,7 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
                                                   const uint32_t* __restrict__ bvh2_prims,
                                                   DpView dp,
                                                   const CwItem* __restrict__ in_items, CwItem* __restrict__ out_items,
+                                                  const CwPre* __restrict__ in_pre, CwPre* __restrict__ out_pre,
                                                   uint32_t level, uint32_t order_children, CwNode* __restrict__ out,
                                                   uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
                                                   CwMisc* misc, const MeshDesc* __restrict__ meshes) {
@@ -392,8 +486,8 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
         eo.isint = 0u;
         eo.rank = 0u;
         if (t < count)
-            emit_node(nodes, bvh2_prims, dp, in_items[t], r, level, order_children, out, out_prims, cw_cap,
-                      prim_cap, misc, meshes, eo);
+            emit_node(nodes, bvh2_prims, dp, in_items[t], in_pre[t], r, level, order_children, out, out_prims,
+                      cw_cap, prim_cap, misc, meshes, eo);
         if (threadIdx.x == 0) s_count = 0u;
         __syncthreads();
         uint32_t off = 0u;
@@ -402,7 +496,10 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
         if (threadIdx.x == 0 && s_count != 0u) s_base = atomicAdd(&misc->level_count[level + 1], s_count);
         __syncthreads();
         off = (uint32_t)__shfl((int)off, 0, 8);
-        if (eo.isint) out_items[s_base + off + eo.rank] = eo.item;
+        if (eo.isint) {
+            out_items[s_base + off + eo.rank] = eo.item;
+            out_pre[s_base + off + eo.rank] = eo.pre;
+        }
         __syncthreads();  // s_count and s_base are reused by the next iteration
     }
 }
@@ -700,7 +797,9 @@ __global__ void __launch_bounds__(T256) k_b_cw_sizes(const Node32* __restrict__ 
 }
 
 __global__ void __launch_bounds__(T256) k_b_cw_roots(const MeshDesc* __restrict__ meshes, uint32_t m,
-                                                     CwItem* __restrict__ items, CwMisc* misc) {
+                                                     CwItem* __restrict__ items, CwMisc* misc,
+                                                     const Node32* __restrict__ nodes, DpView dp,
+                                                     CwPre* __restrict__ pre) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     if (i == 0) {
         misc->err = 0u;
@@ -710,6 +809,12 @@ __global__ void __launch_bounds__(T256) k_b_cw_roots(const MeshDesc* __restrict_
     if (meshes[i].prim_count == 0u) return;
     const uint32_t slot = atomicAdd(&misc->level_count[0], 1u);
     items[slot] = CwItem{0u, 0u, 0u, 0u, i};
+    if (pre) {  // the 8-lane form's root data
+        const MeshDesc md = meshes[i];
+        DpView d = dp;
+        d.kbase = md.prim_base;
+        pre[slot] = pre_of(nodes + md.node_base, d, 0u);
+    }
 }
 
 __global__ void k_zero_level0(CwMisc* misc) {
@@ -796,20 +901,10 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
     memset(&out->total_aabb, 0, sizeof(out->total_aabb));
     if (N == 0) return OBVHS_OK;  // bvh2_to_cwbvh.rs:496-498
     const uint32_t n = bvh2->prim_count;
-    size_t off = 0;
-    auto carve = [&](size_t bytes) {
-        size_t o = off;
-        off += (bytes + 255) & ~size_t(255);
-        return o;
-    };
-    const size_t o_items0 = carve((size_t)N * sizeof(CwItem)), o_items1 = carve((size_t)N * sizeof(CwItem)),
-                 o_misc = carve(sizeof(CwMisc));
-    OBVHS_TRY(ctx->scratch.cw_scratch.ensure(off), "alloc cwbvh scratch");
-    char* base = ctx->scratch.cw_scratch.as<char>();
-    CwItem* items[2] = {reinterpret_cast<CwItem*>(base + o_items0), reinterpret_cast<CwItem*>(base + o_items1)};
-    CwMisc* misc = reinterpret_cast<CwMisc*>(base + o_misc);
+    OBVHS_TRY(ctx->scratch.cw_misc.ensure(sizeof(CwMisc)), "alloc cwbvh misc");
+    CwMisc* misc = ctx->scratch.cw_misc.as<CwMisc>();
     const Node32* nodes = bvh2->nodes.as<Node32>();
-    k_cw_init<<<1, 64, 0, st>>>(items[0], misc);
+    k_cw_init<<<1, 64, 0, st>>>(misc);
     uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
     if (bvh2->dp_max_prims != max_prims_per_leaf && n > 1 && !bvh2->pass_ordered) {
         // children moved by reinsertion: evaluate the table over the level lists, deepest level first
@@ -853,15 +948,31 @@ obvhs_status cwbvh_convert(obvhs_context* ctx, obvhs_bvh2* bvh2, uint32_t max_pr
     if (levels > MAX_CW_LEVELS) return OBVHS_ERR_DEPTH;
     OBVHS_TRY(out->nodes.ensure((size_t)ncw * sizeof(CwNode)), "alloc cwbvh nodes");
     OBVHS_TRY(out->prim_indices.ensure((size_t)n * 4 + 4), "alloc cwbvh primitive indices");
+    // Item lists (and their node data) for two levels; a level holds at most ncw nodes.  Level 0 reads the root's
+    // from the misc block.
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t o_items0 = carve((size_t)ncw * sizeof(CwItem)), o_items1 = carve((size_t)ncw * sizeof(CwItem)),
+                 o_pre0 = carve((size_t)ncw * sizeof(CwPre)), o_pre1 = carve((size_t)ncw * sizeof(CwPre));
+    OBVHS_TRY(ctx->scratch.cw_scratch.ensure(off), "alloc cwbvh scratch");
+    char* base = ctx->scratch.cw_scratch.as<char>();
+    CwItem* items[2] = {reinterpret_cast<CwItem*>(base + o_items0), reinterpret_cast<CwItem*>(base + o_items1)};
+    CwPre* pres[2] = {reinterpret_cast<CwPre*>(base + o_pre0), reinterpret_cast<CwPre*>(base + o_pre1)};
     // One launch per level, no host round trips: a level holds at most min(8^level, ncw) nodes.
     uint64_t bound = 1;
     for (uint32_t level = 0; level < levels; ++level) {
         const uint64_t cap = bound < ncw ? bound : ncw;
         const uint32_t g = (uint32_t)std::min<uint64_t>((cap * 8 + T256 - 1) / T256, EMIT_MAX_BLOCKS);
-        k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2->prim_indices.as<uint32_t>(), dp, items[level & 1],
-                                      items[(level + 1) & 1], level, order_children ? 1u : 0u,
-                                      out->nodes.as<CwNode>(), out->prim_indices.as<uint32_t>(), ncw, n, misc,
-                                      nullptr);
+        const CwItem* in_items = level == 0 ? &misc->root_item : items[level & 1];
+        const CwPre* in_pre = level == 0 ? &misc->root_pre : pres[level & 1];
+        k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2->prim_indices.as<uint32_t>(), dp, in_items,
+                                      items[(level + 1) & 1], in_pre, pres[(level + 1) & 1], level,
+                                      order_children ? 1u : 0u, out->nodes.as<CwNode>(),
+                                      out->prim_indices.as<uint32_t>(), ncw, n, misc, nullptr);
         bound *= 8;
     }
     OBVHS_TRY(hipGetLastError(), "cwbvh emit launch");
@@ -914,29 +1025,35 @@ obvhs_status cwbvh_convert_batch(obvhs_context* ctx, const Node32* nodes, const 
         off += (bytes + 255) & ~size_t(255);
         return o;
     };
-    const size_t cap = (size_t)2 * n_total + m;
+    // a level holds at most `total` nodes; the 8-lane form also keeps each item's node data
+    const bool one_lane = active >= EMIT_ONE_LANE_ROOTS;
+    const size_t cap = (size_t)total;
     const size_t o_items0 = carve(cap * sizeof(CwItem)), o_items1 = carve(cap * sizeof(CwItem)),
-                 o_misc = carve(sizeof(CwMisc));
+                 o_pre0 = one_lane ? 0 : carve(cap * sizeof(CwPre)), o_pre1 = one_lane ? 0 : carve(cap * sizeof(CwPre));
     OBVHS_TRY(ctx->scratch.cw_scratch.ensure(off), "alloc cwbvh scratch");
+    OBVHS_TRY(ctx->scratch.cw_misc.ensure(sizeof(CwMisc)), "alloc cwbvh misc");
     char* base = ctx->scratch.cw_scratch.as<char>();
     CwItem* items[2] = {reinterpret_cast<CwItem*>(base + o_items0), reinterpret_cast<CwItem*>(base + o_items1)};
-    CwMisc* misc = reinterpret_cast<CwMisc*>(base + o_misc);
+    CwPre* pres[2] = {one_lane ? nullptr : reinterpret_cast<CwPre*>(base + o_pre0),
+                      one_lane ? nullptr : reinterpret_cast<CwPre*>(base + o_pre1)};
+    CwMisc* misc = ctx->scratch.cw_misc.as<CwMisc>();
     k_zero_level0<<<1, 64, 0, st>>>(misc);
-    k_b_cw_roots<<<(m + T256 - 1) / T256, T256, 0, st>>>(d_meshes, m, items[0], misc);
+    k_b_cw_roots<<<(m + T256 - 1) / T256, T256, 0, st>>>(d_meshes, m, items[0], misc, nodes, dp, pres[0]);
     uint64_t bound = active;
     for (uint32_t level = 0; level < levels; ++level) {
         const uint64_t capl = bound < total ? bound : total;
         if (capl == 0) continue;
-        if (active >= EMIT_ONE_LANE_ROOTS) {  // every level at least `active` nodes wide: one lane per node
+        if (one_lane) {  // every level at least `active` nodes wide: one lane per node
             const uint32_t g = (uint32_t)std::min<uint64_t>((capl + T256 - 1) / T256, EMIT1_MAX_BLOCKS);
             k_cw_emit1<<<g, T256, 0, st>>>(nodes, bvh2_prims, dp, items[level & 1], items[(level + 1) & 1], level,
                                            order_children ? 1u : 0u, out_nodes.as<CwNode>(),
                                            out_prims.as<uint32_t>(), 0u, 0u, misc, d_meshes);
         } else {
             const uint32_t g = (uint32_t)std::min<uint64_t>((capl * 8 + T256 - 1) / T256, EMIT_MAX_BLOCKS);
-            k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2_prims, dp, items[level & 1], items[(level + 1) & 1], level,
-                                          order_children ? 1u : 0u, out_nodes.as<CwNode>(),
-                                          out_prims.as<uint32_t>(), 0u, 0u, misc, d_meshes);
+            k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2_prims, dp, items[level & 1], items[(level + 1) & 1],
+                                          pres[level & 1], pres[(level + 1) & 1], level, order_children ? 1u : 0u,
+                                          out_nodes.as<CwNode>(), out_prims.as<uint32_t>(), 0u, 0u, misc,
+                                          d_meshes);
         }
         bound *= 8;
     }

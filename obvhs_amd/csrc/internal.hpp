@@ -47,7 +47,8 @@ struct BuildScratch {
     DevBuf blocksums;   // uint2[n/256+1]
     DevBuf sort_scratch;
     DevBuf misc;        // bounds keys, states, error word
-    DevBuf cw_scratch;  // conversion working memory
+    DevBuf cw_scratch;  // conversion working memory: item lists
+    DevBuf cw_misc;     // conversion counters, error word and root item (kept across the item lists' growth)
     DevBuf trav_spill;  // traversal stack spill columns
 };
 
