@@ -187,10 +187,6 @@ __device__ __forceinline__ uint32_t quantise_hi(float b, float p, float rcp) {
 }
 
 constexpr uint32_t EMIT_MAX_BLOCKS = 2048;   // 8 waves per SIMD on 256 CUs; larger levels loop
-constexpr uint32_t EMIT1_MAX_BLOCKS = 8192;  // one-lane form: 2M nodes per sweep
-// Batched conversions with at least this many roots use the one-lane form: every level is that wide, so the
-// 8-lane form's shorter per-node latency no longer pays for its 8x instruction count per node.
-constexpr uint32_t EMIT_ONE_LANE_ROOTS = 512;
 
 // What one node hands to the next level: its internal child count (0 on error), and per slot lane whether the
 // slot holds an internal child, its rank among them and its work item.
@@ -504,263 +500,6 @@ __global__ void __launch_bounds__(T256) k_cw_emit(const Node32* __restrict__ nod
     }
 }
 
-// One CWBVH node by one lane (emit_node1) -- the form for levels many thousands of nodes wide (batched conversions),
-// where one lane per node issues the fewest instructions per node.  Outputs what the caller needs to place the
-// node's internal children: their count, slots, BVH2 slots and first_index words, and the item's running bases.
-__device__ __forceinline__ void emit_node1(const Node32* __restrict__ nodes, const uint32_t* __restrict__ bvh2_prims,
-                                           DpView& dp, const CwItem* __restrict__ in_items, uint32_t t,
-                                           uint32_t level, uint32_t order_children, CwNode* __restrict__ out,
-                                           uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
-                                           CwMisc* misc, const MeshDesc* __restrict__ meshes, CwItem& it,
-                                           uint32_t* ch, uint32_t* cfirst, int* slot_child, uint32_t* internal_slots,
-                                           uint32_t& num_internal, uint32_t& num_primitives, uint32_t& child_base) {
-    it = in_items[t];
-    if (meshes) {  // batched conversions: every array is the item's mesh's slice
-        const MeshDesc md = meshes[it.mesh];
-        nodes += md.node_base;
-        bvh2_prims += md.prim_base;
-        out += md.cw_base;
-        out_prims += md.prim_base;
-        cw_cap = md.cw_count;
-        prim_cap = md.prim_count;
-        dp.kbase = md.prim_base;
-    }
-    const Node32 nd = load_node(&nodes[it.bvh2]);
-    const Box nb = node_box(nd);
-
-    uint32_t ckind = 0u;  // 2 bits per child: its decision-0 kind
-    int cc;
-    if (nd.prim_count != 0u) {  // a single-leaf BVH2: the root is its own only child
-        ch[0] = it.bvh2;
-        cc = 1;
-    } else {
-        cc = get_children_kids(dp, nd.first_index, node_decision(dp, nd.first_index, 0), ch, &ckind);
-    }
-    if (cc < 0) {
-        atomicOr(&misc->err, cc == -2 ? DERR_CW_DISTRIBUTE : DERR_CAPACITY);
-        return;
-    }
-    Box cb[8];
-    uint32_t cleafpc[8];   // prim_count of a child that is a BVH2 leaf, 0 for an internal child
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        cb[c] = Box{0, 0, 0, 0, 0, 0};
-        cfirst[c] = 0u;
-        cleafpc[c] = 0u;
-        if (c < cc) {
-            const Node32 cn = load_node(&nodes[ch[c]]);
-            cb[c] = node_box(cn);
-            cfirst[c] = cn.first_index;
-            cleafpc[c] = cn.prim_count;
-        }
-    }
-    // order_children (bvh2_to_cwbvh.rs:402-467): greedy octant assignment, child-major / slot-minor, strict <.
-    int slot_of[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) slot_of[c] = c;
-    if (order_children) {
-        const float px = (nb.hx + nb.lx) * 0.5f, py = (nb.hy + nb.ly) * 0.5f, pz = (nb.hz + nb.lz) * 0.5f;
-        const float FMAX = 3.40282347e+38f;
-        // Child centre offsets; the 8x8 cost table is evaluated on the fly (d . v with d in {+-1}^3 is exact).
-        float vx[8], vy[8], vz[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            vx[c] = (cb[c].hx + cb[c].lx) * 0.5f - px;
-            vy[c] = (cb[c].hy + cb[c].ly) * 0.5f - py;
-            vz[c] = (cb[c].hz + cb[c].lz) * 0.5f - pz;
-        }
-        uint32_t assigned = 0u, filled = 0u;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) slot_of[c] = -1;
-        for (int round = 0; round < 8; ++round) {
-            float min_cost = FMAX;
-            int min_slot = -1, min_index = -1;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                if (c < cc && !((assigned >> c) & 1u)) {
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) {
-                        const float dx = (s & 4) ? -1.0f : 1.0f, dy = (s & 2) ? -1.0f : 1.0f,
-                                    dz = (s & 1) ? -1.0f : 1.0f;
-                        const float v = (dx * vx[c] + dy * vy[c]) + dz * vz[c];
-                        if (!((filled >> s) & 1u) && v < min_cost) {
-                            min_cost = v;
-                            min_slot = s;
-                            min_index = c;
-                        }
-                    }
-                }
-            }
-            if (min_slot < 0) break;
-            filled |= 1u << min_slot;
-            assigned |= 1u << min_index;
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-                if (c == min_index) slot_of[c] = min_slot;
-        }
-        for (int c = 0; c < cc; ++c) {
-            if (slot_of[c] < 0) {
-                atomicOr(&misc->err, DERR_CW_ORDER);
-                return;
-            }
-        }
-    }
-    // slots[s] = child index placed in slot s (or -1)
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        slot_child[s] = -1;
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (c < cc && slot_of[c] == s) slot_child[s] = c;
-    }
-
-    CwNode o;
-    o.p[0] = nb.lx;
-    o.p[1] = nb.ly;
-    o.p[2] = nb.lz;
-    const uint32_t ex = cw_exponent(nb.hx - nb.lx), ey = cw_exponent(nb.hy - nb.ly), ez = cw_exponent(nb.hz - nb.lz);
-    o.e[0] = (uint8_t)ex;
-    o.e[1] = (uint8_t)ey;
-    o.e[2] = (uint8_t)ez;
-    const float rx = 1.0f / __uint_as_float(ex << 23), ry = 1.0f / __uint_as_float(ey << 23),
-                rz = 1.0f / __uint_as_float(ez << 23);
-    uint32_t imask = 0u;
-    num_internal = 0u;
-    num_primitives = 0u;
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        o.child_meta[s] = 0;
-        o.child_min_x[s] = o.child_max_x[s] = 0;
-        o.child_min_y[s] = o.child_max_y[s] = 0;
-        o.child_min_z[s] = o.child_max_z[s] = 0;
-    }
-    child_base = 1u + it.a;
-    // First sweep: leaves append primitives in slot order before any recursion (bvh2_to_cwbvh.rs:117-170).
-    for (int s = 0; s < 8; ++s) {
-        const int c = slot_child[s];
-        if (c < 0) continue;
-        Box b = cb[0];
-#pragma unroll
-        for (int q = 1; q < 8; ++q)
-            if (q == c) b = cb[q];
-        uint32_t cf = cfirst[0], cpc = cleafpc[0];
-#pragma unroll
-        for (int q = 1; q < 8; ++q)
-            if (q == c) {
-                cf = cfirst[q];
-                cpc = cleafpc[q];
-            }
-        float lo, hi;
-        lo = floorf(((b.lx - o.p[0]) - 0.0f) * rx);
-        hi = ceilf(((b.hx - o.p[0]) + 0.0f) * rx);
-        o.child_min_x[s] = (uint8_t)smin(smax(lo, 0.0f), 255.0f);
-        o.child_max_x[s] = (uint8_t)smin(smax(hi, 0.0f), 255.0f);
-        lo = floorf(((b.ly - o.p[1]) - 0.0f) * ry);
-        hi = ceilf(((b.hy - o.p[1]) + 0.0f) * ry);
-        o.child_min_y[s] = (uint8_t)smin(smax(lo, 0.0f), 255.0f);
-        o.child_max_y[s] = (uint8_t)smin(smax(hi, 0.0f), 255.0f);
-        lo = floorf(((b.lz - o.p[2]) - 0.0f) * rz);
-        hi = ceilf(((b.hz - o.p[2]) + 0.0f) * rz);
-        o.child_min_z[s] = (uint8_t)smin(smax(lo, 0.0f), 255.0f);
-        o.child_max_z[s] = (uint8_t)smin(smax(hi, 0.0f), 255.0f);
-        const uint32_t kind = (ckind >> (2 * c)) & 3u;
-        if (kind == KIND_LEAF) {
-            // count_primitives (bvh2_to_cwbvh.rs:197-211): a BVH2 leaf appends its first primitive and counts
-            // prim_count; a subtree of at most 3 leaves appends them left first, as recorded in the cost table.
-            const uint4 pl = cpc != 0u ? make_uint4(cf, 0u, 0u, 1u) : dp.rec[dp_key(dp, cf)].plist;
-            const uint32_t pc = cpc != 0u ? cpc : pl.w;
-            const uint32_t dst = it.b + num_primitives;
-            if (pl.w >= 1u && dst < prim_cap) out_prims[dst] = bvh2_prims[pl.x];
-            if (pl.w >= 2u && dst + 1u < prim_cap) out_prims[dst + 1u] = bvh2_prims[pl.y];
-            if (pl.w >= 3u && dst + 2u < prim_cap) out_prims[dst + 2u] = bvh2_prims[pl.z];
-            uint32_t unary;
-            if (pc == 1u) unary = 0x20u;
-            else if (pc == 2u) unary = 0x60u;
-            else if (pc == 3u) unary = 0xe0u;
-            else {
-                atomicOr(&misc->err, DERR_CW_PRIMCOUNT);
-                num_internal = 0u;
-                return;
-            }
-            o.child_meta[s] = (uint8_t)(num_primitives | unary);
-            num_primitives += pc;
-        } else if (kind == KIND_INTERNAL) {
-            imask |= 1u << s;
-            o.child_meta[s] = (uint8_t)((24u + (uint32_t)s) | 0x20u);
-            internal_slots[num_internal++] = (uint32_t)s;
-        } else {
-            atomicOr(&misc->err, DERR_CW_DISTRIBUTE);
-            num_internal = 0u;
-            return;
-        }
-    }
-    o.imask = (uint8_t)imask;
-    o.child_base_idx = child_base;
-    o.primitive_base_idx = it.b;
-    if (it.cw_index < cw_cap) {
-        const uint4* src = reinterpret_cast<const uint4*>(&o);
-        uint4* dst = reinterpret_cast<uint4*>(&out[it.cw_index]);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) dst[q] = src[q];
-    } else {
-        atomicOr(&misc->err, DERR_CAPACITY);
-    }
-    if (num_internal != 0u && level + 1 >= MAX_CW_LEVELS) {
-        atomicOr(&misc->err, DERR_CW_DEPTH);
-        num_internal = 0u;
-    }
-}
-
-__global__ void __launch_bounds__(T256) k_cw_emit1(const Node32* __restrict__ nodes,
-                                                   const uint32_t* __restrict__ bvh2_prims,
-                                                   DpView dp,
-                                                   const CwItem* __restrict__ in_items, CwItem* __restrict__ out_items,
-                                                   uint32_t level, uint32_t order_children, CwNode* __restrict__ out,
-                                                   uint32_t* __restrict__ out_prims, uint32_t cw_cap, uint32_t prim_cap,
-                                                   CwMisc* misc, const MeshDesc* __restrict__ meshes) {
-    __shared__ uint32_t s_count, s_base;
-    const uint32_t count = misc->level_count[level];
-    for (uint32_t first = blockIdx.x * T256; first < count; first += gridDim.x * T256) {  // block-uniform
-        const uint32_t t = first + threadIdx.x;
-        CwItem it{};
-        DpView d = dp;
-        uint32_t ch[8], cfirst[8], internal_slots[8];
-        int slot_child[8];
-        uint32_t num_internal = 0u, num_primitives = 0u, child_base = 0u;
-        if (t < count)
-            emit_node1(nodes, bvh2_prims, d, in_items, t, level, order_children, out, out_prims, cw_cap, prim_cap,
-                       misc, meshes, it, ch, cfirst, slot_child, internal_slots, num_internal, num_primitives,
-                       child_base);
-        // Place the next level's items: counted in LDS, one global atomic per block.
-        if (threadIdx.x == 0) s_count = 0u;
-        __syncthreads();
-        uint32_t base = 0u;
-        if (num_internal != 0u) base = atomicAdd(&s_count, num_internal);
-        __syncthreads();
-        if (threadIdx.x == 0 && s_count != 0u) s_base = atomicAdd(&misc->level_count[level + 1], s_count);
-        __syncthreads();
-        base += s_base;
-        // Children's DFS numbering (pre-order, child blocks appended at visit time, bvh2_to_cwbvh.rs:172-191).
-        uint32_t a_run = it.a + num_internal;
-        uint32_t b_run = it.b + num_primitives;
-        for (uint32_t j = 0; j < num_internal; ++j) {
-            const uint32_t s = internal_slots[j];
-            const int c = slot_child[s];
-            uint32_t ci = ch[0], cf = cfirst[0];
-#pragma unroll
-            for (int q = 1; q < 8; ++q)
-                if (q == c) {
-                    ci = ch[q];
-                    cf = cfirst[q];
-                }
-            out_items[base + j] = CwItem{child_base + j, ci, a_run, b_run, it.mesh};
-            a_run += d.rec[dp_key(d, cf)].cwsize - 1u;
-            b_run += d.rec[dp_key(d, cf)].nprims;
-        }
-        __syncthreads();  // s_count and s_base are reused by the next iteration
-    }
-}
-
 __global__ void __launch_bounds__(T256) k_reorder_tris(const float4* __restrict__ tris,
                                                        const uint32_t* __restrict__ prims, uint32_t n,
                                                        float4* __restrict__ out) {
@@ -809,12 +548,10 @@ __global__ void __launch_bounds__(T256) k_b_cw_roots(const MeshDesc* __restrict_
     if (meshes[i].prim_count == 0u) return;
     const uint32_t slot = atomicAdd(&misc->level_count[0], 1u);
     items[slot] = CwItem{0u, 0u, 0u, 0u, i};
-    if (pre) {  // the 8-lane form's root data
-        const MeshDesc md = meshes[i];
-        DpView d = dp;
-        d.kbase = md.prim_base;
-        pre[slot] = pre_of(nodes + md.node_base, d, 0u);
-    }
+    const MeshDesc md = meshes[i];
+    DpView d = dp;
+    d.kbase = md.prim_base;
+    pre[slot] = pre_of(nodes + md.node_base, d, 0u);
 }
 
 __global__ void k_zero_level0(CwMisc* misc) {
@@ -1018,24 +755,22 @@ obvhs_status cwbvh_convert_batch(obvhs_context* ctx, const Node32* nodes, const 
               "write mesh table");
     OBVHS_TRY(out_nodes.ensure((size_t)total * sizeof(CwNode) + 80), "alloc batch cwbvh nodes");
     OBVHS_TRY(out_prims.ensure((size_t)n_total * 4 + 4), "alloc batch cwbvh primitive indices");
-    // Level-0 items: every non-empty mesh's root; items hold at most one entry per BVH2 node of all meshes.
+    // Level-0 items: every non-empty mesh's root.
     size_t off = 0;
     auto carve = [&](size_t bytes) {
         size_t o = off;
         off += (bytes + 255) & ~size_t(255);
         return o;
     };
-    // a level holds at most `total` nodes; the 8-lane form also keeps each item's node data
-    const bool one_lane = active >= EMIT_ONE_LANE_ROOTS;
+    // a level holds at most `total` nodes
     const size_t cap = (size_t)total;
     const size_t o_items0 = carve(cap * sizeof(CwItem)), o_items1 = carve(cap * sizeof(CwItem)),
-                 o_pre0 = one_lane ? 0 : carve(cap * sizeof(CwPre)), o_pre1 = one_lane ? 0 : carve(cap * sizeof(CwPre));
+                 o_pre0 = carve(cap * sizeof(CwPre)), o_pre1 = carve(cap * sizeof(CwPre));
     OBVHS_TRY(ctx->scratch.cw_scratch.ensure(off), "alloc cwbvh scratch");
     OBVHS_TRY(ctx->scratch.cw_misc.ensure(sizeof(CwMisc)), "alloc cwbvh misc");
     char* base = ctx->scratch.cw_scratch.as<char>();
     CwItem* items[2] = {reinterpret_cast<CwItem*>(base + o_items0), reinterpret_cast<CwItem*>(base + o_items1)};
-    CwPre* pres[2] = {one_lane ? nullptr : reinterpret_cast<CwPre*>(base + o_pre0),
-                      one_lane ? nullptr : reinterpret_cast<CwPre*>(base + o_pre1)};
+    CwPre* pres[2] = {reinterpret_cast<CwPre*>(base + o_pre0), reinterpret_cast<CwPre*>(base + o_pre1)};
     CwMisc* misc = ctx->scratch.cw_misc.as<CwMisc>();
     k_zero_level0<<<1, 64, 0, st>>>(misc);
     k_b_cw_roots<<<(m + T256 - 1) / T256, T256, 0, st>>>(d_meshes, m, items[0], misc, nodes, dp, pres[0]);
@@ -1043,18 +778,11 @@ obvhs_status cwbvh_convert_batch(obvhs_context* ctx, const Node32* nodes, const 
     for (uint32_t level = 0; level < levels; ++level) {
         const uint64_t capl = bound < total ? bound : total;
         if (capl == 0) continue;
-        if (one_lane) {  // every level at least `active` nodes wide: one lane per node
-            const uint32_t g = (uint32_t)std::min<uint64_t>((capl + T256 - 1) / T256, EMIT1_MAX_BLOCKS);
-            k_cw_emit1<<<g, T256, 0, st>>>(nodes, bvh2_prims, dp, items[level & 1], items[(level + 1) & 1], level,
-                                           order_children ? 1u : 0u, out_nodes.as<CwNode>(),
-                                           out_prims.as<uint32_t>(), 0u, 0u, misc, d_meshes);
-        } else {
-            const uint32_t g = (uint32_t)std::min<uint64_t>((capl * 8 + T256 - 1) / T256, EMIT_MAX_BLOCKS);
-            k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2_prims, dp, items[level & 1], items[(level + 1) & 1],
-                                          pres[level & 1], pres[(level + 1) & 1], level, order_children ? 1u : 0u,
-                                          out_nodes.as<CwNode>(), out_prims.as<uint32_t>(), 0u, 0u, misc,
-                                          d_meshes);
-        }
+        const uint32_t g = (uint32_t)std::min<uint64_t>((capl * 8 + T256 - 1) / T256, EMIT_MAX_BLOCKS);
+        k_cw_emit<<<g, T256, 0, st>>>(nodes, bvh2_prims, dp, items[level & 1], items[(level + 1) & 1],
+                                      pres[level & 1], pres[(level + 1) & 1], level, order_children ? 1u : 0u,
+                                      out_nodes.as<CwNode>(), out_prims.as<uint32_t>(), 0u, 0u, misc,
+                                      d_meshes);
         bound *= 8;
     }
     OBVHS_TRY(hipGetLastError(), "batch cwbvh emit launch");
