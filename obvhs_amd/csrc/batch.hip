@@ -383,6 +383,27 @@ __global__ void __launch_bounds__(1024) k_b_scan(uint2* __restrict__ blocksums, 
     }
 }
 
+// Two-level form of k_b_scan for passes of more than B_SCAN_ONE_BLOCK tiles (the first passes hold ~78K): chunk
+// totals by many workgroups, then each chunk offset and scanned by its own; workgroup 0 writes what k_b_scan writes.
+constexpr uint32_t B_SCAN_ONE_BLOCK = 16384;
+
+__global__ void __launch_bounds__(1024) k_b_scan_chunks(const uint2* __restrict__ blocksums, uint32_t nblocks,
+                                                        uint2* __restrict__ chunk_tot) {
+    scan_chunk_total(blocksums, nblocks, chunk_tot);
+}
+
+__global__ void __launch_bounds__(1024) k_b_scan_apply(uint2* __restrict__ blocksums, uint32_t nblocks,
+                                                       const uint2* __restrict__ chunk_tot, uint32_t nchunks,
+                                                       BatchMisc* misc, uint32_t q) {
+    const uint2 all = scan_chunk_apply(blocksums, nblocks, chunk_tot, nchunks);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        misc->totals[0] = all.x;
+        misc->totals[1] = all.y;
+        misc->total[q ^ 1u] = all.x;
+        misc->active = 0u;
+    }
+}
+
 // Exclusive (emit, merge) prefixes at the segment's first cluster i0 and one past its last, i1: the scanned offset of
 // the index's block plus the block's clusters before it, counted by the whole wave 64 clusters per step (one ballot
 // each), as b_classify decides.  The up to 4 steps of both prefixes are independent, so all their loads go out in
@@ -595,7 +616,9 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
     OBVHS_TRY(S.vals.ensure((size_t)n * 8), "alloc vals + cluster meshes");
     const size_t cm_off = ((size_t)n + 64 + 255) & ~size_t(255);
     OBVHS_TRY(S.merge.ensure(cm_off + (size_t)n * 4), "alloc merge + cluster meshes");
-    OBVHS_TRY(S.blocksums.ensure((size_t)nblk * sizeof(uint2) + 64), "alloc blocksums");
+    // per-tile counts, then the two-level scan's chunk totals
+    OBVHS_TRY(S.blocksums.ensure((size_t)(nblk + 16 + nblk / SCAN_CHUNK + 1) * sizeof(uint2) + 64),
+              "alloc blocksums");
     OBVHS_TRY(S.sort_scratch.ensure(radix_sort_scratch_bytes(n)), "alloc sort scratch");
     const size_t seg_bytes = (size_t)m * sizeof(SegState);
     OBVHS_TRY(S.misc.ensure(2 * seg_bytes + (size_t)m * 24 + sizeof(BatchMisc) + 1024), "alloc batch misc");
@@ -650,7 +673,14 @@ static obvhs_status batch_build(obvhs_context* ctx, const obvhs_triangle* d_tris
             k_b_search<<<blocks, T256, 0, st>>>(cur, cmesh[cm], seg[cs], misc, d_meshes, rmax,
                                                 pass < sdt ? 1u : 0u, merge, q);
             k_b_count<<<blocks, T256, 0, st>>>(merge, cmesh[cm], seg[cs], misc, blocksums, q);
-            k_b_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, q);
+            if (blocks <= B_SCAN_ONE_BLOCK) {
+                k_b_scan<<<1, 1024, 0, st>>>(blocksums, blocks, misc, q);
+            } else {
+                uint2* chunk_tot = blocksums + blocks + 16;
+                const uint32_t nchunks = (blocks + SCAN_CHUNK - 1) / SCAN_CHUNK;
+                k_b_scan_chunks<<<nchunks, 1024, 0, st>>>(blocksums, blocks, chunk_tot);
+                k_b_scan_apply<<<nchunks, 1024, 0, st>>>(blocksums, blocks, chunk_tot, nchunks, misc, q);
+            }
             k_b_segments<<<(m + 3u) / 4u, T256, 0, st>>>(merge, cmesh[cm], seg[cs], seg[cs ^ 1], blocksums, m,
                                                           misc, q);
             k_b_compact<<<blocks, T256, 0, st>>>(cur, nxt, cmesh[cm], cmesh[cm ^ 1], merge, seg[cs], misc, blocksums,

@@ -79,4 +79,94 @@ __device__ __forceinline__ uint64_t morton_of_box(const Box& b, double sx, doubl
            split_by_3_u64(f64_as_u32(pz * s)) << 2;
 }
 
+// Exclusive scans over a 1024-thread workgroup of two u32 values at once; ta / tb receive the totals.  se / sm: 16
+// words of LDS each (reused: the function ends with a barrier).
+__device__ __forceinline__ void block_exscan2_1024(uint32_t a, uint32_t b, uint32_t* se, uint32_t* sm, uint32_t& ea,
+                                                   uint32_t& eb, uint32_t& ta, uint32_t& tb) {
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t ia = wave_inclusive_scan(a), ib = wave_inclusive_scan(b);
+    if (lane == 63u) {
+        se[wave] = ia;
+        sm[wave] = ib;
+    }
+    __syncthreads();
+    uint32_t pa = 0u, pb = 0u;
+    ta = 0u;
+    tb = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < 16; ++w) {
+        const uint32_t xa = se[w], xb = sm[w];
+        if (w < wave) {
+            pa += xa;
+            pb += xb;
+        }
+        ta += xa;
+        tb += xb;
+    }
+    ea = pa + ia - a;
+    eb = pb + ib - b;
+    __syncthreads();
+}
+
+// Two-level scan of per-tile (emit, merge) counts, SCAN_CHUNK counts (4 per thread) per 1,024-thread workgroup.
+// scan_chunk_total: the chunk's totals into chunk_tot[blockIdx.x].  scan_chunk_apply: every count becomes its
+// exclusive prefix over all tiles (the chunks before this one from chunk_tot); returns the totals of all chunks.
+constexpr uint32_t SCAN_CHUNK = 4096;
+
+__device__ __forceinline__ void scan_chunk_total(const uint2* __restrict__ counts, uint32_t n,
+                                                 uint2* __restrict__ chunk_tot) {
+    __shared__ uint32_t se[16], sm[16];
+    const uint32_t b0 = blockIdx.x * SCAN_CHUNK + threadIdx.x * 4u;
+    uint32_t E = 0u, M = 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; ++k)
+        if (b0 + k < n) {
+            const uint2 v = counts[b0 + k];
+            E += v.x;
+            M += v.y;
+        }
+    uint32_t ee, em, te, tm;
+    block_exscan2_1024(E, M, se, sm, ee, em, te, tm);
+    if (threadIdx.x == 0) chunk_tot[blockIdx.x] = make_uint2(te, tm);
+}
+
+__device__ __forceinline__ uint2 scan_chunk_apply(uint2* __restrict__ counts, uint32_t n,
+                                                  const uint2* __restrict__ chunk_tot, uint32_t nchunks) {
+    __shared__ uint32_t se[16], sm[16];
+    uint32_t pe = 0u, pm = 0u, ae = 0u, am = 0u;
+    for (uint32_t c = threadIdx.x; c < nchunks; c += 1024u) {
+        const uint2 v = chunk_tot[c];
+        if (c < blockIdx.x) {
+            pe += v.x;
+            pm += v.y;
+        }
+        ae += v.x;
+        am += v.y;
+    }
+    uint32_t x0, x1, before_e, before_m, all_e, all_m;
+    block_exscan2_1024(pe, pm, se, sm, x0, x1, before_e, before_m);
+    block_exscan2_1024(ae, am, se, sm, x0, x1, all_e, all_m);
+    const uint32_t b0 = blockIdx.x * SCAN_CHUNK + threadIdx.x * 4u;
+    uint2 v[4];
+    uint32_t E = 0u, M = 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; ++k) {
+        v[k] = b0 + k < n ? counts[b0 + k] : make_uint2(0u, 0u);
+        E += v[k].x;
+        M += v[k].y;
+    }
+    uint32_t ee, em, te, tm;
+    block_exscan2_1024(E, M, se, sm, ee, em, te, tm);
+    ee += before_e;
+    em += before_m;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; ++k)
+        if (b0 + k < n) {
+            counts[b0 + k] = make_uint2(ee, em);
+            ee += v[k].x;
+            em += v[k].y;
+        }
+    return make_uint2(all_e, all_m);
+}
+
 }  // namespace obvhs

@@ -417,108 +417,32 @@ __global__ void __launch_bounds__(1024) k_ploc_scan(uint2* __restrict__ blocksum
     }
 }
 
-// Exclusive scans over a 1024-thread workgroup of two u32 values at once; ta / tb receive the totals.  se / sm: 16
-// words of LDS each (reused: the function ends with a barrier).
-__device__ __forceinline__ void block_exscan2_1024(uint32_t a, uint32_t b, uint32_t* se, uint32_t* sm, uint32_t& ea,
-                                                   uint32_t& eb, uint32_t& ta, uint32_t& tb) {
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t ia = wave_inclusive_scan(a), ib = wave_inclusive_scan(b);
-    if (lane == 63u) {
-        se[wave] = ia;
-        sm[wave] = ib;
-    }
-    __syncthreads();
-    uint32_t pa = 0u, pb = 0u;
-    ta = 0u;
-    tb = 0u;
-#pragma unroll
-    for (uint32_t w = 0; w < 16; ++w) {
-        const uint32_t xa = se[w], xb = sm[w];
-        if (w < wave) {
-            pa += xa;
-            pb += xb;
-        }
-        ta += xa;
-        tb += xb;
-    }
-    ea = pa + ia - a;
-    eb = pb + ib - b;
-    __syncthreads();
-}
-
 // Two-level scan of the tile counts for passes of more than SCAN_ONE_BLOCK tiles (one workgroup reading every count
 // twice takes ~50 us at 74K tiles): k_ploc_scan_chunks sums SCAN_CHUNK counts per workgroup, k_ploc_scan_apply
 // offsets each chunk by the chunks before it and scans it, and its workgroup 0 writes the next pass's state.
-constexpr uint32_t SCAN_CHUNK = 4096;        // tiles per workgroup, 4 per thread
 constexpr uint32_t SCAN_ONE_BLOCK = 16384;   // passes of at most this many tiles use k_ploc_scan
 
 __global__ void __launch_bounds__(1024) k_ploc_scan_chunks(const uint2* __restrict__ blocksums, uint32_t nblocks,
                                                            uint2* __restrict__ chunk_tot) {
-    __shared__ uint32_t se[16], sm[16];
-    const uint32_t b0 = blockIdx.x * SCAN_CHUNK + threadIdx.x * 4u;
-    uint32_t E = 0u, M = 0u;
-#pragma unroll
-    for (uint32_t k = 0; k < 4u; ++k)
-        if (b0 + k < nblocks) {
-            const uint2 v = blocksums[b0 + k];
-            E += v.x;
-            M += v.y;
-        }
-    uint32_t ee, em, te, tm;
-    block_exscan2_1024(E, M, se, sm, ee, em, te, tm);
-    if (threadIdx.x == 0) chunk_tot[blockIdx.x] = make_uint2(te, tm);
+    scan_chunk_total(blocksums, nblocks, chunk_tot);
 }
 
 __global__ void __launch_bounds__(1024) k_ploc_scan_apply(uint2* __restrict__ blocksums, uint32_t nblocks,
                                                           const uint2* __restrict__ chunk_tot, uint32_t nchunks,
                                                           Misc* misc, uint32_t slot, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t se[16], sm[16];
-    // the chunks before this one, and (workgroup 0) all of them
-    uint32_t pe = 0u, pm = 0u, ae = 0u, am = 0u;
-    for (uint32_t c = threadIdx.x; c < nchunks; c += 1024u) {
-        const uint2 v = chunk_tot[c];
-        if (c < blockIdx.x) {
-            pe += v.x;
-            pm += v.y;
-        }
-        ae += v.x;
-        am += v.y;
-    }
-    uint32_t x0, x1, before_e, before_m, all_e, all_m;
-    block_exscan2_1024(pe, pm, se, sm, x0, x1, before_e, before_m);
-    block_exscan2_1024(ae, am, se, sm, x0, x1, all_e, all_m);
+    const uint2 all = scan_chunk_apply(blocksums, nblocks, chunk_tot, nchunks);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const PlocState st = misc->state[slot];
         PlocState nx = st;
         if (st.count > 1) {
-            nx.count = all_e;
-            nx.insert = st.insert - 2u * all_m;
+            nx.count = all.x;
+            nx.insert = st.insert - 2u * all.y;
             nx.depth = st.depth + 1u;
             nx.cluster_sum = st.cluster_sum + st.count;
             hist[nx.depth] = nx.insert;
         }
         misc->state[slot ^ 1u] = nx;
     }
-    const uint32_t b0 = blockIdx.x * SCAN_CHUNK + threadIdx.x * 4u;
-    uint2 v[4];
-    uint32_t E = 0u, M = 0u;
-#pragma unroll
-    for (uint32_t k = 0; k < 4u; ++k) {
-        v[k] = b0 + k < nblocks ? blocksums[b0 + k] : make_uint2(0u, 0u);
-        E += v[k].x;
-        M += v[k].y;
-    }
-    uint32_t ee, em, te, tm;
-    block_exscan2_1024(E, M, se, sm, ee, em, te, tm);
-    ee += before_e;
-    em += before_m;
-#pragma unroll
-    for (uint32_t k = 0; k < 4u; ++k)
-        if (b0 + k < nblocks) {
-            blocksums[b0 + k] = make_uint2(ee, em);
-            ee += v[k].x;
-            em += v[k].y;
-        }
 }
 
 // Compaction of one 256-cluster tile (ploc/mod.rs:420-493): kept clusters and the parents of mutual pairs go to
