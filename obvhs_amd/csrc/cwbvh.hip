@@ -8,8 +8,10 @@
 //                earlier pass, so ranges processed in pass order see finished children (no atomics, no fences).
 //                Alongside the seven decisions each node stores the size of the CWBVH subtree it roots, which
 //                lets the top-down pass number nodes exactly like the reference's pre-order DFS.
-//   k_cw_emit  : one launch per CWBVH level, 8 lanes per node.  The group walks get_children, each lane loads one
-//                child, the octant assignment (order_children) runs as a per-round 8-lane argmin, then lane s
+//   k_cw_emit  : one launch per CWBVH level, 8 lanes per node.  The node's own box, first_index and table words
+//                come with its work item from its parent; the group walks get_children breadth first (lane j holds
+//                the j-th entry of the frontier and loads its node and table entry, one dependent load per
+//                expansion level), the octant assignment (order_children) runs as a per-round 8-lane argmin, then lane s
 //                quantises the child in slot s; prefix sums over the slots give the leaf primitive offsets and the
 //                internal children's DFS-derived node index, child base and primitive base, and the 80-byte node
 //                is assembled with lane shuffles (convert_to_cwbvh_impl, bvh2_to_cwbvh.rs:75-193;

@@ -9,11 +9,13 @@
 //   radix sort     : stable (sort.hip); a U128 key is sorted by its low word, then stably by its high word
 //   k_gather       : sorted leaves become the first cluster array (ploc/mod.rs:829-846)
 //   per pass       : k_ploc_search (ploc/mod.rs:329-419; window in LDS), k_ploc_count + k_ploc_scan (prefix
-//                    counts of emitted clusters and of merges), k_ploc_compact (ploc/mod.rs:420-487: the parent
-//                    of a mutual pair is emitted at the higher index, its children go to
-//                    nodes[insert - 2(s+1)] and [+1] where s is the merge's rank in the pass)
+//                    counts of emitted clusters and of merges; two launches over chunks for the largest passes),
+//                    k_ploc_compact (ploc/mod.rs:420-487: the parent of a mutual pair is emitted at the higher
+//                    index, its children go to nodes[insert - 2(s+1)] and [+1] where s is the merge's rank in the
+//                    pass); passes of at most 1,024 tiles fuse the scan into the compaction (k_ploc_scan_compact)
 //   k_ploc_tail    : once at most 2048 clusters remain, one workgroup finishes every remaining pass with the
-//                    clusters resident in LDS and writes the root (ploc/mod.rs:499)
+//                    clusters resident in LDS and writes the root (ploc/mod.rs:499); half its waves search while
+//                    the other half evaluate the cost-table entries of the previous pass's merges
 // Pass state (cluster count, insert index, depth) lives on the device so passes are enqueued without host syncs;
 // the host only reads the count every few passes to shrink the grids.
 //
