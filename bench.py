@@ -996,7 +996,7 @@ def main():
     ap.add_argument("--serve-scenes", type=int, default=64, help="0 skips the serving leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--serve-sweep", default="250,500,1000,1500,2000,3000,4000,6000",
+    ap.add_argument("--serve-sweep", default="250,500,1000,1500,2000,3000,4000,4500,5000,6000",
                     help="request rates of the serving sweep (empty: no sweep)")
     ap.add_argument("--serve-slo-ms", type=float, default=5.0)
     ap.add_argument("--serve-inflight", type=int, default=3)
