@@ -8,8 +8,9 @@
 //                    morton.rs:35-58), or the 126-bit code of SortPrecision::U128 (morton.rs:63-89) as two words
 //   radix sort     : stable (sort.hip); a U128 key is sorted by its low word, then stably by its high word
 //   k_gather       : sorted leaves become the first cluster array (ploc/mod.rs:829-846)
-//   per pass       : k_ploc_search (ploc/mod.rs:329-419; window in LDS), k_ploc_count + k_ploc_scan (prefix
-//                    counts of emitted clusters and of merges; two launches over chunks for the largest passes),
+//   per pass       : k_ploc_search_count (ploc/mod.rs:329-419, window in LDS, and the per-tile counts of emitted
+//                    clusters and merges), k_ploc_scan (their prefix sums; two launches over chunks for the largest
+//                    passes),
 //                    k_ploc_compact (ploc/mod.rs:420-487: the parent of a mutual pair is emitted at the higher
 //                    index, its children go to nodes[insert - 2(s+1)] and [+1] where s is the merge's rank in the
 //                    pass); passes of at most 1,024 tiles fuse the scan into the compaction (k_ploc_scan_compact)
@@ -238,83 +239,6 @@ __global__ void __launch_bounds__(T256) k_gather(const Node32* __restrict__ leav
     store_node(&out[i], load_node(&leaves[order[i]]));
 }
 
-__device__ __forceinline__ float window_cost(const float (*s)[T256 + 2 * MAX_RADIUS], const Box& me, int j) {
-    Box o{s[0][j], s[1][j], s[2][j], s[3][j], s[4][j], s[5][j]};
-    return half_area(box_union(me, o));
-}
-
-// The search of one 256-cluster tile starting at `base` (one PLOC neighbour pass, ploc/mod.rs:329-419): the window
-// [base - r, base + 256 + r) into the LDS arrays s, then every cluster's best neighbour, the LAST minimum over the
-// window in ascending index order (`<=`, find_best_node_basic ploc/mod.rs:508-528, equal to the cached search).
-// Called by every thread of the workgroup (it holds a workgroup barrier).
-__device__ __forceinline__ void search_tile(const Node32* __restrict__ cur, uint32_t count, uint32_t base, uint32_t r,
-                                            uint32_t dist1, int8_t* __restrict__ merge,
-                                            float (*s)[T256 + 2 * MAX_RADIUS], uint32_t* __restrict__ err) {
-    const int wstart = (int)base - (int)r;
-    const int wlen = T256 + 2 * (int)r;
-    bool nan_here = false;
-    for (int w = threadIdx.x; w < wlen; w += T256) {
-        int g = wstart + w;
-        if (g >= 0 && (uint32_t)g < count) {
-            Node32 nd = load_node(&cur[g]);
-            s[0][w] = nd.minx;
-            s[1][w] = nd.miny;
-            s[2][w] = nd.minz;
-            s[3][w] = nd.maxx;
-            s[4][w] = nd.maxy;
-            s[5][w] = nd.maxz;
-            nan_here |= box_has_nan_any(node_box(nd));
-        }
-    }
-    const bool nonan = !__syncthreads_or(nan_here);  // NaN-free window: costs from union_area_nonan
-    const uint32_t i = base + threadIdx.x;
-    if (i < count) {
-        const int li = (int)threadIdx.x + (int)r;
-        const Box me{s[0][li], s[1][li], s[2][li], s[3][li], s[4][li], s[5][li]};
-        int8_t m;
-        if (dist1) {
-            if (i == count - 1) {
-                m = -1;
-            } else {
-                float cost = window_cost(s, me, li + 1);
-                float last = INFINITY;
-                if (i > 0) {
-                    Box prev{s[0][li - 1], s[1][li - 1], s[2][li - 1], s[3][li - 1], s[4][li - 1], s[5][li - 1]};
-                    last = half_area(box_union(prev, me));
-                }
-                m = last < cost ? (int8_t)-1 : (int8_t)1;
-            }
-        } else {
-            const uint32_t begin = i - min(r, i);
-            const uint32_t end = min(i + r + 1u, count);
-            uint32_t best = i;
-            float best_cost = INFINITY;
-            for (uint32_t j = begin; j < end; ++j) {
-                if (j == i) continue;
-                const int o = (int)j - wstart;
-                const float c = nonan ? union_area_nonan(me, Box{s[0][o], s[1][o], s[2][o], s[3][o], s[4][o], s[5][o]})
-                                      : window_cost(s, me, o);
-                if (c <= best_cost) {
-                    best = j;
-                    best_cost = c;
-                }
-            }
-            if (best == i) atomicOr(err, DERR_SELF_MERGE);
-            m = (int8_t)((int)best - (int)i);
-        }
-        merge[i] = m;
-    }
-}
-
-__global__ void __launch_bounds__(T256) k_ploc_search(const Node32* __restrict__ cur, Misc* misc, uint32_t slot,
-                                                      uint32_t r, uint32_t dist1, int8_t* __restrict__ merge) {
-    __shared__ float s[6][T256 + 2 * MAX_RADIUS];
-    const uint32_t count = misc->state[slot].count;
-    const uint32_t base = blockIdx.x * T256;
-    if (count <= 1 || base >= count) return;
-    search_tile(cur, count, base, r, dist1, merge, s, &misc->err);
-}
-
 // Classification of cluster i (ploc/mod.rs:423-439): `emit` when it goes to the next pass (kept, or the higher
 // index of a mutual pair, which becomes the parent), `mrg` when it is the higher index of a mutual pair.
 // A self-pair (no finite neighbour, already flagged as an error) is treated as kept so that no slot underflows.
@@ -340,23 +264,113 @@ __device__ __forceinline__ void classify(const int8_t* merge, uint32_t i, uint32
     }
 }
 
-__global__ void __launch_bounds__(T256) k_ploc_count(const int8_t* __restrict__ merge, const Misc* misc,
-                                                     uint32_t slot, uint2* __restrict__ blocksums) {
+// The neighbour search of one pass (ploc/mod.rs:329-419) and the per-tile counts of emitted clusters and merges in
+// one launch.  Each cluster's best neighbour is the LAST minimum over its window in ascending index order (`<=`,
+// find_best_node_basic ploc/mod.rs:508-528, equal to the cached search).  Besides its 256 clusters, a workgroup
+// searches the r clusters on either side of its tile (the halo) with threads of their own (SC_THREADS = 256 +
+// 2 * MAX_RADIUS), so that it classifies each of its clusters against its neighbour's choice (ploc/mod.rs:423-439)
+// without another launch; the window in LDS spans [base - 2r, base + 256 + 2r).  Every workgroup writes its (emit,
+// merge) counts, zero beyond the cluster count.
+constexpr int SC_THREADS = T256 + 2 * (int)MAX_RADIUS;
+constexpr int SC_WINDOW = T256 + 4 * (int)MAX_RADIUS;
+
+// Best neighbour offset of cluster c over a window starting at cluster wstart.
+__device__ __forceinline__ int8_t cluster_merge(const float (*s)[SC_WINDOW], int wstart, uint32_t c, uint32_t count,
+                                                uint32_t r, uint32_t dist1, bool nonan, uint32_t* __restrict__ err) {
+    const int lc = (int)c - wstart;
+    const Box me{s[0][lc], s[1][lc], s[2][lc], s[3][lc], s[4][lc], s[5][lc]};
+    if (dist1) {
+        if (c == count - 1) return (int8_t)-1;
+        const float cost = half_area(box_union(me, Box{s[0][lc + 1], s[1][lc + 1], s[2][lc + 1], s[3][lc + 1],
+                                                      s[4][lc + 1], s[5][lc + 1]}));
+        float last = INFINITY;
+        if (c > 0)
+            last = half_area(box_union(Box{s[0][lc - 1], s[1][lc - 1], s[2][lc - 1], s[3][lc - 1], s[4][lc - 1],
+                                           s[5][lc - 1]},
+                                       me));
+        return last < cost ? (int8_t)-1 : (int8_t)1;
+    }
+    const uint32_t begin = c - min(r, c);
+    const uint32_t end = min(c + r + 1u, count);
+    uint32_t best = c;
+    float best_cost = INFINITY;
+    for (uint32_t j = begin; j < end; ++j) {
+        if (j == c) continue;
+        const int o = (int)j - wstart;
+        const Box ob{s[0][o], s[1][o], s[2][o], s[3][o], s[4][o], s[5][o]};
+        const float cost = nonan ? union_area_nonan(me, ob) : half_area(box_union(me, ob));
+        if (cost <= best_cost) {
+            best = j;
+            best_cost = cost;
+        }
+    }
+    if (best == c) atomicOr(err, DERR_SELF_MERGE);
+    return (int8_t)((int)best - (int)c);
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_ploc_search_count(const Node32* __restrict__ cur, Misc* misc,
+                                                                  uint32_t slot, uint32_t r, uint32_t dist1,
+                                                                  int8_t* __restrict__ merge,
+                                                                  uint2* __restrict__ blocksums) {
+    __shared__ float s[6][SC_WINDOW];
+    __shared__ int8_t mm[T256 + 2 * MAX_RADIUS];  // offsets of clusters [base - r, base + 256 + r)
     __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
     const uint32_t count = misc->state[slot].count;
-    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
-    bool emit, mrg;
-    uint32_t b;
-    classify(merge, i, count, emit, mrg, b);
-    const uint32_t e = (uint32_t)__popcll(__ballot(emit));
-    const uint32_t m = (uint32_t)__popcll(__ballot(mrg));
-    const uint32_t wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63u) == 0) {
-        we[wave] = e;
-        wm[wave] = m;
+    const uint32_t base = blockIdx.x * T256;
+    if (count <= 1 || base >= count) {  // workgroup-uniform
+        if (threadIdx.x == 0)
+            blocksums[blockIdx.x] = count == 1 && blockIdx.x == 0 ? make_uint2(1u, 0u) : make_uint2(0u, 0u);
+        return;
+    }
+    const int wstart = (int)base - 2 * (int)r;
+    const int wlen = T256 + 4 * (int)r;
+    bool nan_here = false;
+    for (int w = threadIdx.x; w < wlen; w += SC_THREADS) {
+        const int g = wstart + w;
+        if (g >= 0 && (uint32_t)g < count) {
+            const Node32 nd = load_node(&cur[g]);
+            s[0][w] = nd.minx;
+            s[1][w] = nd.miny;
+            s[2][w] = nd.minz;
+            s[3][w] = nd.maxx;
+            s[4][w] = nd.maxy;
+            s[5][w] = nd.maxz;
+            nan_here |= box_has_nan_any(node_box(nd));
+        }
+    }
+    const bool nonan = !__syncthreads_or(nan_here);  // NaN-free window: costs from union_area_nonan
+    // thread t < 256: cluster base + t; thread 256 + h, h < 2r: halo cluster base - r + h or base + 256 + h - r
+    const uint32_t t = threadIdx.x;
+    int c = -1;
+    if (t < (uint32_t)T256) c = (int)(base + t);
+    else if (t - T256 < 2u * r) c = t - T256 < r ? (int)base - (int)r + (int)(t - T256) : (int)(base + t - r);
+    if (c >= 0 && (uint32_t)c < count) {
+        const int8_t m = cluster_merge(s, wstart, (uint32_t)c, count, r, dist1, nonan, &misc->err);
+        mm[c - ((int)base - (int)r)] = m;
+        if (t < (uint32_t)T256) merge[c] = m;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    // classify (ploc/mod.rs:423-439) the tile's clusters from the offsets in LDS
+    bool emit = false, mrg = false;
+    if (t < (uint32_t)T256 && base + t < count) {
+        const int i = (int)(base + t);
+        const int bi = i + (int)mm[i - ((int)base - (int)r)];
+        const bool mutual = bi + (int)mm[bi - ((int)base - (int)r)] == i;
+        if (!mutual || bi == i) {
+            emit = true;
+        } else if (bi < i) {
+            emit = true;
+            mrg = true;
+        }
+    }
+    const uint64_t be = __ballot(emit), bm = __ballot(mrg);
+    const uint32_t wave = t >> 6;
+    if ((t & 63u) == 0 && wave < (uint32_t)(T256 / 64)) {
+        we[wave] = (uint32_t)__popcll(be);
+        wm[wave] = (uint32_t)__popcll(bm);
+    }
+    __syncthreads();
+    if (t == 0) {
         uint32_t E = 0, M = 0;
         for (int w = 0; w < T256 / 64; ++w) {
             E += we[w];
@@ -849,8 +863,7 @@ static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uin
         const uint32_t blocks = (known + T256 - 1) / T256;
         for (int k = 0; k < PASSES_PER_SYNC; ++k) {
             const uint32_t dist1 = (r == 1 || pass < search_depth_threshold) ? 1u : 0u;
-            k_ploc_search<<<blocks, T256, 0, st>>>(cur, misc, slot, r, dist1, merge);
-            k_ploc_count<<<blocks, T256, 0, st>>>(merge, misc, slot, blocksums);
+            k_ploc_search_count<<<blocks, SC_THREADS, 0, st>>>(cur, misc, slot, r, dist1, merge, blocksums);
             if (blocks <= SCAN_FUSED_BLOCKS) {
                 k_ploc_scan_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, blocks, bvh_nodes,
                                                             dp, fused, slot_map, vtop, map_len, hist);
