@@ -1365,10 +1365,12 @@ obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32
     k_levels_init<<<8, T256, 0, st>>>(meta, words, list);
     uint32_t* h = static_cast<uint32_t*>(ctx->pinned);
     uint32_t level = 0;
+    uint64_t known = 1;  // nodes at `level`, as last read (a level holds at most twice the nodes of the one above)
     for (;;) {
         for (uint32_t q = 0; q < LEVELS_PER_SYNC && level + q < MAX_TREE_LEVELS; ++q) {
-            const uint64_t bound = (level + q) < 31 ? std::min<uint64_t>(1ull << (level + q), N) : N;
-            const uint32_t g = (uint32_t)std::min<uint64_t>((bound + T256 - 1) / T256, 2048);
+            uint64_t bound = q < 31 ? std::min<uint64_t>(known << q, N) : N;
+            if (level + q < 31) bound = std::min<uint64_t>(bound, 1ull << (level + q));
+            const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((bound + T256 - 1) / T256, 2048));
             k_levels_expand<<<g, T256, 0, st>>>(nodes, N, list, meta, level + q, words - 1, expand);
         }
         OBVHS_TRY(hipGetLastError(), "levels launch");
@@ -1378,6 +1380,7 @@ obvhs_status tree_levels(obvhs_context* ctx, obvhs_bvh2* bvh, std::vector<uint32
         OBVHS_TRY(hipStreamSynchronize(st), "levels sync");
         if (h[2] != 0u) return OBVHS_ERR_CAPACITY;  // not a tree
         if (h[1] == 0u) break;
+        known = h[1];
         if (level >= MAX_TREE_LEVELS) return OBVHS_ERR_DEPTH;
     }
     meta_host.resize(2 * (size_t)(level + 1));
@@ -1626,8 +1629,17 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
 
     std::vector<uint32_t> meta;
     uint32_t levels = 0;
-    obvhs_status s = tree_levels(ctx, bvh, meta, levels, nullptr);
-    if (s != OBVHS_OK) return s;
+    bool have_levels = false;  // meta holds the current tree's level lists
+    obvhs_status s = OBVHS_OK;
+    if (bvh->pass_ordered && bvh->ploc_iterations > 0) {
+        // a tree fresh from PLOC: a pass raises a cluster's height by at most one, so its depth is at most the pass
+        // count + 1 -- all the first step needs (its stack bound); no level listing
+        levels = bvh->ploc_iterations + 1u;
+    } else {
+        s = tree_levels(ctx, bvh, meta, levels, nullptr);
+        if (s != OBVHS_OK) return s;
+        have_levels = true;
+    }
     // After a step's moves the level lists are rebuilt at once (the next step's stack bound is the depth) and the
     // refit waits for the next step, which reads the boxes; the last one runs after the loop, fused with the
     // conversion's cost table when one is asked for.
@@ -1715,11 +1727,16 @@ obvhs_status bvh2_reinsertion(obvhs_context* ctx, obvhs_bvh2* bvh, float batch_s
         OBVHS_TRY(hipGetLastError(), "apply launch");
         s = tree_levels(ctx, bvh, meta, levels, nullptr);
         if (s != OBVHS_OK) return s;
+        have_levels = true;
         refit_pending = !replay;
     }
     bvh->pass_ordered = false;  // children are no longer stored in PLOC pass order (reinsertion.rs:102)
     bvh->dp_max_prims = 0;
     if (dp_max_prims != 0u && bvh->prim_count > 1) {
+        if (!have_levels) {  // no step moved anything: the level lists of the tree as it is
+            s = tree_levels(ctx, bvh, meta, levels, nullptr);
+            if (s != OBVHS_OK) return s;
+        }
         OBVHS_TRY(bvh_dp_alloc(bvh), "alloc cost table");
         const DpView dp = bvh_dp_view(bvh, dp_max_prims);
         k_fill_u32<<<1, 64, 0, st>>>(&misc->err, 0u, 1);
