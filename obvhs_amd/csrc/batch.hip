@@ -499,6 +499,11 @@ __global__ void __launch_bounds__(T256) k_b_compact(const Node32* __restrict__ c
                                                     const MeshDesc* __restrict__ meshes, Node32* __restrict__ nodes,
                                                     DpView dp, uint32_t q) {
     __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
+    // The tile's merges, their cost-table entries evaluated by consecutive threads afterwards (a merge is counted at
+    // its higher cluster; a partner lies within its mesh's distance, at most MAX_RADIUS: at most 128 + r / 2).
+    constexpr uint32_t MAX_TILE_MERGES = (T256 + MAX_RADIUS) / 2;
+    __shared__ Node32 s_left[MAX_TILE_MERGES], s_right[MAX_TILE_MERGES];
+    __shared__ uint32_t s_first[MAX_TILE_MERGES], s_kbase[MAX_TILE_MERGES];
     const uint32_t total = misc->total[q];
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     bool emit, mrg;
@@ -512,33 +517,47 @@ __global__ void __launch_bounds__(T256) k_b_compact(const Node32* __restrict__ c
         wm[wave] = (uint32_t)__popcll(bm);
     }
     __syncthreads();
-    if (!emit) return;
-    uint32_t pe = offsets[blockIdx.x].x, pm = offsets[blockIdx.x].y;
-    for (uint32_t w = 0; w < wave; ++w) {
-        pe += we[w];
-        pm += wm[w];
+    if (emit) {
+        uint32_t pe = offsets[blockIdx.x].x, pm = 0u;
+        for (uint32_t w = 0; w < wave; ++w) {
+            pe += we[w];
+            pm += wm[w];
+        }
+        const uint32_t pos = pe + (uint32_t)__popcll(be & lt);
+        const uint32_t mesh = cmesh[i];
+        const Node32 a = load_node(&cur[i]);
+        nmesh[pos] = mesh;
+        if (mrg) {
+            const SegState sg = seg[mesh];
+            const MeshDesc md = meshes[mesh];
+            const uint32_t tq = pm + (uint32_t)__popcll(bm & lt);  // the merge's rank in this tile
+            const uint32_t s = offsets[blockIdx.x].y + tq - sg.mbase;
+            const Node32 c = load_node(&cur[b]);
+            const uint32_t first = sg.insert - 2u * (s + 1u);
+            Node32* mn = nodes + md.node_base;
+            store_node(&mn[first], a);
+            store_node(&mn[first + 1], c);
+            store_node(&next[pos], make_node(box_union(node_box(a), node_box(c)), 0u, first));
+            if (tq < MAX_TILE_MERGES) {
+                s_left[tq] = a;
+                s_right[tq] = c;
+                s_first[tq] = first;
+                s_kbase[tq] = md.prim_base;
+            } else {
+                atomicOr(&misc->err, DERR_CAPACITY);
+            }
+        } else {
+            store_node(&next[pos], a);
+        }
     }
-    const uint32_t pos = pe + (uint32_t)__popcll(be & lt);
-    const uint32_t mesh = cmesh[i];
-    const Node32 a = load_node(&cur[i]);
-    nmesh[pos] = mesh;
-    if (mrg) {
-        const SegState sg = seg[mesh];
-        const MeshDesc md = meshes[mesh];
-        const uint32_t s = pm + (uint32_t)__popcll(bm & lt) - sg.mbase;
-        const Node32 c = load_node(&cur[b]);
-        const uint32_t first = sg.insert - 2u * (s + 1u);
-        Node32* mn = nodes + md.node_base;
-        store_node(&mn[first], a);
-        store_node(&mn[first + 1], c);
-        const Box u = box_union(node_box(a), node_box(c));
-        store_node(&next[pos], make_node(u, 0u, first));
+    __syncthreads();
+    const uint32_t nm = min(wm[0] + wm[1] + wm[2] + wm[3], MAX_TILE_MERGES);
+    if (threadIdx.x < nm) {
+        const Node32 l = s_left[threadIdx.x], r = s_right[threadIdx.x];
         DpView dpm = dp;
-        dpm.kbase = md.prim_base;
-        const uint32_t e = dp_evaluate(dpm, u, first, a, c);
+        dpm.kbase = s_kbase[threadIdx.x];
+        const uint32_t e = dp_evaluate(dpm, box_union(node_box(l), node_box(r)), s_first[threadIdx.x], l, r);
         if (e) atomicOr(&misc->err, e);
-    } else {
-        store_node(&next[pos], a);
     }
 }
 

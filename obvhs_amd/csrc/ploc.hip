@@ -463,14 +463,21 @@ __global__ void __launch_bounds__(1024) k_ploc_scan_apply(uint2* __restrict__ bl
 
 // Compaction of one 256-cluster tile (ploc/mod.rs:420-493): kept clusters and the parents of mutual pairs go to
 // `next` in order; a merge's two children go to the slots below the pass's insert index in the serial loop's order
-// (`pe0` / `pm0`: the emits and merges of the tiles before this one), its cost-table entry is evaluated at once.
-// Called by every thread of the workgroup (it holds a workgroup barrier).
+// (`pe0` / `pm0`: the emits and merges of the tiles before this one).  The tile's merges are then
+// listed in LDS and their cost-table entries evaluated by consecutive threads, so that a pass where a fifth of the
+// clusters merge runs the evaluation in about one wave per tile instead of in every wave with a few lanes active.
+// Called by every thread of the workgroup (it holds workgroup barriers).
 __device__ __forceinline__ void compact_tile(const Node32* __restrict__ cur, Node32* __restrict__ next,
                                              const int8_t* __restrict__ merge, const PlocState& st, uint32_t base,
                                              uint32_t pe0, uint32_t pm0, Node32* __restrict__ bvh_nodes,
                                              const DpView& dp, uint32_t fused, const uint32_t* __restrict__ slot_map,
                                              uint32_t vtop, uint32_t map_len, uint32_t* __restrict__ err,
                                              uint32_t* we, uint32_t* wm) {
+    // A merge is counted at its higher cluster; pairs inside the tile use two of its clusters, and at most r of its
+    // clusters pair with distinct clusters in [base - r, base): at most 128 + r / 2 merges.
+    constexpr uint32_t MAX_TILE_MERGES = (T256 + MAX_RADIUS) / 2;
+    __shared__ Node32 s_left[MAX_TILE_MERGES], s_right[MAX_TILE_MERGES];
+    __shared__ uint32_t s_first[MAX_TILE_MERGES];
     const uint32_t count = st.count;
     const uint32_t i = base + threadIdx.x;
     bool emit, mrg;
@@ -496,21 +503,29 @@ __device__ __forceinline__ void compact_tile(const Node32* __restrict__ cur, Nod
             const uint32_t sq = pm + (uint32_t)__popcll(bm & lt);
             const Node32 c = load_node(&cur[b]);
             uint32_t first;
-            if (!child_slot(st.insert - 2u * (sq + 1u), slot_map, vtop, map_len, first)) {
+            const uint32_t q = sq - pm0;  // the merge's rank in this tile
+            if (q >= MAX_TILE_MERGES || !child_slot(st.insert - 2u * (sq + 1u), slot_map, vtop, map_len, first)) {
                 atomicOr(err, DERR_CAPACITY);
+                if (q < MAX_TILE_MERGES) s_first[q] = UINT32_MAX;
             } else {
                 store_node(&bvh_nodes[first], a);
                 store_node(&bvh_nodes[first + 1], c);
-                const Box u = box_union(node_box(a), node_box(c));
-                store_node(&next[pos], make_node(u, 0u, first));
-                if (fused) {
-                    const uint32_t e = dp_evaluate(dp, u, first, a, c);
-                    if (e) atomicOr(err, e);
-                }
+                store_node(&next[pos], make_node(box_union(node_box(a), node_box(c)), 0u, first));
+                s_left[q] = a;
+                s_right[q] = c;
+                s_first[q] = first;
             }
         } else {
             store_node(&next[pos], a);
         }
+    }
+    if (!fused) return;  // uniform
+    __syncthreads();
+    const uint32_t nm = min(wm[0] + wm[1] + wm[2] + wm[3], MAX_TILE_MERGES);
+    if (threadIdx.x < nm && s_first[threadIdx.x] != UINT32_MAX) {
+        const Node32 l = s_left[threadIdx.x], r = s_right[threadIdx.x];
+        const uint32_t e = dp_evaluate(dp, box_union(node_box(l), node_box(r)), s_first[threadIdx.x], l, r);
+        if (e) atomicOr(err, e);
     }
 }
 
