@@ -308,6 +308,40 @@ __device__ __forceinline__ int8_t cluster_merge(const float (*s)[SC_WINDOW], int
     return (int8_t)((int)best - (int)c);
 }
 
+// The same search for a distance R known at compile time, over a window whose slots outside [0, count) hold NaN
+// boxes: a NaN cost never passes `cost <= best_cost`, so the unrolled offsets -R..R (ascending, as j ascends) pick
+// what the bounded loop picks, with the LDS addresses as immediates and no per-neighbour bounds test.
+template <int R, bool NONAN>
+__device__ __forceinline__ int cluster_merge_unrolled(const float (*s)[SC_WINDOW], int lc) {
+    // U neighbours per unrolled block (U divides R); the blocks themselves stay a loop so that the compiler does
+    // not hoist every neighbour's loads (2R boxes would not fit the registers).
+    constexpr int U = R <= 8 ? R : (R % 8 == 0 ? 8 : (R % 7 == 0 ? 7 : 6));
+    static_assert(R % U == 0, "block size must divide the distance");
+    const Box me{s[0][lc], s[1][lc], s[2][lc], s[3][lc], s[4][lc], s[5][lc]};
+    int best = 0;
+    float best_cost = INFINITY;
+    auto block = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u;
+            const int o = lc + k;
+            const Box ob{s[0][o], s[1][o], s[2][o], s[3][o], s[4][o], s[5][o]};
+            const float cost = NONAN ? union_area_nonan(me, ob) : half_area(box_union(me, ob));
+            if (cost <= best_cost) {
+                best = k;
+                best_cost = cost;
+            }
+        }
+    };
+#pragma unroll 1
+    for (int k0 = -R; k0 < 0; k0 += U) block(k0);
+#pragma unroll 1
+    for (int k0 = 1; k0 <= R; k0 += U) block(k0);
+    return best;
+}
+
+// R > 0: the search distance r, known at compile time (unrolled search over a NaN-padded window); R = 0: any r.
+template <int R>
 __global__ void __launch_bounds__(SC_THREADS) k_ploc_search_count(const Node32* __restrict__ cur, Misc* misc,
                                                                   uint32_t slot, uint32_t r, uint32_t dist1,
                                                                   int8_t* __restrict__ merge,
@@ -336,6 +370,9 @@ __global__ void __launch_bounds__(SC_THREADS) k_ploc_search_count(const Node32* 
             s[4][w] = nd.maxy;
             s[5][w] = nd.maxz;
             nan_here |= box_has_nan_any(node_box(nd));
+        } else if (R > 0) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) s[a][w] = __builtin_nanf("");
         }
     }
     const bool nonan = !__syncthreads_or(nan_here);  // NaN-free window: costs from union_area_nonan
@@ -345,7 +382,18 @@ __global__ void __launch_bounds__(SC_THREADS) k_ploc_search_count(const Node32* 
     if (t < (uint32_t)T256) c = (int)(base + t);
     else if (t - T256 < 2u * r) c = t - T256 < r ? (int)base - (int)r + (int)(t - T256) : (int)(base + t - r);
     if (c >= 0 && (uint32_t)c < count) {
-        const int8_t m = cluster_merge(s, wstart, (uint32_t)c, count, r, dist1, nonan, &misc->err);
+        int8_t m;
+        bool unrolled = false;
+        if constexpr (R > 0) {
+            if (!dist1) {
+                const int b = nonan ? cluster_merge_unrolled<R, true>(s, c - wstart)
+                                    : cluster_merge_unrolled<R, false>(s, c - wstart);
+                if (b == 0) atomicOr(&misc->err, DERR_SELF_MERGE);
+                m = (int8_t)b;
+                unrolled = true;
+            }
+        }
+        if (!unrolled) m = cluster_merge(s, wstart, (uint32_t)c, count, r, dist1, nonan, &misc->err);
         mm[c - ((int)base - (int)r)] = m;
         if (t < (uint32_t)T256) merge[c] = m;
     }
@@ -861,6 +909,19 @@ static hipError_t read_state(const Misc* misc, uint32_t slot, void* pinned, hipS
     return hipSuccess;
 }
 
+// The search-and-count kernel for the search distance (each allowed distance has its unrolled instance).
+static void launch_search_count(uint32_t blocks, hipStream_t st, const Node32* cur, Misc* misc, uint32_t slot,
+                                uint32_t r, uint32_t dist1, int8_t* merge, uint2* blocksums) {
+    switch (r) {
+        case 2: k_ploc_search_count<2><<<blocks, SC_THREADS, 0, st>>>(cur, misc, slot, r, dist1, merge, blocksums); break;
+        case 6: k_ploc_search_count<6><<<blocks, SC_THREADS, 0, st>>>(cur, misc, slot, r, dist1, merge, blocksums); break;
+        case 14: k_ploc_search_count<14><<<blocks, SC_THREADS, 0, st>>>(cur, misc, slot, r, dist1, merge, blocksums); break;
+        case 24: k_ploc_search_count<24><<<blocks, SC_THREADS, 0, st>>>(cur, misc, slot, r, dist1, merge, blocksums); break;
+        case 32: k_ploc_search_count<32><<<blocks, SC_THREADS, 0, st>>>(cur, misc, slot, r, dist1, merge, blocksums); break;
+        default: k_ploc_search_count<0><<<blocks, SC_THREADS, 0, st>>>(cur, misc, slot, r, dist1, merge, blocksums); break;
+    }
+}
+
 static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uint32_t n, uint32_t r,
                                uint32_t search_depth_threshold, Node32* bvh_nodes, uint32_t* hist, DpView dp,
                                uint32_t fused, const uint32_t* slot_map, uint32_t vtop, uint32_t map_len,
@@ -878,7 +939,7 @@ static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uin
         const uint32_t blocks = (known + T256 - 1) / T256;
         for (int k = 0; k < PASSES_PER_SYNC; ++k) {
             const uint32_t dist1 = (r == 1 || pass < search_depth_threshold) ? 1u : 0u;
-            k_ploc_search_count<<<blocks, SC_THREADS, 0, st>>>(cur, misc, slot, r, dist1, merge, blocksums);
+            launch_search_count(blocks, st, cur, misc, slot, r, dist1, merge, blocksums);
             if (blocks <= SCAN_FUSED_BLOCKS) {
                 k_ploc_scan_compact<<<blocks, T256, 0, st>>>(cur, nxt, merge, misc, slot, blocksums, blocks, bvh_nodes,
                                                             dp, fused, slot_map, vtop, map_len, hist);
