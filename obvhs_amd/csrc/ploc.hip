@@ -312,12 +312,15 @@ __device__ __forceinline__ int8_t cluster_merge(const float (*s)[SC_WINDOW], int
 // boxes: a NaN cost never passes `cost <= best_cost`, so the unrolled offsets -R..R (ascending, as j ascends) pick
 // what the bounded loop picks, with the LDS addresses as immediates and no per-neighbour bounds test.
 template <int R, bool NONAN>
-__device__ __forceinline__ int cluster_merge_unrolled(const float (*s)[SC_WINDOW], int lc) {
+__device__ __forceinline__ int cluster_merge_unrolled(const float* __restrict__ x0, const float* __restrict__ y0,
+                                                      const float* __restrict__ z0, const float* __restrict__ x1,
+                                                      const float* __restrict__ y1, const float* __restrict__ z1,
+                                                      int lc) {
     // U neighbours per unrolled block (U divides R); the blocks themselves stay a loop so that the compiler does
     // not hoist every neighbour's loads (2R boxes would not fit the registers).
     constexpr int U = R <= 8 ? R : (R % 8 == 0 ? 8 : (R % 7 == 0 ? 7 : 6));
     static_assert(R % U == 0, "block size must divide the distance");
-    const Box me{s[0][lc], s[1][lc], s[2][lc], s[3][lc], s[4][lc], s[5][lc]};
+    const Box me{x0[lc], y0[lc], z0[lc], x1[lc], y1[lc], z1[lc]};
     int best = 0;
     float best_cost = INFINITY;
     auto block = [&](int k0) {
@@ -325,7 +328,7 @@ __device__ __forceinline__ int cluster_merge_unrolled(const float (*s)[SC_WINDOW
         for (int u = 0; u < U; ++u) {
             const int k = k0 + u;
             const int o = lc + k;
-            const Box ob{s[0][o], s[1][o], s[2][o], s[3][o], s[4][o], s[5][o]};
+            const Box ob{x0[o], y0[o], z0[o], x1[o], y1[o], z1[o]};
             const float cost = NONAN ? union_area_nonan(me, ob) : half_area(box_union(me, ob));
             if (cost <= best_cost) {
                 best = k;
@@ -386,8 +389,9 @@ __global__ void __launch_bounds__(SC_THREADS) k_ploc_search_count(const Node32* 
         bool unrolled = false;
         if constexpr (R > 0) {
             if (!dist1) {
-                const int b = nonan ? cluster_merge_unrolled<R, true>(s, c - wstart)
-                                    : cluster_merge_unrolled<R, false>(s, c - wstart);
+                const int lc = c - wstart;
+                const int b = nonan ? cluster_merge_unrolled<R, true>(s[0], s[1], s[2], s[3], s[4], s[5], lc)
+                                    : cluster_merge_unrolled<R, false>(s[0], s[1], s[2], s[3], s[4], s[5], lc);
                 if (b == 0) atomicOr(&misc->err, DERR_SELF_MERGE);
                 m = (int8_t)b;
                 unrolled = true;
@@ -650,6 +654,8 @@ __global__ void __launch_bounds__(T256) k_ploc_scan_compact(const Node32* __rest
 }
 
 // Finishes the build inside one workgroup once count <= TAIL_MAX.  Clusters live in LDS as SoA.
+// R > 0: the search distance, known at compile time (the unrolled search over NaN-padded arrays); R = 0: any r.
+template <int R>
 __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __restrict__ cur, Misc* misc,
                                                             uint32_t slot, uint32_t first_pass, uint32_t r,
                                                             uint32_t sd_is_one, uint32_t depth_threshold,
@@ -657,7 +663,16 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
                                                             uint32_t* __restrict__ hist, DpView dp, uint32_t fused,
                                                             const uint32_t* __restrict__ slot_map, uint32_t vtop,
                                                             uint32_t map_len) {
-    __shared__ float lx[TAIL_MAX], ly[TAIL_MAX], lz[TAIL_MAX], hx[TAIL_MAX], hy[TAIL_MAX], hz[TAIL_MAX];
+    // Box arrays with MAX_RADIUS slots on either side: NaN boxes before cluster 0 and after the last cluster, so that
+    // the unrolled search needs no bounds test (refreshed after each pass's compaction).
+    constexpr int PADDED = TAIL_MAX + 2 * MAX_RADIUS;
+    __shared__ float lx_p[PADDED], ly_p[PADDED], lz_p[PADDED], hx_p[PADDED], hy_p[PADDED], hz_p[PADDED];
+    float* const lx = lx_p + MAX_RADIUS;
+    float* const ly = ly_p + MAX_RADIUS;
+    float* const lz = lz_p + MAX_RADIUS;
+    float* const hx = hx_p + MAX_RADIUS;
+    float* const hy = hy_p + MAX_RADIUS;
+    float* const hz = hz_p + MAX_RADIUS;
     __shared__ uint32_t pc[TAIL_MAX], fi[TAIL_MAX];
     __shared__ int8_t mg[TAIL_MAX];
     __shared__ uint32_t mlist[TAIL_MAX / 2];  // first child slot of every merge of the pass (cost table work list)
@@ -680,6 +695,10 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
         hz[i] = nd.maxz;
         pc[i] = nd.prim_count;
         fi[i] = nd.first_index;
+    }
+    if (tid < 2u * MAX_RADIUS) {  // the pads: [-MAX_RADIUS, 0) and [count, count + MAX_RADIUS)
+        const int p = tid < MAX_RADIUS ? -1 - (int)tid : (int)(count + tid - MAX_RADIUS);
+        lx[p] = ly[p] = lz[p] = hx[p] = hy[p] = hz[p] = __builtin_nanf("");
     }
     bool nan_here = false;
     for (uint32_t i = tid; i < count; i += TAIL_THREADS)
@@ -723,6 +742,11 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
                     }
                     m = last < cost ? (int8_t)-1 : (int8_t)1;
                 }
+            } else if constexpr (R > 0) {
+                const int b = nonan ? cluster_merge_unrolled<R, true>(lx, ly, lz, hx, hy, hz, (int)i)
+                                    : cluster_merge_unrolled<R, false>(lx, ly, lz, hx, hy, hz, (int)i);
+                if (b == 0) self_merge = true;
+                m = (int8_t)b;
             } else {
                 const uint32_t begin = i - min(r, i);
                 const uint32_t end = min(i + r + 1u, count);
@@ -818,6 +842,10 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_ploc_tail(const Node32* __rest
             pc[pe] = opc;
             fi[pe] = ofi;
             pe++;
+        }
+        if (tid < MAX_RADIUS) {  // the pad after the new count (old clusters' slots, read before the last barrier)
+            const uint32_t p = (total & 0xffffu) + tid;
+            lx[p] = ly[p] = lz[p] = hx[p] = hy[p] = hz[p] = __builtin_nanf("");
         }
         __syncthreads();
         nm_prev = fused ? (total >> 16) : 0u;
@@ -922,6 +950,25 @@ static void launch_search_count(uint32_t blocks, hipStream_t st, const Node32* c
     }
 }
 
+// The tail for the search distance (each allowed distance has its unrolled instance).
+static void launch_tail(hipStream_t st, const Node32* cur, Misc* misc, uint32_t slot, uint32_t pass, uint32_t r,
+                        uint32_t threshold, Node32* bvh_nodes, uint32_t* hist, const DpView& dp, uint32_t fused,
+                        const uint32_t* slot_map, uint32_t vtop, uint32_t map_len) {
+    const uint32_t one = r == 1 ? 1u : 0u;
+#define OBVHS_TAIL(RR)                                                                                           \
+    k_ploc_tail<RR><<<1, TAIL_THREADS, 0, st>>>(cur, misc, slot, pass, r, one, threshold, bvh_nodes, hist, dp, \
+                                                fused, slot_map, vtop, map_len)
+    switch (r) {
+        case 2: OBVHS_TAIL(2); break;
+        case 6: OBVHS_TAIL(6); break;
+        case 14: OBVHS_TAIL(14); break;
+        case 24: OBVHS_TAIL(24); break;
+        case 32: OBVHS_TAIL(32); break;
+        default: OBVHS_TAIL(0); break;
+    }
+#undef OBVHS_TAIL
+}
+
 static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uint32_t n, uint32_t r,
                                uint32_t search_depth_threshold, Node32* bvh_nodes, uint32_t* hist, DpView dp,
                                uint32_t fused, const uint32_t* slot_map, uint32_t vtop, uint32_t map_len,
@@ -971,8 +1018,8 @@ static obvhs_status run_passes(obvhs_context* ctx, Node32* cur, Node32* nxt, uin
         // The device depth counts real passes only; later enqueued passes may have been no-ops.
         pass = hstate.depth;
     }
-    k_ploc_tail<<<1, TAIL_THREADS, 0, st>>>(cur, misc, slot, pass, r, r == 1 ? 1u : 0u, search_depth_threshold,
-                                            bvh_nodes, hist, dp, fused, slot_map, vtop, map_len);
+    launch_tail(st, cur, misc, slot, pass, r, search_depth_threshold, bvh_nodes, hist, dp, fused, slot_map, vtop,
+                map_len);
     OBVHS_TRY(hipGetLastError(), "ploc tail launch");
     uint32_t herr = 0u;
     OBVHS_TRY(read_state(misc, slot, ctx->pinned, st, hstate, herr), "read final ploc state");
