@@ -518,10 +518,13 @@ __global__ void __launch_bounds__(1024) k_ploc_scan_apply(uint2* __restrict__ bl
 // (`pe0` / `pm0`: the emits and merges of the tiles before this one).  The tile's merges are then
 // listed in LDS and their cost-table entries evaluated by consecutive threads, so that a pass where a fifth of the
 // clusters merge runs the evaluation in about one wave per tile instead of in every wave with a few lanes active.
-// Called by every thread of the workgroup (it holds workgroup barriers).
+// The caller classifies its cluster and loads its node (`a`, for i < count) at the start of the kernel, so that
+// those loads overlap the tile offsets' loads.  Called by every thread of the workgroup (it holds workgroup
+// barriers).
 __device__ __forceinline__ void compact_tile(const Node32* __restrict__ cur, Node32* __restrict__ next,
-                                             const int8_t* __restrict__ merge, const PlocState& st, uint32_t base,
-                                             uint32_t pe0, uint32_t pm0, Node32* __restrict__ bvh_nodes,
+                                             const Node32& a, bool emit, bool mrg, uint32_t b,
+                                             const PlocState& st, uint32_t pe0, uint32_t pm0,
+                                             Node32* __restrict__ bvh_nodes,
                                              const DpView& dp, uint32_t fused, const uint32_t* __restrict__ slot_map,
                                              uint32_t vtop, uint32_t map_len, uint32_t* __restrict__ err,
                                              uint32_t* we, uint32_t* wm) {
@@ -530,11 +533,6 @@ __device__ __forceinline__ void compact_tile(const Node32* __restrict__ cur, Nod
     constexpr uint32_t MAX_TILE_MERGES = (T256 + MAX_RADIUS) / 2;
     __shared__ Node32 s_left[MAX_TILE_MERGES], s_right[MAX_TILE_MERGES];
     __shared__ uint32_t s_first[MAX_TILE_MERGES];
-    const uint32_t count = st.count;
-    const uint32_t i = base + threadIdx.x;
-    bool emit, mrg;
-    uint32_t b;
-    classify(merge, i, count, emit, mrg, b);
     const uint64_t be = __ballot(emit), bm = __ballot(mrg);
     const uint64_t lt = lanemask_lt();
     const uint32_t wave = threadIdx.x >> 6;
@@ -550,7 +548,6 @@ __device__ __forceinline__ void compact_tile(const Node32* __restrict__ cur, Nod
             pm += wm[w];
         }
         const uint32_t pos = pe + (uint32_t)__popcll(be & lt);
-        const Node32 a = load_node(&cur[i]);
         if (mrg) {
             const uint32_t sq = pm + (uint32_t)__popcll(bm & lt);
             const Node32 c = load_node(&cur[b]);
@@ -589,9 +586,15 @@ __global__ void __launch_bounds__(T256) k_ploc_compact(const Node32* __restrict_
                                                        uint32_t map_len) {
     __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
     const PlocState st = misc->state[slot];
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    Node32 a{};
+    if (i < st.count) a = load_node(&cur[i]);
+    bool emit, mrg;
+    uint32_t b;
+    classify(merge, i, st.count, emit, mrg, b);
     const uint2 off = offsets[blockIdx.x];
-    compact_tile(cur, next, merge, st, blockIdx.x * T256, off.x, off.y, bvh_nodes, dp, fused, slot_map, vtop,
-                 map_len, &misc->err, we, wm);
+    compact_tile(cur, next, a, emit, mrg, b, st, off.x, off.y, bvh_nodes, dp, fused, slot_map, vtop, map_len,
+                 &misc->err, we, wm);
 }
 
 // k_ploc_scan and k_ploc_compact in one launch for passes of at most SCAN_FUSED_BLOCKS tiles: each workgroup sums the
@@ -608,10 +611,16 @@ __global__ void __launch_bounds__(T256) k_ploc_scan_compact(const Node32* __rest
     __shared__ uint32_t we[T256 / 64], wm[T256 / 64];
     __shared__ uint2 pre;
     const PlocState st = misc->state[slot];
+    const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    Node32 a{};
+    if (i < st.count) a = load_node(&cur[i]);
+    bool emit, mrg;
+    uint32_t b;
+    classify(merge, i, st.count, emit, mrg, b);
     const uint32_t upto = blockIdx.x == 0 ? nblocks : blockIdx.x;  // workgroup 0 sums every tile (the totals)
     uint32_t E = 0, M = 0;
-    for (uint32_t b = threadIdx.x; b < upto; b += T256) {
-        const uint2 v = blocksums[b];
+    for (uint32_t t = threadIdx.x; t < upto; t += T256) {
+        const uint2 v = blocksums[t];
         E += v.x;
         M += v.y;
     }
@@ -649,8 +658,8 @@ __global__ void __launch_bounds__(T256) k_ploc_scan_compact(const Node32* __rest
     __syncthreads();
     const uint2 off = pre;
     __syncthreads();  // compact_tile reuses we / wm
-    compact_tile(cur, next, merge, st, blockIdx.x * T256, off.x, off.y, bvh_nodes, dp, fused, slot_map, vtop,
-                 map_len, &misc->err, we, wm);
+    compact_tile(cur, next, a, emit, mrg, b, st, off.x, off.y, bvh_nodes, dp, fused, slot_map, vtop, map_len,
+                 &misc->err, we, wm);
 }
 
 // Finishes the build inside one workgroup once count <= TAIL_MAX.  Clusters live in LDS as SoA.
