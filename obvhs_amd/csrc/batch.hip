@@ -506,6 +506,8 @@ __global__ void __launch_bounds__(T256) k_b_compact(const Node32* __restrict__ c
     __shared__ uint32_t s_first[MAX_TILE_MERGES], s_kbase[MAX_TILE_MERGES];
     const uint32_t total = misc->total[q];
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
+    Node32 a{};
+    if (i < total) a = load_node(&cur[i]);  // before the classification's dependent loads and the barrier
     bool emit, mrg;
     uint32_t b;
     b_classify(merge, cmesh, seg, i, total, emit, mrg, b);
@@ -525,7 +527,6 @@ __global__ void __launch_bounds__(T256) k_b_compact(const Node32* __restrict__ c
         }
         const uint32_t pos = pe + (uint32_t)__popcll(be & lt);
         const uint32_t mesh = cmesh[i];
-        const Node32 a = load_node(&cur[i]);
         nmesh[pos] = mesh;
         if (mrg) {
             const SegState sg = seg[mesh];
