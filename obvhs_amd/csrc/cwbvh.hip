@@ -158,27 +158,51 @@ __device__ __forceinline__ uint32_t cw_exponent(float ext) {
 }
 
 // Group helpers: the 8 lanes r = 0..7 of an aligned lane group work on one CWBVH node (lane r first owns the
-// node's child r, then slot r).
+// node's child r, then slot r).  The fixed lane patterns inside a group (xor 1, 2, 4 and shifts up by 1, 2, 4) are
+// DPP moves, which stay in the VALU instead of a round trip through the LDS crossbar: quad_perm for xor 1 and 2,
+// and within a row of 16 lanes, row_shl:4 (lane i reads lane i + 4) and row_shr:d (lane i reads lane i - d), used
+// only where the source lies in the same group.
+constexpr int DPP_XOR1 = 0xB1;      // quad_perm [1, 0, 3, 2]
+constexpr int DPP_XOR2 = 0x4E;      // quad_perm [2, 3, 0, 1]
+constexpr int DPP_ROW_SHL4 = 0x104;
+constexpr int DPP_ROW_SHR1 = 0x111;
+constexpr int DPP_ROW_SHR2 = 0x112;
+constexpr int DPP_ROW_SHR4 = 0x114;
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t g8_xor1(uint32_t v) { return dpp<DPP_XOR1>(v); }
+__device__ __forceinline__ uint32_t g8_xor2(uint32_t v) { return dpp<DPP_XOR2>(v); }
+__device__ __forceinline__ uint32_t g8_xor4(uint32_t v, uint32_t r) {
+    const uint32_t up = dpp<DPP_ROW_SHL4>(v), dn = dpp<DPP_ROW_SHR4>(v);
+    return (r & 4u) ? dn : up;
+}
+__device__ __forceinline__ float g8_xor1f(float v) { return __uint_as_float(g8_xor1(__float_as_uint(v))); }
+__device__ __forceinline__ float g8_xor2f(float v) { return __uint_as_float(g8_xor2(__float_as_uint(v))); }
+__device__ __forceinline__ float g8_xor4f(float v, uint32_t r) { return __uint_as_float(g8_xor4(__float_as_uint(v), r)); }
 __device__ __forceinline__ uint32_t g8_or(uint32_t v) {
-    v |= (uint32_t)__shfl_xor((int)v, 1, 8);
-    v |= (uint32_t)__shfl_xor((int)v, 2, 8);
-    v |= (uint32_t)__shfl_xor((int)v, 4, 8);
+    const uint32_t r = threadIdx.x & 7u;
+    v |= g8_xor1(v);
+    v |= g8_xor2(v);
+    v |= g8_xor4(v, r);
     return v;
 }
 __device__ __forceinline__ uint32_t g8_inclusive_sum(uint32_t v, uint32_t r) {
-#pragma unroll
-    for (uint32_t d = 1; d < 8; d <<= 1) {
-        const uint32_t t = (uint32_t)__shfl_up((int)v, d, 8);
-        if (r >= d) v += t;
-    }
+    uint32_t t = dpp<DPP_ROW_SHR1>(v);
+    if (r >= 1u) v += t;
+    t = dpp<DPP_ROW_SHR2>(v);
+    if (r >= 2u) v += t;
+    t = dpp<DPP_ROW_SHR4>(v);
+    if (r >= 4u) v += t;
     return v;
 }
 // One 8-byte child array of the node from one byte per slot lane: (bytes of slots 0-3, bytes of slots 4-7).
 __device__ __forceinline__ uint2 g8_bytes(uint32_t byte, uint32_t r) {
     uint32_t w = byte << (8u * (r & 3u));
-    w |= (uint32_t)__shfl_xor((int)w, 1, 8);
-    w |= (uint32_t)__shfl_xor((int)w, 2, 8);
-    const uint32_t x = (uint32_t)__shfl_xor((int)w, 4, 8);
+    w |= g8_xor1(w);
+    w |= g8_xor2(w);
+    const uint32_t x = g8_xor4(w, r);
     return r < 4u ? make_uint2(w, x) : make_uint2(x, w);
 }
 __device__ __forceinline__ uint32_t quantise_lo(float b, float p, float rcp) {
@@ -344,8 +368,9 @@ __device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, cons
             int bl = (int)r;
 #pragma unroll
             for (int m = 1; m < 8; m <<= 1) {
-                const float oc = __shfl_xor(bc, m, 8);
-                const int os = __shfl_xor(bs, m, 8), ol = __shfl_xor(bl, m, 8);
+                const float oc = m == 1 ? g8_xor1f(bc) : (m == 2 ? g8_xor2f(bc) : g8_xor4f(bc, r));
+                const int os = (int)(m == 1 ? g8_xor1((uint32_t)bs) : (m == 2 ? g8_xor2((uint32_t)bs) : g8_xor4((uint32_t)bs, r)));
+                const int ol = (int)(m == 1 ? g8_xor1((uint32_t)bl) : (m == 2 ? g8_xor2((uint32_t)bl) : g8_xor4((uint32_t)bl, r)));
                 if (os >= 0 && (bs < 0 || oc < bc || (oc == bc && ol < bl))) {
                     bc = oc;
                     bs = os;
