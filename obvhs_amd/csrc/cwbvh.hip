@@ -289,27 +289,29 @@ __device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, cons
         werr = g8_or(werr);
         if (werr != 0u || g8_or(dist ? 1u : 0u) == 0u) break;
         const uint32_t w = mine ? (dist ? 2u : 1u) : 0u;
-        const uint32_t incl = g8_inclusive_sum(w, r);
-        const int F2 = __shfl((int)incl, 7, 8);
+        // every lane's width (2 bits each) in one word; lane r of the new frontier takes entry part of the entry of
+        // source lane q whose range [acc_q, acc_q + w_q) holds r (acc_q: the widths of the lanes before q)
+        const uint32_t widths = g8_or(w << (2u * r));
+        int src = 0;
+        uint32_t part = 0u, acc = 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t wq = (widths >> (2 * q)) & 3u;
+            if (r >= acc && r < acc + wq) {
+                src = q;
+                part = r - acc;
+            }
+            acc += wq;
+        }
+        const int F2 = (int)acc;
         if (F2 > 8) {  // more than 8 children
             werr = DERR_CAPACITY;
             break;
         }
-        // lane r of the new frontier: entry part of the entry of source lane q whose range [incl - w, incl) holds r
-        int src = 0;
-        uint32_t part = 0u;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const uint32_t iq = (uint32_t)__shfl((int)incl, q, 8), wq = (uint32_t)__shfl((int)w, q, 8);
-            if (r + wq >= iq && r < iq) {
-                src = q;
-                part = r + wq - iq;
-            }
-        }
         const uint32_t sx = (uint32_t)__shfl((int)fx, src, 8), sinfo = (uint32_t)__shfl((int)finfo, src, 8);
         const uint32_t sdi = (uint32_t)__shfl((int)fdi, src, 8), sb = (uint32_t)__shfl((int)b, src, 8);
         const uint32_t skx = (uint32_t)__shfl((int)rkids.x, src, 8), sky = (uint32_t)__shfl((int)rkids.y, src, 8);
-        const uint32_t sw = (uint32_t)__shfl((int)w, src, 8);
+        const uint32_t sw = (widths >> (2 * src)) & 3u;
         if (sw == 2u) {  // a DISTRIBUTE entry's children: slots first, first + 1
             fx = (sinfo & ~KID_LEAF) + part;
             finfo = part != 0u ? sky : skx;
@@ -390,11 +392,12 @@ __device__ __forceinline__ void emit_node(const Node32* __restrict__ nodes, cons
         }
     }
 
-    // lane r: slot r (the child placed there, if any)
+    // lane r: slot r (the child placed there, if any), from every lane's slot + 1 (4 bits each) in one word
+    const uint32_t slots = g8_or((uint32_t)(my_slot + 1) << (4u * r));
     int src = -1;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-        if (__shfl(my_slot, q, 8) == (int)r) src = q;
+        if (((slots >> (4 * q)) & 0xfu) == r + 1u) src = q;
     const bool occ = src >= 0;
     const int sl = occ ? src : (int)r;
     const Box sb{__shfl(cb.lx, sl, 8), __shfl(cb.ly, sl, 8), __shfl(cb.lz, sl, 8),
