@@ -238,7 +238,8 @@ __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box
         if (!R.leaf) c = dp.rec[R.key].plist;
         pl = a.w >= 2u ? make_uint4(a.x, a.y, c.x, num_primitives) : make_uint4(a.x, c.x, c.y, num_primitives);
     }
-    // The whole entry in seven 16-byte stores.
+    // The whole 128-byte entry in eight 16-byte stores (the padding too, so that no 32-byte sector of the line is
+    // left partly written).
     uint4* e = reinterpret_cast<uint4*>(&dp.rec[dp_key(dp, first)]);
     e[0] = make_uint4(__float_as_uint(cost[0]), __float_as_uint(cost[1]), __float_as_uint(cost[2]),
                       __float_as_uint(cost[3]));
@@ -250,6 +251,7 @@ __device__ __forceinline__ uint32_t dp_evaluate(const DpView& dp, const Box& box
     e[5] = make_uint4(l.first_index | (L.leaf ? KID_LEAF : 0u), r.first_index | (R.leaf ? KID_LEAF : 0u), 1u + sd[0],
                       0u);
     e[6] = pl;
+    e[7] = make_uint4(0u, 0u, 0u, 0u);
     return 0u;
 }
 
