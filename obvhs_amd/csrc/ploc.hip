@@ -132,12 +132,15 @@ __global__ void __launch_bounds__(1024) k_reduce_bounds(const uint32_t* __restri
 }
 
 // Triangle::aabb = Aabb::from_points([v0, v1, v2]) (triangle.rs:28-30; aabb.rs:59-66).
+// iota_out (when the caller gave no indices): primitive_indices = 0..n-1, written here instead of by a launch of its own.
 __global__ void __launch_bounds__(T256) k_leaf_init_tris(const float4* __restrict__ tris,
                                                          const uint32_t* __restrict__ indices, uint32_t n,
                                                          Node32* __restrict__ leaves, Misc* misc,
-                                                         uint32_t* __restrict__ partials) {
+                                                         uint32_t* __restrict__ partials,
+                                                         uint32_t* __restrict__ iota_out) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     const bool valid = i < n;
+    if (iota_out && valid) iota_out[i] = i;
     Box b{0, 0, 0, 0, 0, 0};
     bool nan = false;
     if (valid) {
@@ -157,9 +160,11 @@ __global__ void __launch_bounds__(T256) k_leaf_init_tris(const float4* __restric
 __global__ void __launch_bounds__(T256) k_leaf_init_aabbs(const float4* __restrict__ aabbs,
                                                           const uint32_t* __restrict__ indices, uint32_t n,
                                                           Node32* __restrict__ leaves, Misc* misc,
-                                                         uint32_t* __restrict__ partials) {
+                                                          uint32_t* __restrict__ partials,
+                                                          uint32_t* __restrict__ iota_out) {
     const uint32_t i = blockIdx.x * T256 + threadIdx.x;
     const bool valid = i < n;
+    if (iota_out && valid) iota_out[i] = i;
     Box b{0, 0, 0, 0, 0, 0};
     bool nan = false;
     if (valid) {
@@ -1090,12 +1095,13 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
                   "copy indices");
     }
     k_init_misc<<<1, 64, 0, st>>>(misc, n, hist);
+    uint32_t* iota_out = d_indices ? nullptr : bvh->prim_indices.as<uint32_t>();  // primitive_indices = 0..n-1
     if (d_tris) {
         k_leaf_init_tris<<<nblk, T256, 0, st>>>(reinterpret_cast<const float4*>(d_tris), d_indices, n, leaves,
-                                                misc, partials);
+                                                misc, partials, iota_out);
     } else {
         k_leaf_init_aabbs<<<nblk, T256, 0, st>>>(reinterpret_cast<const float4*>(d_aabbs), d_indices, n, leaves,
-                                                 misc, partials);
+                                                 misc, partials, iota_out);
     }
     k_reduce_bounds<<<1, 1024, 0, st>>>(partials, nblk, misc);
     k_morton<<<nblk, T256, 0, st>>>(leaves, n, misc, keys, keys_hi);
@@ -1121,7 +1127,6 @@ obvhs_status ploc_build(obvhs_context* ctx, const obvhs_triangle* d_tris, const 
     k_gather<<<nblk, T256, 0, st>>>(leaves, vals, n, cur);
     OBVHS_TRY(hipGetLastError(), "gather launch");
 
-    if (!d_indices) OBVHS_TRY(launch_iota(bvh->prim_indices.as<uint32_t>(), n, st), "iota");
     PlocState fin{};
     obvhs_status ps = run_passes(ctx, cur, nxt, n, search_distance, search_depth_threshold, bvh_nodes, hist, dp,
                                  fused, nullptr, 0u, 0u, fin);
