@@ -348,6 +348,47 @@ __device__ __forceinline__ int cluster_merge_unrolled(const float* __restrict__ 
     return best;
 }
 
+// Symmetric form of the unrolled search for a NaN-free window: union_area_nonan(a, b) == union_area_nonan(b, a) bit
+// for bit (IEEE minimum / maximum are commutative, signed zeros included), so slot w computes its R right-hand pair
+// costs once into pc[k - 1][w] and a cluster reads its left-hand costs from the slots before it.
+template <int R>
+__device__ __forceinline__ void pair_costs(const float (*s)[SC_WINDOW], float (*pc)[SC_WINDOW], int w) {
+    constexpr int U = R <= 8 ? R : (R % 8 == 0 ? 8 : (R % 7 == 0 ? 7 : 6));
+    const Box me{s[0][w], s[1][w], s[2][w], s[3][w], s[4][w], s[5][w]};
+#pragma unroll 1
+    for (int k0 = 1; k0 <= R; k0 += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u, o = w + k;
+            pc[k - 1][w] = union_area_nonan(me, Box{s[0][o], s[1][o], s[2][o], s[3][o], s[4][o], s[5][o]});
+        }
+    }
+}
+
+// The pick over the pair costs: offsets -R..-1 then 1..R (j ascending), the last minimum, as cluster_merge_unrolled.
+template <int R>
+__device__ __forceinline__ int pick_from_pairs(const float (*pc)[SC_WINDOW], int lc) {
+    int best = 0;
+    float best_cost = INFINITY;
+#pragma unroll
+    for (int k = R; k >= 1; --k) {
+        const float cost = pc[k - 1][lc - k];
+        if (cost <= best_cost) {
+            best = -k;
+            best_cost = cost;
+        }
+    }
+#pragma unroll
+    for (int k = 1; k <= R; ++k) {
+        const float cost = pc[k - 1][lc];
+        if (cost <= best_cost) {
+            best = k;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
 // R > 0: the search distance r, known at compile time (unrolled search over a NaN-padded window); R = 0: any r.
 template <int R>
 __global__ void __launch_bounds__(SC_THREADS) k_ploc_search_count(const Node32* __restrict__ cur, Misc* misc,
@@ -386,6 +427,15 @@ __global__ void __launch_bounds__(SC_THREADS) k_ploc_search_count(const Node32* 
     const bool nonan = !__syncthreads_or(nan_here);  // NaN-free window: costs from union_area_nonan
     // thread t < 256: cluster base + t; thread 256 + h, h < 2r: halo cluster base - r + h or base + 256 + h - r
     const uint32_t t = threadIdx.x;
+    // NaN-free windows: every pair cost once (the searchers sit at slots [R, 256 + 3R), their partners below 256 + 4R)
+    __shared__ float pcost[R > 0 ? R : 1][SC_WINDOW];
+    const bool pairs = R > 0 && !dist1 && nonan;  // workgroup-uniform
+    if constexpr (R > 0) {
+        if (pairs) {
+            for (int w = (int)t; w < T256 + 3 * R; w += SC_THREADS) pair_costs<R>(s, pcost, w);
+            __syncthreads();
+        }
+    }
     int c = -1;
     if (t < (uint32_t)T256) c = (int)(base + t);
     else if (t - T256 < 2u * r) c = t - T256 < r ? (int)base - (int)r + (int)(t - T256) : (int)(base + t - r);
@@ -395,7 +445,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_ploc_search_count(const Node32* 
         if constexpr (R > 0) {
             if (!dist1) {
                 const int lc = c - wstart;
-                const int b = nonan ? cluster_merge_unrolled<R, true>(s[0], s[1], s[2], s[3], s[4], s[5], lc)
+                const int b = pairs ? pick_from_pairs<R>(pcost, lc)
                                     : cluster_merge_unrolled<R, false>(s[0], s[1], s[2], s[3], s[4], s[5], lc);
                 if (b == 0) atomicOr(&misc->err, DERR_SELF_MERGE);
                 m = (int8_t)b;
