@@ -267,22 +267,33 @@ __global__ void __launch_bounds__(T256) k_b_search(const Node32* __restrict__ cu
             mv = last < cost ? (int8_t)-1 : (int8_t)1;
         }
     } else {
-        const uint32_t begin = li - min(r, li);
-        const uint32_t end = min(li + r + 1u, sg.count);
-        uint32_t best = li;
+        // offsets k in [-min(r, li), -1] then [1, min(r, count - 1 - li)] (j ascending, no test for j == li), the
+        // NaN-free / NaN branch outside the loops
+        const int klo = -(int)min(r, li), khi = (int)min(r, sg.count - 1u - li);
+        int best = 0;
         float best_cost = INFINITY;
-        for (uint32_t j = begin; j < end; ++j) {
-            if (j == li) continue;
-            const int w = (int)(sg.start + j) - wstart;
-            const Box o{s[0][w], s[1][w], s[2][w], s[3][w], s[4][w], s[5][w]};
-            const float c = nonan ? union_area_nonan(me, o) : half_area(box_union(me, o));
-            if (c <= best_cost) {
-                best = j;
-                best_cost = c;
+        auto scan = [&](auto cost_of) {
+            for (int k = klo; k < 0; ++k) {
+                const float c = cost_of(lw + k);
+                if (c <= best_cost) {
+                    best = k;
+                    best_cost = c;
+                }
             }
-        }
-        if (best == li) atomicOr(&misc->err, DERR_SELF_MERGE);
-        mv = (int8_t)((int)best - (int)li);
+            for (int k = 1; k <= khi; ++k) {
+                const float c = cost_of(lw + k);
+                if (c <= best_cost) {
+                    best = k;
+                    best_cost = c;
+                }
+            }
+        };
+        if (nonan)
+            scan([&](int w) { return union_area_nonan(me, Box{s[0][w], s[1][w], s[2][w], s[3][w], s[4][w], s[5][w]}); });
+        else
+            scan([&](int w) { return half_area(box_union(me, Box{s[0][w], s[1][w], s[2][w], s[3][w], s[4][w], s[5][w]})); });
+        if (best == 0) atomicOr(&misc->err, DERR_SELF_MERGE);
+        mv = (int8_t)best;
     }
     merge[i] = mv;
 }
